@@ -1,0 +1,188 @@
+"""Throughput of the device-resident Gaussian Splatting trainer on one MI355X.
+
+Workload (the shape the headline metric is quoted on): 300,000 Gaussians trained at 640x480
+on 200 views. Targets are renders of a second seeded 300k-Gaussian set (synthetic data, seeds
+1 and 2, random-init parameters with the CPU-runnable workload's distributions); the trained
+set starts at the 300k cap, so every timed step works on 300k Gaussians. One step = one
+Trainer::step of the reference API (project, sort/bin, forward, L1 + 0.2 D-SSIM, backward,
+Adam, and the TrainOptions::forIterations(7000) densify / reset schedule).
+
+Prints one JSON line: iters/s (value), per-stage GPU times from HIP events recorded over the
+timed region, the dominant kernel's roofline, and the reference CPU trainer timed on this
+host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
+
+from univeral_3d_reconstruction_system_amd import _abi  # noqa: E402
+from univeral_3d_reconstruction_system_amd import splat as U  # noqa: E402
+from univeral_3d_reconstruction_system_amd import synthetic as S  # noqa: E402
+
+N_GAUSSIANS = 300_000
+WIDTH, HEIGHT = 640, 480
+N_VIEWS = 200
+ITERATIONS = 7000
+
+# Algorithmic work per unit (SH degree 0), used for the roofline of each stage.
+BYTES_PROJECT_PER_GAUSSIAN = 56 + 48          # 14 parameter floats in, 48 B of screen state out
+BYTES_ADAM_PER_PARAM = 28                      # p, g, m, v read (16 B) + p, m, v written (12 B)
+BYTES_LOSS_PER_PIXEL = 36                      # render + target + grad, 3 x 4 B x 3 channels
+BYTES_RASTER_FWD_PER_PAIR, BYTES_RASTER_FWD_PER_PIXEL = 40, 20
+BYTES_RASTER_BWD_PER_PAIR, BYTES_RASTER_BWD_PER_PIXEL = 40 + 36, 20
+FLOPS_FWD_PER_EVAL, FLOPS_BWD_PER_EVAL = 20, 70  # per evaluated (pixel, Gaussian) pair
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(init, cams, images, extent, opt_gpu, budget_s=15.0):
+    """The compiled reference trainer (oracle/_ref) on this host's cores, same workload."""
+    import oracle as O
+    if O.ref_lib() is None:
+        return None
+    threads = host_cores()
+    opt = O.options_for_iterations(ITERATIONS)
+    opt.max_gaussians = opt_gpu.max_gaussians
+    t = O.Trainer(O.Gaussians(**{k: np.ascontiguousarray(v) for k, v in init.items()}), cams, images,
+                  extent, opt, use_ref=True, threads=threads)
+    for _ in range(2):
+        t.step()
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 200:
+        t.step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    return dict(value=steps / dt, unit="iters/s", cores=threads, kind="reference",
+                sample=f"{steps} reference Trainer::step calls (after 2 warm-up) at 300k Gaussians, "
+                       f"640x480, 200 views, {threads} threads, g++ -O3 build of engine/src/splat")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    ctx = U.Context(0)
+    cams = S.ring_cameras(N_VIEWS, WIDTH, HEIGHT)
+    extent = S.scene_extent(cams)
+    truth = S.random_gaussians(N_GAUSSIANS, 1)
+    dt_ = U.DeviceGaussians.from_host(ctx, truth)
+    r = U.Rasterizer(ctx)
+    out = U.DeviceArray(ctx, (WIDTH * HEIGHT * 3,))
+    images = [S.quantize(r.forward(dt_, c, out=out).numpy()) for c in cams]
+    del dt_, r
+    init = S.random_gaussians(N_GAUSSIANS, 2)
+    opt = U.options_for_iterations(ITERATIONS)
+    opt.max_gaussians = N_GAUSSIANS
+    trainer = U.Trainer(ctx, init, cams, images, extent, opt)
+    lib = _abi.load()
+
+    for _ in range(a.warmup):
+        trainer.step()
+    trainer.synchronize()
+    c0 = (C.c_int64 * 4)()
+    _abi.check(lib.u3d_trainer_counters(trainer.h, c0))
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 1))
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        trainer.step()
+    trainer.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = (C.c_double * _abi.U3D_NUM_STAGES)()
+    cnt = (C.c_int64 * _abi.U3D_NUM_STAGES)()
+    _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
+    c1 = (C.c_int64 * 4)()
+    _abi.check(lib.u3d_trainer_counters(trainer.h, c1))
+    steps = c1[0] - c0[0]
+    mean_n = (c1[1] - c0[1]) / steps
+    pixels = (c1[2] - c0[2]) / steps
+    pairs = (c1[3] - c0[3]) / steps
+    stats = trainer.stats()
+
+    # Evaluated pairs (sum of contributor counts) from one forward of the trained model.
+    r = U.Rasterizer(ctx)
+    m = trainer.model()
+    dm = U.DeviceGaussians.from_host(ctx, m)
+    r.forward(dm, cams[0], out=out)
+    evals = float(r.state()["n_contrib"].astype(np.int64).sum())
+
+    stage_ms = {name: ms[k] / max(cnt[k], 1) for k, name in enumerate(_abi.STAGES)}
+    per_step_ms = {name: ms[k] / steps for k, name in enumerate(_abi.STAGES)}
+    dominant = max(("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam"),
+                   key=lambda k: per_step_ms[k])
+    algo = dict(
+        project=("hbm", BYTES_PROJECT_PER_GAUSSIAN * mean_n),
+        adam=("hbm", BYTES_ADAM_PER_PARAM * 14 * mean_n),
+        loss=("hbm", BYTES_LOSS_PER_PIXEL * pixels),
+        sort=("hbm", 4 * 2 * 8 * mean_n + 8 * pairs + 2 * 2 * 8 * pairs),
+        chain_rule=("hbm", (56 + 36 + 112 + 8) * mean_n),
+        raster_fwd=("mfma", FLOPS_FWD_PER_EVAL * evals),
+        raster_bwd=("mfma", FLOPS_BWD_PER_EVAL * evals),
+    )
+    bound, work = algo[dominant]
+    dur_s = stage_ms[dominant] / 1e3
+    if bound == "hbm":
+        achieved, peak, unit = work / dur_s / 1e9, PEAK_HBM_GBS, "GB/s"
+    else:
+        achieved, peak, unit = work / dur_s / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = dict(bound=bound, kernel=dominant, achieved=round(achieved, 4), peak=peak, unit=unit,
+                    frac=round(achieved / peak, 6), traffic=traffic,
+                    note=("compute roof = MI355X FP32 peak (the compositing kernels run on the "
+                          "FP32 vector ALU); work = algorithmic flops per evaluated pair x pairs")
+                    if bound == "mfma" else "work = algorithmic HBM bytes per launch")
+
+    value = steps / elapsed
+    base = None if a.no_cpu_baseline else cpu_baseline(init, cams, images, extent, opt, a.cpu_seconds)
+    line = dict(
+        metric="train iters/s @300k Gaussians 640px", value=round(value, 3), unit="iters/s",
+        n_gpus=1, steps=a.steps, warmup=a.warmup, ms_per_step=round(1e3 * elapsed / steps, 4),
+        higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
+        config=dict(workload="plain single-view training step, 300k Gaussians (SH degree 0), "
+                             "640x480, 200 ring views, forIterations(7000) schedule, cap 300k",
+                    gaussians=N_GAUSSIANS, width=WIDTH, height=HEIGHT, views=N_VIEWS,
+                    global_batch=1, parallelism="none"),
+        gaussian_pixels_per_s=round(mean_n * pixels * value, 1),
+        mean_pairs_per_step=round(pairs, 1), evaluated_pairs_last_view=evals,
+        stage_ms_per_step={k: round(v, 4) for k, v in per_step_ms.items()},
+        roofline=roofline, cpu_baseline=base,
+        gpu_vs_cpu=round(value / base["value"], 2) if base else None,
+        train_psnr_ema=round(stats["psnr"], 3),
+    )
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,247 @@
+/*
+ * u3d_hip.h -- C ABI of the MI355X Gaussian Splatting trainer library (libu3d_hip.so).
+ *
+ * This is the drop-in boundary for the trainer hot path of engine/src/splat. The reference
+ * exposes a C++ class API (no FFI); every entry point below replaces one of its calls and the
+ * C++ shims in univeral_3d_reconstruction_system_amd/host/ re-create the reference classes
+ * (u3d::Rasterizer, u3d::photometricLoss, u3d::Adam, u3d::Trainer) on top of it.
+ *
+ * Conventions
+ *  - Every function returns U3D_OK (0) or an error code; u3d_last_error() returns a
+ *    thread-local message for the last failure. No C++ exception crosses the ABI.
+ *  - Array arguments of the compute functions are DEVICE pointers (HIP global memory of the
+ *    context's device) unless the name ends in _host. Work is queued on the context's stream;
+ *    functions that return values to the host synchronise that stream.
+ *  - Layouts are the reference's: Gaussian fields are structure-of-arrays with xyz / wxyz
+ *    interleaved per field (GaussianModel.h:23-37), images are height*width*3 interleaved RGB
+ *    (Rasterizer.h:44), pixel (x, y) is sampled at the integer coordinate (Rasterizer.h:36-37).
+ *  - A context is used from one host thread at a time, like the reference objects.
+ */
+#ifndef U3D_HIP_H
+#define U3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define U3D_OK 0
+#define U3D_ERR_INVALID 1     /* bad argument or size */
+#define U3D_ERR_HIP 2         /* HIP runtime failure */
+#define U3D_ERR_CAPACITY 3    /* a fixed capacity was exceeded */
+#define U3D_ERR_UNSUPPORTED 4 /* valid for the reference, not supported by this build */
+
+typedef struct u3d_ctx u3d_ctx;
+typedef struct u3d_raster u3d_raster;
+typedef struct u3d_adam u3d_adam;
+typedef struct u3d_trainer u3d_trainer;
+
+/* RenderCamera (Rasterizer.h:13-21): float pinhole, R row-major world->camera, OpenCV axes. */
+typedef struct {
+    int width, height;
+    float fx, fy, cx, cy;
+    float R[9];
+    float t[3];
+} u3d_camera;
+
+/* RasterSettings (Rasterizer.h:23-32). tile_size must be a multiple of 16 here. */
+typedef struct {
+    int tile_size;
+    float near_plane;
+    float eps2d;
+    float background[3];
+    float min_alpha;
+    float min_transmittance;
+} u3d_raster_settings;
+
+/* GaussianModel (GaussianModel.h:23-37) as device pointers; also used for gradients and
+ * Adam moments. colors holds 3 SH-DC coefficients per Gaussian. */
+typedef struct {
+    float *means;     /* [3n] */
+    float *scales;    /* [3n] log standard deviations */
+    float *quats;     /* [4n] w x y z, not necessarily normalised */
+    float *opacities; /* [n]  logits */
+    float *colors;    /* [3n] SH degree-0 coefficients */
+    int64_t n;
+} u3d_gaussians;
+
+/* LearningRates (Adam.h:8-14). */
+typedef struct {
+    float means, scales, quats, opacities, colors;
+} u3d_learning_rates;
+
+/* LossValue (Loss.h:7-11). */
+typedef struct {
+    float total, l1, ssim;
+} u3d_loss_value;
+
+/* TrainOptions (Trainer.h:19-37). */
+typedef struct {
+    int iterations;
+    int max_gaussians;
+    float ssim_lambda;
+    u3d_learning_rates lr;
+    float means_lr_final;
+    float densify_grad_threshold;
+    float percent_dense;
+    int refine_start, refine_stop, refine_interval, opacity_reset_interval;
+    float prune_opacity;
+    unsigned seed;
+    u3d_raster_settings raster;
+} u3d_train_options;
+
+/* TrainStats (Trainer.h:39-44). */
+typedef struct {
+    int iteration;
+    float loss;
+    float psnr;
+    int64_t gaussians;
+} u3d_train_stats;
+
+/* Device views of the state a forward pass leaves for backward (Rasterizer.h:60-89). */
+typedef struct {
+    int64_t n;               /* Gaussians of the last forward */
+    int64_t num_pairs;       /* K = tile intersections */
+    int tiles_x, tiles_y;
+    const int *radius;       /* [n], 0 = not visible */
+    const int *rect;         /* [4n] tile x0 y0 x1 y1 (exclusive), valid where radius > 0 */
+    const float *depth;      /* [n] camera-space z, valid where radius > 0 */
+    const float *mean2d;     /* [2n] */
+    const float *conic;      /* [3n] A B C of the inverse 2D covariance */
+    const float *color;      /* [3n] clamped rgb */
+    const float *opacity;    /* [n] sigmoid(opacity logit) */
+    const uint8_t *flags;    /* [n] bit0/1 Jacobian x/y clamped, bit2..4 colour clamped */
+    const int *tile_offsets; /* [tiles+1] */
+    const int *tile_list;    /* [K] Gaussian index per (tile, depth) slot */
+    const float *final_t;    /* [H*W] transmittance after compositing */
+    const int *n_contrib;    /* [H*W] last contributing list index + 1 */
+} u3d_raster_state;
+
+/* ------------------------------------------------------------------------------ context */
+const char *u3d_last_error(void);
+const char *u3d_version(void);
+/* Creates a context on HIP device `device` with its own stream (default priority). */
+int u3d_ctx_create(int device, u3d_ctx **out);
+void u3d_ctx_destroy(u3d_ctx *ctx);
+int u3d_ctx_synchronize(u3d_ctx *ctx);
+/* The context's hipStream_t, for callers that enqueue their own work around the library. */
+void *u3d_ctx_stream(u3d_ctx *ctx);
+int u3d_malloc(u3d_ctx *ctx, size_t bytes, void **out);
+int u3d_free(u3d_ctx *ctx, void *ptr);
+int u3d_memcpy_h2d(u3d_ctx *ctx, void *dst, const void *src_host, size_t bytes);
+int u3d_memcpy_d2h(u3d_ctx *ctx, void *dst_host, const void *src, size_t bytes);
+int u3d_memset(u3d_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------------- rasterizer (L1 core) */
+/* Rasterizer(ThreadPool&) (Rasterizer.h:41). */
+int u3d_raster_create(u3d_ctx *ctx, u3d_raster **out);
+void u3d_raster_destroy(u3d_raster *r);
+/* Rasterizer::forward (Rasterizer.h:44-45, Rasterizer.cpp:263-330): renders into rgb
+ * [H*W*3] and keeps the state for the next backward. Bit-identical to the reference. */
+int u3d_raster_forward(u3d_raster *r, const u3d_gaussians *model, const u3d_camera *camera,
+                       const u3d_raster_settings *settings, float *rgb);
+/* Rasterizer::backward (Rasterizer.h:47-51, Rasterizer.cpp:334-545): ADDS the parameter
+ * gradients of the last forward into `grads` (same n) and, if non-null, the densification
+ * statistic |dL/d mean2d| in NDC into mean2d_grad_ndc [n]. */
+int u3d_raster_backward(u3d_raster *r, const u3d_gaussians *model, const float *dl_drgb,
+                        u3d_gaussians *grads, float *mean2d_grad_ndc);
+/* Rasterizer::radii (Rasterizer.h:53-54): device pointer valid until the next forward. */
+int u3d_raster_radii(u3d_raster *r, const int **radii, int64_t *n);
+/* Test and tooling access to the forward state (Rasterizer.h:60-89). */
+int u3d_raster_get_state(u3d_raster *r, u3d_raster_state *out);
+/* The per-Gaussian 2D gradients [9n] (mx my A B C r g b alpha) of the last backward. */
+int u3d_raster_get_grad2d(u3d_raster *r, const float **grad2d);
+
+/* ------------------------------------------------------------------------------- loss */
+/* photometricLoss (Loss.h:18-19, Loss.cpp:47-185): (1-l)*L1 + l*(1-SSIM), 11x11 sigma 1.5
+ * window, zero padding; OVERWRITES grad [H*W*3]; the value is returned to the host. */
+int u3d_photometric_loss(u3d_ctx *ctx, const float *render, const float *target, int width,
+                         int height, float ssim_lambda, float *grad, u3d_loss_value *out_host);
+/* psnr (Loss.h:21, Loss.cpp:187-196) of two device arrays. */
+int u3d_psnr(u3d_ctx *ctx, const float *a, const float *b, int64_t count, float *out_host);
+
+/* ------------------------------------------------------------------------------- Adam */
+/* Adam (Adam.h:18-33): moments live on the device in GaussianModel layout. */
+int u3d_adam_create(u3d_ctx *ctx, u3d_adam **out);
+void u3d_adam_destroy(u3d_adam *a);
+int u3d_adam_reset(u3d_adam *a, int64_t n);                     /* Adam::reset, Adam.cpp:31-38 */
+int u3d_adam_compact(u3d_adam *a, const uint8_t *keep);        /* Adam::compact, :40-44 (device keep[n]) */
+int u3d_adam_grow(u3d_adam *a, int64_t count);                 /* Adam::grow, :46-60 */
+int u3d_adam_reset_opacity_moments(u3d_adam *a);               /* :62-66 */
+/* Adam::step (Adam.cpp:68-78): one step over all five groups with per-group rates. */
+int u3d_adam_step(u3d_adam *a, u3d_gaussians *params, const u3d_gaussians *grads,
+                  const u3d_learning_rates *lr);
+/* Device views of the first and second moments. */
+int u3d_adam_moments(u3d_adam *a, u3d_gaussians *m, u3d_gaussians *v, int *step);
+
+/* --------------------------------------------------------------------- densification */
+/* Decisions of Trainer::densifyAndPrune (Trainer.cpp:105-169) given the statistics:
+ * keep[n] (1 keep / 0 remove), kind[n] (0 none, 1 clone, 2 split), and the counts. Device
+ * arrays; counts are returned to the host. budget = max(0, max_gaussians - n). */
+int u3d_densify_decide(u3d_ctx *ctx, const u3d_gaussians *model, const float *grad_accum,
+                       const int *grad_count, float extent, int max_gaussians,
+                       float grad_threshold, float percent_dense, float prune_opacity,
+                       int opacity_was_reset, uint8_t *keep, uint8_t *kind,
+                       int64_t *n_clone_host, int64_t *n_split_host, int64_t *n_keep_host);
+/* Applies the decisions in place: compacts kept entries (order preserved) and appends the
+ * clones and split children in index order; `normals` holds 6 N(0,1) draws per accepted
+ * split in acceptance order (device, may be NULL if no split). `log_shrink` = log(1.6f).
+ * The model arrays must have capacity for the new size, returned in *new_n_host. */
+int u3d_densify_apply(u3d_ctx *ctx, u3d_gaussians *model, const uint8_t *keep,
+                      const uint8_t *kind, const float *normals, float log_shrink,
+                      int64_t *new_n_host);
+
+/* ------------------------------------------------------------------------------ trainer */
+/* Trainer (Trainer.h:249-292) with all state resident on the device. Host inputs: the initial
+ * model (SoA host arrays), cameras[n_images], images as concatenated uint8 HWC. */
+int u3d_trainer_create(u3d_ctx *ctx, const float *means_host, const float *scales_host,
+                       const float *quats_host, const float *opacities_host,
+                       const float *colors_host, int64_t n, const u3d_camera *cameras,
+                       const uint8_t *rgb8_host, int n_images, float scene_extent,
+                       const u3d_train_options *options, u3d_trainer **out);
+void u3d_trainer_destroy(u3d_trainer *t);
+int u3d_trainer_step(u3d_trainer *t);                 /* Trainer::step, Trainer.cpp:53-103 */
+int u3d_trainer_done(u3d_trainer *t);                 /* 1 when finished */
+int u3d_trainer_iteration(u3d_trainer *t);
+int64_t u3d_trainer_size(u3d_trainer *t);
+/* Trainer::model: copies the current parameters to host arrays sized by u3d_trainer_size. */
+int u3d_trainer_get_model(u3d_trainer *t, float *means_host, float *scales_host,
+                          float *quats_host, float *opacities_host, float *colors_host);
+int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out);      /* Trainer::stats */
+int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out_host);   /* Trainer.cpp:180-192 */
+/* Mean absolute error over all training views (time-to-target metric). */
+int u3d_trainer_evaluate_l1(u3d_trainer *t, float *out_host);
+int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host);
+
+/* ----------------------------------------------------------------------- measurement */
+/* Pipeline stages timed with HIP events on the context's stream (no reference counterpart:
+ * the reference has no profiling; these feed the roofline report of the benchmark). */
+#define U3D_STAGE_PROJECT 0    /* EWA projection + SH-DC colour */
+#define U3D_STAGE_SORT 1       /* depth sort, pair emission, tile sort, tile offsets */
+#define U3D_STAGE_RASTER_FWD 2 /* front-to-back compositing */
+#define U3D_STAGE_LOSS 3       /* fused L1 + D-SSIM value and gradient */
+#define U3D_STAGE_RASTER_BWD 4 /* reverse compositing -> per-Gaussian 2D gradients */
+#define U3D_STAGE_CHAIN 5      /* 2D -> 3D chain rule */
+#define U3D_STAGE_ADAM 6       /* fused Adam over all parameter groups */
+#define U3D_STAGE_DENSIFY 7    /* clone / split / prune / opacity reset */
+#define U3D_NUM_STAGES 8
+/* Turns event timing on or off (off by default) and clears the accumulated totals. */
+int u3d_ctx_set_timing(u3d_ctx *ctx, int enabled);
+/* Waits for the recorded events; writes total milliseconds and launch counts per stage. */
+int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES],
+                        int64_t counts[U3D_NUM_STAGES]);
+/* Work counters summed over the trainer's steps since creation: steps, Gaussians, pixels,
+ * tile intersections K (values[0..3]). */
+int u3d_trainer_counters(u3d_trainer *t, int64_t values[4]);
+
+/* ------------------------------------------------------------------------------ helpers */
+/* The device exp (bit-identical to the host expf) evaluated on the host, for tests. */
+void u3d_expf_host_array(const float *x, float *y, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* U3D_HIP_H */

@@ -1,0 +1,260 @@
+"""GPU parity of the HIP trainer path against the CPU oracle, through the C ABI.
+
+Bars: projection outputs, tile lists, rendered images, final transmittance and contributor
+counts bit-exact; loss gradient bit-exact and loss values within 1e-6 relative; Adam updates
+and densification decisions bit-exact; backward gradients within 1e-4 relative (L2 per
+parameter group) -- the only nondeterminism is the order of float atomic additions.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from scenes import config0_scene, edge_case_model, fd_camera, gaussians, ref_style_model
+from univeral_3d_reconstruction_system_amd import splat as U
+from univeral_3d_reconstruction_system_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = U.Context(0)
+    yield c
+    c.close()
+
+
+def _settings(**kw):
+    return U.make_settings(**kw), O.make_settings(**kw)
+
+
+def _forward_both(ctx, g, cam, **kw):
+    su, so = _settings(**kw)
+    dg = U.DeviceGaussians.from_host(ctx, g)
+    r = U.Rasterizer(ctx)
+    img = r.forward(dg, cam, su).numpy()
+    ro = O.Raster()
+    ref = ro.forward(gaussians(g), cam, so)
+    return dg, r, img, ro, ref
+
+
+def _assert_forward_equal(r, img, ro, ref):
+    assert np.array_equal(img, ref), f"image max |diff| {np.abs(img - ref).max()}"
+    s, so = r.state(), ro.state()
+    assert np.array_equal(s["radius"], so["radius"])
+    assert np.array_equal(s["tile_offsets"], so["tile_offsets"])
+    assert np.array_equal(s["tile_list"], so["tile_list"])
+    assert np.array_equal(s["final_t"], so["final_t"])
+    assert np.array_equal(s["n_contrib"], so["n_contrib"])
+    vis = so["radius"] > 0
+    p = ro.projection()
+    for k in ("mean2d", "conic", "depth", "rgb", "alpha", "rect", "flags"):
+        per = len(p[k]) // max(len(vis), 1)
+        m = np.repeat(vis, per)
+        assert np.array_equal(s[k][m], p[k][m]), k
+
+
+@pytest.mark.parametrize("view", [0, 7])
+def test_forward_bit_exact_config0(ctx, view):
+    g, cams = config0_scene(20000, 1)
+    _, r, img, ro, ref = _forward_both(ctx, g, cams[view])
+    _assert_forward_equal(r, img, ro, ref)
+    assert r.state()["num_pairs"] == ro.num_pairs() > 0
+
+
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_forward_tile_sizes_and_background(ctx, tile):
+    g = ref_style_model(300, 11, 1.0, 0.0, 0.03, 0.12)
+    cam = S.orbit_camera(97, 73, 80.0, 0.7, 4.0, -0.5)
+    _, r, img, ro, ref = _forward_both(ctx, g, cam, tile_size=tile, background=(0.1, 0.2, 0.3))
+    _assert_forward_equal(r, img, ro, ref)
+
+
+def test_forward_edge_cases(ctx):
+    g = edge_case_model()
+    cams = S.ring_cameras(4, 160, 120)
+    for cam in cams:
+        _, r, img, ro, ref = _forward_both(ctx, g, cam)
+        _assert_forward_equal(r, img, ro, ref)
+
+
+def test_forward_empty_model(ctx):
+    g = {k: np.zeros(0, np.float32) for k in ("means", "scales", "quats", "opacities", "colors")}
+    cam = S.ring_cameras(1, 64, 48)[0]
+    _, r, img, ro, ref = _forward_both(ctx, g, cam, background=(0.25, 0.5, 0.75))
+    assert np.array_equal(img, ref)
+    assert r.state()["num_pairs"] == 0
+
+
+def test_forward_no_cutoffs(ctx):
+    """min_alpha = min_transmittance = 0 as in the reference's gradient test."""
+    g = ref_style_model(8, 7, 0.6, 3.0, 0.08, 0.3)
+    _, r, img, ro, ref = _forward_both(ctx, g, fd_camera(), tile_size=64, min_alpha=0.0,
+                                       min_transmittance=0.0, background=(0.1, 0.2, 0.3))
+    _assert_forward_equal(r, img, ro, ref)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b.astype(np.float64)), 1e-30))
+
+
+def _backward_both(ctx, g, cam, dl, **kw):
+    dg, r, img, ro, ref = _forward_both(ctx, g, cam, **kw)
+    n = len(g["opacities"])
+    grads = U.DeviceGaussians(ctx, n)
+    ndc = U.DeviceArray.from_numpy(ctx, np.zeros(max(n, 1), np.float32))
+    r.backward(dg, U.DeviceArray.from_numpy(ctx, dl), grads, ndc)
+    gh = grads.to_host()
+    ndo = np.zeros(n, np.float32)
+    go = ro.backward(gaussians(g), dl, ndc=ndo)
+    return gh, ndc.numpy()[:n], go, ndo, r, ro
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_backward_config0(ctx, seed):
+    g, cams = config0_scene(20000, 1)
+    rng = np.random.default_rng(seed)
+    cam = cams[3 + seed]
+    dl = (rng.uniform(-1, 1, cam["width"] * cam["height"] * 3) * 1e-3).astype(np.float32)
+    gh, ndc, go, ndo, r, ro = _backward_both(ctx, g, cam, dl)
+    for f in ("means", "scales", "quats", "opacities", "colors"):
+        rel = _rel(gh[f], getattr(go, f))
+        assert rel < GRAD_RTOL, (f, rel)
+    assert _rel(ndc, ndo) < GRAD_RTOL
+    g2 = r.grad2d()
+    assert _rel(g2, ro.grad2d()) < GRAD_RTOL
+
+
+def test_backward_background_and_clamps(ctx):
+    g = edge_case_model(9)
+    cam = S.ring_cameras(3, 128, 96)[1]
+    rng = np.random.default_rng(3)
+    dl = rng.uniform(-1, 1, cam["width"] * cam["height"] * 3).astype(np.float32)
+    gh, ndc, go, ndo, _, _ = _backward_both(ctx, g, cam, dl, background=(0.3, 0.1, 0.7))
+    for f in ("means", "scales", "quats", "opacities", "colors"):
+        assert _rel(gh[f], getattr(go, f)) < GRAD_RTOL, f
+    assert _rel(ndc, ndo) < GRAD_RTOL
+
+
+def test_backward_matches_fp64_finite_differences(ctx):
+    """The reference's gradient check (test_splat.cpp:110-151), with an FP64 forward."""
+    g = ref_style_model(8, 7, 0.6, 3.0, 0.08, 0.3)
+    cam = fd_camera()
+    kw = dict(tile_size=64, min_alpha=0.0, min_transmittance=0.0, background=(0.1, 0.2, 0.3))
+    rng = np.random.default_rng(7)
+    w = rng.uniform(-1, 1, cam["width"] * cam["height"] * 3).astype(np.float32)
+    gh, _, _, _, _, _ = _backward_both(ctx, g, cam, w, **kw)
+    so = O.make_settings(**kw)
+    g64 = {k: v.astype(np.float64) for k, v in g.items()}
+    eps = 1e-5
+    for f in ("means", "scales", "quats", "opacities", "colors"):
+        num = np.zeros_like(g64[f])
+        for k in range(len(num)):
+            gp = {kk: vv.copy() for kk, vv in g64.items()}
+            gm = {kk: vv.copy() for kk, vv in g64.items()}
+            gp[f][k] += eps
+            gm[f][k] -= eps
+            lp = float(np.dot(w.astype(np.float64), O.render_f64(gp, cam, so)))
+            lm = float(np.dot(w.astype(np.float64), O.render_f64(gm, cam, so)))
+            num[k] = (lp - lm) / (2 * eps)
+        rel = np.linalg.norm(gh[f] - num) / np.linalg.norm(num)
+        assert rel < 2e-3, (f, rel)
+
+
+def test_loss_bit_exact(ctx):
+    rng = np.random.default_rng(3)
+    for (W, H, lam) in ((21, 17, 1.0), (320, 240, 0.2), (97, 33, 0.2), (64, 48, 0.0)):
+        tgt = rng.uniform(0, 1, W * H * 3).astype(np.float32)
+        ren = (0.5 * tgt + 0.5 * rng.uniform(0, 1, W * H * 3)).astype(np.float32)
+        grad = U.DeviceArray(ctx, (W * H * 3,))
+        v = U.photometric_loss(ctx, U.DeviceArray.from_numpy(ctx, ren), U.DeviceArray.from_numpy(ctx, tgt),
+                               W, H, lam, grad)
+        vo, go = O.photometric_loss(ren, tgt, W, H, lam)
+        gd = grad.numpy()
+        assert np.array_equal(gd, go), (W, H, lam, np.abs(gd - go).max())
+        np.testing.assert_allclose(v, vo, rtol=1e-6, atol=1e-7)
+
+
+def test_adam_bit_exact(ctx):
+    n = 1000
+    g = S.random_gaussians(n, 4)
+    gr = S.random_gaussians(n, 5)
+    lr = O.lrs(means=1.6e-4 * 3.3)
+    po, mo, vo = O.adam_steps(gaussians(g), gaussians(gr), lr, 3)
+    dp = U.DeviceGaussians.from_host(ctx, g)
+    dgr = U.DeviceGaussians.from_host(ctx, gr)
+    adam = U.Adam(ctx)
+    adam.reset(n)
+    lru = U.learning_rates(means=1.6e-4 * 3.3)
+    for _ in range(3):
+        adam.step(dp, dgr, lru)
+    ph = dp.to_host()
+    m, v, step = adam.moments()
+    assert step == 3
+    for f in ("means", "scales", "quats", "opacities", "colors"):
+        assert np.array_equal(ph[f], getattr(po, f)), f
+        assert np.array_equal(m[f], getattr(mo, f)), f
+        assert np.array_equal(v[f], getattr(vo, f)), f
+
+
+@pytest.mark.parametrize("max_g", [10**6, 1500, 1001, 1000, 900])
+def test_densify_decisions_bit_exact(ctx, max_g):
+    n = 1000
+    g = S.random_gaussians(n, 8, log_scale_mean=-4.0, log_scale_std=1.0)
+    r = S.rng(9)
+    accum = (r.random(n) * 4e-4).astype(np.float32)
+    count = r.integers(0, 4, n).astype(np.int32)
+    extent = 3.3
+    normals = r.normal(size=6 * n).astype(np.float32)
+    keep_o, kind_o, extra_o, used = O.densify(gaussians(g), accum, count, extent, max_g, normals,
+                                              opacity_was_reset=True)
+    dg = U.DeviceGaussians.from_host(ctx, g, capacity=3 * n)
+    keep, kind, counts = U.densify_decide(ctx, dg, U.DeviceArray.from_numpy(ctx, accum),
+                                          U.DeviceArray.from_numpy(ctx, count), extent, max_g,
+                                          opacity_was_reset=True)
+    assert np.array_equal(keep.numpy()[:n], keep_o)
+    assert np.array_equal(kind.numpy()[:n], kind_o)
+    assert counts["splits"] * 6 == used
+    log_shrink = float(np.log(np.float32(1.6)).astype(np.float32))
+    new_n = U.densify_apply(ctx, dg, keep, kind, U.DeviceArray.from_numpy(ctx, normals[:max(used, 1)]),
+                            np.float32(np.log(np.float32(1.6))))
+    nk = int(keep_o.sum())
+    assert new_n == nk + extra_o.n
+    out = dg.to_host()
+    go = gaussians(g)
+    kept_idx = np.nonzero(keep_o)[0]
+    for f, d in O.FIELDS:
+        a = out[f].reshape(-1, d)
+        assert np.array_equal(a[:nk], getattr(go, f).reshape(-1, d)[kept_idx]), f
+        assert np.array_equal(a[nk:], getattr(extra_o, f).reshape(-1, d)), f
+    del log_shrink
+
+
+def test_trainer_runs_and_recovers(ctx):
+    """training_recovers_a_synthetic_scene (test_splat.cpp:227-268) on the device trainer."""
+    truth = ref_style_model(300, 11, 1.0, 0.0, 0.03, 0.12)
+    truth["opacities"][:] = np.float32(np.log(0.8 / 0.2))
+    cams, images = [], []
+    ro = O.Raster()
+    st = O.make_settings()
+    for k in range(16):
+        cam = S.orbit_camera(96, 72, 80.0, 6.2831853 * k / 16.0, 4.0, -1.0 + 0.15 * (k % 4))
+        cams.append(cam)
+        images.append(S.quantize(ro.forward(gaussians(truth), cam, st)))
+    r = S.rng(12)
+    xyz = (truth["means"].reshape(-1, 3)[::2] + r.normal(0, 0.08, size=(150, 3))).astype(np.float32)
+    init = O.init_from_points(xyz.reshape(-1), np.full(450, 0.5, np.float32))
+    opt = U.options_for_iterations(1500)
+    opt.max_gaussians = 4000
+    init_d = {f: getattr(init, f) for f, _ in O.FIELDS}
+    t = U.Trainer(ctx, init_d, cams, images, 4.4, opt)
+    before = t.evaluate_psnr()
+    while not t.done():
+        t.step()
+    after = t.evaluate_psnr()
+    assert after > before + 8.0, (before, after)
+    assert after > 26.0, after
+    assert t.size() != init.n
+    st_ = t.stats()
+    assert st_["iteration"] == 1500

@@ -1,0 +1,1125 @@
+// C ABI of libu3d_hip (include/u3d_hip.h): context, rasterizer, loss, Adam, densification and
+// the device-resident trainer. Host orchestration only; every kernel lives in a kernels_*.hip
+// translation unit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "kernels.h"
+
+namespace u3d {
+
+namespace {
+thread_local std::string tError;
+}
+
+void setError(const std::string &msg) { tError = msg; }
+
+int failHip(hipError_t e, const char *what, const char *file, int line)
+{
+    tError = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")";
+    return U3D_ERR_HIP;
+}
+
+int useDevice(u3d_ctx *ctx)
+{
+    U3D_CHECK(ctx != nullptr, U3D_ERR_INVALID, "null context");
+    U3D_HIP(hipSetDevice(ctx->device));
+    return U3D_OK;
+}
+
+namespace {
+
+// Parameter-shaped device storage: [means 3c | scales 3c | quats 4c | opacities c | colors 3c]
+// for capacity c, so a whole model (or its gradients / moments) is one allocation.
+struct DevModel {
+    DevBuf<float> buf;
+    int64_t n = 0;
+    int64_t cap = 0;
+    int reserve(int64_t c)
+    {
+        if (c <= cap)
+            return U3D_OK;
+        U3D_TRY(buf.ensure(size_t(c) * 14));
+        cap = c;
+        return U3D_OK;
+    }
+    u3d_gaussians view() const
+    {
+        float *b = buf.p;
+        return {b, b + 3 * cap, b + 6 * cap, b + 10 * cap, b + 11 * cap, n};
+    }
+    int zero(hipStream_t s) const
+    {
+        if (cap > 0)
+            U3D_HIP(hipMemsetAsync(buf.p, 0, size_t(cap) * 14 * sizeof(float), s));
+        return U3D_OK;
+    }
+    void swap(DevModel &o)
+    {
+        std::swap(buf.p, o.buf.p);
+        std::swap(buf.cap, o.buf.cap);
+        std::swap(n, o.n);
+        std::swap(cap, o.cap);
+    }
+};
+
+int copyFields(hipStream_t s, const u3d_gaussians &src, const u3d_gaussians &dst, int64_t n,
+               hipMemcpyKind kind)
+{
+    if (n <= 0)
+        return U3D_OK;
+    const size_t f = sizeof(float);
+    U3D_HIP(hipMemcpyAsync(dst.means, src.means, size_t(n) * 3 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.scales, src.scales, size_t(n) * 3 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.quats, src.quats, size_t(n) * 4 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.opacities, src.opacities, size_t(n) * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.colors, src.colors, size_t(n) * 3 * f, kind, s));
+    return U3D_OK;
+}
+
+int checkModel(const u3d_gaussians *m, const char *what)
+{
+    U3D_CHECK(m != nullptr, U3D_ERR_INVALID, std::string("null ") + what);
+    U3D_CHECK(m->n >= 0 && m->n < (int64_t(1) << 31), U3D_ERR_INVALID, std::string("bad size of ") + what);
+    if (m->n > 0)
+        U3D_CHECK(m->means && m->scales && m->quats && m->opacities && m->colors, U3D_ERR_INVALID,
+                  std::string("null field in ") + what);
+    return U3D_OK;
+}
+
+int bitsFor(int64_t values)
+{
+    int b = 1;
+    while ((int64_t(1) << b) < values)
+        ++b;
+    return b;
+}
+
+} // namespace
+} // namespace u3d
+
+using namespace u3d;
+
+struct u3d_raster {
+    u3d_ctx *ctx = nullptr;
+    int64_t n = 0;
+    int64_t numPairs = 0;
+    u3d_camera cam{};
+    u3d_raster_settings st{};
+    RasterGeometry geo{};
+    bool haveForward = false;
+    DevBuf<float4> rec0, rec1;
+    DevBuf<float> rec2, depth;
+    DevBuf<int> radius;
+    DevBuf<int4> rect;
+    DevBuf<uint8_t> flags;
+    DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
+    DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
+    DevBuf<uint32_t> hist, scanScratch, total;
+    DevBuf<int> tileOffsets;
+    const int *tileList = nullptr;
+    DevBuf<float> finalT, grad2d;
+    DevBuf<int> nContrib;
+    DevBuf<float> dbgMean2d, dbgConic, dbgColor, dbgOpacity;
+    ScreenBuffers screen() const
+    {
+        return {rec0.p, rec1.p, rec2.p, depth.p, radius.p, rect.p, flags.p, depthKey.p, nTiles.p};
+    }
+};
+
+namespace u3d {
+namespace {
+
+int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
+                      const u3d_raster_settings &st, float *rgb)
+{
+    U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
+              "tile_size must be a positive multiple of 16");
+    U3D_CHECK(cam.width >= 0 && cam.height >= 0, U3D_ERR_INVALID, "negative image size");
+    const hipStream_t s = r->ctx->stream;
+    const int64_t n = m.n;
+    const int ts = st.tile_size;
+    const int tilesX = (cam.width + ts - 1) / ts, tilesY = (cam.height + ts - 1) / ts;
+    const int tiles = tilesX * tilesY;
+    const size_t P = size_t(cam.width) * size_t(cam.height);
+    r->n = n;
+    r->cam = cam;
+    r->st = st;
+    r->geo = {cam.width, cam.height, ts, tilesX, tilesY};
+    const size_t nn = size_t(n > 0 ? n : 1);
+    U3D_TRY(r->rec0.ensure(nn));
+    U3D_TRY(r->rec1.ensure(nn));
+    U3D_TRY(r->rec2.ensure(nn));
+    U3D_TRY(r->depth.ensure(nn));
+    U3D_TRY(r->radius.ensure(nn));
+    U3D_TRY(r->rect.ensure(nn));
+    U3D_TRY(r->flags.ensure(nn));
+    U3D_TRY(r->depthKey.ensure(nn));
+    U3D_TRY(r->nTiles.ensure(nn));
+    U3D_TRY(r->keyAlt.ensure(nn));
+    U3D_TRY(r->valA.ensure(nn));
+    U3D_TRY(r->valAlt.ensure(nn));
+    U3D_TRY(r->ntSorted.ensure(nn));
+    U3D_TRY(r->pairOffsets.ensure(nn));
+    U3D_TRY(r->total.ensure(4));
+    U3D_TRY(r->scanScratch.ensure(scanScratchSize(int64_t(nn))));
+    U3D_TRY(r->tileOffsets.ensure(size_t(tiles) + 1));
+    U3D_TRY(r->finalT.ensure(P > 0 ? P : 1));
+    U3D_TRY(r->nContrib.ensure(P > 0 ? P : 1));
+    const ScreenBuffers sb = r->screen();
+
+    {
+        StageScope scope(r->ctx, U3D_STAGE_PROJECT);
+        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb));
+    }
+    std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
+    // Depth order of the visible Gaussians (invisible keys sort last).
+    U3D_TRY(launchIota(s, r->valA.p, n));
+    U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
+    RadixScratch rs{r->hist.p, r->hist.cap};
+    bool alt = false;
+    U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
+    const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
+    U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n));
+    U3D_TRY(exclusiveScanU32(s, r->ntSorted.p, r->pairOffsets.p, n, r->total.p, r->scanScratch.p,
+                             r->scanScratch.cap));
+    uint32_t K = 0;
+    U3D_HIP(hipMemcpyAsync(&K, r->total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    r->numPairs = int64_t(K);
+    const size_t kk = size_t(K > 0 ? K : 1);
+    U3D_TRY(r->pairTile.ensure(kk));
+    U3D_TRY(r->pairGauss.ensure(kk));
+    U3D_TRY(r->pairTileAlt.ensure(kk));
+    U3D_TRY(r->pairGaussAlt.ensure(kk));
+    U3D_TRY(r->hist.ensure(radixHistSize(int64_t(kk))));
+    RadixScratch rs2{r->hist.p, r->hist.cap};
+    U3D_TRY(launchEmitPairs(s, n, sorted, r->pairOffsets.p, r->rect.p, tilesX, r->pairTile.p,
+                            r->pairGauss.p, r->ntSorted.p));
+    bool alt2 = false;
+    U3D_TRY(radixSortPairs(s, r->pairTile.p, r->pairGauss.p, r->pairTileAlt.p, r->pairGaussAlt.p,
+                           int64_t(K), bitsFor(tiles), rs2, &alt2));
+    const uint32_t *tileSorted = alt2 ? r->pairTileAlt.p : r->pairTile.p;
+    r->tileList = reinterpret_cast<const int *>(alt2 ? r->pairGaussAlt.p : r->pairGauss.p);
+    U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
+    sortScope.reset();
+    StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
+    U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
+                                r->finalT.p, r->nContrib.p));
+    r->haveForward = true;
+    return U3D_OK;
+}
+
+int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
+                       u3d_gaussians &grads, float *ndc, int *gradCount)
+{
+    U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
+    U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
+    const hipStream_t s = r->ctx->stream;
+    const size_t g2 = size_t(r->n) * kGrad2dStride;
+    U3D_TRY(r->grad2d.ensure(g2 > 0 ? g2 : 1));
+    if (g2)
+        U3D_HIP(hipMemsetAsync(r->grad2d.p, 0, g2 * sizeof(float), s));
+    const ScreenBuffers sb = r->screen();
+    {
+        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD);
+        U3D_TRY(launchRasterBackward(s, r->geo, r->st, r->tileOffsets.p, r->tileList, sb, dl,
+                                     r->finalT.p, r->nContrib.p, r->grad2d.p));
+    }
+    StageScope scope(r->ctx, U3D_STAGE_CHAIN);
+    U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount));
+    return U3D_OK;
+}
+
+// Scratch views carved out of the context's shared scratch buffer.
+int ctxScratch(u3d_ctx *ctx, size_t bytes, unsigned char **out)
+{
+    U3D_TRY(ctx->scratch.ensure(bytes));
+    *out = ctx->scratch.p;
+    return U3D_OK;
+}
+
+} // namespace
+} // namespace u3d
+
+struct u3d_adam {
+    u3d_ctx *ctx = nullptr;
+    DevModel m, v, spare;
+    int step = 0;
+};
+
+// ---------------------------------------------------------------------------------- trainer
+struct u3d_trainer {
+    u3d_ctx *ctx = nullptr;
+    DevModel params, spare, grads, adamM, adamV;
+    int adamStep = 0;
+    std::vector<u3d_camera> cams;
+    std::vector<size_t> imageOffset;
+    DevBuf<uint8_t> targets;
+    float extent = 1.0f;
+    u3d_train_options opt{};
+    u3d_raster raster;
+    std::mt19937 rng;
+    std::vector<int> order;
+    size_t orderPos = 0;
+    int iteration = 0;
+    bool opacityWasReset = false;
+    DevBuf<float> render, dloss, gradAccum, stepStats, evalOut;
+    DevBuf<int> gradCount;
+    DevBuf<double> partials;
+    int statsFolded = 0;
+    u3d_train_stats stats{};
+    int64_t counters[4] = {0, 0, 0, 0}; // steps, Gaussians, pixels, pairs
+};
+
+// ----------------------------------------------------------------------------------- C ABI
+extern "C" {
+
+const char *u3d_last_error(void) { return tError.c_str(); }
+const char *u3d_version(void) { return "u3d_hip 0.1 (gfx950)"; }
+
+int u3d_ctx_create(int device, u3d_ctx **out)
+{
+    U3D_CHECK(out != nullptr, U3D_ERR_INVALID, "null out");
+    *out = nullptr;
+    int count = 0;
+    U3D_HIP(hipGetDeviceCount(&count));
+    U3D_CHECK(device >= 0 && device < count, U3D_ERR_INVALID, "no such HIP device");
+    U3D_HIP(hipSetDevice(device));
+    std::unique_ptr<u3d_ctx> c(new u3d_ctx());
+    c->device = device;
+    U3D_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    U3D_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    *out = c.release();
+    return U3D_OK;
+}
+
+void u3d_ctx_destroy(u3d_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int u3d_ctx_synchronize(u3d_ctx *ctx)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_HIP(hipStreamSynchronize(ctx->stream));
+    return U3D_OK;
+}
+
+void *u3d_ctx_stream(u3d_ctx *ctx) { return ctx ? reinterpret_cast<void *>(ctx->stream) : nullptr; }
+
+int u3d_malloc(u3d_ctx *ctx, size_t bytes, void **out)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_HIP(hipMalloc(out, bytes > 0 ? bytes : 1));
+    return U3D_OK;
+}
+
+int u3d_free(u3d_ctx *ctx, void *ptr)
+{
+    U3D_TRY(useDevice(ctx));
+    if (ptr) {
+        U3D_HIP(hipStreamSynchronize(ctx->stream));
+        U3D_HIP(hipFree(ptr));
+    }
+    return U3D_OK;
+}
+
+int u3d_memcpy_h2d(u3d_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    U3D_TRY(useDevice(ctx));
+    if (bytes) {
+        U3D_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        U3D_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return U3D_OK;
+}
+
+int u3d_memcpy_d2h(u3d_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    U3D_TRY(useDevice(ctx));
+    if (bytes) {
+        U3D_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        U3D_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return U3D_OK;
+}
+
+int u3d_memset(u3d_ctx *ctx, void *dst, int value, size_t bytes)
+{
+    U3D_TRY(useDevice(ctx));
+    if (bytes)
+        U3D_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return U3D_OK;
+}
+
+// ------------------------------------------------------------------------------ rasterizer
+int u3d_raster_create(u3d_ctx *ctx, u3d_raster **out)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(out != nullptr, U3D_ERR_INVALID, "null out");
+    *out = new u3d_raster();
+    (*out)->ctx = ctx;
+    return U3D_OK;
+}
+
+void u3d_raster_destroy(u3d_raster *r)
+{
+    if (!r)
+        return;
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    delete r;
+}
+
+int u3d_raster_forward(u3d_raster *r, const u3d_gaussians *model, const u3d_camera *camera,
+                       const u3d_raster_settings *settings, float *rgb)
+{
+    U3D_CHECK(r && camera && settings, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(r->ctx));
+    U3D_TRY(checkModel(model, "model"));
+    U3D_CHECK(rgb != nullptr || size_t(camera->width) * size_t(camera->height) == 0, U3D_ERR_INVALID, "null rgb");
+    return rasterForwardImpl(r, *model, *camera, *settings, rgb);
+}
+
+int u3d_raster_backward(u3d_raster *r, const u3d_gaussians *model, const float *dl_drgb,
+                        u3d_gaussians *grads, float *mean2d_grad_ndc)
+{
+    U3D_CHECK(r && dl_drgb, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(r->ctx));
+    U3D_TRY(checkModel(model, "model"));
+    U3D_TRY(checkModel(grads, "grads"));
+    return rasterBackwardImpl(r, *model, dl_drgb, *grads, mean2d_grad_ndc, nullptr);
+}
+
+int u3d_raster_radii(u3d_raster *r, const int **radii, int64_t *n)
+{
+    U3D_CHECK(r && radii && n, U3D_ERR_INVALID, "null argument");
+    *radii = r->radius.p;
+    *n = r->haveForward ? r->n : 0;
+    return U3D_OK;
+}
+
+int u3d_raster_get_state(u3d_raster *r, u3d_raster_state *out)
+{
+    U3D_CHECK(r && out, U3D_ERR_INVALID, "null argument");
+    U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "no forward pass yet");
+    U3D_TRY(useDevice(r->ctx));
+    const size_t nn = size_t(r->n > 0 ? r->n : 1);
+    U3D_TRY(r->dbgMean2d.ensure(nn * 2));
+    U3D_TRY(r->dbgConic.ensure(nn * 3));
+    U3D_TRY(r->dbgColor.ensure(nn * 3));
+    U3D_TRY(r->dbgOpacity.ensure(nn));
+    U3D_TRY(launchUnpackScreen(r->ctx->stream, r->n, r->screen(), r->dbgMean2d.p, r->dbgConic.p,
+                               r->dbgColor.p, r->dbgOpacity.p));
+    U3D_HIP(hipStreamSynchronize(r->ctx->stream));
+    out->n = r->n;
+    out->num_pairs = r->numPairs;
+    out->tiles_x = r->geo.tilesX;
+    out->tiles_y = r->geo.tilesY;
+    out->radius = r->radius.p;
+    out->rect = reinterpret_cast<const int *>(r->rect.p);
+    out->depth = r->depth.p;
+    out->mean2d = r->dbgMean2d.p;
+    out->conic = r->dbgConic.p;
+    out->color = r->dbgColor.p;
+    out->opacity = r->dbgOpacity.p;
+    out->flags = r->flags.p;
+    out->tile_offsets = r->tileOffsets.p;
+    out->tile_list = r->tileList;
+    out->final_t = r->finalT.p;
+    out->n_contrib = r->nContrib.p;
+    return U3D_OK;
+}
+
+int u3d_raster_get_grad2d(u3d_raster *r, const float **grad2d)
+{
+    U3D_CHECK(r && grad2d, U3D_ERR_INVALID, "null argument");
+    *grad2d = r->grad2d.p;
+    return U3D_OK;
+}
+
+// ------------------------------------------------------------------------------------ loss
+int u3d_photometric_loss(u3d_ctx *ctx, const float *render, const float *target, int width,
+                         int height, float ssim_lambda, float *grad, u3d_loss_value *out_host)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(width >= 0 && height >= 0, U3D_ERR_INVALID, "negative image size");
+    U3D_CHECK((render && target && grad) || size_t(width) * size_t(height) == 0, U3D_ERR_INVALID, "null image");
+    const size_t np = lossPartialsSize(width, height) + 1;
+    unsigned char *sc = nullptr;
+    U3D_TRY(ctxScratch(ctx, np * sizeof(double) + 64, &sc));
+    double *partials = reinterpret_cast<double *>(sc);
+    float *result = reinterpret_cast<float *>(sc + np * sizeof(double));
+    U3D_TRY(launchPhotometricLoss(ctx->stream, render, target, nullptr, width, height, ssim_lambda,
+                                  grad, partials, np, result));
+    float host[4];
+    U3D_HIP(hipMemcpyAsync(host, result, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
+    U3D_HIP(hipStreamSynchronize(ctx->stream));
+    if (out_host)
+        *out_host = {host[0], host[1], host[2]};
+    return U3D_OK;
+}
+
+int u3d_psnr(u3d_ctx *ctx, const float *a, const float *b, int64_t count, float *out_host)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(out_host != nullptr, U3D_ERR_INVALID, "null out");
+    const size_t np = psnrPartialsSize(count);
+    unsigned char *sc = nullptr;
+    U3D_TRY(ctxScratch(ctx, np * sizeof(double) + 64, &sc));
+    float *result = reinterpret_cast<float *>(sc + np * sizeof(double));
+    U3D_TRY(launchPsnr(ctx->stream, a, b, nullptr, count, reinterpret_cast<double *>(sc), np, result));
+    U3D_HIP(hipMemcpyAsync(out_host, result, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    U3D_HIP(hipStreamSynchronize(ctx->stream));
+    return U3D_OK;
+}
+
+// ------------------------------------------------------------------------------------ adam
+int u3d_adam_create(u3d_ctx *ctx, u3d_adam **out)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(out != nullptr, U3D_ERR_INVALID, "null out");
+    *out = new u3d_adam();
+    (*out)->ctx = ctx;
+    return U3D_OK;
+}
+
+void u3d_adam_destroy(u3d_adam *a)
+{
+    if (!a)
+        return;
+    (void)hipSetDevice(a->ctx->device);
+    (void)hipStreamSynchronize(a->ctx->stream);
+    delete a;
+}
+
+int u3d_adam_reset(u3d_adam *a, int64_t n)
+{
+    U3D_CHECK(a && n >= 0, U3D_ERR_INVALID, "bad argument");
+    U3D_TRY(useDevice(a->ctx));
+    U3D_TRY(a->m.reserve(n > 0 ? n : 1));
+    U3D_TRY(a->v.reserve(n > 0 ? n : 1));
+    a->m.n = a->v.n = n;
+    U3D_TRY(a->m.zero(a->ctx->stream));
+    U3D_TRY(a->v.zero(a->ctx->stream));
+    a->step = 0;
+    return U3D_OK;
+}
+
+namespace {
+
+// Compacts `src` by keep (device, n entries) into `dst`, appending `extraZeros` zeroed entries.
+int compactModel(u3d_ctx *ctx, const DevModel &src, DevModel &dst, const uint8_t *keep,
+                 int64_t extraZeros, int64_t *newN)
+{
+    const hipStream_t s = ctx->stream;
+    const int64_t n = src.n;
+    DevBuf<uint32_t> flag, pos, counts, scratch;
+    const size_t nn = size_t(n > 0 ? n : 1);
+    U3D_TRY(flag.ensure(nn));
+    U3D_TRY(pos.ensure(nn));
+    U3D_TRY(counts.ensure(4));
+    U3D_TRY(scratch.ensure(scanScratchSize(int64_t(nn))));
+    DevBuf<uint8_t> noKind;
+    U3D_TRY(noKind.ensure(nn));
+    U3D_HIP(hipMemsetAsync(noKind.p, 0, nn, s));
+    DevBuf<uint32_t> extra, split;
+    U3D_TRY(extra.ensure(nn));
+    U3D_TRY(split.ensure(nn));
+    U3D_TRY(launchDensifyFlags(s, n, keep, noKind.p, flag.p, extra.p, split.p));
+    U3D_TRY(exclusiveScanU32(s, flag.p, pos.p, n, counts.p + 2, scratch.p, scratch.cap));
+    uint32_t host[4] = {uint32_t(extraZeros), 0, 0, 0};
+    U3D_HIP(hipMemcpyAsync(&host[2], counts.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    const int64_t total = int64_t(host[2]) + extraZeros;
+    U3D_TRY(dst.reserve(std::max<int64_t>(std::max<int64_t>(total, src.cap), 1)));
+    U3D_HIP(hipMemcpyAsync(counts.p, host, sizeof host, hipMemcpyHostToDevice, s));
+    DevModel &d = dst;
+    d.n = total;
+    u3d_gaussians sv = src.view(), dv = d.view();
+    // tail zeros: compactZeroTail zeroes [nKeep, nKeep + clones + 2*splits); extras given as clones
+    const int64_t grid = std::max<int64_t>(n, extraZeros);
+    U3D_TRY(launchCompactZeroTail(s, sv, dv, grid, keep, pos.p, counts.p));
+    U3D_HIP(hipStreamSynchronize(s));
+    *newN = total;
+    return U3D_OK;
+}
+
+} // namespace
+
+int u3d_adam_compact(u3d_adam *a, const uint8_t *keep)
+{
+    U3D_CHECK(a && (keep || a->m.n == 0), U3D_ERR_INVALID, "bad argument");
+    U3D_TRY(useDevice(a->ctx));
+    int64_t nn = 0;
+    U3D_TRY(compactModel(a->ctx, a->m, a->spare, keep, 0, &nn));
+    a->m.swap(a->spare);
+    U3D_TRY(compactModel(a->ctx, a->v, a->spare, keep, 0, &nn));
+    a->v.swap(a->spare);
+    return U3D_OK;
+}
+
+int u3d_adam_grow(u3d_adam *a, int64_t count)
+{
+    U3D_CHECK(a && count >= 0, U3D_ERR_INVALID, "bad argument");
+    U3D_TRY(useDevice(a->ctx));
+    const hipStream_t s = a->ctx->stream;
+    for (DevModel *mm : {&a->m, &a->v}) {
+        const int64_t old = mm->n, nn = old + count;
+        if (nn > mm->cap) {
+            DevModel bigger;
+            U3D_TRY(bigger.reserve(nn));
+            U3D_TRY(bigger.zero(s));
+            bigger.n = nn;
+            U3D_TRY(copyFields(s, mm->view(), bigger.view(), old, hipMemcpyDeviceToDevice));
+            U3D_HIP(hipStreamSynchronize(s));
+            mm->swap(bigger);
+        } else {
+            mm->n = nn;
+            const u3d_gaussians v = mm->view();
+            const size_t f = sizeof(float);
+            if (count) {
+                U3D_HIP(hipMemsetAsync(v.means + old * 3, 0, size_t(count) * 3 * f, s));
+                U3D_HIP(hipMemsetAsync(v.scales + old * 3, 0, size_t(count) * 3 * f, s));
+                U3D_HIP(hipMemsetAsync(v.quats + old * 4, 0, size_t(count) * 4 * f, s));
+                U3D_HIP(hipMemsetAsync(v.opacities + old, 0, size_t(count) * f, s));
+                U3D_HIP(hipMemsetAsync(v.colors + old * 3, 0, size_t(count) * 3 * f, s));
+            }
+        }
+    }
+    return U3D_OK;
+}
+
+int u3d_adam_reset_opacity_moments(u3d_adam *a)
+{
+    U3D_CHECK(a != nullptr, U3D_ERR_INVALID, "null adam");
+    U3D_TRY(useDevice(a->ctx));
+    if (a->m.n > 0) {
+        U3D_HIP(hipMemsetAsync(a->m.view().opacities, 0, size_t(a->m.n) * sizeof(float), a->ctx->stream));
+        U3D_HIP(hipMemsetAsync(a->v.view().opacities, 0, size_t(a->v.n) * sizeof(float), a->ctx->stream));
+    }
+    return U3D_OK;
+}
+
+namespace {
+void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2)
+{
+    // Adam.cpp:16-17, 70-72 (host powf, as the reference).
+    const float bc1 = 1.0f - std::pow(0.9f, float(step));
+    const float bc2 = 1.0f - std::pow(0.999f, float(step));
+    const float rates[5] = {lr.means, lr.scales, lr.quats, lr.opacities, lr.colors};
+    for (int k = 0; k < 5; ++k)
+        stepSize[k] = rates[k] / bc1;
+    *invSqrtBc2 = 1.0f / std::sqrt(bc2);
+}
+} // namespace
+
+int u3d_adam_step(u3d_adam *a, u3d_gaussians *params, const u3d_gaussians *grads,
+                  const u3d_learning_rates *lr)
+{
+    U3D_CHECK(a && lr, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(a->ctx));
+    U3D_TRY(checkModel(params, "params"));
+    U3D_TRY(checkModel(grads, "grads"));
+    U3D_CHECK(params->n == a->m.n && grads->n == a->m.n, U3D_ERR_INVALID, "size differs from the moments");
+    ++a->step;
+    float stepSize[5], inv;
+    adamCoefficients(a->step, *lr, stepSize, &inv);
+    u3d_gaussians m = a->m.view(), v = a->v.view();
+    return launchAdamStep(a->ctx->stream, *params, *grads, m, v, stepSize, inv);
+}
+
+int u3d_adam_moments(u3d_adam *a, u3d_gaussians *m, u3d_gaussians *v, int *step)
+{
+    U3D_CHECK(a != nullptr, U3D_ERR_INVALID, "null adam");
+    if (m)
+        *m = a->m.view();
+    if (v)
+        *v = a->v.view();
+    if (step)
+        *step = a->step;
+    return U3D_OK;
+}
+
+// ------------------------------------------------------------------------------ densification
+namespace {
+
+struct DensifyScratch {
+    DevBuf<uint32_t> demand, scan, keepFlag, extraCount, splitFlag, keepPos, extraPos, splitPos;
+    DevBuf<uint32_t> counts, misc, scanScratch;
+    int ensure(int64_t n)
+    {
+        const size_t nn = size_t(n > 0 ? n : 1);
+        for (DevBuf<uint32_t> *b : {&demand, &scan, &keepFlag, &extraCount, &splitFlag, &keepPos,
+                                    &extraPos, &splitPos})
+            U3D_TRY(b->ensure(nn));
+        U3D_TRY(counts.ensure(8));
+        U3D_TRY(misc.ensure(8));
+        U3D_TRY(scanScratch.ensure(scanScratchSize(int64_t(nn))));
+        return U3D_OK;
+    }
+};
+
+int densifyDecideImpl(u3d_ctx *ctx, DensifyScratch &ds, const u3d_gaussians &m, const float *accum,
+                      const int *count, float extent, int maxG, float thr, float pd, float po,
+                      int wasReset, uint8_t *keep, uint8_t *kind, uint32_t hostCounts[4])
+{
+    const hipStream_t s = ctx->stream;
+    const int64_t n = m.n;
+    U3D_TRY(ds.ensure(n));
+    U3D_TRY(launchDensifyDemand(s, m, accum, count, extent, thr, pd, po, wasReset, keep, ds.demand.p));
+    U3D_TRY(exclusiveScanU32(s, ds.demand.p, ds.scan.p, n, ds.misc.p + 4, ds.scanScratch.p,
+                             ds.scanScratch.cap));
+    const uint64_t budget = int64_t(maxG) > n ? uint64_t(int64_t(maxG) - n) : 0; // Trainer.cpp:109
+    U3D_TRY(launchDensifyAccept(s, n, ds.demand.p, ds.scan.p, budget, keep, kind, ds.misc.p, ds.counts.p));
+    U3D_HIP(hipMemcpyAsync(hostCounts, ds.counts.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    return U3D_OK;
+}
+
+// Scatter into dst (capacity checked by the caller); counts must be on the device.
+int densifyApplyImpl(u3d_ctx *ctx, DensifyScratch &ds, const u3d_gaussians &src,
+                     u3d_gaussians &dst, const uint8_t *keep, const uint8_t *kind,
+                     const float *normals, float logShrink)
+{
+    const hipStream_t s = ctx->stream;
+    const int64_t n = src.n;
+    U3D_TRY(ds.ensure(n));
+    U3D_TRY(launchDensifyFlags(s, n, keep, kind, ds.keepFlag.p, ds.extraCount.p, ds.splitFlag.p));
+    U3D_TRY(exclusiveScanU32(s, ds.keepFlag.p, ds.keepPos.p, n, ds.misc.p + 4, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_TRY(exclusiveScanU32(s, ds.extraCount.p, ds.extraPos.p, n, ds.misc.p + 5, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_TRY(exclusiveScanU32(s, ds.splitFlag.p, ds.splitPos.p, n, ds.misc.p + 6, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_TRY(launchDensifyScatter(s, src, dst, n, keep, kind, ds.keepPos.p, ds.extraPos.p,
+                                 ds.splitPos.p, ds.counts.p, normals, logShrink));
+    return U3D_OK;
+}
+
+} // namespace
+
+int u3d_densify_decide(u3d_ctx *ctx, const u3d_gaussians *model, const float *grad_accum,
+                       const int *grad_count, float extent, int max_gaussians,
+                       float grad_threshold, float percent_dense, float prune_opacity,
+                       int opacity_was_reset, uint8_t *keep, uint8_t *kind,
+                       int64_t *n_clone_host, int64_t *n_split_host, int64_t *n_keep_host)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_TRY(checkModel(model, "model"));
+    U3D_CHECK(model->n == 0 || (grad_accum && grad_count && keep && kind), U3D_ERR_INVALID, "null argument");
+    DensifyScratch ds;
+    uint32_t c[4];
+    U3D_TRY(densifyDecideImpl(ctx, ds, *model, grad_accum, grad_count, extent, max_gaussians,
+                              grad_threshold, percent_dense, prune_opacity, opacity_was_reset, keep,
+                              kind, c));
+    if (n_clone_host) *n_clone_host = c[0];
+    if (n_split_host) *n_split_host = c[1];
+    if (n_keep_host) *n_keep_host = c[2];
+    return U3D_OK;
+}
+
+int u3d_densify_apply(u3d_ctx *ctx, u3d_gaussians *model, const uint8_t *keep,
+                      const uint8_t *kind, const float *normals, float log_shrink,
+                      int64_t *new_n_host)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_TRY(checkModel(model, "model"));
+    const hipStream_t s = ctx->stream;
+    const int64_t n = model->n;
+    DensifyScratch ds;
+    U3D_TRY(ds.ensure(n));
+    // counts from keep / kind
+    U3D_HIP(hipMemsetAsync(ds.counts.p, 0, 4 * sizeof(uint32_t), s));
+    U3D_HIP(hipMemsetAsync(ds.misc.p, 0xff, 2 * sizeof(uint32_t), s));
+    // densifyCount (through launchDensifyAccept's counting kernel) needs kind writable; use a copy
+    DevBuf<uint8_t> kindCopy;
+    U3D_TRY(kindCopy.ensure(size_t(n > 0 ? n : 1)));
+    if (n > 0)
+        U3D_HIP(hipMemcpyAsync(kindCopy.p, kind, size_t(n), hipMemcpyDeviceToDevice, s));
+    U3D_TRY(launchDensifyFlags(s, n, keep, kindCopy.p, ds.keepFlag.p, ds.extraCount.p, ds.splitFlag.p));
+    uint32_t tot[3] = {0, 0, 0};
+    U3D_TRY(exclusiveScanU32(s, ds.keepFlag.p, ds.keepPos.p, n, ds.misc.p + 4, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_TRY(exclusiveScanU32(s, ds.extraCount.p, ds.extraPos.p, n, ds.misc.p + 5, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_TRY(exclusiveScanU32(s, ds.splitFlag.p, ds.splitPos.p, n, ds.misc.p + 6, ds.scanScratch.p, ds.scanScratch.cap));
+    U3D_HIP(hipMemcpyAsync(tot, ds.misc.p + 4, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    const uint32_t splits = tot[2], clones = tot[1] - 2 * tot[2], kept = tot[0];
+    const uint32_t counts[4] = {clones, splits, kept, 0};
+    U3D_HIP(hipMemcpyAsync(ds.counts.p, counts, sizeof counts, hipMemcpyHostToDevice, s));
+    const int64_t total = int64_t(kept) + int64_t(tot[1]);
+    DevModel tmp;
+    U3D_TRY(tmp.reserve(std::max<int64_t>(total, 1)));
+    tmp.n = total;
+    u3d_gaussians dv = tmp.view();
+    U3D_TRY(launchDensifyScatter(s, *model, dv, n, keep, kindCopy.p, ds.keepPos.p, ds.extraPos.p,
+                                 ds.splitPos.p, ds.counts.p, normals, log_shrink));
+    U3D_TRY(copyFields(s, dv, *model, total, hipMemcpyDeviceToDevice));
+    U3D_HIP(hipStreamSynchronize(s));
+    model->n = total;
+    if (new_n_host)
+        *new_n_host = total;
+    return U3D_OK;
+}
+
+// -------------------------------------------------------------------------------- trainer
+int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, const float *quats,
+                       const float *opacities, const float *colors, int64_t n,
+                       const u3d_camera *cameras, const uint8_t *rgb8, int n_images,
+                       float scene_extent, const u3d_train_options *options, u3d_trainer **out)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(out && options && n >= 0 && n_images >= 0, U3D_ERR_INVALID, "bad argument");
+    U3D_CHECK(n == 0 || (means && scales && quats && opacities && colors), U3D_ERR_INVALID, "null model");
+    U3D_CHECK(n_images == 0 || (cameras && rgb8), U3D_ERR_INVALID, "null images");
+    U3D_CHECK(options->raster.tile_size > 0 && options->raster.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
+              "tile_size must be a positive multiple of 16");
+    std::unique_ptr<u3d_trainer> t(new u3d_trainer());
+    t->ctx = ctx;
+    t->raster.ctx = ctx;
+    t->opt = *options;
+    t->extent = std::max(scene_extent, 1e-6f); // Trainer.cpp:41
+    t->rng.seed(options->seed);
+    const hipStream_t s = ctx->stream;
+    const int64_t cap = std::max<int64_t>(std::max<int64_t>(n, options->max_gaussians), 1);
+    for (DevModel *m : {&t->params, &t->spare, &t->grads, &t->adamM, &t->adamV}) {
+        U3D_TRY(m->reserve(cap));
+        m->n = n;
+    }
+    U3D_TRY(t->adamM.zero(s));
+    U3D_TRY(t->adamV.zero(s));
+    const u3d_gaussians host{const_cast<float *>(means), const_cast<float *>(scales),
+                             const_cast<float *>(quats), const_cast<float *>(opacities),
+                             const_cast<float *>(colors), n};
+    U3D_TRY(copyFields(s, host, t->params.view(), n, hipMemcpyHostToDevice));
+    size_t total = 0, maxPix = 1;
+    for (int k = 0; k < n_images; ++k) {
+        U3D_CHECK(cameras[k].width >= 0 && cameras[k].height >= 0, U3D_ERR_INVALID, "negative image size");
+        t->cams.push_back(cameras[k]);
+        t->imageOffset.push_back(total);
+        const size_t p = size_t(cameras[k].width) * size_t(cameras[k].height);
+        total += p * 3;
+        maxPix = std::max(maxPix, p);
+    }
+    U3D_TRY(t->targets.ensure(total > 0 ? total : 1));
+    if (total)
+        U3D_HIP(hipMemcpyAsync(t->targets.p, rgb8, total, hipMemcpyHostToDevice, s));
+    U3D_TRY(t->render.ensure(maxPix * 3));
+    U3D_TRY(t->dloss.ensure(maxPix * 3));
+    U3D_TRY(t->gradAccum.ensure(size_t(cap)));
+    U3D_TRY(t->gradCount.ensure(size_t(cap)));
+    U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(cap) * sizeof(float), s));
+    U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(cap) * sizeof(int), s));
+    U3D_TRY(t->stepStats.ensure(size_t(std::max(1, options->iterations)) * 4));
+    U3D_TRY(t->evalOut.ensure(size_t(std::max(1, n_images)) + 4));
+    size_t parts = 1024;
+    for (const u3d_camera &c : t->cams)
+        parts = std::max(parts, lossPartialsSize(c.width, c.height) + 1);
+    U3D_TRY(t->partials.ensure(parts));
+    U3D_HIP(hipStreamSynchronize(s));
+    t->stats.gaussians = n;
+    *out = t.release();
+    return U3D_OK;
+}
+
+void u3d_trainer_destroy(u3d_trainer *t)
+{
+    if (!t)
+        return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    delete t;
+}
+
+int u3d_trainer_done(u3d_trainer *t) { return t && t->iteration >= t->opt.iterations ? 1 : 0; }
+int u3d_trainer_iteration(u3d_trainer *t) { return t ? t->iteration : 0; }
+int64_t u3d_trainer_size(u3d_trainer *t) { return t ? t->params.n : 0; }
+
+namespace {
+
+int trainerDensify(u3d_trainer *t)
+{
+    // Trainer::densifyAndPrune, Trainer.cpp:105-169.
+    u3d_ctx *ctx = t->ctx;
+    const hipStream_t s = ctx->stream;
+    const int64_t n = t->params.n;
+    DensifyScratch ds;
+    DevBuf<uint8_t> keep, kind;
+    U3D_TRY(keep.ensure(size_t(n > 0 ? n : 1)));
+    U3D_TRY(kind.ensure(size_t(n > 0 ? n : 1)));
+    uint32_t c[4];
+    const u3d_gaussians pv = t->params.view();
+    U3D_TRY(densifyDecideImpl(ctx, ds, pv, t->gradAccum.p, t->gradCount.p, t->extent,
+                              t->opt.max_gaussians, t->opt.densify_grad_threshold,
+                              t->opt.percent_dense, t->opt.prune_opacity, t->opacityWasReset ? 1 : 0,
+                              keep.p, kind.p, c));
+    // Split offsets: 6 draws per accepted split from a fresh N(0, 1) over the trainer's
+    // generator, in index order (Trainer.cpp:114, 150).
+    std::normal_distribution<float> normal(0.0f, 1.0f);
+    std::vector<float> normals(size_t(c[1]) * 6);
+    for (float &x : normals)
+        x = normal(t->rng);
+    DevBuf<float> dn;
+    U3D_TRY(dn.ensure(normals.size() > 0 ? normals.size() : 1));
+    if (!normals.empty())
+        U3D_HIP(hipMemcpyAsync(dn.p, normals.data(), normals.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    volatile float shrinkArg = 1.6f;
+    const float logShrink = std::log(float(shrinkArg));
+    const int64_t newN = int64_t(c[2]) + int64_t(c[0]) + 2 * int64_t(c[1]);
+    U3D_CHECK(newN <= t->params.cap, U3D_ERR_CAPACITY, "densification exceeds the model capacity");
+    t->spare.n = newN;
+    u3d_gaussians sv = t->spare.view();
+    U3D_TRY(densifyApplyImpl(ctx, ds, pv, sv, keep.p, kind.p, dn.p, logShrink));
+    t->params.swap(t->spare);
+    t->params.n = newN;
+    // Adam moments: compacted, zeros appended (Adam.cpp:40-60).
+    for (DevModel *mm : {&t->adamM, &t->adamV}) {
+        t->spare.n = newN;
+        u3d_gaussians src = mm->view(), dst = t->spare.view();
+        src.n = n;
+        U3D_TRY(launchCompactZeroTail(s, src, dst, n, keep.p, ds.keepPos.p, ds.counts.p));
+        mm->swap(t->spare);
+        mm->n = newN;
+    }
+    t->grads.n = newN;
+    t->spare.n = 0;
+    // Statistics restart (Trainer.cpp:167-168).
+    if (newN > 0) {
+        U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(newN) * sizeof(float), s));
+        U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(newN) * sizeof(int), s));
+    }
+    U3D_HIP(hipStreamSynchronize(s));
+    return U3D_OK;
+}
+
+int trainerResetOpacity(u3d_trainer *t)
+{
+    // Trainer::resetOpacity, Trainer.cpp:171-178.
+    const hipStream_t s = t->ctx->stream;
+    volatile float p = 0.01f;
+    const float pf = p;
+    const float cap = std::log(pf / (1.0f - pf)); // logit(0.01f), GaussianModel.h:12
+    const int64_t n = t->params.n;
+    U3D_TRY(launchOpacityCap(s, t->params.view().opacities, n, cap));
+    if (n > 0) {
+        U3D_HIP(hipMemsetAsync(t->adamM.view().opacities, 0, size_t(n) * sizeof(float), s));
+        U3D_HIP(hipMemsetAsync(t->adamV.view().opacities, 0, size_t(n) * sizeof(float), s));
+    }
+    t->opacityWasReset = true;
+    return U3D_OK;
+}
+
+} // namespace
+
+int u3d_trainer_step(u3d_trainer *t)
+{
+    // Trainer::step, Trainer.cpp:53-103.
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_TRY(useDevice(t->ctx));
+    if (t->cams.empty() || u3d_trainer_done(t))
+        return U3D_OK;
+    if (t->orderPos >= t->order.size()) {
+        t->order.resize(t->cams.size());
+        for (size_t k = 0; k < t->order.size(); ++k)
+            t->order[k] = int(k);
+        std::shuffle(t->order.begin(), t->order.end(), t->rng);
+        t->orderPos = 0;
+    }
+    const size_t view = size_t(t->order[t->orderPos++]);
+    const u3d_camera &cam = t->cams[view];
+    const hipStream_t s = t->ctx->stream;
+    const u3d_gaussians pv = t->params.view();
+    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p));
+    t->counters[0] += 1;
+    t->counters[1] += t->params.n;
+    t->counters[2] += int64_t(cam.width) * cam.height;
+    t->counters[3] += t->raster.numPairs;
+    float *stats = t->stepStats.p + size_t(t->iteration) * 4;
+    {
+        StageScope scope(t->ctx, U3D_STAGE_LOSS);
+        U3D_TRY(launchPhotometricLoss(s, t->render.p, nullptr, t->targets.p + t->imageOffset[view],
+                                      cam.width, cam.height, t->opt.ssim_lambda, t->dloss.p,
+                                      t->partials.p, t->partials.cap, stats));
+    }
+    t->grads.n = t->params.n;
+    if (t->grads.n > 0)
+        U3D_HIP(hipMemsetAsync(t->grads.buf.p, 0, size_t(t->grads.cap) * 14 * sizeof(float), s));
+    u3d_gaussians gv = t->grads.view();
+    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p));
+    // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
+    u3d_learning_rates lr = t->opt.lr;
+    const float tt = float(t->iteration) / float(std::max(1, t->opt.iterations - 1));
+    lr.means = std::exp(std::log(t->opt.lr.means) * (1 - tt) + std::log(t->opt.means_lr_final) * tt) * t->extent;
+    ++t->adamStep;
+    float stepSize[5], inv;
+    adamCoefficients(t->adamStep, lr, stepSize, &inv);
+    u3d_gaussians pm = t->params.view(), mm = t->adamM.view(), vm = t->adamV.view();
+    {
+        StageScope scope(t->ctx, U3D_STAGE_ADAM);
+        U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv));
+    }
+    ++t->iteration;
+    const u3d_train_options &o = t->opt;
+    const bool densify = t->iteration >= o.refine_start && t->iteration < o.refine_stop
+                         && o.refine_interval > 0 && t->iteration % o.refine_interval == 0;
+    const bool reset = o.opacity_reset_interval > 0 && t->iteration < o.refine_stop
+                       && t->iteration % o.opacity_reset_interval == 0;
+    if (densify || reset) {
+        StageScope scope(t->ctx, U3D_STAGE_DENSIFY);
+        if (densify)
+            U3D_TRY(trainerDensify(t));
+        if (reset)
+            U3D_TRY(trainerResetOpacity(t));
+    }
+    return U3D_OK;
+}
+
+int u3d_trainer_get_model(u3d_trainer *t, float *means, float *scales, float *quats,
+                          float *opacities, float *colors)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_TRY(useDevice(t->ctx));
+    const int64_t n = t->params.n;
+    U3D_CHECK(n == 0 || (means && scales && quats && opacities && colors), U3D_ERR_INVALID, "null output");
+    const u3d_gaussians host{means, scales, quats, opacities, colors, n};
+    U3D_TRY(copyFields(t->ctx->stream, t->params.view(), host, n, hipMemcpyDeviceToHost));
+    U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    return U3D_OK;
+}
+
+int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out)
+{
+    // TrainStats EMA (Trainer.cpp:88-92), folded lazily from the per-step device values.
+    U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(t->ctx));
+    const int from = t->statsFolded, to = t->iteration;
+    if (to > from) {
+        std::vector<float> v(size_t(to - from) * 4);
+        U3D_HIP(hipMemcpyAsync(v.data(), t->stepStats.p + size_t(from) * 4, v.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+        for (int it = from; it < to; ++it) {
+            const float *x = &v[size_t(it - from) * 4];
+            const float k = t->stats.iteration == 0 ? 1.0f : 0.05f;
+            t->stats.loss += k * (x[0] - t->stats.loss);
+            t->stats.psnr += k * (x[3] - t->stats.psnr);
+            t->stats.iteration = it + 1;
+        }
+        t->statsFolded = to;
+    }
+    t->stats.gaussians = t->params.n;
+    *out = t->stats;
+    return U3D_OK;
+}
+
+namespace {
+int evaluateAll(u3d_trainer *t, int mode, float *out)
+{
+    U3D_TRY(useDevice(t->ctx));
+    if (t->cams.empty()) {
+        *out = 0.0f;
+        return U3D_OK;
+    }
+    const hipStream_t s = t->ctx->stream;
+    const u3d_gaussians pv = t->params.view();
+    for (size_t k = 0; k < t->cams.size(); ++k) {
+        const u3d_camera &cam = t->cams[k];
+        U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p));
+        const int64_t count = int64_t(cam.width) * cam.height * 3;
+        if (mode == 0)
+            U3D_TRY(launchPsnr(s, t->render.p, nullptr, t->targets.p + t->imageOffset[k], count,
+                               t->partials.p, t->partials.cap, t->evalOut.p + k));
+        else
+            U3D_TRY(launchMeanAbs(s, t->render.p, t->targets.p + t->imageOffset[k], count,
+                                  t->partials.p, t->partials.cap, t->evalOut.p + k));
+    }
+    std::vector<float> v(t->cams.size());
+    U3D_HIP(hipMemcpyAsync(v.data(), t->evalOut.p, v.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    double sum = 0;
+    for (float x : v)
+        sum += x;
+    *out = float(sum / double(v.size()));
+    return U3D_OK;
+}
+} // namespace
+
+int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out)
+{
+    U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
+    return evaluateAll(t, 0, out);
+}
+
+int u3d_trainer_evaluate_l1(u3d_trainer *t, float *out)
+{
+    U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
+    return evaluateAll(t, 1, out);
+}
+
+int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host)
+{
+    U3D_CHECK(t && camera && rgb_host, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(t->ctx));
+    const size_t count = size_t(camera->width) * size_t(camera->height) * 3;
+    DevBuf<float> out;
+    U3D_TRY(out.ensure(count > 0 ? count : 1));
+    U3D_TRY(rasterForwardImpl(&t->raster, t->params.view(), *camera, t->opt.raster, out.p));
+    U3D_HIP(hipMemcpyAsync(rgb_host, out.p, count * sizeof(float), hipMemcpyDeviceToHost, t->ctx->stream));
+    U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    return U3D_OK;
+}
+
+int u3d_ctx_set_timing(u3d_ctx *ctx, int enabled)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timer.collect();
+    for (int k = 0; k < U3D_NUM_STAGES; ++k) {
+        ctx->timer.totalMs[k] = 0.0;
+        ctx->timer.count[k] = 0;
+    }
+    ctx->timer.enabled = enabled != 0;
+    return U3D_OK;
+}
+
+int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES], int64_t counts[U3D_NUM_STAGES])
+{
+    U3D_TRY(useDevice(ctx));
+    ctx->timer.collect();
+    for (int k = 0; k < U3D_NUM_STAGES; ++k) {
+        if (total_ms)
+            total_ms[k] = ctx->timer.totalMs[k];
+        if (counts)
+            counts[k] = ctx->timer.count[k];
+    }
+    return U3D_OK;
+}
+
+int u3d_trainer_counters(u3d_trainer *t, int64_t values[4])
+{
+    U3D_CHECK(t && values, U3D_ERR_INVALID, "null argument");
+    for (int k = 0; k < 4; ++k)
+        values[k] = t->counters[k];
+    return U3D_OK;
+}
+
+void u3d_expf_host_array(const float *x, float *y, int64_t n)
+{
+    static const uint64_t table[32] = U3D_EXP2_TABLE;
+    for (int64_t i = 0; i < n; ++i) {
+        bool special;
+        const float sp = u3dmath::expf_special(x[i], &special);
+        y[i] = special ? sp : u3dmath::expf_core(x[i], table);
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,124 @@
+// Host-side launchers of the device kernels (one .hip translation unit each). All launch on
+// the given stream and return U3D_OK or an error code; none of them synchronises.
+#pragma once
+#include "common.h"
+
+namespace u3d {
+
+// Per-Gaussian screen-space state written by the projection kernel. rec0/rec1/rec2 pack what
+// the compositing kernels read per (tile, Gaussian) pair: {mx, my, A, B}, {C, alpha, r, g}, b.
+struct ScreenBuffers {
+    float4 *rec0;
+    float4 *rec1;
+    float *rec2;
+    float *depth;
+    int *radius;
+    int4 *rect;
+    uint8_t *flags;
+    uint32_t *depthKey; // depth bits, 0xffffffff when not visible
+    uint32_t *nTiles;   // tiles covered, 0 when not visible
+};
+
+// ---- kernels_project.hip
+int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
+                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out);
+// Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
+// grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
+int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
+                    u3d_gaussians &grads, float *ndc, int *gradCount);
+// Unpacks the records into the separate views of u3d_raster_state.
+int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float *mean2d,
+                       float *conic, float *color, float *opacity);
+
+// ---- kernels_sort.hip
+// Stable LSD radix sort of (key, value) pairs on bits [0, bits) of the key, `count` elements
+// (device pointer to the count when countDev != nullptr, else `count`). Ping-pongs between
+// (keys, vals) and (keysAlt, valsAlt); returns in *resultInAlt where the output landed.
+struct RadixScratch {
+    uint32_t *hist; // >= 256 * maxBlocks
+    size_t histCap;
+};
+int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
+                   uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
+                   bool *resultInAlt);
+size_t radixHistSize(int64_t count);
+// values[i] = i
+int launchIota(hipStream_t s, uint32_t *vals, int64_t count);
+// out[i] = in[idx[i]] for i < count
+int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
+                    int64_t count);
+// Exclusive scan of count u32 values (out may alias in); total written to *total (device).
+int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t count,
+                     uint32_t *total, uint32_t *blockScratch, size_t blockScratchCap);
+size_t scanScratchSize(int64_t count);
+// Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
+// offsets[i] its first pair slot.
+int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
+                    const int4 *rect, int tilesX, uint32_t *pairTile, uint32_t *pairGauss,
+                    const uint32_t *nTilesSorted);
+// tileOffsets[t] = first pair index of tile t, tileOffsets[tiles] = K (tile ids sorted).
+int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs, int tiles,
+                      int *tileOffsets);
+
+// ---- kernels_raster.hip
+struct RasterGeometry {
+    int width, height, tileSize, tilesX, tilesY;
+};
+int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                        const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                        float *rgb, float *finalT, int *nContrib);
+int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                         const float *dlDrgb, const float *finalT, const int *nContrib,
+                         float *grad2d);
+
+// ---- kernels_loss.hip
+// Fused L1 + D-SSIM value and gradient. target is float (targetU8 == nullptr) or uint8/255.
+// Writes per-block double partials (l1, ssim, squared error) into partials and the final
+// {total, l1, ssim, psnr} floats into result (device) via a one-block finalize kernel.
+int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
+                          const uint8_t *targetU8, int width, int height, float lambda,
+                          float *grad, double *partials, size_t partialsCap, float *result);
+size_t lossPartialsSize(int width, int height);
+// Squared-error sum of two float arrays -> result[0] = psnr (double partials scratch).
+int launchPsnr(hipStream_t s, const float *a, const float *b, const uint8_t *bU8, int64_t count,
+               double *partials, size_t partialsCap, float *result);
+size_t psnrPartialsSize(int64_t count);
+// Mean |a - b/255| -> result[0] (double partials).
+int launchMeanAbs(hipStream_t s, const float *a, const uint8_t *bU8, int64_t count,
+                  double *partials, size_t partialsCap, float *result);
+
+// ---- kernels_adam.hip
+int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_gaussians &m,
+                   u3d_gaussians &v, const float stepSize[5], float invSqrtBc2);
+int launchFill(hipStream_t s, float *p, float value, int64_t count);
+int launchFillI32(hipStream_t s, int *p, int value, int64_t count);
+// o = min(o, cap) (Trainer.cpp:171-178)
+int launchOpacityCap(hipStream_t s, float *opacities, int64_t n, float cap);
+// Densification: per-Gaussian demand (Trainer.cpp:116-135 before the budget test).
+int launchDensifyDemand(hipStream_t s, const u3d_gaussians &m, const float *accum,
+                        const int *count, float extent, float gradThreshold, float percentDense,
+                        float pruneOpacity, int opacityWasReset, uint8_t *keep, uint32_t *demand);
+// Greedy budget (Trainer.cpp:130-158): given the exclusive scan of demand, marks the accepted
+// candidates in kind and clears keep for accepted splits; writes {clones, splits, kept,
+// extras} to counts (device, 4 x u32). firstReject is 1-element device scratch.
+int launchDensifyAccept(hipStream_t s, int64_t n, const uint32_t *demand, const uint32_t *scan,
+                        uint64_t budget, uint8_t *keep, uint8_t *kind, uint32_t *firstReject,
+                        uint32_t *counts);
+// Builds keepIdx (exclusive scan input) etc.: writes per-Gaussian u32 keep flags and extra
+// counts for the scatter.
+int launchDensifyFlags(hipStream_t s, int64_t n, const uint8_t *keep, const uint8_t *kind,
+                       uint32_t *keepFlag, uint32_t *extraCount, uint32_t *splitFlag);
+// Scatter into dst (fresh arrays): kept entries at keepPos, extras at nKeep + extraPos.
+int launchDensifyScatter(hipStream_t s, const u3d_gaussians &src, u3d_gaussians &dst,
+                         int64_t n, const uint8_t *keep, const uint8_t *kind,
+                         const uint32_t *keepPos, const uint32_t *extraPos,
+                         const uint32_t *splitPos, const uint32_t *counts, const float *normals,
+                         float logShrink);
+// Moment compaction: kept entries to keepPos, appended zeros for the extras.
+int launchCompactZeroTail(hipStream_t s, const u3d_gaussians &src, u3d_gaussians &dst,
+                          int64_t n, const uint8_t *keep, const uint32_t *keepPos,
+                          const uint32_t *counts);
+
+} // namespace u3d

@@ -1,0 +1,359 @@
+// Fused photometric loss (L1 + lambda * D-SSIM) value and gradient, Loss.cpp:47-185.
+//
+// The reference runs four full-image passes over 11 planar scratch maps. Here one workgroup
+// owns a 32x16 output block and keeps the whole dependency cone in LDS: render/target with a
+// 10-pixel halo, the five horizontally blurred moment maps, the three SSIM partial-derivative
+// maps with a 5-pixel halo, and their horizontal blur; the vertical blur then yields the
+// gradient. Only render, target and grad touch HBM (36 B per pixel). Each blur sums its taps in
+// the reference's order over the reference's truncated (zero-padded) window, with contraction
+// off, so the gradient is bit-identical; the L1 / SSIM / squared-error sums are per-block
+// doubles reduced in a fixed order.
+#include "kernels.h"
+
+namespace u3d {
+
+namespace {
+
+constexpr int kR = 5;          // 11-tap window radius
+constexpr int kTW = 32, kTH = 16;
+constexpr int kIW = kTW + 4 * kR, kIH = kTH + 4 * kR; // inputs: 10-pixel halo
+constexpr int kHW = kTW + 2 * kR, kHH = kIH;          // horizontal moment blur
+constexpr int kAW = kTW + 2 * kR, kAH = kTH + 2 * kR; // SSIM partials
+constexpr int kGW = kTW, kGH = kAH;                   // horizontal blur of partials
+constexpr int kThreads = 256;
+constexpr float kC1 = 0.01f * 0.01f;
+constexpr float kC2 = 0.03f * 0.03f;
+
+struct Window {
+    float w[2 * kR + 1];
+};
+
+__device__ __forceinline__ double blockSumD(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0)
+        red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int w = 0; w < kThreads / 64; ++w)
+            s += red[w];
+    __syncthreads();
+    return s;
+}
+
+template <bool kU8>
+__global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__ render,
+                                                       const float *__restrict__ targetF,
+                                                       const uint8_t *__restrict__ targetU8,
+                                                       int W, int H, float lambda, float invCount,
+                                                       Window win, int blocksX,
+                                                       float *__restrict__ grad,
+                                                       double *__restrict__ partials)
+{
+    // The inputs are dead once the moments are blurred, so the last stage reuses their space
+    // (58 KB of LDS in total).
+    static_assert(3 * kGH * kGW <= 2 * kIH * kIW, "LDS alias");
+    __shared__ float sInG[2 * kIH * kIW];
+    __shared__ float sH[5][kHH][kHW];
+    __shared__ float sA[3][kAH][kAW];
+    auto sIn = reinterpret_cast<float(*)[kIH][kIW]>(sInG);
+    auto sG = reinterpret_cast<float(*)[kGH][kGW]>(sInG);
+    __shared__ double red[kThreads / 64];
+    __shared__ float sw[2 * kR + 1];
+    const int tid = threadIdx.x;
+    if (tid < 2 * kR + 1)
+        sw[tid] = win.w[tid];
+    __syncthreads();
+    const int ox = (blockIdx.x % blocksX) * kTW, oy = (blockIdx.x / blocksX) * kTH;
+    const bool ssimOn = lambda > 0.0f;
+    const float dLds = -lambda * invCount;
+    const float l1Scale = (1.0f - lambda) * invCount;
+    double l1 = 0.0, ssim = 0.0, sq = 0.0;
+    auto tgt = [&](size_t idx) -> float {
+        if (kU8)
+            return float(targetU8[idx]) * (1.0f / 255.0f);
+        return targetF[idx];
+    };
+    for (int c = 0; c < 3; ++c) {
+        if (ssimOn) {
+            // 1. inputs with a 10-pixel halo
+            for (int k = tid; k < kIH * kIW; k += kThreads) {
+                const int ly = k / kIW, lx = k % kIW;
+                const int gx = ox - 2 * kR + lx, gy = oy - 2 * kR + ly;
+                float a = 0.0f, b = 0.0f;
+                if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                    const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
+                    a = render[idx];
+                    b = tgt(idx);
+                }
+                sIn[0][ly][lx] = a;
+                sIn[1][ly][lx] = b;
+            }
+            __syncthreads();
+            // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94)
+            for (int k = tid; k < kHH * kHW; k += kThreads) {
+                const int ly = k / kHW, lx = k % kHW;
+                const int gx = ox - kR + lx, gy = oy - 2 * kR + ly;
+                if (gx < 0 || gx >= W || gy < 0 || gy >= H)
+                    continue;
+                float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+                const int k0 = max(-kR, -gx), k1 = min(kR, W - 1 - gx);
+                for (int t = k0; t <= k1; ++t) {
+                    const float a = sIn[0][ly][lx + kR + t], b = sIn[1][ly][lx + kR + t];
+                    const float w = sw[t + kR];
+                    sx += w * a;
+                    sy += w * b;
+                    sxx += w * a * a;
+                    syy += w * b * b;
+                    sxy += w * a * b;
+                }
+                sH[0][ly][lx] = sx;
+                sH[1][ly][lx] = sy;
+                sH[2][ly][lx] = sxx;
+                sH[3][ly][lx] = syy;
+                sH[4][ly][lx] = sxy;
+            }
+            __syncthreads();
+            // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139)
+            for (int k = tid; k < kAH * kAW; k += kThreads) {
+                const int ly = k / kAW, lx = k % kAW;
+                const int gx = ox - kR + lx, gy = oy - kR + ly;
+                if (gx < 0 || gx >= W || gy < 0 || gy >= H)
+                    continue;
+                float m[5] = {0, 0, 0, 0, 0};
+                const int k0 = max(-kR, -gy), k1 = min(kR, H - 1 - gy);
+                for (int t = k0; t <= k1; ++t) {
+                    const float w = sw[t + kR];
+#pragma unroll
+                    for (int j = 0; j < 5; ++j)
+                        m[j] += w * sH[j][ly + kR + t][lx];
+                }
+                const float mx = m[0], my = m[1];
+                const float sx = m[2] - mx * mx, sy = m[3] - my * my, sxy = m[4] - mx * my;
+                const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
+                const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
+                const float v = (a1 * a2) / (b1 * b2);
+                if (lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
+                    ssim += double(v);
+                const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - v * (2 * mx / b1 - 2 * mx / b2);
+                sA[0][ly][lx] = dLds * dmu;
+                sA[1][ly][lx] = dLds * (-v / b2);
+                sA[2][ly][lx] = dLds * (2 * a1 / (b1 * b2));
+            }
+            __syncthreads();
+            // 4. horizontal blur of the partials (Loss.cpp:142-157)
+            for (int k = tid; k < kGH * kGW; k += kThreads) {
+                const int ly = k / kGW, lx = k % kGW;
+                const int gx = ox + lx, gy = oy - kR + ly;
+                if (gx >= W || gy < 0 || gy >= H)
+                    continue;
+                float acc[3] = {0, 0, 0};
+                const int k0 = max(-kR, -gx), k1 = min(kR, W - 1 - gx);
+                for (int t = k0; t <= k1; ++t) {
+                    const float w = sw[t + kR];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[j] += w * sA[j][ly][lx + kR + t];
+                }
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    sG[j][ly][lx] = acc[j];
+            }
+            __syncthreads();
+        }
+        // 5. L1 term, vertical blur of the partials and the chain rule (Loss.cpp:64-74, 160-177)
+        for (int k = tid; k < kTH * kTW; k += kThreads) {
+            const int ly = k / kTW, lx = k % kTW;
+            const int gx = ox + lx, gy = oy + ly;
+            if (gx >= W || gy >= H)
+                continue;
+            const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
+            const float r = render[idx], t = tgt(idx);
+            const float d = r - t;
+            l1 += double(fabsf(d));
+            const double dd = double(r) - double(t);
+            sq += dd * dd;
+            float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
+            if (ssimOn) {
+                float g[3] = {0, 0, 0};
+                const int k0 = max(-kR, -gy), k1 = min(kR, H - 1 - gy);
+                for (int tt = k0; tt <= k1; ++tt) {
+                    const float w = sw[tt + kR];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        g[j] += w * sG[j][ly + kR + tt][lx];
+                }
+                gval += g[0] + 2.0f * r * g[1] + t * g[2];
+            }
+            grad[idx] = gval;
+        }
+        __syncthreads();
+    }
+    const double s1 = blockSumD(l1, red);
+    const double s2 = blockSumD(ssim, red);
+    const double s3 = blockSumD(sq, red);
+    if (tid == 0) {
+        partials[size_t(blockIdx.x) * 3] = s1;
+        partials[size_t(blockIdx.x) * 3 + 1] = s2;
+        partials[size_t(blockIdx.x) * 3 + 2] = s3;
+    }
+}
+
+// Reduces per-block partials in a fixed order; result = {total, l1, ssim, psnr}.
+__global__ __launch_bounds__(kThreads) void lossFinalize(const double *__restrict__ partials,
+                                                         int numBlocks, float lambda,
+                                                         float invCount, double count,
+                                                         float *result)
+{
+    __shared__ double red[kThreads / 64];
+    double a = 0, b = 0, c = 0;
+    for (int i = threadIdx.x; i < numBlocks; i += kThreads) {
+        a += partials[size_t(i) * 3];
+        b += partials[size_t(i) * 3 + 1];
+        c += partials[size_t(i) * 3 + 2];
+    }
+    a = blockSumD(a, red);
+    b = blockSumD(b, red);
+    c = blockSumD(c, red);
+    if (threadIdx.x == 0) {
+        const float l1 = float(a * double(invCount));
+        float ssim = 0.0f, total = l1;
+        if (lambda > 0.0f) {
+            ssim = float(b * double(invCount));
+            total = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
+        }
+        const double mse = c / (count > 1.0 ? count : 1.0);
+        result[0] = total;
+        result[1] = l1;
+        result[2] = ssim;
+        result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
+    }
+}
+
+// Squared error / absolute error sums for evaluation passes.
+template <int kMode> // 0: squared error, 1: absolute error
+__global__ __launch_bounds__(kThreads) void errSumKernel(const float *__restrict__ a,
+                                                         const float *__restrict__ b,
+                                                         const uint8_t *__restrict__ bU8,
+                                                         int64_t count, double *partials)
+{
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * kThreads) {
+        const double x = double(a[i]);
+        const double y = bU8 ? double(float(bU8[i]) * (1.0f / 255.0f)) : double(b[i]);
+        const double d = x - y;
+        s += kMode == 0 ? d * d : fabs(d);
+    }
+    s = blockSumD(s, red);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void errFinalize(const double *__restrict__ partials,
+                                                        int numBlocks, double count, float *result)
+{
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < numBlocks; i += kThreads)
+        s += partials[i];
+    s = blockSumD(s, red);
+    if (threadIdx.x == 0) {
+        const double m = s / (count > 1.0 ? count : 1.0);
+        if (kMode == 0)
+            result[0] = m <= 1e-12 ? 99.0f : float(-10.0 * log10(m));
+        else
+            result[0] = float(m);
+    }
+}
+
+Window makeWindow()
+{
+    // Loss.cpp:15-27: w_k = exp(-k^2 / (2 * 1.5^2)) normalised, in float with the host expf.
+    static const uint64_t table[32] = U3D_EXP2_TABLE;
+    Window w;
+    float sum = 0.0f;
+    for (int k = -kR; k <= kR; ++k) {
+        const float x = -float(k * k) / (2.0f * 1.5f * 1.5f);
+        w.w[k + kR] = u3dmath::expf_core(x, table);
+        sum += w.w[k + kR];
+    }
+    for (float &v : w.w)
+        v /= sum;
+    return w;
+}
+
+constexpr int kErrBlocks = 1024;
+
+} // namespace
+
+size_t lossPartialsSize(int width, int height)
+{
+    const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
+    return size_t(bx) * size_t(by) * 3;
+}
+
+int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
+                          const uint8_t *targetU8, int width, int height, float lambda,
+                          float *grad, double *partials, size_t partialsCap, float *result)
+{
+    const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
+    const int nb = bx * by;
+    U3D_CHECK(size_t(nb) * 3 <= partialsCap, U3D_ERR_CAPACITY, "loss partials too small");
+    const size_t P = size_t(width) * size_t(height);
+    const float invCount = 1.0f / float(P * 3);
+    static const Window win = makeWindow();
+    if (nb > 0) {
+        if (targetU8)
+            hipLaunchKernelGGL(lossKernel<true>, dim3(nb), dim3(kThreads), 0, s, render, target,
+                               targetU8, width, height, lambda, invCount, win, bx, grad, partials);
+        else
+            hipLaunchKernelGGL(lossKernel<false>, dim3(nb), dim3(kThreads), 0, s, render, target,
+                               targetU8, width, height, lambda, invCount, win, bx, grad, partials);
+        U3D_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, partials, nb, lambda,
+                       invCount, double(P * 3), result);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+size_t psnrPartialsSize(int64_t) { return kErrBlocks; }
+
+static int launchErr(hipStream_t s, int mode, const float *a, const float *b, const uint8_t *bU8,
+                     int64_t count, double *partials, size_t cap, float *result)
+{
+    U3D_CHECK(cap >= size_t(kErrBlocks), U3D_ERR_CAPACITY, "error partials too small");
+    int nb = int((count + kThreads - 1) / kThreads);
+    nb = nb < 1 ? 1 : (nb > kErrBlocks ? kErrBlocks : nb);
+    if (mode == 0) {
+        hipLaunchKernelGGL(errSumKernel<0>, dim3(nb), dim3(kThreads), 0, s, a, b, bU8, count, partials);
+        U3D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(errFinalize<0>, dim3(1), dim3(kThreads), 0, s, partials, nb, double(count), result);
+    } else {
+        hipLaunchKernelGGL(errSumKernel<1>, dim3(nb), dim3(kThreads), 0, s, a, b, bU8, count, partials);
+        U3D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(errFinalize<1>, dim3(1), dim3(kThreads), 0, s, partials, nb, double(count), result);
+    }
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchPsnr(hipStream_t s, const float *a, const float *b, const uint8_t *bU8, int64_t count,
+               double *partials, size_t partialsCap, float *result)
+{
+    return launchErr(s, 0, a, b, bU8, count, partials, partialsCap, result);
+}
+
+int launchMeanAbs(hipStream_t s, const float *a, const uint8_t *bU8, int64_t count,
+                  double *partials, size_t partialsCap, float *result)
+{
+    return launchErr(s, 1, a, nullptr, bU8, count, partials, partialsCap, result);
+}
+
+} // namespace u3d

@@ -1,0 +1,262 @@
+// Projection (Rasterizer.cpp:137-204) and backward chain rule (Rasterizer.cpp:432-544): one
+// thread per Gaussian over structure-of-arrays inputs. Both are HBM-bound streaming passes.
+// Compiled with -ffp-contract=off: projection outputs are bit-identical to the reference.
+#include "kernels.h"
+#include "projection.h"
+
+namespace u3d {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void projectKernel(
+    int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
+    const float *__restrict__ quats, const float *__restrict__ opac,
+    const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
+    int tilesX, int tilesY, ScreenBuffers out)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    int radius = 0;
+    uint8_t flags = 0;
+    uint32_t key = 0xffffffffu, nt = 0;
+    do {
+        const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
+        const float qr[4] = {quats[i * 4], quats[i * 4 + 1], quats[i * 4 + 2], quats[i * 4 + 3]};
+        const float ls[3] = {scales[i * 3], scales[i * 3 + 1], scales[i * 3 + 2]};
+        Projection p;
+        if (!projectGaussian(mu, qr, ls, cam, nearPlane, eps2d, p))
+            break;
+        const float det = p.a * p.c - p.b * p.b;
+        if (!(det > 0.0f))
+            break;
+        const float invDet = 1.0f / det;
+        const float z = p.pc[2];
+        const float u = cam.fx * p.pc[0] / z + cam.cx;
+        const float v = cam.fy * p.pc[1] / z + cam.cy;
+        const float mid = 0.5f * (p.a + p.c);
+        const float disc = mid * mid - det;
+        const float lambda = mid + sqrtf((0.1f < disc) ? disc : 0.1f); // std::max(0.1f, disc)
+        const int r = cvtTruncI32(ceilf(3.0f * sqrtf(lambda)));
+        const float rf = float(r);
+        if (u + rf < 0.0f || v + rf < 0.0f || u - rf >= float(cam.width) || v - rf >= float(cam.height))
+            break;
+        const float tsf = float(ts);
+        int tx0 = cvtTruncI32(floorf((u - rf) / tsf));
+        int ty0 = cvtTruncI32(floorf((v - rf) / tsf));
+        int tx1 = cvtTruncI32(floorf((u + rf) / tsf)) + 1;
+        int ty1 = cvtTruncI32(floorf((v + rf) / tsf)) + 1;
+        tx0 = tx0 > 0 ? tx0 : 0;
+        ty0 = ty0 > 0 ? ty0 : 0;
+        tx1 = tx1 < tilesX ? tx1 : tilesX;
+        ty1 = ty1 < tilesY ? ty1 : tilesY;
+        if (tx0 >= tx1 || ty0 >= ty1)
+            break;
+        flags = uint8_t((p.clampX ? 1 : 0) | (p.clampY ? 2 : 0));
+        float rgb[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float col = 0.5f + kShC0 * colors[i * 3 + ch];
+            if (col < 0.0f)
+                flags |= uint8_t(4 << ch);
+            rgb[ch] = (col < 0.0f) ? 0.0f : col; // std::max(col, 0.0f)
+        }
+        const float alpha = 1.0f / (1.0f + expfExact(-opac[i]));
+        out.rec0[i] = make_float4(u, v, p.c * invDet, -p.b * invDet);
+        out.rec1[i] = make_float4(p.a * invDet, alpha, rgb[0], rgb[1]);
+        out.rec2[i] = rgb[2];
+        out.depth[i] = z;
+        out.rect[i] = make_int4(tx0, ty0, tx1, ty1);
+        radius = r;
+        key = floatBits(z);
+        nt = uint32_t(tx1 - tx0) * uint32_t(ty1 - ty0);
+    } while (false);
+    out.radius[i] = radius;
+    out.flags[i] = flags;
+    out.depthKey[i] = key;
+    out.nTiles[i] = nt;
+}
+
+__global__ __launch_bounds__(kThreads) void chainRuleKernel(
+    int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
+    const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
+    const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
+    const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
+    float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n || radius[i] == 0)
+        return;
+    if (gradCount && radius[i] > 0)
+        gradCount[i] += 1;
+    float g[kGrad2dStride];
+#pragma unroll
+    for (int k = 0; k < kGrad2dStride; ++k)
+        g[k] = grad2d[i * kGrad2dStride + k];
+    const float dmx = g[0], dmy = g[1], dA = g[2], dB = g[3], dC = g[4];
+    if (ndc) {
+        const float nx = dmx * 0.5f * float(cam.width), ny = dmy * 0.5f * float(cam.height);
+        ndc[i] += sqrtf(nx * nx + ny * ny);
+    }
+    const uint8_t flags = flagsArr[i];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        if (!(flags & (4 << ch)))
+            gColors[i * 3 + ch] += kShC0 * g[5 + ch];
+    const float op = rec1[i].y;
+    gOpac[i] += g[8] * op * (1.0f - op);
+
+    const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
+    const float qr[4] = {quats[i * 4], quats[i * 4 + 1], quats[i * 4 + 2], quats[i * 4 + 3]};
+    const float ls[3] = {scales[i * 3], scales[i * 3 + 1], scales[i * 3 + 2]};
+    Projection p;
+    projectGaussian(mu, qr, ls, cam, nearPlane, eps2d, p);
+    const float z = p.pc[2];
+    const float a = p.a, b = p.b, c = p.c;
+    const float det = a * c - b * b;
+    const float det2 = det * det;
+    // Inverse-covariance gradient -> 2D covariance gradient.
+    const float da = (dA * -c * c + dB * b * c + dC * -b * b) / det2;
+    const float db = (dA * 2 * b * c + dC * 2 * a * b) / det2 + dB * (-1.0f / det - 2 * b * b / det2);
+    const float dc = (dA * -b * b + dB * a * b + dC * -a * a) / det2;
+    const float G00 = da, G01 = 0.5f * db, G11 = dc;
+    float dSigma[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+            dSigma[k * 3 + l] = p.T0[k] * (G00 * p.T0[l] + G01 * p.T1[l])
+                                + p.T1[k] * (G01 * p.T0[l] + G11 * p.T1[l]);
+    float TS0[3], TS1[3];
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        TS0[col] = p.T0[0] * p.Sigma[col] + p.T0[1] * p.Sigma[3 + col] + p.T0[2] * p.Sigma[6 + col];
+        TS1[col] = p.T1[0] * p.Sigma[col] + p.T1[1] * p.Sigma[3 + col] + p.T1[2] * p.Sigma[6 + col];
+    }
+    float dT0[3], dT1[3];
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        dT0[col] = 2.0f * (G00 * TS0[col] + G01 * TS1[col]);
+        dT1[col] = 2.0f * (G01 * TS0[col] + G11 * TS1[col]);
+    }
+    float dM[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int col = 0; col < 3; ++col)
+            dM[r * 3 + col] = 2.0f * (dSigma[r * 3] * p.M[col] + dSigma[r * 3 + 1] * p.M[3 + col]
+                                      + dSigma[r * 3 + 2] * p.M[6 + col]);
+    float dRq[9];
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        float ds = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            ds += dM[r * 3 + col] * p.Rq[r * 3 + col];
+            dRq[r * 3 + col] = dM[r * 3 + col] * p.s[col];
+        }
+        gScales[i * 3 + col] += ds * p.s[col];
+    }
+    const float w = p.q[0], x = p.q[1], y = p.q[2], zq = p.q[3];
+    const float dw = 2 * (-zq * dRq[1] + y * dRq[2] + zq * dRq[3] - x * dRq[5] - y * dRq[6] + x * dRq[7]);
+    const float dx = 2 * (y * dRq[1] + zq * dRq[2] + y * dRq[3] - 2 * x * dRq[4] - w * dRq[5]
+                          + zq * dRq[6] + w * dRq[7] - 2 * x * dRq[8]);
+    const float dy = 2 * (-2 * y * dRq[0] + x * dRq[1] + w * dRq[2] + x * dRq[3] + zq * dRq[5]
+                          - w * dRq[6] + zq * dRq[7] - 2 * y * dRq[8]);
+    const float dz = 2 * (-2 * zq * dRq[0] - w * dRq[1] + x * dRq[2] + w * dRq[3] - 2 * zq * dRq[4]
+                          + y * dRq[5] + x * dRq[6] + y * dRq[7]);
+    const float dq[4] = {dw, dx, dy, dz};
+    const float qdot = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        gQuats[i * 4 + k] += (dq[k] - qdot * p.q[k]) / p.qNorm;
+    const float *R = cam.R;
+    const float dJ00 = dT0[0] * R[0] + dT0[1] * R[1] + dT0[2] * R[2];
+    const float dJ02 = dT0[0] * R[6] + dT0[1] * R[7] + dT0[2] * R[8];
+    const float dJ11 = dT1[0] * R[3] + dT1[1] * R[4] + dT1[2] * R[5];
+    const float dJ12 = dT1[0] * R[6] + dT1[1] * R[7] + dT1[2] * R[8];
+    const float z2 = z * z, z3 = z2 * z;
+    float dpc[3] = {0.0f, 0.0f, 0.0f};
+    dpc[2] += dJ00 * -cam.fx / z2 + dJ02 * 2 * cam.fx * p.tx / z3 + dJ11 * -cam.fy / z2
+              + dJ12 * 2 * cam.fy * p.ty / z3;
+    const float dtx = dJ02 * -cam.fx / z2;
+    const float dty = dJ12 * -cam.fy / z2;
+    if (p.clampX)
+        dpc[2] += dtx * p.limTx;
+    else
+        dpc[0] += dtx;
+    if (p.clampY)
+        dpc[2] += dty * p.limTy;
+    else
+        dpc[1] += dty;
+    dpc[0] += dmx * cam.fx / z;
+    dpc[1] += dmy * cam.fy / z;
+    dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        gMeans[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
+}
+
+__global__ __launch_bounds__(kThreads) void unpackKernel(int64_t n, ScreenBuffers sb,
+                                                         float *mean2d, float *conic,
+                                                         float *color, float *opacity)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n || sb.radius[i] <= 0)
+        return;
+    const float4 a = sb.rec0[i], b = sb.rec1[i];
+    mean2d[i * 2] = a.x;
+    mean2d[i * 2 + 1] = a.y;
+    conic[i * 3] = a.z;
+    conic[i * 3 + 1] = a.w;
+    conic[i * 3 + 2] = b.x;
+    opacity[i] = b.y;
+    color[i * 3] = b.z;
+    color[i * 3 + 1] = b.w;
+    color[i * 3 + 2] = sb.rec2[i];
+}
+
+inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+} // namespace
+
+int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
+                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out)
+{
+    if (m.n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(projectKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
+                       m.scales, m.quats, m.opacities, m.colors, cam, st.near_plane, st.eps2d,
+                       st.tile_size, tilesX, tilesY, out);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
+                    u3d_gaussians &grads, float *ndc, int *gradCount)
+{
+    if (m.n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(chainRuleKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
+                       m.means, m.scales, m.quats, cam, st.near_plane, st.eps2d, sb.radius,
+                       sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
+                       grads.opacities, grads.colors, ndc, gradCount);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float *mean2d,
+                       float *conic, float *color, float *opacity)
+{
+    if (n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(unpackKernel, dim3(blocksFor(n)), dim3(kThreads), 0, s, n, sb, mean2d,
+                       conic, color, opacity);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+} // namespace u3d

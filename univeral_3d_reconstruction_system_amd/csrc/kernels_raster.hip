@@ -1,0 +1,307 @@
+// Tile compositing, forward (Rasterizer.cpp:263-330) and backward stage 1 (Rasterizer.cpp:
+// 334-427), for gfx950.
+//
+// One 256-thread workgroup (4 wave64) per 16x16 pixel block, one pixel per thread; a logical
+// tile of tile_size (a multiple of 16) is covered by (tile_size/16)^2 blocks that share its list.
+// The tile's Gaussians are staged through LDS in batches of 256 records (36 B each, gathered
+// once per batch by one thread each) and every thread walks the batch for its pixel.
+//
+// Forward: front-to-back with the reference's exact arithmetic (contraction off, host-identical
+// expf), so images, final transmittance and contributor counts are bit-identical. Most pairs are
+// rejected before any exp: power > 0, or power below log(min_alpha / alpha) minus a 1e-3 margin
+// (then alpha * exp(power) < min_alpha is certain); only the rest evaluate the exact exp.
+// The workgroup stops early once every pixel has saturated (__syncthreads_and).
+//
+// Backward: reverse walk from the block's largest contributor count; per Gaussian the nine 2D
+// gradient terms of the wave's 64 pixels are summed with DPP row shifts and broadcasts (skipped
+// when no lane contributes), the four waves combine in LDS, and the batch is flushed with one
+// global float atomic per non-zero term per (block, Gaussian).
+#include "kernels.h"
+
+namespace u3d {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBatch = 256;
+constexpr int kEdge = 16;
+
+template <int CTRL, int ROW_MASK> __device__ __forceinline__ float dppMove(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum over the 64 lanes of the wave (every lane must be active); result in all lanes.
+__device__ __forceinline__ float waveSum(float x)
+{
+    x += dppMove<0x111, 0xf>(x); // row_shr:1
+    x += dppMove<0x112, 0xf>(x); // row_shr:2
+    x += dppMove<0x114, 0xf>(x); // row_shr:4
+    x += dppMove<0x118, 0xf>(x); // row_shr:8
+    x += dppMove<0x142, 0xa>(x); // row_bcast:15 into rows 1, 3
+    x += dppMove<0x143, 0xc>(x); // row_bcast:31 into rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// log(min_alpha / alpha) - 1e-3: below it, min(0.99, alpha * exp(power)) < min_alpha holds for
+// the rounded float result as well (logf error and the two roundings are < 1e-5).
+__device__ __forceinline__ float skipThreshold(float minAlpha, float alpha)
+{
+    return minAlpha > 0.0f ? logf(minAlpha / alpha) - 1e-3f : -INFINITY;
+}
+
+__global__ __launch_bounds__(kThreads) void rasterForwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib)
+{
+    __shared__ float4 s0[kBatch];
+    __shared__ float4 s1[kBatch];
+    __shared__ float s2[kBatch];
+    __shared__ float sthr[kBatch];
+    __shared__ uint64_t stab[32];
+    const int tid = threadIdx.x;
+    if (tid < 32)
+        stab[tid] = kExp2Table[tid];
+    const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
+    const int px = bx * kEdge + (tid & 15), py = by * kEdge + (tid >> 4);
+    const int tile = ((by * kEdge) / ts) * tilesX + (bx * kEdge) / ts;
+    const bool inside = px < W && py < H;
+    const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
+    const float fpx = float(px), fpy = float(py);
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int last = 0;
+    bool done = !inside;
+    for (int base = begin; base < end; base += kBatch) {
+        if (__syncthreads_and(done))
+            break;
+        const int idx = base + tid;
+        if (idx < end) {
+            const int g = tileList[idx];
+            const float4 b = rec1[g];
+            s0[tid] = rec0[g];
+            s1[tid] = b;
+            s2[tid] = rec2[g];
+            sthr[tid] = skipThreshold(minAlpha, b.y);
+        }
+        __syncthreads();
+        if (!done) {
+            const int cnt = min(kBatch, end - base);
+            for (int j = 0; j < cnt; ++j) {
+                const float4 a = s0[j];
+                const float dx = a.x - fpx, dy = a.y - fpy;
+                const float4 b = s1[j];
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                if (power > 0.0f)
+                    continue;
+                if (power < sthr[j])
+                    continue;
+                const float raw = b.y * expfExactLds(power, stab);
+                const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
+                if (alpha < minAlpha)
+                    continue;
+                const float nextT = T * (1.0f - alpha);
+                if (nextT < minT) {
+                    done = true;
+                    break;
+                }
+                const float w = alpha * T;
+                cr += b.z * w;
+                cg += b.w * w;
+                cb += s2[j] * w;
+                T = nextT;
+                last = base - begin + j + 1;
+            }
+        }
+    }
+    if (inside) {
+        const size_t pix = size_t(py) * size_t(W) + size_t(px);
+        rgb[pix * 3] = cr + T * bg0;
+        rgb[pix * 3 + 1] = cg + T * bg1;
+        rgb[pix * 3 + 2] = cb + T * bg2;
+        finalT[pix] = T;
+        nContrib[pix] = last;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    const int *__restrict__ nContribArr, float *__restrict__ grad2d)
+{
+    __shared__ float4 s0[kBatch];
+    __shared__ float4 s1[kBatch];
+    __shared__ float s2[kBatch];
+    __shared__ float sthr[kBatch];
+    __shared__ int sid[kBatch];
+    __shared__ float acc[kBatch][kGrad2dStride];
+    __shared__ uint64_t stab[32];
+    __shared__ int smax;
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < 32)
+        stab[tid] = kExp2Table[tid];
+    if (tid == 0)
+        smax = 0;
+    for (int k = tid; k < kBatch * kGrad2dStride; k += kThreads)
+        (&acc[0][0])[k] = 0.0f;
+    const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
+    const int px = bx * kEdge + (tid & 15), py = by * kEdge + (tid >> 4);
+    const int tile = ((by * kEdge) / ts) * tilesX + (bx * kEdge) / ts;
+    const bool inside = px < W && py < H;
+    const int begin = tileOffsets[tile];
+    const size_t pix = inside ? size_t(py) * size_t(W) + size_t(px) : 0;
+    const float fT = inside ? finalTArr[pix] : 0.0f;
+    const int nc = inside ? nContribArr[pix] : 0;
+    const float dp0 = inside ? dl[pix * 3] : 0.0f;
+    const float dp1 = inside ? dl[pix * 3 + 1] : 0.0f;
+    const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
+    const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
+    const float fpx = float(px), fpy = float(py);
+    float T = fT;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+    float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
+    __syncthreads();
+    if (nc > 0)
+        atomicMax(&smax, nc);
+    __syncthreads();
+    const int maxN = smax;
+    for (int hi = maxN; hi > 0; hi -= kBatch) {
+        const int lo = hi > kBatch ? hi - kBatch : 0;
+        const int cnt = hi - lo;
+        if (tid < cnt) {
+            const int g = tileList[begin + lo + tid];
+            const float4 b = rec1[g];
+            sid[tid] = g;
+            s0[tid] = rec0[g];
+            s1[tid] = b;
+            s2[tid] = rec2[g];
+            sthr[tid] = skipThreshold(minAlpha, b.y);
+        }
+        __syncthreads();
+        for (int jj = cnt - 1; jj >= 0; --jj) {
+            const int j = lo + jj;
+            float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
+            bool contrib = false;
+            if (j < nc) {
+                const float4 a = s0[jj];
+                const float4 b = s1[jj];
+                const float dx = a.x - fpx, dy = a.y - fpy;
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                if (!(power > 0.0f) && !(power < sthr[jj])) {
+                    const float G = expfExactLds(power, stab);
+                    const float op = b.y;
+                    const float rawAlpha = op * G;
+                    const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
+                    if (!(alpha < minAlpha)) {
+                        contrib = true;
+                        const float c0 = b.z, c1 = b.w, c2 = s2[jj];
+                        T = T / (1.0f - alpha);
+                        const float w = alpha * T;
+                        float dAlpha = 0.0f;
+                        acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
+                        last0 = c0;
+                        dAlpha += (c0 - acc0) * dp0;
+                        v5 = w * dp0;
+                        acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
+                        last1 = c1;
+                        dAlpha += (c1 - acc1) * dp1;
+                        v6 = w * dp1;
+                        acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
+                        last2 = c2;
+                        dAlpha += (c2 - acc2) * dp2;
+                        v7 = w * dp2;
+                        dAlpha *= T;
+                        lastAlpha = alpha;
+                        dAlpha += -fT / (1.0f - alpha) * bgDot;
+                        if (!(rawAlpha > kMaxAlpha)) {
+                            v8 = G * dAlpha;
+                            const float dPower = op * dAlpha * G;
+                            const float A = a.z, B = a.w, C = b.x;
+                            v0 = dPower * -(A * dx + B * dy);
+                            v1 = dPower * -(C * dy + B * dx);
+                            v2 = dPower * -0.5f * dx * dx;
+                            v3 = dPower * -dx * dy;
+                            v4 = dPower * -0.5f * dy * dy;
+                        }
+                    }
+                }
+            }
+            if (__ballot(contrib)) {
+                v0 = waveSum(v0);
+                v1 = waveSum(v1);
+                v2 = waveSum(v2);
+                v3 = waveSum(v3);
+                v4 = waveSum(v4);
+                v5 = waveSum(v5);
+                v6 = waveSum(v6);
+                v7 = waveSum(v7);
+                v8 = waveSum(v8);
+                if (lane == 0) {
+                    float *dst = acc[jj];
+                    if (v0 != 0.0f) atomicAdd(&dst[0], v0);
+                    if (v1 != 0.0f) atomicAdd(&dst[1], v1);
+                    if (v2 != 0.0f) atomicAdd(&dst[2], v2);
+                    if (v3 != 0.0f) atomicAdd(&dst[3], v3);
+                    if (v4 != 0.0f) atomicAdd(&dst[4], v4);
+                    if (v5 != 0.0f) atomicAdd(&dst[5], v5);
+                    if (v6 != 0.0f) atomicAdd(&dst[6], v6);
+                    if (v7 != 0.0f) atomicAdd(&dst[7], v7);
+                    if (v8 != 0.0f) atomicAdd(&dst[8], v8);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < cnt) {
+            float *dst = grad2d + size_t(sid[tid]) * kGrad2dStride;
+#pragma unroll
+            for (int k = 0; k < kGrad2dStride; ++k) {
+                const float x = acc[tid][k];
+                if (x != 0.0f) {
+                    unsafeAtomicAdd(&dst[k], x);
+                    acc[tid][k] = 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace
+
+int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                        const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                        float *rgb, float *finalT, int *nContrib)
+{
+    const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
+    if (blocksX * blocksY == 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(rasterForwardKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
+                       g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
+                       sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                       st.background[2], st.min_alpha, st.min_transmittance, rgb, finalT,
+                       nContrib);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                         const float *dlDrgb, const float *finalT, const int *nContrib,
+                         float *grad2d)
+{
+    const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
+    if (blocksX * blocksY == 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(rasterBackwardKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
+                       g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
+                       sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                       st.background[2], st.min_alpha, dlDrgb, finalT, nContrib, grad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+} // namespace u3d

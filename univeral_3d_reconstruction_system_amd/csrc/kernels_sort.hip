@@ -1,0 +1,399 @@
+// Tile binning on the device (replaces the serial Rasterizer::binTiles, Rasterizer.cpp:206-261).
+//
+// The reference orders every tile list by (depth bits, Gaussian index): a stable radix sort of
+// the visible Gaussians' depth bits followed by a stable per-tile scatter. Here the same order
+// comes from an LSD radix sort on the composite (tile, depth) key, with the depth digits sorted
+// before the (tile, Gaussian) pairs are duplicated, so those passes cost N elements, not K:
+//   1. stable radix sort of (depth bits, index) over all N Gaussians (invisible keys 0xffffffff
+//      sort last, ties keep index order);
+//   2. exclusive scan of the tiles each Gaussian covers, in depth order -> pair slots, K;
+//   3. emit (tile id, Gaussian) pairs in depth order;
+//   4. stable radix sort of the pairs on the tile-id bits;
+//   5. tile offsets from the sorted tile ids.
+// All integer work: HBM-bound. Each radix pass is upsweep (per-block digit histogram), a scan
+// of the digit-major histogram, and a downsweep that ranks keys stably inside each wavefront
+// with ballot-based digit matching and scatters them.
+#include "kernels.h"
+
+namespace u3d {
+
+namespace {
+
+constexpr int kRadixThreads = 256;
+constexpr int kWaves = kRadixThreads / 64;
+constexpr int kItems = 16;                                  // keys per lane per block
+constexpr int kBlockKeys = kRadixThreads * kItems;          // 4096 keys per block
+constexpr int kScanThreads = 1024;
+
+__device__ __forceinline__ uint64_t laneMaskLt()
+{
+    const unsigned lane = __lane_id();
+    return (uint64_t(1) << lane) - 1;
+}
+
+// Mask of the active lanes whose 8-bit digit equals this lane's (restricted to `valid`).
+__device__ __forceinline__ uint64_t matchDigit(uint32_t digit, uint64_t valid, int bits)
+{
+    uint64_t m = valid;
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t bal = __ballot((digit >> b) & 1u);
+        m &= ((digit >> b) & 1u) ? bal : ~bal;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kRadixThreads) void radixUpsweep(const uint32_t *__restrict__ keys,
+                                                              int64_t count, int shift, int bits,
+                                                              int numBlocks, uint32_t *hist)
+{
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int64_t base = int64_t(blockIdx.x) * kBlockKeys + int64_t(wave) * 64 * kItems;
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t idx = base + int64_t(it) * 64 + lane;
+        const bool valid = idx < count;
+        const uint32_t d = valid ? (keys[idx] >> shift) & mask : 0u;
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0)
+            break;
+        const uint64_t m = matchDigit(d, vm, bits);
+        if (valid && (m & laneMaskLt()) == 0) // lowest lane of its digit group
+            atomicAdd(&h[d], uint32_t(__popcll(m)));
+    }
+    __syncthreads();
+    hist[size_t(tid) * size_t(numBlocks) + blockIdx.x] = h[tid];
+}
+
+// Exclusive scan of `len` u32 in place with one block, 4 values per thread per chunk.
+__global__ __launch_bounds__(kScanThreads) void scanSingleBlock(uint32_t *data, int64_t len,
+                                                                 uint32_t *total)
+{
+    __shared__ uint32_t waveSums[kScanThreads / 64];
+    __shared__ uint32_t carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0)
+        carry = 0;
+    __syncthreads();
+    constexpr int kPer = 4;
+    for (int64_t start = 0; start < len; start += int64_t(kScanThreads) * kPer) {
+        uint32_t v[kPer];
+        uint32_t local = 0;
+        const int64_t myBase = start + int64_t(tid) * kPer;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = (myBase + k < len) ? data[myBase + k] : 0u;
+            local += v[k];
+        }
+        // inclusive wave scan of `local`
+        uint32_t x = local;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            waveSums[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t s = lane < kScanThreads / 64 ? waveSums[lane] : 0u;
+#pragma unroll
+            for (int off = 1; off < kScanThreads / 64; off <<= 1) {
+                const uint32_t y = __shfl_up(s, off, 64);
+                if (lane >= off)
+                    s += y;
+            }
+            if (lane < kScanThreads / 64)
+                waveSums[lane] = s; // inclusive
+        }
+        __syncthreads();
+        const uint32_t waveOff = wave > 0 ? waveSums[wave - 1] : 0u;
+        uint32_t run = carry + waveOff + (x - local);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (myBase + k < len)
+                data[myBase + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == 0)
+            carry += waveSums[kScanThreads / 64 - 1];
+        __syncthreads();
+    }
+    if (tid == 0 && total)
+        *total = carry;
+}
+
+__global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
+    const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn, uint32_t *keysOut,
+    uint32_t *valsOut, int64_t count, int shift, int bits, int numBlocks,
+    const uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t waveCount[kWaves][256];
+    __shared__ uint32_t gbase[256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+        waveCount[w][tid] = 0;
+    gbase[tid] = hist[size_t(tid) * size_t(numBlocks) + blockIdx.x];
+    __syncthreads();
+    const uint32_t mask = (1u << bits) - 1u;
+    const int64_t base = int64_t(blockIdx.x) * kBlockKeys + int64_t(wave) * 64 * kItems;
+    uint32_t k[kItems], v[kItems], rank[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t idx = base + int64_t(it) * 64 + lane;
+        const bool valid = idx < count;
+        k[it] = valid ? keysIn[idx] : 0u;
+        v[it] = valid ? valsIn[idx] : 0u;
+        const uint32_t d = (k[it] >> shift) & mask;
+        const uint64_t vm = __ballot(valid);
+        rank[it] = 0;
+        if (vm == 0)
+            continue;
+        const uint64_t m = matchDigit(d, vm, bits);
+        if (valid) {
+            const uint32_t prev = waveCount[wave][d];
+            rank[it] = prev + uint32_t(__popcll(m & laneMaskLt()));
+            if ((m & laneMaskLt()) == 0)
+                waveCount[wave][d] = prev + uint32_t(__popcll(m));
+        }
+    }
+    __syncthreads();
+    {
+        // exclusive prefix over waves, per digit
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = waveCount[w][tid];
+            waveCount[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t idx = base + int64_t(it) * 64 + lane;
+        if (idx < count) {
+            const uint32_t d = (k[it] >> shift) & mask;
+            const uint32_t pos = gbase[d] + waveCount[wave][d] + rank[it];
+            keysOut[pos] = k[it];
+            valsOut[pos] = v[it];
+        }
+    }
+}
+
+__global__ void iotaKernel(uint32_t *vals, int64_t count)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count)
+        vals[i] = uint32_t(i);
+}
+
+__global__ void gatherKernel(const uint32_t *__restrict__ in, const uint32_t *__restrict__ idx,
+                             uint32_t *out, int64_t count)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = in[idx[i]];
+}
+
+// Block-level partial sums for the multi-block scan.
+constexpr int kScanBlockItems = 4096; // 256 threads x 16
+
+__global__ __launch_bounds__(256) void scanReduce(const uint32_t *__restrict__ in, int64_t count,
+                                                  uint32_t *blockSums)
+{
+    __shared__ uint32_t ws[4];
+    const int tid = threadIdx.x;
+    const int64_t base = int64_t(blockIdx.x) * kScanBlockItems;
+    uint32_t s = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = base + int64_t(k) * 256 + tid;
+        if (i < count)
+            s += in[i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, 64);
+    if ((tid & 63) == 0)
+        ws[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0)
+        blockSums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(256) void scanDownsweep(const uint32_t *__restrict__ in,
+                                                     uint32_t *out, int64_t count,
+                                                     const uint32_t *__restrict__ blockOffsets)
+{
+    __shared__ uint32_t ws[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // each thread owns 16 consecutive items
+    const int64_t base = int64_t(blockIdx.x) * kScanBlockItems + int64_t(tid) * 16;
+    uint32_t v[16];
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        v[k] = (base + k < count) ? in[base + k] : 0u;
+        local += v[k];
+    }
+    uint32_t x = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        ws[wave] = x;
+    __syncthreads();
+    uint32_t waveOff = 0;
+    for (int w = 0; w < wave; ++w)
+        waveOff += ws[w];
+    uint32_t run = blockOffsets[blockIdx.x] + waveOff + (x - local);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (base + k < count)
+            out[base + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ void emitPairsKernel(int64_t n, const uint32_t *__restrict__ sorted,
+                                const uint32_t *__restrict__ offsets,
+                                const int4 *__restrict__ rect, int tilesX, uint32_t *pairTile,
+                                uint32_t *pairGauss, const uint32_t *__restrict__ nTilesSorted)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || nTilesSorted[i] == 0)
+        return;
+    const uint32_t g = sorted[i];
+    const int4 r = rect[g];
+    uint32_t o = offsets[i];
+    for (int ty = r.y; ty < r.w; ++ty)
+        for (int tx = r.x; tx < r.z; ++tx) {
+            pairTile[o] = uint32_t(ty * tilesX + tx);
+            pairGauss[o] = g;
+            ++o;
+        }
+}
+
+__global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t numPairs,
+                                  int tiles, int *tileOffsets)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > numPairs)
+        return;
+    const int t = i < numPairs ? int(pairTile[i]) : tiles;
+    const int prev = i > 0 ? int(pairTile[i - 1]) : -1;
+    for (int u = prev + 1; u <= t; ++u)
+        tileOffsets[u] = int(i);
+}
+
+inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
+
+} // namespace
+
+size_t radixHistSize(int64_t count) { return size_t(256) * size_t(blocksFor(count, kBlockKeys) + 1); }
+
+int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
+                   uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
+                   bool *resultInAlt)
+{
+    *resultInAlt = false;
+    if (count <= 0 || bits <= 0)
+        return U3D_OK;
+    const int numBlocks = blocksFor(count, kBlockKeys);
+    U3D_CHECK(size_t(numBlocks) * 256 <= scratch.histCap, U3D_ERR_CAPACITY, "radix histogram scratch too small");
+    uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
+    for (int shift = 0; shift < bits; shift += 8) {
+        const int b = bits - shift < 8 ? bits - shift : 8;
+        hipLaunchKernelGGL(radixUpsweep, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin, count,
+                           shift, b, numBlocks, scratch.hist);
+        U3D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(scanSingleBlock, dim3(1), dim3(kScanThreads), 0, s, scratch.hist,
+                           int64_t(numBlocks) * 256, (uint32_t *)nullptr);
+        U3D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(radixDownsweep, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin, vin,
+                           kout, vout, count, shift, b, numBlocks, scratch.hist);
+        U3D_LAUNCH_CHECK();
+        uint32_t *tk = kin, *tv = vin;
+        kin = kout;
+        vin = vout;
+        kout = tk;
+        vout = tv;
+        *resultInAlt = !*resultInAlt;
+    }
+    return U3D_OK;
+}
+
+int launchIota(hipStream_t s, uint32_t *vals, int64_t count)
+{
+    if (count <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(iotaKernel, dim3(blocksFor(count, 256)), dim3(256), 0, s, vals, count);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
+                    int64_t count)
+{
+    if (count <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(gatherKernel, dim3(blocksFor(count, 256)), dim3(256), 0, s, in, idx, out,
+                       count);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+size_t scanScratchSize(int64_t count) { return size_t(blocksFor(count, kScanBlockItems)) + 2; }
+
+int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t count,
+                     uint32_t *total, uint32_t *blockScratch, size_t blockScratchCap)
+{
+    if (count <= 0) {
+        if (total)
+            U3D_HIP(hipMemsetAsync(total, 0, sizeof(uint32_t), s));
+        return U3D_OK;
+    }
+    const int nb = blocksFor(count, kScanBlockItems);
+    U3D_CHECK(size_t(nb) + 1 <= blockScratchCap, U3D_ERR_CAPACITY, "scan scratch too small");
+    hipLaunchKernelGGL(scanReduce, dim3(nb), dim3(256), 0, s, in, count, blockScratch);
+    U3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scanSingleBlock, dim3(1), dim3(kScanThreads), 0, s, blockScratch,
+                       int64_t(nb), total);
+    U3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scanDownsweep, dim3(nb), dim3(256), 0, s, in, out, count, blockScratch);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
+                    const int4 *rect, int tilesX, uint32_t *pairTile, uint32_t *pairGauss,
+                    const uint32_t *nTilesSorted)
+{
+    if (n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(emitPairsKernel, dim3(blocksFor(n, 256)), dim3(256), 0, s, n, sorted,
+                       offsets, rect, tilesX, pairTile, pairGauss, nTilesSorted);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs, int tiles,
+                      int *tileOffsets)
+{
+    hipLaunchKernelGGL(tileOffsetsKernel, dim3(blocksFor(numPairs + 1, 256)), dim3(256), 0, s,
+                       pairTile, numPairs, tiles, tileOffsets);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+} // namespace u3d

@@ -1,0 +1,375 @@
+"""Python mirror of the reference trainer API (engine/include/u3d/splat/*.h) over libu3d_hip.
+
+Names and argument meaning follow the reference classes: ``Rasterizer.forward/backward/radii``
+(Rasterizer.h:38-90), ``photometric_loss`` (Loss.h:18-19), ``Adam`` (Adam.h:18-33),
+``Trainer`` (Trainer.h:249-292). Arrays cross the boundary as device buffers owned by
+``DeviceArray``; host data is numpy. Every call goes through the C ABI and raises
+``U3DError`` on failure -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import check
+
+FIELDS = (("means", 3), ("scales", 3), ("quats", 4), ("opacities", 1), ("colors", 3))
+
+
+class Context:
+    """u3d_ctx: one HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        check(self.lib.u3d_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(self.lib.u3d_ctx_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.u3d_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation of a fixed dtype and shape, filled from / copied to numpy."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float32):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(ctx.lib.u3d_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_numpy(ctx: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(ctx, a.shape, a.dtype)
+        if a.nbytes:
+            check(ctx.lib.u3d_memcpy_h2d(ctx.h, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(self.ctx.lib.u3d_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(self.ctx.lib.u3d_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.ctx.lib.u3d_free(self.ctx.h, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def _view_numpy(ctx: Context, ptr: int, count: int, dtype) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    out = np.empty(count, dtype)
+    if count:
+        check(ctx.lib.u3d_memcpy_d2h(ctx.h, out.ctypes.data, ptr, out.nbytes))
+    return out
+
+
+class DeviceGaussians:
+    """GaussianModel on the device (also used for gradients): five SoA arrays."""
+
+    def __init__(self, ctx: Context, n: int, capacity: int | None = None):
+        self.ctx = ctx
+        self.n = int(n)
+        cap = max(int(capacity or n), 1)
+        self.arrays = {f: DeviceArray(ctx, (cap * d,)) for f, d in FIELDS}
+        for a in self.arrays.values():
+            a.zero()
+
+    @staticmethod
+    def from_host(ctx: Context, g, capacity: int | None = None) -> "DeviceGaussians":
+        get = (lambda f: g[f]) if isinstance(g, dict) else (lambda f: getattr(g, f))
+        n = len(np.asarray(get("opacities")))
+        d = DeviceGaussians(ctx, n, capacity)
+        for f, _ in FIELDS:
+            a = np.ascontiguousarray(get(f), np.float32).reshape(-1)
+            if a.nbytes:
+                check(ctx.lib.u3d_memcpy_h2d(ctx.h, d.arrays[f].ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def struct(self) -> _abi.Gaussians:
+        a = self.arrays
+        return _abi.Gaussians(a["means"].ptr, a["scales"].ptr, a["quats"].ptr,
+                              a["opacities"].ptr, a["colors"].ptr, self.n)
+
+    def to_host(self) -> dict:
+        return {f: _view_numpy(self.ctx, self.arrays[f].ptr, self.n * d, np.float32) for f, d in FIELDS}
+
+
+def make_camera(cam: dict) -> _abi.Camera:
+    c = _abi.Camera()
+    c.width, c.height = int(cam["width"]), int(cam["height"])
+    c.fx, c.fy, c.cx, c.cy = (float(np.float32(cam[k])) for k in ("fx", "fy", "cx", "cy"))
+    c.R[:] = [float(v) for v in np.asarray(cam["R"], np.float32).reshape(9)]
+    c.t[:] = [float(v) for v in np.asarray(cam["t"], np.float32).reshape(3)]
+    return c
+
+
+def make_settings(tile_size=16, near_plane=0.01, eps2d=0.3, background=(0.0, 0.0, 0.0),
+                  min_alpha=1.0 / 255.0, min_transmittance=1e-4) -> _abi.RasterSettings:
+    """RasterSettings defaults (Rasterizer.h:23-32)."""
+    s = _abi.RasterSettings()
+    s.tile_size = int(tile_size)
+    s.near_plane = float(np.float32(near_plane))
+    s.eps2d = float(np.float32(eps2d))
+    s.background[:] = [float(np.float32(b)) for b in background]
+    s.min_alpha = float(np.float32(min_alpha))
+    s.min_transmittance = float(np.float32(min_transmittance))
+    return s
+
+
+class Rasterizer:
+    """Rasterizer (Rasterizer.h:38-90) on the device."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.u3d_raster_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def __del__(self):
+        try:
+            self.ctx.lib.u3d_raster_destroy(self.h)
+        except Exception:
+            pass
+
+    def forward(self, model: DeviceGaussians, cam: dict, settings=None, out: DeviceArray | None = None):
+        settings = settings or make_settings()
+        c = make_camera(cam)
+        if out is None:
+            out = DeviceArray(self.ctx, (c.height * c.width * 3,))
+        m = model.struct()
+        check(self.ctx.lib.u3d_raster_forward(self.h, C.byref(m), C.byref(c), C.byref(settings), out.ptr))
+        self.n, self.cam = model.n, c
+        return out
+
+    def backward(self, model: DeviceGaussians, dl: DeviceArray, grads: DeviceGaussians,
+                 mean2d_grad_ndc: DeviceArray | None = None):
+        m, g = model.struct(), grads.struct()
+        check(self.ctx.lib.u3d_raster_backward(self.h, C.byref(m), dl.ptr, C.byref(g),
+                                               mean2d_grad_ndc.ptr if mean2d_grad_ndc else None))
+        return grads
+
+    def radii(self) -> np.ndarray:
+        p, n = C.c_void_p(), C.c_int64()
+        check(self.ctx.lib.u3d_raster_radii(self.h, C.byref(p), C.byref(n)))
+        return _view_numpy(self.ctx, p.value, n.value, np.int32)
+
+    def state(self) -> dict:
+        s = _abi.RasterState()
+        check(self.ctx.lib.u3d_raster_get_state(self.h, C.byref(s)))
+        n, K, tiles = s.n, s.num_pairs, s.tiles_x * s.tiles_y
+        P = self.cam.width * self.cam.height
+        v = lambda p, cnt, dt: _view_numpy(self.ctx, p, cnt, dt)
+        return dict(n=n, num_pairs=K, tiles_x=s.tiles_x, tiles_y=s.tiles_y,
+                    radius=v(s.radius, n, np.int32), rect=v(s.rect, 4 * n, np.int32),
+                    depth=v(s.depth, n, np.float32), mean2d=v(s.mean2d, 2 * n, np.float32),
+                    conic=v(s.conic, 3 * n, np.float32), rgb=v(s.color, 3 * n, np.float32),
+                    alpha=v(s.opacity, n, np.float32), flags=v(s.flags, n, np.uint8),
+                    tile_offsets=v(s.tile_offsets, tiles + 1, np.int32),
+                    tile_list=v(s.tile_list, K, np.int32), final_t=v(s.final_t, P, np.float32),
+                    n_contrib=v(s.n_contrib, P, np.int32))
+
+    def grad2d(self) -> np.ndarray:
+        p = C.c_void_p()
+        check(self.ctx.lib.u3d_raster_get_grad2d(self.h, C.byref(p)))
+        return _view_numpy(self.ctx, p.value, self.n * 9, np.float32)
+
+
+def photometric_loss(ctx: Context, render: DeviceArray, target: DeviceArray, width: int,
+                     height: int, ssim_lambda: float, grad: DeviceArray):
+    """photometricLoss (Loss.h:18-19): returns (total, l1, ssim); overwrites grad."""
+    v = _abi.LossValue()
+    check(ctx.lib.u3d_photometric_loss(ctx.h, render.ptr, target.ptr, width, height,
+                                       float(np.float32(ssim_lambda)), grad.ptr, C.byref(v)))
+    return v.total, v.l1, v.ssim
+
+
+def psnr(ctx: Context, a: DeviceArray, b: DeviceArray, count: int) -> float:
+    out = C.c_float()
+    check(ctx.lib.u3d_psnr(ctx.h, a.ptr, b.ptr, count, C.byref(out)))
+    return out.value
+
+
+def learning_rates(means=1.6e-4, scales=5e-3, quats=1e-3, opacities=5e-2, colors=2.5e-3):
+    """LearningRates defaults (Adam.h:8-14)."""
+    return _abi.LearningRates(means, scales, quats, opacities, colors)
+
+
+class Adam:
+    """Adam (Adam.h:18-33) with device moments."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.u3d_adam_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            self.ctx.lib.u3d_adam_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self, n: int):
+        check(self.ctx.lib.u3d_adam_reset(self.h, n))
+
+    def compact(self, keep: DeviceArray):
+        check(self.ctx.lib.u3d_adam_compact(self.h, keep.ptr))
+
+    def grow(self, count: int):
+        check(self.ctx.lib.u3d_adam_grow(self.h, count))
+
+    def reset_opacity_moments(self):
+        check(self.ctx.lib.u3d_adam_reset_opacity_moments(self.h))
+
+    def step(self, params: DeviceGaussians, grads: DeviceGaussians, lr):
+        p, g = params.struct(), grads.struct()
+        check(self.ctx.lib.u3d_adam_step(self.h, C.byref(p), C.byref(g), C.byref(lr)))
+
+    def moments(self):
+        m, v, st = _abi.Gaussians(), _abi.Gaussians(), C.c_int()
+        check(self.ctx.lib.u3d_adam_moments(self.h, C.byref(m), C.byref(v), C.byref(st)))
+        out = []
+        for g in (m, v):
+            out.append({f: _view_numpy(self.ctx, getattr(g, f), g.n * d, np.float32) for f, d in FIELDS})
+        return out[0], out[1], st.value
+
+
+def densify_decide(ctx: Context, model: DeviceGaussians, accum: DeviceArray, count: DeviceArray,
+                   extent: float, max_gaussians: int, grad_threshold=2e-4, percent_dense=0.01,
+                   prune_opacity=0.005, opacity_was_reset=False):
+    """Decisions of Trainer::densifyAndPrune: (keep, kind) device arrays and host counts."""
+    n = model.n
+    keep = DeviceArray(ctx, (max(n, 1),), np.uint8)
+    kind = DeviceArray(ctx, (max(n, 1),), np.uint8)
+    nc, ns, nk = C.c_int64(), C.c_int64(), C.c_int64()
+    m = model.struct()
+    check(ctx.lib.u3d_densify_decide(ctx.h, C.byref(m), accum.ptr, count.ptr, float(extent),
+                                     int(max_gaussians), float(grad_threshold), float(percent_dense),
+                                     float(prune_opacity), int(bool(opacity_was_reset)), keep.ptr,
+                                     kind.ptr, C.byref(nc), C.byref(ns), C.byref(nk)))
+    return keep, kind, dict(clones=nc.value, splits=ns.value, kept=nk.value)
+
+
+def densify_apply(ctx: Context, model: DeviceGaussians, keep: DeviceArray, kind: DeviceArray,
+                  normals: DeviceArray | None, log_shrink: float) -> int:
+    m = model.struct()
+    nn = C.c_int64()
+    check(ctx.lib.u3d_densify_apply(ctx.h, C.byref(m), keep.ptr, kind.ptr,
+                                    normals.ptr if normals is not None else None,
+                                    float(log_shrink), C.byref(nn)))
+    model.n = nn.value
+    return nn.value
+
+
+def options_for_iterations(iterations: int) -> _abi.TrainOptions:
+    """TrainOptions defaults (Trainer.h:19-37) with TrainOptions::forIterations (Trainer.cpp:26-35)."""
+    o = _abi.TrainOptions()
+    o.iterations = max(1, int(iterations))
+    o.max_gaussians = 200000
+    o.ssim_lambda = 0.2
+    o.lr = learning_rates()
+    o.means_lr_final = 1.6e-6
+    o.densify_grad_threshold = 2e-4
+    o.percent_dense = 0.01
+    o.refine_start = min(500, o.iterations // 10)
+    o.refine_stop = o.iterations * 6 // 10
+    o.refine_interval = 100
+    o.opacity_reset_interval = 3000 if o.iterations >= 6000 else 0
+    o.prune_opacity = 0.005
+    o.seed = 42
+    o.raster = make_settings()
+    return o
+
+
+class Trainer:
+    """Trainer (Trainer.h:249-292) with all state on the device."""
+
+    def __init__(self, ctx: Context, initial, cams: list, images_u8: list, extent: float,
+                 options: _abi.TrainOptions):
+        self.ctx = ctx
+        get = (lambda f: initial[f]) if isinstance(initial, dict) else (lambda f: getattr(initial, f))
+        arrs = [np.ascontiguousarray(get(f), np.float32).reshape(-1) for f, _ in FIELDS]
+        n = len(arrs[3])
+        cam_arr = (_abi.Camera * max(len(cams), 1))(*[make_camera(c) for c in cams])
+        rgb = np.ascontiguousarray(np.concatenate([np.asarray(i, np.uint8).reshape(-1) for i in images_u8])
+                                   if images_u8 else np.zeros(1, np.uint8))
+        h = C.c_void_p()
+        check(ctx.lib.u3d_trainer_create(ctx.h, *(a.ctypes.data for a in arrs), n, cam_arr,
+                                         rgb.ctypes.data, len(cams), float(extent),
+                                         C.byref(options), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            self.ctx.lib.u3d_trainer_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self):
+        check(self.ctx.lib.u3d_trainer_step(self.h))
+
+    def done(self) -> bool:
+        return bool(self.ctx.lib.u3d_trainer_done(self.h))
+
+    def iteration(self) -> int:
+        return int(self.ctx.lib.u3d_trainer_iteration(self.h))
+
+    def size(self) -> int:
+        return int(self.ctx.lib.u3d_trainer_size(self.h))
+
+    def model(self) -> dict:
+        n = self.size()
+        out = {f: np.zeros(n * d, np.float32) for f, d in FIELDS}
+        check(self.ctx.lib.u3d_trainer_get_model(self.h, *(out[f].ctypes.data for f, _ in FIELDS)))
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.TrainStats()
+        check(self.ctx.lib.u3d_trainer_stats(self.h, C.byref(s)))
+        return dict(iteration=s.iteration, loss=s.loss, psnr=s.psnr, gaussians=s.gaussians)
+
+    def evaluate_psnr(self) -> float:
+        v = C.c_float()
+        check(self.ctx.lib.u3d_trainer_evaluate_psnr(self.h, C.byref(v)))
+        return v.value
+
+    def evaluate_l1(self) -> float:
+        v = C.c_float()
+        check(self.ctx.lib.u3d_trainer_evaluate_l1(self.h, C.byref(v)))
+        return v.value
+
+    def render(self, cam: dict) -> np.ndarray:
+        c = make_camera(cam)
+        out = np.zeros(c.width * c.height * 3, np.float32)
+        check(self.ctx.lib.u3d_trainer_render(self.h, C.byref(c), out.ctypes.data))
+        return out
+
+    def synchronize(self):
+        self.ctx.synchronize()
