@@ -119,7 +119,7 @@ struct u3d_raster {
     DevBuf<uint8_t> flags;
     DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
     DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
-    DevBuf<uint32_t> hist, scanScratch, total;
+    DevBuf<uint32_t> hist, histScan, scanScratch, total;
     DevBuf<int> tileOffsets;
     const int *tileList = nullptr;
     DevBuf<float> finalT, grad2d;
@@ -180,7 +180,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     // Depth order of the visible Gaussians (invisible keys sort last).
     U3D_TRY(launchIota(s, r->valA.p, n));
     U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
-    RadixScratch rs{r->hist.p, r->hist.cap};
+    U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(nn))));
+    RadixScratch rs{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
     bool alt = false;
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
     const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
@@ -197,7 +198,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     U3D_TRY(r->pairTileAlt.ensure(kk));
     U3D_TRY(r->pairGaussAlt.ensure(kk));
     U3D_TRY(r->hist.ensure(radixHistSize(int64_t(kk))));
-    RadixScratch rs2{r->hist.p, r->hist.cap};
+    U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(kk))));
+    RadixScratch rs2{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
     U3D_TRY(launchEmitPairs(s, n, sorted, r->pairOffsets.p, r->rect.p, tilesX, r->pairTile.p,
                             r->pairGauss.p, r->ntSorted.p));
     bool alt2 = false;

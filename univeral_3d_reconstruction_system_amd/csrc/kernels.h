@@ -36,13 +36,16 @@ int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float 
 // (device pointer to the count when countDev != nullptr, else `count`). Ping-pongs between
 // (keys, vals) and (keysAlt, valsAlt); returns in *resultInAlt where the output landed.
 struct RadixScratch {
-    uint32_t *hist; // >= 256 * maxBlocks
+    uint32_t *hist;    // radixHistSize(count)
     size_t histCap;
+    uint32_t *scanTmp; // radixScanScratchSize(count)
+    size_t scanTmpCap;
 };
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
                    bool *resultInAlt);
 size_t radixHistSize(int64_t count);
+size_t radixScanScratchSize(int64_t count);
 // values[i] = i
 int launchIota(hipStream_t s, uint32_t *vals, int64_t count);
 // out[i] = in[idx[i]] for i < count

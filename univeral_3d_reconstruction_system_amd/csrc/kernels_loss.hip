@@ -77,7 +77,8 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
             return float(targetU8[idx]) * (1.0f / 255.0f);
         return targetF[idx];
     };
-    for (int c = 0; c < 3; ++c) {
+    {
+        const int c = blockIdx.y; // one colour channel per workgroup
         if (ssimOn) {
             // 1. inputs with a 10-pixel halo
             for (int k = tid; k < kIH * kIW; k += kThreads) {
@@ -196,9 +197,9 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
     const double s2 = blockSumD(ssim, red);
     const double s3 = blockSumD(sq, red);
     if (tid == 0) {
-        partials[size_t(blockIdx.x) * 3] = s1;
-        partials[size_t(blockIdx.x) * 3 + 1] = s2;
-        partials[size_t(blockIdx.x) * 3 + 2] = s3;
+        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3] = s1;
+        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3 + 1] = s2;
+        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3 + 2] = s3;
     }
 }
 
@@ -295,7 +296,7 @@ constexpr int kErrBlocks = 1024;
 size_t lossPartialsSize(int width, int height)
 {
     const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
-    return size_t(bx) * size_t(by) * 3;
+    return size_t(bx) * size_t(by) * 9;
 }
 
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
@@ -303,21 +304,21 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
                           float *grad, double *partials, size_t partialsCap, float *result)
 {
     const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
-    const int nb = bx * by;
-    U3D_CHECK(size_t(nb) * 3 <= partialsCap, U3D_ERR_CAPACITY, "loss partials too small");
+    const int nb = bx * by; // per channel
+    U3D_CHECK(size_t(nb) * 9 <= partialsCap, U3D_ERR_CAPACITY, "loss partials too small");
     const size_t P = size_t(width) * size_t(height);
     const float invCount = 1.0f / float(P * 3);
     static const Window win = makeWindow();
     if (nb > 0) {
         if (targetU8)
-            hipLaunchKernelGGL(lossKernel<true>, dim3(nb), dim3(kThreads), 0, s, render, target,
+            hipLaunchKernelGGL(lossKernel<true>, dim3(nb, 3), dim3(kThreads), 0, s, render, target,
                                targetU8, width, height, lambda, invCount, win, bx, grad, partials);
         else
-            hipLaunchKernelGGL(lossKernel<false>, dim3(nb), dim3(kThreads), 0, s, render, target,
+            hipLaunchKernelGGL(lossKernel<false>, dim3(nb, 3), dim3(kThreads), 0, s, render, target,
                                targetU8, width, height, lambda, invCount, win, bx, grad, partials);
         U3D_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, partials, nb, lambda,
+    hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, partials, 3 * nb, lambda,
                        invCount, double(P * 3), result);
     U3D_LAUNCH_CHECK();
     return U3D_OK;

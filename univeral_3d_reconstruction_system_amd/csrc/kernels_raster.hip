@@ -1,21 +1,29 @@
 // Tile compositing, forward (Rasterizer.cpp:263-330) and backward stage 1 (Rasterizer.cpp:
 // 334-427), for gfx950.
 //
-// One 256-thread workgroup (4 wave64) per 16x16 pixel block, one pixel per thread; a logical
-// tile of tile_size (a multiple of 16) is covered by (tile_size/16)^2 blocks that share its list.
-// The tile's Gaussians are staged through LDS in batches of 256 records (36 B each, gathered
-// once per batch by one thread each) and every thread walks the batch for its pixel.
+// One 256-thread workgroup per 16x16 pixel block; each wave64 owns one 8x8 quadrant, one pixel
+// per lane. A logical tile of tile_size (a multiple of 16) is covered by (tile_size/16)^2 blocks
+// that share its list. The tile's Gaussians are staged through LDS in batches of 256 records
+// (36 B each, gathered once per batch, one per thread).
+//
+// Exact culling. While staging, every Gaussian gets a 4-bit mask of the quadrants in which it
+// can contribute: the minimum of its quadratic form over the quadrant's pixel rectangle
+// (interior or the closest point of one of the four edges) must bring the power above
+// log(min_alpha / alpha) - 1e-3, with a rounding margin. A quadrant's wave then visits only
+// the masked Gaussians, in list order, iterating set bits of a ballot. Culled pairs are pairs
+// the reference skips anyway (alpha < min_alpha), so the sequence of contributing pairs, and
+// every result, is unchanged.
 //
 // Forward: front-to-back with the reference's exact arithmetic (contraction off, host-identical
-// expf), so images, final transmittance and contributor counts are bit-identical. Most pairs are
-// rejected before any exp: power > 0, or power below log(min_alpha / alpha) minus a 1e-3 margin
-// (then alpha * exp(power) < min_alpha is certain); only the rest evaluate the exact exp.
-// The workgroup stops early once every pixel has saturated (__syncthreads_and).
+// expf), so images, final transmittance and contributor counts are bit-identical. Pairs below
+// the per-Gaussian power threshold are rejected before any exp. The workgroup stops once every
+// pixel has saturated.
 //
-// Backward: reverse walk from the block's largest contributor count; per Gaussian the nine 2D
-// gradient terms of the wave's 64 pixels are summed with DPP row shifts and broadcasts (skipped
-// when no lane contributes), the four waves combine in LDS, and the batch is flushed with one
-// global float atomic per non-zero term per (block, Gaussian).
+// Backward: reverse walk from each wave's largest contributor count; per Gaussian the nine 2D
+// gradient terms are summed over each 16-lane row with DPP row shifts (only when a lane
+// contributes), the contributing rows of the four waves add their totals into an LDS batch
+// accumulator, and each batch is flushed with one global float atomic per non-zero term per
+// (block, Gaussian).
 #include "kernels.h"
 
 namespace u3d {
@@ -31,16 +39,14 @@ template <int CTRL, int ROW_MASK> __device__ __forceinline__ float dppMove(float
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false));
 }
 
-// Sum over the 64 lanes of the wave (every lane must be active); result in all lanes.
-__device__ __forceinline__ float waveSum(float x)
+// Inclusive sum within each 16-lane row; lane 15 of a row holds the row total.
+__device__ __forceinline__ float rowSum(float x)
 {
     x += dppMove<0x111, 0xf>(x); // row_shr:1
     x += dppMove<0x112, 0xf>(x); // row_shr:2
     x += dppMove<0x114, 0xf>(x); // row_shr:4
     x += dppMove<0x118, 0xf>(x); // row_shr:8
-    x += dppMove<0x142, 0xa>(x); // row_bcast:15 into rows 1, 3
-    x += dppMove<0x143, 0xc>(x); // row_bcast:31 into rows 2, 3
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+    return x;
 }
 
 // log(min_alpha / alpha) - 1e-3: below it, min(0.99, alpha * exp(power)) < min_alpha holds for
@@ -48,6 +54,46 @@ __device__ __forceinline__ float waveSum(float x)
 __device__ __forceinline__ float skipThreshold(float minAlpha, float alpha)
 {
     return minAlpha > 0.0f ? logf(minAlpha / alpha) - 1e-3f : -INFINITY;
+}
+
+// Minimum of Q(d) = A dx^2 + 2B dx dy + C dy^2 over the box d in [x0, x1] x [y0, y1] (Q positive
+// definite); also returns a bound on the magnitude of the terms for the rounding margin.
+__device__ __forceinline__ float boxMinQuad(float A, float B, float C, float x0, float x1, float y0,
+                                            float y1, float *scale)
+{
+    const float dmax = fmaxf(fmaxf(fabsf(x0), fabsf(x1)), fmaxf(fabsf(y0), fabsf(y1)));
+    *scale = (fabsf(A) + 2.0f * fabsf(B) + fabsf(C)) * dmax * dmax;
+    if (x0 <= 0.0f && 0.0f <= x1 && y0 <= 0.0f && 0.0f <= y1)
+        return 0.0f;
+    auto q = [&](float dx, float dy) { return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy; };
+    auto clampf = [](float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); };
+    // edges dx = x0, x1 (minimise over dy) and dy = y0, y1 (minimise over dx)
+    const float ya = clampf(-B * x0 / C, y0, y1), yb = clampf(-B * x1 / C, y0, y1);
+    const float xa = clampf(-B * y0 / A, x0, x1), xb = clampf(-B * y1 / A, x0, x1);
+    return fminf(fminf(q(x0, ya), q(x1, yb)), fminf(q(xa, y0), q(xb, y1)));
+}
+
+// Quadrant mask of one staged Gaussian for the block at (bx0, by0): bit w set when the
+// Gaussian may contribute to a pixel of quadrant w (8x8, w = qy * 2 + qx).
+__device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, int bx0, int by0)
+{
+    // Only positive-definite conics have the box minimum used below; anything else is kept.
+    if (!(thr > -INFINITY) || !(a.z > 0.0f && C > 0.0f && a.z * C - a.w * a.w > 0.0f))
+        return 0xfu;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const float px0 = float(bx0 + (w & 1) * 8), py0 = float(by0 + (w >> 1) * 8);
+        // d = mean - pixel over pixel centres [p0, p0 + 7]
+        float scale;
+        const float qmin = boxMinQuad(a.z, a.w, C, a.x - (px0 + 7.0f), a.x - px0,
+                                      a.y - (py0 + 7.0f), a.y - py0, &scale);
+        const float powerMax = -0.5f * qmin + 1e-6f * scale + 1e-6f;
+        // keep unless certainly below the threshold (NaN keeps)
+        if (!(powerMax < thr))
+            mask |= 1u << w;
+    }
+    return mask;
 }
 
 __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
@@ -61,13 +107,15 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     __shared__ float4 s1[kBatch];
     __shared__ float s2[kBatch];
     __shared__ float sthr[kBatch];
+    __shared__ uint32_t smask[kBatch];
     __shared__ uint64_t stab[32];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 32)
         stab[tid] = kExp2Table[tid];
     const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
-    const int px = bx * kEdge + (tid & 15), py = by * kEdge + (tid >> 4);
-    const int tile = ((by * kEdge) / ts) * tilesX + (bx * kEdge) / ts;
+    const int bx0 = bx * kEdge, by0 = by * kEdge;
+    const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
+    const int tile = (by0 / ts) * tilesX + bx0 / ts;
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
     const float fpx = float(px), fpy = float(py);
@@ -78,18 +126,29 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
         if (__syncthreads_and(done))
             break;
         const int idx = base + tid;
+        uint32_t m = 0;
         if (idx < end) {
             const int g = tileList[idx];
+            const float4 a = rec0[g];
             const float4 b = rec1[g];
-            s0[tid] = rec0[g];
+            const float thr = skipThreshold(minAlpha, b.y);
+            s0[tid] = a;
             s1[tid] = b;
             s2[tid] = rec2[g];
-            sthr[tid] = skipThreshold(minAlpha, b.y);
+            sthr[tid] = thr;
+            m = quadrantMask(a, b.x, thr, bx0, by0);
         }
+        smask[tid] = m;
         __syncthreads();
-        if (!done) {
-            const int cnt = min(kBatch, end - base);
-            for (int j = 0; j < cnt; ++j) {
+        if (__ballot(!done) == 0)
+            continue;
+        for (int c = 0; c < kBatch / 64; ++c) {
+            uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
+            while (bits) {
+                const int j = c * 64 + __builtin_ctzll(bits);
+                bits &= bits - 1;
+                if (done)
+                    continue;
                 const float4 a = s0[j];
                 const float dx = a.x - fpx, dy = a.y - fpy;
                 const float4 b = s1[j];
@@ -105,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
                 const float nextT = T * (1.0f - alpha);
                 if (nextT < minT) {
                     done = true;
-                    break;
+                    continue;
                 }
                 const float w = alpha * T;
                 cr += b.z * w;
@@ -114,6 +173,8 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
                 T = nextT;
                 last = base - begin + j + 1;
             }
+            if (__ballot(!done) == 0)
+                break;
         }
     }
     if (inside) {
@@ -138,19 +199,19 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     __shared__ float s2[kBatch];
     __shared__ float sthr[kBatch];
     __shared__ int sid[kBatch];
+    __shared__ uint32_t smask[kBatch];
     __shared__ float acc[kBatch][kGrad2dStride];
     __shared__ uint64_t stab[32];
-    __shared__ int smax;
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ int waveMax[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 32)
         stab[tid] = kExp2Table[tid];
-    if (tid == 0)
-        smax = 0;
     for (int k = tid; k < kBatch * kGrad2dStride; k += kThreads)
         (&acc[0][0])[k] = 0.0f;
     const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
-    const int px = bx * kEdge + (tid & 15), py = by * kEdge + (tid >> 4);
-    const int tile = ((by * kEdge) / ts) * tilesX + (bx * kEdge) / ts;
+    const int bx0 = bx * kEdge, by0 = by * kEdge;
+    const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
+    const int tile = (by0 / ts) * tilesX + bx0 / ts;
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile];
     const size_t pix = inside ? size_t(py) * size_t(W) + size_t(px) : 0;
@@ -164,93 +225,114 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     float T = fT;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
     float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
+    {
+        int m = nc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 0)
+            waveMax[wave] = m;
+    }
     __syncthreads();
-    if (nc > 0)
-        atomicMax(&smax, nc);
-    __syncthreads();
-    const int maxN = smax;
+    const int wm0 = waveMax[0], wm1 = waveMax[1], wm2 = waveMax[2], wm3 = waveMax[3];
+    const int maxN = max(max(wm0, wm1), max(wm2, wm3));
     for (int hi = maxN; hi > 0; hi -= kBatch) {
         const int lo = hi > kBatch ? hi - kBatch : 0;
         const int cnt = hi - lo;
+        uint32_t m = 0;
         if (tid < cnt) {
-            const int g = tileList[begin + lo + tid];
+            const int j = lo + tid;
+            const int g = tileList[begin + j];
+            const float4 a = rec0[g];
             const float4 b = rec1[g];
+            const float thr = skipThreshold(minAlpha, b.y);
             sid[tid] = g;
-            s0[tid] = rec0[g];
+            s0[tid] = a;
             s1[tid] = b;
             s2[tid] = rec2[g];
-            sthr[tid] = skipThreshold(minAlpha, b.y);
+            sthr[tid] = thr;
+            m = quadrantMask(a, b.x, thr, bx0, by0);
+            m &= (j < wm0 ? 1u : 0u) | (j < wm1 ? 2u : 0u) | (j < wm2 ? 4u : 0u) | (j < wm3 ? 8u : 0u);
         }
+        smask[tid] = m;
         __syncthreads();
-        for (int jj = cnt - 1; jj >= 0; --jj) {
-            const int j = lo + jj;
-            float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
-            bool contrib = false;
-            if (j < nc) {
-                const float4 a = s0[jj];
-                const float4 b = s1[jj];
-                const float dx = a.x - fpx, dy = a.y - fpy;
-                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                if (!(power > 0.0f) && !(power < sthr[jj])) {
-                    const float G = expfExactLds(power, stab);
-                    const float op = b.y;
-                    const float rawAlpha = op * G;
-                    const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
-                    if (!(alpha < minAlpha)) {
-                        contrib = true;
-                        const float c0 = b.z, c1 = b.w, c2 = s2[jj];
-                        T = T / (1.0f - alpha);
-                        const float w = alpha * T;
-                        float dAlpha = 0.0f;
-                        acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
-                        last0 = c0;
-                        dAlpha += (c0 - acc0) * dp0;
-                        v5 = w * dp0;
-                        acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
-                        last1 = c1;
-                        dAlpha += (c1 - acc1) * dp1;
-                        v6 = w * dp1;
-                        acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
-                        last2 = c2;
-                        dAlpha += (c2 - acc2) * dp2;
-                        v7 = w * dp2;
-                        dAlpha *= T;
-                        lastAlpha = alpha;
-                        dAlpha += -fT / (1.0f - alpha) * bgDot;
-                        if (!(rawAlpha > kMaxAlpha)) {
-                            v8 = G * dAlpha;
-                            const float dPower = op * dAlpha * G;
-                            const float A = a.z, B = a.w, C = b.x;
-                            v0 = dPower * -(A * dx + B * dy);
-                            v1 = dPower * -(C * dy + B * dx);
-                            v2 = dPower * -0.5f * dx * dx;
-                            v3 = dPower * -dx * dy;
-                            v4 = dPower * -0.5f * dy * dy;
+        for (int c = kBatch / 64 - 1; c >= 0; --c) {
+            uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
+            while (bits) {
+                const int jj = c * 64 + 63 - __builtin_clzll(bits);
+                bits &= ~(uint64_t(1) << (jj - c * 64));
+                const int j = lo + jj;
+                float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
+                bool contrib = false;
+                if (j < nc) {
+                    const float4 a = s0[jj];
+                    const float4 b = s1[jj];
+                    const float dx = a.x - fpx, dy = a.y - fpy;
+                    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                    if (!(power > 0.0f) && !(power < sthr[jj])) {
+                        const float G = expfExactLds(power, stab);
+                        const float op = b.y;
+                        const float rawAlpha = op * G;
+                        const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
+                        if (!(alpha < minAlpha)) {
+                            contrib = true;
+                            const float c0 = b.z, c1 = b.w, c2 = s2[jj];
+                            T = T / (1.0f - alpha);
+                            const float w = alpha * T;
+                            float dAlpha = 0.0f;
+                            acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
+                            last0 = c0;
+                            dAlpha += (c0 - acc0) * dp0;
+                            v5 = w * dp0;
+                            acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
+                            last1 = c1;
+                            dAlpha += (c1 - acc1) * dp1;
+                            v6 = w * dp1;
+                            acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
+                            last2 = c2;
+                            dAlpha += (c2 - acc2) * dp2;
+                            v7 = w * dp2;
+                            dAlpha *= T;
+                            lastAlpha = alpha;
+                            dAlpha += -fT / (1.0f - alpha) * bgDot;
+                            if (!(rawAlpha > kMaxAlpha)) {
+                                v8 = G * dAlpha;
+                                const float dPower = op * dAlpha * G;
+                                const float A = a.z, B = a.w, C = b.x;
+                                v0 = dPower * -(A * dx + B * dy);
+                                v1 = dPower * -(C * dy + B * dx);
+                                v2 = dPower * -0.5f * dx * dx;
+                                v3 = dPower * -dx * dy;
+                                v4 = dPower * -0.5f * dy * dy;
+                            }
                         }
                     }
                 }
-            }
-            if (__ballot(contrib)) {
-                v0 = waveSum(v0);
-                v1 = waveSum(v1);
-                v2 = waveSum(v2);
-                v3 = waveSum(v3);
-                v4 = waveSum(v4);
-                v5 = waveSum(v5);
-                v6 = waveSum(v6);
-                v7 = waveSum(v7);
-                v8 = waveSum(v8);
-                if (lane == 0) {
-                    float *dst = acc[jj];
-                    if (v0 != 0.0f) atomicAdd(&dst[0], v0);
-                    if (v1 != 0.0f) atomicAdd(&dst[1], v1);
-                    if (v2 != 0.0f) atomicAdd(&dst[2], v2);
-                    if (v3 != 0.0f) atomicAdd(&dst[3], v3);
-                    if (v4 != 0.0f) atomicAdd(&dst[4], v4);
-                    if (v5 != 0.0f) atomicAdd(&dst[5], v5);
-                    if (v6 != 0.0f) atomicAdd(&dst[6], v6);
-                    if (v7 != 0.0f) atomicAdd(&dst[7], v7);
-                    if (v8 != 0.0f) atomicAdd(&dst[8], v8);
+                const uint64_t live = __ballot(contrib);
+                if (live) {
+                    // Sum inside each 16-lane row with DPP, then the rows that hold a
+                    // contribution add their totals to the batch accumulator in LDS.
+                    v0 = rowSum(v0);
+                    v1 = rowSum(v1);
+                    v2 = rowSum(v2);
+                    v3 = rowSum(v3);
+                    v4 = rowSum(v4);
+                    v5 = rowSum(v5);
+                    v6 = rowSum(v6);
+                    v7 = rowSum(v7);
+                    v8 = rowSum(v8);
+                    if ((lane & 15) == 15 && ((live >> (lane & 48)) & 0xffffull)) {
+                        float *dst = acc[jj];
+                        atomicAdd(&dst[0], v0);
+                        atomicAdd(&dst[1], v1);
+                        atomicAdd(&dst[2], v2);
+                        atomicAdd(&dst[3], v3);
+                        atomicAdd(&dst[4], v4);
+                        atomicAdd(&dst[5], v5);
+                        atomicAdd(&dst[6], v6);
+                        atomicAdd(&dst[7], v7);
+                        atomicAdd(&dst[8], v8);
+                    }
                 }
             }
         }

@@ -21,9 +21,12 @@ namespace {
 
 constexpr int kRadixThreads = 256;
 constexpr int kWaves = kRadixThreads / 64;
-constexpr int kItems = 16;                                  // keys per lane per block
-constexpr int kBlockKeys = kRadixThreads * kItems;          // 4096 keys per block
+constexpr int kMinItems = 4;                                // keys per lane per block (smallest)
 constexpr int kScanThreads = 1024;
+
+// Keys per lane: enough blocks to fill the chip for small arrays, more work per block for
+// large ones (amortises the 256-entry histogram each block writes).
+inline int itemsFor(int64_t count) { return count <= (int64_t(1) << 20) ? 4 : 8; }
 
 __device__ __forceinline__ uint64_t laneMaskLt()
 {
@@ -42,6 +45,7 @@ __device__ __forceinline__ uint64_t matchDigit(uint32_t digit, uint64_t valid, i
     return m;
 }
 
+template <int kItems>
 __global__ __launch_bounds__(kRadixThreads) void radixUpsweep(const uint32_t *__restrict__ keys,
                                                               int64_t count, int shift, int bits,
                                                               int numBlocks, uint32_t *hist)
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(kRadixThreads) void radixUpsweep(const uint32_t *__
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     const uint32_t mask = (1u << bits) - 1u;
-    const int64_t base = int64_t(blockIdx.x) * kBlockKeys + int64_t(wave) * 64 * kItems;
+    const int64_t base = int64_t(blockIdx.x) * kRadixThreads * kItems + int64_t(wave) * 64 * kItems;
     for (int it = 0; it < kItems; ++it) {
         const int64_t idx = base + int64_t(it) * 64 + lane;
         const bool valid = idx < count;
@@ -128,6 +132,7 @@ __global__ __launch_bounds__(kScanThreads) void scanSingleBlock(uint32_t *data, 
         *total = carry;
 }
 
+template <int kItems>
 __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn, uint32_t *keysOut,
     uint32_t *valsOut, int64_t count, int shift, int bits, int numBlocks,
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     gbase[tid] = hist[size_t(tid) * size_t(numBlocks) + blockIdx.x];
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
-    const int64_t base = int64_t(blockIdx.x) * kBlockKeys + int64_t(wave) * 64 * kItems;
+    const int64_t base = int64_t(blockIdx.x) * kRadixThreads * kItems + int64_t(wave) * 64 * kItems;
     uint32_t k[kItems], v[kItems], rank[kItems];
 #pragma unroll
     for (int it = 0; it < kItems; ++it) {
@@ -300,28 +305,30 @@ inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
 
 } // namespace
 
-size_t radixHistSize(int64_t count) { return size_t(256) * size_t(blocksFor(count, kBlockKeys) + 1); }
-
-int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
-                   uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
-                   bool *resultInAlt)
+size_t radixHistSize(int64_t count)
 {
-    *resultInAlt = false;
-    if (count <= 0 || bits <= 0)
-        return U3D_OK;
-    const int numBlocks = blocksFor(count, kBlockKeys);
+    return size_t(256) * size_t(blocksFor(count, kRadixThreads * kMinItems) + 1);
+}
+
+size_t radixScanScratchSize(int64_t count) { return scanScratchSize(int64_t(radixHistSize(count))); }
+
+namespace {
+template <int kItems>
+int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt,
+                int64_t count, int bits, RadixScratch &scratch, bool *resultInAlt)
+{
+    const int numBlocks = blocksFor(count, kRadixThreads * kItems);
     U3D_CHECK(size_t(numBlocks) * 256 <= scratch.histCap, U3D_ERR_CAPACITY, "radix histogram scratch too small");
     uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
     for (int shift = 0; shift < bits; shift += 8) {
         const int b = bits - shift < 8 ? bits - shift : 8;
-        hipLaunchKernelGGL(radixUpsweep, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin, count,
-                           shift, b, numBlocks, scratch.hist);
+        hipLaunchKernelGGL(radixUpsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
+                           count, shift, b, numBlocks, scratch.hist);
         U3D_LAUNCH_CHECK();
-        hipLaunchKernelGGL(scanSingleBlock, dim3(1), dim3(kScanThreads), 0, s, scratch.hist,
-                           int64_t(numBlocks) * 256, (uint32_t *)nullptr);
-        U3D_LAUNCH_CHECK();
-        hipLaunchKernelGGL(radixDownsweep, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin, vin,
-                           kout, vout, count, shift, b, numBlocks, scratch.hist);
+        U3D_TRY(exclusiveScanU32(s, scratch.hist, scratch.hist, int64_t(numBlocks) * 256, nullptr,
+                                 scratch.scanTmp, scratch.scanTmpCap));
+        hipLaunchKernelGGL(radixDownsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
+                           vin, kout, vout, count, shift, b, numBlocks, scratch.hist);
         U3D_LAUNCH_CHECK();
         uint32_t *tk = kin, *tv = vin;
         kin = kout;
@@ -331,6 +338,19 @@ int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys
         *resultInAlt = !*resultInAlt;
     }
     return U3D_OK;
+}
+} // namespace
+
+int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
+                   uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
+                   bool *resultInAlt)
+{
+    *resultInAlt = false;
+    if (count <= 0 || bits <= 0)
+        return U3D_OK;
+    if (itemsFor(count) == 4)
+        return radixPasses<4>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
+    return radixPasses<8>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
 }
 
 int launchIota(hipStream_t s, uint32_t *vals, int64_t count)
