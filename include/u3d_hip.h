@@ -90,6 +90,11 @@ typedef struct {
     float prune_opacity;
     unsigned seed;
     u3d_raster_settings raster;
+    /* Extensions (no reference counterpart; zero keeps the reference behaviour). */
+    int sh_degree;       /* 0..3: view-dependent colour, coefficients start at zero */
+    float sh_lr_scale;   /* rate of the SH coefficients = lr.colors * scale (0 -> 1/20) */
+    int views_per_step;  /* B >= 1 views per optimiser step (0 -> 1) */
+    int use_graph;       /* capture each step's device work as a hipGraph */
 } u3d_train_options;
 
 /* TrainStats (Trainer.h:39-44). */
@@ -147,6 +152,16 @@ int u3d_raster_forward(u3d_raster *r, const u3d_gaussians *model, const u3d_came
  * statistic |dL/d mean2d| in NDC into mean2d_grad_ndc [n]. */
 int u3d_raster_backward(u3d_raster *r, const u3d_gaussians *model, const float *dl_drgb,
                         u3d_gaussians *grads, float *mean2d_grad_ndc);
+/* Spherical-harmonic colour of degree 1..3 (extension; the reference trains DC only):
+ * sh_rest holds 15 x 3 coefficients per Gaussian (coefficient-major, [n][15][3]); the first
+ * 3, 8 or 15 are used for degree 1, 2, 3. Degree 0 (or sh_rest == NULL) is the plain call. */
+int u3d_raster_forward_sh(u3d_raster *r, const u3d_gaussians *model, const float *sh_rest,
+                          int sh_degree, const u3d_camera *camera,
+                          const u3d_raster_settings *settings, float *rgb);
+/* Backward of the last forward_sh: also ADDS dL/d sh_rest into grad_sh_rest [45n] (nullable). */
+int u3d_raster_backward_sh(u3d_raster *r, const u3d_gaussians *model, const float *sh_rest,
+                           int sh_degree, const float *dl_drgb, u3d_gaussians *grads,
+                           float *grad_sh_rest, float *mean2d_grad_ndc);
 /* Rasterizer::radii (Rasterizer.h:53-54): device pointer valid until the next forward. */
 int u3d_raster_radii(u3d_raster *r, const int **radii, int64_t *n);
 /* Test and tooling access to the forward state (Rasterizer.h:60-89). */
@@ -209,6 +224,8 @@ int64_t u3d_trainer_size(u3d_trainer *t);
 /* Trainer::model: copies the current parameters to host arrays sized by u3d_trainer_size. */
 int u3d_trainer_get_model(u3d_trainer *t, float *means_host, float *scales_host,
                           float *quats_host, float *opacities_host, float *colors_host);
+/* SH coefficients [45 * size] of a trainer created with sh_degree > 0. */
+int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host);
 int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out);      /* Trainer::stats */
 int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out_host);   /* Trainer.cpp:180-192 */
 /* Mean absolute error over all training views (time-to-target metric). */

@@ -149,6 +149,14 @@ def lib():
         D64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         L.orc_render_f64.argtypes = [C.c_int, D64, D64, D64, D64, D64, C.POINTER(Camera),
                                      C.POINTER(Settings), D64]
+        L.orc_raster_forward_sh.argtypes = [C.c_void_p, C.POINTER(Model), C.c_void_p, C.c_int,
+                                            C.POINTER(Camera), C.POINTER(Settings), F32P]
+        L.orc_raster_backward_sh.argtypes = [C.c_void_p, C.POINTER(Model), C.c_void_p, C.c_int, F32P,
+                                             C.POINTER(Model), C.c_void_p, C.c_void_p]
+        L.orc_render_f64_sh.argtypes = [C.c_int, D64, D64, D64, D64, D64, D64, C.c_int,
+                                        C.POINTER(Camera), C.POINTER(Settings), D64]
+        L.orc_sh_basis.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, D64]
+        L.orc_adam_array.argtypes = [F32P, F32P, F32P, F32P, C.c_long, C.c_float, C.c_int]
         L.orc_photometric_loss.restype = LossValue
         L.orc_photometric_loss.argtypes = [F32P, F32P, C.c_int, C.c_int, C.c_float, F32P]
         L.orc_psnr.restype = C.c_float
@@ -252,14 +260,32 @@ class Raster:
         except Exception:
             pass
 
-    def forward(self, g: Gaussians, cam: dict, st: Settings) -> np.ndarray:
+    def forward(self, g: Gaussians, cam: dict, st: Settings, sh_rest=None, sh_degree=0) -> np.ndarray:
         c = make_camera(cam)
         rgb = np.zeros(c.width * c.height * 3, np.float32)
         m = g.struct()
-        fn = self.L.ref_raster_forward if self.use_ref else self.L.orc_raster_forward
-        fn(self.h, C.byref(m), C.byref(c), C.byref(st), rgb)
+        if sh_rest is not None:
+            assert not self.use_ref, "the reference has no SH beyond degree 0"
+            self._rest = np.ascontiguousarray(sh_rest, np.float32)
+            self.L.orc_raster_forward_sh(self.h, C.byref(m), _ptr(self._rest), sh_degree, C.byref(c),
+                                         C.byref(st), rgb)
+        else:
+            fn = self.L.ref_raster_forward if self.use_ref else self.L.orc_raster_forward
+            fn(self.h, C.byref(m), C.byref(c), C.byref(st), rgb)
         self.n, self.cam, self.st = g.n, c, st
         return rgb
+
+    def backward_sh(self, g: Gaussians, dl: np.ndarray, sh_rest, sh_degree, grads=None, ndc=None):
+        """Backward of an SH forward; returns (grads, grad_sh_rest)."""
+        if grads is None:
+            grads = Gaussians.zeros(g.n)
+        rest = np.ascontiguousarray(sh_rest, np.float32)
+        grest = np.zeros(g.n * 45, np.float32)
+        m, gm = g.struct(), grads.struct()
+        self.L.orc_raster_backward_sh(self.h, C.byref(m), _ptr(rest), sh_degree,
+                                      np.ascontiguousarray(dl, np.float32), C.byref(gm), _ptr(grest),
+                                      _ptr(ndc))
+        return grads, grest
 
     def num_pairs(self) -> int:
         return int((self.L.ref_raster_num_pairs if self.use_ref else self.L.orc_raster_num_pairs)(self.h))
@@ -308,6 +334,21 @@ def render_f64(g: dict, cam: dict, st: Settings) -> np.ndarray:
     arr = [np.ascontiguousarray(g[f], np.float64) for f, _ in FIELDS]
     lib().orc_render_f64(len(arr[3]), *arr, C.byref(c), C.byref(st), rgb)
     return rgb
+
+
+def render_f64_sh(g: dict, rest, degree, cam: dict, st: Settings) -> np.ndarray:
+    c = make_camera(cam)
+    rgb = np.zeros(c.width * c.height * 3, np.float64)
+    arr = [np.ascontiguousarray(g[f], np.float64) for f, _ in FIELDS]
+    lib().orc_render_f64_sh(len(arr[3]), *arr, np.ascontiguousarray(rest, np.float64), degree,
+                            C.byref(c), C.byref(st), rgb)
+    return rgb
+
+
+def sh_basis(x, y, z, degree=3) -> np.ndarray:
+    out = np.zeros(15, np.float64)
+    lib().orc_sh_basis(x, y, z, degree, out)
+    return out
 
 
 def project(g: Gaussians, cam: dict, st: Settings):

@@ -158,11 +158,87 @@ template <typename T> struct Screen {
     }
 };
 
-// Rasterizer.cpp:137-204 for one Gaussian.
+// ---- Spherical harmonics of degree 1..3 (build-side extension; the reference trains the DC
+// term only, GaussianModel.h:21). Real SH basis of a unit direction with the standard
+// normalisation constants sqrt((2l+1)/(4 pi) (l-|m|)!/(l+|m|)!) (signs as in the common 3DGS
+// convention), and its partial derivatives, written out from the polynomials.
+constexpr double kShC1 = 0.4886025119029199;
+constexpr double kShC2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                             -1.0925484305920792, 0.5462742152960396};
+constexpr double kShC3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658,
+                             0.3731763325901154, -0.4570457994644658, 1.445305721320277,
+                             -0.5900435899266435};
+
+inline int shCount(int degree) { return degree <= 0 ? 0 : (degree == 1 ? 3 : (degree == 2 ? 8 : 15)); }
+
+template <typename T>
+void shBasis(T x, T y, T z, int degree, T b[15], T gx[15], T gy[15], T gz[15])
+{
+    const T c1 = T(float(kShC1));
+    const T xx = x * x, yy = y * y, zz = z * z;
+    if (degree >= 1) {
+        b[0] = -c1 * y; b[1] = c1 * z; b[2] = -c1 * x;
+        gx[0] = T(0); gy[0] = -c1; gz[0] = T(0);
+        gx[1] = T(0); gy[1] = T(0); gz[1] = c1;
+        gx[2] = -c1; gy[2] = T(0); gz[2] = T(0);
+    }
+    if (degree >= 2) {
+        const T k0 = T(float(kShC2[0])), k1 = T(float(kShC2[1])), k2 = T(float(kShC2[2])),
+                k3 = T(float(kShC2[3])), k4 = T(float(kShC2[4]));
+        b[3] = k0 * (x * y);
+        b[4] = k1 * (y * z);
+        b[5] = k2 * (T(2) * zz - xx - yy);
+        b[6] = k3 * (x * z);
+        b[7] = k4 * (xx - yy);
+        gx[3] = k0 * y; gy[3] = k0 * x; gz[3] = T(0);
+        gx[4] = T(0); gy[4] = k1 * z; gz[4] = k1 * y;
+        gx[5] = k2 * (T(-2) * x); gy[5] = k2 * (T(-2) * y); gz[5] = k2 * (T(4) * z);
+        gx[6] = k3 * z; gy[6] = T(0); gz[6] = k3 * x;
+        gx[7] = k4 * (T(2) * x); gy[7] = k4 * (T(-2) * y); gz[7] = T(0);
+    }
+    if (degree >= 3) {
+        T k[7];
+        for (int j = 0; j < 7; ++j)
+            k[j] = T(float(kShC3[j]));
+        b[8] = k[0] * (y * (T(3) * xx - yy));
+        b[9] = k[1] * (x * y * z);
+        b[10] = k[2] * (y * (T(4) * zz - xx - yy));
+        b[11] = k[3] * (z * (T(2) * zz - T(3) * xx - T(3) * yy));
+        b[12] = k[4] * (x * (T(4) * zz - xx - yy));
+        b[13] = k[5] * (z * (xx - yy));
+        b[14] = k[6] * (x * (xx - T(3) * yy));
+        gx[8] = k[0] * (T(6) * x * y); gy[8] = k[0] * (T(3) * xx - T(3) * yy); gz[8] = T(0);
+        gx[9] = k[1] * (y * z); gy[9] = k[1] * (x * z); gz[9] = k[1] * (x * y);
+        gx[10] = k[2] * (T(-2) * x * y); gy[10] = k[2] * (T(4) * zz - xx - T(3) * yy); gz[10] = k[2] * (T(8) * y * z);
+        gx[11] = k[3] * (T(-6) * x * z); gy[11] = k[3] * (T(-6) * y * z); gz[11] = k[3] * (T(6) * zz - T(3) * xx - T(3) * yy);
+        gx[12] = k[4] * (T(4) * zz - T(3) * xx - yy); gy[12] = k[4] * (T(-2) * x * y); gz[12] = k[4] * (T(8) * x * z);
+        gx[13] = k[5] * (T(2) * x * z); gy[13] = k[5] * (T(-2) * y * z); gz[13] = k[5] * (xx - yy);
+        gx[14] = k[6] * (T(3) * xx - T(3) * yy); gy[14] = k[6] * (T(-6) * x * y); gz[14] = T(0);
+    }
+}
+
+// Unit view direction from the camera centre to the mean, and the length before normalising.
+template <typename T> void viewDir(const T *mu, const T campos[3], T d[3], T *len)
+{
+    const T vx = mu[0] - campos[0], vy = mu[1] - campos[1], vz = mu[2] - campos[2];
+    *len = std::sqrt(vx * vx + vy * vy + vz * vz);
+    d[0] = vx / *len;
+    d[1] = vy / *len;
+    d[2] = vz / *len;
+}
+
+template <typename T> void cameraCentre(const Cam<T> &cam, T c[3])
+{
+    for (int k = 0; k < 3; ++k)
+        c[k] = -(cam.R[k] * cam.t[0] + cam.R[3 + k] * cam.t[1] + cam.R[6 + k] * cam.t[2]);
+}
+
+// Rasterizer.cpp:137-204 for one Gaussian; `rest` (15 x 3 per Gaussian, coefficient-major) and
+// `degree` add the view-dependent SH terms (degree 0 is the reference's colour exactly).
 template <typename T>
 void projectScreen(size_t i, const T *means, const T *scales, const T *quats, const T *opac,
                    const T *colors, const Cam<T> &cam, const orc_settings &st, int tilesX,
-                   int tilesY, Screen<T> &s)
+                   int tilesY, Screen<T> &s, const T *rest = nullptr, int degree = 0)
 {
     Proj<T> p;
     if (!projectOne(&means[i * 3], &quats[i * 4], &scales[i * 3], cam, T(st.near_plane),
@@ -198,8 +274,24 @@ void projectScreen(size_t i, const T *means, const T *scales, const T *quats, co
     s.conic[i * 3 + 2] = p.a * invDet;
     s.depth[i] = z;
     uint8_t flags = uint8_t((p.clampX ? 1 : 0) | (p.clampY ? 2 : 0));
+    const int nb = rest ? shCount(degree) : 0;
+    T basis[15], gx[15], gy[15], gz[15];
+    if (nb) {
+        T campos[3], d[3], len;
+        cameraCentre(cam, campos);
+        viewDir(&means[i * 3], campos, d, &len);
+        shBasis(d[0], d[1], d[2], degree, basis, gx, gy, gz);
+    }
     for (int ch = 0; ch < 3; ++ch) {
-        const T col = T(0.5f) + T(kShC0) * colors[i * 3 + ch];
+        T col;
+        if (nb == 0) {
+            col = T(0.5f) + T(kShC0) * colors[i * 3 + ch];
+        } else {
+            T sum = T(kShC0) * colors[i * 3 + ch];
+            for (int k = 0; k < nb; ++k)
+                sum += basis[k] * rest[i * 45 + k * 3 + ch];
+            col = T(0.5f) + sum;
+        }
         if (col < T(0))
             flags |= uint8_t(4 << ch);
         s.rgb[i * 3 + ch] = std::max(col, T(0));
@@ -421,8 +513,8 @@ orc_raster *orc_raster_new(void) { return new orc_raster(); }
 void orc_raster_free(orc_raster *r) { delete r; }
 
 // Rasterizer::forward, Rasterizer.cpp:263-330.
-void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *camera,
-                        const orc_settings *st, float *rgb)
+void orc_raster_forward_sh(orc_raster *r, const orc_model *m, const float *rest, int degree,
+                           const orc_camera *camera, const orc_settings *st, float *rgb)
 {
     r->cam = *camera;
     r->st = *st;
@@ -434,7 +526,7 @@ void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *cam
     r->s.resize(r->n);
     for (size_t i = 0; i < r->n; ++i)
         projectScreen<float>(i, m->means, m->scales, m->quats, m->opacities, m->colors, cam, *st,
-                             r->tilesX, r->tilesY, r->s);
+                             r->tilesX, r->tilesY, r->s, rest, degree);
     binTiles<float>(r->n, r->s.radius, r->s.rect, r->s.depth, r->tilesX, r->tilesY, r->offsets,
                     r->list);
     r->finalT.assign(size_t(W) * size_t(H), 0.0f);
@@ -444,10 +536,16 @@ void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *cam
                           r->finalT.data(), r->nContrib.data());
 }
 
+void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *camera,
+                        const orc_settings *st, float *rgb)
+{
+    orc_raster_forward_sh(r, m, nullptr, 0, camera, st, rgb);
+}
+
 // Rasterizer::backward, Rasterizer.cpp:334-545, with the per-tile merge done serially in tile
 // order (the reference's order when it runs on one thread).
-void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl, orc_model *grads,
-                         float *ndc)
+void orc_raster_backward_sh(orc_raster *r, const orc_model *m, const float *rest, int degree,
+                            const float *dl, orc_model *grads, float *grad_rest, float *ndc)
 {
     const size_t n = r->n;
     const Screen<float> &s = r->s;
@@ -533,6 +631,28 @@ void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl, orc
         for (int ch = 0; ch < 3; ++ch)
             if (!(s.flags[i] & (4 << ch)))
                 grads->colors[i * 3 + ch] += kShC0 * g[5 + ch];
+        float dDir[3] = {0, 0, 0}, viewD[3] = {0, 0, 0}, viewLen = 1;
+        const int nb = rest ? shCount(degree) : 0;
+        if (nb) {
+            // View-dependent colour: coefficient gradients and the direction -> mean term.
+            float campos[3], b[15], bx[15], by[15], bz[15];
+            cameraCentre(cam, campos);
+            viewDir(&m->means[i * 3], campos, viewD, &viewLen);
+            shBasis(viewD[0], viewD[1], viewD[2], degree, b, bx, by, bz);
+            for (int ch = 0; ch < 3; ++ch) {
+                if (s.flags[i] & (4 << ch))
+                    continue;
+                const float gc = g[5 + ch];
+                for (int k = 0; k < nb; ++k) {
+                    const float coef = rest[i * 45 + k * 3 + ch];
+                    if (grad_rest)
+                        grad_rest[i * 45 + k * 3 + ch] += b[k] * gc;
+                    dDir[0] += coef * bx[k] * gc;
+                    dDir[1] += coef * by[k] * gc;
+                    dDir[2] += coef * bz[k] * gc;
+                }
+            }
+        }
         const float op = s.alpha[i];
         grads->opacities[i] += g[8] * op * (1.0f - op);
 
@@ -612,7 +732,18 @@ void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl, orc
         dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
         for (int k = 0; k < 3; ++k)
             grads->means[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
+        if (nb) {
+            const float dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
+            for (int k = 0; k < 3; ++k)
+                grads->means[i * 3 + k] += (dDir[k] - dot * viewD[k]) / viewLen;
+        }
     }
+}
+
+void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl, orc_model *grads,
+                         float *ndc)
+{
+    orc_raster_backward_sh(r, m, nullptr, 0, dl, grads, nullptr, ndc);
 }
 
 long orc_raster_num_pairs(const orc_raster *r) { return long(r->list.size()); }
@@ -651,9 +782,9 @@ void orc_raster_get_grad2d(const orc_raster *r, float *grad2d)
     std::copy(r->grad2d.begin(), r->grad2d.end(), grad2d);
 }
 
-void orc_render_f64(int n, const double *means, const double *scales, const double *quats,
-                    const double *opacities, const double *colors, const orc_camera *camera,
-                    const orc_settings *st, double *rgb)
+void orc_render_f64_sh(int n, const double *means, const double *scales, const double *quats,
+                       const double *opacities, const double *colors, const double *rest,
+                       int degree, const orc_camera *camera, const orc_settings *st, double *rgb)
 {
     const Cam<double> cam = camOf<double>(*camera);
     const int ts = st->tile_size, W = cam.width, H = cam.height;
@@ -661,11 +792,43 @@ void orc_render_f64(int n, const double *means, const double *scales, const doub
     Screen<double> s;
     s.resize(size_t(n));
     for (size_t i = 0; i < size_t(n); ++i)
-        projectScreen<double>(i, means, scales, quats, opacities, colors, cam, *st, tilesX, tilesY, s);
+        projectScreen<double>(i, means, scales, quats, opacities, colors, cam, *st, tilesX, tilesY, s,
+                              rest, degree);
     std::vector<int> off, list;
     binTiles<double>(size_t(n), s.radius, s.rect, s.depth, tilesX, tilesY, off, list);
     for (int tile = 0; tile < tilesX * tilesY; ++tile)
         rasterTile<double>(tile, tilesX, W, H, *st, s, off, list, rgb, nullptr, nullptr);
+}
+
+void orc_render_f64(int n, const double *means, const double *scales, const double *quats,
+                    const double *opacities, const double *colors, const orc_camera *camera,
+                    const orc_settings *st, double *rgb)
+{
+    orc_render_f64_sh(n, means, scales, quats, opacities, colors, nullptr, 0, camera, st, rgb);
+}
+
+// SH basis values (degree 1..3) of a unit direction, for the orthonormality check.
+void orc_sh_basis(float x, float y, float z, int degree, double *out15)
+{
+    double b[15] = {}, gx[15], gy[15], gz[15];
+    shBasis<double>(x, y, z, degree, b, gx, gy, gz);
+    for (int k = 0; k < 15; ++k)
+        out15[k] = b[k];
+}
+
+// One Adam step over a single array (used for the SH coefficient group).
+void orc_adam_array(float *p, const float *g, float *m, float *v, long len, float lr, int step)
+{
+    const float bc1 = 1.0f - std::pow(0.9f, float(step));
+    const float bc2 = 1.0f - std::pow(0.999f, float(step));
+    const float stepSize = lr / bc1;
+    const float invSqrtBc2 = 1.0f / std::sqrt(bc2);
+    for (long k = 0; k < len; ++k) {
+        const float gk = g[k];
+        m[k] = 0.9f * m[k] + (1.0f - 0.9f) * gk;
+        v[k] = 0.999f * v[k] + (1.0f - 0.999f) * gk * gk;
+        p[k] -= stepSize * m[k] / (std::sqrt(v[k]) * invSqrtBc2 + 1e-15f);
+    }
 }
 
 // photometricLoss, Loss.cpp:47-185: four separable passes over planar scratch.

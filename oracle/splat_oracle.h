@@ -84,6 +84,13 @@ void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *cam
 // Adds into grads (same n) and, if non-null, into mean2d_grad_ndc[n].
 void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl_drgb,
                          orc_model *grads, float *mean2d_grad_ndc);
+// SH degree 1..3 variants (extension): rest = 15 x 3 coefficients per Gaussian, coefficient-
+// major; grad_rest (may be NULL) is added to.
+void orc_raster_forward_sh(orc_raster *r, const orc_model *m, const float *rest, int degree,
+                           const orc_camera *cam, const orc_settings *st, float *rgb);
+void orc_raster_backward_sh(orc_raster *r, const orc_model *m, const float *rest, int degree,
+                            const float *dl_drgb, orc_model *grads, float *grad_rest,
+                            float *mean2d_grad_ndc);
 // Accessors of the state kept by the last forward.
 long orc_raster_num_pairs(const orc_raster *r);
 int orc_raster_num_tiles(const orc_raster *r);
@@ -97,6 +104,12 @@ void orc_raster_get_grad2d(const orc_raster *r, float *grad2d); // [9n] after ba
 void orc_render_f64(int n, const double *means, const double *scales, const double *quats,
                     const double *opacities, const double *colors, const orc_camera *cam,
                     const orc_settings *st, double *rgb);
+
+void orc_render_f64_sh(int n, const double *means, const double *scales, const double *quats,
+                       const double *opacities, const double *colors, const double *rest,
+                       int degree, const orc_camera *cam, const orc_settings *st, double *rgb);
+void orc_adam_array(float *p, const float *g, float *m, float *v, long len, float lr, int step);
+void orc_sh_basis(float x, float y, float z, int degree, double *out15);
 
 // ---- loss (Loss.cpp:47-185) and psnr (Loss.cpp:187-196)
 orc_loss_value orc_photometric_loss(const float *render, const float *target, int w, int h,

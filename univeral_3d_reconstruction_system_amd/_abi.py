@@ -48,7 +48,10 @@ class TrainOptions(C.Structure):
                 ("densify_grad_threshold", C.c_float), ("percent_dense", C.c_float),
                 ("refine_start", C.c_int), ("refine_stop", C.c_int), ("refine_interval", C.c_int),
                 ("opacity_reset_interval", C.c_int), ("prune_opacity", C.c_float),
-                ("seed", C.c_uint), ("raster", RasterSettings)]
+                ("seed", C.c_uint), ("raster", RasterSettings),
+                # extensions: SH degree, SH rate scale, views per step, hipGraph capture
+                ("sh_degree", C.c_int), ("sh_lr_scale", C.c_float), ("views_per_step", C.c_int),
+                ("use_graph", C.c_int)]
 
 
 class TrainStats(C.Structure):
@@ -87,6 +90,10 @@ SIGNATURES = {
     "u3d_raster_destroy": (None, [P]),
     "u3d_raster_forward": (I, [P, C.POINTER(Gaussians), C.POINTER(Camera), C.POINTER(RasterSettings), P]),
     "u3d_raster_backward": (I, [P, C.POINTER(Gaussians), P, C.POINTER(Gaussians), P]),
+    "u3d_raster_forward_sh": (I, [P, C.POINTER(Gaussians), P, I, C.POINTER(Camera),
+                                  C.POINTER(RasterSettings), P]),
+    "u3d_raster_backward_sh": (I, [P, C.POINTER(Gaussians), P, I, P, C.POINTER(Gaussians), P, P]),
+    "u3d_trainer_get_sh_rest": (I, [P, P]),
     "u3d_raster_radii": (I, [P, C.POINTER(P), C.POINTER(I64)]),
     "u3d_raster_get_state": (I, [P, C.POINTER(RasterState)]),
     "u3d_raster_get_grad2d": (I, [P, C.POINTER(P)]),

@@ -135,8 +135,10 @@ namespace u3d {
 namespace {
 
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
-                      const u3d_raster_settings &st, float *rgb)
+                      const u3d_raster_settings &st, float *rgb, const float *shRest = nullptr,
+                      int shDegree = 0)
 {
+    U3D_CHECK(shDegree >= 0 && shDegree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
     U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
               "tile_size must be a positive multiple of 16");
     U3D_CHECK(cam.width >= 0 && cam.height >= 0, U3D_ERR_INVALID, "negative image size");
@@ -174,7 +176,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
 
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
-        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb));
+        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree));
     }
     std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
     // Depth order of the visible Gaussians (invisible keys sort last).
@@ -217,7 +219,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
 }
 
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
-                       u3d_gaussians &grads, float *ndc, int *gradCount)
+                       u3d_gaussians &grads, float *ndc, int *gradCount,
+                       const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr)
 {
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
@@ -233,7 +236,8 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                                      r->finalT.p, r->nContrib.p, r->grad2d.p));
     }
     StageScope scope(r->ctx, U3D_STAGE_CHAIN);
-    U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount));
+    U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount, shRest,
+                            shDegree, gShRest));
     return U3D_OK;
 }
 
@@ -276,6 +280,9 @@ struct u3d_trainer {
     int statsFolded = 0;
     u3d_train_stats stats{};
     int64_t counters[4] = {0, 0, 0, 0}; // steps, Gaussians, pixels, pairs
+    // Spherical-harmonic coefficients beyond DC (extension), [cap][15][3].
+    int shDegree = 0;
+    DevBuf<float> shP, shG, shM, shV, shSpare;
 };
 
 // ----------------------------------------------------------------------------------- C ABI
@@ -403,6 +410,30 @@ int u3d_raster_backward(u3d_raster *r, const u3d_gaussians *model, const float *
     U3D_TRY(checkModel(model, "model"));
     U3D_TRY(checkModel(grads, "grads"));
     return rasterBackwardImpl(r, *model, dl_drgb, *grads, mean2d_grad_ndc, nullptr);
+}
+
+int u3d_raster_forward_sh(u3d_raster *r, const u3d_gaussians *model, const float *sh_rest,
+                          int sh_degree, const u3d_camera *camera,
+                          const u3d_raster_settings *settings, float *rgb)
+{
+    U3D_CHECK(r && camera && settings, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(r->ctx));
+    U3D_TRY(checkModel(model, "model"));
+    U3D_CHECK(rgb != nullptr || size_t(camera->width) * size_t(camera->height) == 0, U3D_ERR_INVALID, "null rgb");
+    return rasterForwardImpl(r, *model, *camera, *settings, rgb, sh_rest, sh_rest ? sh_degree : 0);
+}
+
+int u3d_raster_backward_sh(u3d_raster *r, const u3d_gaussians *model, const float *sh_rest,
+                           int sh_degree, const float *dl_drgb, u3d_gaussians *grads,
+                           float *grad_sh_rest, float *mean2d_grad_ndc)
+{
+    U3D_CHECK(r && dl_drgb, U3D_ERR_INVALID, "null argument");
+    U3D_CHECK(sh_degree >= 0 && sh_degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    U3D_TRY(useDevice(r->ctx));
+    U3D_TRY(checkModel(model, "model"));
+    U3D_TRY(checkModel(grads, "grads"));
+    return rasterBackwardImpl(r, *model, dl_drgb, *grads, mean2d_grad_ndc, nullptr, sh_rest,
+                              sh_rest ? sh_degree : 0, grad_sh_rest);
 }
 
 int u3d_raster_radii(u3d_raster *r, const int **radii, int64_t *n)
@@ -821,6 +852,16 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_TRY(t->gradCount.ensure(size_t(cap)));
     U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(cap) * sizeof(float), s));
     U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(cap) * sizeof(int), s));
+    U3D_CHECK(options->sh_degree >= 0 && options->sh_degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    U3D_CHECK(options->views_per_step <= 1, U3D_ERR_UNSUPPORTED, "views_per_step > 1 not available");
+    U3D_CHECK(options->use_graph == 0, U3D_ERR_UNSUPPORTED, "use_graph not available");
+    t->shDegree = options->sh_degree;
+    if (t->shDegree > 0) {
+        for (DevBuf<float> *b : {&t->shP, &t->shG, &t->shM, &t->shV, &t->shSpare}) {
+            U3D_TRY(b->ensure(size_t(cap) * 45));
+            U3D_HIP(hipMemsetAsync(b->p, 0, b->cap * sizeof(float), s));
+        }
+    }
     U3D_TRY(t->stepStats.ensure(size_t(std::max(1, options->iterations)) * 4));
     U3D_TRY(t->evalOut.ensure(size_t(std::max(1, n_images)) + 4));
     size_t parts = 1024;
@@ -883,6 +924,18 @@ int trainerDensify(u3d_trainer *t)
     U3D_TRY(densifyApplyImpl(ctx, ds, pv, sv, keep.p, kind.p, dn.p, logShrink));
     t->params.swap(t->spare);
     t->params.n = newN;
+    if (t->shDegree > 0) {
+        U3D_TRY(launchScatterRows(s, t->shP.p, t->shSpare.p, 45, n, keep.p, kind.p, ds.keepPos.p,
+                                  ds.extraPos.p, ds.counts.p, false));
+        std::swap(t->shP.p, t->shSpare.p);
+        std::swap(t->shP.cap, t->shSpare.cap);
+        for (DevBuf<float> *mom : {&t->shM, &t->shV}) {
+            U3D_TRY(launchScatterRows(s, mom->p, t->shSpare.p, 45, n, keep.p, kind.p, ds.keepPos.p,
+                                      ds.extraPos.p, ds.counts.p, true));
+            std::swap(mom->p, t->shSpare.p);
+            std::swap(mom->cap, t->shSpare.cap);
+        }
+    }
     // Adam moments: compacted, zeros appended (Adam.cpp:40-60).
     for (DevModel *mm : {&t->adamM, &t->adamV}) {
         t->spare.n = newN;
@@ -940,7 +993,7 @@ int u3d_trainer_step(u3d_trainer *t)
     const u3d_camera &cam = t->cams[view];
     const hipStream_t s = t->ctx->stream;
     const u3d_gaussians pv = t->params.view();
-    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p));
+    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree));
     t->counters[0] += 1;
     t->counters[1] += t->params.n;
     t->counters[2] += int64_t(cam.width) * cam.height;
@@ -956,7 +1009,10 @@ int u3d_trainer_step(u3d_trainer *t)
     if (t->grads.n > 0)
         U3D_HIP(hipMemsetAsync(t->grads.buf.p, 0, size_t(t->grads.cap) * 14 * sizeof(float), s));
     u3d_gaussians gv = t->grads.view();
-    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p));
+    if (t->shDegree > 0 && t->params.n > 0)
+        U3D_HIP(hipMemsetAsync(t->shG.p, 0, size_t(t->params.n) * 45 * sizeof(float), s));
+    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
+                               t->shP.p, t->shDegree, t->shG.p));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
     const float tt = float(t->iteration) / float(std::max(1, t->opt.iterations - 1));
@@ -968,6 +1024,13 @@ int u3d_trainer_step(u3d_trainer *t)
     {
         StageScope scope(t->ctx, U3D_STAGE_ADAM);
         U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv));
+        if (t->shDegree > 0) {
+            // SH coefficients: the colour rate scaled down (3DGS uses 1/20), same moments rule.
+            const float scale = t->opt.sh_lr_scale > 0.0f ? t->opt.sh_lr_scale : 1.0f / 20.0f;
+            const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
+            U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,
+                                    (lr.colors * scale) / bc1, inv));
+        }
     }
     ++t->iteration;
     const u3d_train_options &o = t->opt;
@@ -995,6 +1058,20 @@ int u3d_trainer_get_model(u3d_trainer *t, float *means, float *scales, float *qu
     const u3d_gaussians host{means, scales, quats, opacities, colors, n};
     U3D_TRY(copyFields(t->ctx->stream, t->params.view(), host, n, hipMemcpyDeviceToHost));
     U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    return U3D_OK;
+}
+
+int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
+    U3D_TRY(useDevice(t->ctx));
+    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
+    if (bytes) {
+        U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null output");
+        U3D_HIP(hipMemcpyAsync(sh_rest_host, t->shP.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
     return U3D_OK;
 }
 
@@ -1035,7 +1112,7 @@ int evaluateAll(u3d_trainer *t, int mode, float *out)
     const u3d_gaussians pv = t->params.view();
     for (size_t k = 0; k < t->cams.size(); ++k) {
         const u3d_camera &cam = t->cams[k];
-        U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p));
+        U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree));
         const int64_t count = int64_t(cam.width) * cam.height * 3;
         if (mode == 0)
             U3D_TRY(launchPsnr(s, t->render.p, nullptr, t->targets.p + t->imageOffset[k], count,
@@ -1074,7 +1151,8 @@ int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host
     const size_t count = size_t(camera->width) * size_t(camera->height) * 3;
     DevBuf<float> out;
     U3D_TRY(out.ensure(count > 0 ? count : 1));
-    U3D_TRY(rasterForwardImpl(&t->raster, t->params.view(), *camera, t->opt.raster, out.p));
+    U3D_TRY(rasterForwardImpl(&t->raster, t->params.view(), *camera, t->opt.raster, out.p, t->shP.p,
+                              t->shDegree));
     U3D_HIP(hipMemcpyAsync(rgb_host, out.p, count * sizeof(float), hipMemcpyDeviceToHost, t->ctx->stream));
     U3D_HIP(hipStreamSynchronize(t->ctx->stream));
     return U3D_OK;

@@ -20,13 +20,16 @@ struct ScreenBuffers {
 };
 
 // ---- kernels_project.hip
+// shRest (nullable): 15 x 3 SH coefficients per Gaussian used up to shDegree (1..3).
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
-                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out);
+                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
+                  const float *shRest, int shDegree);
 // Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
 // grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
-                    u3d_gaussians &grads, float *ndc, int *gradCount);
+                    u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
+                    int shDegree, float *gShRest);
 // Unpacks the records into the separate views of u3d_raster_state.
 int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float *mean2d,
                        float *conic, float *color, float *opacity);
@@ -95,6 +98,15 @@ int launchMeanAbs(hipStream_t s, const float *a, const uint8_t *bU8, int64_t cou
 // ---- kernels_adam.hip
 int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_gaussians &m,
                    u3d_gaussians &v, const float stepSize[5], float invSqrtBc2);
+// Adam over one extra parameter array (SH coefficients).
+int launchAdamArray(hipStream_t s, float *p, const float *g, float *m, float *v, int64_t len,
+                    float stepSize, float invSqrtBc2);
+// Densification of a row array of `width` floats per Gaussian: kept rows compacted, extras
+// copied from their parent (or zeroed when zeroRows, for optimiser moments). Needs the
+// scans and counts of the same densification.
+int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, int64_t n,
+                      const uint8_t *keep, const uint8_t *kind, const uint32_t *keepPos,
+                      const uint32_t *extraPos, const uint32_t *counts, bool zeroRows);
 int launchFill(hipStream_t s, float *p, float value, int64_t count);
 int launchFillI32(hipStream_t s, int *p, int value, int64_t count);
 // o = min(o, cap) (Trainer.cpp:171-178)

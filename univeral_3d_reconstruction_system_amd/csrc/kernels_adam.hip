@@ -243,9 +243,78 @@ __global__ __launch_bounds__(kThreads) void compactZeroTailKernel(
     }
 }
 
+// Adam over one extra array (the SH coefficients), same arithmetic as adamKernel.
+__global__ __launch_bounds__(kThreads) void adamArrayKernel(float *p, const float *__restrict__ g,
+                                                            float *m, float *v, int64_t len,
+                                                            float stepSize, float invSqrtBc2)
+{
+    for (int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x; k < len;
+         k += int64_t(gridDim.x) * kThreads) {
+        const float gk = g[k];
+        const float mk = 0.9f * m[k] + (1.0f - 0.9f) * gk;
+        const float vk = 0.999f * v[k] + (1.0f - 0.999f) * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        p[k] = p[k] - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
+    }
+}
+
+// Densification of a per-Gaussian row array of `width` floats (e.g. SH coefficients): kept
+// rows to keepPos, clones and both split children copy the parent row (zeroRows: write zeros
+// for the appended rows instead, as for optimiser moments).
+__global__ __launch_bounds__(kThreads) void scatterRowsKernel(
+    const float *__restrict__ src, float *__restrict__ dst, int width, int64_t n,
+    const uint8_t *__restrict__ keep, const uint8_t *__restrict__ kind,
+    const uint32_t *__restrict__ keepPos, const uint32_t *__restrict__ extraPos,
+    const uint32_t *__restrict__ counts, int zeroRows)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    const int64_t nKeep = counts[2];
+    const float *row = src + i * width;
+    if (keep[i]) {
+        float *o = dst + int64_t(keepPos[i]) * width;
+        for (int k = 0; k < width; ++k)
+            o[k] = row[k];
+    }
+    const int copies = kind[i] == 1 ? 1 : (kind[i] == 2 ? 2 : 0);
+    for (int c = 0; c < copies; ++c) {
+        float *o = dst + (nKeep + int64_t(extraPos[i]) + c) * width;
+        for (int k = 0; k < width; ++k)
+            o[k] = zeroRows ? 0.0f : row[k];
+    }
+}
+
 inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
 } // namespace
+
+int launchAdamArray(hipStream_t s, float *p, const float *g, float *m, float *v, int64_t len,
+                    float stepSize, float invSqrtBc2)
+{
+    if (len <= 0)
+        return U3D_OK;
+    unsigned blocks = blocksFor(len);
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(adamArrayKernel, dim3(blocks), dim3(kThreads), 0, s, p, g, m, v, len,
+                       stepSize, invSqrtBc2);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, int64_t n,
+                      const uint8_t *keep, const uint8_t *kind, const uint32_t *keepPos,
+                      const uint32_t *extraPos, const uint32_t *counts, bool zeroRows)
+{
+    if (n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(scatterRowsKernel, dim3(blocksFor(n)), dim3(kThreads), 0, s, src, dst, width,
+                       n, keep, kind, keepPos, extraPos, counts, zeroRows ? 1 : 0);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
 
 int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_gaussians &m,
                    u3d_gaussians &v, const float stepSize[5], float invSqrtBc2)

@@ -14,7 +14,8 @@ __global__ __launch_bounds__(kThreads) void projectKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
-    int tilesX, int tilesY, ScreenBuffers out)
+    int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
+    float3 campos)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n)
@@ -56,9 +57,24 @@ __global__ __launch_bounds__(kThreads) void projectKernel(
             break;
         flags = uint8_t((p.clampX ? 1 : 0) | (p.clampY ? 2 : 0));
         float rgb[3];
+        const int nb = shRest ? shCount(shDegree) : 0;
+        float basis[15];
+        if (nb) {
+            float d[3], len;
+            viewDir(mu, campos, d, &len);
+            shBasis(d[0], d[1], d[2], shDegree, basis, nullptr);
+        }
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const float col = 0.5f + kShC0 * colors[i * 3 + ch];
+            float col;
+            if (nb == 0) {
+                col = 0.5f + kShC0 * colors[i * 3 + ch];
+            } else {
+                float sum = kShC0 * colors[i * 3 + ch];
+                for (int k = 0; k < nb; ++k)
+                    sum += basis[k] * shRest[i * 45 + k * 3 + ch];
+                col = 0.5f + sum;
+            }
             if (col < 0.0f)
                 flags |= uint8_t(4 << ch);
             rgb[ch] = (col < 0.0f) ? 0.0f : col; // std::max(col, 0.0f)
@@ -84,7 +100,8 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
     const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
-    float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount)
+    float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
+    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n || radius[i] == 0)
@@ -109,6 +126,28 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     gOpac[i] += g[8] * op * (1.0f - op);
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
+    // View-dependent colour: coefficient gradients and the direction -> mean term.
+    const int nb = shRest ? shCount(shDegree) : 0;
+    float dDir[3] = {0.0f, 0.0f, 0.0f}, viewD[3] = {0.0f, 0.0f, 0.0f}, viewLen = 1.0f;
+    if (nb) {
+        float b[15];
+        float3 gb[15];
+        viewDir(mu, campos, viewD, &viewLen);
+        shBasis(viewD[0], viewD[1], viewD[2], shDegree, b, gb);
+        for (int ch = 0; ch < 3; ++ch) {
+            if (flags & (4 << ch))
+                continue;
+            const float gc = g[5 + ch];
+            for (int k = 0; k < nb; ++k) {
+                const float coef = shRest[i * 45 + k * 3 + ch];
+                if (gShRest)
+                    gShRest[i * 45 + k * 3 + ch] += b[k] * gc;
+                dDir[0] += coef * gb[k].x * gc;
+                dDir[1] += coef * gb[k].y * gc;
+                dDir[2] += coef * gb[k].z * gc;
+            }
+        }
+    }
     const float qr[4] = {quats[i * 4], quats[i * 4 + 1], quats[i * 4 + 2], quats[i * 4 + 3]};
     const float ls[3] = {scales[i * 3], scales[i * 3 + 1], scales[i * 3 + 2]};
     Projection p;
@@ -197,6 +236,12 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         gMeans[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
+    if (nb) {
+        const float dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            gMeans[i * 3 + k] += (dDir[k] - dot * viewD[k]) / viewLen;
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void unpackKernel(int64_t n, ScreenBuffers sb,
@@ -223,27 +268,30 @@ inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThr
 } // namespace
 
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
-                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out)
+                  const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
+                  const float *shRest, int shDegree)
 {
     if (m.n <= 0)
         return U3D_OK;
     hipLaunchKernelGGL(projectKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
                        m.scales, m.quats, m.opacities, m.colors, cam, st.near_plane, st.eps2d,
-                       st.tile_size, tilesX, tilesY, out);
+                       st.tile_size, tilesX, tilesY, out, shRest, shDegree, cameraCentre(cam));
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
 
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
-                    u3d_gaussians &grads, float *ndc, int *gradCount)
+                    u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
+                    int shDegree, float *gShRest)
 {
     if (m.n <= 0)
         return U3D_OK;
     hipLaunchKernelGGL(chainRuleKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
                        m.means, m.scales, m.quats, cam, st.near_plane, st.eps2d, sb.radius,
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
-                       grads.opacities, grads.colors, ndc, gradCount);
+                       grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
+                       cameraCentre(cam), gShRest);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

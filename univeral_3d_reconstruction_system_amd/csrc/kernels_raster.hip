@@ -20,10 +20,10 @@
 // pixel has saturated.
 //
 // Backward: reverse walk from each wave's largest contributor count; per Gaussian the nine 2D
-// gradient terms are summed over each 16-lane row with DPP row shifts (only when a lane
-// contributes), the contributing rows of the four waves add their totals into an LDS batch
-// accumulator, and each batch is flushed with one global float atomic per non-zero term per
-// (block, Gaussian).
+// gradient terms of the wave's 64 pixels are summed with DPP (only when a lane contributes;
+// a 16-lane-row variant feeding LDS atomics measured slower because of atomic conflicts), the
+// four waves combine in an LDS batch accumulator, and each batch is flushed with one global
+// float atomic per non-zero term per (block, Gaussian).
 #include "kernels.h"
 
 namespace u3d {
@@ -39,14 +39,17 @@ template <int CTRL, int ROW_MASK> __device__ __forceinline__ float dppMove(float
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false));
 }
 
-// Inclusive sum within each 16-lane row; lane 15 of a row holds the row total.
-__device__ __forceinline__ float rowSum(float x)
+// Sum over the 64 lanes of the wave (every lane must be active): inclusive row scans with DPP
+// row shifts, then row broadcasts; lane 63 holds the total.
+__device__ __forceinline__ float waveSum(float x)
 {
     x += dppMove<0x111, 0xf>(x); // row_shr:1
     x += dppMove<0x112, 0xf>(x); // row_shr:2
     x += dppMove<0x114, 0xf>(x); // row_shr:4
     x += dppMove<0x118, 0xf>(x); // row_shr:8
-    return x;
+    x += dppMove<0x142, 0xa>(x); // row_bcast:15 into rows 1, 3
+    x += dppMove<0x143, 0xc>(x); // row_bcast:31 into rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 // log(min_alpha / alpha) - 1e-3: below it, min(0.99, alpha * exp(power)) < min_alpha holds for
@@ -308,20 +311,17 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
                         }
                     }
                 }
-                const uint64_t live = __ballot(contrib);
-                if (live) {
-                    // Sum inside each 16-lane row with DPP, then the rows that hold a
-                    // contribution add their totals to the batch accumulator in LDS.
-                    v0 = rowSum(v0);
-                    v1 = rowSum(v1);
-                    v2 = rowSum(v2);
-                    v3 = rowSum(v3);
-                    v4 = rowSum(v4);
-                    v5 = rowSum(v5);
-                    v6 = rowSum(v6);
-                    v7 = rowSum(v7);
-                    v8 = rowSum(v8);
-                    if ((lane & 15) == 15 && ((live >> (lane & 48)) & 0xffffull)) {
+                if (__ballot(contrib)) {
+                    v0 = waveSum(v0);
+                    v1 = waveSum(v1);
+                    v2 = waveSum(v2);
+                    v3 = waveSum(v3);
+                    v4 = waveSum(v4);
+                    v5 = waveSum(v5);
+                    v6 = waveSum(v6);
+                    v7 = waveSum(v7);
+                    v8 = waveSum(v8);
+                    if (lane == 0) {
                         float *dst = acc[jj];
                         atomicAdd(&dst[0], v0);
                         atomicAdd(&dst[1], v1);

@@ -109,4 +109,94 @@ __device__ __forceinline__ bool projectGaussian(const float mu[3], const float q
     return true;
 }
 
+// ------------------------------------------------------------------ spherical harmonics
+// View-dependent colour of degree 1..3 (an extension: the reference trains the DC term only,
+// GaussianModel.h:21). Real SH basis of the unit direction from the camera centre to the mean,
+// with the standard normalisation constants; coefficients are stored 15 x 3 per Gaussian,
+// coefficient-major. Degree 0 keeps the reference's colour expression exactly.
+constexpr float kShC1 = 0.4886025119029199f;
+constexpr float kShC2_0 = 1.0925484305920792f, kShC2_1 = -1.0925484305920792f,
+                kShC2_2 = 0.31539156525252005f, kShC2_3 = -1.0925484305920792f,
+                kShC2_4 = 0.5462742152960396f;
+constexpr float kShC3_0 = -0.5900435899266435f, kShC3_1 = 2.890611442640554f,
+                kShC3_2 = -0.4570457994644658f, kShC3_3 = 0.3731763325901154f,
+                kShC3_4 = -0.4570457994644658f, kShC3_5 = 1.445305721320277f,
+                kShC3_6 = -0.5900435899266435f;
+
+__host__ __device__ inline int shCount(int degree)
+{
+    return degree <= 0 ? 0 : (degree == 1 ? 3 : (degree == 2 ? 8 : 15));
+}
+
+// Basis values b[k] and, when g != nullptr, their partial derivatives g[k] = (d/dx, d/dy, d/dz).
+__device__ __forceinline__ void shBasis(float x, float y, float z, int degree, float b[15],
+                                        float3 *g)
+{
+    const float xx = x * x, yy = y * y, zz = z * z;
+    if (degree >= 1) {
+        b[0] = -kShC1 * y;
+        b[1] = kShC1 * z;
+        b[2] = -kShC1 * x;
+        if (g) {
+            g[0] = make_float3(0.0f, -kShC1, 0.0f);
+            g[1] = make_float3(0.0f, 0.0f, kShC1);
+            g[2] = make_float3(-kShC1, 0.0f, 0.0f);
+        }
+    }
+    if (degree >= 2) {
+        b[3] = kShC2_0 * (x * y);
+        b[4] = kShC2_1 * (y * z);
+        b[5] = kShC2_2 * (2.0f * zz - xx - yy);
+        b[6] = kShC2_3 * (x * z);
+        b[7] = kShC2_4 * (xx - yy);
+        if (g) {
+            g[3] = make_float3(kShC2_0 * y, kShC2_0 * x, 0.0f);
+            g[4] = make_float3(0.0f, kShC2_1 * z, kShC2_1 * y);
+            g[5] = make_float3(kShC2_2 * (-2.0f * x), kShC2_2 * (-2.0f * y), kShC2_2 * (4.0f * z));
+            g[6] = make_float3(kShC2_3 * z, 0.0f, kShC2_3 * x);
+            g[7] = make_float3(kShC2_4 * (2.0f * x), kShC2_4 * (-2.0f * y), 0.0f);
+        }
+    }
+    if (degree >= 3) {
+        b[8] = kShC3_0 * (y * (3.0f * xx - yy));
+        b[9] = kShC3_1 * (x * y * z);
+        b[10] = kShC3_2 * (y * (4.0f * zz - xx - yy));
+        b[11] = kShC3_3 * (z * (2.0f * zz - 3.0f * xx - 3.0f * yy));
+        b[12] = kShC3_4 * (x * (4.0f * zz - xx - yy));
+        b[13] = kShC3_5 * (z * (xx - yy));
+        b[14] = kShC3_6 * (x * (xx - 3.0f * yy));
+        if (g) {
+            g[8] = make_float3(kShC3_0 * (6.0f * x * y), kShC3_0 * (3.0f * xx - 3.0f * yy), 0.0f);
+            g[9] = make_float3(kShC3_1 * (y * z), kShC3_1 * (x * z), kShC3_1 * (x * y));
+            g[10] = make_float3(kShC3_2 * (-2.0f * x * y), kShC3_2 * (4.0f * zz - xx - 3.0f * yy),
+                                kShC3_2 * (8.0f * y * z));
+            g[11] = make_float3(kShC3_3 * (-6.0f * x * z), kShC3_3 * (-6.0f * y * z),
+                                kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy));
+            g[12] = make_float3(kShC3_4 * (4.0f * zz - 3.0f * xx - yy), kShC3_4 * (-2.0f * x * y),
+                                kShC3_4 * (8.0f * x * z));
+            g[13] = make_float3(kShC3_5 * (2.0f * x * z), kShC3_5 * (-2.0f * y * z), kShC3_5 * (xx - yy));
+            g[14] = make_float3(kShC3_6 * (3.0f * xx - 3.0f * yy), kShC3_6 * (-6.0f * x * y), 0.0f);
+        }
+    }
+}
+
+// Unit direction from the camera centre to mean `mu`, and the distance.
+__device__ __forceinline__ void viewDir(const float mu[3], float3 campos, float d[3], float *len)
+{
+    const float vx = mu[0] - campos.x, vy = mu[1] - campos.y, vz = mu[2] - campos.z;
+    *len = sqrtf(vx * vx + vy * vy + vz * vz);
+    d[0] = vx / *len;
+    d[1] = vy / *len;
+    d[2] = vz / *len;
+}
+
+// Camera centre -R^T t in float (host and device compute it the same way).
+__host__ __device__ inline float3 cameraCentre(const u3d_camera &c)
+{
+    float v[3];
+    for (int k = 0; k < 3; ++k)
+        v[k] = -(c.R[k] * c.t[0] + c.R[3 + k] * c.t[1] + c.R[6 + k] * c.t[2]);
+    return make_float3(v[0], v[1], v[2]);
+}
+
 } // namespace u3d

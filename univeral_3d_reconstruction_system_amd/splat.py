@@ -158,21 +158,32 @@ class Rasterizer:
         except Exception:
             pass
 
-    def forward(self, model: DeviceGaussians, cam: dict, settings=None, out: DeviceArray | None = None):
+    def forward(self, model: DeviceGaussians, cam: dict, settings=None, out: DeviceArray | None = None,
+                sh_rest: DeviceArray | None = None, sh_degree: int = 0):
         settings = settings or make_settings()
         c = make_camera(cam)
         if out is None:
             out = DeviceArray(self.ctx, (c.height * c.width * 3,))
         m = model.struct()
-        check(self.ctx.lib.u3d_raster_forward(self.h, C.byref(m), C.byref(c), C.byref(settings), out.ptr))
+        if sh_rest is not None:
+            check(self.ctx.lib.u3d_raster_forward_sh(self.h, C.byref(m), sh_rest.ptr, sh_degree,
+                                                     C.byref(c), C.byref(settings), out.ptr))
+        else:
+            check(self.ctx.lib.u3d_raster_forward(self.h, C.byref(m), C.byref(c), C.byref(settings), out.ptr))
         self.n, self.cam = model.n, c
         return out
 
     def backward(self, model: DeviceGaussians, dl: DeviceArray, grads: DeviceGaussians,
-                 mean2d_grad_ndc: DeviceArray | None = None):
+                 mean2d_grad_ndc: DeviceArray | None = None, sh_rest: DeviceArray | None = None,
+                 sh_degree: int = 0, grad_sh_rest: DeviceArray | None = None):
         m, g = model.struct(), grads.struct()
-        check(self.ctx.lib.u3d_raster_backward(self.h, C.byref(m), dl.ptr, C.byref(g),
-                                               mean2d_grad_ndc.ptr if mean2d_grad_ndc else None))
+        ndc = mean2d_grad_ndc.ptr if mean2d_grad_ndc else None
+        if sh_rest is not None:
+            check(self.ctx.lib.u3d_raster_backward_sh(self.h, C.byref(m), sh_rest.ptr, sh_degree, dl.ptr,
+                                                      C.byref(g), grad_sh_rest.ptr if grad_sh_rest else None,
+                                                      ndc))
+        else:
+            check(self.ctx.lib.u3d_raster_backward(self.h, C.byref(m), dl.ptr, C.byref(g), ndc))
         return grads
 
     def radii(self) -> np.ndarray:
@@ -348,6 +359,12 @@ class Trainer:
         n = self.size()
         out = {f: np.zeros(n * d, np.float32) for f, d in FIELDS}
         check(self.ctx.lib.u3d_trainer_get_model(self.h, *(out[f].ctypes.data for f, _ in FIELDS)))
+        return out
+
+    def sh_rest(self) -> np.ndarray:
+        """SH coefficients [size x 15 x 3] (trainers created with sh_degree > 0)."""
+        out = np.zeros(self.size() * 45, np.float32)
+        check(self.ctx.lib.u3d_trainer_get_sh_rest(self.h, out.ctypes.data))
         return out
 
     def stats(self) -> dict:
