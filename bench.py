@@ -82,11 +82,24 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte "
+                         "counters FETCH_SIZE / WRITE_SIZE against a known traffic) and exit")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         return 2
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe << 20
+        src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes=n, launches=5)))
+        return 0
 
     ctx = U.Context(0)
     cams = S.ring_cameras(N_VIEWS, WIDTH, HEIGHT)
