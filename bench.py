@@ -75,6 +75,41 @@ def cpu_baseline(init, cams, images, extent, opt_gpu, budget_s=15.0):
                        f"640x480, 200 views, {threads} threads, g++ -O3 build of engine/src/splat")
 
 
+def batched_rate(ctx, lib, init, cams, images, extent, views, steps, warmup):
+    """The batched engine on the same workload: `views` views per optimiser step, one persistent
+    compositing launch per direction, the whole step replayed as a hipGraph."""
+    opt = U.options_for_iterations(ITERATIONS)
+    opt.max_gaussians = N_GAUSSIANS
+    opt.views_per_step = views
+    opt.use_graph = 1
+    t = U.Trainer(ctx, init, cams, images, extent, opt)
+    for _ in range(warmup):
+        t.step()
+    t.synchronize()
+    c0 = t.counters()
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 1))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t.step()
+    t.synchronize()
+    dt = time.perf_counter() - t0
+    ms = (C.c_double * _abi.U3D_NUM_STAGES)()
+    cnt = (C.c_int64 * _abi.U3D_NUM_STAGES)()
+    _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
+    c1 = t.counters()
+    v = c1["views"] - c0["views"]
+    gp = (c1["gaussians"] - c0["gaussians"]) / max(v, 1) * (c1["pixels"] - c0["pixels"]) / max(v, 1)
+    k = _abi.STAGES.index("graph_step")
+    return dict(views_per_step=views, steps=steps, graph=True, steps_per_s=round(steps / dt, 3),
+                views_per_s=round(v / dt, 2), ms_per_step=round(1e3 * dt / steps, 4),
+                graph_replay_ms=round(ms[k] / max(cnt[k], 1), 4),
+                gaussian_pixels_per_s=round(gp * v / dt, 1),
+                pairs_per_step=round((c1["pairs"] - c0["pairs"]) / steps, 1),
+                overflow_steps=c1["overflow_steps"] - c0["overflow_steps"],
+                tuning=t.tuning())
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +117,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batched-views", type=int, default=8,
+                    help="also time the batched engine with this many views per step and the "
+                         "whole step captured as a hipGraph (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters FETCH_SIZE / WRITE_SIZE against a known traffic) and exit")
@@ -119,7 +157,7 @@ def main(argv=None) -> int:
     for _ in range(a.warmup):
         trainer.step()
     trainer.synchronize()
-    c0 = (C.c_int64 * 4)()
+    c0 = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
     _abi.check(lib.u3d_trainer_counters(trainer.h, c0))
     _abi.check(lib.u3d_ctx_set_timing(ctx.h, 1))
     t0 = time.perf_counter()
@@ -131,7 +169,7 @@ def main(argv=None) -> int:
     cnt = (C.c_int64 * _abi.U3D_NUM_STAGES)()
     _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
     _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
-    c1 = (C.c_int64 * 4)()
+    c1 = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
     _abi.check(lib.u3d_trainer_counters(trainer.h, c1))
     steps = c1[0] - c0[0]
     mean_n = (c1[1] - c0[1]) / steps
@@ -177,6 +215,10 @@ def main(argv=None) -> int:
                     if bound == "mfma" else "work = algorithmic HBM bytes per launch")
 
     value = steps / elapsed
+    batched = None
+    if a.batched_views > 0:
+        batched = batched_rate(ctx, lib, init, cams, images, extent, a.batched_views,
+                               max(1, a.steps // a.batched_views), max(1, a.warmup // a.batched_views))
     base = None if a.no_cpu_baseline else cpu_baseline(init, cams, images, extent, opt, a.cpu_seconds)
     line = dict(
         metric="train iters/s @300k Gaussians 640px", value=round(value, 3), unit="iters/s",
@@ -192,6 +234,7 @@ def main(argv=None) -> int:
         roofline=roofline, cpu_baseline=base,
         gpu_vs_cpu=round(value / base["value"], 2) if base else None,
         train_psnr_ema=round(stats["psnr"], 3),
+        batched=batched,
     )
     print(json.dumps(line))
     return 0

@@ -224,6 +224,9 @@ int64_t u3d_trainer_size(u3d_trainer *t);
 /* Trainer::model: copies the current parameters to host arrays sized by u3d_trainer_size. */
 int u3d_trainer_get_model(u3d_trainer *t, float *means_host, float *scales_host,
                           float *quats_host, float *opacities_host, float *colors_host);
+/* Adam moments (Adam.h:30-31) as 14 * size floats each: means 3n, scales 3n, quats 4n,
+ * opacities n, colours 3n (checkpointing; tests). */
+int u3d_trainer_get_moments(u3d_trainer *t, float *m_host, float *v_host);
 /* SH coefficients [45 * size] of a trainer created with sh_degree > 0. */
 int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host);
 int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out);      /* Trainer::stats */
@@ -231,6 +234,44 @@ int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out_host);   /* Trainer.cpp
 /* Mean absolute error over all training views (time-to-target metric). */
 int u3d_trainer_evaluate_l1(u3d_trainer *t, float *out_host);
 int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host);
+
+/* --------------------------------------------------------------------------- scene sets */
+/* Independent scenes trained together (no reference counterpart: the reference trains one
+ * scene, one view per step, Trainer.cpp:64). Each step of a set trains every member that is
+ * not done on its views_per_step views: one projection pass, one sort and one persistent
+ * compositing launch per direction for all of them, then one Adam launch. Members keep their
+ * own state, schedule and random stream, so a member trains as it would alone with the
+ * batched engine. All members must share u3d_raster_settings, and each member's cameras one
+ * image size. */
+typedef struct u3d_scene_set u3d_scene_set;
+/* hbm_budget_bytes: device memory the members and the engine may hold (0: 90% of the
+ * device's free memory at creation). */
+int u3d_scene_set_create(u3d_ctx *ctx, int use_graph, size_t hbm_budget_bytes,
+                         u3d_scene_set **out);
+void u3d_scene_set_destroy(u3d_scene_set *set); /* destroys the members too */
+/* Admission by HBM budget: the set takes ownership of `scene` when its resident state plus
+ * the engine's working set for it fits the remaining budget; otherwise returns
+ * U3D_ERR_CAPACITY and the caller keeps the trainer. */
+int u3d_scene_set_add(u3d_scene_set *set, u3d_trainer *scene);
+int u3d_scene_set_size(u3d_scene_set *set);
+/* Borrowed handle of member k, for the u3d_trainer_* queries (never step or destroy it). */
+u3d_trainer *u3d_scene_set_member(u3d_scene_set *set, int k);
+int u3d_scene_set_step(u3d_scene_set *set);
+int u3d_scene_set_done(u3d_scene_set *set); /* 1 when every member is done */
+/* Bytes of the budget in use (members + engine working set). */
+size_t u3d_scene_set_bytes(u3d_scene_set *set);
+
+/* Tuning of the batched engine (trainers with views_per_step > 1 or use_graph, scene sets).
+ * Any choice gives the same images and gradients up to float summation order, except
+ * tile_size, which changes which Gaussians reach a tile exactly as RasterSettings::tileSize
+ * does in the reference. */
+typedef struct {
+    int heavy_first;        /* persistent compositing takes the longest tile lists first */
+    int workgroups_per_cu;  /* persistent workgroups per CU (0: occupancy limit) */
+} u3d_batch_tuning;
+int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out);
+int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning);
+int u3d_scene_set_set_tuning(u3d_scene_set *set, const u3d_batch_tuning *tuning);
 
 /* ----------------------------------------------------------------------- measurement */
 /* Pipeline stages timed with HIP events on the context's stream (no reference counterpart:
@@ -243,15 +284,18 @@ int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host
 #define U3D_STAGE_CHAIN 5      /* 2D -> 3D chain rule */
 #define U3D_STAGE_ADAM 6       /* fused Adam over all parameter groups */
 #define U3D_STAGE_DENSIFY 7    /* clone / split / prune / opacity reset */
-#define U3D_NUM_STAGES 8
+#define U3D_STAGE_GRAPH 8      /* one replay of a captured whole-step graph (batched engine) */
+#define U3D_NUM_STAGES 9
 /* Turns event timing on or off (off by default) and clears the accumulated totals. */
 int u3d_ctx_set_timing(u3d_ctx *ctx, int enabled);
 /* Waits for the recorded events; writes total milliseconds and launch counts per stage. */
 int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES],
                         int64_t counts[U3D_NUM_STAGES]);
-/* Work counters summed over the trainer's steps since creation: steps, Gaussians, pixels,
- * tile intersections K (values[0..3]). */
-int u3d_trainer_counters(u3d_trainer *t, int64_t values[4]);
+/* Work counters summed over the trainer's steps since creation: steps, Gaussians (per view),
+ * pixels, tile intersections K, views, and steps whose tile-pair list overflowed its capacity
+ * (batched engine; the farthest pairs of that step were dropped). */
+#define U3D_NUM_COUNTERS 6
+int u3d_trainer_counters(u3d_trainer *t, int64_t values[U3D_NUM_COUNTERS]);
 
 /* ------------------------------------------------------------------------------ helpers */
 /* The device exp (bit-identical to the host expf) evaluated on the host, for tests. */

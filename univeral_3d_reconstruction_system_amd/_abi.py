@@ -54,6 +54,11 @@ class TrainOptions(C.Structure):
                 ("use_graph", C.c_int)]
 
 
+class BatchTuning(C.Structure):
+    """u3d_batch_tuning: launch choices of the batched engine."""
+    _fields_ = [("heavy_first", C.c_int), ("workgroups_per_cu", C.c_int)]
+
+
 class TrainStats(C.Structure):
     _fields_ = [("iteration", C.c_int), ("loss", C.c_float), ("psnr", C.c_float),
                 ("gaussians", C.c_int64)]
@@ -124,11 +129,26 @@ SIGNATURES = {
     "u3d_ctx_set_timing": (I, [P, I]),
     "u3d_ctx_stage_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64)]),
     "u3d_trainer_counters": (I, [P, C.POINTER(I64)]),
+    "u3d_trainer_get_moments": (I, [P, P, P]),
+    "u3d_trainer_get_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "u3d_trainer_set_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "u3d_scene_set_create": (I, [P, I, C.c_size_t, C.POINTER(P)]),
+    "u3d_scene_set_destroy": (None, [P]),
+    "u3d_scene_set_add": (I, [P, P]),
+    "u3d_scene_set_size": (I, [P]),
+    "u3d_scene_set_member": (P, [P, I]),
+    "u3d_scene_set_step": (I, [P]),
+    "u3d_scene_set_done": (I, [P]),
+    "u3d_scene_set_bytes": (C.c_size_t, [P]),
+    "u3d_scene_set_set_tuning": (I, [P, C.POINTER(BatchTuning)]),
     "u3d_expf_host_array": (None, [P, P, I64]),
 }
 
-STAGES = ("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam", "densify")
+STAGES = ("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam", "densify",
+          "graph_step")
 U3D_NUM_STAGES = len(STAGES)
+COUNTERS = ("steps", "gaussians", "pixels", "pairs", "views", "overflow_steps")
+U3D_NUM_COUNTERS = len(COUNTERS)
 
 _lib = None
 

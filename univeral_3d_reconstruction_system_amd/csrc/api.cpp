@@ -8,7 +8,7 @@
 #include <random>
 #include <vector>
 
-#include "kernels.h"
+#include "trainer.h"
 
 namespace u3d {
 
@@ -31,112 +31,14 @@ int useDevice(u3d_ctx *ctx)
     return U3D_OK;
 }
 
-namespace {
-
-// Parameter-shaped device storage: [means 3c | scales 3c | quats 4c | opacities c | colors 3c]
-// for capacity c, so a whole model (or its gradients / moments) is one allocation.
-struct DevModel {
-    DevBuf<float> buf;
-    int64_t n = 0;
-    int64_t cap = 0;
-    int reserve(int64_t c)
-    {
-        if (c <= cap)
-            return U3D_OK;
-        U3D_TRY(buf.ensure(size_t(c) * 14));
-        cap = c;
-        return U3D_OK;
-    }
-    u3d_gaussians view() const
-    {
-        float *b = buf.p;
-        return {b, b + 3 * cap, b + 6 * cap, b + 10 * cap, b + 11 * cap, n};
-    }
-    int zero(hipStream_t s) const
-    {
-        if (cap > 0)
-            U3D_HIP(hipMemsetAsync(buf.p, 0, size_t(cap) * 14 * sizeof(float), s));
-        return U3D_OK;
-    }
-    void swap(DevModel &o)
-    {
-        std::swap(buf.p, o.buf.p);
-        std::swap(buf.cap, o.buf.cap);
-        std::swap(n, o.n);
-        std::swap(cap, o.cap);
-    }
-};
-
-int copyFields(hipStream_t s, const u3d_gaussians &src, const u3d_gaussians &dst, int64_t n,
-               hipMemcpyKind kind)
-{
-    if (n <= 0)
-        return U3D_OK;
-    const size_t f = sizeof(float);
-    U3D_HIP(hipMemcpyAsync(dst.means, src.means, size_t(n) * 3 * f, kind, s));
-    U3D_HIP(hipMemcpyAsync(dst.scales, src.scales, size_t(n) * 3 * f, kind, s));
-    U3D_HIP(hipMemcpyAsync(dst.quats, src.quats, size_t(n) * 4 * f, kind, s));
-    U3D_HIP(hipMemcpyAsync(dst.opacities, src.opacities, size_t(n) * f, kind, s));
-    U3D_HIP(hipMemcpyAsync(dst.colors, src.colors, size_t(n) * 3 * f, kind, s));
-    return U3D_OK;
-}
-
-int checkModel(const u3d_gaussians *m, const char *what)
-{
-    U3D_CHECK(m != nullptr, U3D_ERR_INVALID, std::string("null ") + what);
-    U3D_CHECK(m->n >= 0 && m->n < (int64_t(1) << 31), U3D_ERR_INVALID, std::string("bad size of ") + what);
-    if (m->n > 0)
-        U3D_CHECK(m->means && m->scales && m->quats && m->opacities && m->colors, U3D_ERR_INVALID,
-                  std::string("null field in ") + what);
-    return U3D_OK;
-}
-
-int bitsFor(int64_t values)
-{
-    int b = 1;
-    while ((int64_t(1) << b) < values)
-        ++b;
-    return b;
-}
-
-} // namespace
 } // namespace u3d
 
 using namespace u3d;
 
-struct u3d_raster {
-    u3d_ctx *ctx = nullptr;
-    int64_t n = 0;
-    int64_t numPairs = 0;
-    u3d_camera cam{};
-    u3d_raster_settings st{};
-    RasterGeometry geo{};
-    bool haveForward = false;
-    DevBuf<float4> rec0, rec1;
-    DevBuf<float> rec2, depth;
-    DevBuf<int> radius;
-    DevBuf<int4> rect;
-    DevBuf<uint8_t> flags;
-    DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
-    DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
-    DevBuf<uint32_t> hist, histScan, scanScratch, total;
-    DevBuf<int> tileOffsets;
-    const int *tileList = nullptr;
-    DevBuf<float> finalT, grad2d;
-    DevBuf<int> nContrib;
-    DevBuf<float> dbgMean2d, dbgConic, dbgColor, dbgOpacity;
-    ScreenBuffers screen() const
-    {
-        return {rec0.p, rec1.p, rec2.p, depth.p, radius.p, rect.p, flags.p, depthKey.p, nTiles.p};
-    }
-};
-
 namespace u3d {
-namespace {
 
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
-                      const u3d_raster_settings &st, float *rgb, const float *shRest = nullptr,
-                      int shDegree = 0)
+                      const u3d_raster_settings &st, float *rgb, const float *shRest, int shDegree)
 {
     U3D_CHECK(shDegree >= 0 && shDegree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
     U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
@@ -218,6 +120,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     return U3D_OK;
 }
 
+namespace {
+
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
                        const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr)
@@ -252,39 +156,7 @@ int ctxScratch(u3d_ctx *ctx, size_t bytes, unsigned char **out)
 } // namespace
 } // namespace u3d
 
-struct u3d_adam {
-    u3d_ctx *ctx = nullptr;
-    DevModel m, v, spare;
-    int step = 0;
-};
-
 // ---------------------------------------------------------------------------------- trainer
-struct u3d_trainer {
-    u3d_ctx *ctx = nullptr;
-    DevModel params, spare, grads, adamM, adamV;
-    int adamStep = 0;
-    std::vector<u3d_camera> cams;
-    std::vector<size_t> imageOffset;
-    DevBuf<uint8_t> targets;
-    float extent = 1.0f;
-    u3d_train_options opt{};
-    u3d_raster raster;
-    std::mt19937 rng;
-    std::vector<int> order;
-    size_t orderPos = 0;
-    int iteration = 0;
-    bool opacityWasReset = false;
-    DevBuf<float> render, dloss, gradAccum, stepStats, evalOut;
-    DevBuf<int> gradCount;
-    DevBuf<double> partials;
-    int statsFolded = 0;
-    u3d_train_stats stats{};
-    int64_t counters[4] = {0, 0, 0, 0}; // steps, Gaussians, pixels, pairs
-    // Spherical-harmonic coefficients beyond DC (extension), [cap][15][3].
-    int shDegree = 0;
-    DevBuf<float> shP, shG, shM, shV, shSpare;
-};
-
 // ----------------------------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -646,7 +518,9 @@ int u3d_adam_reset_opacity_moments(u3d_adam *a)
     return U3D_OK;
 }
 
-namespace {
+} // extern "C"
+
+namespace u3d {
 void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2)
 {
     // Adam.cpp:16-17, 70-72 (host powf, as the reference).
@@ -657,7 +531,9 @@ void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5],
         stepSize[k] = rates[k] / bc1;
     *invSqrtBc2 = 1.0f / std::sqrt(bc2);
 }
-} // namespace
+} // namespace u3d
+
+extern "C" {
 
 int u3d_adam_step(u3d_adam *a, u3d_gaussians *params, const u3d_gaussians *grads,
                   const u3d_learning_rates *lr)
@@ -853,8 +729,10 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(cap) * sizeof(float), s));
     U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(cap) * sizeof(int), s));
     U3D_CHECK(options->sh_degree >= 0 && options->sh_degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
-    U3D_CHECK(options->views_per_step <= 1, U3D_ERR_UNSUPPORTED, "views_per_step > 1 not available");
-    U3D_CHECK(options->use_graph == 0, U3D_ERR_UNSUPPORTED, "use_graph not available");
+    U3D_CHECK(options->views_per_step >= 0 && options->views_per_step <= kMaxViewsPerStep,
+              U3D_ERR_INVALID, "views_per_step must be 0..64");
+    t->viewsPerStep = std::max(1, options->views_per_step);
+    t->useGraph = options->use_graph != 0;
     t->shDegree = options->sh_degree;
     if (t->shDegree > 0) {
         for (DevBuf<float> *b : {&t->shP, &t->shG, &t->shM, &t->shV, &t->shSpare}) {
@@ -862,7 +740,7 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
             U3D_HIP(hipMemsetAsync(b->p, 0, b->cap * sizeof(float), s));
         }
     }
-    U3D_TRY(t->stepStats.ensure(size_t(std::max(1, options->iterations)) * 4));
+    U3D_TRY(t->stepStats.ensure(size_t(std::max(1, options->iterations)) * 4 * size_t(t->viewsPerStep)));
     U3D_TRY(t->evalOut.ensure(size_t(std::max(1, n_images)) + 4));
     size_t parts = 1024;
     for (const u3d_camera &c : t->cams)
@@ -870,6 +748,8 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_TRY(t->partials.ensure(parts));
     U3D_HIP(hipStreamSynchronize(s));
     t->stats.gaussians = n;
+    if (t->viewsPerStep > 1 || t->useGraph)
+        U3D_TRY(createBatchEngine(ctx, t->useGraph, &t->engine));
     *out = t.release();
     return U3D_OK;
 }
@@ -880,6 +760,7 @@ void u3d_trainer_destroy(u3d_trainer *t)
         return;
     (void)hipSetDevice(t->ctx->device);
     (void)hipStreamSynchronize(t->ctx->stream);
+    destroyBatchEngine(t->engine);
     delete t;
 }
 
@@ -975,13 +856,12 @@ int trainerResetOpacity(u3d_trainer *t)
 
 } // namespace
 
-int u3d_trainer_step(u3d_trainer *t)
+} // extern "C"
+
+namespace u3d {
+
+int trainerNextView(u3d_trainer *t)
 {
-    // Trainer::step, Trainer.cpp:53-103.
-    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
-    U3D_TRY(useDevice(t->ctx));
-    if (t->cams.empty() || u3d_trainer_done(t))
-        return U3D_OK;
     if (t->orderPos >= t->order.size()) {
         t->order.resize(t->cams.size());
         for (size_t k = 0; k < t->order.size(); ++k)
@@ -989,7 +869,49 @@ int u3d_trainer_step(u3d_trainer *t)
         std::shuffle(t->order.begin(), t->order.end(), t->rng);
         t->orderPos = 0;
     }
-    const size_t view = size_t(t->order[t->orderPos++]);
+    return t->order[t->orderPos++];
+}
+
+float trainerMeansLr(const u3d_trainer *t)
+{
+    const float tt = float(t->iteration) / float(std::max(1, t->opt.iterations - 1));
+    return std::exp(std::log(t->opt.lr.means) * (1 - tt) + std::log(t->opt.means_lr_final) * tt) * t->extent;
+}
+
+int trainerFinishStep(u3d_trainer *t)
+{
+    ++t->iteration;
+    const u3d_train_options &o = t->opt;
+    const bool densify = t->iteration >= o.refine_start && t->iteration < o.refine_stop
+                         && o.refine_interval > 0 && t->iteration % o.refine_interval == 0;
+    const bool reset = o.opacity_reset_interval > 0 && t->iteration < o.refine_stop
+                       && t->iteration % o.opacity_reset_interval == 0;
+    if (densify || reset) {
+        StageScope scope(t->ctx, U3D_STAGE_DENSIFY);
+        if (densify)
+            U3D_TRY(trainerDensify(t));
+        if (reset)
+            U3D_TRY(trainerResetOpacity(t));
+    }
+    return U3D_OK;
+}
+
+} // namespace u3d
+
+extern "C" {
+
+int u3d_trainer_step(u3d_trainer *t)
+{
+    // Trainer::step, Trainer.cpp:53-103.
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_TRY(useDevice(t->ctx));
+    if (t->cams.empty() || u3d_trainer_done(t))
+        return U3D_OK;
+    if (t->engine) {
+        u3d_trainer *one[1] = {t};
+        return batchTrainStep(t->engine, one, 1);
+    }
+    const size_t view = size_t(trainerNextView(t));
     const u3d_camera &cam = t->cams[view];
     const hipStream_t s = t->ctx->stream;
     const u3d_gaussians pv = t->params.view();
@@ -998,6 +920,7 @@ int u3d_trainer_step(u3d_trainer *t)
     t->counters[1] += t->params.n;
     t->counters[2] += int64_t(cam.width) * cam.height;
     t->counters[3] += t->raster.numPairs;
+    t->counters[4] += 1;
     float *stats = t->stepStats.p + size_t(t->iteration) * 4;
     {
         StageScope scope(t->ctx, U3D_STAGE_LOSS);
@@ -1015,8 +938,7 @@ int u3d_trainer_step(u3d_trainer *t)
                                t->shP.p, t->shDegree, t->shG.p));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
-    const float tt = float(t->iteration) / float(std::max(1, t->opt.iterations - 1));
-    lr.means = std::exp(std::log(t->opt.lr.means) * (1 - tt) + std::log(t->opt.means_lr_final) * tt) * t->extent;
+    lr.means = trainerMeansLr(t);
     ++t->adamStep;
     float stepSize[5], inv;
     adamCoefficients(t->adamStep, lr, stepSize, &inv);
@@ -1025,27 +947,12 @@ int u3d_trainer_step(u3d_trainer *t)
         StageScope scope(t->ctx, U3D_STAGE_ADAM);
         U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv));
         if (t->shDegree > 0) {
-            // SH coefficients: the colour rate scaled down (3DGS uses 1/20), same moments rule.
-            const float scale = t->opt.sh_lr_scale > 0.0f ? t->opt.sh_lr_scale : 1.0f / 20.0f;
             const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
             U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,
-                                    (lr.colors * scale) / bc1, inv));
+                                    (lr.colors * shRateScale(t)) / bc1, inv));
         }
     }
-    ++t->iteration;
-    const u3d_train_options &o = t->opt;
-    const bool densify = t->iteration >= o.refine_start && t->iteration < o.refine_stop
-                         && o.refine_interval > 0 && t->iteration % o.refine_interval == 0;
-    const bool reset = o.opacity_reset_interval > 0 && t->iteration < o.refine_stop
-                       && t->iteration % o.opacity_reset_interval == 0;
-    if (densify || reset) {
-        StageScope scope(t->ctx, U3D_STAGE_DENSIFY);
-        if (densify)
-            U3D_TRY(trainerDensify(t));
-        if (reset)
-            U3D_TRY(trainerResetOpacity(t));
-    }
-    return U3D_OK;
+    return trainerFinishStep(t);
 }
 
 int u3d_trainer_get_model(u3d_trainer *t, float *means, float *scales, float *quats,
@@ -1057,6 +964,25 @@ int u3d_trainer_get_model(u3d_trainer *t, float *means, float *scales, float *qu
     U3D_CHECK(n == 0 || (means && scales && quats && opacities && colors), U3D_ERR_INVALID, "null output");
     const u3d_gaussians host{means, scales, quats, opacities, colors, n};
     U3D_TRY(copyFields(t->ctx->stream, t->params.view(), host, n, hipMemcpyDeviceToHost));
+    U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    return U3D_OK;
+}
+
+int u3d_trainer_get_moments(u3d_trainer *t, float *m_host, float *v_host)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_TRY(useDevice(t->ctx));
+    const int64_t n = t->params.n;
+    U3D_CHECK(n == 0 || (m_host && v_host), U3D_ERR_INVALID, "null output");
+    const DevModel *src[2] = {&t->adamM, &t->adamV};
+    float *dst[2] = {m_host, v_host};
+    for (int k = 0; k < 2; ++k) {
+        float *h = dst[k];
+        const u3d_gaussians hv{h, h + 3 * n, h + 6 * n, h + 10 * n, h + 11 * n, n};
+        u3d_gaussians dv = src[k]->view();
+        dv.n = n;
+        U3D_TRY(copyFields(t->ctx->stream, dv, hv, n, hipMemcpyDeviceToHost));
+    }
     U3D_HIP(hipStreamSynchronize(t->ctx->stream));
     return U3D_OK;
 }
@@ -1080,17 +1006,28 @@ int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out)
     // TrainStats EMA (Trainer.cpp:88-92), folded lazily from the per-step device values.
     U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
     U3D_TRY(useDevice(t->ctx));
+    // With several views per step, a step's value is the mean over its views.
     const int from = t->statsFolded, to = t->iteration;
+    const size_t B = size_t(t->viewsPerStep);
     if (to > from) {
-        std::vector<float> v(size_t(to - from) * 4);
-        U3D_HIP(hipMemcpyAsync(v.data(), t->stepStats.p + size_t(from) * 4, v.size() * sizeof(float),
+        std::vector<float> v(size_t(to - from) * 4 * B);
+        U3D_HIP(hipMemcpyAsync(v.data(), t->stepStats.p + size_t(from) * 4 * B, v.size() * sizeof(float),
                                hipMemcpyDeviceToHost, t->ctx->stream));
         U3D_HIP(hipStreamSynchronize(t->ctx->stream));
         for (int it = from; it < to; ++it) {
-            const float *x = &v[size_t(it - from) * 4];
+            float loss = 0.0f, psnr = 0.0f;
+            for (size_t k = 0; k < B; ++k) {
+                const float *x = &v[(size_t(it - from) * B + k) * 4];
+                loss += x[0];
+                psnr += x[3];
+            }
+            if (B > 1) {
+                loss /= float(B);
+                psnr /= float(B);
+            }
             const float k = t->stats.iteration == 0 ? 1.0f : 0.05f;
-            t->stats.loss += k * (x[0] - t->stats.loss);
-            t->stats.psnr += k * (x[3] - t->stats.psnr);
+            t->stats.loss += k * (loss - t->stats.loss);
+            t->stats.psnr += k * (psnr - t->stats.psnr);
             t->stats.iteration = it + 1;
         }
         t->statsFolded = to;
@@ -1184,10 +1121,10 @@ int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES], int64_t c
     return U3D_OK;
 }
 
-int u3d_trainer_counters(u3d_trainer *t, int64_t values[4])
+int u3d_trainer_counters(u3d_trainer *t, int64_t values[U3D_NUM_COUNTERS])
 {
     U3D_CHECK(t && values, U3D_ERR_INVALID, "null argument");
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < U3D_NUM_COUNTERS; ++k)
         values[k] = t->counters[k];
     return U3D_OK;
 }

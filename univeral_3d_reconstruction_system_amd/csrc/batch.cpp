@@ -1,0 +1,757 @@
+// Batched training engine (no reference counterpart; the reference trains one view of one scene
+// per step, Trainer.cpp:53-103).
+//
+// Every (scene, view) render of a step is an item. All items share one projection launch, one
+// depth sort over their concatenated screen entries, one tile sort over their concatenated
+// tile ids, one persistent compositing launch per direction (workgroups pull 16x16 blocks of
+// any item from an atomic counter, heaviest tile lists first), one loss launch, one chain-rule
+// launch and one Adam launch. What changes from step to step (cameras, target and output
+// pointers, optimiser coefficients, the stats slot) travels in one small table uploaded before
+// the step, so the launches themselves only depend on the layout (Gaussian counts, image
+// sizes, pair capacity): they are captured once as a hipGraph and replayed until the layout
+// changes (densification, a grown pair capacity).
+//
+// The pair count K is never read back inside a step: sorts and tile offsets take it from
+// device memory, with grids sized for a capacity. Pairs past the capacity are dropped (the
+// farthest in depth order) and flagged; the host reads each step's K and flag a few steps later
+// and grows the capacity before it is reached again.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "trainer.h"
+
+namespace u3d {
+
+namespace {
+constexpr int kRing = 4; // table uploads / status read-backs in flight
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+int ceilDiv(int a, int b) { return (a + b - 1) / b; }
+} // namespace
+
+class BatchEngine
+{
+public:
+    explicit BatchEngine(u3d_ctx *c, bool graph) : ctx(c), useGraph(graph) {}
+    ~BatchEngine()
+    {
+        dropGraph();
+        for (int k = 0; k < kRing; ++k) {
+            if (slotDone[k])
+                (void)hipEventDestroy(slotDone[k]);
+            if (statusDone[k])
+                (void)hipEventDestroy(statusDone[k]);
+        }
+        if (pinned)
+            (void)hipHostFree(pinned);
+        if (statusHost)
+            (void)hipHostFree(statusHost);
+    }
+
+    int init()
+    {
+        for (int k = 0; k < kRing; ++k) {
+            U3D_HIP(hipEventCreateWithFlags(&slotDone[k], hipEventDisableTiming));
+            U3D_HIP(hipEventCreateWithFlags(&statusDone[k], hipEventDisableTiming));
+            slotUsed[k] = false;
+            statusStep[k] = -1;
+        }
+        U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 2 * kRing));
+        occFwd = rasterBatchedOccupancy(false);
+        occBwd = rasterBatchedOccupancy(true);
+        return U3D_OK;
+    }
+
+    // New launch parameters: the captured graph (if any) is rebuilt on the next step.
+    int setTuning(int heavy, int perCu)
+    {
+        U3D_HIP(hipStreamSynchronize(ctx->stream));
+        heavyFirst = heavy != 0;
+        workgroupsPerCu = perCu;
+        dropGraph();
+        return U3D_OK;
+    }
+
+    int step(u3d_trainer *const *scenes, int numScenes);
+    size_t workingBytes() const { return allocated; }
+    // Working-set estimate for one more member with n Gaussians, B views of P pixels each.
+    static size_t estimateBytes(int64_t n, int views, size_t pixels)
+    {
+        const size_t entries = size_t(n) * size_t(views);
+        const size_t perEntry = 2 * 16 + 4 * 4 + 16 + 1 + 8 * 4 + 9 * 4;
+        const size_t pairs = entries * 24; // capacity for ~12 tiles per entry, 2 copies
+        return entries * perEntry + pairs * 8 + size_t(views) * pixels * 40;
+    }
+
+    u3d_ctx *ctx;
+    bool useGraph;
+    int heavyFirst = 1;
+    int workgroupsPerCu = 0;
+
+private:
+    int occFwd = 1, occBwd = 1;
+    struct Member {
+        const u3d_trainer *t;
+        int64_t n;
+        int width, height, views, shDegree;
+        bool operator==(const Member &o) const
+        {
+            return t == o.t && n == o.n && width == o.width && height == o.height
+                   && views == o.views && shDegree == o.shDegree;
+        }
+    };
+
+    int relayout(const std::vector<u3d_trainer *> &act);
+    int fillTable(const std::vector<u3d_trainer *> &act, unsigned char *dst);
+    int probePairs();
+    int enqueue();
+    int capture();
+    void dropGraph()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    int poll(bool wait);
+    ScreenBuffers screen() const
+    {
+        return {rec0.p, rec1.p, rec2.p, depth.p, radius.p, rect.p, flags.p, depthKey.p, nTiles.p};
+    }
+
+    // layout
+    std::vector<Member> key;
+    u3d_raster_settings st{};
+    int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
+    int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
+    size_t totalPix = 0, partialsLen = 0;
+    size_t offItems = 0, offLoss = 0, offSteps = 0, offGroups = 0, tableBytes = 0;
+    bool needRelayout = true;
+    int64_t grownCap = 0; // capacity requested by poll()
+    size_t allocated = 0;
+
+    // per-step table: pinned ring -> device
+    unsigned char *pinned = nullptr;
+    size_t pinnedSlot = 0;
+    hipEvent_t slotDone[kRing] = {};
+    bool slotUsed[kRing] = {};
+    uint32_t *statusHost = nullptr; // pinned: {K, overflow} per slot
+    hipEvent_t statusDone[kRing] = {};
+    int64_t statusStep[kRing] = {};
+    std::vector<u3d_trainer *> statusScenes[kRing];
+    int ring = 0;
+    int64_t stepNo = 0;
+
+    // device buffers
+    DevBuf<unsigned char> table;
+    DevBuf<float4> rec0, rec1;
+    DevBuf<float> rec2, depth, grad2d, rgb, dl, finalT;
+    DevBuf<int> radius, entryItem, tileOffsets, nContrib;
+    DevBuf<int4> rect;
+    DevBuf<uint8_t> flags;
+    DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets, total;
+    DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt, hist, histScan, scanScratch;
+    DevBuf<uint32_t> blockKey, blockOrder, blockKeyAlt, blockOrderAlt;
+    DevBuf<unsigned> counters; // persistent work counters (forward, backward)
+    DevBuf<double> partials;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
+{
+    const hipStream_t s = ctx->stream;
+    U3D_HIP(hipStreamSynchronize(s));
+    dropGraph();
+    st = act[0]->opt.raster;
+    key.clear();
+    numItems = 0;
+    numGroups = int(act.size());
+    totalTiles = 0;
+    maxLossBlocks = 0;
+    totalEntries = 0;
+    maxN = 0;
+    totalBlocks = 0;
+    totalPix = 0;
+    partialsLen = 0;
+    const int ts = st.tile_size;
+    for (u3d_trainer *t : act) {
+        U3D_CHECK(std::memcmp(&t->opt.raster, &st, sizeof st) == 0, U3D_ERR_UNSUPPORTED,
+                  "batched scenes must share the raster settings");
+        const int W = t->cams[0].width, H = t->cams[0].height;
+        for (const u3d_camera &c : t->cams)
+            U3D_CHECK(c.width == W && c.height == H, U3D_ERR_UNSUPPORTED,
+                      "batched training needs one image size per scene");
+        U3D_CHECK(t->grads.cap == t->params.cap && t->adamM.cap == t->params.cap
+                      && t->adamV.cap == t->params.cap,
+                  U3D_ERR_INVALID, "parameter, gradient and moment capacities differ");
+        const int B = t->viewsPerStep;
+        const int64_t n = t->params.n;
+        key.push_back({t, n, W, H, B, t->shDegree});
+        int lbx = 0, lnb = 0;
+        lossItemGeometry(W, H, &lbx, &lnb);
+        numItems += B;
+        totalEntries += n * B;
+        maxN = std::max(maxN, n);
+        totalTiles += ceilDiv(W, ts) * ceilDiv(H, ts) * B;
+        totalBlocks += int64_t(ceilDiv(W, 16)) * ceilDiv(H, 16) * B;
+        totalPix += size_t(W) * size_t(H) * size_t(B);
+        partialsLen += size_t(9) * size_t(lnb) * size_t(B);
+        maxLossBlocks = std::max(maxLossBlocks, lnb);
+    }
+    U3D_CHECK(totalEntries < (int64_t(1) << 32) && totalTiles < (1 << 30), U3D_ERR_CAPACITY,
+              "batch too large for 32-bit pair values");
+    tileBits = bitsFor(totalTiles);
+    offItems = 0;
+    offLoss = align256(size_t(numItems) * sizeof(RenderItem));
+    offSteps = offLoss + align256(size_t(numItems) * sizeof(LossItem));
+    offGroups = offSteps + align256(size_t(numGroups) * sizeof(SceneStep));
+    tableBytes = offGroups + align256(size_t(numGroups) * sizeof(int2));
+
+    const size_t ne = size_t(std::max<int64_t>(totalEntries, 1));
+    U3D_TRY(table.ensure(tableBytes));
+    if (pinnedSlot < tableBytes) {
+        U3D_HIP(hipStreamSynchronize(s));
+        if (pinned)
+            U3D_HIP(hipHostFree(pinned));
+        pinned = nullptr;
+        pinnedSlot = align256(tableBytes * 2);
+        U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&pinned), pinnedSlot * kRing));
+        for (int k = 0; k < kRing; ++k)
+            slotUsed[k] = false;
+    }
+    U3D_TRY(rec0.ensure(ne));
+    U3D_TRY(rec1.ensure(ne));
+    U3D_TRY(rec2.ensure(ne));
+    U3D_TRY(depth.ensure(ne));
+    U3D_TRY(radius.ensure(ne));
+    U3D_TRY(rect.ensure(ne));
+    U3D_TRY(flags.ensure(ne));
+    U3D_TRY(depthKey.ensure(ne));
+    U3D_TRY(nTiles.ensure(ne));
+    U3D_TRY(keyAlt.ensure(ne));
+    U3D_TRY(valA.ensure(ne));
+    U3D_TRY(valAlt.ensure(ne));
+    U3D_TRY(ntSorted.ensure(ne));
+    U3D_TRY(pairOffsets.ensure(ne));
+    U3D_TRY(entryItem.ensure(ne));
+    U3D_TRY(grad2d.ensure(ne * kGrad2dStride));
+    U3D_TRY(total.ensure(4));
+    U3D_TRY(counters.ensure(4));
+    U3D_TRY(tileOffsets.ensure(size_t(totalTiles) + 1));
+    const size_t nb = size_t(std::max<int64_t>(totalBlocks, 1));
+    U3D_TRY(blockKey.ensure(nb));
+    U3D_TRY(blockOrder.ensure(nb));
+    U3D_TRY(blockKeyAlt.ensure(nb));
+    U3D_TRY(blockOrderAlt.ensure(nb));
+    const size_t np = std::max<size_t>(totalPix, 1);
+    U3D_TRY(rgb.ensure(np * 3));
+    U3D_TRY(dl.ensure(np * 3));
+    U3D_TRY(finalT.ensure(np));
+    U3D_TRY(nContrib.ensure(np));
+    U3D_TRY(partials.ensure(std::max<size_t>(partialsLen, 1)));
+    // Gradients start at zero; afterwards the chain rule clears the 2D gradients it consumed and
+    // Adam the parameter gradients it applied.
+    U3D_HIP(hipMemsetAsync(grad2d.p, 0, grad2d.cap * sizeof(float), s));
+    for (u3d_trainer *t : act) {
+        t->grads.n = t->params.n;
+        U3D_TRY(t->grads.zero(s));
+        if (t->shDegree > 0)
+            U3D_HIP(hipMemsetAsync(t->shG.p, 0, t->shG.cap * sizeof(float), s));
+    }
+    needRelayout = false;
+    pairCap = 0; // sized by the probe of the first step of this layout (or grownCap)
+    return U3D_OK;
+}
+
+int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char *dst)
+{
+    RenderItem *items = reinterpret_cast<RenderItem *>(dst + offItems);
+    LossItem *loss = reinterpret_cast<LossItem *>(dst + offLoss);
+    SceneStep *steps = reinterpret_cast<SceneStep *>(dst + offSteps);
+    int2 *groups = reinterpret_cast<int2 *>(dst + offGroups);
+    const int ts = st.tile_size;
+    int v = 0, tile = 0;
+    int64_t screen = 0, block = 0;
+    size_t pix = 0, part = 0;
+    for (int g = 0; g < int(act.size()); ++g) {
+        u3d_trainer *t = act[size_t(g)];
+        const int B = t->viewsPerStep;
+        const int W = t->cams[0].width, H = t->cams[0].height;
+        const int64_t n = t->params.n;
+        const int tilesX = ceilDiv(W, ts), tilesY = ceilDiv(H, ts);
+        const int blocksX = ceilDiv(W, 16), blocksY = ceilDiv(H, 16);
+        int lbx = 0, lnb = 0;
+        lossItemGeometry(W, H, &lbx, &lnb);
+        groups[g] = make_int2(v, B);
+        const u3d_gaussians pv = t->params.view(), gv = t->grads.view();
+        for (int k = 0; k < B; ++k, ++v) {
+            const size_t view = size_t(trainerNextView(t));
+            RenderItem it{};
+            it.cam = t->cams[view];
+            it.screenBase = screen;
+            it.n = n;
+            it.blockBase = block;
+            it.tileBase = tile;
+            it.tilesX = tilesX;
+            it.tilesY = tilesY;
+            it.blocksX = blocksX;
+            it.shDegree = t->shDegree;
+            it.means = pv.means;
+            it.scales = pv.scales;
+            it.quats = pv.quats;
+            it.opacities = pv.opacities;
+            it.colors = pv.colors;
+            it.shRest = t->shDegree > 0 ? t->shP.p : nullptr;
+            it.rgb = rgb.p + pix * 3;
+            it.finalT = finalT.p + pix;
+            it.nContrib = nContrib.p + pix;
+            it.dl = dl.p + pix * 3;
+            it.gMeans = gv.means;
+            it.gScales = gv.scales;
+            it.gQuats = gv.quats;
+            it.gOpac = gv.opacities;
+            it.gColors = gv.colors;
+            it.gShRest = t->shDegree > 0 ? t->shG.p : nullptr;
+            it.ndc = t->gradAccum.p;
+            it.gradCount = t->gradCount.p;
+            items[v] = it;
+            LossItem li{};
+            li.render = it.rgb;
+            li.target = t->targets.p + t->imageOffset[view];
+            li.grad = dl.p + pix * 3;
+            li.result = t->stepStats.p + (size_t(t->iteration) * size_t(B) + size_t(k)) * 4;
+            li.partials = partials.p + part;
+            li.width = W;
+            li.height = H;
+            li.blocksX = lbx;
+            li.numBlocks = lnb;
+            li.invCount = 1.0f / float(size_t(W) * size_t(H) * 3);
+            li.gradScale = 1.0f / float(B); // the step's loss is the mean over its views
+            loss[v] = li;
+            screen += n;
+            block += int64_t(blocksX) * blocksY;
+            tile += tilesX * tilesY;
+            pix += size_t(W) * size_t(H);
+            part += size_t(9) * size_t(lnb);
+        }
+        // Optimiser coefficients of this step (Trainer.cpp:82-84, Adam.cpp:70-72).
+        u3d_learning_rates lr = t->opt.lr;
+        lr.means = trainerMeansLr(t);
+        ++t->adamStep;
+        SceneStep sc{};
+        adamCoefficients(t->adamStep, lr, sc.stepSize, &sc.invSqrtBc2);
+        sc.params = t->params.buf.p;
+        sc.grads = t->grads.buf.p;
+        sc.adamM = t->adamM.buf.p;
+        sc.adamV = t->adamV.buf.p;
+        sc.n = n;
+        sc.cap = t->params.cap;
+        sc.shDegree = t->shDegree;
+        if (t->shDegree > 0) {
+            sc.shP = t->shP.p;
+            sc.shG = t->shG.p;
+            sc.shM = t->shM.p;
+            sc.shV = t->shV.p;
+            const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
+            sc.shStepSize = (lr.colors * shRateScale(t)) / bc1;
+        }
+        steps[g] = sc;
+    }
+    return U3D_OK;
+}
+
+// Sizes the pair capacity from this step's real pair count (projection, depth sort and scan
+// run once more inside the step itself).
+int BatchEngine::probePairs()
+{
+    const hipStream_t s = ctx->stream;
+    const RenderItem *dItems = reinterpret_cast<const RenderItem *>(table.p + offItems);
+    const ScreenBuffers sb = screen();
+    U3D_TRY(launchProjectBatched(s, dItems, numItems, maxN, st, sb, entryItem.p));
+    U3D_TRY(scanScratch.ensure(scanScratchSize(totalEntries)));
+    U3D_TRY(exclusiveScanU32(s, nTiles.p, pairOffsets.p, totalEntries, total.p, scanScratch.p,
+                             scanScratch.cap));
+    uint32_t K = 0;
+    U3D_HIP(hipMemcpyAsync(&K, total.p, sizeof K, hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    // Views differ in coverage: leave room for the spread between steps.
+    const double margin = numItems >= 4 ? 1.3 : 1.6;
+    int64_t cap = int64_t(double(K) * margin) + 65536;
+    cap = std::max(cap, grownCap);
+    cap = std::min<int64_t>(cap, (int64_t(1) << 32) - 1);
+    pairCap = cap;
+    const size_t pc = size_t(pairCap);
+    U3D_TRY(pairTile.ensure(pc));
+    U3D_TRY(pairGauss.ensure(pc));
+    U3D_TRY(pairTileAlt.ensure(pc));
+    U3D_TRY(pairGaussAlt.ensure(pc));
+    const int64_t sortMax = std::max(std::max(totalEntries, pairCap), totalBlocks);
+    U3D_TRY(hist.ensure(radixHistSize(sortMax)));
+    U3D_TRY(histScan.ensure(radixScanScratchSize(sortMax)));
+    U3D_TRY(scanScratch.ensure(scanScratchSize(sortMax)));
+    allocated = 0;
+    for (size_t b : {rec0.cap * 16, rec1.cap * 16, rec2.cap * 4, depth.cap * 4, grad2d.cap * 4,
+                     radius.cap * 4, rect.cap * 16, flags.cap, depthKey.cap * 4, nTiles.cap * 4,
+                     keyAlt.cap * 4, valA.cap * 4, valAlt.cap * 4, ntSorted.cap * 4,
+                     pairOffsets.cap * 4, entryItem.cap * 4, pairTile.cap * 16, hist.cap * 4,
+                     histScan.cap * 4, scanScratch.cap * 4, rgb.cap * 4, dl.cap * 4, finalT.cap * 4,
+                     nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4})
+        allocated += b;
+    return U3D_OK;
+}
+
+// Every launch of one step, in stream order. Nothing here allocates, synchronises or writes
+// with a memset, so the sequence can be captured as a graph.
+int BatchEngine::enqueue()
+{
+    const hipStream_t s = ctx->stream;
+    const RenderItem *dItems = reinterpret_cast<const RenderItem *>(table.p + offItems);
+    const LossItem *dLoss = reinterpret_cast<const LossItem *>(table.p + offLoss);
+    const SceneStep *dSteps = reinterpret_cast<const SceneStep *>(table.p + offSteps);
+    const int2 *dGroups = reinterpret_cast<const int2 *>(table.p + offGroups);
+    const ScreenBuffers sb = screen();
+    {
+        StageScope scope(ctx, U3D_STAGE_PROJECT);
+        U3D_TRY(launchProjectBatched(s, dItems, numItems, maxN, st, sb, entryItem.p));
+    }
+    const uint32_t *order = nullptr;
+    const int *tileList = nullptr;
+    {
+        StageScope scope(ctx, U3D_STAGE_SORT);
+        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4)); // K, overflow flag
+        U3D_TRY(launchIota(s, valA.p, totalEntries));
+        RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap};
+        bool alt = false;
+        U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs, &alt));
+        const uint32_t *sorted = alt ? valAlt.p : valA.p;
+        U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries));
+        U3D_TRY(exclusiveScanU32(s, ntSorted.p, pairOffsets.p, totalEntries, total.p,
+                                 scanScratch.p, scanScratch.cap));
+        U3D_TRY(launchEmitPairsBatched(s, totalEntries, sorted, pairOffsets.p, rect.p, entryItem.p,
+                                       dItems, pairTile.p, pairGauss.p, ntSorted.p, pairCap,
+                                       total.p + 1));
+        bool alt2 = false;
+        U3D_TRY(radixSortPairs(s, pairTile.p, pairGauss.p, pairTileAlt.p, pairGaussAlt.p, pairCap,
+                               tileBits, rs, &alt2, total.p));
+        const uint32_t *tileSorted = alt2 ? pairTileAlt.p : pairTile.p;
+        tileList = reinterpret_cast<const int *>(alt2 ? pairGaussAlt.p : pairGauss.p);
+        U3D_TRY(launchTileOffsets(s, tileSorted, pairCap, totalTiles, tileOffsets.p, total.p));
+        if (heavyFirst) {
+            U3D_TRY(launchBlockCosts(s, dItems, numItems, totalBlocks, st.tile_size, tileOffsets.p,
+                                     blockKey.p, blockOrder.p));
+            bool alt3 = false;
+            U3D_TRY(radixSortPairs(s, blockKey.p, blockOrder.p, blockKeyAlt.p, blockOrderAlt.p,
+                                   totalBlocks, 8, rs, &alt3));
+            order = alt3 ? blockOrderAlt.p : blockOrder.p;
+        }
+    }
+    const int perCu = workgroupsPerCu > 0 ? workgroupsPerCu : 0;
+    {
+        StageScope scope(ctx, U3D_STAGE_RASTER_FWD);
+        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(counters.p), 0, 2));
+        const int occ = perCu > 0 ? perCu : occFwd;
+        const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
+        U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
+                                    int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
+                                    st, nullptr));
+    }
+    {
+        StageScope scope(ctx, U3D_STAGE_LOSS);
+        U3D_TRY(launchPhotometricLossBatched(s, dLoss, numItems, maxLossBlocks,
+                                             key.empty() ? 0.2f : key[0].t->opt.ssim_lambda));
+    }
+    {
+        StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
+        const int occ = perCu > 0 ? perCu : occBwd;
+        const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
+        U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks, counters.p + 1,
+                                    int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
+                                    st, grad2d.p));
+    }
+    {
+        StageScope scope(ctx, U3D_STAGE_CHAIN);
+        U3D_TRY(launchChainBatched(s, dItems, dGroups, numGroups, maxN, st, sb, grad2d.p));
+    }
+    {
+        StageScope scope(ctx, U3D_STAGE_ADAM);
+        U3D_TRY(launchAdamBatched(s, dSteps, numGroups, maxN));
+    }
+    return U3D_OK;
+}
+
+int BatchEngine::capture()
+{
+    dropGraph();
+    const hipStream_t s = ctx->stream;
+    const bool timing = ctx->timer.enabled;
+    ctx->timer.enabled = false; // no event records inside the graph
+    U3D_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    ctx->timer.enabled = timing;
+    if (rc != U3D_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    U3D_HIP(e);
+    graph = g;
+    U3D_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return U3D_OK;
+}
+
+// Reads the pair counts and overflow flags of finished steps; asks for a larger capacity when a
+// step came within 10% of it.
+int BatchEngine::poll(bool wait)
+{
+    for (int k = 0; k < kRing; ++k) {
+        if (statusStep[k] < 0)
+            continue;
+        if (wait)
+            U3D_HIP(hipEventSynchronize(statusDone[k]));
+        else if (hipEventQuery(statusDone[k]) != hipSuccess)
+            continue;
+        const int64_t K = int64_t(statusHost[2 * k]);
+        const bool overflow = statusHost[2 * k + 1] != 0;
+        for (u3d_trainer *t : statusScenes[k]) {
+            if (statusScenes[k].size() == 1)
+                t->counters[3] += K;
+            if (overflow)
+                t->counters[5] += 1;
+        }
+        statusStep[k] = -1;
+        if (double(K) > 0.9 * double(pairCap)) {
+            grownCap = std::max(grownCap, int64_t(double(K) * 1.3) + 65536);
+            needRelayout = true;
+        }
+    }
+    return U3D_OK;
+}
+
+int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
+{
+    const hipStream_t s = ctx->stream;
+    std::vector<u3d_trainer *> act;
+    for (int k = 0; k < numScenes; ++k) {
+        u3d_trainer *t = scenes[k];
+        if (t && !t->cams.empty() && t->iteration < t->opt.iterations)
+            act.push_back(t);
+    }
+    if (act.empty())
+        return U3D_OK;
+    U3D_TRY(poll(false));
+    std::vector<Member> now;
+    for (u3d_trainer *t : act)
+        now.push_back({t, t->params.n, t->cams[0].width, t->cams[0].height, t->viewsPerStep,
+                       t->shDegree});
+    if (needRelayout || !(now == key)) {
+        U3D_TRY(poll(true));
+        U3D_TRY(relayout(act));
+    }
+    // This step's table: wait until the slot's previous upload has been consumed.
+    const int slot = ring;
+    if (slotUsed[slot])
+        U3D_HIP(hipEventSynchronize(slotDone[slot]));
+    unsigned char *host = pinned + size_t(slot) * pinnedSlot;
+    U3D_TRY(fillTable(act, host));
+    U3D_HIP(hipMemcpyAsync(table.p, host, tableBytes, hipMemcpyHostToDevice, s));
+    U3D_HIP(hipEventRecord(slotDone[slot], s));
+    slotUsed[slot] = true;
+    if (totalEntries > 0) {
+        if (pairCap == 0)
+            U3D_TRY(probePairs());
+        if (useGraph) {
+            if (!exec)
+                U3D_TRY(capture());
+            StageScope scope(ctx, U3D_STAGE_GRAPH);
+            U3D_HIP(hipGraphLaunch(exec, s));
+        } else {
+            U3D_TRY(enqueue());
+        }
+        if (statusStep[slot] >= 0)
+            U3D_TRY(poll(true));
+        U3D_HIP(hipMemcpyAsync(statusHost + 2 * slot, total.p, 2 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+        U3D_HIP(hipEventRecord(statusDone[slot], s));
+        statusStep[slot] = stepNo;
+        statusScenes[slot] = act;
+    }
+    ring = (ring + 1) % kRing;
+    ++stepNo;
+    for (u3d_trainer *t : act) {
+        const int64_t B = t->viewsPerStep;
+        t->counters[0] += 1;
+        t->counters[1] += t->params.n * B;
+        t->counters[2] += int64_t(t->cams[0].width) * t->cams[0].height * B;
+        t->counters[4] += B;
+        U3D_TRY(trainerFinishStep(t));
+    }
+    return U3D_OK;
+}
+
+int createBatchEngine(u3d_ctx *ctx, bool useGraph, BatchEngine **out)
+{
+    std::unique_ptr<BatchEngine> e(new BatchEngine(ctx, useGraph));
+    U3D_TRY(e->init());
+    *out = e.release();
+    return U3D_OK;
+}
+
+void destroyBatchEngine(BatchEngine *e) { delete e; }
+
+int batchTrainStep(BatchEngine *engine, u3d_trainer *const *scenes, int numScenes)
+{
+    return engine->step(scenes, numScenes);
+}
+
+namespace {
+size_t trainerBytes(const u3d_trainer *t)
+{
+    size_t b = (t->params.buf.cap + t->spare.buf.cap + t->grads.buf.cap + t->adamM.buf.cap
+                + t->adamV.buf.cap) * sizeof(float);
+    b += t->targets.cap + (t->render.cap + t->dloss.cap + t->gradAccum.cap) * sizeof(float);
+    b += t->gradCount.cap * sizeof(int);
+    b += (t->shP.cap + t->shG.cap + t->shM.cap + t->shV.cap + t->shSpare.cap) * sizeof(float);
+    return b;
+}
+} // namespace
+
+} // namespace u3d
+
+using namespace u3d;
+
+struct u3d_scene_set {
+    u3d_ctx *ctx = nullptr;
+    std::unique_ptr<BatchEngine> engine;
+    std::vector<u3d_trainer *> members;
+    size_t budget = 0;
+    size_t memberBytes = 0;
+    size_t engineEstimate = 0;
+};
+
+extern "C" {
+
+int u3d_scene_set_create(u3d_ctx *ctx, int use_graph, size_t hbm_budget_bytes, u3d_scene_set **out)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(out != nullptr, U3D_ERR_INVALID, "null out");
+    std::unique_ptr<u3d_scene_set> set(new u3d_scene_set());
+    set->ctx = ctx;
+    BatchEngine *e = nullptr;
+    U3D_TRY(createBatchEngine(ctx, use_graph != 0, &e));
+    set->engine.reset(e);
+    if (hbm_budget_bytes == 0) {
+        size_t freeB = 0, totalB = 0;
+        U3D_HIP(hipMemGetInfo(&freeB, &totalB));
+        hbm_budget_bytes = size_t(double(freeB) * 0.9);
+    }
+    set->budget = hbm_budget_bytes;
+    *out = set.release();
+    return U3D_OK;
+}
+
+void u3d_scene_set_destroy(u3d_scene_set *set)
+{
+    if (!set)
+        return;
+    (void)hipSetDevice(set->ctx->device);
+    (void)hipStreamSynchronize(set->ctx->stream);
+    for (u3d_trainer *t : set->members)
+        u3d_trainer_destroy(t);
+    delete set;
+}
+
+int u3d_scene_set_add(u3d_scene_set *set, u3d_trainer *scene)
+{
+    U3D_CHECK(set && scene, U3D_ERR_INVALID, "null argument");
+    U3D_CHECK(scene->ctx == set->ctx, U3D_ERR_INVALID, "scene belongs to another context");
+    for (u3d_trainer *t : set->members)
+        U3D_CHECK(t != scene, U3D_ERR_INVALID, "scene already in the set");
+    const size_t own = trainerBytes(scene);
+    size_t pix = 0;
+    for (const u3d_camera &c : scene->cams)
+        pix = std::max(pix, size_t(c.width) * size_t(c.height));
+    const size_t work = BatchEngine::estimateBytes(std::max<int64_t>(scene->params.cap, 1),
+                                                   scene->viewsPerStep, pix);
+    U3D_CHECK(set->memberBytes + set->engineEstimate + own + work <= set->budget, U3D_ERR_CAPACITY,
+              "scene does not fit the scene set's HBM budget");
+    destroyBatchEngine(scene->engine); // the set's engine trains it from now on
+    scene->engine = nullptr;
+    set->members.push_back(scene);
+    set->memberBytes += own;
+    set->engineEstimate += work;
+    return U3D_OK;
+}
+
+int u3d_scene_set_size(u3d_scene_set *set) { return set ? int(set->members.size()) : 0; }
+
+u3d_trainer *u3d_scene_set_member(u3d_scene_set *set, int k)
+{
+    if (!set || k < 0 || k >= int(set->members.size()))
+        return nullptr;
+    return set->members[size_t(k)];
+}
+
+int u3d_scene_set_step(u3d_scene_set *set)
+{
+    U3D_CHECK(set != nullptr, U3D_ERR_INVALID, "null scene set");
+    U3D_TRY(useDevice(set->ctx));
+    return set->engine->step(set->members.data(), int(set->members.size()));
+}
+
+int u3d_scene_set_done(u3d_scene_set *set)
+{
+    if (!set)
+        return 1;
+    for (u3d_trainer *t : set->members)
+        if (!u3d_trainer_done(t))
+            return 0;
+    return 1;
+}
+
+size_t u3d_scene_set_bytes(u3d_scene_set *set)
+{
+    if (!set)
+        return 0;
+    return set->memberBytes + std::max(set->engineEstimate, set->engine->workingBytes());
+}
+
+int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out)
+{
+    U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
+    U3D_CHECK(t->engine != nullptr, U3D_ERR_INVALID, "trainer does not use the batched engine");
+    out->heavy_first = t->engine->heavyFirst;
+    out->workgroups_per_cu = t->engine->workgroupsPerCu;
+    return U3D_OK;
+}
+
+namespace {
+int applyTuning(BatchEngine *e, const u3d_batch_tuning *tuning)
+{
+    U3D_CHECK(tuning->workgroups_per_cu >= 0 && tuning->workgroups_per_cu <= 64, U3D_ERR_INVALID,
+              "workgroups_per_cu must be 0..64");
+    return e->setTuning(tuning->heavy_first, tuning->workgroups_per_cu);
+}
+} // namespace
+
+int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning)
+{
+    U3D_CHECK(t && tuning, U3D_ERR_INVALID, "null argument");
+    U3D_CHECK(t->engine != nullptr, U3D_ERR_INVALID, "trainer does not use the batched engine");
+    return applyTuning(t->engine, tuning);
+}
+
+int u3d_scene_set_set_tuning(u3d_scene_set *set, const u3d_batch_tuning *tuning)
+{
+    U3D_CHECK(set && tuning, U3D_ERR_INVALID, "null argument");
+    return applyTuning(set->engine.get(), tuning);
+}
+
+} // extern "C"

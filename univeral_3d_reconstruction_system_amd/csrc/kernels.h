@@ -19,6 +19,49 @@ struct ScreenBuffers {
     uint32_t *nTiles;   // tiles covered, 0 when not visible
 };
 
+// One (scene, view) render of a batch: several views of one scene and/or views of several
+// scenes share one projection pass, one sort and one persistent compositing launch. Screen
+// entries, tile ids and raster blocks of all items are concatenated; each item records its
+// offsets into them and the scene arrays it reads and writes.
+struct RenderItem {
+    u3d_camera cam;
+    int64_t screenBase; // first screen entry (Gaussian i of the item is entry screenBase + i)
+    int64_t n;          // Gaussians of the scene
+    int64_t blockBase;  // first global 16x16 raster block
+    int tileBase, tilesX, tilesY, blocksX;
+    int shDegree;
+    int pad;
+    const float *means, *scales, *quats, *opacities, *colors, *shRest;
+    float *rgb, *finalT;
+    int *nContrib;
+    const float *dl;
+    float *gMeans, *gScales, *gQuats, *gOpac, *gColors, *gShRest, *ndc;
+    int *gradCount;
+};
+
+// Loss of one item of a batched step (target = uint8 / 255, HWC).
+struct LossItem {
+    const float *render;
+    const uint8_t *target;
+    float *grad;       // dL/drgb of the item, scaled by gradScale
+    float *result;     // {total, l1, ssim, psnr}
+    double *partials;  // 9 doubles per 32x16 block
+    int width, height, blocksX, numBlocks;
+    float invCount, gradScale;
+};
+
+// One scene's optimiser step in a batched step. Parameter-shaped buffers use the packed
+// layout [means 3c | scales 3c | quats 4c | opacities c | colours 3c] for capacity c.
+struct SceneStep {
+    float *params, *grads, *adamM, *adamV;
+    int64_t n, cap;
+    float stepSize[5];
+    float invSqrtBc2;
+    float *shP, *shG, *shM, *shV; // 45 floats per Gaussian when shDegree > 0
+    float shStepSize;
+    int shDegree;
+};
+
 // ---- kernels_project.hip
 // shRest (nullable): 15 x 3 SH coefficients per Gaussian used up to shDegree (1..3).
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
@@ -30,6 +73,15 @@ int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam
                     const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
                     u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
                     int shDegree, float *gShRest);
+// Batched projection over items (grid y = item); entryItem[e] = owning item of screen entry e.
+int launchProjectBatched(hipStream_t s, const RenderItem *items, int numItems, int64_t maxN,
+                         const u3d_raster_settings &st, const ScreenBuffers &out, int *entryItem);
+// Batched chain rule. groups[g] = {first item, item count} of scene g (its items are
+// consecutive); each thread adds the contributions of all items of its scene in item order
+// (deterministic, no atomics) and clears the 2D gradients it consumed.
+int launchChainBatched(hipStream_t s, const RenderItem *items, const int2 *groups, int numGroups,
+                       int64_t maxN, const u3d_raster_settings &st, const ScreenBuffers &sb,
+                       float *grad2d);
 // Unpacks the records into the separate views of u3d_raster_state.
 int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float *mean2d,
                        float *conic, float *color, float *opacity);
@@ -46,7 +98,7 @@ struct RadixScratch {
 };
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
-                   bool *resultInAlt);
+                   bool *resultInAlt, const uint32_t *countDev = nullptr);
 size_t radixHistSize(int64_t count);
 size_t radixScanScratchSize(int64_t count);
 // values[i] = i
@@ -63,9 +115,21 @@ size_t scanScratchSize(int64_t count);
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
                     const int4 *rect, int tilesX, uint32_t *pairTile, uint32_t *pairGauss,
                     const uint32_t *nTilesSorted);
+// Batched emission: tile ids are global (item tileBase + local tile), values are screen entries.
+// Pairs at or past `capacity` are dropped (the farthest in depth order) and *overflow is set.
+int launchEmitPairsBatched(hipStream_t s, int64_t n, const uint32_t *sorted,
+                           const uint32_t *offsets, const int4 *rect, const int *entryItem,
+                           const RenderItem *items, uint32_t *pairTile, uint32_t *pairGauss,
+                           const uint32_t *nTilesSorted, int64_t capacity, unsigned *overflow);
+// Work order for the persistent compositing launch: keys[b] = 8-bit inverted log-scale length
+// of block b's tile list, order[b] = b; a one-pass (8-bit) radix sort then lists heavy blocks
+// first.
+int launchBlockCosts(hipStream_t s, const RenderItem *items, int numItems, int64_t totalBlocks,
+                     int tileSize, const int *tileOffsets, uint32_t *keys, uint32_t *order);
 // tileOffsets[t] = first pair index of tile t, tileOffsets[tiles] = K (tile ids sorted).
+// With countDev, the pair count is min(*countDev, numPairs) read on the device.
 int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs, int tiles,
-                      int *tileOffsets);
+                      int *tileOffsets, const uint32_t *countDev = nullptr);
 
 // ---- kernels_raster.hip
 struct RasterGeometry {
@@ -79,6 +143,16 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
                          const float *dlDrgb, const float *finalT, const int *nContrib,
                          float *grad2d);
 
+// Persistent compositing over the blocks of all items in `order` (nullptr = natural order);
+// `counter` must be zero on entry (one unsigned).
+int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
+                        const uint32_t *order, int64_t totalBlocks, unsigned *counter,
+                        int numWorkgroups, const int *tileOffsets, const int *tileList,
+                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d);
+
+// Resident workgroups per CU of the persistent compositing kernel (occupancy limit).
+int rasterBatchedOccupancy(bool backward);
+
 // ---- kernels_loss.hip
 // Fused L1 + D-SSIM value and gradient. target is float (targetU8 == nullptr) or uint8/255.
 // Writes per-block double partials (l1, ssim, squared error) into partials and the final
@@ -87,6 +161,9 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
                           const uint8_t *targetU8, int width, int height, float lambda,
                           float *grad, double *partials, size_t partialsCap, float *result);
 size_t lossPartialsSize(int width, int height);
+void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks);
+int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
+                                 int maxBlocks, float lambda);
 // Squared-error sum of two float arrays -> result[0] = psnr (double partials scratch).
 int launchPsnr(hipStream_t s, const float *a, const float *b, const uint8_t *bU8, int64_t count,
                double *partials, size_t partialsCap, float *result);
@@ -107,6 +184,8 @@ int launchAdamArray(hipStream_t s, float *p, const float *g, float *m, float *v,
 int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, int64_t n,
                       const uint8_t *keep, const uint8_t *kind, const uint32_t *keepPos,
                       const uint32_t *extraPos, const uint32_t *counts, bool zeroRows);
+// Adam over every scene of a batched step (blockIdx.y = scene); clears the gradients it read.
+int launchAdamBatched(hipStream_t s, const SceneStep *scenes, int numScenes, int64_t maxN);
 int launchFill(hipStream_t s, float *p, float value, int64_t count);
 int launchFillI32(hipStream_t s, int *p, int value, int64_t count);
 // o = min(o, cap) (Trainer.cpp:171-178)

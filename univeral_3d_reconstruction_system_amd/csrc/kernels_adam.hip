@@ -42,6 +42,41 @@ __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSq
     }
 }
 
+__device__ __forceinline__ void adamUpdate(float *__restrict__ p, float *__restrict__ g,
+                                           float *__restrict__ m, float *__restrict__ v,
+                                           int64_t len, float stepSize, float invSqrtBc2,
+                                           int64_t start, int64_t stride)
+{
+    for (int64_t k = start; k < len; k += stride) {
+        const float gk = g[k];
+        const float mk = 0.9f * m[k] + (1.0f - 0.9f) * gk;
+        const float vk = 0.999f * v[k] + (1.0f - 0.999f) * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        p[k] = p[k] - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
+        g[k] = 0.0f;
+    }
+}
+
+// Batched Adam: scene blockIdx.y; the same update as adamKernel, and the gradients are cleared
+// for the next step's accumulation.
+__global__ __launch_bounds__(kThreads) void adamBatchedKernel(const SceneStep *__restrict__ scenes)
+{
+    const SceneStep &sc = scenes[blockIdx.y];
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    const int64_t start = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    const int64_t c = sc.cap, n = sc.n;
+    const int64_t off[5] = {0, 3 * c, 6 * c, 10 * c, 11 * c};
+    const int64_t len[5] = {3 * n, 3 * n, 4 * n, n, 3 * n};
+#pragma unroll
+    for (int grp = 0; grp < 5; ++grp)
+        adamUpdate(sc.params + off[grp], sc.grads + off[grp], sc.adamM + off[grp],
+                   sc.adamV + off[grp], len[grp], sc.stepSize[grp], sc.invSqrtBc2, start, stride);
+    if (sc.shDegree > 0)
+        adamUpdate(sc.shP, sc.shG, sc.shM, sc.shV, 45 * n, sc.shStepSize, sc.invSqrtBc2, start,
+                   stride);
+}
+
 __global__ void fillKernel(float *p, float value, int64_t count)
 {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
@@ -340,6 +375,18 @@ int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_
     if (blocks > 4096)
         blocks = 4096;
     hipLaunchKernelGGL(adamKernel, dim3(blocks), dim3(kThreads), 0, s, a, invSqrtBc2);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchAdamBatched(hipStream_t s, const SceneStep *scenes, int numScenes, int64_t maxN)
+{
+    if (numScenes <= 0 || maxN <= 0)
+        return U3D_OK;
+    int64_t blocks = (4 * maxN + kThreads - 1) / kThreads;
+    blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+    hipLaunchKernelGGL(adamBatchedKernel, dim3(unsigned(blocks), numScenes), dim3(kThreads), 0, s,
+                       scenes);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -44,14 +44,16 @@ __device__ __forceinline__ double blockSumD(double v, double *red)
     return s;
 }
 
-template <bool kU8>
-__global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__ render,
-                                                       const float *__restrict__ targetF,
-                                                       const uint8_t *__restrict__ targetU8,
-                                                       int W, int H, float lambda, float invCount,
-                                                       Window win, int blocksX,
-                                                       float *__restrict__ grad,
-                                                       double *__restrict__ partials)
+// One 32x16 block of one channel; writes its three double partials at partials[0..2].
+// gradScale multiplies the stored gradient (kScaled only: views averaged in a batched step).
+template <bool kU8, bool kScaled>
+__device__ __forceinline__ void lossBlock(const float *__restrict__ render,
+                                          const float *__restrict__ targetF,
+                                          const uint8_t *__restrict__ targetU8, int W, int H,
+                                          float lambda, float invCount, float gradScale,
+                                          const float *__restrict__ winW, int blocksX, int blockId,
+                                          int c, float *__restrict__ grad,
+                                          double *__restrict__ partials)
 {
     // The inputs are dead once the moments are blurred, so the last stage reuses their space
     // (58 KB of LDS in total).
@@ -65,9 +67,9 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
     __shared__ float sw[2 * kR + 1];
     const int tid = threadIdx.x;
     if (tid < 2 * kR + 1)
-        sw[tid] = win.w[tid];
+        sw[tid] = winW[tid];
     __syncthreads();
-    const int ox = (blockIdx.x % blocksX) * kTW, oy = (blockIdx.x / blocksX) * kTH;
+    const int ox = (blockId % blocksX) * kTW, oy = (blockId / blocksX) * kTH;
     const bool ssimOn = lambda > 0.0f;
     const float dLds = -lambda * invCount;
     const float l1Scale = (1.0f - lambda) * invCount;
@@ -78,7 +80,6 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
         return targetF[idx];
     };
     {
-        const int c = blockIdx.y; // one colour channel per workgroup
         if (ssimOn) {
             // 1. inputs with a 10-pixel halo
             for (int k = tid; k < kIH * kIW; k += kThreads) {
@@ -189,6 +190,8 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
                 }
                 gval += g[0] + 2.0f * r * g[1] + t * g[2];
             }
+            if (kScaled)
+                gval *= gradScale;
             grad[idx] = gval;
         }
         __syncthreads();
@@ -197,17 +200,42 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
     const double s2 = blockSumD(ssim, red);
     const double s3 = blockSumD(sq, red);
     if (tid == 0) {
-        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3] = s1;
-        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3 + 1] = s2;
-        partials[(size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3 + 2] = s3;
+        partials[0] = s1;
+        partials[1] = s2;
+        partials[2] = s3;
     }
 }
 
+template <bool kU8>
+__global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__ render,
+                                                       const float *__restrict__ targetF,
+                                                       const uint8_t *__restrict__ targetU8,
+                                                       int W, int H, float lambda, float invCount,
+                                                       Window win, int blocksX,
+                                                       float *__restrict__ grad,
+                                                       double *__restrict__ partials)
+{
+    lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win.w, blocksX,
+                          int(blockIdx.x), int(blockIdx.y), grad,
+                          partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+}
+
+// Batched: item blockIdx.z, channel blockIdx.y; blocks past the item's count exit.
+__global__ __launch_bounds__(kThreads) void lossBatchedKernel(const LossItem *__restrict__ items,
+                                                              float lambda, Window win)
+{
+    const LossItem &it = items[blockIdx.z];
+    if (int(blockIdx.x) >= it.numBlocks)
+        return;
+    lossBlock<true, true>(it.render, nullptr, it.target, it.width, it.height, lambda, it.invCount,
+                          it.gradScale, win.w, it.blocksX, int(blockIdx.x), int(blockIdx.y),
+                          it.grad, it.partials + (size_t(blockIdx.y) * it.numBlocks + blockIdx.x) * 3);
+}
+
 // Reduces per-block partials in a fixed order; result = {total, l1, ssim, psnr}.
-__global__ __launch_bounds__(kThreads) void lossFinalize(const double *__restrict__ partials,
-                                                         int numBlocks, float lambda,
-                                                         float invCount, double count,
-                                                         float *result)
+__device__ __forceinline__ void lossFinalizeBody(const double *__restrict__ partials,
+                                                 int numBlocks, float lambda, float invCount,
+                                                 double count, float *result)
 {
     __shared__ double red[kThreads / 64];
     double a = 0, b = 0, c = 0;
@@ -232,6 +260,22 @@ __global__ __launch_bounds__(kThreads) void lossFinalize(const double *__restric
         result[2] = ssim;
         result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
     }
+}
+
+__global__ __launch_bounds__(kThreads) void lossFinalize(const double *__restrict__ partials,
+                                                         int numBlocks, float lambda,
+                                                         float invCount, double count,
+                                                         float *result)
+{
+    lossFinalizeBody(partials, numBlocks, lambda, invCount, count, result);
+}
+
+__global__ __launch_bounds__(kThreads) void lossFinalizeBatched(const LossItem *__restrict__ items,
+                                                                float lambda)
+{
+    const LossItem &it = items[blockIdx.x];
+    lossFinalizeBody(it.partials, 3 * it.numBlocks, lambda, it.invCount,
+                     double(it.width) * double(it.height) * 3.0, it.result);
 }
 
 // Squared error / absolute error sums for evaluation passes.
@@ -320,6 +364,29 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
     }
     hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, partials, 3 * nb, lambda,
                        invCount, double(P * 3), result);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks)
+{
+    const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
+    *blocksX = bx;
+    *numBlocks = bx * by;
+}
+
+int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
+                                 int maxBlocks, float lambda)
+{
+    if (numItems <= 0)
+        return U3D_OK;
+    static const Window win = makeWindow();
+    if (maxBlocks > 0) {
+        hipLaunchKernelGGL(lossBatchedKernel, dim3(maxBlocks, 3, numItems), dim3(kThreads), 0, s,
+                           items, lambda, win);
+        U3D_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(lossFinalizeBatched, dim3(numItems), dim3(kThreads), 0, s, items, lambda);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -10,16 +10,14 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__global__ __launch_bounds__(kThreads) void projectKernel(
-    int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
+// Projects Gaussian i of a model and writes its screen state at entry o.
+__device__ __forceinline__ void projectBody(
+    int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
     int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
     float3 campos)
 {
-    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= n)
-        return;
     int radius = 0;
     uint8_t flags = 0;
     uint32_t key = 0xffffffffu, nt = 0;
@@ -80,49 +78,79 @@ __global__ __launch_bounds__(kThreads) void projectKernel(
             rgb[ch] = (col < 0.0f) ? 0.0f : col; // std::max(col, 0.0f)
         }
         const float alpha = 1.0f / (1.0f + expfExact(-opac[i]));
-        out.rec0[i] = make_float4(u, v, p.c * invDet, -p.b * invDet);
-        out.rec1[i] = make_float4(p.a * invDet, alpha, rgb[0], rgb[1]);
-        out.rec2[i] = rgb[2];
-        out.depth[i] = z;
-        out.rect[i] = make_int4(tx0, ty0, tx1, ty1);
+        out.rec0[o] = make_float4(u, v, p.c * invDet, -p.b * invDet);
+        out.rec1[o] = make_float4(p.a * invDet, alpha, rgb[0], rgb[1]);
+        out.rec2[o] = rgb[2];
+        out.depth[o] = z;
+        out.rect[o] = make_int4(tx0, ty0, tx1, ty1);
         radius = r;
         key = floatBits(z);
         nt = uint32_t(tx1 - tx0) * uint32_t(ty1 - ty0);
     } while (false);
-    out.radius[i] = radius;
-    out.flags[i] = flags;
-    out.depthKey[i] = key;
-    out.nTiles[i] = nt;
+    out.radius[o] = radius;
+    out.flags[o] = flags;
+    out.depthKey[o] = key;
+    out.nTiles[o] = nt;
 }
 
-__global__ __launch_bounds__(kThreads) void chainRuleKernel(
+__global__ __launch_bounds__(kThreads) void projectKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
+    const float *__restrict__ quats, const float *__restrict__ opac,
+    const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
+    int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
+    float3 campos)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    projectBody(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts, tilesX, tilesY,
+                out, shRest, shDegree, campos);
+}
+
+// Batched projection: item blockIdx.y, Gaussian i of its scene -> entry screenBase + i; also
+// records the owning item of every entry for the pair emission.
+__global__ __launch_bounds__(kThreads) void projectBatchedKernel(const RenderItem *__restrict__ items,
+                                                                 float nearPlane, float eps2d, int ts,
+                                                                 ScreenBuffers out, int *entryItem)
+{
+    const RenderItem &it = items[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= it.n)
+        return;
+    const int64_t o = it.screenBase + i;
+    entryItem[o] = int(blockIdx.y);
+    projectBody(i, o, it.means, it.scales, it.quats, it.opacities, it.colors, it.cam, nearPlane,
+                eps2d, ts, it.tilesX, it.tilesY, out, it.shRest, it.shDegree, cameraCentre(it.cam));
+}
+
+// Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o.
+__device__ __forceinline__ void chainBody(
+    int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
     const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
     float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
     const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest)
 {
-    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= n || radius[i] == 0)
+    if (radius[o] == 0)
         return;
-    if (gradCount && radius[i] > 0)
+    if (gradCount && radius[o] > 0)
         gradCount[i] += 1;
     float g[kGrad2dStride];
 #pragma unroll
     for (int k = 0; k < kGrad2dStride; ++k)
-        g[k] = grad2d[i * kGrad2dStride + k];
+        g[k] = grad2d[o * kGrad2dStride + k];
     const float dmx = g[0], dmy = g[1], dA = g[2], dB = g[3], dC = g[4];
     if (ndc) {
         const float nx = dmx * 0.5f * float(cam.width), ny = dmy * 0.5f * float(cam.height);
         ndc[i] += sqrtf(nx * nx + ny * ny);
     }
-    const uint8_t flags = flagsArr[i];
+    const uint8_t flags = flagsArr[o];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
         if (!(flags & (4 << ch)))
             gColors[i * 3 + ch] += kShC0 * g[5 + ch];
-    const float op = rec1[i].y;
+    const float op = rec1[o].y;
     gOpac[i] += g[8] * op * (1.0f - op);
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
@@ -244,6 +272,49 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     }
 }
 
+__global__ __launch_bounds__(kThreads) void chainRuleKernel(
+    int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
+    const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
+    const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
+    const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
+    float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
+    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    chainBody(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
+                     gMeans, gScales, gQuats, gOpac, gColors, ndc, gradCount, shRest, shDegree, campos,
+                     gShRest);
+}
+
+// Batched chain rule over scene groups: Gaussian i of scene blockIdx.y takes the 2D gradients
+// of each of the scene's items in item order, then clears the entries it consumed (so the next
+// step's backward accumulates into zeros without a separate clearing pass).
+__global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem *__restrict__ items,
+                                                               const int2 *__restrict__ groups,
+                                                               float nearPlane, float eps2d,
+                                                               ScreenBuffers sb, float *grad2d)
+{
+    const int2 grp = groups[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (grp.y <= 0 || i >= items[grp.x].n)
+        return;
+    for (int v = grp.x; v < grp.x + grp.y; ++v) {
+        const RenderItem &it = items[v];
+        const int64_t o = it.screenBase + i;
+        if (sb.radius[o] == 0)
+            continue;
+        chainBody(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
+                         sb.flags, sb.rec1, grad2d, it.gMeans, it.gScales, it.gQuats, it.gOpac,
+                         it.gColors, it.ndc, it.gradCount, it.shRest, it.shDegree,
+                         cameraCentre(it.cam), it.gShRest);
+#pragma unroll
+        for (int k = 0; k < kGrad2dStride; ++k)
+            grad2d[o * kGrad2dStride + k] = 0.0f;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void unpackKernel(int64_t n, ScreenBuffers sb,
                                                          float *mean2d, float *conic,
                                                          float *color, float *opacity)
@@ -292,6 +363,29 @@ int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
                        grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
                        cameraCentre(cam), gShRest);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchProjectBatched(hipStream_t s, const RenderItem *items, int numItems, int64_t maxN,
+                         const u3d_raster_settings &st, const ScreenBuffers &out, int *entryItem)
+{
+    if (numItems <= 0 || maxN <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(projectBatchedKernel, dim3(blocksFor(maxN), numItems), dim3(kThreads), 0, s,
+                       items, st.near_plane, st.eps2d, st.tile_size, out, entryItem);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchChainBatched(hipStream_t s, const RenderItem *items, const int2 *groups, int numGroups,
+                       int64_t maxN, const u3d_raster_settings &st, const ScreenBuffers &sb,
+                       float *grad2d)
+{
+    if (numGroups <= 0 || maxN <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(chainBatchedKernel, dim3(blocksFor(maxN), numGroups), dim3(kThreads), 0, s,
+                       items, groups, st.near_plane, st.eps2d, sb, grad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

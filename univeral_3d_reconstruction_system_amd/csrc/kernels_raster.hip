@@ -99,8 +99,9 @@ __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, i
     return mask;
 }
 
-__global__ __launch_bounds__(kThreads) void rasterForwardKernel(
-    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+// One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase.
+__device__ __forceinline__ void forwardBlock(
+    int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
@@ -115,10 +116,9 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 32)
         stab[tid] = kExp2Table[tid];
-    const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
     const int bx0 = bx * kEdge, by0 = by * kEdge;
     const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
-    const int tile = (by0 / ts) * tilesX + bx0 / ts;
+    const int tile = tileBase + (by0 / ts) * tilesX + bx0 / ts;
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
     const float fpx = float(px), fpy = float(py);
@@ -190,8 +190,8 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     }
 }
 
-__global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
-    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+__device__ __forceinline__ void backwardBlock(
+    int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
@@ -211,10 +211,9 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
         stab[tid] = kExp2Table[tid];
     for (int k = tid; k < kBatch * kGrad2dStride; k += kThreads)
         (&acc[0][0])[k] = 0.0f;
-    const int bx = blockIdx.x % blocksX, by = blockIdx.x / blocksX;
     const int bx0 = bx * kEdge, by0 = by * kEdge;
     const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
-    const int tile = (by0 / ts) * tilesX + bx0 / ts;
+    const int tile = tileBase + (by0 / ts) * tilesX + bx0 / ts;
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile];
     const size_t pix = inside ? size_t(py) * size_t(W) + size_t(px) : 0;
@@ -352,7 +351,110 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     }
 }
 
+__global__ __launch_bounds__(kThreads) void rasterForwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib)
+{
+    forwardBlock(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+                 tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT, nContrib);
+}
+
+__global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    const int *__restrict__ nContribArr, float *__restrict__ grad2d)
+{
+    backwardBlock(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+                  tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
+                  grad2d);
+}
+
+// Index of the item owning global raster block b (items sorted by blockBase).
+__device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items, int numItems,
+                                           int64_t b)
+{
+    int lo = 0, hi = numItems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].blockBase <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Persistent batched compositing: every workgroup takes the next block of the work order from
+// an atomic counter until the list is exhausted. The order lists heavy tiles first.
+template <bool kBackward>
+__global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
+    const RenderItem *__restrict__ items, int numItems, const uint32_t *__restrict__ order,
+    int64_t totalBlocks, unsigned *counter, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2,
+    const u3d_raster_settings st, float *__restrict__ grad2d)
+{
+    __shared__ int64_t sWork;
+    for (;;) {
+        if (threadIdx.x == 0)
+            sWork = int64_t(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int64_t w = sWork;
+        __syncthreads();
+        if (w >= totalBlocks)
+            break;
+        const int64_t b = order ? int64_t(order[w]) : w;
+        const int v = itemOfBlock(items, numItems, b);
+        const RenderItem &it = items[v];
+        const int local = int(b - it.blockBase);
+        const int bx = local % it.blocksX, by = local / it.blocksX;
+        if (!kBackward)
+            forwardBlock(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
+                         tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
+                         st.background[1], st.background[2], st.min_alpha,
+                         st.min_transmittance, it.rgb, it.finalT, it.nContrib);
+        else
+            backwardBlock(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
+                          tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
+                          st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
+                          it.nContrib, grad2d);
+    }
+}
+
 } // namespace
+
+int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
+                        const uint32_t *order, int64_t totalBlocks, unsigned *counter,
+                        int numWorkgroups, const int *tileOffsets, const int *tileList,
+                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d)
+{
+    if (totalBlocks <= 0)
+        return U3D_OK;
+    if (backward)
+        hipLaunchKernelGGL(rasterBatchedKernel<true>, dim3(numWorkgroups), dim3(kThreads), 0, s,
+                           items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
+                           sb.rec0, sb.rec1, sb.rec2, st, grad2d);
+    else
+        hipLaunchKernelGGL(rasterBatchedKernel<false>, dim3(numWorkgroups), dim3(kThreads), 0, s,
+                           items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
+                           sb.rec0, sb.rec1, sb.rec2, st, grad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int rasterBatchedOccupancy(bool backward)
+{
+    int blocks = 0;
+    const hipError_t e = backward
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true>, kThreads, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false>, kThreads, 0);
+    return (e == hipSuccess && blocks > 0) ? blocks : 1;
+}
 
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,

@@ -28,6 +28,16 @@ constexpr int kScanThreads = 1024;
 // large ones (amortises the 256-entry histogram each block writes).
 inline int itemsFor(int64_t count) { return count <= (int64_t(1) << 20) ? 4 : 8; }
 
+// Element count: the host value, or min(*countDev, host value) when the count is on the device
+// (the host value is then the capacity the grid was sized for).
+__device__ __forceinline__ int64_t deviceCount(int64_t count, const uint32_t *countDev)
+{
+    if (!countDev)
+        return count;
+    const int64_t c = int64_t(*countDev);
+    return c < count ? c : count;
+}
+
 __device__ __forceinline__ uint64_t laneMaskLt()
 {
     const unsigned lane = __lane_id();
@@ -48,9 +58,11 @@ __device__ __forceinline__ uint64_t matchDigit(uint32_t digit, uint64_t valid, i
 template <int kItems>
 __global__ __launch_bounds__(kRadixThreads) void radixUpsweep(const uint32_t *__restrict__ keys,
                                                               int64_t count, int shift, int bits,
-                                                              int numBlocks, uint32_t *hist)
+                                                              int numBlocks, uint32_t *hist,
+                                                              const uint32_t *__restrict__ countDev)
 {
     __shared__ uint32_t h[256];
+    count = deviceCount(count, countDev);
     const int tid = threadIdx.x;
     h[tid] = 0;
     __syncthreads();
@@ -136,8 +148,9 @@ template <int kItems>
 __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn, uint32_t *keysOut,
     uint32_t *valsOut, int64_t count, int shift, int bits, int numBlocks,
-    const uint32_t *__restrict__ hist)
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ countDev)
 {
+    count = deviceCount(count, countDev);
     __shared__ uint32_t waveCount[kWaves][256];
     __shared__ uint32_t gbase[256];
     const int tid = threadIdx.x;
@@ -289,9 +302,72 @@ __global__ void emitPairsKernel(int64_t n, const uint32_t *__restrict__ sorted,
         }
 }
 
-__global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t numPairs,
-                                  int tiles, int *tileOffsets)
+__global__ void emitPairsBatchedKernel(int64_t n, const uint32_t *__restrict__ sorted,
+                                       const uint32_t *__restrict__ offsets,
+                                       const int4 *__restrict__ rect, const int *__restrict__ entryItem,
+                                       const RenderItem *__restrict__ items, uint32_t *pairTile,
+                                       uint32_t *pairGauss, const uint32_t *__restrict__ nTilesSorted,
+                                       int64_t capacity, unsigned *overflow)
 {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t nt = nTilesSorted[i];
+    if (nt == 0)
+        return;
+    const uint32_t e = sorted[i];
+    const RenderItem &it = items[entryItem[e]];
+    const int4 r = rect[e];
+    int64_t o = int64_t(offsets[i]);
+    if (o + int64_t(nt) > capacity)
+        atomicOr(overflow, 1u);
+    for (int ty = r.y; ty < r.w; ++ty)
+        for (int tx = r.x; tx < r.z; ++tx) {
+            if (o < capacity) {
+                pairTile[o] = uint32_t(it.tileBase + ty * it.tilesX + tx);
+                pairGauss[o] = e;
+            }
+            ++o;
+        }
+}
+
+// key = inverted log-scale tile list length of the block's tile (8 bits), value = block id.
+__global__ void blockCostKernel(const RenderItem *__restrict__ items, int numItems,
+                                int64_t totalBlocks, int tileSize, const int *__restrict__ tileOffsets,
+                                uint32_t *keys, uint32_t *order)
+{
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= totalBlocks)
+        return;
+    int lo = 0, hi = numItems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].blockBase <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const RenderItem &it = items[lo];
+    const int local = int(b - it.blockBase);
+    const int bx = local % it.blocksX, by = local / it.blocksX;
+    const int tile = it.tileBase + (by * 16 / tileSize) * it.tilesX + bx * 16 / tileSize;
+    const uint32_t len = uint32_t(tileOffsets[tile + 1] - tileOffsets[tile]);
+    // 8-bit log-scale cost (exponent, 3 mantissa bits), inverted so that an ascending one-pass
+    // radix sort lists the heaviest blocks first.
+    uint32_t bucket = 0;
+    if (len > 0) {
+        const int lg = 31 - __clz(len);
+        const uint32_t frac = lg >= 3 ? (len >> (lg - 3)) & 7u : (len << (3 - lg)) & 7u;
+        bucket = uint32_t(lg) * 8u + frac;
+    }
+    keys[b] = 255u - bucket;
+    order[b] = uint32_t(b);
+}
+
+__global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t numPairs,
+                                  int tiles, int *tileOffsets, const uint32_t *__restrict__ countDev)
+{
+    numPairs = deviceCount(numPairs, countDev);
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i > numPairs)
         return;
@@ -315,7 +391,8 @@ size_t radixScanScratchSize(int64_t count) { return scanScratchSize(int64_t(radi
 namespace {
 template <int kItems>
 int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt,
-                int64_t count, int bits, RadixScratch &scratch, bool *resultInAlt)
+                int64_t count, int bits, RadixScratch &scratch, bool *resultInAlt,
+                const uint32_t *countDev)
 {
     const int numBlocks = blocksFor(count, kRadixThreads * kItems);
     U3D_CHECK(size_t(numBlocks) * 256 <= scratch.histCap, U3D_ERR_CAPACITY, "radix histogram scratch too small");
@@ -323,12 +400,12 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
     for (int shift = 0; shift < bits; shift += 8) {
         const int b = bits - shift < 8 ? bits - shift : 8;
         hipLaunchKernelGGL(radixUpsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
-                           count, shift, b, numBlocks, scratch.hist);
+                           count, shift, b, numBlocks, scratch.hist, countDev);
         U3D_LAUNCH_CHECK();
         U3D_TRY(exclusiveScanU32(s, scratch.hist, scratch.hist, int64_t(numBlocks) * 256, nullptr,
                                  scratch.scanTmp, scratch.scanTmpCap));
         hipLaunchKernelGGL(radixDownsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
-                           vin, kout, vout, count, shift, b, numBlocks, scratch.hist);
+                           vin, kout, vout, count, shift, b, numBlocks, scratch.hist, countDev);
         U3D_LAUNCH_CHECK();
         uint32_t *tk = kin, *tv = vin;
         kin = kout;
@@ -343,14 +420,16 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
 
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
-                   bool *resultInAlt)
+                   bool *resultInAlt, const uint32_t *countDev)
 {
     *resultInAlt = false;
     if (count <= 0 || bits <= 0)
         return U3D_OK;
     if (itemsFor(count) == 4)
-        return radixPasses<4>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
-    return radixPasses<8>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
+        return radixPasses<4>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt,
+                              countDev);
+    return radixPasses<8>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt,
+                          countDev);
 }
 
 int launchIota(hipStream_t s, uint32_t *vals, int64_t count)
@@ -407,11 +486,36 @@ int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint
     return U3D_OK;
 }
 
+int launchEmitPairsBatched(hipStream_t s, int64_t n, const uint32_t *sorted,
+                           const uint32_t *offsets, const int4 *rect, const int *entryItem,
+                           const RenderItem *items, uint32_t *pairTile, uint32_t *pairGauss,
+                           const uint32_t *nTilesSorted, int64_t capacity, unsigned *overflow)
+{
+    if (n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(emitPairsBatchedKernel, dim3(blocksFor(n, 256)), dim3(256), 0, s, n, sorted,
+                       offsets, rect, entryItem, items, pairTile, pairGauss, nTilesSorted, capacity,
+                       overflow);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchBlockCosts(hipStream_t s, const RenderItem *items, int numItems, int64_t totalBlocks,
+                     int tileSize, const int *tileOffsets, uint32_t *keys, uint32_t *order)
+{
+    if (totalBlocks <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(blockCostKernel, dim3(blocksFor(totalBlocks, 256)), dim3(256), 0, s, items,
+                       numItems, totalBlocks, tileSize, tileOffsets, keys, order);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
 int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs, int tiles,
-                      int *tileOffsets)
+                      int *tileOffsets, const uint32_t *countDev)
 {
     hipLaunchKernelGGL(tileOffsetsKernel, dim3(blocksFor(numPairs + 1, 256)), dim3(256), 0, s,
-                       pairTile, numPairs, tiles, tileOffsets);
+                       pairTile, numPairs, tiles, tileOffsets, countDev);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -1,0 +1,170 @@
+// Host-side state shared by the C ABI (api.cpp) and the batched engine (batch.cpp): the packed
+// device model, the rasterizer / optimiser / trainer objects and the trainer's step helpers.
+#pragma once
+#include <random>
+#include <vector>
+
+#include "kernels.h"
+
+namespace u3d {
+
+// Parameter-shaped device storage: [means 3c | scales 3c | quats 4c | opacities c | colors 3c]
+// for capacity c, so a whole model (or its gradients / moments) is one allocation.
+struct DevModel {
+    DevBuf<float> buf;
+    int64_t n = 0;
+    int64_t cap = 0;
+    int reserve(int64_t c)
+    {
+        if (c <= cap)
+            return U3D_OK;
+        U3D_TRY(buf.ensure(size_t(c) * 14));
+        cap = c;
+        return U3D_OK;
+    }
+    u3d_gaussians view() const
+    {
+        float *b = buf.p;
+        return {b, b + 3 * cap, b + 6 * cap, b + 10 * cap, b + 11 * cap, n};
+    }
+    int zero(hipStream_t s) const
+    {
+        if (cap > 0)
+            U3D_HIP(hipMemsetAsync(buf.p, 0, size_t(cap) * 14 * sizeof(float), s));
+        return U3D_OK;
+    }
+    void swap(DevModel &o)
+    {
+        std::swap(buf.p, o.buf.p);
+        std::swap(buf.cap, o.buf.cap);
+        std::swap(n, o.n);
+        std::swap(cap, o.cap);
+    }
+};
+
+inline int copyFields(hipStream_t s, const u3d_gaussians &src, const u3d_gaussians &dst, int64_t n,
+               hipMemcpyKind kind)
+{
+    if (n <= 0)
+        return U3D_OK;
+    const size_t f = sizeof(float);
+    U3D_HIP(hipMemcpyAsync(dst.means, src.means, size_t(n) * 3 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.scales, src.scales, size_t(n) * 3 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.quats, src.quats, size_t(n) * 4 * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.opacities, src.opacities, size_t(n) * f, kind, s));
+    U3D_HIP(hipMemcpyAsync(dst.colors, src.colors, size_t(n) * 3 * f, kind, s));
+    return U3D_OK;
+}
+
+inline int checkModel(const u3d_gaussians *m, const char *what)
+{
+    U3D_CHECK(m != nullptr, U3D_ERR_INVALID, std::string("null ") + what);
+    U3D_CHECK(m->n >= 0 && m->n < (int64_t(1) << 31), U3D_ERR_INVALID, std::string("bad size of ") + what);
+    if (m->n > 0)
+        U3D_CHECK(m->means && m->scales && m->quats && m->opacities && m->colors, U3D_ERR_INVALID,
+                  std::string("null field in ") + what);
+    return U3D_OK;
+}
+
+inline int bitsFor(int64_t values)
+{
+    int b = 1;
+    while ((int64_t(1) << b) < values)
+        ++b;
+    return b;
+}
+
+class BatchEngine;
+constexpr int kMaxViewsPerStep = 64;
+int createBatchEngine(u3d_ctx *ctx, bool useGraph, BatchEngine **out);
+void destroyBatchEngine(BatchEngine *e);
+
+} // namespace u3d
+
+struct u3d_raster {
+    u3d_ctx *ctx = nullptr;
+    int64_t n = 0;
+    int64_t numPairs = 0;
+    u3d_camera cam{};
+    u3d_raster_settings st{};
+    u3d::RasterGeometry geo{};
+    bool haveForward = false;
+    u3d::DevBuf<float4> rec0, rec1;
+    u3d::DevBuf<float> rec2, depth;
+    u3d::DevBuf<int> radius;
+    u3d::DevBuf<int4> rect;
+    u3d::DevBuf<uint8_t> flags;
+    u3d::DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
+    u3d::DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
+    u3d::DevBuf<uint32_t> hist, histScan, scanScratch, total;
+    u3d::DevBuf<int> tileOffsets;
+    const int *tileList = nullptr;
+    u3d::DevBuf<float> finalT, grad2d;
+    u3d::DevBuf<int> nContrib;
+    u3d::DevBuf<float> dbgMean2d, dbgConic, dbgColor, dbgOpacity;
+    u3d::ScreenBuffers screen() const
+    {
+        return {rec0.p, rec1.p, rec2.p, depth.p, radius.p, rect.p, flags.p, depthKey.p, nTiles.p};
+    }
+};
+
+struct u3d_adam {
+    u3d_ctx *ctx = nullptr;
+    u3d::DevModel m, v, spare;
+    int step = 0;
+};
+
+struct u3d_trainer {
+    u3d_ctx *ctx = nullptr;
+    u3d::DevModel params, spare, grads, adamM, adamV;
+    int adamStep = 0;
+    std::vector<u3d_camera> cams;
+    std::vector<size_t> imageOffset;
+    u3d::DevBuf<uint8_t> targets;
+    float extent = 1.0f;
+    u3d_train_options opt{};
+    u3d_raster raster;
+    std::mt19937 rng;
+    std::vector<int> order;
+    size_t orderPos = 0;
+    int iteration = 0;
+    bool opacityWasReset = false;
+    u3d::DevBuf<float> render, dloss, gradAccum, stepStats, evalOut;
+    u3d::DevBuf<int> gradCount;
+    u3d::DevBuf<double> partials;
+    int statsFolded = 0;
+    u3d_train_stats stats{};
+    // steps, Gaussians, pixels, pairs, views, steps whose pair list overflowed its capacity
+    int64_t counters[U3D_NUM_COUNTERS] = {};
+    // Spherical-harmonic coefficients beyond DC (extension), [cap][15][3].
+    int shDegree = 0;
+    u3d::DevBuf<float> shP, shG, shM, shV, shSpare;
+    // Batched training (extension): views per step, whole-step graph, the engine that runs it
+    // (owned by this trainer when it trains alone; null inside a scene set).
+    int viewsPerStep = 1;
+    bool useGraph = false;
+    u3d::BatchEngine *engine = nullptr;
+};
+
+namespace u3d {
+
+int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
+                      const u3d_raster_settings &st, float *rgb, const float *shRest = nullptr,
+                      int shDegree = 0);
+void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2);
+// Next view of the trainer's shuffled epoch order (Trainer.cpp:57-64).
+int trainerNextView(u3d_trainer *t);
+// Position learning rate of the current iteration (Trainer.cpp:82-84).
+float trainerMeansLr(const u3d_trainer *t);
+// ++iteration, then the densification / opacity-reset schedule (Trainer.cpp:94-101).
+int trainerFinishStep(u3d_trainer *t);
+// SH coefficients train at the colour rate times this scale (3DGS uses 1/20).
+inline float shRateScale(const u3d_trainer *t)
+{
+    return t->opt.sh_lr_scale > 0.0f ? t->opt.sh_lr_scale : 1.0f / 20.0f;
+}
+// One optimiser step of every trainer in `scenes` (viewsPerStep views each) through `engine`:
+// one projection, one sort and one persistent compositing launch per direction for all views.
+int batchTrainStep(BatchEngine *engine, u3d_trainer *const *scenes, int numScenes);
+
+} // namespace u3d

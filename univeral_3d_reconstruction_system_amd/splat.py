@@ -9,6 +9,7 @@ Names and argument meaning follow the reference classes: ``Rasterizer.forward/ba
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -19,10 +20,12 @@ FIELDS = (("means", 3), ("scales", 3), ("quats", 4), ("opacities", 1), ("colors"
 
 
 class Context:
-    """u3d_ctx: one HIP device and stream."""
+    """u3d_ctx: one HIP device and stream. Closing it first releases every library object
+    created on it that is still alive, so no C object outlives its context."""
 
     def __init__(self, device: int = 0):
         self.lib = _abi.load()
+        self._children = weakref.WeakSet()
         h = C.c_void_p()
         check(self.lib.u3d_ctx_create(device, C.byref(h)))
         self.h = h
@@ -30,8 +33,15 @@ class Context:
     def synchronize(self):
         check(self.lib.u3d_ctx_synchronize(self.h))
 
+    def _adopt(self, obj):
+        self._children.add(obj)
+
     def close(self):
         if getattr(self, "h", None):
+            # scene sets first: they destroy their members
+            kids = sorted(list(self._children), key=lambda o: not isinstance(o, SceneSet))
+            for obj in kids:
+                obj._release()
             self.lib.u3d_ctx_destroy(self.h)
             self.h = None
 
@@ -53,6 +63,7 @@ class DeviceArray:
         p = C.c_void_p()
         check(ctx.lib.u3d_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
         self.ptr = p.value
+        ctx._adopt(self)
 
     @staticmethod
     def from_numpy(ctx: Context, a: np.ndarray) -> "DeviceArray":
@@ -72,11 +83,14 @@ class DeviceArray:
         check(self.ctx.lib.u3d_memset(self.ctx.h, self.ptr, 0, self.nbytes))
         return self
 
+    def _release(self):
+        if getattr(self, "ptr", None) and self.ctx.h:
+            self.ctx.lib.u3d_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
     def __del__(self):
         try:
-            if self.ptr:
-                self.ctx.lib.u3d_free(self.ctx.h, self.ptr)
-                self.ptr = None
+            self._release()
         except Exception:
             pass
 
@@ -151,10 +165,16 @@ class Rasterizer:
         check(ctx.lib.u3d_raster_create(ctx.h, C.byref(h)))
         self.h = h
         self.n = 0
+        ctx._adopt(self)
+
+    def _release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.u3d_raster_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            self.ctx.lib.u3d_raster_destroy(self.h)
+            self._release()
         except Exception:
             pass
 
@@ -240,10 +260,16 @@ class Adam:
         h = C.c_void_p()
         check(ctx.lib.u3d_adam_create(ctx.h, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
+
+    def _release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.u3d_adam_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            self.ctx.lib.u3d_adam_destroy(self.h)
+            self._release()
         except Exception:
             pass
 
@@ -336,10 +362,16 @@ class Trainer:
                                          rgb.ctypes.data, len(cams), float(extent),
                                          C.byref(options), C.byref(h)))
         self.h = h
+        ctx._adopt(self)
+
+    def _release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.u3d_trainer_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            self.ctx.lib.u3d_trainer_destroy(self.h)
+            self._release()
         except Exception:
             pass
 
@@ -360,6 +392,28 @@ class Trainer:
         out = {f: np.zeros(n * d, np.float32) for f, d in FIELDS}
         check(self.ctx.lib.u3d_trainer_get_model(self.h, *(out[f].ctypes.data for f, _ in FIELDS)))
         return out
+
+    def moments(self) -> tuple:
+        """Adam moments (m, v), each a dict of the model fields."""
+        n = self.size()
+        m = np.zeros(14 * n, np.float32)
+        v = np.zeros(14 * n, np.float32)
+        check(self.ctx.lib.u3d_trainer_get_moments(self.h, m.ctypes.data, v.ctypes.data))
+        return _split_fields(m, n), _split_fields(v, n)
+
+    def counters(self) -> dict:
+        c = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
+        check(self.ctx.lib.u3d_trainer_counters(self.h, c))
+        return dict(zip(_abi.COUNTERS, list(c)))
+
+    def tuning(self) -> dict:
+        t = _abi.BatchTuning()
+        check(self.ctx.lib.u3d_trainer_get_tuning(self.h, C.byref(t)))
+        return dict(heavy_first=t.heavy_first, workgroups_per_cu=t.workgroups_per_cu)
+
+    def set_tuning(self, heavy_first: int = 1, workgroups_per_cu: int = 0):
+        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu))
+        check(self.ctx.lib.u3d_trainer_set_tuning(self.h, C.byref(t)))
 
     def sh_rest(self) -> np.ndarray:
         """SH coefficients [size x 15 x 3] (trainers created with sh_degree > 0)."""
@@ -390,3 +444,80 @@ class Trainer:
 
     def synchronize(self):
         self.ctx.synchronize()
+
+
+def _split_fields(flat: np.ndarray, n: int) -> dict:
+    out, o = {}, 0
+    for f, d in FIELDS:
+        out[f] = flat[o:o + d * n]
+        o += d * n
+    return out
+
+
+class _Member(Trainer):
+    """Borrowed handle of a scene-set member: the set owns (and destroys) the C object, and the
+    handle is valid while the set lives."""
+
+    def _release(self):
+        self.h = None
+
+    def __del__(self):
+        pass
+
+    def step(self):
+        raise RuntimeError("members of a scene set are stepped through the set")
+
+
+class SceneSet:
+    """Independent scenes trained together by the batched engine (u3d_scene_set)."""
+
+    def __init__(self, ctx: Context, use_graph: bool = True, hbm_budget_bytes: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.u3d_scene_set_create(ctx.h, int(bool(use_graph)), int(hbm_budget_bytes),
+                                           C.byref(h)))
+        self.h = h
+        self._members = []
+        ctx._adopt(self)
+
+    def _release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.u3d_scene_set_destroy(self.h)
+        self.h = None
+        for m in self._members:
+            m.h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    def add(self, trainer: Trainer) -> bool:
+        """Admits the trainer when it fits the HBM budget (the set then owns it)."""
+        rc = self.ctx.lib.u3d_scene_set_add(self.h, trainer.h)
+        if rc == 3:  # U3D_ERR_CAPACITY
+            return False
+        check(rc)
+        trainer.__class__ = _Member  # the set owns the C object now (no back reference: no cycle)
+        self._members.append(trainer)
+        return True
+
+    def __len__(self):
+        return int(self.ctx.lib.u3d_scene_set_size(self.h))
+
+    def member(self, k: int) -> Trainer:
+        return self._members[k]
+
+    def step(self):
+        check(self.ctx.lib.u3d_scene_set_step(self.h))
+
+    def done(self) -> bool:
+        return bool(self.ctx.lib.u3d_scene_set_done(self.h))
+
+    def bytes(self) -> int:
+        return int(self.ctx.lib.u3d_scene_set_bytes(self.h))
+
+    def set_tuning(self, heavy_first: int = 1, workgroups_per_cu: int = 0):
+        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu))
+        check(self.ctx.lib.u3d_scene_set_set_tuning(self.h, C.byref(t)))
