@@ -95,6 +95,10 @@ typedef struct {
     float sh_lr_scale;   /* rate of the SH coefficients = lr.colors * scale (0 -> 1/20) */
     int views_per_step;  /* B >= 1 views per optimiser step (0 -> 1) */
     int use_graph;       /* capture each step's device work as a hipGraph */
+    /* With B > 1: 1 scales the learning rates by sqrt(B) and the Adam decay rates to beta^B
+     * (the batch-size rule of Zhao et al., "On Scaling Up 3D Gaussian Splatting Training",
+     * 2024); 0 keeps them. */
+    int batch_scaling;
 } u3d_train_options;
 
 /* TrainStats (Trainer.h:39-44). */
@@ -234,6 +238,47 @@ int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out_host);   /* Trainer.cpp
 /* Mean absolute error over all training views (time-to-target metric). */
 int u3d_trainer_evaluate_l1(u3d_trainer *t, float *out_host);
 int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host);
+
+/* ------------------------------------------------------------------ seed-point initialisation */
+/* initGaussiansFromPoints (GaussianModel.cpp:68-144) on the device: xyz [3n] and rgb
+ * [rgb_count] are device arrays (rgb_count < 3n leaves the missing colours at 0.5, as the
+ * reference); `out` needs n entries of capacity (out->n >= n). The hash-grid neighbour search
+ * reproduces the reference's buckets, so the mean 3-NN squared distance is bit-identical. */
+int u3d_points_knn_init(u3d_ctx *ctx, const float *xyz, const float *rgb, int rgb_count,
+                        int64_t n, u3d_gaussians *out);
+
+/* Learned initialiser (no reference counterpart): an MLP 80 -> 64 -> 256 -> 256 -> 256 -> 16,
+ * GELU, BF16 weights and activations with FP32 accumulation on MFMA, trained on the device.
+ * Its 11 outputs correct the kNN initialisation of each point: 3 log-scales, 4 quaternion
+ * components, the opacity logit and 3 DC colour coefficients. */
+typedef struct u3d_init_net u3d_init_net;
+#define U3D_FEATURE_DIM 80
+#define U3D_NET_OUT 16
+/* Per-point network input [n x U3D_FEATURE_DIM] (device): offsets and distances of the 16
+ * nearest neighbours in units of the point's kNN scale, colour - 0.5, and the fraction of the
+ * cameras (host array, may be NULL) that see the point. */
+int u3d_points_features(u3d_ctx *ctx, const float *xyz, const float *rgb, int rgb_count,
+                        int64_t n, const u3d_camera *cameras_host, int n_cameras,
+                        float near_plane, float *features);
+int u3d_init_net_create(u3d_ctx *ctx, unsigned seed, u3d_init_net **out);
+void u3d_init_net_destroy(u3d_init_net *net);
+/* Parameters: per layer, W [out x in] row-major then b [out], FP32 master copies. */
+int64_t u3d_init_net_param_count(void);
+int u3d_init_net_get_params(u3d_init_net *net, float *params_host);
+int u3d_init_net_set_params(u3d_init_net *net, const float *params_host);
+/* Gradients of the last training step, same layout as the parameters. */
+int u3d_init_net_get_grads(u3d_init_net *net, float *grads_host);
+/* out [n x U3D_NET_OUT] (device) = network(features). */
+int u3d_init_net_forward(u3d_init_net *net, const float *features, int64_t n, float *out);
+/* One Adam step (lr; betas 0.9 / 0.999, eps 1e-8) on the mean over the n rows of
+ * sum_c w[c] (out[c] - target[c])^2 for the first 11 columns (targets [n x 16] device,
+ * weights16 host). *loss_host (nullable) receives the loss before the step. */
+int u3d_init_net_train_step(u3d_init_net *net, const float *features, const float *targets,
+                            const float *weights16_host, int64_t n, float lr, float *loss_host);
+/* kNN initialisation corrected by the network (net == NULL: the kNN initialisation). */
+int u3d_points_learned_init(u3d_ctx *ctx, u3d_init_net *net, const float *xyz, const float *rgb,
+                            int rgb_count, int64_t n, const u3d_camera *cameras_host,
+                            int n_cameras, float near_plane, u3d_gaussians *out);
 
 /* --------------------------------------------------------------------------- scene sets */
 /* Independent scenes trained together (no reference counterpart: the reference trains one

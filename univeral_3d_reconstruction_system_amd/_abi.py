@@ -51,7 +51,7 @@ class TrainOptions(C.Structure):
                 ("seed", C.c_uint), ("raster", RasterSettings),
                 # extensions: SH degree, SH rate scale, views per step, hipGraph capture
                 ("sh_degree", C.c_int), ("sh_lr_scale", C.c_float), ("views_per_step", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("batch_scaling", C.c_int)]
 
 
 class BatchTuning(C.Structure):
@@ -132,6 +132,17 @@ SIGNATURES = {
     "u3d_trainer_get_moments": (I, [P, P, P]),
     "u3d_trainer_get_tuning": (I, [P, C.POINTER(BatchTuning)]),
     "u3d_trainer_set_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "u3d_points_knn_init": (I, [P, P, P, I, I64, C.POINTER(Gaussians)]),
+    "u3d_points_features": (I, [P, P, P, I, I64, P, I, F, P]),
+    "u3d_init_net_create": (I, [P, C.c_uint, C.POINTER(P)]),
+    "u3d_init_net_destroy": (None, [P]),
+    "u3d_init_net_param_count": (I64, []),
+    "u3d_init_net_get_params": (I, [P, P]),
+    "u3d_init_net_set_params": (I, [P, P]),
+    "u3d_init_net_get_grads": (I, [P, P]),
+    "u3d_init_net_forward": (I, [P, P, I64, P]),
+    "u3d_init_net_train_step": (I, [P, P, P, P, I64, F, C.POINTER(F)]),
+    "u3d_points_learned_init": (I, [P, P, P, P, I, I64, P, I, F, C.POINTER(Gaussians)]),
     "u3d_scene_set_create": (I, [P, I, C.c_size_t, C.POINTER(P)]),
     "u3d_scene_set_destroy": (None, [P]),
     "u3d_scene_set_add": (I, [P, P]),
@@ -147,6 +158,10 @@ SIGNATURES = {
 STAGES = ("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam", "densify",
           "graph_step")
 U3D_NUM_STAGES = len(STAGES)
+FEATURE_DIM = 80
+NET_OUT = 16
+NET_LAYERS = ((80, 64), (64, 256), (256, 256), (256, 256), (256, 16))  # (in, out)
+NET_GELU = (False, True, True, True, False)
 COUNTERS = ("steps", "gaussians", "pixels", "pairs", "views", "overflow_steps")
 U3D_NUM_COUNTERS = len(COUNTERS)
 

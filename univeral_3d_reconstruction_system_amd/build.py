@@ -23,7 +23,8 @@ LIB = os.path.join(OUT_DIR, "libu3d_hip.so")
 ARCH = os.environ.get("U3D_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels_project.hip", "kernels_sort.hip", "kernels_raster.hip", "kernels_loss.hip",
-           "kernels_adam.hip", "api.cpp", "batch.cpp"]
+           "kernels_adam.hip", "kernels_points.hip", "kernels_mlp.hip", "api.cpp", "batch.cpp",
+           "initnet.cpp"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

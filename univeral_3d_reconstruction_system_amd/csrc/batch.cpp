@@ -302,6 +302,7 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             it.tilesY = tilesY;
             it.blocksX = blocksX;
             it.shDegree = t->shDegree;
+            it.ndcScale = float(B);
             it.means = pv.means;
             it.scales = pv.scales;
             it.quats = pv.quats;
@@ -340,12 +341,31 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             pix += size_t(W) * size_t(H);
             part += size_t(9) * size_t(lnb);
         }
-        // Optimiser coefficients of this step (Trainer.cpp:82-84, Adam.cpp:70-72).
+        // Optimiser coefficients of this step (Trainer.cpp:82-84, Adam.cpp:70-72); with batch
+        // scaling, rates x sqrt(B) and decay rates beta^B.
         u3d_learning_rates lr = t->opt.lr;
         lr.means = trainerMeansLr(t);
+        float b1 = 0.9f, b2 = 0.999f;
+        if (t->opt.batch_scaling && B > 1) {
+            const float r = std::sqrt(float(B));
+            lr.means *= r;
+            lr.scales *= r;
+            lr.quats *= r;
+            lr.opacities *= r;
+            lr.colors *= r;
+            b1 = std::pow(0.9f, float(B));
+            b2 = std::pow(0.999f, float(B));
+        }
         ++t->adamStep;
         SceneStep sc{};
-        adamCoefficients(t->adamStep, lr, sc.stepSize, &sc.invSqrtBc2);
+        const float bc1 = 1.0f - std::pow(b1, float(t->adamStep));
+        const float bc2 = 1.0f - std::pow(b2, float(t->adamStep));
+        const float rates[5] = {lr.means, lr.scales, lr.quats, lr.opacities, lr.colors};
+        for (int q = 0; q < 5; ++q)
+            sc.stepSize[q] = rates[q] / bc1;
+        sc.invSqrtBc2 = 1.0f / std::sqrt(bc2);
+        sc.beta1 = b1;
+        sc.beta2 = b2;
         sc.params = t->params.buf.p;
         sc.grads = t->grads.buf.p;
         sc.adamM = t->adamM.buf.p;
@@ -358,7 +378,6 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             sc.shG = t->shG.p;
             sc.shM = t->shM.p;
             sc.shV = t->shV.p;
-            const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
             sc.shStepSize = (lr.colors * shRateScale(t)) / bc1;
         }
         steps[g] = sc;

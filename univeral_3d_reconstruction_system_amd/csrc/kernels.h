@@ -30,7 +30,7 @@ struct RenderItem {
     int64_t blockBase;  // first global 16x16 raster block
     int tileBase, tilesX, tilesY, blocksX;
     int shDegree;
-    int pad;
+    float ndcScale;     // densification statistic per view: undoes the 1/B of the averaged loss
     const float *means, *scales, *quats, *opacities, *colors, *shRest;
     float *rgb, *finalT;
     int *nContrib;
@@ -60,6 +60,7 @@ struct SceneStep {
     float *shP, *shG, *shM, *shV; // 45 floats per Gaussian when shDegree > 0
     float shStepSize;
     int shDegree;
+    float beta1, beta2;           // Adam decay rates (0.9, 0.999 unless batch-scaled)
 };
 
 // ---- kernels_project.hip
@@ -152,6 +153,71 @@ int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, i
 
 // Resident workgroups per CU of the persistent compositing kernel (occupancy limit).
 int rasterBatchedOccupancy(bool backward);
+
+// ---- kernels_points.hip
+// Hash grid of seed points: point indices sorted by their 64-bit cell key (hi, lo halves).
+struct PointGrid {
+    const uint32_t *keyHi;
+    const uint32_t *keyLo;
+    const uint32_t *idx;
+};
+constexpr int kFeatureNeighbours = 16;
+constexpr int kFeatureDim = 80; // 16 x (offset, distance) + colour + view fraction, zero padded
+constexpr int kBoundsBlocks = 1024;
+// out6 = {min x, min y, min z, max x, max y, max z}; partial holds 6 * kBoundsBlocks floats.
+int launchPointBounds(hipStream_t s, const float *xyz, int64_t n, float *partial, float *out6);
+int launchCellKeys(hipStream_t s, const float *xyz, int64_t n, const float lo[3], float cell,
+                   uint32_t *keyLo, uint32_t *keyHi, uint32_t *idx);
+// initGaussiansFromPoints per point (GaussianModel.cpp:103-141) except the log-scales: writes
+// means, quaternions, opacities, colours and each point's mean squared distance to its 3
+// nearest neighbours (the caller turns those into log-scales with the host libm).
+int launchKnnInit(hipStream_t s, const float *xyz, const float *rgb, int rgbCount, int64_t n,
+                  const float lo[3], float cell, const PointGrid &g, float opacityLogit,
+                  u3d_gaussians &out, float *meanD2);
+int launchKnnFeatures(hipStream_t s, const float *xyz, const float *rgb, int rgbCount, int64_t n,
+                      const float lo[3], float cell, const PointGrid &g, const float *meanD2,
+                      const float *viewFrac, float *features);
+int launchViewFraction(hipStream_t s, const float *xyz, int64_t n, const u3d_camera *cams,
+                       int numCams, float nearPlane, float *out);
+
+// Adds the initialiser network's outputs (row stride ld: 3 log-scale, 4 quaternion, 1 opacity
+// logit, 3 DC colour corrections) to a kNN-initialised model.
+int launchApplyCorrections(hipStream_t s, const float *net, int ld, int64_t n, u3d_gaussians &m);
+
+// ---- kernels_mlp.hip
+// C[m][n] = sum_k A(m, k) B(k, n) on BF16 MFMA with FP32 accumulation; A(m, k) = A[m*sam + k*sak],
+// B(k, n) = B[k*sbk + n*sbn]. Epilogue (no split): + bias[n]; C (fp32, ldc) receives the value;
+// when gate is set the value is first multiplied by GELU'(gate[m][n]); Y (bf16, ldy) receives
+// act(value). With splitK > 1 every split atomically adds its partial sum into C (zeroed by
+// the caller) and no epilogue runs.
+struct GemmArgs {
+    const uint16_t *A;
+    int64_t sam, sak;
+    const uint16_t *B;
+    int64_t sbk, sbn;
+    int M, N, K;
+    float *C;
+    int64_t ldc;
+    const float *bias;
+    uint16_t *Y;
+    int64_t ldy;
+    int gelu;
+    const float *gate;
+    int64_t ldg;
+    int splitK;
+};
+int launchGemmBf16(hipStream_t s, const GemmArgs &a);
+// bf16 copy of an fp32 array (round to nearest even).
+int launchToBf16(hipStream_t s, const float *in, uint16_t *out, int64_t count);
+// Column sums: out[n] += sum_m in[m * ld + n], m < M, n < N (fp32 atomics).
+int launchColumnSums(hipStream_t s, const float *in, int64_t ld, int M, int N, float *out);
+// Masked mean squared error of the network output: y [M x ld] vs target [M x ld] over the
+// first `width` columns weighted by w[col]; writes dL/dy (fp32) and adds the loss to *loss.
+int launchMseLoss(hipStream_t s, const float *y, const float *target, const float *w, int M,
+                  int width, int64_t ld, float *dy, float *loss);
+// Adam over a flat fp32 array, then its bf16 copy.
+int launchAdamBf16(hipStream_t s, float *p, const float *g, float *m, float *v, int64_t n,
+                   float stepSize, float invSqrtBc2, uint16_t *pBf16);
 
 // ---- kernels_loss.hip
 // Fused L1 + D-SSIM value and gradient. target is float (targetU8 == nullptr) or uint8/255.

@@ -45,12 +45,12 @@ __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSq
 __device__ __forceinline__ void adamUpdate(float *__restrict__ p, float *__restrict__ g,
                                            float *__restrict__ m, float *__restrict__ v,
                                            int64_t len, float stepSize, float invSqrtBc2,
-                                           int64_t start, int64_t stride)
+                                           float b1, float b2, int64_t start, int64_t stride)
 {
     for (int64_t k = start; k < len; k += stride) {
         const float gk = g[k];
-        const float mk = 0.9f * m[k] + (1.0f - 0.9f) * gk;
-        const float vk = 0.999f * v[k] + (1.0f - 0.999f) * gk * gk;
+        const float mk = b1 * m[k] + (1.0f - b1) * gk;
+        const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
         m[k] = mk;
         v[k] = vk;
         p[k] = p[k] - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
@@ -71,10 +71,11 @@ __global__ __launch_bounds__(kThreads) void adamBatchedKernel(const SceneStep *_
 #pragma unroll
     for (int grp = 0; grp < 5; ++grp)
         adamUpdate(sc.params + off[grp], sc.grads + off[grp], sc.adamM + off[grp],
-                   sc.adamV + off[grp], len[grp], sc.stepSize[grp], sc.invSqrtBc2, start, stride);
+                   sc.adamV + off[grp], len[grp], sc.stepSize[grp], sc.invSqrtBc2, sc.beta1,
+                   sc.beta2, start, stride);
     if (sc.shDegree > 0)
-        adamUpdate(sc.shP, sc.shG, sc.shM, sc.shV, 45 * n, sc.shStepSize, sc.invSqrtBc2, start,
-                   stride);
+        adamUpdate(sc.shP, sc.shG, sc.shM, sc.shV, 45 * n, sc.shStepSize, sc.invSqrtBc2, sc.beta1,
+                   sc.beta2, start, stride);
 }
 
 __global__ void fillKernel(float *p, float value, int64_t count)

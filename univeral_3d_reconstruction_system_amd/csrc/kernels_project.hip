@@ -130,7 +130,7 @@ __device__ __forceinline__ void chainBody(
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
     const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
     float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
-    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest)
+    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest, float ndcScale)
 {
     if (radius[o] == 0)
         return;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void chainBody(
     const float dmx = g[0], dmy = g[1], dA = g[2], dB = g[3], dC = g[4];
     if (ndc) {
         const float nx = dmx * 0.5f * float(cam.width), ny = dmy * 0.5f * float(cam.height);
-        ndc[i] += sqrtf(nx * nx + ny * ny);
+        ndc[i] += sqrtf(nx * nx + ny * ny) * ndcScale;
     }
     const uint8_t flags = flagsArr[o];
 #pragma unroll
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
         return;
     chainBody(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
                      gMeans, gScales, gQuats, gOpac, gColors, ndc, gradCount, shRest, shDegree, campos,
-                     gShRest);
+                     gShRest, 1.0f);
 }
 
 // Batched chain rule over scene groups: Gaussian i of scene blockIdx.y takes the 2D gradients
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem 
         chainBody(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
                          sb.flags, sb.rec1, grad2d, it.gMeans, it.gScales, it.gQuats, it.gOpac,
                          it.gColors, it.ndc, it.gradCount, it.shRest, it.shDegree,
-                         cameraCentre(it.cam), it.gShRest);
+                         cameraCentre(it.cam), it.gShRest, it.ndcScale);
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
             grad2d[o * kGrad2dStride + k] = 0.0f;

@@ -454,6 +454,113 @@ def _split_fields(flat: np.ndarray, n: int) -> dict:
     return out
 
 
+def _cams_array(cams):
+    if not cams:
+        return None, 0
+    arr = (_abi.Camera * len(cams))(*[make_camera(c) for c in cams])
+    return arr, len(cams)
+
+
+def points_knn_init(ctx: Context, xyz: np.ndarray, rgb: np.ndarray | None = None) -> dict:
+    """initGaussiansFromPoints (GaussianModel.cpp:68-144) on the device; returns host arrays."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    n = len(xyz) // 3
+    rgb = np.zeros(0, np.float32) if rgb is None else np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    dx = DeviceArray.from_numpy(ctx, xyz)
+    dr = DeviceArray.from_numpy(ctx, rgb if len(rgb) else np.zeros(1, np.float32))
+    out = DeviceGaussians(ctx, n)
+    m = out.struct()
+    check(ctx.lib.u3d_points_knn_init(ctx.h, dx.ptr, dr.ptr, len(rgb), n, C.byref(m)))
+    return {k: v.copy() for k, v in out.to_host().items()}
+
+
+def points_features(ctx: Context, xyz, rgb, cams=None, near_plane: float = 0.01) -> np.ndarray:
+    """Learned-initialiser inputs [n x 80] (host copy)."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    n = len(xyz) // 3
+    dx, dr = DeviceArray.from_numpy(ctx, xyz), DeviceArray.from_numpy(ctx, rgb)
+    f = DeviceArray(ctx, (max(n, 1) * _abi.FEATURE_DIM,))
+    arr, nc = _cams_array(cams)
+    check(ctx.lib.u3d_points_features(ctx.h, dx.ptr, dr.ptr, len(rgb), n, arr, nc,
+                                      float(near_plane), f.ptr))
+    return f.numpy()[:n * _abi.FEATURE_DIM].reshape(n, _abi.FEATURE_DIM)
+
+
+class InitNet:
+    """The learned initialiser's network on the device (u3d_init_net)."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.u3d_init_net_create(ctx.h, int(seed), C.byref(h)))
+        self.h = h
+        self.count = int(ctx.lib.u3d_init_net_param_count())
+        ctx._adopt(self)
+
+    def _release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.u3d_init_net_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.count, np.float32)
+        check(self.ctx.lib.u3d_init_net_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.count
+        check(self.ctx.lib.u3d_init_net_set_params(self.h, p.ctypes.data))
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(self.count, np.float32)
+        check(self.ctx.lib.u3d_init_net_get_grads(self.h, out.ctypes.data))
+        return out
+
+    def forward(self, features: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, _abi.FEATURE_DIM)
+        n = len(f)
+        df = DeviceArray.from_numpy(self.ctx, f)
+        out = DeviceArray(self.ctx, (max(n, 1) * _abi.NET_OUT,))
+        check(self.ctx.lib.u3d_init_net_forward(self.h, df.ptr, n, out.ptr))
+        return out.numpy()[:n * _abi.NET_OUT].reshape(n, _abi.NET_OUT)
+
+    def train_step(self, features, targets, weights, lr: float) -> float:
+        """features / targets: DeviceArray (n x 80, n x 16) or numpy arrays."""
+        if isinstance(features, np.ndarray):
+            features = DeviceArray.from_numpy(self.ctx, np.ascontiguousarray(features, np.float32))
+        if isinstance(targets, np.ndarray):
+            targets = DeviceArray.from_numpy(self.ctx, np.ascontiguousarray(targets, np.float32))
+        n = features.nbytes // (4 * _abi.FEATURE_DIM)
+        w = np.ascontiguousarray(weights, np.float32).reshape(16)
+        loss = C.c_float()
+        check(self.ctx.lib.u3d_init_net_train_step(self.h, features.ptr, targets.ptr, w.ctypes.data,
+                                                   n, float(lr), C.byref(loss)))
+        return loss.value
+
+
+def learned_init(ctx: Context, net: "InitNet | None", xyz, rgb, cams=None,
+                 near_plane: float = 0.01) -> dict:
+    """kNN initialisation corrected by the network (net None: kNN only); host arrays."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    n = len(xyz) // 3
+    dx, dr = DeviceArray.from_numpy(ctx, xyz), DeviceArray.from_numpy(ctx, rgb)
+    out = DeviceGaussians(ctx, n)
+    m = out.struct()
+    arr, nc = _cams_array(cams)
+    check(ctx.lib.u3d_points_learned_init(ctx.h, net.h if net else None, dx.ptr, dr.ptr, len(rgb), n,
+                                          arr, nc, float(near_plane), C.byref(m)))
+    return {k: v.copy() for k, v in out.to_host().items()}
+
+
 class _Member(Trainer):
     """Borrowed handle of a scene-set member: the set owns (and destroys) the C object, and the
     handle is valid while the set lives."""
