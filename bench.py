@@ -110,6 +110,52 @@ def batched_rate(ctx, lib, init, cams, images, extent, views, steps, warmup):
                 tuning=t.tuning())
 
 
+def scene_set_rate(ctx, scenes: int, steps: int, warmup: int, seed: int = 3):
+    """Independent scenes of uneven size trained concurrently: N log-uniform in [10k, 300k],
+    image heights from {240, 360, 480, 640, 720} at 4:3, 24-200 ring cameras, the CPU-runnable
+    workload's Gaussian distributions; one view per scene per step, all scenes through one
+    projection, sort and persistent compositing launch per direction, replayed as a hipGraph."""
+    g = S.rng(seed)
+    sset = U.SceneSet(ctx, use_graph=True)
+    r = U.Rasterizer(ctx)
+    info = []
+    for k in range(scenes):
+        n = int(round(float(np.exp(g.uniform(np.log(1e4), np.log(3e5))))))
+        h = int(g.choice([240, 360, 480, 640, 720]))
+        w = int(round(4 * h / 3))
+        views = int(g.integers(24, 201))
+        cams = S.ring_cameras(views, w, h)
+        gt = U.DeviceGaussians.from_host(ctx, S.random_gaussians(n, 1000 + k))
+        out = U.DeviceArray(ctx, (w * h * 3,))
+        images = [S.quantize(r.forward(gt, c, out=out).numpy()) for c in cams]
+        del gt, out
+        opt = U.options_for_iterations(ITERATIONS)
+        opt.max_gaussians = n
+        opt.seed = 42 + k
+        t = U.Trainer(ctx, S.random_gaussians(n, 2000 + k), cams, images, S.scene_extent(cams), opt)
+        if not sset.add(t):
+            break
+        info.append(dict(gaussians=n, width=w, height=h, views=views))
+    for _ in range(warmup):
+        sset.step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sset.step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    rate = steps / dt
+    per = []
+    for k, d in enumerate(info):
+        m = sset.member(k)
+        per.append(dict(d, iters_per_s=round(rate, 2),
+                        gaussian_pixels_per_s=round(m.size() * d["width"] * d["height"] * rate, 1),
+                        overflow_steps=m.counters()["overflow_steps"]))
+    return dict(scenes=len(info), steps=steps, ms_per_step=round(1e3 * dt / steps, 4),
+                aggregate_gaussian_pixels_per_s=round(sum(p["gaussian_pixels_per_s"] for p in per), 1),
+                resident_bytes=sset.bytes(), per_scene=per)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +166,9 @@ def main(argv=None) -> int:
     ap.add_argument("--batched-views", type=int, default=8,
                     help="also time the batched engine with this many views per step and the "
                          "whole step captured as a hipGraph (0: skip)")
+    ap.add_argument("--scene-set", type=int, default=16,
+                    help="also time this many independent uneven scenes trained as one scene set "
+                         "(0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters FETCH_SIZE / WRITE_SIZE against a known traffic) and exit")
@@ -219,6 +268,9 @@ def main(argv=None) -> int:
     if a.batched_views > 0:
         batched = batched_rate(ctx, lib, init, cams, images, extent, a.batched_views,
                                max(1, a.steps // a.batched_views), max(1, a.warmup // a.batched_views))
+    multi = None
+    if a.scene_set > 0:
+        multi = scene_set_rate(ctx, a.scene_set, max(1, a.steps // 4), max(1, a.warmup // 4))
     base = None if a.no_cpu_baseline else cpu_baseline(init, cams, images, extent, opt, a.cpu_seconds)
     line = dict(
         metric="train iters/s @300k Gaussians 640px", value=round(value, 3), unit="iters/s",
@@ -235,6 +287,7 @@ def main(argv=None) -> int:
         gpu_vs_cpu=round(value / base["value"], 2) if base else None,
         train_psnr_ema=round(stats["psnr"], 3),
         batched=batched,
+        scene_set=multi,
     )
     print(json.dumps(line))
     return 0

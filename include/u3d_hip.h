@@ -280,6 +280,24 @@ int u3d_points_learned_init(u3d_ctx *ctx, u3d_init_net *net, const float *xyz, c
                             int rgb_count, int64_t n, const u3d_camera *cameras_host,
                             int n_cameras, float near_plane, u3d_gaussians *out);
 
+/* -------------------------------------------------------------------------- model files */
+/* writeSplatPly (SplatPly.h:12-13, SplatPly.cpp:261-281) from host arrays: binary little
+ * endian x y z nx ny nz f_dc_0..2 [f_rest_*] opacity scale_0..2 rot_0..3. With sh_degree 1..3
+ * the SH coefficients sh_rest [n][15][3] are written as f_rest_0 .. f_rest_{3K-1}
+ * (channel-major, K = 3, 8, 15), the usual 3DGS extension of the reference's layout. */
+int u3d_write_splat_ply(const char *path, const float *means_host, const float *scales_host,
+                        const float *quats_host, const float *opacities_host,
+                        const float *colors_host, int64_t n, const float *sh_rest_host,
+                        int sh_degree);
+/* readSplatPly for 3DGS files (binary little endian or ASCII; SplatPly.cpp:284-342): with
+ * capacity 0 only *n_out and *sh_degree_out are set; otherwise fills host arrays of at least
+ * *n_out entries (sh_rest [n][15][3] when the file has f_rest_*, missing ones zero). */
+int u3d_read_splat_ply(const char *path, int64_t capacity, float *means_host, float *scales_host,
+                       float *quats_host, float *opacities_host, float *colors_host,
+                       float *sh_rest_host, int64_t *n_out, int *sh_degree_out);
+/* writeSplatPly(trainer.model()) (Reconstructor.cpp:361): one copy from the device. */
+int u3d_trainer_export_ply(u3d_trainer *t, const char *path);
+
 /* --------------------------------------------------------------------------- scene sets */
 /* Independent scenes trained together (no reference counterpart: the reference trains one
  * scene, one view per step, Trainer.cpp:64). Each step of a set trains every member that is
@@ -307,16 +325,27 @@ int u3d_scene_set_done(u3d_scene_set *set); /* 1 when every member is done */
 size_t u3d_scene_set_bytes(u3d_scene_set *set);
 
 /* Tuning of the batched engine (trainers with views_per_step > 1 or use_graph, scene sets).
- * Any choice gives the same images and gradients up to float summation order, except
- * tile_size, which changes which Gaussians reach a tile exactly as RasterSettings::tileSize
- * does in the reference. */
+ * Every choice gives the same images and gradients up to float summation order, except
+ * tile_size, which changes which Gaussians reach a tile near the 3-sigma cut exactly as
+ * RasterSettings::tileSize does in the reference. */
 typedef struct {
     int heavy_first;        /* persistent compositing takes the longest tile lists first */
     int workgroups_per_cu;  /* persistent workgroups per CU (0: occupancy limit) */
+    int lds_batch;          /* tile-list records staged in LDS per round: 64, 128, 256 (0: 256) */
+    int sort_items;         /* radix sort keys per lane per block: 4 or 8 (0: by size) */
+    int tile_size;          /* 0: keep the raster settings' tile size */
 } u3d_batch_tuning;
 int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out);
 int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning);
 int u3d_scene_set_set_tuning(u3d_scene_set *set, const u3d_batch_tuning *tuning);
+/* Autotuner: times each candidate configuration (LDS batch x block order x workgroups per CU x
+ * sort items, and tile sizes 16 / 32 when tune_tile_size) over steps_per_candidate of the
+ * trainer's next steps (after one warm-up step each; they are real training steps) and keeps
+ * the fastest, written to *chosen (nullable). */
+int u3d_trainer_autotune(u3d_trainer *t, int steps_per_candidate, int tune_tile_size,
+                         u3d_batch_tuning *chosen);
+int u3d_scene_set_autotune(u3d_scene_set *set, int steps_per_candidate, int tune_tile_size,
+                           u3d_batch_tuning *chosen);
 
 /* ----------------------------------------------------------------------- measurement */
 /* Pipeline stages timed with HIP events on the context's stream (no reference counterpart:

@@ -212,6 +212,10 @@ def ref_lib():
         L.ref_adam_steps.argtypes = [C.POINTER(Model), C.POINTER(Model), C.POINTER(Lrs), C.c_int,
                                      C.c_void_p, C.c_void_p]
         L.ref_init_from_points.argtypes = [C.c_int, F32P, C.c_void_p, C.c_int, C.POINTER(Model)]
+        L.ref_write_splat_ply.restype = C.c_int
+        L.ref_write_splat_ply.argtypes = [C.c_char_p, C.POINTER(Model)]
+        L.ref_read_splat_ply.restype = C.c_long
+        L.ref_read_splat_ply.argtypes = [C.c_char_p, C.c_void_p]
         L.ref_trainer_new.restype = C.c_void_p
         L.ref_trainer_new.argtypes = [C.POINTER(Model), C.POINTER(Camera), C.c_void_p, C.c_int,
                                       C.c_float, C.POINTER(TrainOptions), C.c_int]

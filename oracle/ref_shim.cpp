@@ -12,6 +12,7 @@
 #include "u3d/splat/GaussianModel.h"
 #include "u3d/splat/Loss.h"
 #include "u3d/splat/Rasterizer.h"
+#include "u3d/splat/SplatPly.h"
 #include "u3d/splat/Trainer.h"
 #undef private
 
@@ -193,6 +194,24 @@ void ref_adam_steps(orc_model *p, const orc_model *g, const orc_lrs *lr, int ste
         fromModel(adam.m_m, m_out);
     if (v_out)
         fromModel(adam.m_v, v_out);
+}
+
+// SplatPly.h: the reference's own .ply writer and reader (for the exporter's cross-check).
+int ref_write_splat_ply(const char *path, const orc_model *m)
+{
+    return writeSplatPly(path, toModel(m)) ? 1 : 0;
+}
+
+// Returns the vertex count (-1 on error); fills `out` when it is non-null.
+long ref_read_splat_ply(const char *path, orc_model *out)
+{
+    GaussianModel g;
+    bool splat = false;
+    if (!readSplatPly(path, g, nullptr, &splat) || !splat)
+        return -1;
+    if (out)
+        fromModel(g, out);
+    return long(g.size());
 }
 
 void ref_init_from_points(int n, const float *xyz, const float *rgb, int rgb_count, orc_model *out)

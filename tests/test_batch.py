@@ -198,3 +198,42 @@ def test_batched_training_with_densification_converges(ctx):
     assert after > before + 3.0, (before, after)
     c = t.counters()
     assert c["overflow_steps"] == 0 and c["views"] == 4 * 600
+
+
+@pytest.mark.parametrize("lds,heavy,per_cu,items", [(64, 1, 0, 0), (128, 0, 2, 4), (256, 1, 1, 8)])
+def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items):
+    """LDS batch, block order, workgroups per CU and radix items per lane: same forward (first
+    loss bit-identical), same gradient (1e-4)."""
+    init, cams, images = _scene(3000, 4, 160, 120)
+    t = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
+    t.set_tuning(heavy_first=heavy, workgroups_per_cu=per_cu, lds_batch=lds, sort_items=items)
+    assert t.tuning()["lds_batch"] == lds
+    t.step()
+    m, _ = t.moments()
+    got = {f: m[f].astype(np.float64) / np.float64(ONE_MINUS_B1) for f in m}
+    want, losses = _plain_mean_gradient(ctx, init, cams, images)
+    _assert_grad_close(got, want, (lds, heavy, per_cu, items))
+    ref = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
+    ref.step()
+    assert t.stats()["loss"] == ref.stats()["loss"]
+
+
+def test_autotuner_picks_a_configuration_and_training_continues(ctx):
+    init, cams, images = _scene(4000, 16, 160, 120)
+    opt = U.options_for_iterations(300)
+    opt.views_per_step = 4
+    opt.use_graph = 1
+    t = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), opt)
+    chosen = t.autotune(steps_per_candidate=2)
+    assert chosen["lds_batch"] in (64, 128, 256) and chosen["tile_size"] == 16
+    assert t.tuning() == chosen
+    it = t.iteration()
+    assert it == 24 * 3  # 24 candidates x (1 warm-up + 2 timed) steps
+    for _ in range(5):
+        t.step()
+    assert t.iteration() == it + 5 and t.counters()["overflow_steps"] == 0
+    s = U.SceneSet(ctx, use_graph=True)
+    opt.seed = 7
+    assert s.add(U.Trainer(ctx, init, cams, images, S.scene_extent(cams), opt))
+    c2 = s.autotune(steps_per_candidate=1, tune_tile_size=True)
+    assert c2["tile_size"] in (16, 32)

@@ -56,7 +56,8 @@ class TrainOptions(C.Structure):
 
 class BatchTuning(C.Structure):
     """u3d_batch_tuning: launch choices of the batched engine."""
-    _fields_ = [("heavy_first", C.c_int), ("workgroups_per_cu", C.c_int)]
+    _fields_ = [("heavy_first", C.c_int), ("workgroups_per_cu", C.c_int), ("lds_batch", C.c_int),
+                ("sort_items", C.c_int), ("tile_size", C.c_int)]
 
 
 class TrainStats(C.Structure):
@@ -143,6 +144,9 @@ SIGNATURES = {
     "u3d_init_net_forward": (I, [P, P, I64, P]),
     "u3d_init_net_train_step": (I, [P, P, P, P, I64, F, C.POINTER(F)]),
     "u3d_points_learned_init": (I, [P, P, P, P, I, I64, P, I, F, C.POINTER(Gaussians)]),
+    "u3d_write_splat_ply": (I, [C.c_char_p, P, P, P, P, P, I64, P, I]),
+    "u3d_read_splat_ply": (I, [C.c_char_p, I64, P, P, P, P, P, P, C.POINTER(I64), C.POINTER(I)]),
+    "u3d_trainer_export_ply": (I, [P, C.c_char_p]),
     "u3d_scene_set_create": (I, [P, I, C.c_size_t, C.POINTER(P)]),
     "u3d_scene_set_destroy": (None, [P]),
     "u3d_scene_set_add": (I, [P, P]),
@@ -152,6 +156,8 @@ SIGNATURES = {
     "u3d_scene_set_done": (I, [P]),
     "u3d_scene_set_bytes": (C.c_size_t, [P]),
     "u3d_scene_set_set_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "u3d_trainer_autotune": (I, [P, I, I, C.POINTER(BatchTuning)]),
+    "u3d_scene_set_autotune": (I, [P, I, I, C.POINTER(BatchTuning)]),
     "u3d_expf_host_array": (None, [P, P, I64]),
 }
 

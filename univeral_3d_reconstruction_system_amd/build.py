@@ -24,7 +24,7 @@ ARCH = os.environ.get("U3D_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels_project.hip", "kernels_sort.hip", "kernels_raster.hip", "kernels_loss.hip",
            "kernels_adam.hip", "kernels_points.hip", "kernels_mlp.hip", "api.cpp", "batch.cpp",
-           "initnet.cpp"]
+           "initnet.cpp", "ply.cpp"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -59,22 +59,36 @@ public:
             statusStep[k] = -1;
         }
         U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 2 * kRing));
-        occFwd = rasterBatchedOccupancy(false);
-        occBwd = rasterBatchedOccupancy(true);
+        for (int k = 0; k < 3; ++k) {
+            occFwd[k] = rasterBatchedOccupancy(false, 64 << k);
+            occBwd[k] = rasterBatchedOccupancy(true, 64 << k);
+        }
         return U3D_OK;
     }
 
     // New launch parameters: the captured graph (if any) is rebuilt on the next step.
-    int setTuning(int heavy, int perCu)
+    int setTuning(const u3d_batch_tuning &t)
     {
+        U3D_CHECK(t.workgroups_per_cu >= 0 && t.workgroups_per_cu <= 64, U3D_ERR_INVALID,
+                  "workgroups_per_cu must be 0..64");
+        U3D_CHECK(t.lds_batch == 0 || t.lds_batch == 64 || t.lds_batch == 128 || t.lds_batch == 256,
+                  U3D_ERR_INVALID, "lds_batch must be 0, 64, 128 or 256");
+        U3D_CHECK(t.sort_items == 0 || t.sort_items == 4 || t.sort_items == 8, U3D_ERR_INVALID,
+                  "sort_items must be 0, 4 or 8");
         U3D_HIP(hipStreamSynchronize(ctx->stream));
-        heavyFirst = heavy != 0;
-        workgroupsPerCu = perCu;
+        tuning.heavy_first = t.heavy_first != 0;
+        tuning.workgroups_per_cu = t.workgroups_per_cu;
+        tuning.lds_batch = t.lds_batch ? t.lds_batch : 256;
+        tuning.sort_items = t.sort_items;
+        tuning.tile_size = t.tile_size;
         dropGraph();
         return U3D_OK;
     }
+    u3d_batch_tuning tuning{1, 0, 256, 0, 0};
 
     int step(u3d_trainer *const *scenes, int numScenes);
+    int autotune(u3d_trainer *const *scenes, int numScenes, int stepsPerCandidate, bool tuneTile,
+                 u3d_batch_tuning *chosen);
     size_t workingBytes() const { return allocated; }
     // Working-set estimate for one more member with n Gaussians, B views of P pixels each.
     static size_t estimateBytes(int64_t n, int views, size_t pixels)
@@ -87,11 +101,9 @@ public:
 
     u3d_ctx *ctx;
     bool useGraph;
-    int heavyFirst = 1;
-    int workgroupsPerCu = 0;
 
 private:
-    int occFwd = 1, occBwd = 1;
+    int occFwd[3] = {1, 1, 1}, occBwd[3] = {1, 1, 1}; // per LDS batch 64, 128, 256
     struct Member {
         const u3d_trainer *t;
         int64_t n;
@@ -445,7 +457,7 @@ int BatchEngine::enqueue()
         StageScope scope(ctx, U3D_STAGE_SORT);
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4)); // K, overflow flag
         U3D_TRY(launchIota(s, valA.p, totalEntries));
-        RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap};
+        RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap, tuning.sort_items};
         bool alt = false;
         U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs, &alt));
         const uint32_t *sorted = alt ? valAlt.p : valA.p;
@@ -461,7 +473,7 @@ int BatchEngine::enqueue()
         const uint32_t *tileSorted = alt2 ? pairTileAlt.p : pairTile.p;
         tileList = reinterpret_cast<const int *>(alt2 ? pairGaussAlt.p : pairGauss.p);
         U3D_TRY(launchTileOffsets(s, tileSorted, pairCap, totalTiles, tileOffsets.p, total.p));
-        if (heavyFirst) {
+        if (tuning.heavy_first) {
             U3D_TRY(launchBlockCosts(s, dItems, numItems, totalBlocks, st.tile_size, tileOffsets.p,
                                      blockKey.p, blockOrder.p));
             bool alt3 = false;
@@ -470,15 +482,16 @@ int BatchEngine::enqueue()
             order = alt3 ? blockOrderAlt.p : blockOrder.p;
         }
     }
-    const int perCu = workgroupsPerCu > 0 ? workgroupsPerCu : 0;
+    const int perCu = tuning.workgroups_per_cu;
+    const int lds = tuning.lds_batch, li = lds == 64 ? 0 : (lds == 128 ? 1 : 2);
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_FWD);
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(counters.p), 0, 2));
-        const int occ = perCu > 0 ? perCu : occFwd;
+        const int occ = perCu > 0 ? perCu : occFwd[li];
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
         U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, nullptr));
+                                    st, nullptr, lds));
     }
     {
         StageScope scope(ctx, U3D_STAGE_LOSS);
@@ -487,11 +500,11 @@ int BatchEngine::enqueue()
     }
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
-        const int occ = perCu > 0 ? perCu : occBwd;
+        const int occ = perCu > 0 ? perCu : occBwd[li];
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
         U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks, counters.p + 1,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, grad2d.p));
+                                    st, grad2d.p, lds));
     }
     {
         StageScope scope(ctx, U3D_STAGE_CHAIN);
@@ -612,6 +625,70 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
         t->counters[4] += B;
         U3D_TRY(trainerFinishStep(t));
     }
+    return U3D_OK;
+}
+
+// Times candidate launch configurations on the scenes' next training steps (real steps: the
+// choices below change no result beyond float summation order) and keeps the fastest:
+// persistent workgroups per CU, heavy-first block order, LDS batch, radix items per lane and,
+// when tuneTile, the tile size (which, as RasterSettings::tileSize, changes which Gaussians a
+// tile keeps near the 3-sigma cut).
+int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPerCandidate,
+                          bool tuneTile, u3d_batch_tuning *chosen)
+{
+    const hipStream_t s = ctx->stream;
+    std::vector<u3d_batch_tuning> cands;
+    const int baseTile = numScenes > 0 ? scenes[0]->opt.raster.tile_size : 16;
+    for (int tile : tuneTile ? std::vector<int>{16, 32} : std::vector<int>{baseTile})
+        for (int lds : {64, 128, 256})
+            for (int heavy : {1, 0})
+                for (int perCu : {0, 2})
+                    for (int items : {0, 4})
+                        cands.push_back({heavy, perCu, lds, items, tile});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    U3D_HIP(hipEventCreate(&e0));
+    U3D_HIP(hipEventCreate(&e1));
+    float bestMs = 3.4e38f;
+    u3d_batch_tuning best = tuning;
+    best.tile_size = baseTile;
+    int rc = U3D_OK;
+    for (const u3d_batch_tuning &c : cands) {
+        bool live = false;
+        for (int k = 0; k < numScenes; ++k)
+            live |= scenes[k]->iteration < scenes[k]->opt.iterations;
+        if (!live)
+            break;
+        if ((rc = setTuning(c)) != U3D_OK)
+            break;
+        for (int k = 0; k < numScenes; ++k)
+            scenes[k]->opt.raster.tile_size = c.tile_size;
+        needRelayout = needRelayout || c.tile_size != st.tile_size;
+        if ((rc = step(scenes, numScenes)) != U3D_OK) // warm-up: relayout, probe, capture
+            break;
+        if ((rc = hipEventRecord(e0, s) == hipSuccess ? U3D_OK : U3D_ERR_HIP) != U3D_OK)
+            break;
+        int done = 0;
+        for (; done < stepsPerCandidate; ++done)
+            if ((rc = step(scenes, numScenes)) != U3D_OK)
+                break;
+        if (rc != U3D_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (done > 0 && ms / float(done) < bestMs) {
+            bestMs = ms / float(done);
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    U3D_TRY(rc);
+    U3D_TRY(setTuning(best));
+    for (int k = 0; k < numScenes; ++k)
+        scenes[k]->opt.raster.tile_size = best.tile_size;
+    needRelayout = true;
+    if (chosen)
+        *chosen = best;
     return U3D_OK;
 }
 
@@ -746,31 +823,50 @@ int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out)
 {
     U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
     U3D_CHECK(t->engine != nullptr, U3D_ERR_INVALID, "trainer does not use the batched engine");
-    out->heavy_first = t->engine->heavyFirst;
-    out->workgroups_per_cu = t->engine->workgroupsPerCu;
+    *out = t->engine->tuning;
+    out->tile_size = t->opt.raster.tile_size;
     return U3D_OK;
 }
-
-namespace {
-int applyTuning(BatchEngine *e, const u3d_batch_tuning *tuning)
-{
-    U3D_CHECK(tuning->workgroups_per_cu >= 0 && tuning->workgroups_per_cu <= 64, U3D_ERR_INVALID,
-              "workgroups_per_cu must be 0..64");
-    return e->setTuning(tuning->heavy_first, tuning->workgroups_per_cu);
-}
-} // namespace
 
 int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning)
 {
     U3D_CHECK(t && tuning, U3D_ERR_INVALID, "null argument");
     U3D_CHECK(t->engine != nullptr, U3D_ERR_INVALID, "trainer does not use the batched engine");
-    return applyTuning(t->engine, tuning);
+    U3D_CHECK(tuning->tile_size == 0 || (tuning->tile_size > 0 && tuning->tile_size % 16 == 0),
+              U3D_ERR_UNSUPPORTED, "tile_size must be a positive multiple of 16");
+    if (tuning->tile_size)
+        t->opt.raster.tile_size = tuning->tile_size;
+    return t->engine->setTuning(*tuning);
 }
 
 int u3d_scene_set_set_tuning(u3d_scene_set *set, const u3d_batch_tuning *tuning)
 {
     U3D_CHECK(set && tuning, U3D_ERR_INVALID, "null argument");
-    return applyTuning(set->engine.get(), tuning);
+    U3D_CHECK(tuning->tile_size == 0 || (tuning->tile_size > 0 && tuning->tile_size % 16 == 0),
+              U3D_ERR_UNSUPPORTED, "tile_size must be a positive multiple of 16");
+    if (tuning->tile_size)
+        for (u3d_trainer *t : set->members)
+            t->opt.raster.tile_size = tuning->tile_size;
+    return set->engine->setTuning(*tuning);
+}
+
+int u3d_trainer_autotune(u3d_trainer *t, int steps_per_candidate, int tune_tile_size,
+                         u3d_batch_tuning *chosen)
+{
+    U3D_CHECK(t != nullptr && steps_per_candidate > 0, U3D_ERR_INVALID, "bad argument");
+    U3D_CHECK(t->engine != nullptr, U3D_ERR_INVALID, "trainer does not use the batched engine");
+    U3D_TRY(useDevice(t->ctx));
+    u3d_trainer *one[1] = {t};
+    return t->engine->autotune(one, 1, steps_per_candidate, tune_tile_size != 0, chosen);
+}
+
+int u3d_scene_set_autotune(u3d_scene_set *set, int steps_per_candidate, int tune_tile_size,
+                           u3d_batch_tuning *chosen)
+{
+    U3D_CHECK(set != nullptr && steps_per_candidate > 0, U3D_ERR_INVALID, "bad argument");
+    U3D_TRY(useDevice(set->ctx));
+    return set->engine->autotune(set->members.data(), int(set->members.size()), steps_per_candidate,
+                                 tune_tile_size != 0, chosen);
 }
 
 } // extern "C"

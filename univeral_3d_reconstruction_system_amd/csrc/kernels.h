@@ -96,6 +96,7 @@ struct RadixScratch {
     size_t histCap;
     uint32_t *scanTmp; // radixScanScratchSize(count)
     size_t scanTmpCap;
+    int items = 0;     // keys per lane per block: 4 or 8 (0: by size)
 };
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
@@ -145,14 +146,16 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
                          float *grad2d);
 
 // Persistent compositing over the blocks of all items in `order` (nullptr = natural order);
-// `counter` must be zero on entry (one unsigned).
+// `counter` must be zero on entry (one unsigned). ldsBatch: tile-list records staged in LDS per
+// round (64, 128 or 256; the compositing order and results do not depend on it).
 int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
-                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d);
+                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
+                        int ldsBatch);
 
 // Resident workgroups per CU of the persistent compositing kernel (occupancy limit).
-int rasterBatchedOccupancy(bool backward);
+int rasterBatchedOccupancy(bool backward, int ldsBatch);
 
 // ---- kernels_points.hip
 // Hash grid of seed points: point indices sorted by their 64-bit cell key (hi, lo halves).

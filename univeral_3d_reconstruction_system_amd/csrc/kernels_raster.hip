@@ -31,7 +31,7 @@ namespace u3d {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kBatch = 256;
+
 constexpr int kEdge = 16;
 
 template <int CTRL, int ROW_MASK> __device__ __forceinline__ float dppMove(float x)
@@ -99,7 +99,9 @@ __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, i
     return mask;
 }
 
-// One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase.
+// One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase,
+// staging kBatch (<= 256) records of the tile list in LDS at a time.
+template <int kBatch>
 __device__ __forceinline__ void forwardBlock(
     int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -130,7 +132,7 @@ __device__ __forceinline__ void forwardBlock(
             break;
         const int idx = base + tid;
         uint32_t m = 0;
-        if (idx < end) {
+        if (tid < kBatch && idx < end) {
             const int g = tileList[idx];
             const float4 a = rec0[g];
             const float4 b = rec1[g];
@@ -141,7 +143,8 @@ __device__ __forceinline__ void forwardBlock(
             sthr[tid] = thr;
             m = quadrantMask(a, b.x, thr, bx0, by0);
         }
-        smask[tid] = m;
+        if (tid < kBatch)
+            smask[tid] = m;
         __syncthreads();
         if (__ballot(!done) == 0)
             continue;
@@ -190,6 +193,7 @@ __device__ __forceinline__ void forwardBlock(
     }
 }
 
+template <int kBatch>
 __device__ __forceinline__ void backwardBlock(
     int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -256,7 +260,8 @@ __device__ __forceinline__ void backwardBlock(
             m = quadrantMask(a, b.x, thr, bx0, by0);
             m &= (j < wm0 ? 1u : 0u) | (j < wm1 ? 2u : 0u) | (j < wm2 ? 4u : 0u) | (j < wm3 ? 8u : 0u);
         }
-        smask[tid] = m;
+        if (tid < kBatch)
+            smask[tid] = m;
         __syncthreads();
         for (int c = kBatch / 64 - 1; c >= 0; --c) {
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib)
 {
-    forwardBlock(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+    forwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
                  tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT, nContrib);
 }
 
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, float *__restrict__ grad2d)
 {
-    backwardBlock(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+    backwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
                   tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
                   grad2d);
 }
@@ -391,7 +396,7 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
 
 // Persistent batched compositing: every workgroup takes the next block of the work order from
 // an atomic counter until the list is exhausted. The order lists heavy tiles first.
-template <bool kBackward>
+template <bool kBackward, int kBatch>
 __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
     const RenderItem *__restrict__ items, int numItems, const uint32_t *__restrict__ order,
     int64_t totalBlocks, unsigned *counter, const int *__restrict__ tileOffsets,
@@ -414,12 +419,12 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
         const int local = int(b - it.blockBase);
         const int bx = local % it.blocksX, by = local / it.blocksX;
         if (!kBackward)
-            forwardBlock(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
+            forwardBlock<kBatch>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                          tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
                          st.background[1], st.background[2], st.min_alpha,
                          st.min_transmittance, it.rgb, it.finalT, it.nContrib);
         else
-            backwardBlock(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
+            backwardBlock<kBatch>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                           tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
                           st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
                           it.nContrib, grad2d);
@@ -428,31 +433,63 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
 
 } // namespace
 
+template <bool kBackward, int kBatch>
+static void launchBatchedVariant(hipStream_t s, int numWorkgroups, const RenderItem *items,
+                                 int numItems, const uint32_t *order, int64_t totalBlocks,
+                                 unsigned *counter, const int *tileOffsets, const int *tileList,
+                                 const ScreenBuffers &sb, const u3d_raster_settings &st,
+                                 float *grad2d)
+{
+    hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch>), dim3(numWorkgroups), dim3(kThreads),
+                       0, s, items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
+                       sb.rec0, sb.rec1, sb.rec2, st, grad2d);
+}
+
 int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
-                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d)
+                        const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
+                        int ldsBatch)
 {
     if (totalBlocks <= 0)
         return U3D_OK;
-    if (backward)
-        hipLaunchKernelGGL(rasterBatchedKernel<true>, dim3(numWorkgroups), dim3(kThreads), 0, s,
-                           items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
-                           sb.rec0, sb.rec1, sb.rec2, st, grad2d);
-    else
-        hipLaunchKernelGGL(rasterBatchedKernel<false>, dim3(numWorkgroups), dim3(kThreads), 0, s,
-                           items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
-                           sb.rec0, sb.rec1, sb.rec2, st, grad2d);
+    U3D_CHECK(ldsBatch == 64 || ldsBatch == 128 || ldsBatch == 256, U3D_ERR_INVALID,
+              "LDS batch must be 64, 128 or 256");
+#define U3D_BATCHED(B, L)                                                                        \
+    launchBatchedVariant<B, L>(s, numWorkgroups, items, numItems, order, totalBlocks, counter,  \
+                               tileOffsets, tileList, sb, st, grad2d)
+    if (backward) {
+        if (ldsBatch == 64)
+            U3D_BATCHED(true, 64);
+        else if (ldsBatch == 128)
+            U3D_BATCHED(true, 128);
+        else
+            U3D_BATCHED(true, 256);
+    } else {
+        if (ldsBatch == 64)
+            U3D_BATCHED(false, 64);
+        else if (ldsBatch == 128)
+            U3D_BATCHED(false, 128);
+        else
+            U3D_BATCHED(false, 256);
+    }
+#undef U3D_BATCHED
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
 
-int rasterBatchedOccupancy(bool backward)
+int rasterBatchedOccupancy(bool backward, int ldsBatch)
 {
     int blocks = 0;
-    const hipError_t e = backward
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true>, kThreads, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false>, kThreads, 0);
+    hipError_t e = hipErrorInvalidValue;
+    if (backward)
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 64>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 128>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 256>, kThreads, 0);
+    else
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 64>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 128>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 256>, kThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 

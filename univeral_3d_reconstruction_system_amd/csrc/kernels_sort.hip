@@ -425,7 +425,8 @@ int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys
     *resultInAlt = false;
     if (count <= 0 || bits <= 0)
         return U3D_OK;
-    if (itemsFor(count) == 4)
+    const int items = scratch.items == 4 || scratch.items == 8 ? scratch.items : itemsFor(count);
+    if (items == 4)
         return radixPasses<4>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt,
                               countDev);
     return radixPasses<8>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt,

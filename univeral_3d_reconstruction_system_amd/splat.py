@@ -409,11 +409,21 @@ class Trainer:
     def tuning(self) -> dict:
         t = _abi.BatchTuning()
         check(self.ctx.lib.u3d_trainer_get_tuning(self.h, C.byref(t)))
-        return dict(heavy_first=t.heavy_first, workgroups_per_cu=t.workgroups_per_cu)
+        return _tuning_dict(t)
 
-    def set_tuning(self, heavy_first: int = 1, workgroups_per_cu: int = 0):
-        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu))
+    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0):
+        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
+                             int(tile_size))
         check(self.ctx.lib.u3d_trainer_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
+        t = _abi.BatchTuning()
+        check(self.ctx.lib.u3d_trainer_autotune(self.h, int(steps_per_candidate), int(tune_tile_size),
+                                                C.byref(t)))
+        return _tuning_dict(t)
+
+    def export_ply(self, path: str):
+        check(self.ctx.lib.u3d_trainer_export_ply(self.h, path.encode()))
 
     def sh_rest(self) -> np.ndarray:
         """SH coefficients [size x 15 x 3] (trainers created with sh_degree > 0)."""
@@ -444,6 +454,11 @@ class Trainer:
 
     def synchronize(self):
         self.ctx.synchronize()
+
+
+def _tuning_dict(t) -> dict:
+    return dict(heavy_first=t.heavy_first, workgroups_per_cu=t.workgroups_per_cu,
+                lds_batch=t.lds_batch, sort_items=t.sort_items, tile_size=t.tile_size)
 
 
 def _split_fields(flat: np.ndarray, n: int) -> dict:
@@ -485,6 +500,30 @@ def points_features(ctx: Context, xyz, rgb, cams=None, near_plane: float = 0.01)
     check(ctx.lib.u3d_points_features(ctx.h, dx.ptr, dr.ptr, len(rgb), n, arr, nc,
                                       float(near_plane), f.ptr))
     return f.numpy()[:n * _abi.FEATURE_DIM].reshape(n, _abi.FEATURE_DIM)
+
+
+def write_splat_ply(path: str, model: dict, sh_rest=None, sh_degree: int = 0):
+    """writeSplatPly (SplatPly.cpp:261-281) with optional f_rest_* SH coefficients."""
+    lib = _abi.load()
+    arrs = [np.ascontiguousarray(model[f], np.float32).reshape(-1) for f, _ in FIELDS]
+    n = len(arrs[3])
+    sh = None if sh_rest is None else np.ascontiguousarray(sh_rest, np.float32).reshape(-1)
+    check(lib.u3d_write_splat_ply(path.encode(), *(a.ctypes.data for a in arrs), n,
+                                  sh.ctypes.data if sh is not None else None, int(sh_degree)))
+
+
+def read_splat_ply(path: str):
+    """readSplatPly for 3DGS files: (model dict, sh_rest [n*45] or None, sh_degree)."""
+    lib = _abi.load()
+    n, deg = C.c_int64(), C.c_int()
+    check(lib.u3d_read_splat_ply(path.encode(), 0, None, None, None, None, None, None,
+                                 C.byref(n), C.byref(deg)))
+    out = {f: np.zeros(max(n.value * d, 1), np.float32) for f, d in FIELDS}
+    sh = np.zeros(max(n.value * 45, 1), np.float32)
+    check(lib.u3d_read_splat_ply(path.encode(), max(n.value, 1), *(out[f].ctypes.data for f, _ in FIELDS),
+                                 sh.ctypes.data, C.byref(n), C.byref(deg)))
+    model = {f: out[f][:n.value * d] for f, d in FIELDS}
+    return model, (sh[:n.value * 45] if deg.value > 0 else None), deg.value
 
 
 class InitNet:
@@ -625,6 +664,13 @@ class SceneSet:
     def bytes(self) -> int:
         return int(self.ctx.lib.u3d_scene_set_bytes(self.h))
 
-    def set_tuning(self, heavy_first: int = 1, workgroups_per_cu: int = 0):
-        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu))
+    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0):
+        t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
+                             int(tile_size))
         check(self.ctx.lib.u3d_scene_set_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
+        t = _abi.BatchTuning()
+        check(self.ctx.lib.u3d_scene_set_autotune(self.h, int(steps_per_candidate),
+                                                  int(tune_tile_size), C.byref(t)))
+        return _tuning_dict(t)
