@@ -334,12 +334,15 @@ typedef struct {
     int lds_batch;          /* tile-list records staged in LDS per round: 64, 128, 256 (0: 256) */
     int sort_items;         /* radix sort keys per lane per block: 4 or 8 (0: by size) */
     int tile_size;          /* 0: keep the raster settings' tile size */
+    int atomic_lanes;       /* backward: Gaussians reaching at most this many lanes of a wave
+                             * (0, 4, 16) add per lane to LDS instead of a wave reduction */
 } u3d_batch_tuning;
 int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out);
 int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning);
 int u3d_scene_set_set_tuning(u3d_scene_set *set, const u3d_batch_tuning *tuning);
 /* Autotuner: times each candidate configuration (LDS batch x block order x workgroups per CU x
- * sort items, and tile sizes 16 / 32 when tune_tile_size) over steps_per_candidate of the
+ * sort items, and tile sizes 16 / 32 when tune_tile_size; atomic_lanes stays 0, measured
+ * slower at 300k / 640x480) over steps_per_candidate of the
  * trainer's next steps (after one warm-up step each; they are real training steps) and keeps
  * the fastest, written to *chosen (nullable). */
 int u3d_trainer_autotune(u3d_trainer *t, int steps_per_candidate, int tune_tile_size,

@@ -200,19 +200,21 @@ def test_batched_training_with_densification_converges(ctx):
     assert c["overflow_steps"] == 0 and c["views"] == 4 * 600
 
 
-@pytest.mark.parametrize("lds,heavy,per_cu,items", [(64, 1, 0, 0), (128, 0, 2, 4), (256, 1, 1, 8)])
-def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items):
+@pytest.mark.parametrize("lds,heavy,per_cu,items,few", [(64, 1, 0, 0, 4), (128, 0, 2, 4, 16),
+                                                       (256, 1, 1, 8, 0)])
+def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items, few):
     """LDS batch, block order, workgroups per CU and radix items per lane: same forward (first
     loss bit-identical), same gradient (1e-4)."""
     init, cams, images = _scene(3000, 4, 160, 120)
     t = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
-    t.set_tuning(heavy_first=heavy, workgroups_per_cu=per_cu, lds_batch=lds, sort_items=items)
+    t.set_tuning(heavy_first=heavy, workgroups_per_cu=per_cu, lds_batch=lds, sort_items=items,
+                 atomic_lanes=few)
     assert t.tuning()["lds_batch"] == lds
     t.step()
     m, _ = t.moments()
     got = {f: m[f].astype(np.float64) / np.float64(ONE_MINUS_B1) for f in m}
     want, losses = _plain_mean_gradient(ctx, init, cams, images)
-    _assert_grad_close(got, want, (lds, heavy, per_cu, items))
+    _assert_grad_close(got, want, (lds, heavy, per_cu, items, few))
     ref = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
     ref.step()
     assert t.stats()["loss"] == ref.stats()["loss"]

@@ -75,16 +75,19 @@ public:
                   U3D_ERR_INVALID, "lds_batch must be 0, 64, 128 or 256");
         U3D_CHECK(t.sort_items == 0 || t.sort_items == 4 || t.sort_items == 8, U3D_ERR_INVALID,
                   "sort_items must be 0, 4 or 8");
+        U3D_CHECK(t.atomic_lanes == 0 || t.atomic_lanes == 4 || t.atomic_lanes == 16,
+                  U3D_ERR_INVALID, "atomic_lanes must be 0, 4 or 16");
         U3D_HIP(hipStreamSynchronize(ctx->stream));
         tuning.heavy_first = t.heavy_first != 0;
         tuning.workgroups_per_cu = t.workgroups_per_cu;
         tuning.lds_batch = t.lds_batch ? t.lds_batch : 256;
         tuning.sort_items = t.sort_items;
         tuning.tile_size = t.tile_size;
+        tuning.atomic_lanes = t.atomic_lanes;
         dropGraph();
         return U3D_OK;
     }
-    u3d_batch_tuning tuning{1, 0, 256, 0, 0};
+    u3d_batch_tuning tuning{1, 0, 256, 0, 0, 0};
 
     int step(u3d_trainer *const *scenes, int numScenes);
     int autotune(u3d_trainer *const *scenes, int numScenes, int stepsPerCandidate, bool tuneTile,
@@ -491,7 +494,7 @@ int BatchEngine::enqueue()
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
         U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, nullptr, lds));
+                                    st, nullptr, lds, 0));
     }
     {
         StageScope scope(ctx, U3D_STAGE_LOSS);
@@ -504,7 +507,7 @@ int BatchEngine::enqueue()
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
         U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks, counters.p + 1,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, grad2d.p, lds));
+                                    st, grad2d.p, lds, tuning.atomic_lanes));
     }
     {
         StageScope scope(ctx, U3D_STAGE_CHAIN);
@@ -644,7 +647,7 @@ int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPe
             for (int heavy : {1, 0})
                 for (int perCu : {0, 2})
                     for (int items : {0, 4})
-                        cands.push_back({heavy, perCu, lds, items, tile});
+                        cands.push_back({heavy, perCu, lds, items, tile, 0});
     hipEvent_t e0 = nullptr, e1 = nullptr;
     U3D_HIP(hipEventCreate(&e0));
     U3D_HIP(hipEventCreate(&e1));

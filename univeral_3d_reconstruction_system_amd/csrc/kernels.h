@@ -128,6 +128,8 @@ int launchEmitPairsBatched(hipStream_t s, int64_t n, const uint32_t *sorted,
 // first.
 int launchBlockCosts(hipStream_t s, const RenderItem *items, int numItems, int64_t totalBlocks,
                      int tileSize, const int *tileOffsets, uint32_t *keys, uint32_t *order);
+int launchBlockCostsPlain(hipStream_t s, int blocksX, int totalBlocks, int tileSize, int tilesX,
+                          const int *tileOffsets, uint32_t *keys, uint32_t *order);
 // tileOffsets[t] = first pair index of tile t, tileOffsets[tiles] = K (tile ids sorted).
 // With countDev, the pair count is min(*countDev, numPairs) read on the device.
 int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs, int tiles,
@@ -147,12 +149,14 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
 
 // Persistent compositing over the blocks of all items in `order` (nullptr = natural order);
 // `counter` must be zero on entry (one unsigned). ldsBatch: tile-list records staged in LDS per
-// round (64, 128 or 256; the compositing order and results do not depend on it).
+// round (64, 128 or 256; the compositing order and results do not depend on it). fewLanes
+// (backward; 0, 4 or 16): Gaussians with at most that many contributing lanes in a wave skip
+// the wave reduction and add per lane.
 int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
                         const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
-                        int ldsBatch);
+                        int ldsBatch, int fewLanes);
 
 // Resident workgroups per CU of the persistent compositing kernel (occupancy limit).
 int rasterBatchedOccupancy(bool backward, int ldsBatch);

@@ -33,6 +33,7 @@ namespace {
 constexpr int kThreads = 256;
 
 constexpr int kEdge = 16;
+constexpr int kPlainFewLanes = 0; // lane threshold of the plain backward (see backwardBlock)
 
 template <int CTRL, int ROW_MASK> __device__ __forceinline__ float dppMove(float x)
 {
@@ -193,7 +194,10 @@ __device__ __forceinline__ void forwardBlock(
     }
 }
 
-template <int kBatch>
+// kFewLanes: when at most this many lanes of a wave contribute to a Gaussian, they add their
+// values to its LDS accumulator directly instead of paying nine 64-lane reductions (0: always
+// reduce).
+template <int kBatch, int kFewLanes>
 __device__ __forceinline__ void backwardBlock(
     int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -315,7 +319,21 @@ __device__ __forceinline__ void backwardBlock(
                         }
                     }
                 }
-                if (__ballot(contrib)) {
+                const uint64_t who = __ballot(contrib);
+                if (kFewLanes > 0 && who && __popcll(who) <= kFewLanes) {
+                    if (contrib) {
+                        float *dst = acc[jj];
+                        atomicAdd(&dst[0], v0);
+                        atomicAdd(&dst[1], v1);
+                        atomicAdd(&dst[2], v2);
+                        atomicAdd(&dst[3], v3);
+                        atomicAdd(&dst[4], v4);
+                        atomicAdd(&dst[5], v5);
+                        atomicAdd(&dst[6], v6);
+                        atomicAdd(&dst[7], v7);
+                        atomicAdd(&dst[8], v8);
+                    }
+                } else if (who) {
                     v0 = waveSum(v0);
                     v1 = waveSum(v1);
                     v2 = waveSum(v2);
@@ -374,7 +392,7 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, float *__restrict__ grad2d)
 {
-    backwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+    backwardBlock<256, kPlainFewLanes>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
                   tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
                   grad2d);
 }
@@ -396,7 +414,7 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
 
 // Persistent batched compositing: every workgroup takes the next block of the work order from
 // an atomic counter until the list is exhausted. The order lists heavy tiles first.
-template <bool kBackward, int kBatch>
+template <bool kBackward, int kBatch, int kFewLanes>
 __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
     const RenderItem *__restrict__ items, int numItems, const uint32_t *__restrict__ order,
     int64_t totalBlocks, unsigned *counter, const int *__restrict__ tileOffsets,
@@ -424,7 +442,7 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
                          st.background[1], st.background[2], st.min_alpha,
                          st.min_transmittance, it.rgb, it.finalT, it.nContrib);
         else
-            backwardBlock<kBatch>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
+            backwardBlock<kBatch, kFewLanes>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                           tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
                           st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
                           it.nContrib, grad2d);
@@ -433,14 +451,14 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
 
 } // namespace
 
-template <bool kBackward, int kBatch>
+template <bool kBackward, int kBatch, int kFewLanes>
 static void launchBatchedVariant(hipStream_t s, int numWorkgroups, const RenderItem *items,
                                  int numItems, const uint32_t *order, int64_t totalBlocks,
                                  unsigned *counter, const int *tileOffsets, const int *tileList,
                                  const ScreenBuffers &sb, const u3d_raster_settings &st,
                                  float *grad2d)
 {
-    hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch>), dim3(numWorkgroups), dim3(kThreads),
+    hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch, kFewLanes>), dim3(numWorkgroups), dim3(kThreads),
                        0, s, items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
                        sb.rec0, sb.rec1, sb.rec2, st, grad2d);
 }
@@ -449,30 +467,35 @@ int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, i
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
                         const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
-                        int ldsBatch)
+                        int ldsBatch, int fewLanes)
 {
     if (totalBlocks <= 0)
         return U3D_OK;
     U3D_CHECK(ldsBatch == 64 || ldsBatch == 128 || ldsBatch == 256, U3D_ERR_INVALID,
               "LDS batch must be 64, 128 or 256");
-#define U3D_BATCHED(B, L)                                                                        \
-    launchBatchedVariant<B, L>(s, numWorkgroups, items, numItems, order, totalBlocks, counter,  \
-                               tileOffsets, tileList, sb, st, grad2d)
-    if (backward) {
-        if (ldsBatch == 64)
-            U3D_BATCHED(true, 64);
-        else if (ldsBatch == 128)
-            U3D_BATCHED(true, 128);
-        else
-            U3D_BATCHED(true, 256);
-    } else {
-        if (ldsBatch == 64)
-            U3D_BATCHED(false, 64);
-        else if (ldsBatch == 128)
-            U3D_BATCHED(false, 128);
-        else
-            U3D_BATCHED(false, 256);
-    }
+    U3D_CHECK(fewLanes == 0 || fewLanes == 4 || fewLanes == 16, U3D_ERR_INVALID,
+              "atomic lane threshold must be 0, 4 or 16");
+#define U3D_BATCHED(B, L, F)                                                                     \
+    launchBatchedVariant<B, L, F>(s, numWorkgroups, items, numItems, order, totalBlocks, counter, \
+                                  tileOffsets, tileList, sb, st, grad2d)
+#define U3D_BATCHED_L(B, F)                                                                      \
+    do {                                                                                         \
+        if (ldsBatch == 64)                                                                      \
+            U3D_BATCHED(B, 64, F);                                                               \
+        else if (ldsBatch == 128)                                                                \
+            U3D_BATCHED(B, 128, F);                                                              \
+        else                                                                                     \
+            U3D_BATCHED(B, 256, F);                                                              \
+    } while (0)
+    if (!backward)
+        U3D_BATCHED_L(false, 0);
+    else if (fewLanes == 0)
+        U3D_BATCHED_L(true, 0);
+    else if (fewLanes == 4)
+        U3D_BATCHED_L(true, 4);
+    else
+        U3D_BATCHED_L(true, 16);
+#undef U3D_BATCHED_L
 #undef U3D_BATCHED
     U3D_LAUNCH_CHECK();
     return U3D_OK;
@@ -483,13 +506,13 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
     if (backward)
-        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 64>, kThreads, 0)
-            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 128>, kThreads, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 256>, kThreads, 0);
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 64, 0>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 128, 0>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 256, 0>, kThreads, 0);
     else
-        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 64>, kThreads, 0)
-            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 128>, kThreads, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 256>, kThreads, 0);
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 64, 0>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 128, 0>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 256, 0>, kThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 

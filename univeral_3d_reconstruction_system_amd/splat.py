@@ -411,9 +411,10 @@ class Trainer:
         check(self.ctx.lib.u3d_trainer_get_tuning(self.h, C.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0):
+    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0,
+                   atomic_lanes=0):
         t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
-                             int(tile_size))
+                             int(tile_size), int(atomic_lanes))
         check(self.ctx.lib.u3d_trainer_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
@@ -458,7 +459,8 @@ class Trainer:
 
 def _tuning_dict(t) -> dict:
     return dict(heavy_first=t.heavy_first, workgroups_per_cu=t.workgroups_per_cu,
-                lds_batch=t.lds_batch, sort_items=t.sort_items, tile_size=t.tile_size)
+                lds_batch=t.lds_batch, sort_items=t.sort_items, tile_size=t.tile_size,
+                atomic_lanes=t.atomic_lanes)
 
 
 def _split_fields(flat: np.ndarray, n: int) -> dict:
@@ -664,9 +666,10 @@ class SceneSet:
     def bytes(self) -> int:
         return int(self.ctx.lib.u3d_scene_set_bytes(self.h))
 
-    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0):
+    def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0,
+                   atomic_lanes=0):
         t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
-                             int(tile_size))
+                             int(tile_size), int(atomic_lanes))
         check(self.ctx.lib.u3d_scene_set_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
