@@ -128,6 +128,7 @@ SIGNATURES = {
     "u3d_trainer_evaluate_l1": (I, [P, C.POINTER(F)]),
     "u3d_trainer_render": (I, [P, C.POINTER(Camera), P]),
     "u3d_ctx_set_timing": (I, [P, I]),
+    "u3d_ctx_set_option": (I, [P, I, I]),
     "u3d_ctx_stage_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64)]),
     "u3d_trainer_counters": (I, [P, C.POINTER(I64)]),
     "u3d_trainer_get_moments": (I, [P, P, P]),
@@ -164,6 +165,7 @@ SIGNATURES = {
 STAGES = ("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam", "densify",
           "graph_step")
 U3D_NUM_STAGES = len(STAGES)
+OPT_SEGMENTED_BACKWARD = 1
 FEATURE_DIM = 80
 NET_OUT = 16
 NET_LAYERS = ((80, 64), (64, 256), (256, 256), (256, 256), (256, 16))  # (in, out)

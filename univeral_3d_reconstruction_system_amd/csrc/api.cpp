@@ -113,9 +113,19 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     r->tileList = reinterpret_cast<const int *>(alt2 ? r->pairGaussAlt.p : r->pairGauss.p);
     U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
     sortScope.reset();
+    // Checkpoints for the segmented backward (16-pixel tiles): state at every 256-record batch
+    // boundary of each tile list.
+    r->segmented = r->ctx->segmentedBackward && ts == 16;
+    SegmentState seg{nullptr, nullptr};
+    if (r->segmented) {
+        const size_t slots = size_t(K) / 256 + size_t(tiles) + 1;
+        U3D_TRY(r->ckpt.ensure(slots * 256));
+        U3D_TRY(r->segCount.ensure(size_t(tiles) + 1));
+        seg = SegmentState{r->ckpt.p, r->segCount.p};
+    }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
     U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
-                                r->finalT.p, r->nContrib.p));
+                                r->finalT.p, r->nContrib.p, seg));
     r->haveForward = true;
     return U3D_OK;
 }
@@ -134,7 +144,23 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
     if (g2)
         U3D_HIP(hipMemsetAsync(r->grad2d.p, 0, g2 * sizeof(float), s));
     const ScreenBuffers sb = r->screen();
-    {
+    if (r->segmented) {
+        const int tiles = r->geo.tilesX * r->geo.tilesY;
+        U3D_TRY(r->segStart.ensure(size_t(tiles) + 1));
+        U3D_TRY(r->segTotal.ensure(4));
+        U3D_TRY(r->segCounter.ensure(1));
+        U3D_TRY(r->scanScratch.ensure(scanScratchSize(int64_t(tiles) + 1)));
+        if (!r->occSeg)
+            r->occSeg = rasterSegmentedOccupancy();
+        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD);
+        U3D_TRY(exclusiveScanU32(s, r->segCount.p, r->segStart.p, tiles, r->segTotal.p,
+                                 r->scanScratch.p, r->scanScratch.cap));
+        U3D_HIP(hipMemsetAsync(r->segCounter.p, 0, sizeof(unsigned), s));
+        U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segStart.p, r->segTotal.p,
+                                      r->segCounter.p, r->ctx->num_cus * r->occSeg,
+                                      r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
+                                      r->nContrib.p, r->ckpt.p, r->grad2d.p));
+    } else {
         StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD);
         U3D_TRY(launchRasterBackward(s, r->geo, r->st, r->tileOffsets.p, r->tileList, sb, dl,
                                      r->finalT.p, r->nContrib.p, r->grad2d.p));
@@ -1092,6 +1118,20 @@ int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host
                               t->shDegree));
     U3D_HIP(hipMemcpyAsync(rgb_host, out.p, count * sizeof(float), hipMemcpyDeviceToHost, t->ctx->stream));
     U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    return U3D_OK;
+}
+
+int u3d_ctx_set_option(u3d_ctx *ctx, int option, int value)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_HIP(hipStreamSynchronize(ctx->stream));
+    switch (option) {
+    case U3D_OPT_SEGMENTED_BACKWARD:
+        ctx->segmentedBackward = value != 0;
+        return U3D_OK;
+    default:
+        U3D_CHECK(false, U3D_ERR_INVALID, "unknown option");
+    }
     return U3D_OK;
 }
 

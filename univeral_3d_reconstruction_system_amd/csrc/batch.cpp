@@ -63,6 +63,7 @@ public:
             occFwd[k] = rasterBatchedOccupancy(false, 64 << k);
             occBwd[k] = rasterBatchedOccupancy(true, 64 << k);
         }
+        occSeg = rasterSegmentedOccupancy();
         return U3D_OK;
     }
 
@@ -173,6 +174,13 @@ private:
     DevBuf<uint32_t> blockKey, blockOrder, blockKeyAlt, blockOrderAlt;
     DevBuf<unsigned> counters; // persistent work counters (forward, backward)
     DevBuf<double> partials;
+    // segmented backward (16-pixel tiles): forward checkpoints, per-tile segment counts
+    bool segmented = false;
+    int occSeg = 1;
+    DevBuf<float4> ckpt;
+    DevBuf<uint32_t> segCount, segStart, segTotal;
+    DevBuf<unsigned> segCounter;
+    int ensureSegments();
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -312,6 +320,7 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             it.screenBase = screen;
             it.n = n;
             it.blockBase = block;
+            it.pixBase = int64_t(pix);
             it.tileBase = tile;
             it.tilesX = tilesX;
             it.tilesY = tilesY;
@@ -402,6 +411,33 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
 
 // Sizes the pair capacity from this step's real pair count (projection, depth sort and scan
 // run once more inside the step itself).
+// Checkpoint storage of the segmented backward for the current pair capacity and LDS batch;
+// a reallocation or a switch of mode invalidates the captured graph. The segmented walk is
+// switched off in the batched engine (kSegmentedBatched): its first GPU run did not finish and
+// the cause is not established; the batched engine keeps the whole-list walk, whose many
+// concurrent views already keep the GPU busy.
+constexpr bool kSegmentedBatched = false;
+
+int BatchEngine::ensureSegments()
+{
+    const bool want = kSegmentedBatched && ctx->segmentedBackward && st.tile_size == 16;
+    if (want != segmented)
+        dropGraph();
+    segmented = want;
+    if (!segmented)
+        return U3D_OK;
+    const void *before[3] = {ckpt.p, segCount.p, segStart.p};
+    const size_t slots = size_t(pairCap) / size_t(tuning.lds_batch) + size_t(totalTiles) + 1;
+    U3D_TRY(ckpt.ensure(slots * 256));
+    U3D_TRY(segCount.ensure(size_t(totalTiles) + 1));
+    U3D_TRY(segStart.ensure(size_t(totalTiles) + 1));
+    U3D_TRY(segTotal.ensure(4));
+    U3D_TRY(segCounter.ensure(4));
+    if (before[0] != ckpt.p || before[1] != segCount.p || before[2] != segStart.p)
+        dropGraph();
+    return U3D_OK;
+}
+
 int BatchEngine::probePairs()
 {
     const hipStream_t s = ctx->stream;
@@ -492,9 +528,11 @@ int BatchEngine::enqueue()
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(counters.p), 0, 2));
         const int occ = perCu > 0 ? perCu : occFwd[li];
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
+        const SegmentState seg = segmented ? SegmentState{ckpt.p, segCount.p}
+                                           : SegmentState{nullptr, nullptr};
         U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, nullptr, lds, 0));
+                                    st, nullptr, lds, 0, seg));
     }
     {
         StageScope scope(ctx, U3D_STAGE_LOSS);
@@ -503,11 +541,23 @@ int BatchEngine::enqueue()
     }
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
-        const int occ = perCu > 0 ? perCu : occBwd[li];
-        const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
-        U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks, counters.p + 1,
-                                    int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, grad2d.p, lds, tuning.atomic_lanes));
+        if (segmented) {
+            U3D_TRY(exclusiveScanU32(s, segCount.p, segStart.p, totalTiles, segTotal.p,
+                                     scanScratch.p, scanScratch.cap));
+            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(segCounter.p), 0, 1));
+            const int occ = perCu > 0 ? perCu : occSeg;
+            U3D_TRY(launchRasterSegmentedBatched(s, dItems, numItems, totalTiles, segStart.p,
+                                                 segTotal.p, segCounter.p, ctx->num_cus * occ,
+                                                 tileOffsets.p, tileList, sb, st, ckpt.p,
+                                                 grad2d.p, lds));
+        } else {
+            const int occ = perCu > 0 ? perCu : occBwd[li];
+            const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
+            U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks,
+                                        counters.p + 1, int(std::max<int64_t>(wgs, 1)),
+                                        tileOffsets.p, tileList, sb, st, grad2d.p, lds,
+                                        tuning.atomic_lanes, SegmentState{nullptr, nullptr}));
+        }
     }
     {
         StageScope scope(ctx, U3D_STAGE_CHAIN);
@@ -602,6 +652,7 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
     if (totalEntries > 0) {
         if (pairCap == 0)
             U3D_TRY(probePairs());
+        U3D_TRY(ensureSegments());
         if (useGraph) {
             if (!exec)
                 U3D_TRY(capture());

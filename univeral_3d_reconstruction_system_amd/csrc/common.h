@@ -178,6 +178,7 @@ struct u3d_ctx {
     u3d::PinnedScratch pinned;
     u3d::DevBuf<unsigned char> scratch; // shared short-lived device scratch (loss partials, ...)
     u3d::StageTimer timer;
+    bool segmentedBackward = true; // U3D_OPT_SEGMENTED_BACKWARD
 };
 
 namespace u3d {

@@ -28,6 +28,7 @@ struct RenderItem {
     int64_t screenBase; // first screen entry (Gaussian i of the item is entry screenBase + i)
     int64_t n;          // Gaussians of the scene
     int64_t blockBase;  // first global 16x16 raster block
+    int64_t pixBase;    // first pixel of the item in the concatenated per-pixel buffers
     int tileBase, tilesX, tilesY, blocksX;
     int shDegree;
     float ndcScale;     // densification statistic per view: undoes the 1/B of the averaged loss
@@ -139,9 +140,32 @@ int launchTileOffsets(hipStream_t s, const uint32_t *pairTile, int64_t numPairs,
 struct RasterGeometry {
     int width, height, tileSize, tilesX, tilesY;
 };
+// Forward state the segmented backward starts from (16x16 tiles; ckpt == nullptr: not kept).
+// For a tile t whose list starts at pair `begin`, checkpoint s >= 1 of its pixel p,
+// ckpt[((begin / B) + t + s) * 256 + p] (B = LDS batch), holds the transmittance before list
+// position s * B and the colour composited from that position on; segCount[t] = ceil(max
+// contributor count of the tile / B). Capacity: (K / B + tiles + 1) * 256 entries.
+struct SegmentState {
+    float4 *ckpt;
+    uint32_t *segCount;
+};
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
-                        float *rgb, float *finalT, int *nContrib);
+                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg);
+// Segmented backward: one work item per (tile, segment of B = 256 list positions); segStart =
+// exclusive scan of segCount (tiles + 1 entries), *totalItems its total; counter zero on entry.
+int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                          const uint32_t *segStart, const uint32_t *totalItems, unsigned *counter,
+                          int numWorkgroups, const int *tileOffsets, const int *tileList,
+                          const ScreenBuffers &sb, const float *dl, const float *finalT,
+                          const int *nContrib, const float4 *ckpt, float *grad2d);
+int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
+                                 const uint32_t *segStart, const uint32_t *totalItems,
+                                 unsigned *counter, int numWorkgroups, const int *tileOffsets,
+                                 const int *tileList, const ScreenBuffers &sb,
+                                 const u3d_raster_settings &st, const float4 *ckpt,
+                                 float *grad2d, int ldsBatch);
+int rasterSegmentedOccupancy();
 int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                          const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                          const float *dlDrgb, const float *finalT, const int *nContrib,
@@ -156,7 +180,7 @@ int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, i
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
                         const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
-                        int ldsBatch, int fewLanes);
+                        int ldsBatch, int fewLanes, const SegmentState &seg);
 
 // Resident workgroups per CU of the persistent compositing kernel (occupancy limit).
 int rasterBatchedOccupancy(bool backward, int ldsBatch);

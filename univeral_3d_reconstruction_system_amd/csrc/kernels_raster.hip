@@ -108,7 +108,7 @@ __device__ __forceinline__ void forwardBlock(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib)
+    int *__restrict__ nContrib, const SegmentState &seg)
 {
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
@@ -126,11 +126,19 @@ __device__ __forceinline__ void forwardBlock(
     const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
     const float fpx = float(px), fpy = float(py);
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    int last = 0;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f; // colour of the current segment (checkpoints only)
+    int last = 0, batches = 0;
     bool done = !inside;
+    const size_t slot0 = size_t(begin / kBatch + tile);
     for (int base = begin; base < end; base += kBatch) {
         if (__syncthreads_and(done))
             break;
+        if (seg.ckpt && base > begin) { // T before this batch and the previous segment's colour
+            if (inside)
+                seg.ckpt[(slot0 + size_t(batches)) * kThreads + tid] = make_float4(T, sr, sg, sb);
+            sr = sg = sb = 0.0f;
+        }
+        ++batches;
         const int idx = base + tid;
         uint32_t m = 0;
         if (tid < kBatch && idx < end) {
@@ -177,6 +185,11 @@ __device__ __forceinline__ void forwardBlock(
                 cr += b.z * w;
                 cg += b.w * w;
                 cb += s2[j] * w;
+                if (seg.ckpt) {
+                    sr += b.z * w;
+                    sg += b.w * w;
+                    sb += s2[j] * w;
+                }
                 T = nextT;
                 last = base - begin + j + 1;
             }
@@ -191,6 +204,210 @@ __device__ __forceinline__ void forwardBlock(
         rgb[pix * 3 + 2] = cb + T * bg2;
         finalT[pix] = T;
         nContrib[pix] = last;
+        if (seg.ckpt) {
+            // Reverse pass over this pixel's checkpoints: replace each segment colour by the
+            // colour of everything from that boundary on (suffix sums of positive terms, so
+            // the backward needs no cancelling difference).
+            float ar = sr, ag = sg, ab = sb;
+            for (int k = batches - 1; k >= 1; --k) {
+                float4 *c = seg.ckpt + (slot0 + size_t(k)) * kThreads + tid;
+                const float4 v = *c;
+                *c = make_float4(v.x, ar, ag, ab);
+                ar += v.y;
+                ag += v.z;
+                ab += v.w;
+            }
+        }
+    }
+    if (seg.ckpt) {
+        // segments of this tile the backward visits: ceil(max contributor count / kBatch)
+        __shared__ int sMax[kThreads / 64];
+        int m = last;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 0)
+            sMax[wave] = m;
+        __syncthreads();
+        if (tid == 0)
+            seg.segCount[tile] = uint32_t((max(max(sMax[0], sMax[1]), max(sMax[2], sMax[3])) + kBatch - 1) / kBatch);
+    }
+}
+
+// Reverse walk over one segment [lo, lo + kBatch) of a 16x16 tile's list. A pixel whose last
+// contributor lies in the segment starts from its final state, as backwardBlock; a pixel that
+// continues past the segment starts from the forward's checkpoint at the segment's end:
+// transmittance T there and the colour composited from there on, which divided by T is the
+// running "colour behind" of the reference's reverse recursion. Segments of a tile are
+// independent work items.
+template <int kBatch>
+__device__ __forceinline__ void backwardSegment(
+    int bx, int by, int tileBase, int segIdx, int W, int H, int tilesX,
+    const int *__restrict__ tileOffsets, const int *__restrict__ tileList,
+    const float4 *__restrict__ rec0, const float4 *__restrict__ rec1, const float *__restrict__ rec2,
+    float bg0, float bg1, float bg2, float minAlpha, const float *__restrict__ dl,
+    const float *__restrict__ finalTArr, const int *__restrict__ nContribArr,
+    const float4 *__restrict__ ckpt, float *__restrict__ grad2d)
+{
+    __shared__ float4 s0[kBatch];
+    __shared__ float4 s1[kBatch];
+    __shared__ float s2[kBatch];
+    __shared__ float sthr[kBatch];
+    __shared__ int sid[kBatch];
+    __shared__ uint32_t smask[kBatch];
+    __shared__ float acc[kBatch][kGrad2dStride];
+    __shared__ uint64_t stab[32];
+    __shared__ int waveMax[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid < 32)
+        stab[tid] = kExp2Table[tid];
+    for (int k = tid; k < kBatch * kGrad2dStride; k += kThreads)
+        (&acc[0][0])[k] = 0.0f;
+    const int bx0 = bx * kEdge, by0 = by * kEdge;
+    const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
+    const int tile = tileBase + by * tilesX + bx;
+    const bool inside = px < W && py < H;
+    const int begin = tileOffsets[tile];
+    const int lo = segIdx * kBatch, hi = lo + kBatch;
+    const size_t pix = inside ? size_t(py) * size_t(W) + size_t(px) : 0;
+    const float fT = inside ? finalTArr[pix] : 0.0f;
+    const int nc = inside ? nContribArr[pix] : 0;
+    const int jEnd = nc < hi ? nc : hi; // this pixel walks j = jEnd - 1 ... lo
+    const float dp0 = inside ? dl[pix * 3] : 0.0f;
+    const float dp1 = inside ? dl[pix * 3 + 1] : 0.0f;
+    const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
+    const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
+    const float fpx = float(px), fpy = float(py);
+    float T = fT;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+    if (nc > hi) {
+        const float4 ck = ckpt[(size_t(begin / kBatch + tile) + size_t(segIdx + 1)) * kThreads + tid];
+        T = ck.x;
+        if (T > 0.0f) {
+            acc0 = ck.y / T;
+            acc1 = ck.z / T;
+            acc2 = ck.w / T;
+        }
+    }
+    float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
+    {
+        int m = jEnd;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 0)
+            waveMax[wave] = m;
+    }
+    __syncthreads();
+    const int wm0 = waveMax[0], wm1 = waveMax[1], wm2 = waveMax[2], wm3 = waveMax[3];
+    const int maxN = max(max(wm0, wm1), max(wm2, wm3));
+    if (maxN <= lo)
+        return; // uniform: no pixel of the block reaches this segment
+    const int cnt = maxN - lo;
+    uint32_t m = 0;
+    if (tid < cnt) {
+        const int j = lo + tid;
+        const int g = tileList[begin + j];
+        const float4 a = rec0[g];
+        const float4 b = rec1[g];
+        const float thr = skipThreshold(minAlpha, b.y);
+        sid[tid] = g;
+        s0[tid] = a;
+        s1[tid] = b;
+        s2[tid] = rec2[g];
+        sthr[tid] = thr;
+        m = quadrantMask(a, b.x, thr, bx0, by0);
+        m &= (j < wm0 ? 1u : 0u) | (j < wm1 ? 2u : 0u) | (j < wm2 ? 4u : 0u) | (j < wm3 ? 8u : 0u);
+    }
+    if (tid < kBatch)
+        smask[tid] = m;
+    __syncthreads();
+    for (int c = kBatch / 64 - 1; c >= 0; --c) {
+        uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
+        while (bits) {
+            const int jj = c * 64 + 63 - __builtin_clzll(bits);
+            bits &= ~(uint64_t(1) << (jj - c * 64));
+            const int j = lo + jj;
+            float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
+            bool contrib = false;
+            if (j < jEnd) {
+                const float4 a = s0[jj];
+                const float4 b = s1[jj];
+                const float dx = a.x - fpx, dy = a.y - fpy;
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                if (!(power > 0.0f) && !(power < sthr[jj])) {
+                    const float G = expfExactLds(power, stab);
+                    const float op = b.y;
+                    const float rawAlpha = op * G;
+                    const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
+                    if (!(alpha < minAlpha)) {
+                        contrib = true;
+                        const float c0 = b.z, c1 = b.w, c2 = s2[jj];
+                        T = T / (1.0f - alpha);
+                        const float w = alpha * T;
+                        float dAlpha = 0.0f;
+                        acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
+                        last0 = c0;
+                        dAlpha += (c0 - acc0) * dp0;
+                        v5 = w * dp0;
+                        acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
+                        last1 = c1;
+                        dAlpha += (c1 - acc1) * dp1;
+                        v6 = w * dp1;
+                        acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
+                        last2 = c2;
+                        dAlpha += (c2 - acc2) * dp2;
+                        v7 = w * dp2;
+                        dAlpha *= T;
+                        lastAlpha = alpha;
+                        dAlpha += -fT / (1.0f - alpha) * bgDot;
+                        if (!(rawAlpha > kMaxAlpha)) {
+                            v8 = G * dAlpha;
+                            const float dPower = op * dAlpha * G;
+                            const float A = a.z, B = a.w, C = b.x;
+                            v0 = dPower * -(A * dx + B * dy);
+                            v1 = dPower * -(C * dy + B * dx);
+                            v2 = dPower * -0.5f * dx * dx;
+                            v3 = dPower * -dx * dy;
+                            v4 = dPower * -0.5f * dy * dy;
+                        }
+                    }
+                }
+            }
+            if (__ballot(contrib)) {
+                v0 = waveSum(v0);
+                v1 = waveSum(v1);
+                v2 = waveSum(v2);
+                v3 = waveSum(v3);
+                v4 = waveSum(v4);
+                v5 = waveSum(v5);
+                v6 = waveSum(v6);
+                v7 = waveSum(v7);
+                v8 = waveSum(v8);
+                if (lane == 0) {
+                    float *dst = acc[jj];
+                    atomicAdd(&dst[0], v0);
+                    atomicAdd(&dst[1], v1);
+                    atomicAdd(&dst[2], v2);
+                    atomicAdd(&dst[3], v3);
+                    atomicAdd(&dst[4], v4);
+                    atomicAdd(&dst[5], v5);
+                    atomicAdd(&dst[6], v6);
+                    atomicAdd(&dst[7], v7);
+                    atomicAdd(&dst[8], v8);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < cnt) {
+        float *dst = grad2d + size_t(sid[tid]) * kGrad2dStride;
+#pragma unroll
+        for (int k = 0; k < kGrad2dStride; ++k) {
+            const float x = acc[tid][k];
+            if (x != 0.0f)
+                unsafeAtomicAdd(&dst[k], x);
+        }
     }
 }
 
@@ -374,29 +591,6 @@ __device__ __forceinline__ void backwardBlock(
     }
 }
 
-__global__ __launch_bounds__(kThreads) void rasterForwardKernel(
-    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
-    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
-    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
-    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib)
-{
-    forwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
-                 tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT, nContrib);
-}
-
-__global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
-    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
-    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
-    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
-    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
-    const int *__restrict__ nContribArr, float *__restrict__ grad2d)
-{
-    backwardBlock<256, kPlainFewLanes>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
-                  tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
-                  grad2d);
-}
-
 // Index of the item owning global raster block b (items sorted by blockBase).
 __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items, int numItems,
                                            int64_t b)
@@ -412,6 +606,105 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
     return lo;
 }
 
+__global__ __launch_bounds__(kThreads) void rasterForwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib, SegmentState seg)
+{
+    forwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+                      tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT,
+                      nContrib, seg);
+}
+
+// First index in sorted[0..n) greater than w, minus one: the tile owning work item w.
+__device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart, int tiles, uint32_t w)
+{
+    int lo = 0, hi = tiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segStart[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Segmented backward of one image (16x16 tiles): workgroups take (tile, segment) items from
+// an atomic counter until *totalItems; segStart is the exclusive scan of the per-tile segment
+// counts written by the forward.
+__global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
+    int W, int H, int tilesX, int tiles, const uint32_t *__restrict__ segStart,
+    const uint32_t *__restrict__ totalItems, unsigned *counter, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    const int *__restrict__ nContribArr, const float4 *__restrict__ ckpt,
+    float *__restrict__ grad2d)
+{
+    __shared__ uint32_t sWork;
+    const uint32_t total = *totalItems;
+    for (;;) {
+        if (threadIdx.x == 0)
+            sWork = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = sWork;
+        __syncthreads();
+        if (w >= total)
+            break;
+        const int tile = tileOfItem(segStart, tiles, w);
+        backwardSegment<256>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H, tilesX,
+                             tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl,
+                             finalTArr, nContribArr, ckpt, grad2d);
+    }
+}
+
+// Batched form: tiles are global (16x16, so global tile = global raster block).
+template <int kBatch>
+__global__ __launch_bounds__(kThreads) void rasterSegmentedBatchedKernel(
+    const RenderItem *__restrict__ items, int numItems, int tiles,
+    const uint32_t *__restrict__ segStart, const uint32_t *__restrict__ totalItems,
+    unsigned *counter, const int *__restrict__ tileOffsets, const int *__restrict__ tileList,
+    const float4 *__restrict__ rec0, const float4 *__restrict__ rec1,
+    const float *__restrict__ rec2, const u3d_raster_settings st, const float4 *__restrict__ ckpt,
+    float *__restrict__ grad2d)
+{
+    __shared__ uint32_t sWork;
+    const uint32_t total = *totalItems;
+    for (;;) {
+        if (threadIdx.x == 0)
+            sWork = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = sWork;
+        __syncthreads();
+        if (w >= total)
+            break;
+        const int tile = tileOfItem(segStart, tiles, w);
+        const int v = itemOfBlock(items, numItems, tile);
+        const RenderItem &it = items[v];
+        const int local = int(tile - it.blockBase);
+        backwardSegment<kBatch>(local % it.blocksX, local / it.blocksX, it.tileBase,
+                                int(w - segStart[tile]), it.cam.width, it.cam.height, it.tilesX,
+                                tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
+                                st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
+                                it.nContrib, ckpt, grad2d);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    const int *__restrict__ nContribArr, float *__restrict__ grad2d)
+{
+    backwardBlock<256, kPlainFewLanes>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+                  tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
+                  grad2d);
+}
+
 // Persistent batched compositing: every workgroup takes the next block of the work order from
 // an atomic counter until the list is exhausted. The order lists heavy tiles first.
 template <bool kBackward, int kBatch, int kFewLanes>
@@ -420,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
     int64_t totalBlocks, unsigned *counter, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2,
-    const u3d_raster_settings st, float *__restrict__ grad2d)
+    const u3d_raster_settings st, float *__restrict__ grad2d, SegmentState seg)
 {
     __shared__ int64_t sWork;
     for (;;) {
@@ -440,7 +733,8 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
             forwardBlock<kBatch>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                          tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
                          st.background[1], st.background[2], st.min_alpha,
-                         st.min_transmittance, it.rgb, it.finalT, it.nContrib);
+                         st.min_transmittance, it.rgb, it.finalT, it.nContrib,
+                         seg);
         else
             backwardBlock<kBatch, kFewLanes>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                           tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
@@ -456,18 +750,18 @@ static void launchBatchedVariant(hipStream_t s, int numWorkgroups, const RenderI
                                  int numItems, const uint32_t *order, int64_t totalBlocks,
                                  unsigned *counter, const int *tileOffsets, const int *tileList,
                                  const ScreenBuffers &sb, const u3d_raster_settings &st,
-                                 float *grad2d)
+                                 float *grad2d, const SegmentState &seg)
 {
     hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch, kFewLanes>), dim3(numWorkgroups), dim3(kThreads),
                        0, s, items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
-                       sb.rec0, sb.rec1, sb.rec2, st, grad2d);
+                       sb.rec0, sb.rec1, sb.rec2, st, grad2d, seg);
 }
 
 int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
                         const uint32_t *order, int64_t totalBlocks, unsigned *counter,
                         int numWorkgroups, const int *tileOffsets, const int *tileList,
                         const ScreenBuffers &sb, const u3d_raster_settings &st, float *grad2d,
-                        int ldsBatch, int fewLanes)
+                        int ldsBatch, int fewLanes, const SegmentState &seg)
 {
     if (totalBlocks <= 0)
         return U3D_OK;
@@ -477,7 +771,7 @@ int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, i
               "atomic lane threshold must be 0, 4 or 16");
 #define U3D_BATCHED(B, L, F)                                                                     \
     launchBatchedVariant<B, L, F>(s, numWorkgroups, items, numItems, order, totalBlocks, counter, \
-                                  tileOffsets, tileList, sb, st, grad2d)
+                                  tileOffsets, tileList, sb, st, grad2d, seg)
 #define U3D_BATCHED_L(B, F)                                                                      \
     do {                                                                                         \
         if (ldsBatch == 64)                                                                      \
@@ -518,8 +812,9 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
 
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
-                        float *rgb, float *finalT, int *nContrib)
+                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg)
 {
+    U3D_CHECK(!seg.ckpt || g.tileSize == 16, U3D_ERR_INVALID, "segment checkpoints need 16-pixel tiles");
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
     if (blocksX * blocksY == 0)
         return U3D_OK;
@@ -527,7 +822,7 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
                        g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
                        sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
                        st.background[2], st.min_alpha, st.min_transmittance, rgb, finalT,
-                       nContrib);
+                       nContrib, seg);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
@@ -546,6 +841,58 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
                        st.background[2], st.min_alpha, dlDrgb, finalT, nContrib, grad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
+}
+
+int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
+                          const uint32_t *segStart, const uint32_t *totalItems, unsigned *counter,
+                          int numWorkgroups, const int *tileOffsets, const int *tileList,
+                          const ScreenBuffers &sb, const float *dl, const float *finalT,
+                          const int *nContrib, const float4 *ckpt, float *grad2d)
+{
+    const int tiles = g.tilesX * g.tilesY;
+    if (tiles == 0)
+        return U3D_OK;
+    U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
+    hipLaunchKernelGGL(rasterSegmentedKernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
+                       g.height, g.tilesX, tiles, segStart, totalItems, counter, tileOffsets, tileList,
+                       sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                       st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
+                                 const uint32_t *segStart, const uint32_t *totalItems,
+                                 unsigned *counter, int numWorkgroups, const int *tileOffsets,
+                                 const int *tileList, const ScreenBuffers &sb,
+                                 const u3d_raster_settings &st, const float4 *ckpt,
+                                 float *grad2d, int ldsBatch)
+{
+    if (tiles <= 0)
+        return U3D_OK;
+    U3D_CHECK(st.tile_size == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
+    if (ldsBatch == 64)
+        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<64>, dim3(numWorkgroups), dim3(kThreads), 0, s,
+                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
+                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
+    else if (ldsBatch == 128)
+        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<128>, dim3(numWorkgroups), dim3(kThreads), 0, s,
+                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
+                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
+    else
+        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<256>, dim3(numWorkgroups), dim3(kThreads), 0, s,
+                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
+                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int rasterSegmentedOccupancy()
+{
+    int blocks = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterSegmentedKernel, kThreads, 0);
+    return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
 } // namespace u3d

@@ -100,6 +100,12 @@ struct u3d_raster {
     u3d::DevBuf<int> tileOffsets;
     const int *tileList = nullptr;
     u3d::DevBuf<float> finalT, grad2d;
+    // segmented backward (16-pixel tiles): forward checkpoints and the work-item scan
+    bool segmented = false;
+    u3d::DevBuf<float4> ckpt;
+    u3d::DevBuf<uint32_t> segCount, segStart, segTotal;
+    u3d::DevBuf<unsigned> segCounter;
+    int occSeg = 0;
     u3d::DevBuf<int> nContrib;
     u3d::DevBuf<float> dbgMean2d, dbgConic, dbgColor, dbgOpacity;
     u3d::ScreenBuffers screen() const

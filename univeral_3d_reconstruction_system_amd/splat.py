@@ -33,6 +33,9 @@ class Context:
     def synchronize(self):
         check(self.lib.u3d_ctx_synchronize(self.h))
 
+    def set_option(self, option: int, value: int):
+        check(self.lib.u3d_ctx_set_option(self.h, int(option), int(value)))
+
     def _adopt(self, obj):
         self._children.add(obj)
 
