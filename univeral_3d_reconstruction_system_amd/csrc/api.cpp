@@ -152,10 +152,10 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
         U3D_TRY(r->scanScratch.ensure(scanScratchSize(int64_t(tiles) + 1)));
         if (!r->occSeg)
             r->occSeg = rasterSegmentedOccupancy();
-        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD);
         U3D_TRY(exclusiveScanU32(s, r->segCount.p, r->segStart.p, tiles, r->segTotal.p,
                                  r->scanScratch.p, r->scanScratch.cap));
         U3D_HIP(hipMemsetAsync(r->segCounter.p, 0, sizeof(unsigned), s));
+        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // the compositing kernel alone
         U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segStart.p, r->segTotal.p,
                                       r->segCounter.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,

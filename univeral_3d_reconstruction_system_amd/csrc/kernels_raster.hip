@@ -159,39 +159,53 @@ __device__ __forceinline__ void forwardBlock(
             continue;
         for (int c = 0; c < kBatch / 64; ++c) {
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
-            while (bits) {
-                const int j = c * 64 + __builtin_ctzll(bits);
+            if (!bits)
+                continue;
+            // Software-pipelined walk over the set bits: the next Gaussian's LDS records are
+            // loaded while the current one is composited (same Gaussians, same order).
+            int j = c * 64 + __builtin_ctzll(bits);
+            bits &= bits - 1;
+            float4 a = s0[j], b = s1[j];
+            float thr = sthr[j], bl = s2[j];
+            for (;;) {
+                const bool more = bits != 0;
+                const int jn = more ? c * 64 + __builtin_ctzll(bits) : j;
                 bits &= bits - 1;
-                if (done)
-                    continue;
-                const float4 a = s0[j];
-                const float dx = a.x - fpx, dy = a.y - fpy;
-                const float4 b = s1[j];
-                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                if (power > 0.0f)
-                    continue;
-                if (power < sthr[j])
-                    continue;
-                const float raw = b.y * expfExactLds(power, stab);
-                const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
-                if (alpha < minAlpha)
-                    continue;
-                const float nextT = T * (1.0f - alpha);
-                if (nextT < minT) {
-                    done = true;
-                    continue;
+                const float4 an = s0[jn], bn = s1[jn];
+                const float thrn = sthr[jn], bln = s2[jn];
+                if (!done) {
+                    const float dx = a.x - fpx, dy = a.y - fpy;
+                    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                    if (!(power > 0.0f) && !(power < thr)) {
+                        const float raw = b.y * expfExactLds(power, stab);
+                        const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
+                        if (!(alpha < minAlpha)) {
+                            const float nextT = T * (1.0f - alpha);
+                            if (nextT < minT) {
+                                done = true;
+                            } else {
+                                const float w = alpha * T;
+                                cr += b.z * w;
+                                cg += b.w * w;
+                                cb += bl * w;
+                                if (seg.ckpt) {
+                                    sr += b.z * w;
+                                    sg += b.w * w;
+                                    sb += bl * w;
+                                }
+                                T = nextT;
+                                last = base - begin + j + 1;
+                            }
+                        }
+                    }
                 }
-                const float w = alpha * T;
-                cr += b.z * w;
-                cg += b.w * w;
-                cb += s2[j] * w;
-                if (seg.ckpt) {
-                    sr += b.z * w;
-                    sg += b.w * w;
-                    sb += s2[j] * w;
-                }
-                T = nextT;
-                last = base - begin + j + 1;
+                if (!more)
+                    break;
+                j = jn;
+                a = an;
+                b = bn;
+                thr = thrn;
+                bl = bln;
             }
             if (__ballot(!done) == 0)
                 break;
@@ -324,25 +338,37 @@ __device__ __forceinline__ void backwardSegment(
     __syncthreads();
     for (int c = kBatch / 64 - 1; c >= 0; --c) {
         uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
-        while (bits) {
-            const int jj = c * 64 + 63 - __builtin_clzll(bits);
-            bits &= ~(uint64_t(1) << (jj - c * 64));
+        if (!bits)
+            continue;
+        // Software-pipelined reverse walk: the next Gaussian's LDS records are loaded
+        // while the current one is processed (same Gaussians, same order).
+        int jj = c * 64 + 63 - __builtin_clzll(bits);
+        bits &= ~(uint64_t(1) << (jj - c * 64));
+        float4 ra = s0[jj], rb = s1[jj];
+        float rthr = sthr[jj], rc2 = s2[jj];
+        for (;;) {
+            const bool more = bits != 0;
+            const int jn = more ? c * 64 + 63 - __builtin_clzll(bits) : jj;
+            if (more)
+                bits &= ~(uint64_t(1) << (jn - c * 64));
+            const float4 ran = s0[jn], rbn = s1[jn];
+            const float rthrn = sthr[jn], rc2n = s2[jn];
             const int j = lo + jj;
             float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
             bool contrib = false;
             if (j < jEnd) {
-                const float4 a = s0[jj];
-                const float4 b = s1[jj];
+                const float4 a = ra;
+                const float4 b = rb;
                 const float dx = a.x - fpx, dy = a.y - fpy;
                 const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                if (!(power > 0.0f) && !(power < sthr[jj])) {
+                if (!(power > 0.0f) && !(power < rthr)) {
                     const float G = expfExactLds(power, stab);
                     const float op = b.y;
                     const float rawAlpha = op * G;
                     const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
                     if (!(alpha < minAlpha)) {
                         contrib = true;
-                        const float c0 = b.z, c1 = b.w, c2 = s2[jj];
+                        const float c0 = b.z, c1 = b.w, c2 = rc2;
                         T = T / (1.0f - alpha);
                         const float w = alpha * T;
                         float dAlpha = 0.0f;
@@ -397,6 +423,13 @@ __device__ __forceinline__ void backwardSegment(
                     atomicAdd(&dst[8], v8);
                 }
             }
+            if (!more)
+                break;
+            jj = jn;
+            ra = ran;
+            rb = rbn;
+            rthr = rthrn;
+            rc2 = rc2n;
         }
     }
     __syncthreads();
@@ -486,25 +519,37 @@ __device__ __forceinline__ void backwardBlock(
         __syncthreads();
         for (int c = kBatch / 64 - 1; c >= 0; --c) {
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
-            while (bits) {
-                const int jj = c * 64 + 63 - __builtin_clzll(bits);
-                bits &= ~(uint64_t(1) << (jj - c * 64));
+            if (!bits)
+                continue;
+            // Software-pipelined reverse walk: the next Gaussian's LDS records are loaded
+            // while the current one is processed (same Gaussians, same order).
+            int jj = c * 64 + 63 - __builtin_clzll(bits);
+            bits &= ~(uint64_t(1) << (jj - c * 64));
+            float4 ra = s0[jj], rb = s1[jj];
+            float rthr = sthr[jj], rc2 = s2[jj];
+            for (;;) {
+                const bool more = bits != 0;
+                const int jn = more ? c * 64 + 63 - __builtin_clzll(bits) : jj;
+                if (more)
+                    bits &= ~(uint64_t(1) << (jn - c * 64));
+                const float4 ran = s0[jn], rbn = s1[jn];
+                const float rthrn = sthr[jn], rc2n = s2[jn];
                 const int j = lo + jj;
                 float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
                 bool contrib = false;
                 if (j < nc) {
-                    const float4 a = s0[jj];
-                    const float4 b = s1[jj];
+                    const float4 a = ra;
+                    const float4 b = rb;
                     const float dx = a.x - fpx, dy = a.y - fpy;
                     const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                    if (!(power > 0.0f) && !(power < sthr[jj])) {
+                    if (!(power > 0.0f) && !(power < rthr)) {
                         const float G = expfExactLds(power, stab);
                         const float op = b.y;
                         const float rawAlpha = op * G;
                         const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
                         if (!(alpha < minAlpha)) {
                             contrib = true;
-                            const float c0 = b.z, c1 = b.w, c2 = s2[jj];
+                            const float c0 = b.z, c1 = b.w, c2 = rc2;
                             T = T / (1.0f - alpha);
                             const float w = alpha * T;
                             float dAlpha = 0.0f;
@@ -573,6 +618,13 @@ __device__ __forceinline__ void backwardBlock(
                         atomicAdd(&dst[8], v8);
                     }
                 }
+                if (!more)
+                    break;
+                jj = jn;
+                ra = ran;
+                rb = rbn;
+                rthr = rthrn;
+                rc2 = rc2n;
             }
         }
         __syncthreads();
