@@ -235,8 +235,9 @@ def main(argv=None) -> int:
 
     stage_ms = {name: ms[k] / max(cnt[k], 1) for k, name in enumerate(_abi.STAGES)}
     per_step_ms = {name: ms[k] / steps for k, name in enumerate(_abi.STAGES)}
-    dominant = max(("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam"),
-                   key=lambda k: per_step_ms[k])
+    # The roofline is reported for the dominant single kernel (every stage but "sort" is one
+    # compositing, loss, projection or update launch; the sort stage is a chain of ~20 short
+    # radix/scan launches and one host read of the pair count, so it is listed in "stages").
     algo = dict(
         project=("hbm", BYTES_PROJECT_PER_GAUSSIAN * mean_n),
         adam=("hbm", BYTES_ADAM_PER_PARAM * 14 * mean_n),
@@ -246,12 +247,26 @@ def main(argv=None) -> int:
         raster_fwd=("mfma", FLOPS_FWD_PER_EVAL * evals),
         raster_bwd=("mfma", FLOPS_BWD_PER_EVAL * evals),
     )
-    bound, work = algo[dominant]
-    dur_s = stage_ms[dominant] / 1e3
-    if bound == "hbm":
-        achieved, peak, unit = work / dur_s / 1e9, PEAK_HBM_GBS, "GB/s"
-    else:
-        achieved, peak, unit = work / dur_s / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"
+
+    def stage_roof(name):
+        bound, work = algo[name]
+        dur_s = stage_ms[name] / 1e3
+        if bound == "hbm":
+            achieved, peak, unit = work / dur_s / 1e9, PEAK_HBM_GBS, "GB/s"
+        else:
+            achieved, peak, unit = work / dur_s / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"
+        return bound, achieved, peak, unit
+
+    stages = {}
+    for name in algo:
+        if cnt[_abi.STAGES.index(name)] == 0:
+            continue
+        bound, achieved, peak, unit = stage_roof(name)
+        stages[name] = dict(bound=bound, ms=round(stage_ms[name], 4), achieved=round(achieved, 4),
+                            unit=unit, frac=round(achieved / peak, 6))
+    dominant = max(("project", "raster_fwd", "loss", "raster_bwd", "chain_rule", "adam"),
+                   key=lambda k: per_step_ms[k])
+    bound, achieved, peak, unit = stage_roof(dominant)
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
     if os.path.exists(tf):
@@ -261,7 +276,8 @@ def main(argv=None) -> int:
                     frac=round(achieved / peak, 6), traffic=traffic,
                     note=("compute roof = MI355X FP32 peak (the compositing kernels run on the "
                           "FP32 vector ALU); work = algorithmic flops per evaluated pair x pairs")
-                    if bound == "mfma" else "work = algorithmic HBM bytes per launch")
+                    if bound == "mfma" else "work = algorithmic HBM bytes per launch",
+                    stages=stages)
 
     value = steps / elapsed
     batched = None
