@@ -76,42 +76,65 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     U3D_TRY(r->nContrib.ensure(P > 0 ? P : 1));
     const ScreenBuffers sb = r->screen();
 
+    U3D_TRY(r->pinnedTotal.ensure(sizeof(uint32_t)));
+    U3D_TRY(r->blockTiles.ensure(size_t(std::max(1, projectBlocks(n)))));
+    *static_cast<uint32_t *>(r->pinnedTotal.p) = 0;
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
-        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree));
+        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree, r->blockTiles.p));
+        // The pair count K is known after the projection; the host reads it (pinned memory)
+        // while the depth sort runs, so the device never waits for the host.
+        if (n > 0)
+            U3D_TRY(launchSumU32(s, r->blockTiles.p, projectBlocks(n), r->total.p,
+                                 static_cast<uint32_t *>(r->pinnedTotal.p)));
     }
+    U3D_TRY(r->totalReady.record(s));
     std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
-    // Depth order of the visible Gaussians (invisible keys sort last).
-    U3D_TRY(launchIota(s, r->valA.p, n));
+    // Depth order of the visible Gaussians (invisible keys sort last; ties keep index order).
     U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
     U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(nn))));
     RadixScratch rs{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
+    rs.valsAreIndex = true;
     bool alt = false;
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
     const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
-    U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n));
-    U3D_TRY(exclusiveScanU32(s, r->ntSorted.p, r->pairOffsets.p, n, r->total.p, r->scanScratch.p,
-                             r->scanScratch.cap));
-    uint32_t K = 0;
-    U3D_HIP(hipMemcpyAsync(&K, r->total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    U3D_HIP(hipStreamSynchronize(s));
+    U3D_TRY(r->totalReady.wait());
+    const uint32_t K = n > 0 ? *static_cast<const uint32_t *>(r->pinnedTotal.p) : 0u;
     r->numPairs = int64_t(K);
     const size_t kk = size_t(K > 0 ? K : 1);
-    U3D_TRY(r->pairTile.ensure(kk));
-    U3D_TRY(r->pairGauss.ensure(kk));
-    U3D_TRY(r->pairTileAlt.ensure(kk));
-    U3D_TRY(r->pairGaussAlt.ensure(kk));
-    U3D_TRY(r->hist.ensure(radixHistSize(int64_t(kk))));
-    U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(kk))));
-    RadixScratch rs2{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
-    U3D_TRY(launchEmitPairs(s, n, sorted, r->pairOffsets.p, r->rect.p, tilesX, r->pairTile.p,
-                            r->pairGauss.p, r->ntSorted.p));
-    bool alt2 = false;
-    U3D_TRY(radixSortPairs(s, r->pairTile.p, r->pairGauss.p, r->pairTileAlt.p, r->pairGaussAlt.p,
-                           int64_t(K), bitsFor(tiles), rs2, &alt2));
-    const uint32_t *tileSorted = alt2 ? r->pairTileAlt.p : r->pairTile.p;
-    r->tileList = reinterpret_cast<const int *>(alt2 ? r->pairGaussAlt.p : r->pairGauss.p);
-    U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
+    if (tileBinningGroup(n, tiles) > 0) {
+        U3D_TRY(r->tileListBuf.ensure(kk));
+        U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tiles))));
+        U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
+        if (!r->binDone.p) {
+            U3D_TRY(r->binDone.ensure(1));
+            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(r->binDone.p), 0, 1));
+        }
+        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tiles,
+                                  r->binCounts.p, r->binCounts.cap, r->tileTotal.p, r->binDone.p,
+                                  r->tileOffsets.p, r->tileListBuf.p));
+        r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
+    } else {
+        // Many tiles: pairs emitted in depth order, then a stable radix sort on the tile id.
+        U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n));
+        U3D_TRY(exclusiveScanU32(s, r->ntSorted.p, r->pairOffsets.p, n, r->total.p,
+                                 r->scanScratch.p, r->scanScratch.cap));
+        U3D_TRY(r->pairTile.ensure(kk));
+        U3D_TRY(r->pairGauss.ensure(kk));
+        U3D_TRY(r->pairTileAlt.ensure(kk));
+        U3D_TRY(r->pairGaussAlt.ensure(kk));
+        U3D_TRY(r->hist.ensure(radixHistSize(int64_t(kk))));
+        U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(kk))));
+        RadixScratch rs2{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
+        U3D_TRY(launchEmitPairs(s, n, sorted, r->pairOffsets.p, r->rect.p, tilesX, r->pairTile.p,
+                                r->pairGauss.p, r->ntSorted.p));
+        bool alt2 = false;
+        U3D_TRY(radixSortPairs(s, r->pairTile.p, r->pairGauss.p, r->pairTileAlt.p,
+                               r->pairGaussAlt.p, int64_t(K), bitsFor(tiles), rs2, &alt2));
+        const uint32_t *tileSorted = alt2 ? r->pairTileAlt.p : r->pairTile.p;
+        r->tileList = reinterpret_cast<const int *>(alt2 ? r->pairGaussAlt.p : r->pairGauss.p);
+        U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
+    }
     sortScope.reset();
     // Checkpoints for the segmented backward (16-pixel tiles): state at every 256-record batch
     // boundary of each tile list.

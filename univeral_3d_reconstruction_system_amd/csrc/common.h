@@ -122,6 +122,33 @@ struct PinnedScratch {
     }
 };
 
+// A timing-free HIP event, created on first record.
+struct SyncEvent {
+    hipEvent_t e = nullptr;
+    SyncEvent() = default;
+    SyncEvent(const SyncEvent &) = delete;
+    SyncEvent &operator=(const SyncEvent &) = delete;
+    ~SyncEvent()
+    {
+        if (e)
+            (void)hipEventDestroy(e);
+    }
+    int record(hipStream_t s)
+    {
+        hipError_t r = hipSuccess;
+        if (!e)
+            r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (r == hipSuccess)
+            r = hipEventRecord(e, s);
+        return r == hipSuccess ? U3D_OK : failHip(r, "hipEventRecord", __FILE__, __LINE__);
+    }
+    int wait()
+    {
+        const hipError_t r = hipEventSynchronize(e);
+        return r == hipSuccess ? U3D_OK : failHip(r, "hipEventSynchronize", __FILE__, __LINE__);
+    }
+};
+
 // Per-stage GPU time accumulated from HIP events recorded on the context's stream.
 struct StageTimer {
     bool enabled = false;

@@ -66,9 +66,11 @@ struct SceneStep {
 
 // ---- kernels_project.hip
 // shRest (nullable): 15 x 3 SH coefficients per Gaussian used up to shDegree (1..3).
+// blockTiles (optional, projectBlocks(n) entries): per projection block, the tiles covered.
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                   const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
-                  const float *shRest, int shDegree);
+                  const float *shRest, int shDegree, uint32_t *blockTiles = nullptr);
+int projectBlocks(int64_t n);
 // Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
 // grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
@@ -98,6 +100,7 @@ struct RadixScratch {
     uint32_t *scanTmp; // radixScanScratchSize(count)
     size_t scanTmpCap;
     int items = 0;     // keys per lane per block: 4 or 8 (0: by size)
+    bool valsAreIndex = false; // the input values are 0..count-1 (vals is then not read)
 };
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
@@ -109,10 +112,26 @@ int launchIota(hipStream_t s, uint32_t *vals, int64_t count);
 // out[i] = in[idx[i]] for i < count
 int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
                     int64_t count);
+// Sum of count u32 values (one workgroup) to *devOut and, if given, to *hostOut (pinned host
+// memory mapped for the device; visible to the host once the launch has completed).
+int launchSumU32(hipStream_t s, const uint32_t *in, int64_t count, uint32_t *devOut,
+                 uint32_t *hostOut);
 // Exclusive scan of count u32 values (out may alias in); total written to *total (device).
 int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t count,
                      uint32_t *total, uint32_t *blockScratch, size_t blockScratchCap);
 size_t scanScratchSize(int64_t count);
+// Tile binning of depth-ordered Gaussians (sorted[i] = Gaussian, invisible ones last): writes
+// each tile's list of Gaussian ids in depth order to tileList (tileOffsets[tiles] entries,
+// tileOffsets[0, tiles] exclusive) without materialising or sorting (tile, Gaussian) pairs.
+// Supports up to kMaxBinTiles tiles (one LDS counter per tile). counts: tileBinningCountsSize;
+// tileTotal: tiles entries; *doneCounter must be 0 before the first call and is left at 0.
+constexpr int kMaxBinTiles = 12288;
+int tileBinningGroup(int64_t n, int tiles); // Gaussians per group, 0 when unsupported
+size_t tileBinningCountsSize(int64_t n, int tiles);
+int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
+                      const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
+                      uint32_t *tileTotal, unsigned *doneCounter, int *tileOffsets,
+                      uint32_t *tileList);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,

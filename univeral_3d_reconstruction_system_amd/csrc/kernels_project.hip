@@ -10,8 +10,9 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Projects Gaussian i of a model and writes its screen state at entry o.
-__device__ __forceinline__ void projectBody(
+// Projects Gaussian i of a model and writes its screen state at entry o; returns the number of
+// tiles it covers.
+__device__ __forceinline__ uint32_t projectBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
@@ -91,6 +92,7 @@ __device__ __forceinline__ void projectBody(
     out.flags[o] = flags;
     out.depthKey[o] = key;
     out.nTiles[o] = nt;
+    return nt;
 }
 
 __global__ __launch_bounds__(kThreads) void projectKernel(
@@ -98,13 +100,30 @@ __global__ __launch_bounds__(kThreads) void projectKernel(
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
     int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
-    float3 campos)
+    float3 campos, uint32_t *blockTiles)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= n)
+    uint32_t nt = 0;
+    if (i < n)
+        nt = projectBody(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts,
+                         tilesX, tilesY, out, shRest, shDegree, campos);
+    if (!blockTiles)
         return;
-    projectBody(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts, tilesX, tilesY,
-                out, shRest, shDegree, campos);
+    // this block's sum of the covered tiles
+    __shared__ uint32_t waveSum[kThreads / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        nt += __shfl_xor(nt, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        waveSum[threadIdx.x >> 6] = nt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w)
+            sum += waveSum[w];
+        blockTiles[blockIdx.x] = sum;
+    }
 }
 
 // Batched projection: item blockIdx.y, Gaussian i of its scene -> entry screenBase + i; also
@@ -338,15 +357,18 @@ inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThr
 
 } // namespace
 
+int projectBlocks(int64_t n) { return n > 0 ? int(blocksFor(n)) : 0; }
+
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                   const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
-                  const float *shRest, int shDegree)
+                  const float *shRest, int shDegree, uint32_t *blockTiles)
 {
     if (m.n <= 0)
         return U3D_OK;
     hipLaunchKernelGGL(projectKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
                        m.scales, m.quats, m.opacities, m.colors, cam, st.near_plane, st.eps2d,
-                       st.tile_size, tilesX, tilesY, out, shRest, shDegree, cameraCentre(cam));
+                       st.tile_size, tilesX, tilesY, out, shRest, shDegree, cameraCentre(cam),
+                       blockTiles);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

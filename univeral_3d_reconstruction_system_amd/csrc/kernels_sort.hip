@@ -10,9 +10,16 @@
 //   3. emit (tile id, Gaussian) pairs in depth order;
 //   4. stable radix sort of the pairs on the tile-id bits;
 //   5. tile offsets from the sorted tile ids.
-// All integer work: HBM-bound. Each radix pass is upsweep (per-block digit histogram), a scan
-// of the digit-major histogram, and a downsweep that ranks keys stably inside each wavefront
-// with ballot-based digit matching and scatters them.
+// All integer work: HBM-bound. Each radix pass is three launches: an upsweep (per-block digit
+// histogram, digit-major), a row scan (one workgroup per digit scans that digit's counts over
+// the blocks and records the digit total), and a downsweep that adds the exclusive scan of the
+// 256 digit totals, ranks keys stably inside each wavefront with ballot-based digit matching and
+// scatters them.
+//
+// The plain (single-view) path replaces steps 2-5 with a direct tile binning of the depth-ordered
+// Gaussians (tileCountKernel / tileScanKernel / tileEmitKernel below): the pairs are generated
+// twice from the rectangles, once to count them per (group of Gaussians, tile) and once to write
+// each Gaussian id at its final slot, so the K pairs are written once (4 B each) and never sorted.
 #include "kernels.h"
 
 namespace u3d {
@@ -144,21 +151,91 @@ __global__ __launch_bounds__(kScanThreads) void scanSingleBlock(uint32_t *data, 
         *total = carry;
 }
 
-template <int kItems>
+// Exclusive scan in place of one digit's row hist[d][0, numBlocks) (one workgroup per digit);
+// the row total goes to digitTotal[d].
+__global__ __launch_bounds__(kRadixThreads) void radixRowScan(uint32_t *hist, int numBlocks,
+                                                              uint32_t *digitTotal)
+{
+    __shared__ uint32_t waveSums[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t *row = hist + size_t(blockIdx.x) * size_t(numBlocks);
+    constexpr int kPer = 4;
+    uint32_t carry = 0;
+    for (int start = 0; start < numBlocks; start += kRadixThreads * kPer) {
+        const int base = start + tid * kPer;
+        uint32_t v[kPer], local = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = base + k < numBlocks ? row[base + k] : 0u;
+            local += v[k];
+        }
+        uint32_t x = local;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            waveSums[wave] = x;
+        __syncthreads();
+        uint32_t waveOff = 0, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t ws = waveSums[w];
+            waveOff += w < wave ? ws : 0u;
+            chunk += ws;
+        }
+        uint32_t run = carry + waveOff + (x - local);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (base + k < numBlocks)
+                row[base + k] = run;
+            run += v[k];
+        }
+        carry += chunk;
+        __syncthreads();
+    }
+    if (tid == 0)
+        digitTotal[blockIdx.x] = carry;
+}
+
+// kIota: the values are the input positions (valsIn is not read).
+template <int kItems, bool kIota>
 __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn, uint32_t *keysOut,
     uint32_t *valsOut, int64_t count, int shift, int bits, int numBlocks,
-    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ countDev)
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digitTotal,
+    const uint32_t *__restrict__ countDev)
 {
     count = deviceCount(count, countDev);
     __shared__ uint32_t waveCount[kWaves][256];
     __shared__ uint32_t gbase[256];
+    __shared__ uint32_t waveTot[kWaves];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w)
         waveCount[w][tid] = 0;
-    gbase[tid] = hist[size_t(tid) * size_t(numBlocks) + blockIdx.x];
+    {
+        // digit base = exclusive scan of the digit totals + this block's row prefix
+        const uint32_t tot = digitTotal[tid];
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            waveTot[wave] = x;
+        __syncthreads();
+        uint32_t off = x - tot;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+            off += w < wave ? waveTot[w] : 0u;
+        gbase[tid] = off + hist[size_t(tid) * size_t(numBlocks) + blockIdx.x];
+    }
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
     const int64_t base = int64_t(blockIdx.x) * kRadixThreads * kItems + int64_t(wave) * 64 * kItems;
@@ -168,7 +245,7 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
         const int64_t idx = base + int64_t(it) * 64 + lane;
         const bool valid = idx < count;
         k[it] = valid ? keysIn[idx] : 0u;
-        v[it] = valid ? valsIn[idx] : 0u;
+        v[it] = valid ? (kIota ? uint32_t(idx) : valsIn[idx]) : 0u;
         const uint32_t d = (k[it] >> shift) & mask;
         const uint64_t vm = __ballot(valid);
         rank[it] = 0;
@@ -283,6 +360,91 @@ __global__ __launch_bounds__(256) void scanDownsweep(const uint32_t *__restrict_
     }
 }
 
+__global__ __launch_bounds__(1024) void sumKernel(const uint32_t *__restrict__ in, int64_t count,
+                                                  uint32_t *devOut, uint32_t *hostOut)
+{
+    __shared__ uint32_t ws[16];
+    const int tid = threadIdx.x;
+    uint32_t v = 0;
+    for (int64_t i = tid; i < count; i += 1024)
+        v += in[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    if ((tid & 63) == 0)
+        ws[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w)
+            t += ws[w];
+        *devOut = t;
+        if (hostOut)
+            __hip_atomic_store(hostOut, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Scan downsweep that sums the previous blocks' partial sums itself (for up to
+// kScanFusedBlocks blocks), so the scan needs no single-block pass; the last block writes the
+// total.
+constexpr int kScanFusedBlocks = 4096;
+
+__global__ __launch_bounds__(256) void scanDownsweepFused(const uint32_t *__restrict__ in,
+                                                          uint32_t *out, int64_t count,
+                                                          const uint32_t *__restrict__ blockSums,
+                                                          uint32_t *total)
+{
+    __shared__ uint32_t ws[4];
+    __shared__ uint32_t blockOff;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        uint32_t p = 0;
+        for (int j = tid; j < int(blockIdx.x); j += 256)
+            p += blockSums[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            p += __shfl_xor(p, off, 64);
+        if (lane == 0)
+            ws[wave] = p;
+        __syncthreads();
+        if (tid == 0) {
+            blockOff = ws[0] + ws[1] + ws[2] + ws[3];
+            if (total && blockIdx.x == gridDim.x - 1)
+                *total = blockOff + blockSums[blockIdx.x];
+        }
+        __syncthreads();
+    }
+    const int64_t base = int64_t(blockIdx.x) * kScanBlockItems + int64_t(tid) * 16;
+    uint32_t v[16];
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        v[k] = (base + k < count) ? in[base + k] : 0u;
+        local += v[k];
+    }
+    uint32_t x = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        ws[wave] = x;
+    __syncthreads();
+    uint32_t waveOff = 0;
+    for (int w = 0; w < wave; ++w)
+        waveOff += ws[w];
+    uint32_t run = blockOff + waveOff + (x - local);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (base + k < count)
+            out[base + k] = run;
+        run += v[k];
+    }
+}
+
 __global__ void emitPairsKernel(int64_t n, const uint32_t *__restrict__ sorted,
                                 const uint32_t *__restrict__ offsets,
                                 const int4 *__restrict__ rect, int tilesX, uint32_t *pairTile,
@@ -377,6 +539,257 @@ __global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t
         tileOffsets[u] = int(i);
 }
 
+// ---- Tile binning (plain path) ----
+// The depth-ordered Gaussians are cut into groups of `group` (a multiple of 256), one workgroup
+// per group; each of its W wavefronts takes a contiguous slice of the group. Within a slice the
+// pairs are enumerated in the order the per-tile lists need (Gaussian by Gaussian in depth
+// order, then the rectangle's tiles row by row), 64 pairs at a time: lane p finds its Gaussian
+// by a binary search over the 64 exclusive pair prefixes of the current 64 Gaussians in LDS.
+
+constexpr int kBinThreads = 256;      // counting: four wavefronts share the LDS counters
+constexpr int kBinScanThreads = 1024;
+constexpr int kBinLdsCounters = 12288; // u32 counters of the emission (W wavefronts x tiles)
+
+struct BinLds {
+    uint32_t pre[64];
+    int4 rect[64];
+    uint32_t gauss[64];
+};
+
+// LDS written by one lane and read by another lane of the same wavefront.
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Calls f(valid, tile, gaussian) for each 64-pair chunk of Gaussians [first, first + count) in
+// depth order (all lanes of the wavefront call f; wavefront-private LDS L).
+template <typename F>
+__device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restrict__ sorted,
+                                            const uint32_t *__restrict__ nTiles,
+                                            const int4 *__restrict__ rect, int tilesX,
+                                            int64_t first, int count, BinLds &L, F &&f)
+{
+    const int lane = threadIdx.x & 63;
+    for (int sub = 0; sub < count; sub += 64) {
+        const int64_t i = first + sub + lane;
+        uint32_t g = 0, nt = 0;
+        int4 r = make_int4(0, 0, 1, 1);
+        if (i < n) {
+            g = sorted[i];
+            nt = nTiles[g];
+            if (nt)
+                r = rect[g];
+        }
+        // Invisible Gaussians (no tiles) sort last: nothing follows a chunk without pairs.
+        if (__ballot(nt != 0) == 0)
+            break;
+        uint32_t x = nt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        const uint32_t pairs = __shfl(x, 63, 64);
+        waveSync();
+        L.pre[lane] = x - nt;
+        L.rect[lane] = r;
+        L.gauss[lane] = g;
+        waveSync();
+        for (uint32_t p0 = 0; p0 < pairs; p0 += 64) {
+            const uint32_t p = p0 + uint32_t(lane);
+            const bool valid = p < pairs;
+            int tile = 0;
+            uint32_t gg = 0;
+            if (valid) {
+                // last j with pre[j] <= p: a Gaussian with pairs (p < pairs)
+                int j = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1)
+                    j += L.pre[j + step] <= p ? step : 0;
+                const int4 rr = L.rect[j];
+                const uint32_t local = p - L.pre[j];
+                const uint32_t w = uint32_t(rr.z - rr.x);
+                const uint32_t ty = local / w;
+                tile = (rr.y + int(ty)) * tilesX + rr.x + int(local - ty * w);
+                gg = L.gauss[j];
+            }
+            f(valid, tile, gg);
+        }
+    }
+}
+
+// counts[grp * tiles + t] = pairs of group grp in tile t.
+__global__ __launch_bounds__(kBinThreads) void tileCountKernel(
+    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
+    const int4 *__restrict__ rect, int tilesX, int tiles, int group, uint32_t *counts)
+{
+    extern __shared__ uint32_t cnt[];
+    __shared__ BinLds L[kBinThreads / 64];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int t = tid; t < tiles; t += kBinThreads)
+        cnt[t] = 0;
+    __syncthreads();
+    const int slice = group / (kBinThreads / 64);
+    forEachPair(n, sorted, nTiles, rect, tilesX, int64_t(blockIdx.x) * group + wave * slice, slice,
+                L[wave], [&](bool valid, int tile, uint32_t) {
+                    if (valid)
+                        atomicAdd(&cnt[tile], 1u);
+                });
+    __syncthreads();
+    uint32_t *row = counts + size_t(blockIdx.x) * size_t(tiles);
+    for (int t = tid; t < tiles; t += kBinThreads)
+        row[t] = cnt[t];
+}
+
+// Per tile, the exclusive scan of counts over the groups (in place) and the tile total; the last
+// workgroup to finish turns the totals into tileOffsets[0, tiles] and re-arms the counter.
+// Workgroup: 64 tiles (one per lane) x 16 wavefronts, each wavefront a contiguous range of
+// groups read 16 rows at a time.
+__global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *counts, int numGroups,
+                                                                  int tiles, uint32_t *tileTotal,
+                                                                  unsigned *doneCounter,
+                                                                  int *tileOffsets)
+{
+    constexpr int kW = kBinScanThreads / 64;
+    constexpr int kRows = 16;
+    __shared__ uint32_t waveSum[kW][64];
+    __shared__ uint32_t carry[kW];
+    __shared__ bool last;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int t = int(blockIdx.x) * 64 + lane;
+    const bool tileOk = t < tiles;
+    const int per = (numGroups + kW - 1) / kW;
+    const int g0 = min(numGroups, wave * per), g1 = min(numGroups, g0 + per);
+    uint32_t sum = 0;
+    if (tileOk)
+        for (int g = g0; g < g1; g += kRows) {
+            uint32_t v[kRows];
+#pragma unroll
+            for (int k = 0; k < kRows; ++k)
+                v[k] = g + k < g1 ? counts[size_t(g + k) * size_t(tiles) + t] : 0u;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k)
+                sum += v[k];
+        }
+    waveSum[wave][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) {
+        const uint32_t v = waveSum[w][lane];
+        run += w < wave ? v : 0u;
+        total += v;
+    }
+    if (tileOk) {
+        for (int g = g0; g < g1; g += kRows) {
+            uint32_t v[kRows];
+#pragma unroll
+            for (int k = 0; k < kRows; ++k)
+                v[k] = g + k < g1 ? counts[size_t(g + k) * size_t(tiles) + t] : 0u;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                if (g + k < g1)
+                    counts[size_t(g + k) * size_t(tiles) + t] = run;
+                run += v[k];
+            }
+        }
+        if (wave == 0)
+            tileTotal[t] = total;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0)
+        last = atomicAdd(doneCounter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    // exclusive scan of the tile totals, 1024 at a time
+    uint32_t base = 0;
+    for (int start = 0; start < tiles; start += kBinScanThreads) {
+        const int u = start + tid;
+        const uint32_t v = u < tiles ? tileTotal[u] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            carry[wave] = x;
+        __syncthreads();
+        uint32_t off = base, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) {
+            off += w < wave ? carry[w] : 0u;
+            chunk += carry[w];
+        }
+        if (u < tiles)
+            tileOffsets[u] = int(off + x - v);
+        base += chunk;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        tileOffsets[tiles] = int(base);
+        *doneCounter = 0u;
+    }
+}
+
+// Writes every Gaussian id at its final slot of its tile's list. W = blockDim.x / 64 wavefronts
+// split the group; each first counts its slice per tile (cur[w][t]), the counts become per-wave
+// cursors (tile offset + earlier groups + earlier slices), then each emits its slice in order,
+// ranking equal tiles inside a 64-pair chunk by ballot matching.
+__global__ __launch_bounds__(256) void tileEmitKernel(
+    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
+    const int4 *__restrict__ rect, int tilesX, int tiles, int tileBits, int group,
+    const uint32_t *__restrict__ counts, const int *__restrict__ tileOffsets, uint32_t *tileList)
+{
+    extern __shared__ uint32_t cur[]; // [W][tiles]
+    __shared__ BinLds L[4];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int W = int(blockDim.x) >> 6;
+    const int slice = group / W;
+    const int64_t first = int64_t(blockIdx.x) * group + int64_t(wave) * slice;
+    uint32_t *mine = cur + size_t(wave) * size_t(tiles);
+    for (int t = tid; t < W * tiles; t += int(blockDim.x))
+        cur[t] = 0;
+    __syncthreads();
+    if (W > 1)
+        forEachPair(n, sorted, nTiles, rect, tilesX, first, slice, L[wave],
+                    [&](bool valid, int tile, uint32_t) {
+                        if (valid)
+                            atomicAdd(&mine[tile], 1u);
+                    });
+    __syncthreads();
+    const uint32_t *row = counts + size_t(blockIdx.x) * size_t(tiles);
+    for (int t = tid; t < tiles; t += int(blockDim.x)) {
+        uint32_t base = uint32_t(tileOffsets[t]) + row[t];
+        for (int w = 0; w < W; ++w) {
+            const uint32_t c = cur[size_t(w) * size_t(tiles) + t];
+            cur[size_t(w) * size_t(tiles) + t] = base;
+            base += c;
+        }
+    }
+    __syncthreads();
+    forEachPair(n, sorted, nTiles, rect, tilesX, first, slice, L[wave],
+                [&](bool valid, int tile, uint32_t g) {
+                    const uint64_t vm = __ballot(valid);
+                    const uint64_t m = matchDigit(uint32_t(tile), vm, tileBits);
+                    if (valid) {
+                        const uint32_t below = uint32_t(__popcll(m & laneMaskLt()));
+                        const uint32_t base = mine[tile];
+                        tileList[base + below] = g;
+                        if (below == 0)
+                            mine[tile] = base + uint32_t(__popcll(m));
+                    }
+                    waveSync();
+                });
+}
+
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
 
 } // namespace
@@ -386,7 +799,7 @@ size_t radixHistSize(int64_t count)
     return size_t(256) * size_t(blocksFor(count, kRadixThreads * kMinItems) + 1);
 }
 
-size_t radixScanScratchSize(int64_t count) { return scanScratchSize(int64_t(radixHistSize(count))); }
+size_t radixScanScratchSize(int64_t) { return 256; } // the digit totals
 
 namespace {
 template <int kItems>
@@ -396,16 +809,24 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
 {
     const int numBlocks = blocksFor(count, kRadixThreads * kItems);
     U3D_CHECK(size_t(numBlocks) * 256 <= scratch.histCap, U3D_ERR_CAPACITY, "radix histogram scratch too small");
+    U3D_CHECK(scratch.scanTmpCap >= 256, U3D_ERR_CAPACITY, "radix digit-total scratch too small");
     uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
     for (int shift = 0; shift < bits; shift += 8) {
         const int b = bits - shift < 8 ? bits - shift : 8;
         hipLaunchKernelGGL(radixUpsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
                            count, shift, b, numBlocks, scratch.hist, countDev);
         U3D_LAUNCH_CHECK();
-        U3D_TRY(exclusiveScanU32(s, scratch.hist, scratch.hist, int64_t(numBlocks) * 256, nullptr,
-                                 scratch.scanTmp, scratch.scanTmpCap));
-        hipLaunchKernelGGL(radixDownsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
-                           vin, kout, vout, count, shift, b, numBlocks, scratch.hist, countDev);
+        hipLaunchKernelGGL(radixRowScan, dim3(256), dim3(kRadixThreads), 0, s, scratch.hist,
+                           numBlocks, scratch.scanTmp);
+        U3D_LAUNCH_CHECK();
+        if (shift == 0 && scratch.valsAreIndex)
+            hipLaunchKernelGGL((radixDownsweep<kItems, true>), dim3(numBlocks), dim3(kRadixThreads),
+                               0, s, kin, vin, kout, vout, count, shift, b, numBlocks, scratch.hist,
+                               scratch.scanTmp, countDev);
+        else
+            hipLaunchKernelGGL((radixDownsweep<kItems, false>), dim3(numBlocks), dim3(kRadixThreads),
+                               0, s, kin, vin, kout, vout, count, shift, b, numBlocks, scratch.hist,
+                               scratch.scanTmp, countDev);
         U3D_LAUNCH_CHECK();
         uint32_t *tk = kin, *tv = vin;
         kin = kout;
@@ -455,22 +876,84 @@ int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint
 
 size_t scanScratchSize(int64_t count) { return size_t(blocksFor(count, kScanBlockItems)) + 2; }
 
+int launchSumU32(hipStream_t s, const uint32_t *in, int64_t count, uint32_t *devOut,
+                 uint32_t *hostOut)
+{
+    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, in, count, devOut, hostOut);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
 int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t count,
                      uint32_t *total, uint32_t *blockScratch, size_t blockScratchCap)
 {
     if (count <= 0) {
         if (total)
-            U3D_HIP(hipMemsetAsync(total, 0, sizeof(uint32_t), s));
+            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total), 0, 1));
         return U3D_OK;
     }
     const int nb = blocksFor(count, kScanBlockItems);
     U3D_CHECK(size_t(nb) + 1 <= blockScratchCap, U3D_ERR_CAPACITY, "scan scratch too small");
     hipLaunchKernelGGL(scanReduce, dim3(nb), dim3(256), 0, s, in, count, blockScratch);
     U3D_LAUNCH_CHECK();
+    if (nb <= kScanFusedBlocks) {
+        hipLaunchKernelGGL(scanDownsweepFused, dim3(nb), dim3(256), 0, s, in, out, count,
+                           blockScratch, total);
+        U3D_LAUNCH_CHECK();
+        return U3D_OK;
+    }
     hipLaunchKernelGGL(scanSingleBlock, dim3(1), dim3(kScanThreads), 0, s, blockScratch,
                        int64_t(nb), total);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(scanDownsweep, dim3(nb), dim3(256), 0, s, in, out, count, blockScratch);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int tileBinningGroup(int64_t n, int tiles)
+{
+    if (tiles <= 0 || tiles > kMaxBinTiles)
+        return 0;
+    // about 2^20 (group, tile) counters at most; a multiple of 256 (four wavefront slices)
+    const int64_t want = (n * int64_t(tiles) + (int64_t(1) << 20) - 1) >> 20;
+    int64_t g = want < 256 ? 256 : want;
+    g = (g + 255) / 256 * 256;
+    return int(g < (int64_t(1) << 20) ? g : (int64_t(1) << 20));
+}
+
+size_t tileBinningCountsSize(int64_t n, int tiles)
+{
+    const int g = tileBinningGroup(n, tiles);
+    return g > 0 ? size_t(blocksFor(n > 0 ? n : 1, g)) * size_t(tiles) : 0;
+}
+
+int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
+                      const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
+                      uint32_t *tileTotal, unsigned *doneCounter, int *tileOffsets,
+                      uint32_t *tileList)
+{
+    const int group = tileBinningGroup(n, tiles);
+    U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning supports 1.." + std::to_string(kMaxBinTiles) + " tiles");
+    if (n <= 0)
+        return launchFillI32(s, tileOffsets, 0, int64_t(tiles) + 1);
+    const int numGroups = blocksFor(n, group);
+    U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
+    static_assert(kMaxBinTiles <= kBinLdsCounters, "one wavefront's counters must fit");
+    hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(kBinThreads),
+                       size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect, tilesX, tiles,
+                       group, counts);
+    U3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tileScanKernel, dim3(blocksFor(tiles, 64)), dim3(kBinScanThreads), 0, s,
+                       counts, numGroups, tiles, tileTotal, doneCounter, tileOffsets);
+    U3D_LAUNCH_CHECK();
+    int tileBits = 1;
+    while ((1 << tileBits) < tiles)
+        ++tileBits;
+    // wavefronts per emission workgroup: as many slices as their counters fit in LDS
+    const int waves = tiles * 4 <= kBinLdsCounters ? 4 : (tiles * 2 <= kBinLdsCounters ? 2 : 1);
+    hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
+                       size_t(waves) * size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
+                       tilesX, tiles, tileBits, group, counts, tileOffsets, tileList);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

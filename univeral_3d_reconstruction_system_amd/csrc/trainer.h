@@ -99,6 +99,12 @@ struct u3d_raster {
     u3d::DevBuf<uint32_t> hist, histScan, scanScratch, total;
     u3d::DevBuf<int> tileOffsets;
     const int *tileList = nullptr;
+    // tile binning (tiles <= kMaxBinTiles): final lists, per-(group, tile) counters, the pair
+    // total read back early through pinned memory
+    u3d::DevBuf<uint32_t> tileListBuf, binCounts, tileTotal, blockTiles;
+    u3d::DevBuf<unsigned> binDone;
+    u3d::PinnedScratch pinnedTotal;
+    u3d::SyncEvent totalReady;
     u3d::DevBuf<float> finalT, grad2d;
     // segmented backward (16-pixel tiles): forward checkpoints and the work-item scan
     bool segmented = false;
