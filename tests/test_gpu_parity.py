@@ -72,11 +72,13 @@ def test_forward_tile_sizes_and_background(ctx, tile):
     _assert_forward_equal(r, img, ro, ref)
 
 
-def test_forward_many_tiles_pair_sort_path(ctx):
-    """12800 tiles: more than the tile binning's per-tile LDS counters (12288), so the lists come
-    from the emitted (tile, Gaussian) pairs and their stable radix sort."""
+@pytest.mark.parametrize("w,h", [(1024, 768), (1600, 1200), (2048, 1600)])
+def test_forward_many_tiles(ctx, w, h):
+    """3072 and 7500 tiles: the tile binning with four and one emitting wavefronts per group;
+    12800 tiles: more than its per-tile LDS counters (12288), so the lists come from the emitted
+    (tile, Gaussian) pairs and their stable radix sort."""
     g = ref_style_model(1500, 13, 1.0, 0.0, 0.02, 0.1)
-    cam = S.orbit_camera(2048, 1600, 1800.0, 0.3, 4.0, -0.4)
+    cam = S.orbit_camera(w, h, 1800.0 * w / 2048, 0.3, 4.0, -0.4)
     _, r, img, ro, ref = _forward_both(ctx, g, cam)
     _assert_forward_equal(r, img, ro, ref)
     assert r.state()["num_pairs"] == ro.num_pairs() > 0

@@ -106,12 +106,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         U3D_TRY(r->tileListBuf.ensure(kk));
         U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tiles))));
         U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
-        if (!r->binDone.p) {
-            U3D_TRY(r->binDone.ensure(1));
-            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(r->binDone.p), 0, 1));
-        }
         U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tiles,
-                                  r->binCounts.p, r->binCounts.cap, r->tileTotal.p, r->binDone.p,
+                                  r->binCounts.p, r->binCounts.cap, r->tileTotal.p,
                                   r->tileOffsets.p, r->tileListBuf.p));
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {

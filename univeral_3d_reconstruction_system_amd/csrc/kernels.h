@@ -124,14 +124,13 @@ size_t scanScratchSize(int64_t count);
 // each tile's list of Gaussian ids in depth order to tileList (tileOffsets[tiles] entries,
 // tileOffsets[0, tiles] exclusive) without materialising or sorting (tile, Gaussian) pairs.
 // Supports up to kMaxBinTiles tiles (one LDS counter per tile). counts: tileBinningCountsSize;
-// tileTotal: tiles entries; *doneCounter must be 0 before the first call and is left at 0.
+// tileTotal: tiles entries.
 constexpr int kMaxBinTiles = 12288;
 int tileBinningGroup(int64_t n, int tiles); // Gaussians per group, 0 when unsupported
 size_t tileBinningCountsSize(int64_t n, int tiles);
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, unsigned *doneCounter, int *tileOffsets,
-                      uint32_t *tileList);
+                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,

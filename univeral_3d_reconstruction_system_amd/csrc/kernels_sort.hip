@@ -546,8 +546,8 @@ __global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t
 // order, then the rectangle's tiles row by row), 64 pairs at a time: lane p finds its Gaussian
 // by a binary search over the 64 exclusive pair prefixes of the current 64 Gaussians in LDS.
 
-constexpr int kBinThreads = 256;      // counting: four wavefronts share the LDS counters
-constexpr int kBinScanThreads = 1024;
+constexpr int kBinWaves = 8;           // wavefronts per binning workgroup (counting)
+constexpr int kBinScanThreads = 256;
 constexpr int kBinLdsCounters = 12288; // u32 counters of the emission (W wavefronts x tiles)
 
 struct BinLds {
@@ -621,135 +621,116 @@ __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restric
     }
 }
 
-// counts[grp * tiles + t] = pairs of group grp in tile t.
-__global__ __launch_bounds__(kBinThreads) void tileCountKernel(
+// counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
+// groups reads contiguous rows). The group's kBinWaves wavefronts share the LDS counters.
+__global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tiles, int group, uint32_t *counts)
+    const int4 *__restrict__ rect, int tilesX, int tiles, int group, int numGroups,
+    uint32_t *counts)
 {
     extern __shared__ uint32_t cnt[];
-    __shared__ BinLds L[kBinThreads / 64];
+    __shared__ BinLds L[kBinWaves];
     const int tid = threadIdx.x, wave = tid >> 6;
-    for (int t = tid; t < tiles; t += kBinThreads)
+    for (int t = tid; t < tiles; t += 64 * kBinWaves)
         cnt[t] = 0;
     __syncthreads();
-    const int slice = group / (kBinThreads / 64);
+    const int slice = group / kBinWaves;
     forEachPair(n, sorted, nTiles, rect, tilesX, int64_t(blockIdx.x) * group + wave * slice, slice,
                 L[wave], [&](bool valid, int tile, uint32_t) {
                     if (valid)
                         atomicAdd(&cnt[tile], 1u);
                 });
     __syncthreads();
-    uint32_t *row = counts + size_t(blockIdx.x) * size_t(tiles);
-    for (int t = tid; t < tiles; t += kBinThreads)
-        row[t] = cnt[t];
+    for (int t = tid; t < tiles; t += 64 * kBinWaves)
+        counts[size_t(t) * size_t(numGroups) + blockIdx.x] = cnt[t];
 }
 
-// Per tile, the exclusive scan of counts over the groups (in place) and the tile total; the last
-// workgroup to finish turns the totals into tileOffsets[0, tiles] and re-arms the counter.
-// Workgroup: 64 tiles (one per lane) x 16 wavefronts, each wavefront a contiguous range of
-// groups read 16 rows at a time.
-__global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *counts, int numGroups,
-                                                                  int tiles, uint32_t *tileTotal,
-                                                                  unsigned *doneCounter,
-                                                                  int *tileOffsets)
+// Exclusive scan of `len` values at data (kBinScanThreads x 4 per round, whole workgroup), written
+// back in place or, when asInt is given, to asInt; returns the total.
+__device__ __forceinline__ uint32_t workgroupScan(uint32_t *data, int len, int *asInt,
+                                                  uint32_t *carry)
 {
     constexpr int kW = kBinScanThreads / 64;
-    constexpr int kRows = 16;
-    __shared__ uint32_t waveSum[kW][64];
-    __shared__ uint32_t carry[kW];
-    __shared__ bool last;
+    constexpr int kPer = 4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int t = int(blockIdx.x) * 64 + lane;
-    const bool tileOk = t < tiles;
-    const int per = (numGroups + kW - 1) / kW;
-    const int g0 = min(numGroups, wave * per), g1 = min(numGroups, g0 + per);
-    uint32_t sum = 0;
-    if (tileOk)
-        for (int g = g0; g < g1; g += kRows) {
-            uint32_t v[kRows];
+    {
+        uint32_t base = 0;
+        for (int start = 0; start < len; start += kBinScanThreads * kPer) {
+            const int i0 = start + tid * kPer;
+            uint32_t v[kPer], local = 0;
 #pragma unroll
-            for (int k = 0; k < kRows; ++k)
-                v[k] = g + k < g1 ? counts[size_t(g + k) * size_t(tiles) + t] : 0u;
+            for (int k = 0; k < kPer; ++k) {
+                v[k] = i0 + k < len ? data[i0 + k] : 0u;
+                local += v[k];
+            }
+            uint32_t x = local;
 #pragma unroll
-            for (int k = 0; k < kRows; ++k)
-                sum += v[k];
-        }
-    waveSum[wave][lane] = sum;
-    __syncthreads();
-    uint32_t run = 0, total = 0;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(x, off, 64);
+                if (lane >= off)
+                    x += y;
+            }
+            if (lane == 63)
+                carry[wave] = x;
+            __syncthreads();
+            uint32_t run = base, chunk = 0;
 #pragma unroll
-    for (int w = 0; w < kW; ++w) {
-        const uint32_t v = waveSum[w][lane];
-        run += w < wave ? v : 0u;
-        total += v;
-    }
-    if (tileOk) {
-        for (int g = g0; g < g1; g += kRows) {
-            uint32_t v[kRows];
+            for (int w = 0; w < kW; ++w) {
+                run += w < wave ? carry[w] : 0u;
+                chunk += carry[w];
+            }
+            run += x - local;
 #pragma unroll
-            for (int k = 0; k < kRows; ++k)
-                v[k] = g + k < g1 ? counts[size_t(g + k) * size_t(tiles) + t] : 0u;
-#pragma unroll
-            for (int k = 0; k < kRows; ++k) {
-                if (g + k < g1)
-                    counts[size_t(g + k) * size_t(tiles) + t] = run;
+            for (int k = 0; k < kPer; ++k) {
+                if (i0 + k < len) {
+                    if (asInt)
+                        asInt[i0 + k] = int(run);
+                    else
+                        data[i0 + k] = run;
+                }
                 run += v[k];
             }
+            base += chunk;
+            __syncthreads();
         }
-        if (wave == 0)
-            tileTotal[t] = total;
+        return base;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-        last = atomicAdd(doneCounter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
-    // exclusive scan of the tile totals, 1024 at a time
-    uint32_t base = 0;
-    for (int start = 0; start < tiles; start += kBinScanThreads) {
-        const int u = start + tid;
-        const uint32_t v = u < tiles ? tileTotal[u] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off)
-                x += y;
-        }
-        if (lane == 63)
-            carry[wave] = x;
-        __syncthreads();
-        uint32_t off = base, chunk = 0;
-#pragma unroll
-        for (int w = 0; w < kW; ++w) {
-            off += w < wave ? carry[w] : 0u;
-            chunk += carry[w];
-        }
-        if (u < tiles)
-            tileOffsets[u] = int(off + x - v);
-        base += chunk;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        tileOffsets[tiles] = int(base);
-        *doneCounter = 0u;
-    }
+}
+
+// One workgroup per tile: exclusive scan of its row of group counts (in place), total to
+// tileTotal[t].
+__global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *counts, int numGroups,
+                                                                  uint32_t *tileTotal)
+{
+    __shared__ uint32_t carry[kBinScanThreads / 64];
+    const uint32_t total = workgroupScan(counts + size_t(blockIdx.x) * size_t(numGroups),
+                                         numGroups, nullptr, carry);
+    if (threadIdx.x == 0)
+        tileTotal[blockIdx.x] = total;
+}
+
+// tileOffsets[0, tiles] = exclusive scan of the tile totals (one workgroup).
+__global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_t *tileTotal,
+                                                                         int tiles,
+                                                                         int *tileOffsets)
+{
+    __shared__ uint32_t carry[kBinScanThreads / 64];
+    const uint32_t all = workgroupScan(tileTotal, tiles, tileOffsets, carry);
+    if (threadIdx.x == 0)
+        tileOffsets[tiles] = int(all);
 }
 
 // Writes every Gaussian id at its final slot of its tile's list. W = blockDim.x / 64 wavefronts
 // split the group; each first counts its slice per tile (cur[w][t]), the counts become per-wave
 // cursors (tile offset + earlier groups + earlier slices), then each emits its slice in order,
 // ranking equal tiles inside a 64-pair chunk by ballot matching.
-__global__ __launch_bounds__(256) void tileEmitKernel(
+__global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tiles, int tileBits, int group,
+    const int4 *__restrict__ rect, int tilesX, int tiles, int tileBits, int group, int numGroups,
     const uint32_t *__restrict__ counts, const int *__restrict__ tileOffsets, uint32_t *tileList)
 {
     extern __shared__ uint32_t cur[]; // [W][tiles]
-    __shared__ BinLds L[4];
+    __shared__ BinLds L[kBinWaves];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int W = int(blockDim.x) >> 6;
     const int slice = group / W;
@@ -765,9 +746,8 @@ __global__ __launch_bounds__(256) void tileEmitKernel(
                             atomicAdd(&mine[tile], 1u);
                     });
     __syncthreads();
-    const uint32_t *row = counts + size_t(blockIdx.x) * size_t(tiles);
     for (int t = tid; t < tiles; t += int(blockDim.x)) {
-        uint32_t base = uint32_t(tileOffsets[t]) + row[t];
+        uint32_t base = uint32_t(tileOffsets[t]) + counts[size_t(t) * size_t(numGroups) + blockIdx.x];
         for (int w = 0; w < W; ++w) {
             const uint32_t c = cur[size_t(w) * size_t(tiles) + t];
             cur[size_t(w) * size_t(tiles) + t] = base;
@@ -786,7 +766,6 @@ __global__ __launch_bounds__(256) void tileEmitKernel(
                         if (below == 0)
                             mine[tile] = base + uint32_t(__popcll(m));
                     }
-                    waveSync();
                 });
 }
 
@@ -914,11 +893,12 @@ int tileBinningGroup(int64_t n, int tiles)
 {
     if (tiles <= 0 || tiles > kMaxBinTiles)
         return 0;
-    // about 2^20 (group, tile) counters at most; a multiple of 256 (four wavefront slices)
+    // about 2^20 (group, tile) counters at most; 64 Gaussians per wavefront slice at least
+    constexpr int64_t kUnit = 64 * kBinWaves;
     const int64_t want = (n * int64_t(tiles) + (int64_t(1) << 20) - 1) >> 20;
-    int64_t g = want < 256 ? 256 : want;
-    g = (g + 255) / 256 * 256;
-    return int(g < (int64_t(1) << 20) ? g : (int64_t(1) << 20));
+    int64_t g = want < kUnit ? kUnit : want;
+    g = (g + kUnit - 1) / kUnit * kUnit;
+    return int(g < (int64_t(1) << 22) ? g : (int64_t(1) << 22));
 }
 
 size_t tileBinningCountsSize(int64_t n, int tiles)
@@ -929,8 +909,7 @@ size_t tileBinningCountsSize(int64_t n, int tiles)
 
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, unsigned *doneCounter, int *tileOffsets,
-                      uint32_t *tileList)
+                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList)
 {
     const int group = tileBinningGroup(n, tiles);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning supports 1.." + std::to_string(kMaxBinTiles) + " tiles");
@@ -939,21 +918,26 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     const int numGroups = blocksFor(n, group);
     U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
     static_assert(kMaxBinTiles <= kBinLdsCounters, "one wavefront's counters must fit");
-    hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(kBinThreads),
+    hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(64 * kBinWaves),
                        size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect, tilesX, tiles,
-                       group, counts);
+                       group, numGroups, counts);
     U3D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tileScanKernel, dim3(blocksFor(tiles, 64)), dim3(kBinScanThreads), 0, s,
-                       counts, numGroups, tiles, tileTotal, doneCounter, tileOffsets);
+    hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
+                       tileTotal);
+    U3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tileOffsetsScanKernel, dim3(1), dim3(kBinScanThreads), 0, s, tileTotal,
+                       tiles, tileOffsets);
     U3D_LAUNCH_CHECK();
     int tileBits = 1;
     while ((1 << tileBits) < tiles)
         ++tileBits;
     // wavefronts per emission workgroup: as many slices as their counters fit in LDS
-    const int waves = tiles * 4 <= kBinLdsCounters ? 4 : (tiles * 2 <= kBinLdsCounters ? 2 : 1);
+    int waves = kBinWaves;
+    while (waves > 1 && waves * tiles > kBinLdsCounters)
+        waves >>= 1;
     hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
                        size_t(waves) * size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
-                       tilesX, tiles, tileBits, group, counts, tileOffsets, tileList);
+                       tilesX, tiles, tileBits, group, numGroups, counts, tileOffsets, tileList);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
