@@ -74,8 +74,8 @@ def test_forward_tile_sizes_and_background(ctx, tile):
 
 @pytest.mark.parametrize("w,h", [(1024, 768), (1600, 1200), (2048, 1600)])
 def test_forward_many_tiles(ctx, w, h):
-    """3072 and 7500 tiles: the tile binning with four and one emitting wavefronts per group;
-    12800 tiles: more than its per-tile LDS counters (12288), so the lists come from the emitted
+    """64x48 and 100x75 tiles: the tile binning with two and one emitting wavefronts per group;
+    128x100 tiles: more grid cells than its LDS holds (12288), so the lists come from the emitted
     (tile, Gaussian) pairs and their stable radix sort."""
     g = ref_style_model(1500, 13, 1.0, 0.0, 0.02, 0.1)
     cam = S.orbit_camera(w, h, 1800.0 * w / 2048, 0.3, 4.0, -0.4)
@@ -177,7 +177,8 @@ def test_backward_matches_fp64_finite_differences(ctx):
 
 def test_loss_bit_exact(ctx):
     rng = np.random.default_rng(3)
-    for (W, H, lam) in ((21, 17, 1.0), (320, 240, 0.2), (97, 33, 0.2), (64, 48, 0.0)):
+    for (W, H, lam) in ((21, 17, 1.0), (320, 240, 0.2), (97, 33, 0.2), (64, 48, 0.0), (7, 5, 0.2),
+                        (1, 1, 0.2), (40, 3, 0.2), (3, 70, 0.5)):
         tgt = rng.uniform(0, 1, W * H * 3).astype(np.float32)
         ren = (0.5 * tgt + 0.5 * rng.uniform(0, 1, W * H * 3)).astype(np.float32)
         grad = U.DeviceArray(ctx, (W * H * 3,))

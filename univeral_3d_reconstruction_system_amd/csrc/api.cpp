@@ -102,11 +102,11 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     const uint32_t K = n > 0 ? *static_cast<const uint32_t *>(r->pinnedTotal.p) : 0u;
     r->numPairs = int64_t(K);
     const size_t kk = size_t(K > 0 ? K : 1);
-    if (tileBinningGroup(n, tiles) > 0) {
+    if (tileBinningGroup(n, tilesX, tilesY) > 0) {
         U3D_TRY(r->tileListBuf.ensure(kk));
-        U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tiles))));
+        U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tilesX, tilesY))));
         U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
-        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tiles,
+        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tilesY,
                                   r->binCounts.p, r->binCounts.cap, r->tileTotal.p,
                                   r->tileOffsets.p, r->tileListBuf.p));
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);

@@ -123,13 +123,12 @@ size_t scanScratchSize(int64_t count);
 // Tile binning of depth-ordered Gaussians (sorted[i] = Gaussian, invisible ones last): writes
 // each tile's list of Gaussian ids in depth order to tileList (tileOffsets[tiles] entries,
 // tileOffsets[0, tiles] exclusive) without materialising or sorting (tile, Gaussian) pairs.
-// Supports up to kMaxBinTiles tiles (one LDS counter per tile). counts: tileBinningCountsSize;
-// tileTotal: tiles entries.
-constexpr int kMaxBinTiles = 12288;
-int tileBinningGroup(int64_t n, int tiles); // Gaussians per group, 0 when unsupported
-size_t tileBinningCountsSize(int64_t n, int tiles);
+// Supports tile grids with (tilesX + 1) (tilesY + 1) <= 12288 (one LDS grid cell per tile).
+// counts: tileBinningCountsSize entries; tileTotal: tilesX * tilesY entries.
+int tileBinningGroup(int64_t n, int tilesX, int tilesY); // Gaussians per group, 0: unsupported
+size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
-                      const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
+                      const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
                       uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot.

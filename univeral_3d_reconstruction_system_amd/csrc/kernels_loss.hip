@@ -46,29 +46,36 @@ __device__ __forceinline__ double blockSumD(double v, double *red)
 
 // One 32x16 block of one channel; writes its three double partials at partials[0..2].
 // gradScale multiplies the stored gradient (kScaled only: views averaged in a batched step).
+//
+// Every blur sums all 11 taps in the reference's order over maps that are exactly zero outside
+// the image, which equals the reference's truncated window bit for bit: an added term w * 0 is
+// +0, and a running sum that starts at +0 never becomes -0 in round-to-nearest, so x + (+0) = x.
+// The inputs are zero outside the image, so H1 of an outside row is +0; the SSIM partials are
+// stored as 0 outside the image, so H2 of an outside row is +0 as well.
 template <bool kU8, bool kScaled>
 __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
                                           const float *__restrict__ targetF,
                                           const uint8_t *__restrict__ targetU8, int W, int H,
                                           float lambda, float invCount, float gradScale,
-                                          const float *__restrict__ winW, int blocksX, int blockId,
-                                          int c, float *__restrict__ grad,
-                                          double *__restrict__ partials)
+                                          const Window &win, int blocksX, int blockId, int c,
+                                          float *__restrict__ grad, double *__restrict__ partials)
 {
-    // The inputs are dead once the moments are blurred, so the last stage reuses their space
-    // (58 KB of LDS in total).
-    static_assert(3 * kGH * kGW <= 2 * kIH * kIW, "LDS alias");
-    __shared__ float sInG[2 * kIH * kIW];
-    __shared__ float sH[5][kHH][kHW];
-    __shared__ float sA[3][kAH][kAW];
-    auto sIn = reinterpret_cast<float(*)[kIH][kIW]>(sInG);
-    auto sG = reinterpret_cast<float(*)[kGH][kGW]>(sInG);
-    __shared__ double red[kThreads / 64];
-    __shared__ float sw[2 * kR + 1];
+    // LDS (45 KB): the inputs (render, target) are dead once H1 is done, so the SSIM partials
+    // reuse their space; the H1 maps are dead once V1 is done, so H2 reuses theirs.
+    constexpr int kInBytes = kIH * kIW * 8, kABytes = 3 * kAH * kAW * 4;
+    constexpr int kHBytes = 5 * kHH * kHW * 4, kGBytes = 3 * kGH * kGW * 4;
+    __shared__ __align__(16) unsigned char ldsA[kInBytes > kABytes ? kInBytes : kABytes];
+    __shared__ __align__(16) unsigned char ldsB[kHBytes > kGBytes ? kHBytes : kGBytes];
+    __shared__ double red[3][kThreads / 64];
+    auto sIn = reinterpret_cast<float2(*)[kIW]>(ldsA);
+    auto sA = reinterpret_cast<float(*)[kAH][kAW]>(ldsA);
+    auto sH = reinterpret_cast<float(*)[kHH][kHW]>(ldsB);
+    auto sG = reinterpret_cast<float(*)[kGH][kGW]>(ldsB);
     const int tid = threadIdx.x;
-    if (tid < 2 * kR + 1)
-        sw[tid] = winW[tid];
-    __syncthreads();
+    float w[2 * kR + 1];
+#pragma unroll
+    for (int t = 0; t <= 2 * kR; ++t)
+        w[t] = win.w[t];
     const int ox = (blockId % blocksX) * kTW, oy = (blockId / blocksX) * kTH;
     const bool ssimOn = lambda > 0.0f;
     const float dLds = -lambda * invCount;
@@ -79,130 +86,145 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
             return float(targetU8[idx]) * (1.0f / 255.0f);
         return targetF[idx];
     };
-    {
-        if (ssimOn) {
-            // 1. inputs with a 10-pixel halo
-            for (int k = tid; k < kIH * kIW; k += kThreads) {
-                const int ly = k / kIW, lx = k % kIW;
-                const int gx = ox - 2 * kR + lx, gy = oy - 2 * kR + ly;
-                float a = 0.0f, b = 0.0f;
-                if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-                    const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
-                    a = render[idx];
-                    b = tgt(idx);
-                }
-                sIn[0][ly][lx] = a;
-                sIn[1][ly][lx] = b;
+    auto inImage = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
+    if (ssimOn) {
+        // 1. inputs with a 10-pixel halo (all loads issued before the LDS stores)
+        constexpr int kIn = kIH * kIW, kInIter = (kIn + kThreads - 1) / kThreads;
+        float2 v[kInIter];
+#pragma unroll
+        for (int j = 0; j < kInIter; ++j) {
+            const int k = tid + j * kThreads;
+            const int ly = k / kIW, lx = k - ly * kIW;
+            const int gx = ox - 2 * kR + lx, gy = oy - 2 * kR + ly;
+            v[j] = make_float2(0.0f, 0.0f);
+            if (k < kIn && inImage(gx, gy)) {
+                const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
+                v[j] = make_float2(render[idx], tgt(idx));
             }
-            __syncthreads();
-            // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94)
-            for (int k = tid; k < kHH * kHW; k += kThreads) {
-                const int ly = k / kHW, lx = k % kHW;
-                const int gx = ox - kR + lx, gy = oy - 2 * kR + ly;
-                if (gx < 0 || gx >= W || gy < 0 || gy >= H)
-                    continue;
-                float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
-                const int k0 = max(-kR, -gx), k1 = min(kR, W - 1 - gx);
-                for (int t = k0; t <= k1; ++t) {
-                    const float a = sIn[0][ly][lx + kR + t], b = sIn[1][ly][lx + kR + t];
-                    const float w = sw[t + kR];
-                    sx += w * a;
-                    sy += w * b;
-                    sxx += w * a * a;
-                    syy += w * b * b;
-                    sxy += w * a * b;
-                }
-                sH[0][ly][lx] = sx;
-                sH[1][ly][lx] = sy;
-                sH[2][ly][lx] = sxx;
-                sH[3][ly][lx] = syy;
-                sH[4][ly][lx] = sxy;
+        }
+#pragma unroll
+        for (int j = 0; j < kInIter; ++j) {
+            const int k = tid + j * kThreads;
+            if (k < kIn)
+                sIn[k / kIW][k % kIW] = v[j];
+        }
+        __syncthreads();
+        // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94)
+        for (int k = tid; k < kHH * kHW; k += kThreads) {
+            const int ly = k / kHW, lx = k - ly * kHW;
+            float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+#pragma unroll
+            for (int t = 0; t <= 2 * kR; ++t) {
+                const float2 ab = sIn[ly][lx + t];
+                const float a = ab.x, b = ab.y;
+                sx += w[t] * a;
+                sy += w[t] * b;
+                sxx += w[t] * a * a;
+                syy += w[t] * b * b;
+                sxy += w[t] * a * b;
             }
-            __syncthreads();
-            // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139)
-            for (int k = tid; k < kAH * kAW; k += kThreads) {
-                const int ly = k / kAW, lx = k % kAW;
-                const int gx = ox - kR + lx, gy = oy - kR + ly;
-                if (gx < 0 || gx >= W || gy < 0 || gy >= H)
-                    continue;
+            sH[0][ly][lx] = sx;
+            sH[1][ly][lx] = sy;
+            sH[2][ly][lx] = sxx;
+            sH[3][ly][lx] = syy;
+            sH[4][ly][lx] = sxy;
+        }
+        __syncthreads();
+        // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139); 0 outside
+        for (int k = tid; k < kAH * kAW; k += kThreads) {
+            const int ly = k / kAW, lx = k - ly * kAW;
+            const int gx = ox - kR + lx, gy = oy - kR + ly;
+            float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+            if (inImage(gx, gy)) {
                 float m[5] = {0, 0, 0, 0, 0};
-                const int k0 = max(-kR, -gy), k1 = min(kR, H - 1 - gy);
-                for (int t = k0; t <= k1; ++t) {
-                    const float w = sw[t + kR];
+#pragma unroll
+                for (int t = 0; t <= 2 * kR; ++t) {
 #pragma unroll
                     for (int j = 0; j < 5; ++j)
-                        m[j] += w * sH[j][ly + kR + t][lx];
+                        m[j] += w[t] * sH[j][ly + t][lx];
                 }
                 const float mx = m[0], my = m[1];
                 const float sx = m[2] - mx * mx, sy = m[3] - my * my, sxy = m[4] - mx * my;
                 const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
                 const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
-                const float v = (a1 * a2) / (b1 * b2);
+                const float vv = (a1 * a2) / (b1 * b2);
                 if (lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
-                    ssim += double(v);
-                const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - v * (2 * mx / b1 - 2 * mx / b2);
-                sA[0][ly][lx] = dLds * dmu;
-                sA[1][ly][lx] = dLds * (-v / b2);
-                sA[2][ly][lx] = dLds * (2 * a1 / (b1 * b2));
+                    ssim += double(vv);
+                const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - vv * (2 * mx / b1 - 2 * mx / b2);
+                p0 = dLds * dmu;
+                p1 = dLds * (-vv / b2);
+                p2 = dLds * (2 * a1 / (b1 * b2));
             }
-            __syncthreads();
-            // 4. horizontal blur of the partials (Loss.cpp:142-157)
-            for (int k = tid; k < kGH * kGW; k += kThreads) {
-                const int ly = k / kGW, lx = k % kGW;
-                const int gx = ox + lx, gy = oy - kR + ly;
-                if (gx >= W || gy < 0 || gy >= H)
-                    continue;
-                float acc[3] = {0, 0, 0};
-                const int k0 = max(-kR, -gx), k1 = min(kR, W - 1 - gx);
-                for (int t = k0; t <= k1; ++t) {
-                    const float w = sw[t + kR];
+            // sA aliases the inputs, which every thread finished reading before the barrier
+            sA[0][ly][lx] = p0;
+            sA[1][ly][lx] = p1;
+            sA[2][ly][lx] = p2;
+        }
+        __syncthreads();
+        // 4. horizontal blur of the partials (Loss.cpp:142-157)
+        for (int k = tid; k < kGH * kGW; k += kThreads) {
+            const int ly = k / kGW, lx = k - ly * kGW;
+            float acc[3] = {0, 0, 0};
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        acc[j] += w * sA[j][ly][lx + kR + t];
-                }
+            for (int t = 0; t <= 2 * kR; ++t) {
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
-                    sG[j][ly][lx] = acc[j];
+                    acc[j] += w[t] * sA[j][ly][lx + t];
             }
-            __syncthreads();
-        }
-        // 5. L1 term, vertical blur of the partials and the chain rule (Loss.cpp:64-74, 160-177)
-        for (int k = tid; k < kTH * kTW; k += kThreads) {
-            const int ly = k / kTW, lx = k % kTW;
-            const int gx = ox + lx, gy = oy + ly;
-            if (gx >= W || gy >= H)
-                continue;
-            const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
-            const float r = render[idx], t = tgt(idx);
-            const float d = r - t;
-            l1 += double(fabsf(d));
-            const double dd = double(r) - double(t);
-            sq += dd * dd;
-            float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
-            if (ssimOn) {
-                float g[3] = {0, 0, 0};
-                const int k0 = max(-kR, -gy), k1 = min(kR, H - 1 - gy);
-                for (int tt = k0; tt <= k1; ++tt) {
-                    const float w = sw[tt + kR];
+            // sG aliases the H1 maps, dead since the last barrier
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        g[j] += w * sG[j][ly + kR + tt][lx];
-                }
-                gval += g[0] + 2.0f * r * g[1] + t * g[2];
-            }
-            if (kScaled)
-                gval *= gradScale;
-            grad[idx] = gval;
+            for (int j = 0; j < 3; ++j)
+                sG[j][ly][lx] = acc[j];
         }
         __syncthreads();
     }
-    const double s1 = blockSumD(l1, red);
-    const double s2 = blockSumD(ssim, red);
-    const double s3 = blockSumD(sq, red);
-    if (tid == 0) {
-        partials[0] = s1;
-        partials[1] = s2;
-        partials[2] = s3;
+    // 5. L1 term, vertical blur of the partials and the chain rule (Loss.cpp:64-74, 160-177)
+    for (int k = tid; k < kTH * kTW; k += kThreads) {
+        const int ly = k / kTW, lx = k - ly * kTW;
+        const int gx = ox + lx, gy = oy + ly;
+        if (gx >= W || gy >= H)
+            continue;
+        const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
+        const float r = render[idx], t = tgt(idx);
+        const float d = r - t;
+        l1 += double(fabsf(d));
+        const double dd = double(r) - double(t);
+        sq += dd * dd;
+        float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
+        if (ssimOn) {
+            float g[3] = {0, 0, 0};
+#pragma unroll
+            for (int tt = 0; tt <= 2 * kR; ++tt) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    g[j] += w[tt] * sG[j][ly + tt][lx];
+            }
+            gval += g[0] + 2.0f * r * g[1] + t * g[2];
+        }
+        if (kScaled)
+            gval *= gradScale;
+        grad[idx] = gval;
+    }
+    // block sums of the three doubles (fixed order)
+    double v3[3] = {l1, ssim, sq};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v3[j] += __shfl_xor(v3[j], off, 64);
+    }
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            red[j][tid >> 6] = v3[j];
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double sum = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kThreads / 64; ++wv)
+            sum += red[tid][wv];
+        partials[tid] = sum;
     }
 }
 
@@ -215,7 +237,7 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
                                                        float *__restrict__ grad,
                                                        double *__restrict__ partials)
 {
-    lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win.w, blocksX,
+    lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win, blocksX,
                           int(blockIdx.x), int(blockIdx.y), grad,
                           partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
@@ -228,7 +250,7 @@ __global__ __launch_bounds__(kThreads) void lossBatchedKernel(const LossItem *__
     if (int(blockIdx.x) >= it.numBlocks)
         return;
     lossBlock<true, true>(it.render, nullptr, it.target, it.width, it.height, lambda, it.invCount,
-                          it.gradScale, win.w, it.blocksX, int(blockIdx.x), int(blockIdx.y),
+                          it.gradScale, win, it.blocksX, int(blockIdx.x), int(blockIdx.y),
                           it.grad, it.partials + (size_t(blockIdx.y) * it.numBlocks + blockIdx.x) * 3);
 }
 

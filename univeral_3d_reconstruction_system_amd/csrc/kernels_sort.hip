@@ -540,19 +540,23 @@ __global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t
 }
 
 // ---- Tile binning (plain path) ----
-// The depth-ordered Gaussians are cut into groups of `group` (a multiple of 256), one workgroup
-// per group; each of its W wavefronts takes a contiguous slice of the group. Within a slice the
-// pairs are enumerated in the order the per-tile lists need (Gaussian by Gaussian in depth
-// order, then the rectangle's tiles row by row), 64 pairs at a time: lane p finds its Gaussian
-// by a binary search over the 64 exclusive pair prefixes of the current 64 Gaussians in LDS.
+// The depth-ordered Gaussians are cut into groups of `group` (a multiple of 512), one workgroup
+// per group; each of its W wavefronts takes a contiguous slice of the group. Per-tile pair counts
+// come from the rectangles directly: +1 / -1 at the four corners of each rectangle in a 2-D
+// difference grid of (tilesY + 1) x (tilesX + 1) cells, then 2-D prefix sums. The emission
+// enumerates each slice's pairs in the order the per-tile lists need (Gaussian by Gaussian in
+// depth order, then the rectangle's tiles row by row), 64 pairs at a time: lane p finds its
+// Gaussian by a binary search over the 64 exclusive pair prefixes of the current 64 Gaussians.
 
-constexpr int kBinWaves = 8;           // wavefronts per binning workgroup (counting)
+constexpr int kBinWaves = 8;           // wavefronts per binning workgroup
 constexpr int kBinScanThreads = 256;
-constexpr int kBinLdsCounters = 12288; // u32 counters of the emission (W wavefronts x tiles)
+constexpr int kBinLdsCells = 12288;    // u32 grid cells in LDS (W wavefronts x cells each)
 
 struct BinLds {
-    uint32_t pre[64];
-    int4 rect[64];
+    uint32_t pre[64];    // exclusive pair prefix inside the 64-Gaussian chunk
+    uint32_t origin[64]; // x0 | y0 << 16 of the rectangle
+    uint32_t width[64];  // rectangle width in tiles
+    float rcpWidth[64];  // approximately 1 / width
     uint32_t gauss[64];
 };
 
@@ -563,13 +567,69 @@ __device__ __forceinline__ void waveSync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Calls f(valid, tile, gaussian) for each 64-pair chunk of Gaussians [first, first + count) in
+// Corner updates of the difference grid D (row stride `stride`) for the rectangles of the
+// Gaussians at depth positions [first, first + count) (one wavefront).
+__device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
+                                         const uint32_t *__restrict__ nTiles,
+                                         const int4 *__restrict__ rect, int64_t first, int count,
+                                         uint32_t *D, int stride)
+{
+    const int lane = threadIdx.x & 63;
+    for (int sub = 0; sub < count; sub += 64) {
+        const int64_t i = first + sub + lane;
+        uint32_t nt = 0;
+        int4 r = make_int4(0, 0, 0, 0);
+        if (i < n) {
+            const uint32_t g = sorted[i];
+            nt = nTiles[g];
+            if (nt)
+                r = rect[g];
+        }
+        if (__ballot(nt != 0) == 0) // invisible Gaussians sort last
+            break;
+        if (nt) {
+            atomicAdd(&D[r.y * stride + r.x], 1u);
+            atomicAdd(&D[r.y * stride + r.z], ~0u); // -1 modulo 2^32
+            atomicAdd(&D[r.w * stride + r.x], ~0u);
+            atomicAdd(&D[r.w * stride + r.z], 1u);
+        }
+    }
+}
+
+// Inclusive 2-D prefix sums, in place, of `grids` grids of rows x cols cells stored back to back
+// (all threads of the workgroup; ends with a barrier).
+__device__ __forceinline__ void prefix2d(uint32_t *D, int grids, int rows, int cols)
+{
+    const int cells = rows * cols;
+    for (int task = threadIdx.x; task < grids * rows; task += int(blockDim.x)) {
+        uint32_t *row = D + (task / rows) * cells + (task % rows) * cols;
+        uint32_t run = 0;
+#pragma unroll 4
+        for (int x = 0; x < cols; ++x) {
+            run += row[x];
+            row[x] = run;
+        }
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < grids * cols; task += int(blockDim.x)) {
+        uint32_t *col = D + (task / cols) * cells + task % cols;
+        uint32_t run = 0;
+#pragma unroll 4
+        for (int y = 0; y < rows; ++y) {
+            run += col[y * cols];
+            col[y * cols] = run;
+        }
+    }
+    __syncthreads();
+}
+
+// Calls f(valid, tx, ty, gaussian) for each 64-pair chunk of Gaussians [first, first + count) in
 // depth order (all lanes of the wavefront call f; wavefront-private LDS L).
 template <typename F>
 __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restrict__ sorted,
                                             const uint32_t *__restrict__ nTiles,
-                                            const int4 *__restrict__ rect, int tilesX,
-                                            int64_t first, int count, BinLds &L, F &&f)
+                                            const int4 *__restrict__ rect, int64_t first,
+                                            int count, BinLds &L, F &&f)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -582,7 +642,6 @@ __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restric
             if (nt)
                 r = rect[g];
         }
-        // Invisible Gaussians (no tiles) sort last: nothing follows a chunk without pairs.
         if (__ballot(nt != 0) == 0)
             break;
         uint32_t x = nt;
@@ -593,15 +652,18 @@ __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restric
                 x += y;
         }
         const uint32_t pairs = __shfl(x, 63, 64);
+        const uint32_t width = uint32_t(r.z - r.x);
         waveSync();
         L.pre[lane] = x - nt;
-        L.rect[lane] = r;
+        L.origin[lane] = uint32_t(r.x) | (uint32_t(r.y) << 16);
+        L.width[lane] = width;
+        L.rcpWidth[lane] = __builtin_amdgcn_rcpf(float(width));
         L.gauss[lane] = g;
         waveSync();
         for (uint32_t p0 = 0; p0 < pairs; p0 += 64) {
             const uint32_t p = p0 + uint32_t(lane);
             const bool valid = p < pairs;
-            int tile = 0;
+            int tx = 0, ty = 0;
             uint32_t gg = 0;
             if (valid) {
                 // last j with pre[j] <= p: a Gaussian with pairs (p < pairs)
@@ -609,40 +671,51 @@ __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restric
 #pragma unroll
                 for (int step = 32; step > 0; step >>= 1)
                     j += L.pre[j + step] <= p ? step : 0;
-                const int4 rr = L.rect[j];
                 const uint32_t local = p - L.pre[j];
-                const uint32_t w = uint32_t(rr.z - rr.x);
-                const uint32_t ty = local / w;
-                tile = (rr.y + int(ty)) * tilesX + rr.x + int(local - ty * w);
+                const uint32_t w = L.width[j];
+                // local / w from the reciprocal (local < 2^14, so the estimate is off by at most
+                // one), then corrected on the remainder
+                uint32_t q = uint32_t(float(local) * L.rcpWidth[j]);
+                int rem = int(local) - int(q * w);
+                if (rem < 0) {
+                    --q;
+                    rem += int(w);
+                } else if (rem >= int(w)) {
+                    ++q;
+                    rem -= int(w);
+                }
+                const uint32_t o = L.origin[j];
+                tx = int(o & 0xffffu) + rem;
+                ty = int(o >> 16) + int(q);
                 gg = L.gauss[j];
             }
-            f(valid, tile, gg);
+            f(valid, tx, ty, gg);
         }
     }
 }
 
 // counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
-// groups reads contiguous rows). The group's kBinWaves wavefronts share the LDS counters.
+// groups reads contiguous rows).
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tiles, int group, int numGroups,
+    const int4 *__restrict__ rect, int tilesX, int tilesY, int group, int numGroups,
     uint32_t *counts)
 {
-    extern __shared__ uint32_t cnt[];
-    __shared__ BinLds L[kBinWaves];
+    extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
-    for (int t = tid; t < tiles; t += 64 * kBinWaves)
-        cnt[t] = 0;
+    const int stride = tilesX + 1, cells = stride * (tilesY + 1);
+    for (int t = tid; t < cells; t += 64 * kBinWaves)
+        D[t] = 0;
     __syncthreads();
     const int slice = group / kBinWaves;
-    forEachPair(n, sorted, nTiles, rect, tilesX, int64_t(blockIdx.x) * group + wave * slice, slice,
-                L[wave], [&](bool valid, int tile, uint32_t) {
-                    if (valid)
-                        atomicAdd(&cnt[tile], 1u);
-                });
+    addRects(n, sorted, nTiles, rect, int64_t(blockIdx.x) * group + int64_t(wave) * slice, slice, D,
+             stride);
     __syncthreads();
-    for (int t = tid; t < tiles; t += 64 * kBinWaves)
-        counts[size_t(t) * size_t(numGroups) + blockIdx.x] = cnt[t];
+    prefix2d(D, 1, tilesY + 1, stride);
+    for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
+        const int ty = t / tilesX, tx = t - ty * tilesX;
+        counts[size_t(t) * size_t(numGroups) + blockIdx.x] = D[ty * stride + tx];
+    }
 }
 
 // Exclusive scan of `len` values at data (kBinScanThreads x 4 per round, whole workgroup), written
@@ -653,48 +726,46 @@ __device__ __forceinline__ uint32_t workgroupScan(uint32_t *data, int len, int *
     constexpr int kW = kBinScanThreads / 64;
     constexpr int kPer = 4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        uint32_t base = 0;
-        for (int start = 0; start < len; start += kBinScanThreads * kPer) {
-            const int i0 = start + tid * kPer;
-            uint32_t v[kPer], local = 0;
+    uint32_t base = 0;
+    for (int start = 0; start < len; start += kBinScanThreads * kPer) {
+        const int i0 = start + tid * kPer;
+        uint32_t v[kPer], local = 0;
 #pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                v[k] = i0 + k < len ? data[i0 + k] : 0u;
-                local += v[k];
-            }
-            uint32_t x = local;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t y = __shfl_up(x, off, 64);
-                if (lane >= off)
-                    x += y;
-            }
-            if (lane == 63)
-                carry[wave] = x;
-            __syncthreads();
-            uint32_t run = base, chunk = 0;
-#pragma unroll
-            for (int w = 0; w < kW; ++w) {
-                run += w < wave ? carry[w] : 0u;
-                chunk += carry[w];
-            }
-            run += x - local;
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                if (i0 + k < len) {
-                    if (asInt)
-                        asInt[i0 + k] = int(run);
-                    else
-                        data[i0 + k] = run;
-                }
-                run += v[k];
-            }
-            base += chunk;
-            __syncthreads();
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = i0 + k < len ? data[i0 + k] : 0u;
+            local += v[k];
         }
-        return base;
+        uint32_t x = local;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            carry[wave] = x;
+        __syncthreads();
+        uint32_t run = base, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) {
+            run += w < wave ? carry[w] : 0u;
+            chunk += carry[w];
+        }
+        run += x - local;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (i0 + k < len) {
+                if (asInt)
+                    asInt[i0 + k] = int(run);
+                else
+                    data[i0 + k] = run;
+            }
+            run += v[k];
+        }
+        base += chunk;
+        __syncthreads();
     }
+    return base;
 }
 
 // One workgroup per tile: exclusive scan of its row of group counts (in place), total to
@@ -721,50 +792,53 @@ __global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_
 }
 
 // Writes every Gaussian id at its final slot of its tile's list. W = blockDim.x / 64 wavefronts
-// split the group; each first counts its slice per tile (cur[w][t]), the counts become per-wave
-// cursors (tile offset + earlier groups + earlier slices), then each emits its slice in order,
-// ranking equal tiles inside a 64-pair chunk by ballot matching.
+// split the group; each counts its slice per tile in its own difference grid, the counts become
+// per-wavefront cursors (tile offset + earlier groups + earlier slices) in place, then each
+// emits its slice in order, ranking equal tiles inside a 64-pair chunk by ballot matching.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tiles, int tileBits, int group, int numGroups,
-    const uint32_t *__restrict__ counts, const int *__restrict__ tileOffsets, uint32_t *tileList)
+    const int4 *__restrict__ rect, int tilesX, int tilesY, int cellBits, int group,
+    int numGroups, const uint32_t *__restrict__ counts, const int *__restrict__ tileOffsets,
+    uint32_t *tileList)
 {
-    extern __shared__ uint32_t cur[]; // [W][tiles]
+    extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int W = int(blockDim.x) >> 6;
+    const int stride = tilesX + 1, cells = stride * (tilesY + 1);
     const int slice = group / W;
     const int64_t first = int64_t(blockIdx.x) * group + int64_t(wave) * slice;
-    uint32_t *mine = cur + size_t(wave) * size_t(tiles);
-    for (int t = tid; t < W * tiles; t += int(blockDim.x))
+    uint32_t *mine = cur + size_t(wave) * size_t(cells);
+    for (int t = tid; t < W * cells; t += int(blockDim.x))
         cur[t] = 0;
     __syncthreads();
-    if (W > 1)
-        forEachPair(n, sorted, nTiles, rect, tilesX, first, slice, L[wave],
-                    [&](bool valid, int tile, uint32_t) {
-                        if (valid)
-                            atomicAdd(&mine[tile], 1u);
-                    });
-    __syncthreads();
-    for (int t = tid; t < tiles; t += int(blockDim.x)) {
+    if (W > 1) {
+        addRects(n, sorted, nTiles, rect, first, slice, mine, stride);
+        __syncthreads();
+        prefix2d(cur, W, tilesY + 1, stride);
+    }
+    for (int t = tid; t < tilesX * tilesY; t += int(blockDim.x)) {
+        const int ty = t / tilesX, tx = t - ty * tilesX;
+        const int cell = ty * stride + tx;
         uint32_t base = uint32_t(tileOffsets[t]) + counts[size_t(t) * size_t(numGroups) + blockIdx.x];
         for (int w = 0; w < W; ++w) {
-            const uint32_t c = cur[size_t(w) * size_t(tiles) + t];
-            cur[size_t(w) * size_t(tiles) + t] = base;
+            const uint32_t c = cur[w * cells + cell];
+            cur[w * cells + cell] = base;
             base += c;
         }
     }
     __syncthreads();
-    forEachPair(n, sorted, nTiles, rect, tilesX, first, slice, L[wave],
-                [&](bool valid, int tile, uint32_t g) {
+    forEachPair(n, sorted, nTiles, rect, first, slice, L[wave],
+                [&](bool valid, int tx, int ty, uint32_t g) {
+                    const int cell = ty * stride + tx;
                     const uint64_t vm = __ballot(valid);
-                    const uint64_t m = matchDigit(uint32_t(tile), vm, tileBits);
+                    const uint64_t m = matchDigit(uint32_t(cell), vm, cellBits);
                     if (valid) {
                         const uint32_t below = uint32_t(__popcll(m & laneMaskLt()));
-                        const uint32_t base = mine[tile];
+                        const uint32_t base = mine[cell];
                         tileList[base + below] = g;
                         if (below == 0)
-                            mine[tile] = base + uint32_t(__popcll(m));
+                            mine[cell] = base + uint32_t(__popcll(m));
                     }
                 });
 }
@@ -889,37 +963,39 @@ int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t c
     return U3D_OK;
 }
 
-int tileBinningGroup(int64_t n, int tiles)
+int tileBinningGroup(int64_t n, int tilesX, int tilesY)
 {
-    if (tiles <= 0 || tiles > kMaxBinTiles)
+    const int64_t tiles = int64_t(tilesX) * tilesY;
+    if (tiles <= 0 || int64_t(tilesX + 1) * (tilesY + 1) > kBinLdsCells)
         return 0;
     // about 2^20 (group, tile) counters at most; 64 Gaussians per wavefront slice at least
     constexpr int64_t kUnit = 64 * kBinWaves;
-    const int64_t want = (n * int64_t(tiles) + (int64_t(1) << 20) - 1) >> 20;
+    const int64_t want = (n * tiles + (int64_t(1) << 20) - 1) >> 20;
     int64_t g = want < kUnit ? kUnit : want;
     g = (g + kUnit - 1) / kUnit * kUnit;
     return int(g < (int64_t(1) << 22) ? g : (int64_t(1) << 22));
 }
 
-size_t tileBinningCountsSize(int64_t n, int tiles)
+size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
 {
-    const int g = tileBinningGroup(n, tiles);
-    return g > 0 ? size_t(blocksFor(n > 0 ? n : 1, g)) * size_t(tiles) : 0;
+    const int g = tileBinningGroup(n, tilesX, tilesY);
+    return g > 0 ? size_t(blocksFor(n > 0 ? n : 1, g)) * size_t(tilesX) * size_t(tilesY) : 0;
 }
 
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
-                      const int4 *rect, int tilesX, int tiles, uint32_t *counts, size_t countsCap,
+                      const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
                       uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList)
 {
-    const int group = tileBinningGroup(n, tiles);
-    U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning supports 1.." + std::to_string(kMaxBinTiles) + " tiles");
+    const int group = tileBinningGroup(n, tilesX, tilesY);
+    U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
+    const int tiles = tilesX * tilesY;
     if (n <= 0)
         return launchFillI32(s, tileOffsets, 0, int64_t(tiles) + 1);
     const int numGroups = blocksFor(n, group);
     U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
-    static_assert(kMaxBinTiles <= kBinLdsCounters, "one wavefront's counters must fit");
+    const int cells = (tilesX + 1) * (tilesY + 1);
     hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(64 * kBinWaves),
-                       size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect, tilesX, tiles,
+                       size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, tilesX, tilesY,
                        group, numGroups, counts);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
@@ -928,16 +1004,16 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     hipLaunchKernelGGL(tileOffsetsScanKernel, dim3(1), dim3(kBinScanThreads), 0, s, tileTotal,
                        tiles, tileOffsets);
     U3D_LAUNCH_CHECK();
-    int tileBits = 1;
-    while ((1 << tileBits) < tiles)
-        ++tileBits;
-    // wavefronts per emission workgroup: as many slices as their counters fit in LDS
+    int cellBits = 1;
+    while ((1 << cellBits) < cells)
+        ++cellBits;
+    // wavefronts per emission workgroup: as many slices as their grids fit in LDS
     int waves = kBinWaves;
-    while (waves > 1 && waves * tiles > kBinLdsCounters)
+    while (waves > 1 && waves * cells > kBinLdsCells)
         waves >>= 1;
     hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
-                       size_t(waves) * size_t(tiles) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
-                       tilesX, tiles, tileBits, group, numGroups, counts, tileOffsets, tileList);
+                       size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
+                       tilesX, tilesY, cellBits, group, numGroups, counts, tileOffsets, tileList);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
