@@ -280,6 +280,19 @@ __device__ inline float expfExactLds(float x, const uint64_t *ldsTable)
     return u3dmath::expf_core(x, ldsTable);
 }
 
+// Same, for callers that know (wave-uniformly, noSpecial) that x is NaN or lies in (-87, 0]:
+// the core alone then gives the library result (NaN in, NaN out).
+__device__ inline float expfExactLdsIf(float x, const uint64_t *ldsTable, bool noSpecial)
+{
+    if (!noSpecial) {
+        bool special;
+        const float s = u3dmath::expf_special(x, &special);
+        if (special)
+            return s;
+    }
+    return u3dmath::expf_core(x, ldsTable);
+}
+
 __device__ inline uint32_t floatBits(float f) { return __float_as_uint(f); }
 
 } // namespace u3d

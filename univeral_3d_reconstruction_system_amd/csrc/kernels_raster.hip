@@ -53,12 +53,138 @@ __device__ __forceinline__ float waveSum(float x)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
+// Sums of nine per-lane values over the 64 lanes of the wavefront (every lane active) by
+// recursive halving: at each lane distance a lane hands half of its partial sums to its partner
+// and keeps the other half (v_permlane32_swap / v_permlane16_swap for distances 32 and 16, DPP
+// row rotate by 8, half-row mirror and quad permutes below), 25 instructions for all nine.
+// Lane l with (l & 3) == 0 ends with the total of value k = bit5 + 2 bit4 + 4 bit3 of l when
+// bit 2 of l is clear; lane 4 ends with value 8; *index is k, 8 or -1.
+__device__ __forceinline__ float pairSum32(float x, float y)
+{
+    // lanes 0-31: x[l] + x[l + 32]; lanes 32-63: y[l - 32] + y[l]
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float pairSum16(float x, float y)
+{
+    // rows 0, 2: x summed with the next row; rows 1, 3: y summed with the previous row
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// DPP exchange with a partner whose `upper` flag is the opposite: keeps x where !upper, y where
+// upper, each summed with the partner's copy.
+template <int CTRL> __device__ __forceinline__ float pairSumDpp(float x, float y, bool upper)
+{
+    const float send = upper ? x : y, keep = upper ? y : x;
+    return keep + dppMove<CTRL, 0xf>(send);
+}
+
+__device__ __forceinline__ float waveReduceScatter9(const float v[9], int lane, int *index)
+{
+    const float p0 = pairSum32(v[0], v[1]), p1 = pairSum32(v[2], v[3]);
+    const float p2 = pairSum32(v[4], v[5]), p3 = pairSum32(v[6], v[7]);
+    float e = pairSum32(v[8], v[8]);
+    const float q0 = pairSum16(p0, p1), q1 = pairSum16(p2, p3);
+    e = pairSum16(e, e);
+    const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
+    const float r0 = pairSumDpp<0x128>(q0, q1, b3); // row_ror:8 = lane xor 8
+    e += dppMove<0x128, 0xf>(e);
+    float t = pairSumDpp<0x141>(r0, e, b2);         // row_half_mirror = lane xor 7
+    t += dppMove<0x4e, 0xf>(t);                     // quad_perm [2,3,0,1] = lane xor 2
+    t += dppMove<0xb1, 0xf>(t);                     // quad_perm [1,0,3,2] = lane xor 1
+    *index = (lane & 3) ? -1 : (b2 ? (lane == 4 ? 8 : -1) : ((lane >> 5) & 1) | (((lane >> 4) & 1) << 1) | (int(b3) << 2));
+    return t;
+}
+
+// State of one pixel's reverse walk (Rasterizer.cpp:370-430): transmittance, the running
+// colour behind, the previous contributor's colour and alpha.
+struct BackState {
+    float T, acc0, acc1, acc2, last0, last1, last2, lastAlpha;
+};
+
+// One Gaussian of the reverse walk for every lane of the wavefront (uniform control flow):
+// lanes with `active` and a contribution update their state with the reference's arithmetic
+// (kept by selects) and the Gaussian's nine screen-space gradient terms are summed over the
+// wavefront into accRow (LDS). kFewLanes > 0: when at most that many lanes contribute they add
+// their terms to accRow directly instead.
+template <int kFewLanes>
+__device__ __forceinline__ void backStep(BackState &st, bool active, const float4 &a, const float4 &b,
+                                         float thr, float c2, float fpx, float fpy, float dp0,
+                                         float dp1, float dp2, float fT, float bgDot, float minAlpha,
+                                         bool fastExp, const uint64_t *stab, int lane, float *accRow)
+{
+    const float dx = a.x - fpx, dy = a.y - fpy;
+    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+    const bool inRange = active && !(power > 0.0f) && !(power < thr);
+    if (__ballot(inRange) == 0)
+        return;
+    const float G = expfExactLdsIf(power, stab, fastExp);
+    const float op = b.y;
+    const float rawAlpha = op * G;
+    const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
+    const bool contrib = inRange && !(alpha < minAlpha);
+    const uint64_t who = __ballot(contrib);
+    if (who == 0)
+        return;
+    const float c0 = b.z, c1 = b.w;
+    const float Tn = st.T / (1.0f - alpha);
+    const float w = alpha * Tn;
+    const float acc0 = st.lastAlpha * st.last0 + (1.0f - st.lastAlpha) * st.acc0;
+    const float acc1 = st.lastAlpha * st.last1 + (1.0f - st.lastAlpha) * st.acc1;
+    const float acc2 = st.lastAlpha * st.last2 + (1.0f - st.lastAlpha) * st.acc2;
+    float dAlpha = 0.0f;
+    dAlpha += (c0 - acc0) * dp0;
+    dAlpha += (c1 - acc1) * dp1;
+    dAlpha += (c2 - acc2) * dp2;
+    dAlpha *= Tn;
+    dAlpha += -fT / (1.0f - alpha) * bgDot;
+    const bool grads = contrib && !(rawAlpha > kMaxAlpha);
+    const float dPower = op * dAlpha * G;
+    const float A = a.z, B = a.w, C = b.x;
+    float v[9];
+    v[0] = grads ? dPower * -(A * dx + B * dy) : 0.0f;
+    v[1] = grads ? dPower * -(C * dy + B * dx) : 0.0f;
+    v[2] = grads ? dPower * -0.5f * dx * dx : 0.0f;
+    v[3] = grads ? dPower * -dx * dy : 0.0f;
+    v[4] = grads ? dPower * -0.5f * dy * dy : 0.0f;
+    v[5] = contrib ? w * dp0 : 0.0f;
+    v[6] = contrib ? w * dp1 : 0.0f;
+    v[7] = contrib ? w * dp2 : 0.0f;
+    v[8] = grads ? G * dAlpha : 0.0f;
+    st.T = contrib ? Tn : st.T;
+    st.acc0 = contrib ? acc0 : st.acc0;
+    st.acc1 = contrib ? acc1 : st.acc1;
+    st.acc2 = contrib ? acc2 : st.acc2;
+    st.last0 = contrib ? c0 : st.last0;
+    st.last1 = contrib ? c1 : st.last1;
+    st.last2 = contrib ? c2 : st.last2;
+    st.lastAlpha = contrib ? alpha : st.lastAlpha;
+    if (kFewLanes > 0 && __popcll(who) <= kFewLanes) {
+        if (contrib) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                atomicAdd(&accRow[k], v[k]);
+        }
+        return;
+    }
+    int idx;
+    const float tot = waveReduceScatter9(v, lane, &idx);
+    if (idx >= 0)
+        atomicAdd(&accRow[idx], tot);
+}
+
 // log(min_alpha / alpha) - 1e-3: below it, min(0.99, alpha * exp(power)) < min_alpha holds for
 // the rounded float result as well (logf error and the two roundings are < 1e-5).
 __device__ __forceinline__ float skipThreshold(float minAlpha, float alpha)
 {
     return minAlpha > 0.0f ? logf(minAlpha / alpha) - 1e-3f : -INFINITY;
 }
+
+// With minAlpha >= 1e-37 every threshold is above ln(1e-37) - 1e-3 > -86 (opacity <= 1), so a
+// power that passes the threshold test lies in (-86, 0] or is NaN: no special exp inputs.
+__device__ __forceinline__ bool noSpecialExp(float minAlpha) { return minAlpha >= 1e-37f; }
 
 // Minimum of Q(d) = A dx^2 + 2B dx dy + C dy^2 over the box d in [x0, x1] x [y0, y1] (Q positive
 // definite); also returns a bound on the magnitude of the terms for the rounding margin.
@@ -108,7 +234,7 @@ __device__ __forceinline__ void forwardBlock(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, const SegmentState &seg)
+    int *__restrict__ nContrib, const SegmentState &seg, bool fastExp)
 {
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
@@ -161,51 +287,64 @@ __device__ __forceinline__ void forwardBlock(
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
             if (!bits)
                 continue;
-            // Software-pipelined walk over the set bits: the next Gaussian's LDS records are
-            // loaded while the current one is composited (same Gaussians, same order).
-            int j = c * 64 + __builtin_ctzll(bits);
-            bits &= bits - 1;
-            float4 a = s0[j], b = s1[j];
-            float thr = sthr[j], bl = s2[j];
-            for (;;) {
-                const bool more = bits != 0;
-                const int jn = more ? c * 64 + __builtin_ctzll(bits) : j;
-                bits &= bits - 1;
-                const float4 an = s0[jn], bn = s1[jn];
-                const float thrn = sthr[jn], bln = s2[jn];
-                if (!done) {
-                    const float dx = a.x - fpx, dy = a.y - fpy;
-                    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                    if (!(power > 0.0f) && !(power < thr)) {
-                        const float raw = b.y * expfExactLds(power, stab);
-                        const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
-                        if (!(alpha < minAlpha)) {
-                            const float nextT = T * (1.0f - alpha);
-                            if (nextT < minT) {
-                                done = true;
-                            } else {
-                                const float w = alpha * T;
-                                cr += b.z * w;
-                                cg += b.w * w;
-                                cb += bl * w;
-                                if (seg.ckpt) {
-                                    sr += b.z * w;
-                                    sg += b.w * w;
-                                    sb += bl * w;
-                                }
-                                T = nextT;
-                                last = base - begin + j + 1;
-                            }
-                        }
-                    }
+            // One Gaussian for every pixel of the wavefront, without divergent branches: the
+            // contribution is computed for all lanes and kept by selects where the reference
+            // composites (same expressions, so the same values). Lanes outside the window,
+            // below the threshold or already done keep their state.
+            const bool ckpt = seg.ckpt != nullptr;
+            auto composite = [&](const float4 &a, const float4 &b, float thr, float bl, int j) {
+                const float dx = a.x - fpx, dy = a.y - fpy;
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                const bool inRange = !done && !(power > 0.0f) && !(power < thr);
+                if (__ballot(inRange) == 0)
+                    return;
+                const float raw = b.y * expfExactLdsIf(power, stab, fastExp);
+                const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
+                const bool contrib = inRange && !(alpha < minAlpha);
+                const float nextT = T * (1.0f - alpha);
+                const bool stop = contrib && nextT < minT;
+                const bool take = contrib && !stop;
+                const float w = alpha * T;
+                const float wr = b.z * w, wg = b.w * w, wb = bl * w;
+                cr = take ? cr + wr : cr;
+                cg = take ? cg + wg : cg;
+                cb = take ? cb + wb : cb;
+                if (ckpt) {
+                    sr = take ? sr + wr : sr;
+                    sg = take ? sg + wg : sg;
+                    sb = take ? sb + wb : sb;
                 }
-                if (!more)
+                T = take ? nextT : T;
+                last = take ? base - begin + j + 1 : last;
+                done = done || stop;
+            };
+            // Software-pipelined walk over the set bits, two register sets in turn: the next
+            // Gaussian's LDS records are loaded while the current one is composited.
+            int jA = c * 64 + __builtin_ctzll(bits);
+            bits &= bits - 1;
+            float4 aA = s0[jA], bA = s1[jA];
+            float thrA = sthr[jA], blA = s2[jA];
+            for (;;) {
+                if (!bits) {
+                    composite(aA, bA, thrA, blA, jA);
                     break;
-                j = jn;
-                a = an;
-                b = bn;
-                thr = thrn;
-                bl = bln;
+                }
+                const int jB = c * 64 + __builtin_ctzll(bits);
+                bits &= bits - 1;
+                const float4 aB = s0[jB], bB = s1[jB];
+                const float thrB = sthr[jB], blB = s2[jB];
+                composite(aA, bA, thrA, blA, jA);
+                if (!bits) {
+                    composite(aB, bB, thrB, blB, jB);
+                    break;
+                }
+                jA = c * 64 + __builtin_ctzll(bits);
+                bits &= bits - 1;
+                aA = s0[jA];
+                bA = s1[jA];
+                thrA = sthr[jA];
+                blA = s2[jA];
+                composite(aB, bB, thrB, blB, jB);
             }
             if (__ballot(!done) == 0)
                 break;
@@ -263,6 +402,7 @@ __device__ __forceinline__ void backwardSegment(
     const float *__restrict__ finalTArr, const int *__restrict__ nContribArr,
     const float4 *__restrict__ ckpt, float *__restrict__ grad2d)
 {
+    const bool fastExp = noSpecialExp(minAlpha);
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
     __shared__ float s2[kBatch];
@@ -292,18 +432,16 @@ __device__ __forceinline__ void backwardSegment(
     const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
     const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
     const float fpx = float(px), fpy = float(py);
-    float T = fT;
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+    BackState bs{fT, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (nc > hi) {
         const float4 ck = ckpt[(size_t(begin / kBatch + tile) + size_t(segIdx + 1)) * kThreads + tid];
-        T = ck.x;
-        if (T > 0.0f) {
-            acc0 = ck.y / T;
-            acc1 = ck.z / T;
-            acc2 = ck.w / T;
+        bs.T = ck.x;
+        if (bs.T > 0.0f) {
+            bs.acc0 = ck.y / bs.T;
+            bs.acc1 = ck.z / bs.T;
+            bs.acc2 = ck.w / bs.T;
         }
     }
-    float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
     {
         int m = jEnd;
 #pragma unroll
@@ -340,96 +478,37 @@ __device__ __forceinline__ void backwardSegment(
         uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
         if (!bits)
             continue;
-        // Software-pipelined reverse walk: the next Gaussian's LDS records are loaded
-        // while the current one is processed (same Gaussians, same order).
-        int jj = c * 64 + 63 - __builtin_clzll(bits);
-        bits &= ~(uint64_t(1) << (jj - c * 64));
-        float4 ra = s0[jj], rb = s1[jj];
-        float rthr = sthr[jj], rc2 = s2[jj];
+        // Reverse walk over the set bits, two register sets in turn: the next Gaussian's LDS
+        // records are loaded while the current one is processed (same Gaussians, same order).
+        auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
+            backStep<0>(bs, lo + jj < jEnd, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2, fT, bgDot,
+                        minAlpha, fastExp, stab, lane, acc[jj]);
+        };
+        int jA = c * 64 + 63 - __builtin_clzll(bits);
+        bits &= ~(uint64_t(1) << (jA - c * 64));
+        float4 aA = s0[jA], bA = s1[jA];
+        float thrA = sthr[jA], cA = s2[jA];
         for (;;) {
-            const bool more = bits != 0;
-            const int jn = more ? c * 64 + 63 - __builtin_clzll(bits) : jj;
-            if (more)
-                bits &= ~(uint64_t(1) << (jn - c * 64));
-            const float4 ran = s0[jn], rbn = s1[jn];
-            const float rthrn = sthr[jn], rc2n = s2[jn];
-            const int j = lo + jj;
-            float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
-            bool contrib = false;
-            if (j < jEnd) {
-                const float4 a = ra;
-                const float4 b = rb;
-                const float dx = a.x - fpx, dy = a.y - fpy;
-                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                if (!(power > 0.0f) && !(power < rthr)) {
-                    const float G = expfExactLds(power, stab);
-                    const float op = b.y;
-                    const float rawAlpha = op * G;
-                    const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
-                    if (!(alpha < minAlpha)) {
-                        contrib = true;
-                        const float c0 = b.z, c1 = b.w, c2 = rc2;
-                        T = T / (1.0f - alpha);
-                        const float w = alpha * T;
-                        float dAlpha = 0.0f;
-                        acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
-                        last0 = c0;
-                        dAlpha += (c0 - acc0) * dp0;
-                        v5 = w * dp0;
-                        acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
-                        last1 = c1;
-                        dAlpha += (c1 - acc1) * dp1;
-                        v6 = w * dp1;
-                        acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
-                        last2 = c2;
-                        dAlpha += (c2 - acc2) * dp2;
-                        v7 = w * dp2;
-                        dAlpha *= T;
-                        lastAlpha = alpha;
-                        dAlpha += -fT / (1.0f - alpha) * bgDot;
-                        if (!(rawAlpha > kMaxAlpha)) {
-                            v8 = G * dAlpha;
-                            const float dPower = op * dAlpha * G;
-                            const float A = a.z, B = a.w, C = b.x;
-                            v0 = dPower * -(A * dx + B * dy);
-                            v1 = dPower * -(C * dy + B * dx);
-                            v2 = dPower * -0.5f * dx * dx;
-                            v3 = dPower * -dx * dy;
-                            v4 = dPower * -0.5f * dy * dy;
-                        }
-                    }
-                }
-            }
-            if (__ballot(contrib)) {
-                v0 = waveSum(v0);
-                v1 = waveSum(v1);
-                v2 = waveSum(v2);
-                v3 = waveSum(v3);
-                v4 = waveSum(v4);
-                v5 = waveSum(v5);
-                v6 = waveSum(v6);
-                v7 = waveSum(v7);
-                v8 = waveSum(v8);
-                if (lane == 0) {
-                    float *dst = acc[jj];
-                    atomicAdd(&dst[0], v0);
-                    atomicAdd(&dst[1], v1);
-                    atomicAdd(&dst[2], v2);
-                    atomicAdd(&dst[3], v3);
-                    atomicAdd(&dst[4], v4);
-                    atomicAdd(&dst[5], v5);
-                    atomicAdd(&dst[6], v6);
-                    atomicAdd(&dst[7], v7);
-                    atomicAdd(&dst[8], v8);
-                }
-            }
-            if (!more)
+            if (!bits) {
+                step(aA, bA, thrA, cA, jA);
                 break;
-            jj = jn;
-            ra = ran;
-            rb = rbn;
-            rthr = rthrn;
-            rc2 = rc2n;
+            }
+            const int jB = c * 64 + 63 - __builtin_clzll(bits);
+            bits &= ~(uint64_t(1) << (jB - c * 64));
+            const float4 aB = s0[jB], bB = s1[jB];
+            const float thrB = sthr[jB], cB = s2[jB];
+            step(aA, bA, thrA, cA, jA);
+            if (!bits) {
+                step(aB, bB, thrB, cB, jB);
+                break;
+            }
+            jA = c * 64 + 63 - __builtin_clzll(bits);
+            bits &= ~(uint64_t(1) << (jA - c * 64));
+            aA = s0[jA];
+            bA = s1[jA];
+            thrA = sthr[jA];
+            cA = s2[jA];
+            step(aB, bB, thrB, cB, jB);
         }
     }
     __syncthreads();
@@ -482,9 +561,8 @@ __device__ __forceinline__ void backwardBlock(
     const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
     const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
     const float fpx = float(px), fpy = float(py);
-    float T = fT;
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
-    float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
+    const bool fastExp = noSpecialExp(minAlpha);
+    BackState bs{fT, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     {
         int m = nc;
 #pragma unroll
@@ -521,110 +599,36 @@ __device__ __forceinline__ void backwardBlock(
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
             if (!bits)
                 continue;
-            // Software-pipelined reverse walk: the next Gaussian's LDS records are loaded
-            // while the current one is processed (same Gaussians, same order).
-            int jj = c * 64 + 63 - __builtin_clzll(bits);
-            bits &= ~(uint64_t(1) << (jj - c * 64));
-            float4 ra = s0[jj], rb = s1[jj];
-            float rthr = sthr[jj], rc2 = s2[jj];
+            // Reverse walk over the set bits, two register sets in turn (as backwardSegment).
+            auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
+                backStep<kFewLanes>(bs, lo + jj < nc, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2, fT,
+                                    bgDot, minAlpha, fastExp, stab, lane, acc[jj]);
+            };
+            int jA = c * 64 + 63 - __builtin_clzll(bits);
+            bits &= ~(uint64_t(1) << (jA - c * 64));
+            float4 aA = s0[jA], bA = s1[jA];
+            float thrA = sthr[jA], cA = s2[jA];
             for (;;) {
-                const bool more = bits != 0;
-                const int jn = more ? c * 64 + 63 - __builtin_clzll(bits) : jj;
-                if (more)
-                    bits &= ~(uint64_t(1) << (jn - c * 64));
-                const float4 ran = s0[jn], rbn = s1[jn];
-                const float rthrn = sthr[jn], rc2n = s2[jn];
-                const int j = lo + jj;
-                float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
-                bool contrib = false;
-                if (j < nc) {
-                    const float4 a = ra;
-                    const float4 b = rb;
-                    const float dx = a.x - fpx, dy = a.y - fpy;
-                    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                    if (!(power > 0.0f) && !(power < rthr)) {
-                        const float G = expfExactLds(power, stab);
-                        const float op = b.y;
-                        const float rawAlpha = op * G;
-                        const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
-                        if (!(alpha < minAlpha)) {
-                            contrib = true;
-                            const float c0 = b.z, c1 = b.w, c2 = rc2;
-                            T = T / (1.0f - alpha);
-                            const float w = alpha * T;
-                            float dAlpha = 0.0f;
-                            acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
-                            last0 = c0;
-                            dAlpha += (c0 - acc0) * dp0;
-                            v5 = w * dp0;
-                            acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
-                            last1 = c1;
-                            dAlpha += (c1 - acc1) * dp1;
-                            v6 = w * dp1;
-                            acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
-                            last2 = c2;
-                            dAlpha += (c2 - acc2) * dp2;
-                            v7 = w * dp2;
-                            dAlpha *= T;
-                            lastAlpha = alpha;
-                            dAlpha += -fT / (1.0f - alpha) * bgDot;
-                            if (!(rawAlpha > kMaxAlpha)) {
-                                v8 = G * dAlpha;
-                                const float dPower = op * dAlpha * G;
-                                const float A = a.z, B = a.w, C = b.x;
-                                v0 = dPower * -(A * dx + B * dy);
-                                v1 = dPower * -(C * dy + B * dx);
-                                v2 = dPower * -0.5f * dx * dx;
-                                v3 = dPower * -dx * dy;
-                                v4 = dPower * -0.5f * dy * dy;
-                            }
-                        }
-                    }
-                }
-                const uint64_t who = __ballot(contrib);
-                if (kFewLanes > 0 && who && __popcll(who) <= kFewLanes) {
-                    if (contrib) {
-                        float *dst = acc[jj];
-                        atomicAdd(&dst[0], v0);
-                        atomicAdd(&dst[1], v1);
-                        atomicAdd(&dst[2], v2);
-                        atomicAdd(&dst[3], v3);
-                        atomicAdd(&dst[4], v4);
-                        atomicAdd(&dst[5], v5);
-                        atomicAdd(&dst[6], v6);
-                        atomicAdd(&dst[7], v7);
-                        atomicAdd(&dst[8], v8);
-                    }
-                } else if (who) {
-                    v0 = waveSum(v0);
-                    v1 = waveSum(v1);
-                    v2 = waveSum(v2);
-                    v3 = waveSum(v3);
-                    v4 = waveSum(v4);
-                    v5 = waveSum(v5);
-                    v6 = waveSum(v6);
-                    v7 = waveSum(v7);
-                    v8 = waveSum(v8);
-                    if (lane == 0) {
-                        float *dst = acc[jj];
-                        atomicAdd(&dst[0], v0);
-                        atomicAdd(&dst[1], v1);
-                        atomicAdd(&dst[2], v2);
-                        atomicAdd(&dst[3], v3);
-                        atomicAdd(&dst[4], v4);
-                        atomicAdd(&dst[5], v5);
-                        atomicAdd(&dst[6], v6);
-                        atomicAdd(&dst[7], v7);
-                        atomicAdd(&dst[8], v8);
-                    }
-                }
-                if (!more)
+                if (!bits) {
+                    step(aA, bA, thrA, cA, jA);
                     break;
-                jj = jn;
-                ra = ran;
-                rb = rbn;
-                rthr = rthrn;
-                rc2 = rc2n;
+                }
+                const int jB = c * 64 + 63 - __builtin_clzll(bits);
+                bits &= ~(uint64_t(1) << (jB - c * 64));
+                const float4 aB = s0[jB], bB = s1[jB];
+                const float thrB = sthr[jB], cB = s2[jB];
+                step(aA, bA, thrA, cA, jA);
+                if (!bits) {
+                    step(aB, bB, thrB, cB, jB);
+                    break;
+                }
+                jA = c * 64 + 63 - __builtin_clzll(bits);
+                bits &= ~(uint64_t(1) << (jA - c * 64));
+                aA = s0[jA];
+                bA = s1[jA];
+                thrA = sthr[jA];
+                cA = s2[jA];
+                step(aB, bB, thrB, cB, jB);
             }
         }
         __syncthreads();
@@ -667,7 +671,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
 {
     forwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
                       tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT,
-                      nContrib, seg);
+                      nContrib, seg, noSpecialExp(minAlpha));
 }
 
 // First index in sorted[0..n) greater than w, minus one: the tile owning work item w.
@@ -786,7 +790,7 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
                          tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
                          st.background[1], st.background[2], st.min_alpha,
                          st.min_transmittance, it.rgb, it.finalT, it.nContrib,
-                         seg);
+                         seg, noSpecialExp(st.min_alpha));
         else
             backwardBlock<kBatch, kFewLanes>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
                           tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
