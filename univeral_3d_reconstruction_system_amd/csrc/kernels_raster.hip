@@ -109,18 +109,18 @@ struct BackState {
 // (kept by selects) and the Gaussian's nine screen-space gradient terms are summed over the
 // wavefront into accRow (LDS). kFewLanes > 0: when at most that many lanes contribute they add
 // their terms to accRow directly instead.
-template <int kFewLanes>
+template <int kFewLanes, bool kFastExp>
 __device__ __forceinline__ void backStep(BackState &st, bool active, const float4 &a, const float4 &b,
                                          float thr, float c2, float fpx, float fpy, float dp0,
                                          float dp1, float dp2, float fT, float bgDot, float minAlpha,
-                                         bool fastExp, const uint64_t *stab, int lane, float *accRow)
+                                         const uint64_t *stab, int lane, float *accRow)
 {
     const float dx = a.x - fpx, dy = a.y - fpy;
     const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
     const bool inRange = active && !(power > 0.0f) && !(power < thr);
     if (__ballot(inRange) == 0)
         return;
-    const float G = expfExactLdsIf(power, stab, fastExp);
+    const float G = expfExactLdsIf(power, stab, kFastExp);
     const float op = b.y;
     const float rawAlpha = op * G;
     const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
@@ -227,14 +227,15 @@ __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, i
 }
 
 // One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase,
-// staging kBatch (<= 256) records of the tile list in LDS at a time.
-template <int kBatch>
+// staging kBatch (<= 256) records of the tile list in LDS at a time. kCkpt: seg.ckpt is set
+// (segment checkpoints for the segmented backward are written).
+template <int kBatch, bool kCkpt, bool kFastExp>
 __device__ __forceinline__ void forwardBlock(
     int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, const SegmentState &seg, bool fastExp)
+    int *__restrict__ nContrib, const SegmentState &seg)
 {
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
@@ -256,30 +257,44 @@ __device__ __forceinline__ void forwardBlock(
     int last = 0, batches = 0;
     bool done = !inside;
     const size_t slot0 = size_t(begin / kBatch + tile);
+    // The next batch's records (and the ids of the one after) are loaded while the current
+    // batch is walked.
+    auto loadId = [&](int b0) { return (tid < kBatch && b0 + tid < end) ? tileList[b0 + tid] : -1; };
+    int gCur = loadId(begin), gAhead = loadId(begin + kBatch);
+    float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+    float rc = 0.0f;
+    if (gCur >= 0) {
+        ra = rec0[gCur];
+        rb = rec1[gCur];
+        rc = rec2[gCur];
+    }
     for (int base = begin; base < end; base += kBatch) {
         if (__syncthreads_and(done))
             break;
-        if (seg.ckpt && base > begin) { // T before this batch and the previous segment's colour
+        if (kCkpt && base > begin) { // T before this batch and the previous segment's colour
             if (inside)
                 seg.ckpt[(slot0 + size_t(batches)) * kThreads + tid] = make_float4(T, sr, sg, sb);
             sr = sg = sb = 0.0f;
         }
         ++batches;
-        const int idx = base + tid;
         uint32_t m = 0;
-        if (tid < kBatch && idx < end) {
-            const int g = tileList[idx];
-            const float4 a = rec0[g];
-            const float4 b = rec1[g];
-            const float thr = skipThreshold(minAlpha, b.y);
-            s0[tid] = a;
-            s1[tid] = b;
-            s2[tid] = rec2[g];
+        if (gCur >= 0) {
+            const float thr = skipThreshold(minAlpha, rb.y);
+            s0[tid] = ra;
+            s1[tid] = rb;
+            s2[tid] = rc;
             sthr[tid] = thr;
-            m = quadrantMask(a, b.x, thr, bx0, by0);
+            m = quadrantMask(ra, rb.x, thr, bx0, by0);
         }
         if (tid < kBatch)
             smask[tid] = m;
+        gCur = gAhead;
+        if (gCur >= 0) {
+            ra = rec0[gCur];
+            rb = rec1[gCur];
+            rc = rec2[gCur];
+        }
+        gAhead = loadId(base + 2 * kBatch);
         __syncthreads();
         if (__ballot(!done) == 0)
             continue;
@@ -289,27 +304,34 @@ __device__ __forceinline__ void forwardBlock(
                 continue;
             // One Gaussian for every pixel of the wavefront, without divergent branches: the
             // contribution is computed for all lanes and kept by selects where the reference
-            // composites (same expressions, so the same values). Lanes outside the window,
-            // below the threshold or already done keep their state.
-            const bool ckpt = seg.ckpt != nullptr;
-            auto composite = [&](const float4 &a, const float4 &b, float thr, float bl, int j) {
+            // composites (same expressions, so the same values); lanes outside the window,
+            // below the thresholds or already done keep their state. The pixel-local part
+            // (power, exp, alpha and the reference's skip tests) does not depend on the walk's
+            // state, so several Gaussians' parts are computed together and their long
+            // dependency chains overlap; the sequential transmittance / colour update follows.
+            struct Pre {
+                float alpha, c0, c1, c2;
+                bool cand;
+            };
+            auto prepare = [&](const float4 &a, const float4 &b, float thr, float bl) -> Pre {
                 const float dx = a.x - fpx, dy = a.y - fpy;
                 const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                const bool inRange = !done && !(power > 0.0f) && !(power < thr);
-                if (__ballot(inRange) == 0)
-                    return;
-                const float raw = b.y * expfExactLdsIf(power, stab, fastExp);
+                const float raw = b.y * expfExactLdsIf(power, stab, kFastExp);
                 const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
-                const bool contrib = inRange && !(alpha < minAlpha);
-                const float nextT = T * (1.0f - alpha);
+                const bool cand = !(power > 0.0f) && !(power < thr) && !(alpha < minAlpha);
+                return Pre{alpha, b.z, b.w, bl, cand};
+            };
+            auto composite = [&](const Pre &p, int j) {
+                const bool contrib = p.cand && !done;
+                const float nextT = T * (1.0f - p.alpha);
                 const bool stop = contrib && nextT < minT;
                 const bool take = contrib && !stop;
-                const float w = alpha * T;
-                const float wr = b.z * w, wg = b.w * w, wb = bl * w;
+                const float w = p.alpha * T;
+                const float wr = p.c0 * w, wg = p.c1 * w, wb = p.c2 * w;
                 cr = take ? cr + wr : cr;
                 cg = take ? cg + wg : cg;
                 cb = take ? cb + wb : cb;
-                if (ckpt) {
+                if (kCkpt) {
                     sr = take ? sr + wr : sr;
                     sg = take ? sg + wg : sg;
                     sb = take ? sb + wb : sb;
@@ -318,33 +340,24 @@ __device__ __forceinline__ void forwardBlock(
                 last = take ? base - begin + j + 1 : last;
                 done = done || stop;
             };
-            // Software-pipelined walk over the set bits, two register sets in turn: the next
-            // Gaussian's LDS records are loaded while the current one is composited.
-            int jA = c * 64 + __builtin_ctzll(bits);
-            bits &= bits - 1;
-            float4 aA = s0[jA], bA = s1[jA];
-            float thrA = sthr[jA], blA = s2[jA];
-            for (;;) {
-                if (!bits) {
-                    composite(aA, bA, thrA, blA, jA);
-                    break;
+            // Groups of kGroup Gaussians: their pixel-local parts are independent dependency
+            // chains computed side by side, then composited in list order.
+            constexpr int kGroup = 4;
+            while (bits) {
+                int jj[kGroup];
+                Pre pp[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    const bool has = bits != 0; // wave-uniform
+                    jj[k] = has ? c * 64 + __builtin_ctzll(bits) : -1;
+                    bits &= bits - 1;
+                    const int jl = has ? jj[k] : c * 64;
+                    pp[k] = prepare(s0[jl], s1[jl], sthr[jl], s2[jl]);
                 }
-                const int jB = c * 64 + __builtin_ctzll(bits);
-                bits &= bits - 1;
-                const float4 aB = s0[jB], bB = s1[jB];
-                const float thrB = sthr[jB], blB = s2[jB];
-                composite(aA, bA, thrA, blA, jA);
-                if (!bits) {
-                    composite(aB, bB, thrB, blB, jB);
-                    break;
-                }
-                jA = c * 64 + __builtin_ctzll(bits);
-                bits &= bits - 1;
-                aA = s0[jA];
-                bA = s1[jA];
-                thrA = sthr[jA];
-                blA = s2[jA];
-                composite(aB, bB, thrB, blB, jB);
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (jj[k] >= 0)
+                        composite(pp[k], jj[k]);
             }
             if (__ballot(!done) == 0)
                 break;
@@ -357,7 +370,7 @@ __device__ __forceinline__ void forwardBlock(
         rgb[pix * 3 + 2] = cb + T * bg2;
         finalT[pix] = T;
         nContrib[pix] = last;
-        if (seg.ckpt) {
+        if (kCkpt) {
             // Reverse pass over this pixel's checkpoints: replace each segment colour by the
             // colour of everything from that boundary on (suffix sums of positive terms, so
             // the backward needs no cancelling difference).
@@ -372,7 +385,7 @@ __device__ __forceinline__ void forwardBlock(
             }
         }
     }
-    if (seg.ckpt) {
+    if (kCkpt) {
         // segments of this tile the backward visits: ceil(max contributor count / kBatch)
         __shared__ int sMax[kThreads / 64];
         int m = last;
@@ -393,7 +406,7 @@ __device__ __forceinline__ void forwardBlock(
 // transmittance T there and the colour composited from there on, which divided by T is the
 // running "colour behind" of the reference's reverse recursion. Segments of a tile are
 // independent work items.
-template <int kBatch>
+template <int kBatch, bool kFastExp>
 __device__ __forceinline__ void backwardSegment(
     int bx, int by, int tileBase, int segIdx, int W, int H, int tilesX,
     const int *__restrict__ tileOffsets, const int *__restrict__ tileList,
@@ -402,7 +415,6 @@ __device__ __forceinline__ void backwardSegment(
     const float *__restrict__ finalTArr, const int *__restrict__ nContribArr,
     const float4 *__restrict__ ckpt, float *__restrict__ grad2d)
 {
-    const bool fastExp = noSpecialExp(minAlpha);
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
     __shared__ float s2[kBatch];
@@ -481,8 +493,8 @@ __device__ __forceinline__ void backwardSegment(
         // Reverse walk over the set bits, two register sets in turn: the next Gaussian's LDS
         // records are loaded while the current one is processed (same Gaussians, same order).
         auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
-            backStep<0>(bs, lo + jj < jEnd, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2, fT, bgDot,
-                        minAlpha, fastExp, stab, lane, acc[jj]);
+            backStep<0, kFastExp>(bs, lo + jj < jEnd, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2,
+                                  fT, bgDot, minAlpha, stab, lane, acc[jj]);
         };
         int jA = c * 64 + 63 - __builtin_clzll(bits);
         bits &= ~(uint64_t(1) << (jA - c * 64));
@@ -526,7 +538,7 @@ __device__ __forceinline__ void backwardSegment(
 // kFewLanes: when at most this many lanes of a wave contribute to a Gaussian, they add their
 // values to its LDS accumulator directly instead of paying nine 64-lane reductions (0: always
 // reduce).
-template <int kBatch, int kFewLanes>
+template <int kBatch, int kFewLanes, bool kFastExp>
 __device__ __forceinline__ void backwardBlock(
     int bx, int by, int tileBase, int W, int H, int ts, int tilesX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -561,7 +573,6 @@ __device__ __forceinline__ void backwardBlock(
     const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
     const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
     const float fpx = float(px), fpy = float(py);
-    const bool fastExp = noSpecialExp(minAlpha);
     BackState bs{fT, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     {
         int m = nc;
@@ -601,8 +612,8 @@ __device__ __forceinline__ void backwardBlock(
                 continue;
             // Reverse walk over the set bits, two register sets in turn (as backwardSegment).
             auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
-                backStep<kFewLanes>(bs, lo + jj < nc, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2, fT,
-                                    bgDot, minAlpha, fastExp, stab, lane, acc[jj]);
+                backStep<kFewLanes, kFastExp>(bs, lo + jj < nc, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1,
+                                              dp2, fT, bgDot, minAlpha, stab, lane, acc[jj]);
             };
             int jA = c * 64 + 63 - __builtin_clzll(bits);
             bits &= ~(uint64_t(1) << (jA - c * 64));
@@ -662,6 +673,7 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
     return lo;
 }
 
+template <bool kCkpt, bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -669,9 +681,9 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib, SegmentState seg)
 {
-    forwardBlock<256>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
-                      tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, minT, rgb, finalT,
-                      nContrib, seg, noSpecialExp(minAlpha));
+    forwardBlock<256, kCkpt, kFastExp>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts,
+                                       tilesX, tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1,
+                                       bg2, minAlpha, minT, rgb, finalT, nContrib, seg);
 }
 
 // First index in sorted[0..n) greater than w, minus one: the tile owning work item w.
@@ -691,6 +703,7 @@ __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart,
 // Segmented backward of one image (16x16 tiles): workgroups take (tile, segment) items from
 // an atomic counter until *totalItems; segStart is the exclusive scan of the per-tile segment
 // counts written by the forward.
+template <bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
     int W, int H, int tilesX, int tiles, const uint32_t *__restrict__ segStart,
     const uint32_t *__restrict__ totalItems, unsigned *counter, const int *__restrict__ tileOffsets,
@@ -711,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
         if (w >= total)
             break;
         const int tile = tileOfItem(segStart, tiles, w);
-        backwardSegment<256>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H, tilesX,
+        backwardSegment<256, kFastExp>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H, tilesX,
                              tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl,
                              finalTArr, nContribArr, ckpt, grad2d);
     }
@@ -741,14 +754,22 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedBatchedKernel(
         const int v = itemOfBlock(items, numItems, tile);
         const RenderItem &it = items[v];
         const int local = int(tile - it.blockBase);
-        backwardSegment<kBatch>(local % it.blocksX, local / it.blocksX, it.tileBase,
-                                int(w - segStart[tile]), it.cam.width, it.cam.height, it.tilesX,
-                                tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
-                                st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
-                                it.nContrib, ckpt, grad2d);
+        if (noSpecialExp(st.min_alpha))
+            backwardSegment<kBatch, true>(local % it.blocksX, local / it.blocksX, it.tileBase,
+                                          int(w - segStart[tile]), it.cam.width, it.cam.height,
+                                          it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,
+                                          st.background[0], st.background[1], st.background[2],
+                                          st.min_alpha, it.dl, it.finalT, it.nContrib, ckpt, grad2d);
+        else
+            backwardSegment<kBatch, false>(local % it.blocksX, local / it.blocksX, it.tileBase,
+                                           int(w - segStart[tile]), it.cam.width, it.cam.height,
+                                           it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,
+                                           st.background[0], st.background[1], st.background[2],
+                                           st.min_alpha, it.dl, it.finalT, it.nContrib, ckpt, grad2d);
     }
 }
 
+template <bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -756,7 +777,7 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, float *__restrict__ grad2d)
 {
-    backwardBlock<256, kPlainFewLanes>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
+    backwardBlock<256, kPlainFewLanes, kFastExp>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts, tilesX, tileOffsets,
                   tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
                   grad2d);
 }
@@ -785,17 +806,37 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
         const RenderItem &it = items[v];
         const int local = int(b - it.blockBase);
         const int bx = local % it.blocksX, by = local / it.blocksX;
-        if (!kBackward)
-            forwardBlock<kBatch>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
-                         tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
-                         st.background[1], st.background[2], st.min_alpha,
-                         st.min_transmittance, it.rgb, it.finalT, it.nContrib,
-                         seg, noSpecialExp(st.min_alpha));
-        else
-            backwardBlock<kBatch, kFewLanes>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size, it.tilesX,
-                          tileOffsets, tileList, rec0, rec1, rec2, st.background[0],
-                          st.background[1], st.background[2], st.min_alpha, it.dl, it.finalT,
-                          it.nContrib, grad2d);
+#define U3D_FWD(CK, FE)                                                                          \
+    forwardBlock<kBatch, CK, FE>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size,   \
+                                 it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,               \
+                                 st.background[0], st.background[1], st.background[2],             \
+                                 st.min_alpha, st.min_transmittance, it.rgb, it.finalT,            \
+                                 it.nContrib, seg)
+#define U3D_BWD(FE)                                                                              \
+    backwardBlock<kBatch, kFewLanes, FE>(bx, by, it.tileBase, it.cam.width, it.cam.height,         \
+                                         st.tile_size, it.tilesX, tileOffsets, tileList, rec0,     \
+                                         rec1, rec2, st.background[0], st.background[1],           \
+                                         st.background[2], st.min_alpha, it.dl, it.finalT,         \
+                                         it.nContrib, grad2d)
+        const bool fast = noSpecialExp(st.min_alpha);
+        if (kBackward) {
+            if (fast)
+                U3D_BWD(true);
+            else
+                U3D_BWD(false);
+        } else if (seg.ckpt) {
+            if (fast)
+                U3D_FWD(true, true);
+            else
+                U3D_FWD(true, false);
+        } else {
+            if (fast)
+                U3D_FWD(false, true);
+            else
+                U3D_FWD(false, false);
+        }
+#undef U3D_BWD
+#undef U3D_FWD
     }
 }
 
@@ -874,11 +915,13 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
     if (blocksX * blocksY == 0)
         return U3D_OK;
-    hipLaunchKernelGGL(rasterForwardKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
-                       g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
-                       sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
-                       st.background[2], st.min_alpha, st.min_transmittance, rgb, finalT,
-                       nContrib, seg);
+    const bool fast = st.min_alpha >= 1e-37f; // as noSpecialExp
+    auto kernel = seg.ckpt ? (fast ? rasterForwardKernel<true, true> : rasterForwardKernel<true, false>)
+                           : (fast ? rasterForwardKernel<false, true> : rasterForwardKernel<false, false>);
+    hipLaunchKernelGGL(kernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s, g.width, g.height,
+                       g.tileSize, g.tilesX, blocksX, tileOffsets, tileList, sb.rec0, sb.rec1,
+                       sb.rec2, st.background[0], st.background[1], st.background[2],
+                       st.min_alpha, st.min_transmittance, rgb, finalT, nContrib, seg);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
@@ -891,7 +934,8 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
     if (blocksX * blocksY == 0)
         return U3D_OK;
-    hipLaunchKernelGGL(rasterBackwardKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
+    auto kernel = st.min_alpha >= 1e-37f ? rasterBackwardKernel<true> : rasterBackwardKernel<false>;
+    hipLaunchKernelGGL(kernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
                        g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
                        sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
                        st.background[2], st.min_alpha, dlDrgb, finalT, nContrib, grad2d);
@@ -909,7 +953,8 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
     if (tiles == 0)
         return U3D_OK;
     U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
-    hipLaunchKernelGGL(rasterSegmentedKernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
+    auto kernel = st.min_alpha >= 1e-37f ? rasterSegmentedKernel<true> : rasterSegmentedKernel<false>;
+    hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
                        g.height, g.tilesX, tiles, segStart, totalItems, counter, tileOffsets, tileList,
                        sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
                        st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d);
@@ -947,7 +992,7 @@ int rasterSegmentedOccupancy()
 {
     int blocks = 0;
     const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterSegmentedKernel, kThreads, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterSegmentedKernel<true>, kThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
