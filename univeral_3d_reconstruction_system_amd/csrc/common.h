@@ -295,4 +295,11 @@ __device__ inline float expfExactLdsIf(float x, const uint64_t *ldsTable, bool n
 
 __device__ inline uint32_t floatBits(float f) { return __float_as_uint(f); }
 
+// Orders LDS accesses of one wavefront: data one lane wrote is read by another lane after it.
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 } // namespace u3d

@@ -203,6 +203,23 @@ __device__ __forceinline__ float boxMinQuad(float A, float B, float C, float x0,
     return fminf(fminf(q(x0, ya), q(x1, yb)), fminf(q(xa, y0), q(xb, y1)));
 }
 
+// Whether one staged Gaussian may contribute to a pixel of the 8x8 quadrant with top-left
+// pixel (qx0, qy0): false only when the power bound over the quadrant is certainly below the
+// threshold.
+__device__ __forceinline__ bool quadrantHit(float4 a, float C, float thr, int qx0, int qy0)
+{
+    // Only positive-definite conics have the box minimum used below; anything else is kept.
+    if (!(thr > -INFINITY) || !(a.z > 0.0f && C > 0.0f && a.z * C - a.w * a.w > 0.0f))
+        return true;
+    const float px0 = float(qx0), py0 = float(qy0);
+    // d = mean - pixel over pixel centres [p0, p0 + 7]
+    float scale;
+    const float qmin = boxMinQuad(a.z, a.w, C, a.x - (px0 + 7.0f), a.x - px0, a.y - (py0 + 7.0f),
+                                  a.y - py0, &scale);
+    const float powerMax = -0.5f * qmin + 1e-6f * scale + 1e-6f;
+    return !(powerMax < thr); // keep unless certainly below the threshold (NaN keeps)
+}
+
 // Quadrant mask of one staged Gaussian for the block at (bx0, by0): bit w set when the
 // Gaussian may contribute to a pixel of quadrant w (8x8, w = qy * 2 + qx).
 __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, int bx0, int by0)
@@ -226,8 +243,10 @@ __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, i
     return mask;
 }
 
-// One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase,
-// staging kBatch (<= 256) records of the tile list in LDS at a time. kCkpt: seg.ckpt is set
+// One 16x16 pixel block (bx, by) of an image whose tiles start at global tile id tileBase.
+// Each wavefront walks the list for its own 8x8 quadrant, staging 64 entries at a time in its
+// own LDS slice: no workgroup barrier couples the four walks, and a wavefront whose pixels are
+// all done stops. kBatch: the checkpoint segment length (entries); kCkpt: seg.ckpt is set
 // (segment checkpoints for the segmented backward are written).
 template <int kBatch, bool kCkpt, bool kFastExp>
 __device__ __forceinline__ void forwardBlock(
@@ -237,30 +256,35 @@ __device__ __forceinline__ void forwardBlock(
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib, const SegmentState &seg)
 {
-    __shared__ float4 s0[kBatch];
-    __shared__ float4 s1[kBatch];
-    __shared__ float s2[kBatch];
-    __shared__ float sthr[kBatch];
-    __shared__ uint32_t smask[kBatch];
+    constexpr int kStep = 64;
+    static_assert(kBatch % kStep == 0, "segments are whole steps");
+    __shared__ float4 s0[kThreads / 64][kStep];
+    __shared__ float4 s1[kThreads / 64][kStep];
+    __shared__ float s2[kThreads / 64][kStep];
+    __shared__ float sthr[kThreads / 64][kStep];
     __shared__ uint64_t stab[32];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 32)
         stab[tid] = kExp2Table[tid];
+    __syncthreads();
     const int bx0 = bx * kEdge, by0 = by * kEdge;
-    const int px = bx0 + (wave & 1) * 8 + (lane & 7), py = by0 + (wave >> 1) * 8 + (lane >> 3);
+    const int qx0 = bx0 + (wave & 1) * 8, qy0 = by0 + (wave >> 1) * 8;
+    const int px = qx0 + (lane & 7), py = qy0 + (lane >> 3);
     const int tile = tileBase + (by0 / ts) * tilesX + bx0 / ts;
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
     const float fpx = float(px), fpy = float(py);
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f; // colour of the current segment (checkpoints only)
-    int last = 0, batches = 0;
+    int last = 0, batches = 0;             // batches: checkpoint segments started
     bool done = !inside;
     const size_t slot0 = size_t(begin / kBatch + tile);
-    // The next batch's records (and the ids of the one after) are loaded while the current
-    // batch is walked.
-    auto loadId = [&](int b0) { return (tid < kBatch && b0 + tid < end) ? tileList[b0 + tid] : -1; };
-    int gCur = loadId(begin), gAhead = loadId(begin + kBatch);
+    float4 *ls0 = s0[wave], *ls1 = s1[wave];
+    float *ls2 = s2[wave], *lthr = sthr[wave];
+    // The next step's records (and the ids of the one after) are loaded while the current
+    // step is walked.
+    auto loadId = [&](int b0) { return b0 + lane < end ? tileList[b0 + lane] : -1; };
+    int gCur = loadId(begin), gAhead = loadId(begin + kStep);
     float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
     float rc = 0.0f;
     if (gCur >= 0) {
@@ -268,40 +292,36 @@ __device__ __forceinline__ void forwardBlock(
         rb = rec1[gCur];
         rc = rec2[gCur];
     }
-    for (int base = begin; base < end; base += kBatch) {
-        if (__syncthreads_and(done))
+    for (int base = begin; base < end; base += kStep) {
+        if (__ballot(!done) == 0)
             break;
-        if (kCkpt && base > begin) { // T before this batch and the previous segment's colour
-            if (inside)
-                seg.ckpt[(slot0 + size_t(batches)) * kThreads + tid] = make_float4(T, sr, sg, sb);
-            sr = sg = sb = 0.0f;
+        if ((base - begin) % kBatch == 0) {
+            if (kCkpt && base > begin) { // T before this segment and the previous one's colour
+                if (inside)
+                    seg.ckpt[(slot0 + size_t(batches)) * kThreads + tid] = make_float4(T, sr, sg, sb);
+                sr = sg = sb = 0.0f;
+            }
+            ++batches;
         }
-        ++batches;
-        uint32_t m = 0;
+        bool hit = false;
         if (gCur >= 0) {
             const float thr = skipThreshold(minAlpha, rb.y);
-            s0[tid] = ra;
-            s1[tid] = rb;
-            s2[tid] = rc;
-            sthr[tid] = thr;
-            m = quadrantMask(ra, rb.x, thr, bx0, by0);
+            ls0[lane] = ra;
+            ls1[lane] = rb;
+            ls2[lane] = rc;
+            lthr[lane] = thr;
+            hit = quadrantHit(ra, rb.x, thr, qx0, qy0);
         }
-        if (tid < kBatch)
-            smask[tid] = m;
+        uint64_t bits = __ballot(hit);
         gCur = gAhead;
         if (gCur >= 0) {
             ra = rec0[gCur];
             rb = rec1[gCur];
             rc = rec2[gCur];
         }
-        gAhead = loadId(base + 2 * kBatch);
-        __syncthreads();
-        if (__ballot(!done) == 0)
-            continue;
-        for (int c = 0; c < kBatch / 64; ++c) {
-            uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
-            if (!bits)
-                continue;
+        gAhead = loadId(base + 2 * kStep);
+        waveSync();
+        {
             // One Gaussian for every pixel of the wavefront, without divergent branches: the
             // contribution is computed for all lanes and kept by selects where the reference
             // composites (same expressions, so the same values); lanes outside the window,
@@ -349,19 +369,18 @@ __device__ __forceinline__ void forwardBlock(
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k) {
                     const bool has = bits != 0; // wave-uniform
-                    jj[k] = has ? c * 64 + __builtin_ctzll(bits) : -1;
+                    jj[k] = has ? __builtin_ctzll(bits) : -1;
                     bits &= bits - 1;
-                    const int jl = has ? jj[k] : c * 64;
-                    pp[k] = prepare(s0[jl], s1[jl], sthr[jl], s2[jl]);
+                    const int jl = has ? jj[k] : 0;
+                    pp[k] = prepare(ls0[jl], ls1[jl], lthr[jl], ls2[jl]);
                 }
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k)
                     if (jj[k] >= 0)
                         composite(pp[k], jj[k]);
             }
-            if (__ballot(!done) == 0)
-                break;
         }
+        waveSync(); // the slice is restaged next step
     }
     if (inside) {
         const size_t pix = size_t(py) * size_t(W) + size_t(px);

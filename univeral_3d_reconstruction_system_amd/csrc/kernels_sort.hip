@@ -560,13 +560,6 @@ struct BinLds {
     uint32_t gauss[64];
 };
 
-// LDS written by one lane and read by another lane of the same wavefront.
-__device__ __forceinline__ void waveSync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // Corner updates of the difference grid D (row stride `stride`) for the rectangles of the
 // Gaussians at depth positions [first, first + count) (one wavefront).
 __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
