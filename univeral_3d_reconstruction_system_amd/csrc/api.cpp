@@ -174,9 +174,11 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
         U3D_TRY(exclusiveScanU32(s, r->segCount.p, r->segStart.p, tiles, r->segTotal.p,
                                  r->scanScratch.p, r->scanScratch.cap));
         U3D_HIP(hipMemsetAsync(r->segCounter.p, 0, sizeof(unsigned), s));
-        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // the compositing kernel alone
-        U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segStart.p, r->segTotal.p,
-                                      r->segCounter.p, r->ctx->num_cus * r->occSeg,
+        // items <= segments of all lists (one per 256 positions and tile) + 1
+        U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
+        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // item map + compositing kernel
+        U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segStart.p, r->segCount.p, r->segTotal.p,
+                                      r->segCounter.p, r->itemTile.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
                                       r->nContrib.p, r->ckpt.p, r->grad2d.p));
     } else {

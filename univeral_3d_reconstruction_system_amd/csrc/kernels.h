@@ -170,9 +170,11 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg);
 // Segmented backward: one work item per (tile, segment of B = 256 list positions); segStart =
-// exclusive scan of segCount (tiles + 1 entries), *totalItems its total; counter zero on entry.
+// exclusive scan of segCount (tiles + 1 entries), *totalItems its total; counter zero on entry;
+// itemTile: scratch of at least *totalItems entries (the owning tile of each item).
 int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
-                          const uint32_t *segStart, const uint32_t *totalItems, unsigned *counter,
+                          const uint32_t *segStart, const uint32_t *segCount,
+                          const uint32_t *totalItems, unsigned *counter, int *itemTile,
                           int numWorkgroups, const int *tileOffsets, const int *tileList,
                           const ScreenBuffers &sb, const float *dl, const float *finalT,
                           const int *nContrib, const float4 *ckpt, float *grad2d);

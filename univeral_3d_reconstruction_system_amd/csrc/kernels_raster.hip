@@ -139,7 +139,8 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     dAlpha += (c1 - acc1) * dp1;
     dAlpha += (c2 - acc2) * dp2;
     dAlpha *= Tn;
-    dAlpha += -fT / (1.0f - alpha) * bgDot;
+    if (bgDot != 0.0f || bgDot != bgDot) // the background term; adding +-0 changes nothing
+        dAlpha += -fT / (1.0f - alpha) * bgDot;
     const bool grads = contrib && !(rawAlpha > kMaxAlpha);
     const float dPower = op * dAlpha * G;
     const float A = a.z, B = a.w, C = b.x;
@@ -454,6 +455,18 @@ __device__ __forceinline__ void backwardSegment(
     const bool inside = px < W && py < H;
     const int begin = tileOffsets[tile];
     const int lo = segIdx * kBatch, hi = lo + kBatch;
+    // The segment's records are loaded before the walk's extent is known (their latency
+    // overlaps the per-pixel loads and the barrier below).
+    const int segLen = min(kBatch, tileOffsets[tile + 1] - begin - lo);
+    int g = -1;
+    float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+    float rc = 0.0f;
+    if (tid < segLen) {
+        g = tileList[begin + lo + tid];
+        ra = rec0[g];
+        rb = rec1[g];
+        rc = rec2[g];
+    }
     const size_t pix = inside ? size_t(py) * size_t(W) + size_t(px) : 0;
     const float fT = inside ? finalTArr[pix] : 0.0f;
     const int nc = inside ? nContribArr[pix] : 0;
@@ -486,20 +499,17 @@ __device__ __forceinline__ void backwardSegment(
     const int maxN = max(max(wm0, wm1), max(wm2, wm3));
     if (maxN <= lo)
         return; // uniform: no pixel of the block reaches this segment
-    const int cnt = maxN - lo;
+    const int cnt = maxN - lo; // <= segLen: contributor counts do not exceed the list
     uint32_t m = 0;
     if (tid < cnt) {
         const int j = lo + tid;
-        const int g = tileList[begin + j];
-        const float4 a = rec0[g];
-        const float4 b = rec1[g];
-        const float thr = skipThreshold(minAlpha, b.y);
+        const float thr = skipThreshold(minAlpha, rb.y);
         sid[tid] = g;
-        s0[tid] = a;
-        s1[tid] = b;
-        s2[tid] = rec2[g];
+        s0[tid] = ra;
+        s1[tid] = rb;
+        s2[tid] = rc;
         sthr[tid] = thr;
-        m = quadrantMask(a, b.x, thr, bx0, by0);
+        m = quadrantMask(ra, rb.x, thr, bx0, by0);
         m &= (j < wm0 ? 1u : 0u) | (j < wm1 ? 2u : 0u) | (j < wm2 ? 4u : 0u) | (j < wm3 ? 8u : 0u);
     }
     if (tid < kBatch)
@@ -719,16 +729,31 @@ __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart,
     return lo;
 }
 
+// Owning tile of every work item: itemTile[segStart[t] + k] = t for k < segCount[t].
+__global__ void segmentItemsKernel(const uint32_t *__restrict__ segStart,
+                                   const uint32_t *__restrict__ segCount, int tiles,
+                                   int *__restrict__ itemTile)
+{
+    const int t = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= tiles)
+        return;
+    const uint32_t s0 = segStart[t], n = segCount[t];
+    for (uint32_t k = 0; k < n; ++k)
+        itemTile[s0 + k] = t;
+}
+
 // Segmented backward of one image (16x16 tiles): workgroups take (tile, segment) items from
-// an atomic counter until *totalItems; segStart is the exclusive scan of the per-tile segment
-// counts written by the forward.
+// an atomic counter until *totalItems (itemTile maps an item to its tile, segStart is the
+// exclusive scan of the per-tile segment counts written by the forward). An item is claimed
+// only when the workgroup is free, so no item waits behind a busy workgroup.
 template <bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
-    int W, int H, int tilesX, int tiles, const uint32_t *__restrict__ segStart,
-    const uint32_t *__restrict__ totalItems, unsigned *counter, const int *__restrict__ tileOffsets,
-    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
-    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
-    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    int W, int H, int tilesX, const int *__restrict__ itemTile,
+    const uint32_t *__restrict__ segStart, const uint32_t *__restrict__ totalItems,
+    unsigned *counter, const int *__restrict__ tileOffsets, const int *__restrict__ tileList,
+    const float4 *__restrict__ rec0, const float4 *__restrict__ rec1,
+    const float *__restrict__ rec2, float bg0, float bg1, float bg2, float minAlpha,
+    const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, const float4 *__restrict__ ckpt,
     float *__restrict__ grad2d)
 {
@@ -742,10 +767,10 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
         __syncthreads();
         if (w >= total)
             break;
-        const int tile = tileOfItem(segStart, tiles, w);
-        backwardSegment<256, kFastExp>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H, tilesX,
-                             tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2, minAlpha, dl,
-                             finalTArr, nContribArr, ckpt, grad2d);
+        const int tile = itemTile[w];
+        backwardSegment<256, kFastExp>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H,
+                                       tilesX, tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1,
+                                       bg2, minAlpha, dl, finalTArr, nContribArr, ckpt, grad2d);
     }
 }
 
@@ -963,7 +988,8 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
 }
 
 int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
-                          const uint32_t *segStart, const uint32_t *totalItems, unsigned *counter,
+                          const uint32_t *segStart, const uint32_t *segCount,
+                          const uint32_t *totalItems, unsigned *counter, int *itemTile,
                           int numWorkgroups, const int *tileOffsets, const int *tileList,
                           const ScreenBuffers &sb, const float *dl, const float *finalT,
                           const int *nContrib, const float4 *ckpt, float *grad2d)
@@ -972,10 +998,13 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
     if (tiles == 0)
         return U3D_OK;
     U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
+    hipLaunchKernelGGL(segmentItemsKernel, dim3((tiles + 255) / 256), dim3(256), 0, s, segStart,
+                       segCount, tiles, itemTile);
+    U3D_LAUNCH_CHECK();
     auto kernel = st.min_alpha >= 1e-37f ? rasterSegmentedKernel<true> : rasterSegmentedKernel<false>;
     hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
-                       g.height, g.tilesX, tiles, segStart, totalItems, counter, tileOffsets, tileList,
-                       sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                       g.height, g.tilesX, itemTile, segStart, totalItems, counter, tileOffsets,
+                       tileList, sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
                        st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;

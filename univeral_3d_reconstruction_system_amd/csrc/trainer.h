@@ -110,6 +110,7 @@ struct u3d_raster {
     u3d::DevBuf<float4> ckpt;
     u3d::DevBuf<uint32_t> segCount, segStart, segTotal;
     u3d::DevBuf<unsigned> segCounter;
+    u3d::DevBuf<int> itemTile;
     int occSeg = 0;
     u3d::DevBuf<int> nContrib;
     u3d::DevBuf<float> dbgMean2d, dbgConic, dbgColor, dbgOpacity;
