@@ -242,7 +242,7 @@ def main(argv=None) -> int:
         project=("hbm", BYTES_PROJECT_PER_GAUSSIAN * mean_n),
         adam=("hbm", BYTES_ADAM_PER_PARAM * 14 * mean_n),
         loss=("hbm", BYTES_LOSS_PER_PIXEL * pixels),
-        sort=("hbm", 4 * 2 * 8 * mean_n + 8 * pairs + 2 * 2 * 8 * pairs),
+        sort=("hbm", 4 * 16 * mean_n + 2 * 24 * mean_n + 4 * pairs),
         chain_rule=("hbm", (56 + 36 + 112 + 8) * mean_n),
         raster_fwd=("mfma", FLOPS_FWD_PER_EVAL * evals),
         raster_bwd=("mfma", FLOPS_BWD_PER_EVAL * evals),
