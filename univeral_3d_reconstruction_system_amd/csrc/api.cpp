@@ -151,33 +151,35 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
 
 namespace {
 
+// consumeGrad2d: the chain rule zeroes the 2D gradients it reads (the trainer's step), so the
+// next backward needs no clearing pass; otherwise they stay readable (u3d_raster_get_grad2d).
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
-                       const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr)
+                       const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr,
+                       bool consumeGrad2d = false)
 {
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
     const hipStream_t s = r->ctx->stream;
     const size_t g2 = size_t(r->n) * kGrad2dStride;
+    const float *before = r->grad2d.p;
     U3D_TRY(r->grad2d.ensure(g2 > 0 ? g2 : 1));
-    if (g2)
-        U3D_HIP(hipMemsetAsync(r->grad2d.p, 0, g2 * sizeof(float), s));
+    if (r->grad2d.p != before)
+        r->grad2dZero = false;
+    if (g2 && !r->grad2dZero)
+        U3D_HIP(hipMemsetAsync(r->grad2d.p, 0, r->grad2d.cap * sizeof(float), s));
     const ScreenBuffers sb = r->screen();
     if (r->segmented) {
         const int tiles = r->geo.tilesX * r->geo.tilesY;
         U3D_TRY(r->segStart.ensure(size_t(tiles) + 1));
         U3D_TRY(r->segTotal.ensure(4));
         U3D_TRY(r->segCounter.ensure(1));
-        U3D_TRY(r->scanScratch.ensure(scanScratchSize(int64_t(tiles) + 1)));
         if (!r->occSeg)
             r->occSeg = rasterSegmentedOccupancy();
-        U3D_TRY(exclusiveScanU32(s, r->segCount.p, r->segStart.p, tiles, r->segTotal.p,
-                                 r->scanScratch.p, r->scanScratch.cap));
-        U3D_HIP(hipMemsetAsync(r->segCounter.p, 0, sizeof(unsigned), s));
         // items <= segments of all lists (one per 256 positions and tile) + 1
         U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
-        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // item map + compositing kernel
-        U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segStart.p, r->segCount.p, r->segTotal.p,
+        StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // item scan + compositing kernel
+        U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segCount.p, r->segStart.p, r->segTotal.p,
                                       r->segCounter.p, r->itemTile.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
                                       r->nContrib.p, r->ckpt.p, r->grad2d.p));
@@ -188,7 +190,8 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
     }
     StageScope scope(r->ctx, U3D_STAGE_CHAIN);
     U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount, shRest,
-                            shDegree, gShRest));
+                            shDegree, gShRest, consumeGrad2d));
+    r->grad2dZero = consumeGrad2d;
     return U3D_OK;
 }
 
@@ -976,13 +979,16 @@ int u3d_trainer_step(u3d_trainer *t)
                                       t->partials.p, t->partials.cap, stats));
     }
     t->grads.n = t->params.n;
-    if (t->grads.n > 0)
+    // The plain step's Adam zeroes the gradients it applies, so the buffer is cleared here only
+    // when something else may have left values in it.
+    if (t->grads.n > 0 && !t->gradsZero)
         U3D_HIP(hipMemsetAsync(t->grads.buf.p, 0, size_t(t->grads.cap) * 14 * sizeof(float), s));
+    t->gradsZero = false;
     u3d_gaussians gv = t->grads.view();
     if (t->shDegree > 0 && t->params.n > 0)
         U3D_HIP(hipMemsetAsync(t->shG.p, 0, size_t(t->params.n) * 45 * sizeof(float), s));
     U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
-                               t->shP.p, t->shDegree, t->shG.p));
+                               t->shP.p, t->shDegree, t->shG.p, true));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
     lr.means = trainerMeansLr(t);
@@ -992,7 +998,8 @@ int u3d_trainer_step(u3d_trainer *t)
     u3d_gaussians pm = t->params.view(), mm = t->adamM.view(), vm = t->adamV.view();
     {
         StageScope scope(t->ctx, U3D_STAGE_ADAM);
-        U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv));
+        U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv, true));
+        t->gradsZero = true;
         if (t->shDegree > 0) {
             const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
             U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,

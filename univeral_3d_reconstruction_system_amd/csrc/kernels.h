@@ -73,10 +73,11 @@ int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
 int projectBlocks(int64_t n);
 // Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
 // grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
+// clearGrad2d: zero the consumed 2D gradients (the buffer is then all zero again).
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
-                    const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
                     u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
-                    int shDegree, float *gShRest);
+                    int shDegree, float *gShRest, bool clearGrad2d = false);
 // Batched projection over items (grid y = item); entryItem[e] = owning item of screen entry e.
 int launchProjectBatched(hipStream_t s, const RenderItem *items, int numItems, int64_t maxN,
                          const u3d_raster_settings &st, const ScreenBuffers &out, int *entryItem);
@@ -169,15 +170,16 @@ struct SegmentState {
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg);
-// Segmented backward: one work item per (tile, segment of B = 256 list positions); segStart =
-// exclusive scan of segCount (tiles + 1 entries), *totalItems its total; counter zero on entry;
-// itemTile: scratch of at least *totalItems entries (the owning tile of each item).
+// Segmented backward: one work item per (tile, segment of B = 256 list positions) from the
+// forward's per-tile segment counts. Scratch written here: segStart (tiles + 1 entries, the
+// exclusive scan of segCount), *totalItems, counter, itemTile (at least Σ segCount entries:
+// the owning tile of each item).
 int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
-                          const uint32_t *segStart, const uint32_t *segCount,
-                          const uint32_t *totalItems, unsigned *counter, int *itemTile,
-                          int numWorkgroups, const int *tileOffsets, const int *tileList,
-                          const ScreenBuffers &sb, const float *dl, const float *finalT,
-                          const int *nContrib, const float4 *ckpt, float *grad2d);
+                          const uint32_t *segCount, uint32_t *segStart, uint32_t *totalItems,
+                          unsigned *counter, int *itemTile, int numWorkgroups,
+                          const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                          const float *dl, const float *finalT, const int *nContrib,
+                          const float4 *ckpt, float *grad2d);
 int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
                                  const uint32_t *segStart, const uint32_t *totalItems,
                                  unsigned *counter, int numWorkgroups, const int *tileOffsets,
@@ -289,8 +291,10 @@ int launchMeanAbs(hipStream_t s, const float *a, const uint8_t *bU8, int64_t cou
                   double *partials, size_t partialsCap, float *result);
 
 // ---- kernels_adam.hip
+// clearGrads: each gradient is zeroed once applied.
 int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_gaussians &m,
-                   u3d_gaussians &v, const float stepSize[5], float invSqrtBc2);
+                   u3d_gaussians &v, const float stepSize[5], float invSqrtBc2,
+                   bool clearGrads = false);
 // Adam over one extra parameter array (SH coefficients).
 int launchAdamArray(hipStream_t s, float *p, const float *g, float *m, float *v, int64_t len,
                     float stepSize, float invSqrtBc2);

@@ -12,7 +12,7 @@ constexpr int kThreads = 256;
 
 struct AdamGroups {
     float *p[5];
-    const float *g[5];
+    float *g[5];
     float *m[5];
     float *v[5];
     int64_t len[5];
@@ -20,6 +20,8 @@ struct AdamGroups {
 };
 
 // One pass over the five parameter groups (means, scales, quats, opacities, colours).
+// kClear: each gradient is zeroed once applied (the trainer's buffer is then all zero again).
+template <bool kClear>
 __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSqrtBc2)
 {
     const int64_t stride = int64_t(gridDim.x) * kThreads;
@@ -27,7 +29,7 @@ __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSq
 #pragma unroll
     for (int grp = 0; grp < 5; ++grp) {
         float *__restrict__ p = a.p[grp];
-        const float *__restrict__ g = a.g[grp];
+        float *__restrict__ g = a.g[grp];
         float *__restrict__ m = a.m[grp];
         float *__restrict__ v = a.v[grp];
         const float stepSize = a.stepSize[grp];
@@ -38,6 +40,8 @@ __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSq
             m[k] = mk;
             v[k] = vk;
             p[k] = p[k] - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
+            if (kClear)
+                g[k] = 0.0f;
         }
     }
 }
@@ -353,13 +357,13 @@ int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, in
 }
 
 int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_gaussians &m,
-                   u3d_gaussians &v, const float stepSize[5], float invSqrtBc2)
+                   u3d_gaussians &v, const float stepSize[5], float invSqrtBc2, bool clearGrads)
 {
     if (p.n <= 0)
         return U3D_OK;
     AdamGroups a;
     float *pp[5] = {p.means, p.scales, p.quats, p.opacities, p.colors};
-    const float *gg[5] = {g.means, g.scales, g.quats, g.opacities, g.colors};
+    float *gg[5] = {g.means, g.scales, g.quats, g.opacities, g.colors};
     float *mm[5] = {m.means, m.scales, m.quats, m.opacities, m.colors};
     float *vv[5] = {v.means, v.scales, v.quats, v.opacities, v.colors};
     const int dims[5] = {3, 3, 4, 1, 3};
@@ -375,7 +379,10 @@ int launchAdamStep(hipStream_t s, u3d_gaussians &p, const u3d_gaussians &g, u3d_
     unsigned blocks = blocksFor(maxLen);
     if (blocks > 4096)
         blocks = 4096;
-    hipLaunchKernelGGL(adamKernel, dim3(blocks), dim3(kThreads), 0, s, a, invSqrtBc2);
+    if (clearGrads)
+        hipLaunchKernelGGL(adamKernel<true>, dim3(blocks), dim3(kThreads), 0, s, a, invSqrtBc2);
+    else
+        hipLaunchKernelGGL(adamKernel<false>, dim3(blocks), dim3(kThreads), 0, s, a, invSqrtBc2);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -291,13 +291,15 @@ __device__ __forceinline__ void chainBody(
     }
 }
 
+// clear: the 2D gradients are zeroed after use (only visible Gaussians' entries are ever
+// written, so the buffer is all zero again for the next backward).
 __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
-    const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
+    const float4 *__restrict__ rec1, float *grad2d, float *gMeans,
     float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
-    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest)
+    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest, bool clear)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n)
@@ -305,6 +307,11 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     chainBody(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
                      gMeans, gScales, gQuats, gOpac, gColors, ndc, gradCount, shRest, shDegree, campos,
                      gShRest, 1.0f);
+    if (clear && radius[i] != 0) {
+#pragma unroll
+        for (int k = 0; k < kGrad2dStride; ++k)
+            grad2d[i * kGrad2dStride + k] = 0.0f;
+    }
 }
 
 // Batched chain rule over scene groups: Gaussian i of scene blockIdx.y takes the 2D gradients
@@ -374,9 +381,9 @@ int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
 }
 
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
-                    const u3d_raster_settings &st, const ScreenBuffers &sb, const float *grad2d,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
                     u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
-                    int shDegree, float *gShRest)
+                    int shDegree, float *gShRest, bool clearGrad2d)
 {
     if (m.n <= 0)
         return U3D_OK;
@@ -384,7 +391,7 @@ int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam
                        m.means, m.scales, m.quats, cam, st.near_plane, st.eps2d, sb.radius,
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
                        grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
-                       cameraCentre(cam), gShRest);
+                       cameraCentre(cam), gShRest, clearGrad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -729,17 +729,50 @@ __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart,
     return lo;
 }
 
-// Owning tile of every work item: itemTile[segStart[t] + k] = t for k < segCount[t].
-__global__ void segmentItemsKernel(const uint32_t *__restrict__ segStart,
-                                   const uint32_t *__restrict__ segCount, int tiles,
-                                   int *__restrict__ itemTile)
+// One workgroup: segStart = exclusive scan of segCount over the tiles (tiles + 1 entries),
+// *totalItems, the owning tile of every work item (itemTile[segStart[t] + k] = t for
+// k < segCount[t]) and the work counter cleared.
+constexpr int kPrepThreads = 1024;
+__global__ __launch_bounds__(kPrepThreads) void segmentPrepKernel(
+    const uint32_t *__restrict__ segCount, int tiles, uint32_t *__restrict__ segStart,
+    uint32_t *__restrict__ totalItems, int *__restrict__ itemTile, unsigned *counter)
 {
-    const int t = int(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= tiles)
-        return;
-    const uint32_t s0 = segStart[t], n = segCount[t];
-    for (uint32_t k = 0; k < n; ++k)
-        itemTile[s0 + k] = t;
+    __shared__ uint32_t carry[kPrepThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t base = 0;
+    for (int start = 0; start < tiles; start += kPrepThreads) {
+        const int t = start + tid;
+        const uint32_t v = t < tiles ? segCount[t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            carry[wave] = x;
+        __syncthreads();
+        uint32_t offs = base, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < kPrepThreads / 64; ++w) {
+            offs += w < wave ? carry[w] : 0u;
+            chunk += carry[w];
+        }
+        offs += x - v;
+        if (t < tiles) {
+            segStart[t] = offs;
+            for (uint32_t k = 0; k < v; ++k)
+                itemTile[offs + k] = t;
+        }
+        base += chunk;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        segStart[tiles] = base;
+        *totalItems = base;
+        *counter = 0u;
+    }
 }
 
 // Segmented backward of one image (16x16 tiles): workgroups take (tile, segment) items from
@@ -988,18 +1021,18 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
 }
 
 int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
-                          const uint32_t *segStart, const uint32_t *segCount,
-                          const uint32_t *totalItems, unsigned *counter, int *itemTile,
-                          int numWorkgroups, const int *tileOffsets, const int *tileList,
-                          const ScreenBuffers &sb, const float *dl, const float *finalT,
-                          const int *nContrib, const float4 *ckpt, float *grad2d)
+                          const uint32_t *segCount, uint32_t *segStart, uint32_t *totalItems,
+                          unsigned *counter, int *itemTile, int numWorkgroups,
+                          const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
+                          const float *dl, const float *finalT, const int *nContrib,
+                          const float4 *ckpt, float *grad2d)
 {
     const int tiles = g.tilesX * g.tilesY;
     if (tiles == 0)
         return U3D_OK;
     U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
-    hipLaunchKernelGGL(segmentItemsKernel, dim3((tiles + 255) / 256), dim3(256), 0, s, segStart,
-                       segCount, tiles, itemTile);
+    hipLaunchKernelGGL(segmentPrepKernel, dim3(1), dim3(kPrepThreads), 0, s, segCount, tiles,
+                       segStart, totalItems, itemTile, counter);
     U3D_LAUNCH_CHECK();
     auto kernel = st.min_alpha >= 1e-37f ? rasterSegmentedKernel<true> : rasterSegmentedKernel<false>;
     hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,

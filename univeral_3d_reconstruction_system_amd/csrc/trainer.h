@@ -105,6 +105,7 @@ struct u3d_raster {
     u3d::PinnedScratch pinnedTotal;
     u3d::SyncEvent totalReady;
     u3d::DevBuf<float> finalT, grad2d;
+    bool grad2dZero = false; // grad2d holds zeros in all cap entries
     // segmented backward (16-pixel tiles): forward checkpoints and the work-item scan
     bool segmented = false;
     u3d::DevBuf<float4> ckpt;
@@ -129,6 +130,7 @@ struct u3d_adam {
 struct u3d_trainer {
     u3d_ctx *ctx = nullptr;
     u3d::DevModel params, spare, grads, adamM, adamV;
+    bool gradsZero = false; // grads holds zeros in all cap entries
     int adamStep = 0;
     std::vector<u3d_camera> cams;
     std::vector<size_t> imageOffset;
