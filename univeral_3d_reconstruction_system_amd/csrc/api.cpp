@@ -102,13 +102,16 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     const uint32_t K = n > 0 ? *static_cast<const uint32_t *>(r->pinnedTotal.p) : 0u;
     r->numPairs = int64_t(K);
     const size_t kk = size_t(K > 0 ? K : 1);
+    const int *order = nullptr; // heavy-first tile order for the forward (binning path)
     if (tileBinningGroup(n, tilesX, tilesY) > 0) {
         U3D_TRY(r->tileListBuf.ensure(kk));
         U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tilesX, tilesY))));
         U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
+        U3D_TRY(r->heavyFirst.ensure(size_t(tiles)));
         U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tilesY,
                                   r->binCounts.p, r->binCounts.cap, r->tileTotal.p,
-                                  r->tileOffsets.p, r->tileListBuf.p));
+                                  r->tileOffsets.p, r->tileListBuf.p, r->heavyFirst.p));
+        order = n > 0 ? r->heavyFirst.p : nullptr;
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {
         // Many tiles: pairs emitted in depth order, then a stable radix sort on the tile id.
@@ -144,7 +147,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
     U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
-                                r->finalT.p, r->nContrib.p, seg));
+                                r->finalT.p, r->nContrib.p, seg, order));
     r->haveForward = true;
     return U3D_OK;
 }

@@ -128,9 +128,12 @@ size_t scanScratchSize(int64_t count);
 // counts: tileBinningCountsSize entries; tileTotal: tilesX * tilesY entries.
 int tileBinningGroup(int64_t n, int tilesX, int tilesY); // Gaussians per group, 0: unsupported
 size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
+// heavyFirst (optional, tiles entries): the tiles by decreasing list length, written unless
+// n <= 0.
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList);
+                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList,
+                      int *heavyFirst = nullptr);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
@@ -167,9 +170,11 @@ struct SegmentState {
     float4 *ckpt;
     uint32_t *segCount;
 };
+// blockOrder (optional, 16-pixel tiles only): workgroup i renders tile blockOrder[i].
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
-                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg);
+                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
+                        const int *blockOrder = nullptr);
 // Segmented backward: one work item per (tile, segment of B = 256 list positions) from the
 // forward's per-tile segment counts. Scratch written here: segStart (tiles + 1 entries, the
 // exclusive scan of segCount), *totalItems, counter, itemTile (at least Σ segCount entries:

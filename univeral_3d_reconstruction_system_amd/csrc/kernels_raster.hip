@@ -708,9 +708,10 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, SegmentState seg)
+    int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder)
 {
-    forwardBlock<256, kCkpt, kFastExp>(blockIdx.x % blocksX, blockIdx.x / blocksX, 0, W, H, ts,
+    const int blk = blockOrder ? blockOrder[blockIdx.x] : int(blockIdx.x);
+    forwardBlock<256, kCkpt, kFastExp>(blk % blocksX, blk / blocksX, 0, W, H, ts,
                                        tilesX, tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1,
                                        bg2, minAlpha, minT, rgb, finalT, nContrib, seg);
 }
@@ -986,7 +987,8 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
 
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
-                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg)
+                        float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
+                        const int *blockOrder)
 {
     U3D_CHECK(!seg.ckpt || g.tileSize == 16, U3D_ERR_INVALID, "segment checkpoints need 16-pixel tiles");
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
@@ -998,7 +1000,8 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
     hipLaunchKernelGGL(kernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s, g.width, g.height,
                        g.tileSize, g.tilesX, blocksX, tileOffsets, tileList, sb.rec0, sb.rec1,
                        sb.rec2, st.background[0], st.background[1], st.background[2],
-                       st.min_alpha, st.min_transmittance, rgb, finalT, nContrib, seg);
+                       st.min_alpha, st.min_transmittance, rgb, finalT, nContrib, seg,
+                       g.tileSize == kEdge ? blockOrder : nullptr);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -774,14 +774,46 @@ __global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *coun
 }
 
 // tileOffsets[0, tiles] = exclusive scan of the tile totals (one workgroup).
+// With heavyFirst: the tiles ordered by decreasing list length (64 log-scale classes, any
+// order inside a class), so that a launch that maps workgroup i to tile heavyFirst[i] spreads
+// the longest lists over different compute units.
 __global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_t *tileTotal,
                                                                          int tiles,
-                                                                         int *tileOffsets)
+                                                                         int *tileOffsets,
+                                                                         int *heavyFirst)
 {
     __shared__ uint32_t carry[kBinScanThreads / 64];
+    __shared__ uint32_t cls[64];
     const uint32_t all = workgroupScan(tileTotal, tiles, tileOffsets, carry);
-    if (threadIdx.x == 0)
+    const int tid = threadIdx.x;
+    if (tid == 0)
         tileOffsets[tiles] = int(all);
+    if (!heavyFirst)
+        return;
+    auto classOf = [](uint32_t len) {
+        const uint32_t v = len + 1u;
+        const int lg = 31 - __clz(v);
+        const int q = lg >= 2 ? int((v >> (lg - 2)) & 3u) : int((v << (2 - lg)) & 3u);
+        const int c = lg * 4 + q;
+        return 63 - (c < 63 ? c : 63); // heavier first
+    };
+    for (int k = tid; k < 64; k += kBinScanThreads)
+        cls[k] = 0;
+    __syncthreads();
+    for (int t = tid; t < tiles; t += kBinScanThreads)
+        atomicAdd(&cls[classOf(tileTotal[t])], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 64; ++k) {
+            const uint32_t c = cls[k];
+            cls[k] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < tiles; t += kBinScanThreads)
+        heavyFirst[atomicAdd(&cls[classOf(tileTotal[t])], 1u)] = t;
 }
 
 // Writes every Gaussian id at its final slot of its tile's list. W = blockDim.x / 64 wavefronts
@@ -977,7 +1009,7 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
 
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList)
+                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList, int *heavyFirst)
 {
     const int group = tileBinningGroup(n, tilesX, tilesY);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
@@ -995,7 +1027,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                        tileTotal);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(tileOffsetsScanKernel, dim3(1), dim3(kBinScanThreads), 0, s, tileTotal,
-                       tiles, tileOffsets);
+                       tiles, tileOffsets, heavyFirst);
     U3D_LAUNCH_CHECK();
     int cellBits = 1;
     while ((1 << cellBits) < cells)

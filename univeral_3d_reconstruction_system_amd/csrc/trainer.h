@@ -102,6 +102,7 @@ struct u3d_raster {
     // tile binning (tiles <= kMaxBinTiles): final lists, per-(group, tile) counters, the pair
     // total read back early through pinned memory
     u3d::DevBuf<uint32_t> tileListBuf, binCounts, tileTotal, blockTiles;
+    u3d::DevBuf<int> heavyFirst;
     u3d::PinnedScratch pinnedTotal;
     u3d::SyncEvent totalReady;
     u3d::DevBuf<float> finalT, grad2d;
