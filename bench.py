@@ -7,9 +7,10 @@ set starts at the 300k cap, so every timed step works on 300k Gaussians. One ste
 Trainer::step of the reference API (project, sort/bin, forward, L1 + 0.2 D-SSIM, backward,
 Adam, and the TrainOptions::forIterations(7000) densify / reset schedule).
 
-Prints one JSON line: iters/s (value), per-stage GPU times from HIP events recorded over the
-timed region, the dominant kernel's roofline, and the reference CPU trainer timed on this
-host's cores on a bounded sample of the same workload.
+Prints one JSON line: iters/s (value, timed with no instrumentation on the stream), per-stage
+GPU times from HIP events recorded over a second timed region of the same length, the dominant
+kernel's roofline, and the reference CPU trainer timed on this host's cores on a bounded sample
+of the same workload.
 """
 from __future__ import annotations
 
@@ -208,16 +209,11 @@ def main(argv=None) -> int:
     trainer.synchronize()
     c0 = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
     _abi.check(lib.u3d_trainer_counters(trainer.h, c0))
-    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 1))
     t0 = time.perf_counter()
     for _ in range(a.steps):
         trainer.step()
     trainer.synchronize()
     elapsed = time.perf_counter() - t0
-    ms = (C.c_double * _abi.U3D_NUM_STAGES)()
-    cnt = (C.c_int64 * _abi.U3D_NUM_STAGES)()
-    _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
-    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
     c1 = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
     _abi.check(lib.u3d_trainer_counters(trainer.h, c1))
     steps = c1[0] - c0[0]
@@ -225,6 +221,20 @@ def main(argv=None) -> int:
     pixels = (c1[2] - c0[2]) / steps
     pairs = (c1[3] - c0[3]) / steps
     stats = trainer.stats()
+
+    # Per-stage kernel times: the same training continued for the same number of steps with a
+    # pair of HIP timing events around every stage. Each timing event costs the stream an idle
+    # gap of ~10 us, so these steps are timed separately from the throughput loop above.
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 1))
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        trainer.step()
+    trainer.synchronize()
+    instrumented_elapsed = time.perf_counter() - t1
+    ms = (C.c_double * _abi.U3D_NUM_STAGES)()
+    cnt = (C.c_int64 * _abi.U3D_NUM_STAGES)()
+    _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
+    _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
 
     # Evaluated pairs (sum of contributor counts) from one forward of the trained model.
     r = U.Rasterizer(ctx)
@@ -234,7 +244,7 @@ def main(argv=None) -> int:
     evals = float(r.state()["n_contrib"].astype(np.int64).sum())
 
     stage_ms = {name: ms[k] / max(cnt[k], 1) for k, name in enumerate(_abi.STAGES)}
-    per_step_ms = {name: ms[k] / steps for k, name in enumerate(_abi.STAGES)}
+    per_step_ms = {name: ms[k] / a.steps for k, name in enumerate(_abi.STAGES)}
     # The roofline is reported for the dominant single kernel (every stage but "sort" is one
     # compositing, loss, projection or update launch; the sort stage is a chain of ~20 short
     # radix/scan launches and one host read of the pair count, so it is listed in "stages").
@@ -299,6 +309,7 @@ def main(argv=None) -> int:
         gaussian_pixels_per_s=round(mean_n * pixels * value, 1),
         mean_pairs_per_step=round(pairs, 1), evaluated_pairs_last_view=evals,
         stage_ms_per_step={k: round(v, 4) for k, v in per_step_ms.items()},
+        instrumented_ms_per_step=round(1e3 * instrumented_elapsed / a.steps, 4),
         roofline=roofline, cpu_baseline=base,
         gpu_vs_cpu=round(value / base["value"], 2) if base else None,
         train_psnr_ema=round(stats["psnr"], 3),
