@@ -84,6 +84,38 @@ def test_forward_many_tiles(ctx, w, h):
     assert r.state()["num_pairs"] == ro.num_pairs() > 0
 
 
+@pytest.mark.parametrize("n_base", [200_000, 400_000])
+def test_tile_lists_at_scale(ctx, n_base):
+    """600k and 1.2M Gaussians (every one repeated three times, so equal depths occur and must
+    keep index order): the tile lists equal a numpy restatement of the reference's order --
+    a stable sort of the depth bits with the invisible last, then a stable scatter by tile
+    (Rasterizer.cpp:206-261) -- built from the device's own rectangles and depths. 600k keys
+    take 147 blocks of the one-launch-per-pass depth sort; 1.2M keys exceed its 2^20 limit and
+    take the three-launch passes."""
+    base = S.random_gaussians(n_base, 5)
+    g = {k: np.tile(v.reshape(n_base, -1), (3, 1)).reshape(-1) for k, v in base.items()}
+    cam = S.ring_cameras(8, 640, 480)[2]
+    r = U.Rasterizer(ctx)
+    r.forward(U.DeviceGaussians.from_host(ctx, g), cam, U.make_settings())
+    s = r.state()
+    n, tiles_x, tiles = s["n"], s["tiles_x"], s["tiles_x"] * s["tiles_y"]
+    assert n == 3 * n_base
+    vis = s["radius"] > 0
+    key = np.where(vis, s["depth"].view(np.uint32), np.uint32(0xFFFFFFFF))
+    order = np.argsort(key, kind="stable")
+    rect = s["rect"].reshape(-1, 4)[order].astype(np.int64)
+    w = rect[:, 2] - rect[:, 0]
+    nt = np.where(vis[order], w * (rect[:, 3] - rect[:, 1]), 0)
+    rep = np.repeat(np.arange(n), nt)
+    k = np.arange(len(rep)) - np.repeat(np.cumsum(nt) - nt, nt)
+    tile = (rect[rep, 1] + k // w[rep]) * tiles_x + rect[rep, 0] + k % w[rep]
+    by = np.argsort(tile, kind="stable")
+    assert s["num_pairs"] == len(rep) > 0
+    assert np.array_equal(s["tile_list"], order[rep][by].astype(np.int32))
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(tile, minlength=tiles))])
+    assert np.array_equal(s["tile_offsets"], offsets.astype(np.int32))
+
+
 def test_forward_edge_cases(ctx):
     g = edge_case_model()
     cams = S.ring_cameras(4, 160, 120)

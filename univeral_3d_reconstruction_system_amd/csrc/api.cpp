@@ -95,6 +95,16 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(nn))));
     RadixScratch rs{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
     rs.valsAreIndex = true;
+    {
+        const uint64_t *before = r->sortScratch.p;
+        U3D_TRY(r->sortScratch.ensure(onesweepScratchWords()));
+        if (r->sortScratch.p != before) // the small sort expects zeros in a new scratch
+            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(r->sortScratch.p), 0,
+                                  int64_t(r->sortScratch.cap) * 2));
+        rs.onesweep = r->sortScratch.p;
+        rs.onesweepCap = r->sortScratch.cap;
+        rs.onesweepEpoch = &r->sortEpoch;
+    }
     bool alt = false;
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
     const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;

@@ -102,7 +102,15 @@ struct RadixScratch {
     size_t scanTmpCap;
     int items = 0;     // keys per lane per block: 4 or 8 (0: by size)
     bool valsAreIndex = false; // the input values are 0..count-1 (vals is then not read)
+    // Optional small-sort scratch (onesweepScratchWords() words, zero-filled once when
+    // allocated and then left to the sort) and its host-side call counter: counts up to 2^20
+    // with no device-side count sort in one launch per 8-bit pass plus one histogram launch.
+    // The launches carry the counter's value, so they must not be replayed from a graph.
+    uint64_t *onesweep = nullptr;
+    size_t onesweepCap = 0;
+    uint32_t *onesweepEpoch = nullptr;
 };
+size_t onesweepScratchWords();
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
                    bool *resultInAlt, const uint32_t *countDev = nullptr);

@@ -22,6 +22,8 @@
 // each Gaussian id at its final slot, so the K pairs are written once (4 B each) and never sorted.
 #include "kernels.h"
 
+#include <utility>
+
 namespace u3d {
 
 namespace {
@@ -279,6 +281,273 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
             const uint32_t pos = gbase[d] + waveCount[wave][d] + rank[it];
             keysOut[pos] = k[it];
             valsOut[pos] = v[it];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ small-sort path
+// Up to kOsMaxBlocks * 4096 keys the sort is bound by launch and memory latency, not by bytes,
+// so it runs as one histogram launch for every digit followed by one launch per 8-bit pass
+// (instead of three per pass). A pass workgroup draws a ticket (= its block of 4096 keys in
+// the current order), ranks its keys per wavefront, publishes its per-digit counts as 64-bit
+// words tagged with the call's epoch and the pass, and adds up the counts of every block with a
+// smaller ticket. Those blocks started earlier and wait for nobody before publishing, so every
+// wait ends; a thread that has re-read kOsSpinLimit words in all sets the error word and stops
+// waiting, so a fault elsewhere cannot hang the stream. The tags make resetting the lookback
+// words between calls unnecessary; the epoch comes from the host, one per call.
+constexpr int kOsThreads = 1024;
+constexpr int kOsWaves = kOsThreads / 64;
+constexpr int kOsItems = 4;
+constexpr int kOsBlockKeys = kOsThreads * kOsItems;
+constexpr int kOsMaxBlocks = 256;
+constexpr int kOsHistBlocks = 64;
+constexpr int kOsSpinLimit = 1 << 20;
+constexpr int kOsBatch = 24;
+// Scratch (u64 words): the state (u32 tickets[4], u32 error), the lookback words
+// [kOsMaxBlocks][256] and the histogram launch's per-block digit counts as u32
+// [kOsHistBlocks][4][256].
+constexpr size_t kOsStateWords = 8;
+constexpr size_t kOsLookbackWords = size_t(kOsMaxBlocks) * 256;
+constexpr size_t kOsPartialWords = size_t(kOsHistBlocks) * 4 * 256 / 2;
+constexpr int kOsTicket = 0, kOsError = 4;
+
+// Buffer resource word 3 for gfx9-family raw buffers; load cache policy sc0 | sc1 plus the
+// compiler's volatile bit (every load is issued, none is merged or hoisted).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+constexpr int kAuxCoherentVolatile = 1 | 16 | int(1u << 31);
+typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
+
+// Adds 1 to h[d] for every active lane; one add when the whole wavefront shares the digit.
+__device__ __forceinline__ void countDigit(uint32_t *h, uint32_t d, bool valid)
+{
+    const uint64_t vm = __ballot(valid);
+    if (vm == 0)
+        return;
+    const uint32_t d0 = uint32_t(__shfl(int(d), __ffsll((long long)vm) - 1, 64));
+    if (__ballot(valid && d == d0) == vm) {
+        if ((vm & laneMaskLt() & __ballot(valid)) == 0 && valid)
+            atomicAdd(&h[d0], uint32_t(__popcll(vm)));
+    } else if (valid) {
+        atomicAdd(&h[d], 1u);
+    }
+}
+
+// Digit counts of every pass over the keys [blockIdx.x * per, (blockIdx.x + 1) * per); resets
+// the pass tickets.
+__global__ __launch_bounds__(kOsThreads) void onesweepHistKernel(const uint32_t *__restrict__ keys,
+                                                                 int64_t count, int64_t per,
+                                                                 int bits,
+                                                                 uint32_t *__restrict__ partials,
+                                                                 uint32_t *state)
+{
+    __shared__ uint32_t h[4][256];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < 4 * 256; k += kOsThreads)
+        h[k >> 8][k & 255] = 0;
+    if (blockIdx.x == 0 && tid < 4)
+        state[kOsTicket + tid] = 0u;
+    __syncthreads();
+    const int passes = (bits + 7) >> 3;
+    const int64_t lo = int64_t(blockIdx.x) * per;
+    const int64_t hi = lo + per < count ? lo + per : count;
+    for (int64_t i0 = lo; i0 < hi; i0 += int64_t(kOsThreads) * 4) {
+        uint32_t key[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t i = i0 + int64_t(j) * kOsThreads + tid;
+            key[j] = i < hi ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool valid = i0 + int64_t(j) * kOsThreads + tid < hi;
+            for (int p = 0; p < passes; ++p) {
+                const int b = bits - 8 * p < 8 ? bits - 8 * p : 8;
+                countDigit(h[p], (key[j] >> (8 * p)) & ((1u << b) - 1u), valid);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < passes * 256; k += kOsThreads)
+        partials[size_t(blockIdx.x) * 1024 + size_t(k)] = h[k >> 8][k & 255];
+}
+
+// One stable counting pass on bits [shift, shift + bits) of the keys. kIota: the values are
+// the input positions (valsIn is not read). The block's keys are first placed in LDS in digit
+// order, so that each digit's keys leave as one contiguous run of addresses.
+template <bool kIota>
+__global__ __launch_bounds__(kOsThreads) void onesweepPassKernel(
+    const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
+    uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut, int64_t count, int shift,
+    int bits, int pass, uint32_t epoch, const uint32_t *__restrict__ partials,
+    uint64_t *lookback, uint32_t *state)
+{
+    __shared__ uint32_t waveCount[kOsWaves][256];
+    __shared__ uint32_t sKey[kOsBlockKeys], sVal[kOsBlockKeys];
+    __shared__ uint32_t sTot[4][256];
+    __shared__ uint32_t sPre[4][256];
+    __shared__ uint32_t blockStart[256];
+    __shared__ uint32_t waveTotA[4], waveTotT[4];
+    __shared__ int sBlock;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // a wavefront's 64 threads share their quarter: uniform, so per-block addresses are scalar
+    const int d = tid & 255, quarter = __builtin_amdgcn_readfirstlane(tid >> 8);
+    if (tid == 0)
+        sBlock = int(atomicAdd(&state[kOsTicket + pass], 1u));
+    for (int k = tid; k < kOsWaves * 256; k += kOsThreads)
+        waveCount[k >> 8][k & 255] = 0;
+    {
+        // this digit's total over the whole input (a quarter of the histogram blocks each)
+        uint32_t part[kOsHistBlocks / 4];
+#pragma unroll
+        for (int j = 0; j < kOsHistBlocks / 4; ++j)
+            part[j] = partials[size_t(quarter + 4 * j) * 1024 + size_t(pass) * 256 + size_t(d)];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int j = 0; j < kOsHistBlocks / 4; ++j)
+            tot += part[j];
+        sTot[quarter][d] = tot;
+    }
+    __syncthreads();
+    const int blk = __builtin_amdgcn_readfirstlane(sBlock);
+    const uint32_t tag = (epoch << 2) | uint32_t(pass);
+    const uint32_t mask = (1u << bits) - 1u;
+    const int64_t blockBase = int64_t(blk) * kOsBlockKeys;
+    const int blockKeys = int(count - blockBase < kOsBlockKeys ? count - blockBase : kOsBlockKeys);
+    const int waveBase = wave * 64 * kOsItems;
+    uint32_t k[kOsItems], v[kOsItems], rank[kOsItems];
+#pragma unroll
+    for (int it = 0; it < kOsItems; ++it) {
+        const int li = waveBase + it * 64 + lane;
+        const bool valid = li < blockKeys;
+        k[it] = valid ? keysIn[blockBase + li] : 0u;
+        v[it] = valid ? (kIota ? uint32_t(blockBase + li) : valsIn[blockBase + li]) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kOsItems; ++it) {
+        const bool valid = waveBase + it * 64 + lane < blockKeys;
+        const uint32_t dg = (k[it] >> shift) & mask;
+        const uint64_t vm = __ballot(valid);
+        rank[it] = 0;
+        if (vm == 0)
+            continue;
+        const uint64_t m = matchDigit(dg, vm, bits);
+        if (valid) {
+            const uint32_t prev = waveCount[wave][dg];
+            rank[it] = prev + uint32_t(__popcll(m & laneMaskLt()));
+            if ((m & laneMaskLt()) == 0)
+                waveCount[wave][dg] = prev + uint32_t(__popcll(m));
+        }
+    }
+    __syncthreads();
+    // Per digit: exclusive prefix over the wavefronts, the block's count (published at once),
+    // and inclusive wavefront scans over the digits of the block counts and of the totals.
+    uint32_t agg = 0, xa = 0, tot = 0, xt = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int w = 0; w < kOsWaves; ++w) {
+            const uint32_t c = waveCount[w][tid];
+            waveCount[w][tid] = agg;
+            agg += c;
+        }
+        __hip_atomic_store(&lookback[size_t(blk) * 256 + size_t(tid)],
+                           (uint64_t(tag) << 32) | uint64_t(agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        tot = sTot[0][tid] + sTot[1][tid] + sTot[2][tid] + sTot[3][tid];
+        xa = agg;
+        xt = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t ya = __shfl_up(xa, off, 64);
+            const uint32_t yt = __shfl_up(xt, off, 64);
+            if (lane >= off) {
+                xa += ya;
+                xt += yt;
+            }
+        }
+        if (lane == 63) {
+            waveTotA[wave] = xa;
+            waveTotT[wave] = xt;
+        }
+    }
+    __syncthreads();
+    uint32_t globalStart = 0;
+    if (tid < 256) {
+        uint32_t sa = xa - agg, st = xt - tot;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            sa += w < wave ? waveTotA[w] : 0u;
+            st += w < wave ? waveTotT[w] : 0u;
+        }
+        blockStart[tid] = sa;
+        globalStart = st;
+    }
+    __syncthreads();
+    // the block's keys in digit order (stable) in LDS
+#pragma unroll
+    for (int it = 0; it < kOsItems; ++it) {
+        if (waveBase + it * 64 + lane < blockKeys) {
+            const uint32_t dg = (k[it] >> shift) & mask;
+            const uint32_t lp = blockStart[dg] + waveCount[wave][dg] + rank[it];
+            sKey[lp] = k[it];
+            sVal[lp] = v[it];
+        }
+    }
+    // counts of this digit in the blocks with smaller tickets (a quarter of them each); all
+    // pending words of a batch are read again together. Buffer loads: one lane offset for all
+    // blocks, the block's row offset in a scalar register; volatile and coherent (sc0 sc1).
+    const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
+        lookback, 0, int(kOsLookbackWords * sizeof(uint64_t)), kBufferRsrcWord3);
+    auto readWord = [&](int b) {
+        const v2u32 x = __builtin_amdgcn_raw_buffer_load_b64(rows, d * 8, b * 256 * 8,
+                                                             kAuxCoherentVolatile);
+        return (uint64_t(x.y) << 32) | uint64_t(x.x);
+    };
+    uint32_t pre = 0;
+    int spins = 0; // rounds of re-reads, for the whole wait
+    for (int b0 = quarter; b0 < blk; b0 += 4 * kOsBatch) {
+        uint64_t w[kOsBatch];
+        uint32_t pending = 0;
+#pragma unroll
+        for (int j = 0; j < kOsBatch; ++j) {
+            const int b = b0 + 4 * j;
+            w[j] = b < blk ? readWord(b) : uint64_t(tag) << 32;
+        }
+#pragma unroll
+        for (int j = 0; j < kOsBatch; ++j)
+            pending |= uint32_t(w[j] >> 32) != tag ? 1u << j : 0u;
+        while (pending) {
+            if (++spins > kOsSpinLimit) {
+                __hip_atomic_store(&state[kOsError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+#pragma unroll
+            for (int j = 0; j < kOsBatch; ++j)
+                if (pending & (1u << j))
+                    w[j] = readWord(b0 + 4 * j);
+#pragma unroll
+            for (int j = 0; j < kOsBatch; ++j)
+                if (uint32_t(w[j] >> 32) == tag)
+                    pending &= ~(1u << j);
+        }
+#pragma unroll
+        for (int j = 0; j < kOsBatch; ++j)
+            pre += uint32_t(w[j]);
+    }
+    sPre[quarter][d] = pre;
+    __syncthreads();
+    // LDS slot i of digit dg goes to digitBase[dg] + i (unsigned arithmetic wraps back)
+    if (tid < 256)
+        sTot[0][tid] = globalStart + sPre[0][tid] + sPre[1][tid] + sPre[2][tid] + sPre[3][tid] -
+                       blockStart[tid];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kOsItems; ++it) {
+        const int i = it * kOsThreads + tid;
+        if (i < blockKeys) {
+            const uint32_t key = sKey[i];
+            const uint32_t pos = sTot[0][(key >> shift) & mask] + uint32_t(i);
+            keysOut[pos] = key;
+            valsOut[pos] = sVal[i];
         }
     }
 }
@@ -917,6 +1186,47 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
 }
 } // namespace
 
+size_t onesweepScratchWords() { return kOsStateWords + kOsLookbackWords + kOsPartialWords; }
+
+namespace {
+int onesweepSort(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
+                 uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
+                 bool *resultInAlt)
+{
+    const int numBlocks = blocksFor(count, kOsBlockKeys);
+    uint32_t *state = reinterpret_cast<uint32_t *>(scratch.onesweep);
+    uint64_t *lookback = scratch.onesweep + kOsStateWords;
+    uint32_t *partials = reinterpret_cast<uint32_t *>(lookback + kOsLookbackWords);
+    // a new epoch per call (never 0, the value of a zero-filled word); 30 bits with the pass
+    uint32_t &epoch = *scratch.onesweepEpoch;
+    epoch = (epoch + 1u) & 0x3fffffffu;
+    if (epoch == 0)
+        epoch = 1;
+    const int64_t per = (count + kOsHistBlocks - 1) / kOsHistBlocks;
+    hipLaunchKernelGGL(onesweepHistKernel, dim3(kOsHistBlocks), dim3(kOsThreads), 0, s, keys,
+                       count, per, bits, partials, state);
+    U3D_LAUNCH_CHECK();
+    uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
+    for (int pass = 0; pass * 8 < bits; ++pass) {
+        const int shift = pass * 8;
+        const int b = bits - shift < 8 ? bits - shift : 8;
+        if (pass == 0 && scratch.valsAreIndex)
+            hipLaunchKernelGGL(onesweepPassKernel<true>, dim3(numBlocks), dim3(kOsThreads), 0, s,
+                               kin, vin, kout, vout, count, shift, b, pass, epoch, partials,
+                               lookback, state);
+        else
+            hipLaunchKernelGGL(onesweepPassKernel<false>, dim3(numBlocks), dim3(kOsThreads), 0, s,
+                               kin, vin, kout, vout, count, shift, b, pass, epoch, partials,
+                               lookback, state);
+        U3D_LAUNCH_CHECK();
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        *resultInAlt = !*resultInAlt;
+    }
+    return U3D_OK;
+}
+} // namespace
+
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
                    bool *resultInAlt, const uint32_t *countDev)
@@ -924,6 +1234,12 @@ int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys
     *resultInAlt = false;
     if (count <= 0 || bits <= 0)
         return U3D_OK;
+    if (scratch.onesweep && scratch.onesweepEpoch && !countDev && bits <= 32 &&
+        count <= int64_t(kOsMaxBlocks) * kOsBlockKeys) {
+        U3D_CHECK(scratch.onesweepCap >= onesweepScratchWords(), U3D_ERR_CAPACITY,
+                  "small-sort scratch too small");
+        return onesweepSort(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
+    }
     const int items = scratch.items == 4 || scratch.items == 8 ? scratch.items : itemsFor(count);
     if (items == 4)
         return radixPasses<4>(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt,

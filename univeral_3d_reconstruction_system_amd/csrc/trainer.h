@@ -97,6 +97,8 @@ struct u3d_raster {
     u3d::DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
     u3d::DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
     u3d::DevBuf<uint32_t> hist, histScan, scanScratch, total;
+    u3d::DevBuf<uint64_t> sortScratch; // small-sort state, lookback words and histograms
+    uint32_t sortEpoch = 0;            // small-sort calls so far (tags its lookback words)
     u3d::DevBuf<int> tileOffsets;
     const int *tileList = nullptr;
     // tile binning (tiles <= kMaxBinTiles): final lists, per-(group, tile) counters, the pair
