@@ -2,6 +2,7 @@
 // the device-resident trainer. Host orchestration only; every kernel lives in a kernels_*.hip
 // translation unit.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -22,6 +23,37 @@ int failHip(hipError_t e, const char *what, const char *file, int line)
 {
     tError = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")";
     return U3D_ERR_HIP;
+}
+
+int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value)
+{
+    // Tight polling for the first millisecond (the usual wait is tens of microseconds); after
+    // that the stream is also asked for an error about every 100 microseconds.
+    using Clock = std::chrono::steady_clock;
+    const Clock::time_point start = Clock::now();
+    Clock::time_point nextQuery = start + std::chrono::milliseconds(1);
+    for (uint32_t polls = 1;; ++polls) {
+        const uint64_t w = __atomic_load_n(p, __ATOMIC_ACQUIRE);
+        if (uint32_t(w >> 32) == seq) {
+            *value = uint32_t(w);
+            return U3D_OK;
+        }
+        __builtin_ia32_pause();
+        if ((polls & 255u) != 0 || Clock::now() < nextQuery)
+            continue;
+        nextQuery = Clock::now() + std::chrono::microseconds(100);
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) {
+            // the stream has drained, so the word was stored unless something went wrong
+            const uint64_t w2 = __atomic_load_n(p, __ATOMIC_ACQUIRE);
+            U3D_CHECK(uint32_t(w2 >> 32) == seq, U3D_ERR_HIP,
+                      "the device-side pair count never arrived");
+            *value = uint32_t(w2);
+            return U3D_OK;
+        }
+        if (e != hipErrorNotReady)
+            return failHip(e, "hipStreamQuery", __FILE__, __LINE__);
+    }
 }
 
 int useDevice(u3d_ctx *ctx)
@@ -76,19 +108,22 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     U3D_TRY(r->nContrib.ensure(P > 0 ? P : 1));
     const ScreenBuffers sb = r->screen();
 
-    U3D_TRY(r->pinnedTotal.ensure(sizeof(uint32_t)));
+    U3D_TRY(r->pinnedTotal.ensure(sizeof(uint64_t)));
     U3D_TRY(r->blockTiles.ensure(size_t(std::max(1, projectBlocks(n)))));
-    *static_cast<uint32_t *>(r->pinnedTotal.p) = 0;
+    // the device stores (seq << 32 | K) there; seq is never 0, the word's value until then
+    uint64_t *pinnedWord = static_cast<uint64_t *>(r->pinnedTotal.p);
+    __atomic_store_n(pinnedWord, uint64_t(0), __ATOMIC_RELEASE);
+    r->totalSeq = r->totalSeq + 1u == 0u ? 1u : r->totalSeq + 1u;
+    const uint32_t seq = r->totalSeq;
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
         U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree, r->blockTiles.p));
         // The pair count K is known after the projection; the host reads it (pinned memory)
         // while the depth sort runs, so the device never waits for the host.
         if (n > 0)
-            U3D_TRY(launchSumU32(s, r->blockTiles.p, projectBlocks(n), r->total.p,
-                                 static_cast<uint32_t *>(r->pinnedTotal.p)));
+            U3D_TRY(launchSumU32(s, r->blockTiles.p, projectBlocks(n), r->total.p, pinnedWord,
+                                 seq));
     }
-    U3D_TRY(r->totalReady.record(s));
     std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
     // Depth order of the visible Gaussians (invisible keys sort last; ties keep index order).
     U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
@@ -108,8 +143,9 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     bool alt = false;
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
     const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
-    U3D_TRY(r->totalReady.wait());
-    const uint32_t K = n > 0 ? *static_cast<const uint32_t *>(r->pinnedTotal.p) : 0u;
+    uint32_t K = 0;
+    if (n > 0)
+        U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K));
     r->numPairs = int64_t(K);
     const size_t kk = size_t(K > 0 ? K : 1);
     const int *order = nullptr; // heavy-first tile order for the forward (binning path)

@@ -122,32 +122,11 @@ struct PinnedScratch {
     }
 };
 
-// A timing-free HIP event, created on first record.
-struct SyncEvent {
-    hipEvent_t e = nullptr;
-    SyncEvent() = default;
-    SyncEvent(const SyncEvent &) = delete;
-    SyncEvent &operator=(const SyncEvent &) = delete;
-    ~SyncEvent()
-    {
-        if (e)
-            (void)hipEventDestroy(e);
-    }
-    int record(hipStream_t s)
-    {
-        hipError_t r = hipSuccess;
-        if (!e)
-            r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
-        if (r == hipSuccess)
-            r = hipEventRecord(e, s);
-        return r == hipSuccess ? U3D_OK : failHip(r, "hipEventRecord", __FILE__, __LINE__);
-    }
-    int wait()
-    {
-        const hipError_t r = hipEventSynchronize(e);
-        return r == hipSuccess ? U3D_OK : failHip(r, "hipEventSynchronize", __FILE__, __LINE__);
-    }
-};
+// Waits until the pinned word *p carries seq in its high half (stored by a device-side
+// system-scope release) and returns its low half. Polls without any event on the stream; every
+// few thousand polls it asks the stream for an error, and fails if the stream has drained or
+// faulted without the word arriving.
+int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value);
 
 // Per-stage GPU time accumulated from HIP events recorded on the context's stream.
 struct StageTimer {

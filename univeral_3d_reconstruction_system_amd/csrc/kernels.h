@@ -122,9 +122,10 @@ int launchIota(hipStream_t s, uint32_t *vals, int64_t count);
 int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
                     int64_t count);
 // Sum of count u32 values (one workgroup) to *devOut and, if given, to *hostOut (pinned host
-// memory mapped for the device; visible to the host once the launch has completed).
+// memory mapped for the device) as the word (seq << 32 | sum), published by one system-scope
+// release store; waitPinnedWord() reads it without an event on the stream.
 int launchSumU32(hipStream_t s, const uint32_t *in, int64_t count, uint32_t *devOut,
-                 uint32_t *hostOut);
+                 uint64_t *hostOut = nullptr, uint32_t seq = 0);
 // Exclusive scan of count u32 values (out may alias in); total written to *total (device).
 int exclusiveScanU32(hipStream_t s, const uint32_t *in, uint32_t *out, int64_t count,
                      uint32_t *total, uint32_t *blockScratch, size_t blockScratchCap);

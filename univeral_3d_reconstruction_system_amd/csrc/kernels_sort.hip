@@ -629,8 +629,9 @@ __global__ __launch_bounds__(256) void scanDownsweep(const uint32_t *__restrict_
     }
 }
 
+// hostOut (optional): one 64-bit word (seq << 32 | sum) stored with a system-scope release.
 __global__ __launch_bounds__(1024) void sumKernel(const uint32_t *__restrict__ in, int64_t count,
-                                                  uint32_t *devOut, uint32_t *hostOut)
+                                                  uint32_t *devOut, uint64_t *hostOut, uint32_t seq)
 {
     __shared__ uint32_t ws[16];
     const int tid = threadIdx.x;
@@ -650,7 +651,8 @@ __global__ __launch_bounds__(1024) void sumKernel(const uint32_t *__restrict__ i
             t += ws[w];
         *devOut = t;
         if (hostOut)
-            __hip_atomic_store(hostOut, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(hostOut, (uint64_t(seq) << 32) | uint64_t(t), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1271,9 +1273,9 @@ int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint
 size_t scanScratchSize(int64_t count) { return size_t(blocksFor(count, kScanBlockItems)) + 2; }
 
 int launchSumU32(hipStream_t s, const uint32_t *in, int64_t count, uint32_t *devOut,
-                 uint32_t *hostOut)
+                 uint64_t *hostOut, uint32_t seq)
 {
-    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, in, count, devOut, hostOut);
+    hipLaunchKernelGGL(sumKernel, dim3(1), dim3(1024), 0, s, in, count, devOut, hostOut, seq);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -105,8 +105,8 @@ struct u3d_raster {
     // total read back early through pinned memory
     u3d::DevBuf<uint32_t> tileListBuf, binCounts, tileTotal, blockTiles;
     u3d::DevBuf<int> heavyFirst;
-    u3d::PinnedScratch pinnedTotal;
-    u3d::SyncEvent totalReady;
+    u3d::PinnedScratch pinnedTotal; // (call number << 32 | pair count), written by the device
+    uint32_t totalSeq = 0;
     u3d::DevBuf<float> finalT, grad2d;
     bool grad2dZero = false; // grad2d holds zeros in all cap entries
     // segmented backward (16-pixel tiles): forward checkpoints and the work-item scan
