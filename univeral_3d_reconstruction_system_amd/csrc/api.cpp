@@ -118,11 +118,6 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
         U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree, r->blockTiles.p));
-        // The pair count K is known after the projection; the host reads it (pinned memory)
-        // while the depth sort runs, so the device never waits for the host.
-        if (n > 0)
-            U3D_TRY(launchSumU32(s, r->blockTiles.p, projectBlocks(n), r->total.p, pinnedWord,
-                                 seq));
     }
     std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
     // Depth order of the visible Gaussians (invisible keys sort last; ties keep index order).
@@ -139,6 +134,16 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         rs.onesweep = r->sortScratch.p;
         rs.onesweepCap = r->sortScratch.cap;
         rs.onesweepEpoch = &r->sortEpoch;
+    }
+    // The pair count K is known after the projection; it is summed with the sort's first
+    // launch and the host reads it (pinned memory) while the depth sort runs, so the device
+    // never waits for the host.
+    if (n > 0) {
+        rs.sideIn = r->blockTiles.p;
+        rs.sideCount = projectBlocks(n);
+        rs.sideDev = r->total.p;
+        rs.sideHost = pinnedWord;
+        rs.sideSeq = seq;
     }
     bool alt = false;
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
@@ -205,7 +210,7 @@ namespace {
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
                        const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr,
-                       bool consumeGrad2d = false)
+                       bool consumeGrad2d = false, const LossFinalize *lossFinalize = nullptr)
 {
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
@@ -231,8 +236,10 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
         U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segCount.p, r->segStart.p, r->segTotal.p,
                                       r->segCounter.p, r->itemTile.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
-                                      r->nContrib.p, r->ckpt.p, r->grad2d.p));
+                                      r->nContrib.p, r->ckpt.p, r->grad2d.p, lossFinalize));
     } else {
+        if (lossFinalize)
+            U3D_TRY(launchLossFinalize(s, *lossFinalize));
         StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD);
         U3D_TRY(launchRasterBackward(s, r->geo, r->st, r->tileOffsets.p, r->tileList, sb, dl,
                                      r->finalT.p, r->nContrib.p, r->grad2d.p));
@@ -1021,11 +1028,13 @@ int u3d_trainer_step(u3d_trainer *t)
     t->counters[3] += t->raster.numPairs;
     t->counters[4] += 1;
     float *stats = t->stepStats.p + size_t(t->iteration) * 4;
+    // The loss value's final reduction rides on the backward's first launch.
+    LossFinalize lossFinalize;
     {
         StageScope scope(t->ctx, U3D_STAGE_LOSS);
         U3D_TRY(launchPhotometricLoss(s, t->render.p, nullptr, t->targets.p + t->imageOffset[view],
                                       cam.width, cam.height, t->opt.ssim_lambda, t->dloss.p,
-                                      t->partials.p, t->partials.cap, stats));
+                                      t->partials.p, t->partials.cap, stats, &lossFinalize));
     }
     t->grads.n = t->params.n;
     // The plain step's Adam zeroes the gradients it applies, so the buffer is cleared here only
@@ -1037,7 +1046,7 @@ int u3d_trainer_step(u3d_trainer *t)
     if (t->shDegree > 0 && t->params.n > 0)
         U3D_HIP(hipMemsetAsync(t->shG.p, 0, size_t(t->params.n) * 45 * sizeof(float), s));
     U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
-                               t->shP.p, t->shDegree, t->shG.p, true));
+                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
     lr.means = trainerMeansLr(t);

@@ -40,6 +40,64 @@ struct RenderItem {
     int *gradCount;
 };
 
+// Final reduction of the photometric loss (Loss.cpp:180-185): the per-block double partials
+// (l1, ssim, squared error) summed in a fixed order, result = {total, l1, ssim, psnr}.
+struct LossFinalize {
+    const double *partials = nullptr; // numBlocks x 3
+    int numBlocks = 0;
+    float lambda = 0.0f, invCount = 0.0f;
+    double count = 0.0;
+    float *result = nullptr;
+};
+
+// The reduction on the calling workgroup (a multiple of 64 threads, at least 256): the first 256
+// threads accumulate, so the summation order does not depend on the workgroup size.
+__device__ __forceinline__ void lossFinalizeBlock(const LossFinalize &f)
+{
+    __shared__ double red[3][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (tid < 256) {
+        for (int i = tid; i < f.numBlocks; i += 256) {
+            v[0] += f.partials[size_t(i) * 3];
+            v[1] += f.partials[size_t(i) * 3 + 1];
+            v[2] += f.partials[size_t(i) * 3 + 2];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v[j] += __shfl_xor(v[j], off, 64);
+    }
+    if (lane == 0 && wave < 16) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            red[j][wave] = v[j];
+    }
+    __syncthreads();
+    if (tid != 0)
+        return;
+    const int waves = int(blockDim.x) / 64 < 16 ? int(blockDim.x) / 64 : 16;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int w = 0; w < waves; ++w) {
+        a += red[0][w];
+        b += red[1][w];
+        c += red[2][w];
+    }
+    const float l1 = float(a * double(f.invCount));
+    float ssim = 0.0f, total = l1;
+    if (f.lambda > 0.0f) {
+        ssim = float(b * double(f.invCount));
+        total = (1.0f - f.lambda) * l1 + f.lambda * (1.0f - ssim);
+    }
+    const double mse = c / (f.count > 1.0 ? f.count : 1.0);
+    f.result[0] = total;
+    f.result[1] = l1;
+    f.result[2] = ssim;
+    f.result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
+}
+
 // Loss of one item of a batched step (target = uint8 / 255, HWC).
 struct LossItem {
     const float *render;
@@ -109,6 +167,14 @@ struct RadixScratch {
     uint64_t *onesweep = nullptr;
     size_t onesweepCap = 0;
     uint32_t *onesweepEpoch = nullptr;
+    // Optional sum of sideIn[0, sideCount) run with the sort, as launchSumU32 would (to sideDev
+    // and, if given, to the pinned word sideHost with sideSeq): an extra workgroup of the
+    // histogram launch on the small-sort path, its own launch before the sort otherwise.
+    const uint32_t *sideIn = nullptr;
+    int64_t sideCount = 0;
+    uint32_t *sideDev = nullptr;
+    uint64_t *sideHost = nullptr;
+    uint32_t sideSeq = 0;
 };
 size_t onesweepScratchWords();
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
@@ -193,7 +259,7 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           unsigned *counter, int *itemTile, int numWorkgroups,
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
-                          const float4 *ckpt, float *grad2d);
+                          const float4 *ckpt, float *grad2d, const LossFinalize *fin = nullptr);
 int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
                                  const uint32_t *segStart, const uint32_t *totalItems,
                                  unsigned *counter, int numWorkgroups, const int *tileOffsets,
@@ -288,10 +354,13 @@ int launchAdamBf16(hipStream_t s, float *p, const float *g, float *m, float *v, 
 // ---- kernels_loss.hip
 // Fused L1 + D-SSIM value and gradient. target is float (targetU8 == nullptr) or uint8/255.
 // Writes per-block double partials (l1, ssim, squared error) into partials and the final
-// {total, l1, ssim, psnr} floats into result (device) via a one-block finalize kernel.
+// {total, l1, ssim, psnr} floats into result (device) via a one-block finalize kernel; with
+// `deferred`, the finalize is not launched but described there, for a later launch to run it.
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
-                          float *grad, double *partials, size_t partialsCap, float *result);
+                          float *grad, double *partials, size_t partialsCap, float *result,
+                          LossFinalize *deferred = nullptr);
+int launchLossFinalize(hipStream_t s, const LossFinalize &f);
 size_t lossPartialsSize(int width, int height);
 void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks);
 int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,

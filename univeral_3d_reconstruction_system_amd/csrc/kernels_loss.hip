@@ -254,50 +254,20 @@ __global__ __launch_bounds__(kThreads) void lossBatchedKernel(const LossItem *__
                           it.grad, it.partials + (size_t(blockIdx.y) * it.numBlocks + blockIdx.x) * 3);
 }
 
-// Reduces per-block partials in a fixed order; result = {total, l1, ssim, psnr}.
-__device__ __forceinline__ void lossFinalizeBody(const double *__restrict__ partials,
-                                                 int numBlocks, float lambda, float invCount,
-                                                 double count, float *result)
-{
-    __shared__ double red[kThreads / 64];
-    double a = 0, b = 0, c = 0;
-    for (int i = threadIdx.x; i < numBlocks; i += kThreads) {
-        a += partials[size_t(i) * 3];
-        b += partials[size_t(i) * 3 + 1];
-        c += partials[size_t(i) * 3 + 2];
-    }
-    a = blockSumD(a, red);
-    b = blockSumD(b, red);
-    c = blockSumD(c, red);
-    if (threadIdx.x == 0) {
-        const float l1 = float(a * double(invCount));
-        float ssim = 0.0f, total = l1;
-        if (lambda > 0.0f) {
-            ssim = float(b * double(invCount));
-            total = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
-        }
-        const double mse = c / (count > 1.0 ? count : 1.0);
-        result[0] = total;
-        result[1] = l1;
-        result[2] = ssim;
-        result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void lossFinalize(const double *__restrict__ partials,
-                                                         int numBlocks, float lambda,
-                                                         float invCount, double count,
-                                                         float *result)
-{
-    lossFinalizeBody(partials, numBlocks, lambda, invCount, count, result);
-}
+__global__ __launch_bounds__(kThreads) void lossFinalize(LossFinalize f) { lossFinalizeBlock(f); }
 
 __global__ __launch_bounds__(kThreads) void lossFinalizeBatched(const LossItem *__restrict__ items,
                                                                 float lambda)
 {
     const LossItem &it = items[blockIdx.x];
-    lossFinalizeBody(it.partials, 3 * it.numBlocks, lambda, it.invCount,
-                     double(it.width) * double(it.height) * 3.0, it.result);
+    LossFinalize f;
+    f.partials = it.partials;
+    f.numBlocks = 3 * it.numBlocks;
+    f.lambda = lambda;
+    f.invCount = it.invCount;
+    f.count = double(it.width) * double(it.height) * 3.0;
+    f.result = it.result;
+    lossFinalizeBlock(f);
 }
 
 // Squared error / absolute error sums for evaluation passes.
@@ -367,7 +337,8 @@ size_t lossPartialsSize(int width, int height)
 
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
-                          float *grad, double *partials, size_t partialsCap, float *result)
+                          float *grad, double *partials, size_t partialsCap, float *result,
+                          LossFinalize *deferred)
 {
     const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
     const int nb = bx * by; // per channel
@@ -384,8 +355,23 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
                                targetU8, width, height, lambda, invCount, win, bx, grad, partials);
         U3D_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, partials, 3 * nb, lambda,
-                       invCount, double(P * 3), result);
+    LossFinalize f;
+    f.partials = partials;
+    f.numBlocks = 3 * nb;
+    f.lambda = lambda;
+    f.invCount = invCount;
+    f.count = double(P * 3);
+    f.result = result;
+    if (deferred) {
+        *deferred = f;
+        return U3D_OK;
+    }
+    return launchLossFinalize(s, f);
+}
+
+int launchLossFinalize(hipStream_t s, const LossFinalize &f)
+{
+    hipLaunchKernelGGL(lossFinalize, dim3(1), dim3(kThreads), 0, s, f);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

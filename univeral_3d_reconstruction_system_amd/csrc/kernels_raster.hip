@@ -734,10 +734,16 @@ __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart,
 // *totalItems, the owning tile of every work item (itemTile[segStart[t] + k] = t for
 // k < segCount[t]) and the work counter cleared.
 constexpr int kPrepThreads = 1024;
+// Workgroup 1 (when launched) finishes the photometric loss of the same step instead.
 __global__ __launch_bounds__(kPrepThreads) void segmentPrepKernel(
     const uint32_t *__restrict__ segCount, int tiles, uint32_t *__restrict__ segStart,
-    uint32_t *__restrict__ totalItems, int *__restrict__ itemTile, unsigned *counter)
+    uint32_t *__restrict__ totalItems, int *__restrict__ itemTile, unsigned *counter,
+    LossFinalize fin)
 {
+    if (blockIdx.x == 1) {
+        lossFinalizeBlock(fin);
+        return;
+    }
     __shared__ uint32_t carry[kPrepThreads / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t base = 0;
@@ -1028,14 +1034,15 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           unsigned *counter, int *itemTile, int numWorkgroups,
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
-                          const float4 *ckpt, float *grad2d)
+                          const float4 *ckpt, float *grad2d, const LossFinalize *fin)
 {
     const int tiles = g.tilesX * g.tilesY;
     if (tiles == 0)
-        return U3D_OK;
+        return fin ? launchLossFinalize(s, *fin) : U3D_OK;
     U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
-    hipLaunchKernelGGL(segmentPrepKernel, dim3(1), dim3(kPrepThreads), 0, s, segCount, tiles,
-                       segStart, totalItems, itemTile, counter);
+    hipLaunchKernelGGL(segmentPrepKernel, dim3(fin ? 2 : 1), dim3(kPrepThreads), 0, s, segCount,
+                       tiles, segStart, totalItems, itemTile, counter,
+                       fin ? *fin : LossFinalize{});
     U3D_LAUNCH_CHECK();
     auto kernel = st.min_alpha >= 1e-37f ? rasterSegmentedKernel<true> : rasterSegmentedKernel<false>;
     hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,

@@ -285,6 +285,34 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     }
 }
 
+// Sum on a workgroup of 1024 threads. hostOut (optional): one 64-bit word (seq << 32 | sum)
+// stored with a system-scope release.
+__device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_t count,
+                                         uint32_t *devOut, uint64_t *hostOut, uint32_t seq)
+{
+    __shared__ uint32_t ws[16];
+    const int tid = threadIdx.x;
+    uint32_t v = 0;
+    for (int64_t i = tid; i < count; i += 1024)
+        v += in[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    if ((tid & 63) == 0)
+        ws[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w)
+            t += ws[w];
+        *devOut = t;
+        if (hostOut)
+            __hip_atomic_store(hostOut, (uint64_t(seq) << 32) | uint64_t(t), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // ------------------------------------------------------------------ small-sort path
 // Up to kOsMaxBlocks * 4096 keys the sort is bound by launch and memory latency, not by bytes,
 // so it runs as one histogram launch for every digit followed by one launch per 8-bit pass
@@ -333,13 +361,16 @@ __device__ __forceinline__ void countDigit(uint32_t *h, uint32_t d, bool valid)
 }
 
 // Digit counts of every pass over the keys [blockIdx.x * per, (blockIdx.x + 1) * per); resets
-// the pass tickets.
-__global__ __launch_bounds__(kOsThreads) void onesweepHistKernel(const uint32_t *__restrict__ keys,
-                                                                 int64_t count, int64_t per,
-                                                                 int bits,
-                                                                 uint32_t *__restrict__ partials,
-                                                                 uint32_t *state)
+// the pass tickets. Workgroup kOsHistBlocks (when launched) runs the side sum instead.
+__global__ __launch_bounds__(kOsThreads) void onesweepHistKernel(
+    const uint32_t *__restrict__ keys, int64_t count, int64_t per, int bits,
+    uint32_t *__restrict__ partials, uint32_t *state, const uint32_t *__restrict__ sideIn,
+    int64_t sideCount, uint32_t *sideDev, uint64_t *sideHost, uint32_t sideSeq)
 {
+    if (blockIdx.x == kOsHistBlocks) {
+        sumBlock(sideIn, sideCount, sideDev, sideHost, sideSeq);
+        return;
+    }
     __shared__ uint32_t h[4][256];
     const int tid = threadIdx.x;
     for (int k = tid; k < 4 * 256; k += kOsThreads)
@@ -629,31 +660,10 @@ __global__ __launch_bounds__(256) void scanDownsweep(const uint32_t *__restrict_
     }
 }
 
-// hostOut (optional): one 64-bit word (seq << 32 | sum) stored with a system-scope release.
 __global__ __launch_bounds__(1024) void sumKernel(const uint32_t *__restrict__ in, int64_t count,
                                                   uint32_t *devOut, uint64_t *hostOut, uint32_t seq)
 {
-    __shared__ uint32_t ws[16];
-    const int tid = threadIdx.x;
-    uint32_t v = 0;
-    for (int64_t i = tid; i < count; i += 1024)
-        v += in[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
-    if ((tid & 63) == 0)
-        ws[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w)
-            t += ws[w];
-        *devOut = t;
-        if (hostOut)
-            __hip_atomic_store(hostOut, (uint64_t(seq) << 32) | uint64_t(t), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    sumBlock(in, count, devOut, hostOut, seq);
 }
 
 // Scan downsweep that sums the previous blocks' partial sums itself (for up to
@@ -1205,8 +1215,10 @@ int onesweepSort(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAl
     if (epoch == 0)
         epoch = 1;
     const int64_t per = (count + kOsHistBlocks - 1) / kOsHistBlocks;
-    hipLaunchKernelGGL(onesweepHistKernel, dim3(kOsHistBlocks), dim3(kOsThreads), 0, s, keys,
-                       count, per, bits, partials, state);
+    hipLaunchKernelGGL(onesweepHistKernel, dim3(kOsHistBlocks + (scratch.sideIn ? 1 : 0)),
+                       dim3(kOsThreads), 0, s, keys, count, per, bits, partials, state,
+                       scratch.sideIn, scratch.sideCount, scratch.sideDev, scratch.sideHost,
+                       scratch.sideSeq);
     U3D_LAUNCH_CHECK();
     uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
     for (int pass = 0; pass * 8 < bits; ++pass) {
@@ -1234,10 +1246,14 @@ int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys
                    bool *resultInAlt, const uint32_t *countDev)
 {
     *resultInAlt = false;
+    const bool small = scratch.onesweep && scratch.onesweepEpoch && !countDev && bits <= 32 &&
+                       count > 0 && count <= int64_t(kOsMaxBlocks) * kOsBlockKeys;
+    if (scratch.sideIn && !(small && bits > 0))
+        U3D_TRY(launchSumU32(s, scratch.sideIn, scratch.sideCount, scratch.sideDev,
+                             scratch.sideHost, scratch.sideSeq));
     if (count <= 0 || bits <= 0)
         return U3D_OK;
-    if (scratch.onesweep && scratch.onesweepEpoch && !countDev && bits <= 32 &&
-        count <= int64_t(kOsMaxBlocks) * kOsBlockKeys) {
+    if (small) {
         U3D_CHECK(scratch.onesweepCap >= onesweepScratchWords(), U3D_ERR_CAPACITY,
                   "small-sort scratch too small");
         return onesweepSort(s, keys, vals, keysAlt, valsAlt, count, bits, scratch, resultInAlt);
