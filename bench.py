@@ -38,6 +38,10 @@ ITERATIONS = 7000
 # Algorithmic work per unit (SH degree 0), used for the roofline of each stage.
 BYTES_PROJECT_PER_GAUSSIAN = 56 + 48          # 14 parameter floats in, 48 B of screen state out
 BYTES_ADAM_PER_PARAM = 28                      # p, g, m, v read (16 B) + p, m, v written (12 B)
+# Chain rule fused with Adam, per Gaussian: p, m, v (3 x 56 B), the 2D gradient (36 B), radius,
+# flags, opacity record (16 B), densify accumulators (8 B) read; p, m, v, the cleared 2D
+# gradient and the densify accumulators written.
+BYTES_CHAIN_ADAM_PER_GAUSSIAN = (3 * 56 + 36 + 4 + 1 + 16 + 8) + (3 * 56 + 36 + 8)
 BYTES_LOSS_PER_PIXEL = 36                      # render + target + grad, 3 x 4 B x 3 channels
 BYTES_RASTER_FWD_PER_PAIR, BYTES_RASTER_FWD_PER_PIXEL = 40, 20
 BYTES_RASTER_BWD_PER_PAIR, BYTES_RASTER_BWD_PER_PIXEL = 40 + 36, 20
@@ -253,7 +257,7 @@ def main(argv=None) -> int:
         adam=("hbm", BYTES_ADAM_PER_PARAM * 14 * mean_n),
         loss=("hbm", BYTES_LOSS_PER_PIXEL * pixels),
         sort=("hbm", 4 * 16 * mean_n + 2 * 24 * mean_n + 4 * pairs),
-        chain_rule=("hbm", (56 + 36 + 112 + 8) * mean_n),
+        chain_rule=("hbm", BYTES_CHAIN_ADAM_PER_GAUSSIAN * mean_n),
         raster_fwd=("mfma", FLOPS_FWD_PER_EVAL * evals),
         raster_bwd=("mfma", FLOPS_BWD_PER_EVAL * evals),
     )
@@ -269,7 +273,7 @@ def main(argv=None) -> int:
 
     stages = {}
     for name in algo:
-        if cnt[_abi.STAGES.index(name)] == 0:
+        if cnt[_abi.STAGES.index(name)] == 0 or stage_ms[name] <= 0.0:
             continue
         bound, achieved, peak, unit = stage_roof(name)
         stages[name] = dict(bound=bound, ms=round(stage_ms[name], 4), achieved=round(achieved, 4),

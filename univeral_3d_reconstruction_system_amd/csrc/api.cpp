@@ -210,7 +210,8 @@ namespace {
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
                        const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr,
-                       bool consumeGrad2d = false, const LossFinalize *lossFinalize = nullptr)
+                       bool consumeGrad2d = false, const LossFinalize *lossFinalize = nullptr,
+                       const AdamStepArgs *adam = nullptr)
 {
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
@@ -245,6 +246,13 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                                      r->finalT.p, r->nContrib.p, r->grad2d.p));
     }
     StageScope scope(r->ctx, U3D_STAGE_CHAIN);
+    if (adam) {
+        // the trainer's step: chain rule and Adam in one pass, grads untouched
+        U3D_TRY(launchChainAdam(s, m, r->cam, r->st, sb, r->grad2d.p, ndc, gradCount, shRest,
+                                shDegree, gShRest, *adam));
+        r->grad2dZero = true;
+        return U3D_OK;
+    }
     U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount, shRest,
                             shDegree, gShRest, consumeGrad2d));
     r->grad2dZero = consumeGrad2d;
@@ -1037,32 +1045,27 @@ int u3d_trainer_step(u3d_trainer *t)
                                       t->partials.p, t->partials.cap, stats, &lossFinalize));
     }
     t->grads.n = t->params.n;
-    // The plain step's Adam zeroes the gradients it applies, so the buffer is cleared here only
-    // when something else may have left values in it.
-    if (t->grads.n > 0 && !t->gradsZero)
-        U3D_HIP(hipMemsetAsync(t->grads.buf.p, 0, size_t(t->grads.cap) * 14 * sizeof(float), s));
-    t->gradsZero = false;
     u3d_gaussians gv = t->grads.view();
     if (t->shDegree > 0 && t->params.n > 0)
         U3D_HIP(hipMemsetAsync(t->shG.p, 0, size_t(t->params.n) * 45 * sizeof(float), s));
-    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
-                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
     lr.means = trainerMeansLr(t);
     ++t->adamStep;
-    float stepSize[5], inv;
-    adamCoefficients(t->adamStep, lr, stepSize, &inv);
-    u3d_gaussians pm = t->params.view(), mm = t->adamM.view(), vm = t->adamV.view();
-    {
+    AdamStepArgs adam;
+    adamCoefficients(t->adamStep, lr, adam.stepSize, &adam.invSqrtBc2);
+    adam.m = t->adamM.view();
+    adam.v = t->adamV.view();
+    const float inv = adam.invSqrtBc2;
+    // Backward, then the chain rule and Adam in one pass: the 14 parameter gradients of a
+    // Gaussian stay in registers (the gradient buffer is not used by this step).
+    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
+                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize, &adam));
+    if (t->shDegree > 0) {
         StageScope scope(t->ctx, U3D_STAGE_ADAM);
-        U3D_TRY(launchAdamStep(s, pm, gv, mm, vm, stepSize, inv, true));
-        t->gradsZero = true;
-        if (t->shDegree > 0) {
-            const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
-            U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,
-                                    (lr.colors * shRateScale(t)) / bc1, inv));
-        }
+        const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
+        U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,
+                                (lr.colors * shRateScale(t)) / bc1, inv));
     }
     return trainerFinishStep(t);
 }

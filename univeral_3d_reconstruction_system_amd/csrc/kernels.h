@@ -40,6 +40,18 @@ struct RenderItem {
     int *gradCount;
 };
 
+// One Adam update of a scalar parameter (Adam.cpp:13-28): beta1 0.9, beta2 0.999, and 1e-15
+// added to sqrt(v) * invSqrtBc2; stepSize is the learning rate over 1 - beta1^t.
+__device__ __forceinline__ void adamScalar(float &p, float &m, float &v, float g, float stepSize,
+                                           float invSqrtBc2)
+{
+    const float mk = 0.9f * m + (1.0f - 0.9f) * g;
+    const float vk = 0.999f * v + (1.0f - 0.999f) * g * g;
+    m = mk;
+    v = vk;
+    p = p - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
+}
+
 // Final reduction of the photometric loss (Loss.cpp:180-185): the per-block double partials
 // (l1, ssim, squared error) summed in a fixed order, result = {total, l1, ssim, psnr}.
 struct LossFinalize {
@@ -132,6 +144,21 @@ int projectBlocks(int64_t n);
 // Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
 // grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
 // clearGrad2d: zero the consumed 2D gradients (the buffer is then all zero again).
+// Adam coefficients of one step for the five parameter groups (means, scales, quats,
+// opacities, colours) and the model-shaped first and second moments.
+struct AdamStepArgs {
+    u3d_gaussians m, v;
+    float stepSize[5];
+    float invSqrtBc2;
+};
+// The trainer's update after the backward: the chain rule of each Gaussian into registers and
+// its Adam update from them (Adam runs for every Gaussian, visible or not), so the parameter
+// gradients never reach memory; the 2D gradients are cleared after use. Same arithmetic as
+// launchChainRule into zeroed gradients followed by launchAdamStep.
+int launchChainAdam(hipStream_t s, const u3d_gaussians &params, const u3d_camera &cam,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
+                    float *ndc, int *gradCount, const float *shRest, int shDegree, float *gShRest,
+                    const AdamStepArgs &adam);
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
                     u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,

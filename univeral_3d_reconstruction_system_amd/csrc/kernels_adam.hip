@@ -34,12 +34,11 @@ __global__ __launch_bounds__(kThreads) void adamKernel(AdamGroups a, float invSq
         float *__restrict__ v = a.v[grp];
         const float stepSize = a.stepSize[grp];
         for (int64_t k = start; k < a.len[grp]; k += stride) {
-            const float gk = g[k];
-            const float mk = 0.9f * m[k] + (1.0f - 0.9f) * gk;
-            const float vk = 0.999f * v[k] + (1.0f - 0.999f) * gk * gk;
+            float pk = p[k], mk = m[k], vk = v[k];
+            adamScalar(pk, mk, vk, g[k], stepSize, invSqrtBc2);
             m[k] = mk;
             v[k] = vk;
-            p[k] = p[k] - stepSize * mk / (sqrtf(vk) * invSqrtBc2 + 1e-15f);
+            p[k] = pk;
             if (kClear)
                 g[k] = 0.0f;
         }

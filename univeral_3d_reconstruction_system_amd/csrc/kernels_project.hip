@@ -142,14 +142,36 @@ __global__ __launch_bounds__(kThreads) void projectBatchedKernel(const RenderIte
                 eps2d, ts, it.tilesX, it.tilesY, out, it.shRest, it.shDegree, cameraCentre(it.cam));
 }
 
+// Where chainBody adds the parameter gradients of Gaussian i: the model-shaped gradient arrays
+// (added to in place) or registers that start at zero (the same sums, term by term).
+struct GradArrays {
+    float *means, *scales, *quats, *opac, *colors;
+    int64_t i;
+    __device__ void mean(int k, float v) const { means[i * 3 + k] += v; }
+    __device__ void scale(int k, float v) const { scales[i * 3 + k] += v; }
+    __device__ void quat(int k, float v) const { quats[i * 4 + k] += v; }
+    __device__ void opacity(float v) const { opac[i] += v; }
+    __device__ void color(int k, float v) const { colors[i * 3 + k] += v; }
+};
+struct GradRegs {
+    float means[3] = {0.0f, 0.0f, 0.0f}, scales[3] = {0.0f, 0.0f, 0.0f};
+    float quats[4] = {0.0f, 0.0f, 0.0f, 0.0f}, opac = 0.0f, colors[3] = {0.0f, 0.0f, 0.0f};
+    __device__ void mean(int k, float v) { means[k] += v; }
+    __device__ void scale(int k, float v) { scales[k] += v; }
+    __device__ void quat(int k, float v) { quats[k] += v; }
+    __device__ void opacity(float v) { opac += v; }
+    __device__ void color(int k, float v) { colors[k] += v; }
+};
+
 // Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o.
+template <class Sink>
 __device__ __forceinline__ void chainBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
-    const float4 *__restrict__ rec1, const float *__restrict__ grad2d, float *gMeans,
-    float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
-    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest, float ndcScale)
+    const float4 *__restrict__ rec1, const float *__restrict__ grad2d, Sink &out, float *ndc,
+    int *gradCount, const float *__restrict__ shRest, int shDegree, float3 campos,
+    float *gShRest, float ndcScale)
 {
     if (radius[o] == 0)
         return;
@@ -168,9 +190,9 @@ __device__ __forceinline__ void chainBody(
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
         if (!(flags & (4 << ch)))
-            gColors[i * 3 + ch] += kShC0 * g[5 + ch];
+            out.color(ch, kShC0 * g[5 + ch]);
     const float op = rec1[o].y;
-    gOpac[i] += g[8] * op * (1.0f - op);
+    out.opacity(g[8] * op * (1.0f - op));
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
     // View-dependent colour: coefficient gradients and the direction -> mean term.
@@ -243,7 +265,7 @@ __device__ __forceinline__ void chainBody(
             ds += dM[r * 3 + col] * p.Rq[r * 3 + col];
             dRq[r * 3 + col] = dM[r * 3 + col] * p.s[col];
         }
-        gScales[i * 3 + col] += ds * p.s[col];
+        out.scale(col, ds * p.s[col]);
     }
     const float w = p.q[0], x = p.q[1], y = p.q[2], zq = p.q[3];
     const float dw = 2 * (-zq * dRq[1] + y * dRq[2] + zq * dRq[3] - x * dRq[5] - y * dRq[6] + x * dRq[7]);
@@ -257,7 +279,7 @@ __device__ __forceinline__ void chainBody(
     const float qdot = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        gQuats[i * 4 + k] += (dq[k] - qdot * p.q[k]) / p.qNorm;
+        out.quat(k, (dq[k] - qdot * p.q[k]) / p.qNorm);
     const float *R = cam.R;
     const float dJ00 = dT0[0] * R[0] + dT0[1] * R[1] + dT0[2] * R[2];
     const float dJ02 = dT0[0] * R[6] + dT0[1] * R[7] + dT0[2] * R[8];
@@ -282,12 +304,12 @@ __device__ __forceinline__ void chainBody(
     dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        gMeans[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
+        out.mean(k, R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2]);
     if (nb) {
         const float dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            gMeans[i * 3 + k] += (dDir[k] - dot * viewD[k]) / viewLen;
+            out.mean(k, (dDir[k] - dot * viewD[k]) / viewLen);
     }
 }
 
@@ -304,14 +326,56 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n)
         return;
+    GradArrays out{gMeans, gScales, gQuats, gOpac, gColors, i};
     chainBody(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
-                     gMeans, gScales, gQuats, gOpac, gColors, ndc, gradCount, shRest, shDegree, campos,
-                     gShRest, 1.0f);
+              out, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
     if (clear && radius[i] != 0) {
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
             grad2d[i * kGrad2dStride + k] = 0.0f;
     }
+}
+
+// Chain rule of Gaussian i into registers, then the Adam update of its 14 parameters from them.
+// The registers start at zero and take the same terms in the same order as the zeroed gradient
+// arrays would, so the update equals chainRuleKernel + adamKernel bit for bit.
+__global__ __launch_bounds__(kThreads) void chainAdamKernel(
+    int64_t n, u3d_gaussians prm, u3d_camera cam, float nearPlane, float eps2d,
+    const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
+    const float4 *__restrict__ rec1, float *grad2d, float *ndc, int *gradCount,
+    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest,
+    AdamStepArgs adam)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    GradRegs gr;
+    chainBody(i, i, prm.means, prm.scales, prm.quats, cam, nearPlane, eps2d, radius, flagsArr,
+              rec1, grad2d, gr, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
+    if (radius[i] != 0) {
+#pragma unroll
+        for (int k = 0; k < kGrad2dStride; ++k)
+            grad2d[i * kGrad2dStride + k] = 0.0f;
+    }
+    const float inv = adam.invSqrtBc2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        adamScalar(prm.means[i * 3 + k], adam.m.means[i * 3 + k], adam.v.means[i * 3 + k],
+                   gr.means[k], adam.stepSize[0], inv);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        adamScalar(prm.scales[i * 3 + k], adam.m.scales[i * 3 + k], adam.v.scales[i * 3 + k],
+                   gr.scales[k], adam.stepSize[1], inv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        adamScalar(prm.quats[i * 4 + k], adam.m.quats[i * 4 + k], adam.v.quats[i * 4 + k],
+                   gr.quats[k], adam.stepSize[2], inv);
+    adamScalar(prm.opacities[i], adam.m.opacities[i], adam.v.opacities[i], gr.opac,
+               adam.stepSize[3], inv);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        adamScalar(prm.colors[i * 3 + k], adam.m.colors[i * 3 + k], adam.v.colors[i * 3 + k],
+                   gr.colors[k], adam.stepSize[4], inv);
 }
 
 // Batched chain rule over scene groups: Gaussian i of scene blockIdx.y takes the 2D gradients
@@ -331,10 +395,10 @@ __global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem 
         const int64_t o = it.screenBase + i;
         if (sb.radius[o] == 0)
             continue;
+        GradArrays out{it.gMeans, it.gScales, it.gQuats, it.gOpac, it.gColors, i};
         chainBody(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
-                         sb.flags, sb.rec1, grad2d, it.gMeans, it.gScales, it.gQuats, it.gOpac,
-                         it.gColors, it.ndc, it.gradCount, it.shRest, it.shDegree,
-                         cameraCentre(it.cam), it.gShRest, it.ndcScale);
+                  sb.flags, sb.rec1, grad2d, out, it.ndc, it.gradCount, it.shRest, it.shDegree,
+                  cameraCentre(it.cam), it.gShRest, it.ndcScale);
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
             grad2d[o * kGrad2dStride + k] = 0.0f;
@@ -392,6 +456,22 @@ int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
                        grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
                        cameraCentre(cam), gShRest, clearGrad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchChainAdam(hipStream_t s, const u3d_gaussians &params, const u3d_camera &cam,
+                    const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
+                    float *ndc, int *gradCount, const float *shRest, int shDegree, float *gShRest,
+                    const AdamStepArgs &adam)
+{
+    if (params.n <= 0)
+        return U3D_OK;
+    U3D_CHECK(adam.m.n == params.n && adam.v.n == params.n, U3D_ERR_INVALID,
+              "Adam moments differ in size from the parameters");
+    hipLaunchKernelGGL(chainAdamKernel, dim3(blocksFor(params.n)), dim3(kThreads), 0, s, params.n,
+                       params, cam, st.near_plane, st.eps2d, sb.radius, sb.flags, sb.rec1, grad2d,
+                       ndc, gradCount, shRest, shDegree, cameraCentre(cam), gShRest, adam);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -133,7 +133,6 @@ struct u3d_adam {
 struct u3d_trainer {
     u3d_ctx *ctx = nullptr;
     u3d::DevModel params, spare, grads, adamM, adamV;
-    bool gradsZero = false; // grads holds zeros in all cap entries
     int adamStep = 0;
     std::vector<u3d_camera> cams;
     std::vector<size_t> imageOffset;
