@@ -116,7 +116,7 @@ struct LossItem {
     const uint8_t *target;
     float *grad;       // dL/drgb of the item, scaled by gradScale
     float *result;     // {total, l1, ssim, psnr}
-    double *partials;  // 9 doubles per 32x16 block
+    double *partials;  // 9 doubles per 32x32 block
     int width, height, blocksX, numBlocks;
     float invCount, gradScale;
 };

@@ -1,7 +1,7 @@
 // Fused photometric loss (L1 + lambda * D-SSIM) value and gradient, Loss.cpp:47-185.
 //
 // The reference runs four full-image passes over 11 planar scratch maps. Here one workgroup
-// owns a 32x16 output block and keeps the whole dependency cone in LDS: render/target with a
+// owns a 32x32 output block and keeps the whole dependency cone in LDS: render/target with a
 // 10-pixel halo, the five horizontally blurred moment maps, the three SSIM partial-derivative
 // maps with a 5-pixel halo, and their horizontal blur; the vertical blur then yields the
 // gradient. Only render, target and grad touch HBM (36 B per pixel). Each blur sums its taps in
@@ -15,17 +15,27 @@ namespace u3d {
 namespace {
 
 constexpr int kR = 5;          // 11-tap window radius
-constexpr int kTW = 32, kTH = 16;
+constexpr int kTaps = 2 * kR + 1;
+constexpr int kTW = 32, kTH = 32;
 constexpr int kIW = kTW + 4 * kR, kIH = kTH + 4 * kR; // inputs: 10-pixel halo
 constexpr int kHW = kTW + 2 * kR, kHH = kIH;          // horizontal moment blur
 constexpr int kAW = kTW + 2 * kR, kAH = kTH + 2 * kR; // SSIM partials
 constexpr int kGW = kTW, kGH = kAH;                   // horizontal blur of partials
+// LDS row strides, padded so that lanes walking different rows hit different banks
+constexpr int kInStride = kIW + 1, kHStride = kHW + 1, kAStride = kAW + 1, kGStride = kGW + 1;
+// Outputs per thread along a row (horizontal blurs) or a column (vertical blurs), computed from
+// a sliding window of kSeg + 10 inputs held in registers.
+constexpr int kH1Seg = 11, kV1Seg = 7, kH2Seg = 6, kV2Seg = 4;
 constexpr int kThreads = 256;
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
+static_assert(kIH * ((kHW + kH1Seg - 1) / kH1Seg) <= kThreads, "one H1 task per thread");
+static_assert(kAW * ((kAH + kV1Seg - 1) / kV1Seg) <= kThreads, "one V1 task per thread");
+static_assert(kGH * ((kGW + kH2Seg - 1) / kH2Seg) <= kThreads, "one H2 task per thread");
+static_assert(kTW * ((kTH + kV2Seg - 1) / kV2Seg) <= kThreads, "one output task per thread");
 
 struct Window {
-    float w[2 * kR + 1];
+    float w[kTaps];
 };
 
 __device__ __forceinline__ double blockSumD(double v, double *red)
@@ -44,14 +54,23 @@ __device__ __forceinline__ double blockSumD(double v, double *red)
     return s;
 }
 
-// One 32x16 block of one channel; writes its three double partials at partials[0..2].
+// First output of segment s of kSeg outputs along a line of len (the last segment is moved
+// back to end at len, so it overlaps the one before and recomputes the same values).
+__device__ __forceinline__ int segStart(int s, int seg, int len)
+{
+    return s * seg < len - seg ? s * seg : len - seg;
+}
+
+// One 32x32 block of one channel; writes its three double partials at partials[0..2].
 // gradScale multiplies the stored gradient (kScaled only: views averaged in a batched step).
 //
 // Every blur sums all 11 taps in the reference's order over maps that are exactly zero outside
 // the image, which equals the reference's truncated window bit for bit: an added term w * 0 is
 // +0, and a running sum that starts at +0 never becomes -0 in round-to-nearest, so x + (+0) = x.
 // The inputs are zero outside the image, so H1 of an outside row is +0; the SSIM partials are
-// stored as 0 outside the image, so H2 of an outside row is +0 as well.
+// stored as 0 outside the image, so H2 of an outside row is +0 as well. Each thread computes a
+// run of consecutive outputs from a window of inputs in registers; every output still sums its
+// own 11 terms in tap order from +0.
 template <bool kU8, bool kScaled>
 __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
                                           const float *__restrict__ targetF,
@@ -60,21 +79,21 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
                                           const Window &win, int blocksX, int blockId, int c,
                                           float *__restrict__ grad, double *__restrict__ partials)
 {
-    // LDS (45 KB): the inputs (render, target) are dead once H1 is done, so the SSIM partials
+    // LDS (67 KB): the inputs (render, target) are dead once H1 is done, so the SSIM partials
     // reuse their space; the H1 maps are dead once V1 is done, so H2 reuses theirs.
-    constexpr int kInBytes = kIH * kIW * 8, kABytes = 3 * kAH * kAW * 4;
-    constexpr int kHBytes = 5 * kHH * kHW * 4, kGBytes = 3 * kGH * kGW * 4;
+    constexpr int kInBytes = kIH * kInStride * 8, kABytes = 3 * kAH * kAStride * 4;
+    constexpr int kHBytes = 5 * kHH * kHStride * 4, kGBytes = 3 * kGH * kGStride * 4;
     __shared__ __align__(16) unsigned char ldsA[kInBytes > kABytes ? kInBytes : kABytes];
     __shared__ __align__(16) unsigned char ldsB[kHBytes > kGBytes ? kHBytes : kGBytes];
     __shared__ double red[3][kThreads / 64];
-    auto sIn = reinterpret_cast<float2(*)[kIW]>(ldsA);
-    auto sA = reinterpret_cast<float(*)[kAH][kAW]>(ldsA);
-    auto sH = reinterpret_cast<float(*)[kHH][kHW]>(ldsB);
-    auto sG = reinterpret_cast<float(*)[kGH][kGW]>(ldsB);
+    auto sIn = reinterpret_cast<float2(*)[kInStride]>(ldsA);
+    auto sA = reinterpret_cast<float(*)[kAH][kAStride]>(ldsA);
+    auto sH = reinterpret_cast<float(*)[kHH][kHStride]>(ldsB);
+    auto sG = reinterpret_cast<float(*)[kGH][kGStride]>(ldsB);
     const int tid = threadIdx.x;
-    float w[2 * kR + 1];
+    float w[kTaps];
 #pragma unroll
-    for (int t = 0; t <= 2 * kR; ++t)
+    for (int t = 0; t < kTaps; ++t)
         w[t] = win.w[t];
     const int ox = (blockId % blocksX) * kTW, oy = (blockId / blocksX) * kTH;
     const bool ssimOn = lambda > 0.0f;
@@ -87,6 +106,21 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
         return targetF[idx];
     };
     auto inImage = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
+    // L1 term and the pixel's gradient (Loss.cpp:64-74, 160-177); g = the blurred partials
+    auto finish = [&](int gx, int gy, const float g[3]) {
+        const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
+        const float r = render[idx], t = tgt(idx);
+        const float d = r - t;
+        l1 += double(fabsf(d));
+        const double dd = double(r) - double(t);
+        sq += dd * dd;
+        float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
+        if (ssimOn)
+            gval += g[0] + 2.0f * r * g[1] + t * g[2];
+        if (kScaled)
+            gval *= gradScale;
+        grad[idx] = gval;
+    };
     if (ssimOn) {
         // 1. inputs with a 10-pixel halo (all loads issued before the LDS stores)
         constexpr int kIn = kIH * kIW, kInIter = (kIn + kThreads - 1) / kThreads;
@@ -109,101 +143,137 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
                 sIn[k / kIW][k % kIW] = v[j];
         }
         __syncthreads();
-        // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94)
-        for (int k = tid; k < kHH * kHW; k += kThreads) {
-            const int ly = k / kHW, lx = k - ly * kHW;
-            float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+        // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94): row ly, kH1Seg outputs
+        if (tid < kIH * ((kHW + kH1Seg - 1) / kH1Seg)) {
+            const int ly = tid % kIH, x0 = segStart(tid / kIH, kH1Seg, kHW);
+            float2 in[kH1Seg + 2 * kR];
 #pragma unroll
-            for (int t = 0; t <= 2 * kR; ++t) {
-                const float2 ab = sIn[ly][lx + t];
-                const float a = ab.x, b = ab.y;
-                sx += w[t] * a;
-                sy += w[t] * b;
-                sxx += w[t] * a * a;
-                syy += w[t] * b * b;
-                sxy += w[t] * a * b;
-            }
-            sH[0][ly][lx] = sx;
-            sH[1][ly][lx] = sy;
-            sH[2][ly][lx] = sxx;
-            sH[3][ly][lx] = syy;
-            sH[4][ly][lx] = sxy;
-        }
-        __syncthreads();
-        // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139); 0 outside
-        for (int k = tid; k < kAH * kAW; k += kThreads) {
-            const int ly = k / kAW, lx = k - ly * kAW;
-            const int gx = ox - kR + lx, gy = oy - kR + ly;
-            float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
-            if (inImage(gx, gy)) {
-                float m[5] = {0, 0, 0, 0, 0};
+            for (int i = 0; i < kH1Seg + 2 * kR; ++i)
+                in[i] = sIn[ly][x0 + i];
 #pragma unroll
-                for (int t = 0; t <= 2 * kR; ++t) {
+            for (int o = 0; o < kH1Seg; ++o) {
+                float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
 #pragma unroll
-                    for (int j = 0; j < 5; ++j)
-                        m[j] += w[t] * sH[j][ly + t][lx];
+                for (int t = 0; t < kTaps; ++t) {
+                    const float a = in[o + t].x, b = in[o + t].y;
+                    sx += w[t] * a;
+                    sy += w[t] * b;
+                    sxx += w[t] * a * a;
+                    syy += w[t] * b * b;
+                    sxy += w[t] * a * b;
                 }
-                const float mx = m[0], my = m[1];
-                const float sx = m[2] - mx * mx, sy = m[3] - my * my, sxy = m[4] - mx * my;
-                const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
-                const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
-                const float vv = (a1 * a2) / (b1 * b2);
-                if (lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
-                    ssim += double(vv);
-                const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - vv * (2 * mx / b1 - 2 * mx / b2);
-                p0 = dLds * dmu;
-                p1 = dLds * (-vv / b2);
-                p2 = dLds * (2 * a1 / (b1 * b2));
+                sH[0][ly][x0 + o] = sx;
+                sH[1][ly][x0 + o] = sy;
+                sH[2][ly][x0 + o] = sxx;
+                sH[3][ly][x0 + o] = syy;
+                sH[4][ly][x0 + o] = sxy;
             }
-            // sA aliases the inputs, which every thread finished reading before the barrier
-            sA[0][ly][lx] = p0;
-            sA[1][ly][lx] = p1;
-            sA[2][ly][lx] = p2;
         }
         __syncthreads();
-        // 4. horizontal blur of the partials (Loss.cpp:142-157)
-        for (int k = tid; k < kGH * kGW; k += kThreads) {
-            const int ly = k / kGW, lx = k - ly * kGW;
-            float acc[3] = {0, 0, 0};
+        // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139); 0 outside.
+        //    Column lx, kV1Seg outputs.
+        if (tid < kAW * ((kAH + kV1Seg - 1) / kV1Seg)) {
+            const int lx = tid % kAW, y0 = segStart(tid / kAW, kV1Seg, kAH);
+            float m[5][kV1Seg];
 #pragma unroll
-            for (int t = 0; t <= 2 * kR; ++t) {
+            for (int j = 0; j < 5; ++j) {
+                float col[kV1Seg + 2 * kR];
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    acc[j] += w[t] * sA[j][ly][lx + t];
+                for (int i = 0; i < kV1Seg + 2 * kR; ++i)
+                    col[i] = sH[j][y0 + i][lx];
+#pragma unroll
+                for (int o = 0; o < kV1Seg; ++o) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < kTaps; ++t)
+                        acc += w[t] * col[o + t];
+                    m[j][o] = acc;
+                }
             }
-            // sG aliases the H1 maps, dead since the last barrier
+            // the segment's outputs are read by nobody else before the next barrier, and the
+            // inputs sA overwrites were all read before the last one
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                sG[j][ly][lx] = acc[j];
+            for (int o = 0; o < kV1Seg; ++o) {
+                const int ly = y0 + o;
+                const int gx = ox - kR + lx, gy = oy - kR + ly;
+                float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+                if (inImage(gx, gy)) {
+                    const float mx = m[0][o], my = m[1][o];
+                    const float sx = m[2][o] - mx * mx, sy = m[3][o] - my * my;
+                    const float sxy = m[4][o] - mx * my;
+                    const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
+                    const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
+                    const float vv = (a1 * a2) / (b1 * b2);
+                    // interior pixels once each (the overlapping last segment repeats rows)
+                    const bool counted = (tid / kAW) * kV1Seg <= ly;
+                    if (counted && lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
+                        ssim += double(vv);
+                    const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - vv * (2 * mx / b1 - 2 * mx / b2);
+                    p0 = dLds * dmu;
+                    p1 = dLds * (-vv / b2);
+                    p2 = dLds * (2 * a1 / (b1 * b2));
+                }
+                sA[0][ly][lx] = p0;
+                sA[1][ly][lx] = p1;
+                sA[2][ly][lx] = p2;
+            }
         }
         __syncthreads();
-    }
-    // 5. L1 term, vertical blur of the partials and the chain rule (Loss.cpp:64-74, 160-177)
-    for (int k = tid; k < kTH * kTW; k += kThreads) {
-        const int ly = k / kTW, lx = k - ly * kTW;
-        const int gx = ox + lx, gy = oy + ly;
-        if (gx >= W || gy >= H)
-            continue;
-        const size_t idx = (size_t(gy) * size_t(W) + size_t(gx)) * 3 + size_t(c);
-        const float r = render[idx], t = tgt(idx);
-        const float d = r - t;
-        l1 += double(fabsf(d));
-        const double dd = double(r) - double(t);
-        sq += dd * dd;
-        float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
-        if (ssimOn) {
-            float g[3] = {0, 0, 0};
+        // 4. horizontal blur of the partials (Loss.cpp:142-157): row ly, kH2Seg outputs
+        if (tid < kGH * ((kGW + kH2Seg - 1) / kH2Seg)) {
+            const int ly = tid % kGH, x0 = segStart(tid / kGH, kH2Seg, kGW);
 #pragma unroll
-            for (int tt = 0; tt <= 2 * kR; ++tt) {
+            for (int j = 0; j < 3; ++j) {
+                float row[kH2Seg + 2 * kR];
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    g[j] += w[tt] * sG[j][ly + tt][lx];
+                for (int i = 0; i < kH2Seg + 2 * kR; ++i)
+                    row[i] = sA[j][ly][x0 + i];
+#pragma unroll
+                for (int o = 0; o < kH2Seg; ++o) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < kTaps; ++t)
+                        acc += w[t] * row[o + t];
+                    sG[j][ly][x0 + o] = acc; // sG aliases the H1 maps, dead since the last barrier
+                }
             }
-            gval += g[0] + 2.0f * r * g[1] + t * g[2];
         }
-        if (kScaled)
-            gval *= gradScale;
-        grad[idx] = gval;
+        __syncthreads();
+        // 5. vertical blur of the partials, L1 term and chain rule: column lx, kV2Seg pixels
+        {
+            const int lx = tid % kTW, y0 = segStart(tid / kTW, kV2Seg, kTH);
+            float g[3][kV2Seg];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float col[kV2Seg + 2 * kR];
+#pragma unroll
+                for (int i = 0; i < kV2Seg + 2 * kR; ++i)
+                    col[i] = sG[j][y0 + i][lx];
+#pragma unroll
+                for (int o = 0; o < kV2Seg; ++o) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < kTaps; ++t)
+                        acc += w[t] * col[o + t];
+                    g[j][o] = acc;
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < kV2Seg; ++o) {
+                const int gx = ox + lx, gy = oy + y0 + o;
+                if (gx < W && gy < H) {
+                    const float gp[3] = {g[0][o], g[1][o], g[2][o]};
+                    finish(gx, gy, gp);
+                }
+            }
+        }
+    } else {
+        for (int k = tid; k < kTH * kTW; k += kThreads) {
+            const int gx = ox + k % kTW, gy = oy + k / kTW;
+            const float gp[3] = {0.0f, 0.0f, 0.0f};
+            if (gx < W && gy < H)
+                finish(gx, gy, gp);
+        }
     }
     // block sums of the three doubles (fixed order)
     double v3[3] = {l1, ssim, sq};
