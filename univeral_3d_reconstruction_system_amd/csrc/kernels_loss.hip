@@ -25,8 +25,8 @@ constexpr int kGW = kTW, kGH = kAH;                   // horizontal blur of part
 constexpr int kInStride = kIW + 1, kHStride = kHW + 1, kAStride = kAW + 1, kGStride = kGW + 1;
 // Outputs per thread along a row (horizontal blurs) or a column (vertical blurs), computed from
 // a sliding window of kSeg + 10 inputs held in registers.
-constexpr int kH1Seg = 11, kV1Seg = 7, kH2Seg = 6, kV2Seg = 4;
-constexpr int kThreads = 256;
+constexpr int kH1Seg = 6, kV1Seg = 4, kH2Seg = 3, kV2Seg = 2;
+constexpr int kThreads = 512;
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 static_assert(kIH * ((kHW + kH1Seg - 1) / kH1Seg) <= kThreads, "one H1 task per thread");
