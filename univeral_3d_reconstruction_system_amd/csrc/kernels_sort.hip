@@ -968,6 +968,18 @@ __device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restric
     }
 }
 
+// Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD b % 8). The group a
+// workgroup handles is chosen so that each XCD takes a contiguous range of groups: the short
+// runs that neighbouring groups write next to each other (their counters, their slices of a
+// tile's list) then meet in one L2 and leave it as whole lines.
+__device__ __forceinline__ int xcdGroup(int b, int numGroups)
+{
+    constexpr int kXcds = 8;
+    const int xcd = b % kXcds, k = b / kXcds;
+    const int per = numGroups / kXcds, rem = numGroups % kXcds;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
 // counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
 // groups reads contiguous rows).
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
@@ -982,13 +994,14 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
         D[t] = 0;
     __syncthreads();
     const int slice = group / kBinWaves;
-    addRects(n, sorted, nTiles, rect, int64_t(blockIdx.x) * group + int64_t(wave) * slice, slice, D,
+    const int grp = xcdGroup(int(blockIdx.x), numGroups);
+    addRects(n, sorted, nTiles, rect, int64_t(grp) * group + int64_t(wave) * slice, slice, D,
              stride);
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
     for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
         const int ty = t / tilesX, tx = t - ty * tilesX;
-        counts[size_t(t) * size_t(numGroups) + blockIdx.x] = D[ty * stride + tx];
+        counts[size_t(t) * size_t(numGroups) + grp] = D[ty * stride + tx];
     }
 }
 
@@ -1113,7 +1126,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     const int W = int(blockDim.x) >> 6;
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
     const int slice = group / W;
-    const int64_t first = int64_t(blockIdx.x) * group + int64_t(wave) * slice;
+    const int grp = xcdGroup(int(blockIdx.x), numGroups);
+    const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
     uint32_t *mine = cur + size_t(wave) * size_t(cells);
     for (int t = tid; t < W * cells; t += int(blockDim.x))
         cur[t] = 0;
@@ -1126,7 +1140,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     for (int t = tid; t < tilesX * tilesY; t += int(blockDim.x)) {
         const int ty = t / tilesX, tx = t - ty * tilesX;
         const int cell = ty * stride + tx;
-        uint32_t base = uint32_t(tileOffsets[t]) + counts[size_t(t) * size_t(numGroups) + blockIdx.x];
+        uint32_t base = uint32_t(tileOffsets[t]) + counts[size_t(t) * size_t(numGroups) + grp];
         for (int w = 0; w < W; ++w) {
             const uint32_t c = cur[w * cells + cell];
             cur[w * cells + cell] = base;
