@@ -1067,23 +1067,13 @@ __global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *coun
         tileTotal[blockIdx.x] = total;
 }
 
-// tileOffsets[0, tiles] = exclusive scan of the tile totals (one workgroup).
-// With heavyFirst: the tiles ordered by decreasing list length (64 log-scale classes, any
-// order inside a class), so that a launch that maps workgroup i to tile heavyFirst[i] spreads
-// the longest lists over different compute units.
-__global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_t *tileTotal,
-                                                                         int tiles,
-                                                                         int *tileOffsets,
-                                                                         int *heavyFirst)
+// The tiles ordered by decreasing list length (64 log-scale classes, any order inside a class),
+// so that a launch that maps workgroup i to tile heavyFirst[i] spreads the longest lists over
+// different compute units (the whole workgroup; LDS counters cls[64]).
+__device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ tileTotal, int tiles,
+                                                int *heavyFirst, uint32_t *cls)
 {
-    __shared__ uint32_t carry[kBinScanThreads / 64];
-    __shared__ uint32_t cls[64];
-    const uint32_t all = workgroupScan(tileTotal, tiles, tileOffsets, carry);
-    const int tid = threadIdx.x;
-    if (tid == 0)
-        tileOffsets[tiles] = int(all);
-    if (!heavyFirst)
-        return;
+    const int tid = threadIdx.x, nt = int(blockDim.x);
     auto classOf = [](uint32_t len) {
         const uint32_t v = len + 1u;
         const int lg = 31 - __clz(v);
@@ -1091,10 +1081,10 @@ __global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_
         const int c = lg * 4 + q;
         return 63 - (c < 63 ? c : 63); // heavier first
     };
-    for (int k = tid; k < 64; k += kBinScanThreads)
+    for (int k = tid; k < 64; k += nt)
         cls[k] = 0;
     __syncthreads();
-    for (int t = tid; t < tiles; t += kBinScanThreads)
+    for (int t = tid; t < tiles; t += nt)
         atomicAdd(&cls[classOf(tileTotal[t])], 1u);
     __syncthreads();
     if (tid == 0) {
@@ -1106,7 +1096,7 @@ __global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_
         }
     }
     __syncthreads();
-    for (int t = tid; t < tiles; t += kBinScanThreads)
+    for (int t = tid; t < tiles; t += nt)
         heavyFirst[atomicAdd(&cls[classOf(tileTotal[t])], 1u)] = t;
 }
 
@@ -1114,11 +1104,13 @@ __global__ __launch_bounds__(kBinScanThreads) void tileOffsetsScanKernel(uint32_
 // split the group; each counts its slice per tile in its own difference grid, the counts become
 // per-wavefront cursors (tile offset + earlier groups + earlier slices) in place, then each
 // emits its slice in order, ranking equal tiles inside a 64-pair chunk by ballot matching.
+// Every workgroup scans the tile totals for the tile offsets itself; group 0 also writes them
+// (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
     const int4 *__restrict__ rect, int tilesX, int tilesY, int cellBits, int group,
-    int numGroups, const uint32_t *__restrict__ counts, const int *__restrict__ tileOffsets,
-    uint32_t *tileList)
+    int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
+    int *tileOffsets, int *heavyFirst, uint32_t *tileList)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1137,17 +1129,52 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         __syncthreads();
         prefix2d(cur, W, tilesY + 1, stride);
     }
-    for (int t = tid; t < tilesX * tilesY; t += int(blockDim.x)) {
-        const int ty = t / tilesX, tx = t - ty * tilesX;
-        const int cell = ty * stride + tx;
-        uint32_t base = uint32_t(tileOffsets[t]) + counts[size_t(t) * size_t(numGroups) + grp];
-        for (int w = 0; w < W; ++w) {
-            const uint32_t c = cur[w * cells + cell];
-            cur[w * cells + cell] = base;
-            base += c;
+    // tile offsets chunk by chunk (exclusive scan of the totals), then the cursors of the tiles
+    __shared__ uint32_t waveTot[kBinWaves];
+    __shared__ uint32_t cls[64];
+    const int lane = tid & 63, tiles = tilesX * tilesY;
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < tiles; t0 += int(blockDim.x)) {
+        const int t = t0 + tid;
+        const uint32_t tot = t < tiles ? tileTotal[t] : 0u;
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
         }
+        if (lane == 63)
+            waveTot[wave] = x;
+        __syncthreads();
+        uint32_t before = carry, chunk = 0;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t sw = waveTot[w];
+            before += w < wave ? sw : 0u;
+            chunk += sw;
+        }
+        if (t < tiles) {
+            const uint32_t off = before + x - tot;
+            if (grp == 0)
+                tileOffsets[t] = int(off);
+            const int ty = t / tilesX, tx = t - ty * tilesX;
+            const int cell = ty * stride + tx;
+            uint32_t base = off + counts[size_t(t) * size_t(numGroups) + grp];
+            for (int w = 0; w < W; ++w) {
+                const uint32_t c = cur[w * cells + cell];
+                cur[w * cells + cell] = base;
+                base += c;
+            }
+        }
+        carry += chunk;
+        __syncthreads();
     }
-    __syncthreads();
+    if (grp == 0) {
+        if (tid == 0)
+            tileOffsets[tiles] = int(carry);
+        if (heavyFirst)
+            writeHeavyFirst(tileTotal, tiles, heavyFirst, cls);
+    }
     forEachPair(n, sorted, nTiles, rect, first, slice, L[wave],
                 [&](bool valid, int tx, int ty, uint32_t g) {
                     const int cell = ty * stride + tx;
@@ -1374,9 +1401,6 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
                        tileTotal);
     U3D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tileOffsetsScanKernel, dim3(1), dim3(kBinScanThreads), 0, s, tileTotal,
-                       tiles, tileOffsets, heavyFirst);
-    U3D_LAUNCH_CHECK();
     int cellBits = 1;
     while ((1 << cellBits) < cells)
         ++cellBits;
@@ -1386,7 +1410,8 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
         waves >>= 1;
     hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
                        size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
-                       tilesX, tilesY, cellBits, group, numGroups, counts, tileOffsets, tileList);
+                       tilesX, tilesY, cellBits, group, numGroups, counts, tileTotal, tileOffsets,
+                       heavyFirst, tileList);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
