@@ -205,13 +205,32 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
 
 namespace {
 
+// Buffers and arguments of the segmented backward's work-item scan for the last forward.
+int segmentPrepArgs(u3d_raster *r, SegmentPrep *p)
+{
+    const int tiles = r->geo.tilesX * r->geo.tilesY;
+    U3D_TRY(r->segStart.ensure(size_t(tiles) + 1));
+    U3D_TRY(r->segTotal.ensure(4));
+    U3D_TRY(r->segCounter.ensure(1));
+    // items <= segments of all lists (one per 256 positions and tile) + 1
+    U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
+    p->segCount = r->segCount.p;
+    p->tiles = tiles;
+    p->segStart = r->segStart.p;
+    p->totalItems = r->segTotal.p;
+    p->itemTile = r->itemTile.p;
+    p->counter = r->segCounter.p;
+    return U3D_OK;
+}
+
 // consumeGrad2d: the chain rule zeroes the 2D gradients it reads (the trainer's step), so the
 // next backward needs no clearing pass; otherwise they stay readable (u3d_raster_get_grad2d).
+// prepDone: the work-item scan already ran (with the loss, segmentPrepArgs' buffers).
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
                        const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr,
                        bool consumeGrad2d = false, const LossFinalize *lossFinalize = nullptr,
-                       const AdamStepArgs *adam = nullptr)
+                       const AdamStepArgs *adam = nullptr, bool prepDone = false)
 {
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
@@ -225,19 +244,16 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
         U3D_HIP(hipMemsetAsync(r->grad2d.p, 0, r->grad2d.cap * sizeof(float), s));
     const ScreenBuffers sb = r->screen();
     if (r->segmented) {
-        const int tiles = r->geo.tilesX * r->geo.tilesY;
-        U3D_TRY(r->segStart.ensure(size_t(tiles) + 1));
-        U3D_TRY(r->segTotal.ensure(4));
-        U3D_TRY(r->segCounter.ensure(1));
+        SegmentPrep prep;
+        U3D_TRY(segmentPrepArgs(r, &prep));
         if (!r->occSeg)
             r->occSeg = rasterSegmentedOccupancy();
-        // items <= segments of all lists (one per 256 positions and tile) + 1
-        U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
         StageScope scope(r->ctx, U3D_STAGE_RASTER_BWD); // item scan + compositing kernel
         U3D_TRY(launchRasterSegmented(s, r->geo, r->st, r->segCount.p, r->segStart.p, r->segTotal.p,
                                       r->segCounter.p, r->itemTile.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
-                                      r->nContrib.p, r->ckpt.p, r->grad2d.p, lossFinalize));
+                                      r->nContrib.p, r->ckpt.p, r->grad2d.p, lossFinalize,
+                                      prepDone));
     } else {
         if (lossFinalize)
             U3D_TRY(launchLossFinalize(s, *lossFinalize));
@@ -1036,13 +1052,19 @@ int u3d_trainer_step(u3d_trainer *t)
     t->counters[3] += t->raster.numPairs;
     t->counters[4] += 1;
     float *stats = t->stepStats.p + size_t(t->iteration) * 4;
-    // The loss value's final reduction rides on the backward's first launch.
+    // The loss value's final reduction rides on the backward's first launch, and the
+    // segmented backward's work-item scan on the loss launch.
     LossFinalize lossFinalize;
+    SegmentPrep prep;
+    const bool prepWithLoss = t->raster.segmented;
+    if (prepWithLoss)
+        U3D_TRY(segmentPrepArgs(&t->raster, &prep));
     {
         StageScope scope(t->ctx, U3D_STAGE_LOSS);
         U3D_TRY(launchPhotometricLoss(s, t->render.p, nullptr, t->targets.p + t->imageOffset[view],
                                       cam.width, cam.height, t->opt.ssim_lambda, t->dloss.p,
-                                      t->partials.p, t->partials.cap, stats, &lossFinalize));
+                                      t->partials.p, t->partials.cap, stats, &lossFinalize,
+                                      prepWithLoss ? &prep : nullptr));
     }
     t->grads.n = t->params.n;
     u3d_gaussians gv = t->grads.view();
@@ -1060,7 +1082,8 @@ int u3d_trainer_step(u3d_trainer *t)
     // Backward, then the chain rule and Adam in one pass: the 14 parameter gradients of a
     // Gaussian stay in registers (the gradient buffer is not used by this step).
     U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
-                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize, &adam));
+                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize, &adam,
+                               prepWithLoss));
     if (t->shDegree > 0) {
         StageScope scope(t->ctx, U3D_STAGE_ADAM);
         const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));

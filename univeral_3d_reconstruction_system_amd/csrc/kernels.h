@@ -110,6 +110,58 @@ __device__ __forceinline__ void lossFinalizeBlock(const LossFinalize &f)
     f.result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
 }
 
+// Work-item scan of the segmented backward: segStart = exclusive scan of the per-tile segment
+// counts (tiles + 1 entries), itemTile maps each item to its tile, and totalItems and the
+// claim counter are set for the backward's workgroups.
+struct SegmentPrep {
+    const uint32_t *segCount = nullptr;
+    int tiles = 0;
+    uint32_t *segStart = nullptr, *totalItems = nullptr;
+    int *itemTile = nullptr;
+    unsigned *counter = nullptr;
+};
+
+// The scan on the calling workgroup (a multiple of 64 threads, at most 1024).
+__device__ __forceinline__ void segmentPrepBlock(const SegmentPrep &p)
+{
+    __shared__ uint32_t carry[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nt = int(blockDim.x), nw = nt / 64;
+    uint32_t base = 0;
+    for (int start = 0; start < p.tiles; start += nt) {
+        const int t = start + tid;
+        const uint32_t v = t < p.tiles ? p.segCount[t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        if (lane == 63)
+            carry[wave] = x;
+        __syncthreads();
+        uint32_t offs = base, chunk = 0;
+        for (int w = 0; w < nw; ++w) {
+            offs += w < wave ? carry[w] : 0u;
+            chunk += carry[w];
+        }
+        offs += x - v;
+        if (t < p.tiles) {
+            p.segStart[t] = offs;
+            for (uint32_t k = 0; k < v; ++k)
+                p.itemTile[offs + k] = t;
+        }
+        base += chunk;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.segStart[p.tiles] = base;
+        *p.totalItems = base;
+        *p.counter = 0u;
+    }
+}
+
 // Loss of one item of a batched step (target = uint8 / 255, HWC).
 struct LossItem {
     const float *render;
@@ -286,7 +338,8 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           unsigned *counter, int *itemTile, int numWorkgroups,
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
-                          const float4 *ckpt, float *grad2d, const LossFinalize *fin = nullptr);
+                          const float4 *ckpt, float *grad2d, const LossFinalize *fin = nullptr,
+                          bool prepDone = false);
 int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
                                  const uint32_t *segStart, const uint32_t *totalItems,
                                  unsigned *counter, int numWorkgroups, const int *tileOffsets,
@@ -386,7 +439,7 @@ int launchAdamBf16(hipStream_t s, float *p, const float *g, float *m, float *v, 
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
                           float *grad, double *partials, size_t partialsCap, float *result,
-                          LossFinalize *deferred = nullptr);
+                          LossFinalize *deferred = nullptr, const SegmentPrep *prep = nullptr);
 int launchLossFinalize(hipStream_t s, const LossFinalize &f);
 size_t lossPartialsSize(int width, int height);
 void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks);

@@ -298,6 +298,9 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
     }
 }
 
+// With prep.segCount set, an extra first column of workgroups (blockIdx.x == 0) runs the
+// segmented backward's work-item scan instead (channel 0's; the other two exit), dispatched
+// ahead of the loss blocks so that it never extends the launch.
 template <bool kU8>
 __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__ render,
                                                        const float *__restrict__ targetF,
@@ -305,11 +308,20 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
                                                        int W, int H, float lambda, float invCount,
                                                        Window win, int blocksX,
                                                        float *__restrict__ grad,
-                                                       double *__restrict__ partials)
+                                                       double *__restrict__ partials,
+                                                       SegmentPrep prep)
 {
+    const int skip = prep.segCount ? 1 : 0;
+    if (int(blockIdx.x) < skip) {
+        if (blockIdx.y == 0)
+            segmentPrepBlock(prep);
+        return;
+    }
+    const int block = int(blockIdx.x) - skip;
+    const size_t perChannel = gridDim.x - unsigned(skip); // blocks of one channel
     lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win, blocksX,
-                          int(blockIdx.x), int(blockIdx.y), grad,
-                          partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+                          block, int(blockIdx.y), grad,
+                          partials + (size_t(blockIdx.y) * perChannel + size_t(block)) * 3);
 }
 
 // Batched: item blockIdx.z, channel blockIdx.y; blocks past the item's count exit.
@@ -408,7 +420,7 @@ size_t lossPartialsSize(int width, int height)
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
                           float *grad, double *partials, size_t partialsCap, float *result,
-                          LossFinalize *deferred)
+                          LossFinalize *deferred, const SegmentPrep *prep)
 {
     const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
     const int nb = bx * by; // per channel
@@ -416,13 +428,17 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
     const size_t P = size_t(width) * size_t(height);
     const float invCount = 1.0f / float(P * 3);
     static const Window win = makeWindow();
-    if (nb > 0) {
+    const SegmentPrep side = prep ? *prep : SegmentPrep{};
+    const int cols = nb + (prep ? 1 : 0);
+    if (cols > 0) {
         if (targetU8)
-            hipLaunchKernelGGL(lossKernel<true>, dim3(nb, 3), dim3(kThreads), 0, s, render, target,
-                               targetU8, width, height, lambda, invCount, win, bx, grad, partials);
+            hipLaunchKernelGGL(lossKernel<true>, dim3(cols, 3), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, grad,
+                               partials, side);
         else
-            hipLaunchKernelGGL(lossKernel<false>, dim3(nb, 3), dim3(kThreads), 0, s, render, target,
-                               targetU8, width, height, lambda, invCount, win, bx, grad, partials);
+            hipLaunchKernelGGL(lossKernel<false>, dim3(cols, 3), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, grad,
+                               partials, side);
         U3D_LAUNCH_CHECK();
     }
     LossFinalize f;

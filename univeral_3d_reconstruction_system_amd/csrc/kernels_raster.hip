@@ -735,51 +735,14 @@ __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart,
 // k < segCount[t]) and the work counter cleared.
 constexpr int kPrepThreads = 1024;
 // Workgroup 1 (when launched) finishes the photometric loss of the same step instead.
-__global__ __launch_bounds__(kPrepThreads) void segmentPrepKernel(
-    const uint32_t *__restrict__ segCount, int tiles, uint32_t *__restrict__ segStart,
-    uint32_t *__restrict__ totalItems, int *__restrict__ itemTile, unsigned *counter,
-    LossFinalize fin)
+__global__ __launch_bounds__(kPrepThreads) void segmentPrepKernel(SegmentPrep prep,
+                                                                  LossFinalize fin)
 {
     if (blockIdx.x == 1) {
         lossFinalizeBlock(fin);
         return;
     }
-    __shared__ uint32_t carry[kPrepThreads / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t base = 0;
-    for (int start = 0; start < tiles; start += kPrepThreads) {
-        const int t = start + tid;
-        const uint32_t v = t < tiles ? segCount[t] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off)
-                x += y;
-        }
-        if (lane == 63)
-            carry[wave] = x;
-        __syncthreads();
-        uint32_t offs = base, chunk = 0;
-#pragma unroll
-        for (int w = 0; w < kPrepThreads / 64; ++w) {
-            offs += w < wave ? carry[w] : 0u;
-            chunk += carry[w];
-        }
-        offs += x - v;
-        if (t < tiles) {
-            segStart[t] = offs;
-            for (uint32_t k = 0; k < v; ++k)
-                itemTile[offs + k] = t;
-        }
-        base += chunk;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        segStart[tiles] = base;
-        *totalItems = base;
-        *counter = 0u;
-    }
+    segmentPrepBlock(prep);
 }
 
 // Segmented backward of one image (16x16 tiles): workgroups take (tile, segment) items from
@@ -795,9 +758,11 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
     const float *__restrict__ rec2, float bg0, float bg1, float bg2, float minAlpha,
     const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, const float4 *__restrict__ ckpt,
-    float *__restrict__ grad2d)
+    float *__restrict__ grad2d, LossFinalize fin)
 {
     __shared__ uint32_t sWork;
+    if (fin.partials && blockIdx.x == 0) // the step's loss value, before taking work items
+        lossFinalizeBlock(fin);
     const uint32_t total = *totalItems;
     for (;;) {
         if (threadIdx.x == 0)
@@ -1034,21 +999,33 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           unsigned *counter, int *itemTile, int numWorkgroups,
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
-                          const float4 *ckpt, float *grad2d, const LossFinalize *fin)
+                          const float4 *ckpt, float *grad2d, const LossFinalize *fin,
+                          bool prepDone)
 {
     const int tiles = g.tilesX * g.tilesY;
     if (tiles == 0)
         return fin ? launchLossFinalize(s, *fin) : U3D_OK;
     U3D_CHECK(g.tileSize == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
-    hipLaunchKernelGGL(segmentPrepKernel, dim3(fin ? 2 : 1), dim3(kPrepThreads), 0, s, segCount,
-                       tiles, segStart, totalItems, itemTile, counter,
-                       fin ? *fin : LossFinalize{});
-    U3D_LAUNCH_CHECK();
+    LossFinalize inMain; // the loss finalize rides on whichever launch comes first
+    if (!prepDone) {
+        SegmentPrep prep;
+        prep.segCount = segCount;
+        prep.tiles = tiles;
+        prep.segStart = segStart;
+        prep.totalItems = totalItems;
+        prep.itemTile = itemTile;
+        prep.counter = counter;
+        hipLaunchKernelGGL(segmentPrepKernel, dim3(fin ? 2 : 1), dim3(kPrepThreads), 0, s, prep,
+                           fin ? *fin : LossFinalize{});
+        U3D_LAUNCH_CHECK();
+    } else if (fin) {
+        inMain = *fin;
+    }
     auto kernel = st.min_alpha >= 1e-37f ? rasterSegmentedKernel<true> : rasterSegmentedKernel<false>;
     hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
                        g.height, g.tilesX, itemTile, segStart, totalItems, counter, tileOffsets,
                        tileList, sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
-                       st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d);
+                       st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d, inMain);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
