@@ -833,11 +833,12 @@ constexpr int kBinWaves = 8;           // wavefronts per binning workgroup
 constexpr int kBinScanThreads = 256;
 constexpr int kBinLdsCells = 12288;    // u32 grid cells in LDS (W wavefronts x cells each)
 
+// Per-wavefront record of 64 Gaussians of the emission.
 struct BinLds {
-    uint32_t pre[64];    // exclusive pair prefix inside the 64-Gaussian chunk
+    uint32_t nt[64];     // tiles of the rectangle
     uint32_t origin[64]; // x0 | y0 << 16 of the rectangle
-    uint32_t width[64];  // rectangle width in tiles
-    float rcpWidth[64];  // approximately 1 / width
+    uint32_t size[64];   // width | height << 16 in tiles
+    uint32_t magic[64];  // 2^32 / width rounded up (width >= 2): k / width = mulhi(k, magic)
     uint32_t gauss[64];
 };
 
@@ -895,77 +896,6 @@ __device__ __forceinline__ void prefix2d(uint32_t *D, int grids, int rows, int c
         }
     }
     __syncthreads();
-}
-
-// Calls f(valid, tx, ty, gaussian) for each 64-pair chunk of Gaussians [first, first + count) in
-// depth order (all lanes of the wavefront call f; wavefront-private LDS L).
-template <typename F>
-__device__ __forceinline__ void forEachPair(int64_t n, const uint32_t *__restrict__ sorted,
-                                            const uint32_t *__restrict__ nTiles,
-                                            const int4 *__restrict__ rect, int64_t first,
-                                            int count, BinLds &L, F &&f)
-{
-    const int lane = threadIdx.x & 63;
-    for (int sub = 0; sub < count; sub += 64) {
-        const int64_t i = first + sub + lane;
-        uint32_t g = 0, nt = 0;
-        int4 r = make_int4(0, 0, 1, 1);
-        if (i < n) {
-            g = sorted[i];
-            nt = nTiles[g];
-            if (nt)
-                r = rect[g];
-        }
-        if (__ballot(nt != 0) == 0)
-            break;
-        uint32_t x = nt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off)
-                x += y;
-        }
-        const uint32_t pairs = __shfl(x, 63, 64);
-        const uint32_t width = uint32_t(r.z - r.x);
-        waveSync();
-        L.pre[lane] = x - nt;
-        L.origin[lane] = uint32_t(r.x) | (uint32_t(r.y) << 16);
-        L.width[lane] = width;
-        L.rcpWidth[lane] = __builtin_amdgcn_rcpf(float(width));
-        L.gauss[lane] = g;
-        waveSync();
-        for (uint32_t p0 = 0; p0 < pairs; p0 += 64) {
-            const uint32_t p = p0 + uint32_t(lane);
-            const bool valid = p < pairs;
-            int tx = 0, ty = 0;
-            uint32_t gg = 0;
-            if (valid) {
-                // last j with pre[j] <= p: a Gaussian with pairs (p < pairs)
-                int j = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1)
-                    j += L.pre[j + step] <= p ? step : 0;
-                const uint32_t local = p - L.pre[j];
-                const uint32_t w = L.width[j];
-                // local / w from the reciprocal (local < 2^14, so the estimate is off by at most
-                // one), then corrected on the remainder
-                uint32_t q = uint32_t(float(local) * L.rcpWidth[j]);
-                int rem = int(local) - int(q * w);
-                if (rem < 0) {
-                    --q;
-                    rem += int(w);
-                } else if (rem >= int(w)) {
-                    ++q;
-                    rem -= int(w);
-                }
-                const uint32_t o = L.origin[j];
-                tx = int(o & 0xffffu) + rem;
-                ty = int(o >> 16) + int(q);
-                gg = L.gauss[j];
-            }
-            f(valid, tx, ty, gg);
-        }
-    }
 }
 
 // Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD b % 8). The group a
@@ -1100,15 +1030,93 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
         heavyFirst[atomicAdd(&cls[classOf(tileTotal[t])], 1u)] = t;
 }
 
+// Writes the ids of the Gaussians [first, first + count) (depth order; one wavefront) at their
+// tiles' cursors in mine, advancing them. A step gives each of up to four Gaussians a run of
+// lanes, one lane per tile of its rectangle (16 lanes each when all four have at most 16 tiles,
+// else 32 lanes for two with at most 32, else 64 lanes for one, in as many rounds as it needs).
+// Inside one Gaussian the tiles are distinct, so its lanes take their slots with one LDS atomic
+// add; the Gaussians of a step take theirs in separate rounds in list order, so equal tiles keep
+// the depth order.
+__device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict__ sorted,
+                                          const uint32_t *__restrict__ nTiles,
+                                          const int4 *__restrict__ rect, int64_t first, int count,
+                                          int stride, BinLds &L, uint32_t *mine,
+                                          uint32_t *tileList)
+{
+    const int lane = threadIdx.x & 63;
+    for (int sub = 0; sub < count; sub += 64) {
+        const int64_t i = first + sub + lane;
+        uint32_t g = 0, nt = 0;
+        int4 r = make_int4(0, 0, 1, 1);
+        if (i < n) {
+            g = sorted[i];
+            nt = nTiles[g];
+            if (nt)
+                r = rect[g];
+        }
+        const uint64_t vis = __ballot(nt != 0);
+        if (vis == 0) // invisible Gaussians sort last
+            break;
+        // the visible Gaussians are a prefix of the lanes (depth order puts invisible ones last)
+        const int cnt = __popcll(vis);
+        const uint32_t w = uint32_t(r.z - r.x), h = uint32_t(r.w - r.y);
+        waveSync();
+        L.nt[lane] = nt;
+        L.origin[lane] = uint32_t(r.x) | (uint32_t(r.y) << 16);
+        L.size[lane] = w | (h << 16);
+        // q = mulhi(k, magic) = k / w exactly for k * w < 2^32 (w >= 2); w == 1 is q = k
+        L.magic[lane] = w > 1u ? 0xffffffffu / w + 1u : 0u;
+        L.gauss[lane] = g;
+        waveSync();
+        for (int j = 0; j < cnt;) {
+            const uint32_t n0 = uint32_t(__builtin_amdgcn_readlane(int(nt), j));
+            const uint32_t n1 = j + 1 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 1)) : 0u;
+            const uint32_t n2 = j + 2 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 2)) : 0u;
+            const uint32_t n3 = j + 3 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 3)) : 0u;
+            const uint32_t m4 = max(max(n0, n1), max(n2, n3));
+            const int shift = m4 <= 16u ? 4 : (max(n0, n1) <= 32u ? 5 : 6); // lanes per Gaussian
+            const int slots = 64 >> shift;
+            const int q = lane >> shift, k0 = lane & ((1 << shift) - 1);
+            const int gi = (j + q) & 63;
+            const uint32_t ntg = j + q < cnt ? L.nt[gi] : 0u;
+            const uint32_t org = L.origin[gi], wh = L.size[gi];
+            const uint32_t magic = L.magic[gi], gg = L.gauss[gi];
+            const uint32_t gw = wh & 0xffffu;
+            for (uint32_t c0 = 0;; c0 += 64u) {
+                const uint32_t k = c0 + uint32_t(k0);
+                const bool act = k < ntg;
+                const uint32_t qy = gw > 1u ? __umulhi(k, magic) : k;
+                const uint32_t qx = k - qy * gw;
+                const int cell = int((org >> 16) + qy) * stride + int((org & 0xffffu) + qx);
+                // One LDS atomic per round for every lane (the round's lanes add 1, the others 0;
+                // lanes without a tile add 0 to a word of their own): a wavefront's LDS
+                // instructions execute in order, so a tile shared by two Gaussians of the step
+                // gives the earlier one the lower slot.
+                uint32_t *dst = act ? &mine[cell] : &L.gauss[lane];
+                uint32_t slot = 0;
+                for (int sl = 0; sl < slots; ++sl) {
+                    const uint32_t v = atomicAdd(dst, (act && q == sl) ? 1u : 0u);
+                    slot = q == sl ? v : slot;
+                }
+                if (act)
+                    tileList[slot] = gg;
+                if (shift < 6 || c0 + 64u >= n0)
+                    break;
+            }
+            j += slots;
+        }
+    }
+}
+
 // Writes every Gaussian id at its final slot of its tile's list. W = blockDim.x / 64 wavefronts
 // split the group; each counts its slice per tile in its own difference grid, the counts become
 // per-wavefront cursors (tile offset + earlier groups + earlier slices) in place, then each
-// emits its slice in order, ranking equal tiles inside a 64-pair chunk by ballot matching.
+// emits its slice in order (emitSlice).
 // Every workgroup scans the tile totals for the tile offsets itself; group 0 also writes them
 // (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tilesY, int cellBits, int group,
+    const int4 *__restrict__ rect, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
     int *tileOffsets, int *heavyFirst, uint32_t *tileList)
 {
@@ -1175,19 +1183,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls);
     }
-    forEachPair(n, sorted, nTiles, rect, first, slice, L[wave],
-                [&](bool valid, int tx, int ty, uint32_t g) {
-                    const int cell = ty * stride + tx;
-                    const uint64_t vm = __ballot(valid);
-                    const uint64_t m = matchDigit(uint32_t(cell), vm, cellBits);
-                    if (valid) {
-                        const uint32_t below = uint32_t(__popcll(m & laneMaskLt()));
-                        const uint32_t base = mine[cell];
-                        tileList[base + below] = g;
-                        if (below == 0)
-                            mine[cell] = base + uint32_t(__popcll(m));
-                    }
-                });
+    emitSlice(n, sorted, nTiles, rect, first, slice, stride, L[wave], mine, tileList);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1401,16 +1397,13 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
                        tileTotal);
     U3D_LAUNCH_CHECK();
-    int cellBits = 1;
-    while ((1 << cellBits) < cells)
-        ++cellBits;
     // wavefronts per emission workgroup: as many slices as their grids fit in LDS
     int waves = kBinWaves;
     while (waves > 1 && waves * cells > kBinLdsCells)
         waves >>= 1;
     hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
                        size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
-                       tilesX, tilesY, cellBits, group, numGroups, counts, tileTotal, tileOffsets,
+                       tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
