@@ -159,9 +159,11 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tilesX, tilesY))));
         U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
         U3D_TRY(r->heavyFirst.ensure(size_t(tiles)));
-        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, tilesX, tilesY,
-                                  r->binCounts.p, r->binCounts.cap, r->tileTotal.p,
-                                  r->tileOffsets.p, r->tileListBuf.p, r->heavyFirst.p));
+        U3D_TRY(r->rectSorted.ensure(nn));
+        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, r->ntSorted.p,
+                                  r->rectSorted.p, tilesX, tilesY, r->binCounts.p,
+                                  r->binCounts.cap, r->tileTotal.p, r->tileOffsets.p,
+                                  r->tileListBuf.p, r->heavyFirst.p));
         order = n > 0 ? r->heavyFirst.p : nullptr;
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {

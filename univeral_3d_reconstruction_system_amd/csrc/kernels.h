@@ -282,12 +282,13 @@ size_t scanScratchSize(int64_t count);
 // counts: tileBinningCountsSize entries; tileTotal: tilesX * tilesY entries.
 int tileBinningGroup(int64_t n, int tilesX, int tilesY); // Gaussians per group, 0: unsupported
 size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
+// ntS, rectS (n entries each): scratch for the tile counts and rectangles in depth order.
 // heavyFirst (optional, tiles entries): the tiles by decreasing list length, written unless
 // n <= 0.
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
-                      const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList,
-                      int *heavyFirst = nullptr);
+                      const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+                      uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
+                      uint32_t *tileList, int *heavyFirst = nullptr);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,

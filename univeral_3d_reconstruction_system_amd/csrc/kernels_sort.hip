@@ -843,11 +843,16 @@ struct BinLds {
 };
 
 // Corner updates of the difference grid D (row stride `stride`) for the rectangles of the
-// Gaussians at depth positions [first, first + count) (one wavefront).
+// Gaussians at depth positions [first, first + count) (one wavefront). kGather: the tile counts
+// and rectangles are gathered through sorted (nTiles, rect by Gaussian) and written in depth
+// order to ntS / rectS for every position (rectS only where the count is non-zero); otherwise
+// they are read from ntS / rectS.
+template <bool kGather>
 __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
                                          const uint32_t *__restrict__ nTiles,
-                                         const int4 *__restrict__ rect, int64_t first, int count,
-                                         uint32_t *D, int stride)
+                                         const int4 *__restrict__ rect, uint32_t *ntS,
+                                         int4 *rectS, int64_t first, int count, uint32_t *D,
+                                         int stride)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -855,13 +860,25 @@ __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__
         uint32_t nt = 0;
         int4 r = make_int4(0, 0, 0, 0);
         if (i < n) {
-            const uint32_t g = sorted[i];
-            nt = nTiles[g];
-            if (nt)
-                r = rect[g];
+            if (kGather) {
+                const uint32_t g = sorted[i];
+                nt = nTiles[g];
+                if (nt)
+                    r = rect[g];
+                ntS[i] = nt;
+                if (nt)
+                    rectS[i] = r;
+            } else {
+                nt = ntS[i];
+                if (nt)
+                    r = rectS[i];
+            }
         }
-        if (__ballot(nt != 0) == 0) // invisible Gaussians sort last
+        if (__ballot(nt != 0) == 0) { // invisible Gaussians sort last
+            if (kGather)
+                continue; // every position is still written
             break;
+        }
         if (nt) {
             atomicAdd(&D[r.y * stride + r.x], 1u);
             atomicAdd(&D[r.y * stride + r.z], ~0u); // -1 modulo 2^32
@@ -912,10 +929,11 @@ __device__ __forceinline__ int xcdGroup(int b, int numGroups)
 
 // counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
 // groups reads contiguous rows).
+// Also writes the tile counts and rectangles in depth order (ntS, rectS) for the emission.
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tilesY, int group, int numGroups,
-    uint32_t *counts)
+    const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+    int group, int numGroups, uint32_t *counts)
 {
     extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -925,8 +943,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     __syncthreads();
     const int slice = group / kBinWaves;
     const int grp = xcdGroup(int(blockIdx.x), numGroups);
-    addRects(n, sorted, nTiles, rect, int64_t(grp) * group + int64_t(wave) * slice, slice, D,
-             stride);
+    addRects<true>(n, sorted, nTiles, rect, ntS, rectS, int64_t(grp) * group + int64_t(wave) * slice,
+                   slice, D, stride);
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
     for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
@@ -1038,9 +1056,9 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
 // add; the Gaussians of a step take theirs in separate rounds in list order, so equal tiles keep
 // the depth order.
 __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict__ sorted,
-                                          const uint32_t *__restrict__ nTiles,
-                                          const int4 *__restrict__ rect, int64_t first, int count,
-                                          int stride, BinLds &L, uint32_t *mine,
+                                          const uint32_t *__restrict__ ntS,
+                                          const int4 *__restrict__ rectS, int64_t first,
+                                          int count, int stride, BinLds &L, uint32_t *mine,
                                           uint32_t *tileList)
 {
     const int lane = threadIdx.x & 63;
@@ -1050,9 +1068,9 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
         int4 r = make_int4(0, 0, 1, 1);
         if (i < n) {
             g = sorted[i];
-            nt = nTiles[g];
+            nt = ntS[i];
             if (nt)
-                r = rect[g];
+                r = rectS[i];
         }
         const uint64_t vis = __ballot(nt != 0);
         if (vis == 0) // invisible Gaussians sort last
@@ -1115,8 +1133,8 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
 // Every workgroup scans the tile totals for the tile offsets itself; group 0 also writes them
 // (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
-    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, int tilesX, int tilesY, int group,
+    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
+    const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
     int *tileOffsets, int *heavyFirst, uint32_t *tileList)
 {
@@ -1133,7 +1151,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         cur[t] = 0;
     __syncthreads();
     if (W > 1) {
-        addRects(n, sorted, nTiles, rect, first, slice, mine, stride);
+        addRects<false>(n, sorted, nullptr, nullptr, const_cast<uint32_t *>(ntS),
+                        const_cast<int4 *>(rectS), first, slice, mine, stride);
         __syncthreads();
         prefix2d(cur, W, tilesY + 1, stride);
     }
@@ -1183,7 +1202,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls);
     }
-    emitSlice(n, sorted, nTiles, rect, first, slice, stride, L[wave], mine, tileList);
+    emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1379,8 +1398,9 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
 }
 
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
-                      const int4 *rect, int tilesX, int tilesY, uint32_t *counts, size_t countsCap,
-                      uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList, int *heavyFirst)
+                      const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+                      uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
+                      uint32_t *tileList, int *heavyFirst)
 {
     const int group = tileBinningGroup(n, tilesX, tilesY);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
@@ -1391,8 +1411,8 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
     const int cells = (tilesX + 1) * (tilesY + 1);
     hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(64 * kBinWaves),
-                       size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, tilesX, tilesY,
-                       group, numGroups, counts);
+                       size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
+                       tilesX, tilesY, group, numGroups, counts);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
                        tileTotal);
@@ -1402,7 +1422,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     while (waves > 1 && waves * cells > kBinLdsCells)
         waves >>= 1;
     hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
-                       size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect,
+                       size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList);
     U3D_LAUNCH_CHECK();

@@ -92,7 +92,7 @@ struct u3d_raster {
     u3d::DevBuf<float4> rec0, rec1;
     u3d::DevBuf<float> rec2, depth;
     u3d::DevBuf<int> radius;
-    u3d::DevBuf<int4> rect;
+    u3d::DevBuf<int4> rect, rectSorted; // rectSorted: in depth order (tile binning)
     u3d::DevBuf<uint8_t> flags;
     u3d::DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
     u3d::DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
