@@ -358,24 +358,38 @@ __global__ __launch_bounds__(kThreads) void chainAdamKernel(
             grad2d[i * kGrad2dStride + k] = 0.0f;
     }
     const float inv = adam.invSqrtBc2;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        adamScalar(prm.means[i * 3 + k], adam.m.means[i * 3 + k], adam.v.means[i * 3 + k],
-                   gr.means[k], adam.stepSize[0], inv);
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        adamScalar(prm.scales[i * 3 + k], adam.m.scales[i * 3 + k], adam.v.scales[i * 3 + k],
-                   gr.scales[k], adam.stepSize[1], inv);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        adamScalar(prm.quats[i * 4 + k], adam.m.quats[i * 4 + k], adam.v.quats[i * 4 + k],
-                   gr.quats[k], adam.stepSize[2], inv);
+    // each group's parameter, first and second moment as one 12- or 16-byte access
+    struct F3 {
+        float x, y, z;
+    };
+    auto adam3 = [&](float *p, float *m, float *v, const float *g, float step) {
+        F3 pp = *reinterpret_cast<const F3 *>(p + i * 3);
+        F3 mm = *reinterpret_cast<const F3 *>(m + i * 3);
+        F3 vv = *reinterpret_cast<const F3 *>(v + i * 3);
+        adamScalar(pp.x, mm.x, vv.x, g[0], step, inv);
+        adamScalar(pp.y, mm.y, vv.y, g[1], step, inv);
+        adamScalar(pp.z, mm.z, vv.z, g[2], step, inv);
+        *reinterpret_cast<F3 *>(p + i * 3) = pp;
+        *reinterpret_cast<F3 *>(m + i * 3) = mm;
+        *reinterpret_cast<F3 *>(v + i * 3) = vv;
+    };
+    adam3(prm.means, adam.m.means, adam.v.means, gr.means, adam.stepSize[0]);
+    adam3(prm.scales, adam.m.scales, adam.v.scales, gr.scales, adam.stepSize[1]);
+    {
+        float4 pp = reinterpret_cast<const float4 *>(prm.quats)[i];
+        float4 mm = reinterpret_cast<const float4 *>(adam.m.quats)[i];
+        float4 vv = reinterpret_cast<const float4 *>(adam.v.quats)[i];
+        adamScalar(pp.x, mm.x, vv.x, gr.quats[0], adam.stepSize[2], inv);
+        adamScalar(pp.y, mm.y, vv.y, gr.quats[1], adam.stepSize[2], inv);
+        adamScalar(pp.z, mm.z, vv.z, gr.quats[2], adam.stepSize[2], inv);
+        adamScalar(pp.w, mm.w, vv.w, gr.quats[3], adam.stepSize[2], inv);
+        reinterpret_cast<float4 *>(prm.quats)[i] = pp;
+        reinterpret_cast<float4 *>(adam.m.quats)[i] = mm;
+        reinterpret_cast<float4 *>(adam.v.quats)[i] = vv;
+    }
     adamScalar(prm.opacities[i], adam.m.opacities[i], adam.v.opacities[i], gr.opac,
                adam.stepSize[3], inv);
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        adamScalar(prm.colors[i * 3 + k], adam.m.colors[i * 3 + k], adam.v.colors[i * 3 + k],
-                   gr.colors[k], adam.stepSize[4], inv);
+    adam3(prm.colors, adam.m.colors, adam.v.colors, gr.colors, adam.stepSize[4]);
 }
 
 // Batched chain rule over scene groups: Gaussian i of scene blockIdx.y takes the 2D gradients
