@@ -577,8 +577,12 @@ __global__ __launch_bounds__(kOsThreads) void onesweepPassKernel(
         if (i < blockKeys) {
             const uint32_t key = sKey[i];
             const uint32_t pos = sTot[0][(key >> shift) & mask] + uint32_t(i);
-            keysOut[pos] = key;
-            valsOut[pos] = sVal[i];
+            // always in range when every wait ended normally; after an abandoned wait (error
+            // word set) the order is wrong but no write leaves the arrays
+            if (int64_t(pos) < count) {
+                keysOut[pos] = key;
+                valsOut[pos] = sVal[i];
+            }
         }
     }
 }
