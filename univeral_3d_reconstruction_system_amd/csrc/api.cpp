@@ -461,6 +461,12 @@ int u3d_raster_get_state(u3d_raster *r, u3d_raster_state *out)
     U3D_TRY(launchUnpackScreen(r->ctx->stream, r->n, r->screen(), r->dbgMean2d.p, r->dbgConic.p,
                                r->dbgColor.p, r->dbgOpacity.p));
     U3D_HIP(hipStreamSynchronize(r->ctx->stream));
+    if (r->sortScratch.p) {
+        uint32_t sortError = 0;
+        U3D_TRY(onesweepErrorWord(r->sortScratch.p, r->ctx->stream, &sortError));
+        U3D_CHECK(sortError == 0, U3D_ERR_HIP,
+                  "depth sort: a lookback wait was abandoned (the order is not valid)");
+    }
     out->n = r->n;
     out->num_pairs = r->numPairs;
     out->tiles_x = r->geo.tilesX;

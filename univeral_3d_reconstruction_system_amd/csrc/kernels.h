@@ -256,6 +256,8 @@ struct RadixScratch {
     uint32_t sideSeq = 0;
 };
 size_t onesweepScratchWords();
+// The small sort's error word (non-zero once a lookback wait was abandoned; synchronous read).
+int onesweepErrorWord(const uint64_t *scratch, hipStream_t s, uint32_t *value);
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
                    bool *resultInAlt, const uint32_t *countDev = nullptr);

@@ -1260,6 +1260,14 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
 
 size_t onesweepScratchWords() { return kOsStateWords + kOsLookbackWords + kOsPartialWords; }
 
+int onesweepErrorWord(const uint64_t *scratch, hipStream_t s, uint32_t *value)
+{
+    const uint32_t *state = reinterpret_cast<const uint32_t *>(scratch);
+    U3D_HIP(hipMemcpyAsync(value, state + kOsError, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    U3D_HIP(hipStreamSynchronize(s));
+    return U3D_OK;
+}
+
 namespace {
 int onesweepSort(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                  uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
