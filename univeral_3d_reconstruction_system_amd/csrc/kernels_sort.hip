@@ -919,17 +919,25 @@ __device__ __forceinline__ void prefix2d(uint32_t *D, int grids, int rows, int c
     __syncthreads();
 }
 
-// Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD b % 8). The group a
-// workgroup handles is chosen so that each XCD takes a contiguous range of groups: the short
+// Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD b % 8). Groups are
+// given to workgroups in chunks of kChunk consecutive groups, chunk k to XCD k % 8: the short
 // runs that neighbouring groups write next to each other (their counters, their slices of a
-// tile's list) then meet in one L2 and leave it as whole lines.
-__device__ __forceinline__ int xcdGroup(int b, int numGroups)
+// tile's list) meet in one L2 and leave it as whole lines, while the first chunks (the nearest
+// Gaussians, the largest footprints) start first and spread over all XCDs. The grid has
+// xcdGridSize(numGroups, kChunk) workgroups; those without a group (-1) exit.
+constexpr int kXcds = 8;
+inline int xcdGridSize(int numGroups, int chunk)
 {
-    constexpr int kXcds = 8;
-    const int xcd = b % kXcds, k = b / kXcds;
-    const int per = numGroups / kXcds, rem = numGroups % kXcds;
-    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+    const int chunks = (numGroups + chunk - 1) / chunk;
+    return (chunks + kXcds - 1) / kXcds * kXcds * chunk;
 }
+template <int kChunk> __device__ __forceinline__ int xcdGroup(int b, int numGroups)
+{
+    const int i = b / kXcds;
+    const int grp = ((i / kChunk) * kXcds + b % kXcds) * kChunk + i % kChunk;
+    return grp < numGroups ? grp : -1;
+}
+constexpr int kCountChunk = 32, kEmitChunk = 8;
 
 // counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
 // groups reads contiguous rows).
@@ -946,7 +954,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
         D[t] = 0;
     __syncthreads();
     const int slice = group / kBinWaves;
-    const int grp = xcdGroup(int(blockIdx.x), numGroups);
+    const int grp = xcdGroup<kCountChunk>(int(blockIdx.x), numGroups);
+    if (grp < 0)
+        return;
     addRects<true>(n, sorted, nTiles, rect, ntS, rectS, int64_t(grp) * group + int64_t(wave) * slice,
                    slice, D, stride);
     __syncthreads();
@@ -1148,7 +1158,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     const int W = int(blockDim.x) >> 6;
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
     const int slice = group / W;
-    const int grp = xcdGroup(int(blockIdx.x), numGroups);
+    const int grp = xcdGroup<kEmitChunk>(int(blockIdx.x), numGroups);
+    if (grp < 0)
+        return;
     const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
     uint32_t *mine = cur + size_t(wave) * size_t(cells);
     for (int t = tid; t < W * cells; t += int(blockDim.x))
@@ -1422,7 +1434,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     const int numGroups = blocksFor(n, group);
     U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
     const int cells = (tilesX + 1) * (tilesY + 1);
-    hipLaunchKernelGGL(tileCountKernel, dim3(numGroups), dim3(64 * kBinWaves),
+    hipLaunchKernelGGL(tileCountKernel, dim3(xcdGridSize(numGroups, kCountChunk)), dim3(64 * kBinWaves),
                        size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts);
     U3D_LAUNCH_CHECK();
@@ -1433,7 +1445,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     int waves = kBinWaves;
     while (waves > 1 && waves * cells > kBinLdsCells)
         waves >>= 1;
-    hipLaunchKernelGGL(tileEmitKernel, dim3(numGroups), dim3(64 * waves),
+    hipLaunchKernelGGL(tileEmitKernel, dim3(xcdGridSize(numGroups, kEmitChunk)), dim3(64 * waves),
                        size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList);
