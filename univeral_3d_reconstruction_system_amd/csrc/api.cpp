@@ -195,7 +195,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     if (r->segmented) {
         const size_t slots = size_t(K) / 256 + size_t(tiles) + 1;
         U3D_TRY(r->ckpt.ensure(slots * 256));
-        U3D_TRY(r->segCount.ensure(size_t(tiles) + 1));
+        U3D_TRY(r->segCount.ensure(size_t(tiles) * kSegCountsPerTile + 1));
         seg = SegmentState{r->ckpt.p, r->segCount.p};
     }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
@@ -217,6 +217,7 @@ int segmentPrepArgs(u3d_raster *r, SegmentPrep *p)
     // items <= segments of all lists (one per 256 positions and tile) + 1
     U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
     p->segCount = r->segCount.p;
+    p->perTile = kSegCountsPerTile;
     p->tiles = tiles;
     p->segStart = r->segStart.p;
     p->totalItems = r->segTotal.p;

@@ -115,6 +115,7 @@ __device__ __forceinline__ void lossFinalizeBlock(const LossFinalize &f)
 // claim counter are set for the backward's workgroups.
 struct SegmentPrep {
     const uint32_t *segCount = nullptr;
+    int perTile = 1; // segment counts per tile (their maximum is the tile's)
     int tiles = 0;
     uint32_t *segStart = nullptr, *totalItems = nullptr;
     int *itemTile = nullptr;
@@ -130,7 +131,10 @@ __device__ __forceinline__ void segmentPrepBlock(const SegmentPrep &p)
     uint32_t base = 0;
     for (int start = 0; start < p.tiles; start += nt) {
         const int t = start + tid;
-        const uint32_t v = t < p.tiles ? p.segCount[t] : 0u;
+        uint32_t v = 0;
+        if (t < p.tiles)
+            for (int q = 0; q < p.perTile; ++q)
+                v = max(v, p.segCount[size_t(t) * size_t(p.perTile) + size_t(q)]);
         uint32_t x = v;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -327,6 +331,8 @@ struct SegmentState {
     float4 *ckpt;
     uint32_t *segCount;
 };
+// The single-image forward (16-pixel tiles) writes one segment count per tile quadrant.
+constexpr int kSegCountsPerTile = 4;
 // blockOrder (optional, 16-pixel tiles only): workgroup i renders tile blockOrder[i].
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,

@@ -420,6 +420,247 @@ __device__ __forceinline__ void forwardBlock(
     }
 }
 
+// A uniform 64-bit value read from LDS, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Forward of one 8x8 quadrant of a 16x16 tile by a whole 256-thread workgroup. A wavefront
+// walking a long list alone issues ~90 instructions per Gaussian, most of them the pixel-local
+// part (power, exact exp, alpha, the skip tests), which does not depend on the walk's state;
+// the sequential update is ~20. Here wavefront 0 only composites; wavefronts 1-3 compute the
+// pixel-local parts of the next step's entries into LDS, a third each, and take turns staging
+// the records of each step of kQStep list entries. Between two barriers: stage step s + 2,
+// prepare step s + 1, composite step s. The values and the update are the same expressions in
+// the same order as forwardBlock's, so every output (colour, final transmittance, contributor
+// count, checkpoints) is identical to it.
+//
+// Staging keeps only the entries that pass the quadrant cull, packed in list order (rank k):
+// pos[k] = mean and conic (x, y, A, B), shape[k] = (C, opacity, skip threshold, -),
+// col[k] = (colour, list position + 1 as int bits). An entry that is not a candidate for a
+// pixel (power above 0 or below the threshold, alpha below min_alpha) is passed as alpha = NaN:
+// a candidate's alpha is never NaN, since a NaN product fails `raw < 0.99` and becomes 0.99.
+// The ranks are padded with NaN up to a multiple of kCGroup, the compositor's group.
+constexpr int kQStep = 64;
+struct QuadLds {
+    float4 pos[2][kQStep];   // by step parity: read while preparing, written while staging
+    float4 shape[2][kQStep];
+    float4 col[3][kQStep];   // by step % 3: also read while compositing
+    float alpha[2][kQStep][64];
+    int count[3];
+    uint64_t tab[32];
+    int stop[2]; // by step parity: a flag is rewritten only after every wave has read it
+};
+
+template <bool kCkpt, bool kFastExp>
+__device__ __forceinline__ void forwardQuadrant(
+    int bx, int by, int quad, int W, int H, int tilesX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib, const SegmentState &seg)
+{
+    constexpr int kBatch = 256; // checkpoint segment, list entries
+    static_assert(kBatch % kQStep == 0, "segments are whole steps");
+    constexpr int kGroup = 4, kCGroup = 8, kPrep = 3;
+    __shared__ QuadLds L;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int qx0 = bx * kEdge + (quad & 1) * 8, qy0 = by * kEdge + (quad >> 1) * 8;
+    const int px = qx0 + (lane & 7), py = qy0 + (lane >> 3);
+    const int tile = by * tilesX + bx;
+    const bool inside = px < W && py < H;
+    const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
+    const int steps = (end - begin + kQStep - 1) / kQStep;
+    const float fpx = float(px), fpy = float(py);
+    if (tid < 32)
+        L.tab[tid] = kExp2Table[tid];
+    if (wave == 0)
+        __builtin_amdgcn_s_setprio(3); // the sequential walk is the critical path
+
+    // Staging: wavefront w (1-3) stages the steps s with s % 3 == w - 1; the records of its
+    // next step and the ids of the one after are in registers (loaded kPrep steps ahead).
+    int gCur = -1, gNext = -1;
+    float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+    float rc = 0.0f;
+    auto loadId = [&](int st) {
+        const int e = begin + st * kQStep + lane;
+        return e < end ? tileList[e] : -1;
+    };
+    auto loadRec = [&]() {
+        if (gCur >= 0) {
+            ra = rec0[gCur];
+            rb = rec1[gCur];
+            rc = rec2[gCur];
+        }
+    };
+    auto stage = [&](int st) {
+        const int b = st % 3;
+        bool hit = false;
+        float thr = 0.0f;
+        if (gCur >= 0) {
+            thr = skipThreshold(minAlpha, rb.y);
+            hit = quadrantHit(ra, rb.x, thr, qx0, qy0);
+        }
+        const uint64_t bits = __ballot(hit);
+        if (hit) {
+            const int k = int(__builtin_amdgcn_mbcnt_hi(uint32_t(bits >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(uint32_t(bits), 0u)));
+            L.pos[st & 1][k] = ra;
+            L.shape[st & 1][k] = make_float4(rb.x, rb.y, thr, 0.0f);
+            L.col[b][k] = make_float4(rb.z, rb.w, rc, __int_as_float(st * kQStep + lane + 1));
+        }
+        if (lane == 0)
+            L.count[b] = __popcll(bits);
+        gCur = gNext;
+        loadRec();
+        gNext = loadId(st + 2 * kPrep);
+    };
+    auto stages = [&](int st) { return st % 3 == wave - 1; };
+
+    // Wavefronts 1-3: alpha (NaN when not a candidate) of the ranks k = wave - 1 (mod 3) of
+    // step st, padded to a multiple of kCGroup.
+    auto prepareStep = [&](int st) {
+        const int b = st % 3, a = st & 1;
+        const int n = __builtin_amdgcn_readfirstlane(L.count[b]);
+        const int nPad = (n + kCGroup - 1) / kCGroup * kCGroup;
+        for (int k0 = wave - 1; k0 < nPad; k0 += kPrep * kGroup) {
+            float al[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int k = k0 + kPrep * g;
+                const bool valid = k < n; // wave-uniform
+                const int kl = valid ? k : 0;
+                const float4 ga = L.pos[a][kl], gs = L.shape[a][kl];
+                const float dx = ga.x - fpx, dy = ga.y - fpy;
+                const float power = -0.5f * (ga.z * dx * dx + gs.x * dy * dy) - ga.w * dx * dy;
+                const float raw = gs.y * expfExactLdsIf(power, L.tab, kFastExp);
+                const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
+                const bool cand = valid && !(power > 0.0f) && !(power < gs.z) && !(alpha < minAlpha);
+                al[g] = cand ? alpha : __builtin_nanf("");
+            }
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                if (k0 + kPrep * g < nPad)
+                    L.alpha[a][k0 + kPrep * g][lane] = al[g];
+        }
+    };
+
+    // Wavefront 0's walk state.
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f; // colour of the current segment (checkpoints only)
+    int last = 0, batches = 0;             // batches: checkpoint segments started
+    bool done = !inside;
+    const size_t slot0 = size_t(begin / kBatch + tile);
+    const int ckTid = quad * 64 + lane; // the pixel's index in the tile's checkpoint rows
+    auto compositeStep = [&](int st) {
+        const int b = st % 3, a = st & 1;
+        const int n = __builtin_amdgcn_readfirstlane(L.count[b]);
+        for (int k0 = 0; k0 < n; k0 += kCGroup) {
+            float al[kCGroup];
+            float4 cv[kCGroup];
+#pragma unroll
+            for (int g = 0; g < kCGroup; ++g) {
+                al[g] = L.alpha[a][k0 + g][lane];
+                cv[g] = L.col[b][k0 + g];
+            }
+#pragma unroll
+            for (int g = 0; g < kCGroup; ++g) {
+                const bool contrib = !__builtin_isnan(al[g]) && !done;
+                const float nextT = T * (1.0f - al[g]);
+                const bool stop = contrib && nextT < minT;
+                const bool take = contrib && !stop;
+                const float w = al[g] * T;
+                const float wr = cv[g].x * w, wg = cv[g].y * w, wb = cv[g].z * w;
+                cr = take ? cr + wr : cr;
+                cg = take ? cg + wg : cg;
+                cb = take ? cb + wb : cb;
+                if (kCkpt) {
+                    sr = take ? sr + wr : sr;
+                    sg = take ? sg + wg : sg;
+                    sb = take ? sb + wb : sb;
+                }
+                T = take ? nextT : T;
+                last = take ? __float_as_int(cv[g].w) : last;
+                done = done || stop;
+            }
+        }
+    };
+
+    if (wave > 0) {
+        gCur = loadId(wave - 1);
+        gNext = loadId(wave - 1 + kPrep);
+        loadRec();
+        if (wave <= 2)
+            stage(wave - 1);
+    }
+    __syncthreads();
+    if (wave > 0 && steps > 0)
+        prepareStep(0);
+    __syncthreads();
+    for (int st = 0; st < steps; ++st) {
+        if (wave == 0) {
+            const bool allDone = __ballot(!done) == 0;
+            if (lane == 0)
+                L.stop[st & 1] = allDone ? 1 : 0;
+            if (!allDone) {
+                if ((st * kQStep) % kBatch == 0) {
+                    if (kCkpt && st > 0) { // T before this segment and the previous one's colour
+                        if (inside)
+                            seg.ckpt[(slot0 + size_t(batches)) * 256 + ckTid] = make_float4(T, sr, sg, sb);
+                        sr = sg = sb = 0.0f;
+                    }
+                    ++batches;
+                }
+                compositeStep(st);
+            }
+        } else {
+            if (stages(st + 2))
+                stage(st + 2);
+            if (st + 1 < steps)
+                prepareStep(st + 1);
+        }
+        __syncthreads();
+        if (L.stop[st & 1])
+            break;
+    }
+    if (wave != 0)
+        return;
+    if (inside) {
+        const size_t pix = size_t(py) * size_t(W) + size_t(px);
+        rgb[pix * 3] = cr + T * bg0;
+        rgb[pix * 3 + 1] = cg + T * bg1;
+        rgb[pix * 3 + 2] = cb + T * bg2;
+        finalT[pix] = T;
+        nContrib[pix] = last;
+        if (kCkpt) {
+            // Reverse pass over this pixel's checkpoints: each segment colour becomes the colour
+            // of everything from that boundary on (as forwardBlock).
+            float ar = sr, ag = sg, ab = sb;
+            for (int k = batches - 1; k >= 1; --k) {
+                float4 *c = seg.ckpt + (slot0 + size_t(k)) * 256 + ckTid;
+                const float4 v = *c;
+                *c = make_float4(v.x, ar, ag, ab);
+                ar += v.y;
+                ag += v.z;
+                ab += v.w;
+            }
+        }
+    }
+    if (kCkpt) {
+        // this quadrant's segments: ceil(max contributor count / kBatch); the tile's count is
+        // the largest of its four quadrants'
+        int m = last;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 0)
+            seg.segCount[size_t(tile) * kSegCountsPerTile + quad] = uint32_t((m + kBatch - 1) / kBatch);
+    }
+}
+
 // Reverse walk over one segment [lo, lo + kBatch) of a 16x16 tile's list. A pixel whose last
 // contributor lies in the segment starts from its final state, as backwardBlock; a pixel that
 // continues past the segment starts from the forward's checkpoint at the segment's end:
@@ -702,6 +943,23 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
     return lo;
 }
 
+// 16-pixel tiles: four workgroups per tile (one per quadrant, quadrant = blockIdx.x % 4), the
+// tiles in blockOrder's order when given.
+template <bool kCkpt, bool kFastExp>
+__global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
+    int W, int H, int tilesX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder)
+{
+    const int b = int(blockIdx.x >> 2);
+    const int blk = blockOrder ? blockOrder[b] : b;
+    forwardQuadrant<kCkpt, kFastExp>(blk % tilesX, blk / tilesX, int(blockIdx.x & 3), W, H, tilesX,
+                                     tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2,
+                                     minAlpha, minT, rgb, finalT, nContrib, seg);
+}
+
 template <bool kCkpt, bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
@@ -966,6 +1224,16 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
     if (blocksX * blocksY == 0)
         return U3D_OK;
     const bool fast = st.min_alpha >= 1e-37f; // as noSpecialExp
+    if (g.tileSize == kEdge) {
+        auto quad = seg.ckpt ? (fast ? rasterForwardQuadKernel<true, true> : rasterForwardQuadKernel<true, false>)
+                             : (fast ? rasterForwardQuadKernel<false, true> : rasterForwardQuadKernel<false, false>);
+        hipLaunchKernelGGL(quad, dim3(4 * blocksX * blocksY), dim3(kThreads), 0, s, g.width, g.height,
+                           g.tilesX, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2,
+                           st.background[0], st.background[1], st.background[2], st.min_alpha,
+                           st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder);
+        U3D_LAUNCH_CHECK();
+        return U3D_OK;
+    }
     auto kernel = seg.ckpt ? (fast ? rasterForwardKernel<true, true> : rasterForwardKernel<true, false>)
                            : (fast ? rasterForwardKernel<false, true> : rasterForwardKernel<false, false>);
     hipLaunchKernelGGL(kernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s, g.width, g.height,
@@ -1010,6 +1278,7 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
     if (!prepDone) {
         SegmentPrep prep;
         prep.segCount = segCount;
+        prep.perTile = kSegCountsPerTile;
         prep.tiles = tiles;
         prep.segStart = segStart;
         prep.totalItems = totalItems;
