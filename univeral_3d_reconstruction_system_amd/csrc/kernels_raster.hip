@@ -221,6 +221,20 @@ __device__ __forceinline__ bool quadrantHit(float4 a, float C, float thr, int qx
     return !(powerMax < thr); // keep unless certainly below the threshold (NaN keeps)
 }
 
+// Whether one staged Gaussian may contribute to a pixel of the box with pixel centres
+// [x0, x1] x [y0, y1]: false only when the power bound over the box is certainly below the
+// threshold (quadrantHit for any box).
+__device__ __forceinline__ bool boxHit(float4 a, float C, float thr, int x0, int x1, int y0, int y1)
+{
+    if (!(thr > -INFINITY) || !(a.z > 0.0f && C > 0.0f && a.z * C - a.w * a.w > 0.0f))
+        return true;
+    float scale;
+    const float qmin = boxMinQuad(a.z, a.w, C, a.x - float(x1), a.x - float(x0), a.y - float(y1),
+                                  a.y - float(y0), &scale);
+    const float powerMax = -0.5f * qmin + 1e-6f * scale + 1e-6f;
+    return !(powerMax < thr);
+}
+
 // Quadrant mask of one staged Gaussian for the block at (bx0, by0): bit w set when the
 // Gaussian may contribute to a pixel of quadrant w (8x8, w = qy * 2 + qx).
 __device__ __forceinline__ uint32_t quadrantMask(float4 a, float C, float thr, int bx0, int by0)
@@ -444,6 +458,11 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
 // pixel (power above 0 or below the threshold, alpha below min_alpha) is passed as alpha = NaN:
 // a candidate's alpha is never NaN, since a NaN product fails `raw < 0.99` and becomes 0.99.
 // The ranks are padded with NaN up to a multiple of kCGroup, the compositor's group.
+//
+// A pixel that is done (saturated) takes no further contribution, so the cull of a step only
+// needs the box of the pixels still taking contributions: after each step the compositor
+// publishes that box, and the steps staged later are culled against it (pixels only ever
+// leave the set, so the box is never too small).
 constexpr int kQStep = 64;
 struct QuadLds {
     float4 pos[2][kQStep];   // by step parity: read while preparing, written while staging
@@ -452,7 +471,8 @@ struct QuadLds {
     float alpha[2][kQStep][64];
     int count[3];
     uint64_t tab[32];
-    int stop[2]; // by step parity: a flag is rewritten only after every wave has read it
+    int stop[2];  // by step parity: a flag is rewritten only after every wave has read it
+    int alive[2]; // by step parity: box of the pixels still taking contributions (4 x 4 bits)
 };
 
 template <bool kCkpt, bool kFastExp>
@@ -466,6 +486,7 @@ __device__ __forceinline__ void forwardQuadrant(
     constexpr int kBatch = 256; // checkpoint segment, list entries
     static_assert(kBatch % kQStep == 0, "segments are whole steps");
     constexpr int kGroup = 4, kCGroup = 8, kPrep = 3;
+    constexpr int kFullBox = 0x7070; // x in [0, 7], y in [0, 7]
     __shared__ QuadLds L;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int qx0 = bx * kEdge + (quad & 1) * 8, qy0 = by * kEdge + (quad >> 1) * 8;
@@ -496,13 +517,14 @@ __device__ __forceinline__ void forwardQuadrant(
             rc = rec2[gCur];
         }
     };
-    auto stage = [&](int st) {
+    auto stage = [&](int st, int box) {
         const int b = st % 3;
         bool hit = false;
         float thr = 0.0f;
         if (gCur >= 0) {
             thr = skipThreshold(minAlpha, rb.y);
-            hit = quadrantHit(ra, rb.x, thr, qx0, qy0);
+            hit = boxHit(ra, rb.x, thr, qx0 + (box & 15), qx0 + ((box >> 4) & 15),
+                         qy0 + ((box >> 8) & 15), qy0 + ((box >> 12) & 15));
         }
         const uint64_t bits = __ballot(hit);
         if (hit) {
@@ -538,7 +560,9 @@ __device__ __forceinline__ void forwardQuadrant(
                 const float power = -0.5f * (ga.z * dx * dx + gs.x * dy * dy) - ga.w * dx * dy;
                 const float raw = gs.y * expfExactLdsIf(power, L.tab, kFastExp);
                 const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
-                const bool cand = valid && !(power > 0.0f) && !(power < gs.z) && !(alpha < minAlpha);
+                // bitwise: a short-circuit test lets the compiler branch around each exp,
+                // which serialises the group's dependency chains
+                const bool cand = valid & !(power > 0.0f) & !(power < gs.z) & !(alpha < minAlpha);
                 al[g] = cand ? alpha : __builtin_nanf("");
             }
 #pragma unroll
@@ -552,7 +576,9 @@ __device__ __forceinline__ void forwardQuadrant(
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f; // colour of the current segment (checkpoints only)
     int last = 0, batches = 0;             // batches: checkpoint segments started
-    bool done = !inside;
+    // +0 while the pixel takes contributions, NaN once it is done (saturated or outside the
+    // image): alpha + doneNaN is NaN exactly when the entry is no contribution.
+    float doneNaN = inside ? 0.0f : __builtin_nanf("");
     const size_t slot0 = size_t(begin / kBatch + tile);
     const int ckTid = quad * 64 + lane; // the pixel's index in the tile's checkpoint rows
     auto compositeStep = [&](int st) {
@@ -566,25 +592,30 @@ __device__ __forceinline__ void forwardQuadrant(
                 al[g] = L.alpha[a][k0 + g][lane];
                 cv[g] = L.col[b][k0 + g];
             }
+            // Bitwise logic on the flags (no short-circuit branches). A skipped term is added
+            // as +0, which leaves every sum unchanged: the sums start at +0 and cannot become
+            // -0, and x + 0 == x for every other x.
 #pragma unroll
             for (int g = 0; g < kCGroup; ++g) {
-                const bool contrib = !__builtin_isnan(al[g]) && !done;
+                const bool contrib = !__builtin_isnan(al[g] + doneNaN);
                 const float nextT = T * (1.0f - al[g]);
-                const bool stop = contrib && nextT < minT;
-                const bool take = contrib && !stop;
+                const bool stop = contrib & (nextT < minT);
+                const bool take = contrib & !stop;
                 const float w = al[g] * T;
-                const float wr = cv[g].x * w, wg = cv[g].y * w, wb = cv[g].z * w;
-                cr = take ? cr + wr : cr;
-                cg = take ? cg + wg : cg;
-                cb = take ? cb + wb : cb;
+                const float wr = take ? cv[g].x * w : 0.0f;
+                const float wg = take ? cv[g].y * w : 0.0f;
+                const float wb = take ? cv[g].z * w : 0.0f;
+                cr += wr;
+                cg += wg;
+                cb += wb;
                 if (kCkpt) {
-                    sr = take ? sr + wr : sr;
-                    sg = take ? sg + wg : sg;
-                    sb = take ? sb + wb : sb;
+                    sr += wr;
+                    sg += wg;
+                    sb += wb;
                 }
                 T = take ? nextT : T;
                 last = take ? __float_as_int(cv[g].w) : last;
-                done = done || stop;
+                doneNaN = stop ? __builtin_nanf("") : doneNaN;
             }
         }
     };
@@ -594,15 +625,17 @@ __device__ __forceinline__ void forwardQuadrant(
         gNext = loadId(wave - 1 + kPrep);
         loadRec();
         if (wave <= 2)
-            stage(wave - 1);
+            stage(wave - 1, kFullBox);
     }
+    if (tid == 0)
+        L.alive[1] = kFullBox; // the box the stagers of the first step read
     __syncthreads();
     if (wave > 0 && steps > 0)
         prepareStep(0);
     __syncthreads();
     for (int st = 0; st < steps; ++st) {
         if (wave == 0) {
-            const bool allDone = __ballot(!done) == 0;
+            const bool allDone = __ballot(!__builtin_isnan(doneNaN)) == 0;
             if (lane == 0)
                 L.stop[st & 1] = allDone ? 1 : 0;
             if (!allDone) {
@@ -615,10 +648,25 @@ __device__ __forceinline__ void forwardQuadrant(
                     ++batches;
                 }
                 compositeStep(st);
+                // box of the pixels still taking contributions (rows and columns of the mask)
+                const uint64_t live = __ballot(!__builtin_isnan(doneNaN));
+                uint32_t cols = uint32_t(live | (live >> 32));
+                cols |= cols >> 16;
+                cols |= cols >> 8;
+                cols &= 0xffu;
+                uint32_t rows = 0;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    rows |= ((live >> (8 * r)) & 0xffu) ? (1u << r) : 0u;
+                const int box = live ? (__builtin_ctz(cols) | ((31 - __builtin_clz(cols)) << 4) |
+                                        (__builtin_ctz(rows) << 8) | ((31 - __builtin_clz(rows)) << 12))
+                                     : kFullBox;
+                if (lane == 0)
+                    L.alive[st & 1] = box;
             }
         } else {
             if (stages(st + 2))
-                stage(st + 2);
+                stage(st + 2, __builtin_amdgcn_readfirstlane(L.alive[(st + 1) & 1]));
             if (st + 1 < steps)
                 prepareStep(st + 1);
         }
