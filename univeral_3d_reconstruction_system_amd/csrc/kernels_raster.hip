@@ -445,12 +445,12 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
 // Forward of one 8x8 quadrant of a 16x16 tile by a whole 256-thread workgroup. A wavefront
 // walking a long list alone issues ~90 instructions per Gaussian, most of them the pixel-local
 // part (power, exact exp, alpha, the skip tests), which does not depend on the walk's state;
-// the sequential update is ~20. Here wavefront 0 only composites; wavefronts 1-3 compute the
-// pixel-local parts of the next step's entries into LDS, a third each, and take turns staging
-// the records of each step of kQStep list entries. Between two barriers: stage step s + 2,
-// prepare step s + 1, composite step s. The values and the update are the same expressions in
-// the same order as forwardBlock's, so every output (colour, final transmittance, contributor
-// count, checkpoints) is identical to it.
+// the sequential update is ~20. Here wavefront 0 only composites; wavefronts 1-3 take turns
+// staging the records of each step of kQStep list entries, and the two not staging compute the
+// pixel-local parts of the next step's entries into LDS, half each. Between two barriers: stage
+// step s + 2, prepare step s + 1, composite step s. The values and the update are the same
+// expressions in the same order as forwardBlock's, so every output (colour, final
+// transmittance, contributor count, checkpoints) is identical to it.
 //
 // Staging keeps only the entries that pass the quadrant cull, packed in list order (rank k):
 // pos[k] = mean and conic (x, y, A, B), shape[k] = (C, opacity, skip threshold, -),
@@ -542,17 +542,22 @@ __device__ __forceinline__ void forwardQuadrant(
     };
     auto stages = [&](int st) { return st % 3 == wave - 1; };
 
-    // Wavefronts 1-3: alpha (NaN when not a candidate) of the ranks k = wave - 1 (mod 3) of
-    // step st, padded to a multiple of kCGroup.
+    // Wavefronts 1-3 except the one staging alongside (wavefront 1 + (st + 1) % 3, which
+    // stages step st + 1 while step st is prepared): alpha (NaN when not a candidate) of the
+    // ranks k = idx (mod 2) of step st, padded to a multiple of kCGroup; idx numbers the two.
     auto prepareStep = [&](int st) {
         const int b = st % 3, a = st & 1;
+        const int stager = 1 + (st + 1) % 3;
+        if (wave == stager)
+            return;
+        const int idx = wave < stager ? wave - 1 : wave - 2;
         const int n = __builtin_amdgcn_readfirstlane(L.count[b]);
         const int nPad = (n + kCGroup - 1) / kCGroup * kCGroup;
-        for (int k0 = wave - 1; k0 < nPad; k0 += kPrep * kGroup) {
+        for (int k0 = idx; k0 < nPad; k0 += 2 * kGroup) {
             float al[kGroup];
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
-                const int k = k0 + kPrep * g;
+                const int k = k0 + 2 * g;
                 const bool valid = k < n; // wave-uniform
                 const int kl = valid ? k : 0;
                 const float4 ga = L.pos[a][kl], gs = L.shape[a][kl];
@@ -567,8 +572,8 @@ __device__ __forceinline__ void forwardQuadrant(
             }
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
-                if (k0 + kPrep * g < nPad)
-                    L.alpha[a][k0 + kPrep * g][lane] = al[g];
+                if (k0 + 2 * g < nPad)
+                    L.alpha[a][k0 + 2 * g][lane] = al[g];
         }
     };
 
