@@ -117,17 +117,24 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
 {
     const float dx = a.x - fpx, dy = a.y - fpy;
     const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-    const bool inRange = active && !(power > 0.0f) && !(power < thr);
+    // bitwise tests: short-circuit ones become branches whose merges copy the walk state
+    const bool inRange = active & !(power > 0.0f) & !(power < thr);
     if (__ballot(inRange) == 0)
         return;
     const float G = expfExactLdsIf(power, stab, kFastExp);
     const float op = b.y;
     const float rawAlpha = op * G;
     const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
-    const bool contrib = inRange && !(alpha < minAlpha);
-    const uint64_t who = __ballot(contrib);
-    if (who == 0)
-        return;
+    const bool contrib = inRange & !(alpha < minAlpha);
+    // A lane in range almost always contributes (the threshold is within 1e-3 of min_alpha),
+    // so only the few-lane path tests for no contribution; otherwise the selects below leave
+    // the state unchanged.
+    uint64_t who = 0;
+    if (kFewLanes > 0) {
+        who = __ballot(contrib);
+        if (who == 0)
+            return;
+    }
     const float c0 = b.z, c1 = b.w;
     const float Tn = st.T / (1.0f - alpha);
     const float w = alpha * Tn;
@@ -141,7 +148,7 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     dAlpha *= Tn;
     if (bgDot != 0.0f || bgDot != bgDot) // the background term; adding +-0 changes nothing
         dAlpha += -fT / (1.0f - alpha) * bgDot;
-    const bool grads = contrib && !(rawAlpha > kMaxAlpha);
+    const bool grads = contrib & !(rawAlpha > kMaxAlpha);
     const float dPower = op * dAlpha * G;
     const float A = a.z, B = a.w, C = b.x;
     float v[9];
