@@ -188,15 +188,16 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
     }
     sortScope.reset();
-    // Checkpoints for the segmented backward (16-pixel tiles): state at every 256-record batch
-    // boundary of each tile list.
+    // Checkpoints for the segmented backward (16-pixel tiles): state at every boundary of
+    // kSegEntries records of each tile list.
     r->segmented = r->ctx->segmentedBackward && ts == 16;
     SegmentState seg{nullptr, nullptr};
     if (r->segmented) {
-        const size_t slots = size_t(K) / 256 + size_t(tiles) + 1;
+        const size_t slots = size_t(K) / kSegEntries + size_t(tiles) + 1;
         U3D_TRY(r->ckpt.ensure(slots * 256));
         U3D_TRY(r->segCount.ensure(size_t(tiles) * kSegCountsPerTile + 1));
-        seg = SegmentState{r->ckpt.p, r->segCount.p};
+        U3D_TRY(r->hitBits.ensure((size_t(K) / 64 + size_t(tiles) + 2) * 4));
+        seg = SegmentState{r->ckpt.p, r->segCount.p, r->hitBits.p};
     }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
     U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
@@ -214,8 +215,8 @@ int segmentPrepArgs(u3d_raster *r, SegmentPrep *p)
     U3D_TRY(r->segStart.ensure(size_t(tiles) + 1));
     U3D_TRY(r->segTotal.ensure(4));
     U3D_TRY(r->segCounter.ensure(1));
-    // items <= segments of all lists (one per 256 positions and tile) + 1
-    U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / 256 + size_t(tiles) + 1));
+    // items <= segments of all lists (one per kSegEntries positions and tile) + 1
+    U3D_TRY(r->itemTile.ensure(size_t(r->numPairs) / kSegEntries + size_t(tiles) + 1));
     p->segCount = r->segCount.p;
     p->perTile = kSegCountsPerTile;
     p->tiles = tiles;
@@ -256,7 +257,7 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                                       r->segCounter.p, r->itemTile.p, r->ctx->num_cus * r->occSeg,
                                       r->tileOffsets.p, r->tileList, sb, dl, r->finalT.p,
                                       r->nContrib.p, r->ckpt.p, r->grad2d.p, lossFinalize,
-                                      prepDone));
+                                      prepDone, r->hitBits.p));
     } else {
         if (lossFinalize)
             U3D_TRY(launchLossFinalize(s, *lossFinalize));

@@ -330,15 +330,22 @@ struct RasterGeometry {
 struct SegmentState {
     float4 *ckpt;
     uint32_t *segCount;
+    // Optional (single-image forward): the cull of every 64-entry step of a tile list by
+    // quadrant, hitBits[(begin / 64 + tile + step) * 4 + quadrant], bit i for entry
+    // begin + 64 step + i; the segmented backward reads it instead of recomputing the cull.
+    uint64_t *hitBits = nullptr;
 };
 // The single-image forward (16-pixel tiles) writes one segment count per tile quadrant.
 constexpr int kSegCountsPerTile = 4;
+// List entries per segment of the single-image segmented backward (one work item each; the
+// forward checkpoints every kSegEntries entries).
+constexpr int kSegEntries = 64;
 // blockOrder (optional, 16-pixel tiles only): workgroup i renders tile blockOrder[i].
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
                         const int *blockOrder = nullptr);
-// Segmented backward: one work item per (tile, segment of B = 256 list positions) from the
+// Segmented backward: one work item per (tile, segment of kSegEntries list positions) from the
 // forward's per-tile segment counts. Scratch written here: segStart (tiles + 1 entries, the
 // exclusive scan of segCount), *totalItems, counter, itemTile (at least Σ segCount entries:
 // the owning tile of each item).
@@ -348,7 +355,7 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
                           const float4 *ckpt, float *grad2d, const LossFinalize *fin = nullptr,
-                          bool prepDone = false);
+                          bool prepDone = false, const uint64_t *hitBits = nullptr);
 int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
                                  const uint32_t *segStart, const uint32_t *totalItems,
                                  unsigned *counter, int numWorkgroups, const int *tileOffsets,

@@ -493,7 +493,7 @@ __device__ __forceinline__ void forwardQuadrant(
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib, const SegmentState &seg)
 {
-    constexpr int kBatch = 256; // checkpoint segment, list entries
+    constexpr int kBatch = kSegEntries; // checkpoint segment, list entries
     static_assert(kBatch % kQStep == 0, "segments are whole steps");
     constexpr int kGroup = 4, kCGroup = 8, kPrep = 3;
     constexpr int kFullBox = 0x7070; // x in [0, 7], y in [0, 7]
@@ -537,6 +537,8 @@ __device__ __forceinline__ void forwardQuadrant(
                          qy0 + ((box >> 8) & 15), qy0 + ((box >> 12) & 15));
         }
         const uint64_t bits = __ballot(hit);
+        if (kCkpt && seg.hitBits && lane == 0 && st < steps)
+            seg.hitBits[(size_t(begin / kQStep + tile) + size_t(st)) * 4 + quad] = bits;
         if (hit) {
             const int k = int(__builtin_amdgcn_mbcnt_hi(uint32_t(bits >> 32),
                                                         __builtin_amdgcn_mbcnt_lo(uint32_t(bits), 0u)));
@@ -700,15 +702,25 @@ __device__ __forceinline__ void forwardQuadrant(
         nContrib[pix] = last;
         if (kCkpt) {
             // Reverse pass over this pixel's checkpoints: each segment colour becomes the colour
-            // of everything from that boundary on (as forwardBlock).
+            // of everything from that boundary on (as forwardBlock). Eight checkpoints are
+            // loaded before any is rewritten, so their round trips overlap.
             float ar = sr, ag = sg, ab = sb;
-            for (int k = batches - 1; k >= 1; --k) {
-                float4 *c = seg.ckpt + (slot0 + size_t(k)) * 256 + ckTid;
-                const float4 v = *c;
-                *c = make_float4(v.x, ar, ag, ab);
-                ar += v.y;
-                ag += v.z;
-                ab += v.w;
+            constexpr int kAhead = 8;
+            for (int k = batches - 1; k >= 1; k -= kAhead) {
+                float4 v[kAhead];
+#pragma unroll
+                for (int u = 0; u < kAhead; ++u)
+                    if (k - u >= 1)
+                        v[u] = seg.ckpt[(slot0 + size_t(k - u)) * 256 + ckTid];
+#pragma unroll
+                for (int u = 0; u < kAhead; ++u) {
+                    if (k - u >= 1) {
+                        seg.ckpt[(slot0 + size_t(k - u)) * 256 + ckTid] = make_float4(v[u].x, ar, ag, ab);
+                        ar += v[u].y;
+                        ag += v[u].z;
+                        ab += v[u].w;
+                    }
+                }
             }
         }
     }
@@ -737,7 +749,8 @@ __device__ __forceinline__ void backwardSegment(
     const float4 *__restrict__ rec0, const float4 *__restrict__ rec1, const float *__restrict__ rec2,
     float bg0, float bg1, float bg2, float minAlpha, const float *__restrict__ dl,
     const float *__restrict__ finalTArr, const int *__restrict__ nContribArr,
-    const float4 *__restrict__ ckpt, float *__restrict__ grad2d)
+    const float4 *__restrict__ ckpt, float *__restrict__ grad2d,
+    const uint64_t *__restrict__ hitBits = nullptr)
 {
     __shared__ float4 s0[kBatch];
     __shared__ float4 s1[kBatch];
@@ -813,7 +826,17 @@ __device__ __forceinline__ void backwardSegment(
         s1[tid] = rb;
         s2[tid] = rc;
         sthr[tid] = thr;
-        m = quadrantMask(ra, rb.x, thr, bx0, by0);
+        if (hitBits) {
+            // the forward's cull of this entry (valid for every pixel that walks it here: a
+            // pixel with more than j contributors was live when the forward culled entry j)
+            const uint64_t *w = hitBits + (size_t(begin / 64 + tile) + size_t(j / 64)) * 4;
+            const int bit = j & 63;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                m |= uint32_t((w[q] >> bit) & 1u) << q;
+        } else {
+            m = quadrantMask(ra, rb.x, thr, bx0, by0);
+        }
         m &= (j < wm0 ? 1u : 0u) | (j < wm1 ? 2u : 0u) | (j < wm2 ? 4u : 0u) | (j < wm3 ? 8u : 0u);
     }
     if (tid < kBatch)
@@ -1032,7 +1055,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
     int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder)
 {
     const int blk = blockOrder ? blockOrder[blockIdx.x] : int(blockIdx.x);
-    forwardBlock<256, kCkpt, kFastExp>(blk % blocksX, blk / blocksX, 0, W, H, ts,
+    forwardBlock<kSegEntries, kCkpt, kFastExp>(blk % blocksX, blk / blocksX, 0, W, H, ts,
                                        tilesX, tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1,
                                        bg2, minAlpha, minT, rgb, finalT, nContrib, seg);
 }
@@ -1079,7 +1102,7 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
     const float *__restrict__ rec2, float bg0, float bg1, float bg2, float minAlpha,
     const float *__restrict__ dl, const float *__restrict__ finalTArr,
     const int *__restrict__ nContribArr, const float4 *__restrict__ ckpt,
-    float *__restrict__ grad2d, LossFinalize fin)
+    float *__restrict__ grad2d, LossFinalize fin, const uint64_t *__restrict__ hitBits)
 {
     __shared__ uint32_t sWork;
     if (fin.partials && blockIdx.x == 0) // the step's loss value, before taking work items
@@ -1094,9 +1117,10 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
         if (w >= total)
             break;
         const int tile = itemTile[w];
-        backwardSegment<256, kFastExp>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]), W, H,
-                                       tilesX, tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1,
-                                       bg2, minAlpha, dl, finalTArr, nContribArr, ckpt, grad2d);
+        backwardSegment<kSegEntries, kFastExp>(tile % tilesX, tile / tilesX, 0, int(w - segStart[tile]),
+                                               W, H, tilesX, tileOffsets, tileList, rec0, rec1, rec2,
+                                               bg0, bg1, bg2, minAlpha, dl, finalTArr, nContribArr,
+                                               ckpt, grad2d, hitBits);
     }
 }
 
@@ -1331,7 +1355,7 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                           const float *dl, const float *finalT, const int *nContrib,
                           const float4 *ckpt, float *grad2d, const LossFinalize *fin,
-                          bool prepDone)
+                          bool prepDone, const uint64_t *hitBits)
 {
     const int tiles = g.tilesX * g.tilesY;
     if (tiles == 0)
@@ -1357,7 +1381,8 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
     hipLaunchKernelGGL(kernel, dim3(numWorkgroups), dim3(kThreads), 0, s, g.width,
                        g.height, g.tilesX, itemTile, segStart, totalItems, counter, tileOffsets,
                        tileList, sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
-                       st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d, inMain);
+                       st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d, inMain,
+                       hitBits);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

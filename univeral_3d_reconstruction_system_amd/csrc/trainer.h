@@ -112,6 +112,7 @@ struct u3d_raster {
     // segmented backward (16-pixel tiles): forward checkpoints and the work-item scan
     bool segmented = false;
     u3d::DevBuf<float4> ckpt;
+    u3d::DevBuf<uint64_t> hitBits; // the forward's cull per (tile step, quadrant)
     u3d::DevBuf<uint32_t> segCount, segStart, segTotal;
     u3d::DevBuf<unsigned> segCounter;
     u3d::DevBuf<int> itemTile;
