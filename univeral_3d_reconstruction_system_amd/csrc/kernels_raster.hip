@@ -1029,19 +1029,24 @@ __device__ __forceinline__ int itemOfBlock(const RenderItem *__restrict__ items,
     return lo;
 }
 
-// 16-pixel tiles: four workgroups per tile (one per quadrant, quadrant = blockIdx.x % 4), the
-// tiles in blockOrder's order when given.
+// 16-pixel tiles: four workgroups per tile, one per quadrant, the tiles in blockOrder's order
+// when given. Workgroup x runs on XCD x % 8; the four quadrants of a tile get workgroups
+// x, x + 8, x + 16, x + 24 (one XCD), so their gathers of the tile's records share an L2. The
+// grid is 32 * ceil(tiles / 8) workgroups; those past the last tile exit.
 template <bool kCkpt, bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
-    int W, int H, int tilesX, const int *__restrict__ tileOffsets,
+    int W, int H, int tilesX, int numTiles, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder)
 {
-    const int b = int(blockIdx.x >> 2);
+    const int x = int(blockIdx.x), k = x >> 3;
+    const int b = (k >> 2) * 8 + (x & 7);
+    if (b >= numTiles)
+        return;
     const int blk = blockOrder ? blockOrder[b] : b;
-    forwardQuadrant<kCkpt, kFastExp>(blk % tilesX, blk / tilesX, int(blockIdx.x & 3), W, H, tilesX,
+    forwardQuadrant<kCkpt, kFastExp>(blk % tilesX, blk / tilesX, k & 3, W, H, tilesX,
                                      tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2,
                                      minAlpha, minT, rgb, finalT, nContrib, seg);
 }
@@ -1314,8 +1319,9 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
     if (g.tileSize == kEdge) {
         auto quad = seg.ckpt ? (fast ? rasterForwardQuadKernel<true, true> : rasterForwardQuadKernel<true, false>)
                              : (fast ? rasterForwardQuadKernel<false, true> : rasterForwardQuadKernel<false, false>);
-        hipLaunchKernelGGL(quad, dim3(4 * blocksX * blocksY), dim3(kThreads), 0, s, g.width, g.height,
-                           g.tilesX, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2,
+        const int numTiles = blocksX * blocksY;
+        hipLaunchKernelGGL(quad, dim3(32 * ((numTiles + 7) / 8)), dim3(kThreads), 0, s, g.width,
+                           g.height, g.tilesX, numTiles, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2,
                            st.background[0], st.background[1], st.background[2], st.min_alpha,
                            st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder);
         U3D_LAUNCH_CHECK();
