@@ -12,6 +12,9 @@ constexpr int kThreads = 256;
 
 // Projects Gaussian i of a model and writes its screen state at entry o; returns the number of
 // tiles it covers.
+// kSH: the spherical-harmonics colour terms are compiled in (degree > 0 callers); without them
+// the kernel needs no private memory for the basis arrays.
+template <bool kSH>
 __device__ __forceinline__ uint32_t projectBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
@@ -56,7 +59,7 @@ __device__ __forceinline__ uint32_t projectBody(
             break;
         flags = uint8_t((p.clampX ? 1 : 0) | (p.clampY ? 2 : 0));
         float rgb[3];
-        const int nb = shRest ? shCount(shDegree) : 0;
+        const int nb = (kSH && shRest) ? shCount(shDegree) : 0;
         float basis[15];
         if (nb) {
             float d[3], len;
@@ -95,6 +98,7 @@ __device__ __forceinline__ uint32_t projectBody(
     return nt;
 }
 
+template <bool kSH>
 __global__ __launch_bounds__(kThreads) void projectKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void projectKernel(
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     uint32_t nt = 0;
     if (i < n)
-        nt = projectBody(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts,
+        nt = projectBody<kSH>(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts,
                          tilesX, tilesY, out, shRest, shDegree, campos);
     if (!blockTiles)
         return;
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void projectBatchedKernel(const RenderIte
         return;
     const int64_t o = it.screenBase + i;
     entryItem[o] = int(blockIdx.y);
-    projectBody(i, o, it.means, it.scales, it.quats, it.opacities, it.colors, it.cam, nearPlane,
+    projectBody<true>(i, o, it.means, it.scales, it.quats, it.opacities, it.colors, it.cam, nearPlane,
                 eps2d, ts, it.tilesX, it.tilesY, out, it.shRest, it.shDegree, cameraCentre(it.cam));
 }
 
@@ -164,7 +168,7 @@ struct GradRegs {
 };
 
 // Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o.
-template <class Sink>
+template <bool kSH, class Sink>
 __device__ __forceinline__ void chainBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
@@ -196,7 +200,7 @@ __device__ __forceinline__ void chainBody(
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
     // View-dependent colour: coefficient gradients and the direction -> mean term.
-    const int nb = shRest ? shCount(shDegree) : 0;
+    const int nb = (kSH && shRest) ? shCount(shDegree) : 0;
     float dDir[3] = {0.0f, 0.0f, 0.0f}, viewD[3] = {0.0f, 0.0f, 0.0f}, viewLen = 1.0f;
     if (nb) {
         float b[15];
@@ -315,6 +319,7 @@ __device__ __forceinline__ void chainBody(
 
 // clear: the 2D gradients are zeroed after use (only visible Gaussians' entries are ever
 // written, so the buffer is all zero again for the next backward).
+template <bool kSH>
 __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
@@ -327,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     if (i >= n)
         return;
     GradArrays out{gMeans, gScales, gQuats, gOpac, gColors, i};
-    chainBody(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
+    chainBody<kSH>(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
               out, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
     if (clear && radius[i] != 0) {
 #pragma unroll
@@ -339,6 +344,7 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
 // Chain rule of Gaussian i into registers, then the Adam update of its 14 parameters from them.
 // The registers start at zero and take the same terms in the same order as the zeroed gradient
 // arrays would, so the update equals chainRuleKernel + adamKernel bit for bit.
+template <bool kSH>
 __global__ __launch_bounds__(kThreads) void chainAdamKernel(
     int64_t n, u3d_gaussians prm, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(kThreads) void chainAdamKernel(
     if (i >= n)
         return;
     GradRegs gr;
-    chainBody(i, i, prm.means, prm.scales, prm.quats, cam, nearPlane, eps2d, radius, flagsArr,
+    chainBody<kSH>(i, i, prm.means, prm.scales, prm.quats, cam, nearPlane, eps2d, radius, flagsArr,
               rec1, grad2d, gr, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
     if (radius[i] != 0) {
 #pragma unroll
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem 
         if (sb.radius[o] == 0)
             continue;
         GradArrays out{it.gMeans, it.gScales, it.gQuats, it.gOpac, it.gColors, i};
-        chainBody(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
+        chainBody<true>(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
                   sb.flags, sb.rec1, grad2d, out, it.ndc, it.gradCount, it.shRest, it.shDegree,
                   cameraCentre(it.cam), it.gShRest, it.ndcScale);
 #pragma unroll
@@ -450,7 +456,8 @@ int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
 {
     if (m.n <= 0)
         return U3D_OK;
-    hipLaunchKernelGGL(projectKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
+    const bool sh = shRest && shDegree > 0;
+    hipLaunchKernelGGL(sh ? projectKernel<true> : projectKernel<false>, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
                        m.scales, m.quats, m.opacities, m.colors, cam, st.near_plane, st.eps2d,
                        st.tile_size, tilesX, tilesY, out, shRest, shDegree, cameraCentre(cam),
                        blockTiles);
@@ -465,7 +472,8 @@ int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam
 {
     if (m.n <= 0)
         return U3D_OK;
-    hipLaunchKernelGGL(chainRuleKernel, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
+    const bool sh = shRest && shDegree > 0;
+    hipLaunchKernelGGL(sh ? chainRuleKernel<true> : chainRuleKernel<false>, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
                        m.means, m.scales, m.quats, cam, st.near_plane, st.eps2d, sb.radius,
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
                        grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
@@ -483,7 +491,8 @@ int launchChainAdam(hipStream_t s, const u3d_gaussians &params, const u3d_camera
         return U3D_OK;
     U3D_CHECK(adam.m.n == params.n && adam.v.n == params.n, U3D_ERR_INVALID,
               "Adam moments differ in size from the parameters");
-    hipLaunchKernelGGL(chainAdamKernel, dim3(blocksFor(params.n)), dim3(kThreads), 0, s, params.n,
+    const bool sh = shRest && shDegree > 0;
+    hipLaunchKernelGGL(sh ? chainAdamKernel<true> : chainAdamKernel<false>, dim3(blocksFor(params.n)), dim3(kThreads), 0, s, params.n,
                        params, cam, st.near_plane, st.eps2d, sb.radius, sb.flags, sb.rec1, grad2d,
                        ndc, gradCount, shRest, shDegree, cameraCentre(cam), gShRest, adam);
     U3D_LAUNCH_CHECK();

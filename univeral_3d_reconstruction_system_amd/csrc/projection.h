@@ -46,16 +46,14 @@ __device__ __forceinline__ bool projectGaussian(const float mu[3], const float q
     const float z = p.pc[2];
     if (z < nearPlane)
         return false;
-    p.qNorm = sqrtf(qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
-    if (p.qNorm < 1e-12f) {
-        p.qNorm = 1.0f;
-        p.q[0] = 1.0f;
-        p.q[1] = p.q[2] = p.q[3] = 0.0f;
-    } else {
+    // A near-zero quaternion becomes the identity with norm 1. Written as selects: a branch
+    // with a chained assignment kept two components in private memory.
+    const float qn = sqrtf(qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+    const bool tiny = qn < 1e-12f;
+    p.qNorm = tiny ? 1.0f : qn;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            p.q[k] = qr[k] / p.qNorm;
-    }
+    for (int k = 0; k < 4; ++k)
+        p.q[k] = tiny ? (k == 0 ? 1.0f : 0.0f) : qr[k] / qn;
     rotOfQuat(p.q[0], p.q[1], p.q[2], p.q[3], p.Rq);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
