@@ -132,10 +132,12 @@ def test_forward_empty_model(ctx):
     assert r.state()["num_pairs"] == 0
 
 
-def test_forward_no_cutoffs(ctx):
-    """min_alpha = min_transmittance = 0 as in the reference's gradient test."""
+@pytest.mark.parametrize("tile", [16, 64])
+def test_forward_no_cutoffs(ctx, tile):
+    """min_alpha = min_transmittance = 0 as in the reference's gradient test (no skip threshold,
+    so no cull; with 16-pixel tiles the per-quadrant walk with the general exp)."""
     g = ref_style_model(8, 7, 0.6, 3.0, 0.08, 0.3)
-    _, r, img, ro, ref = _forward_both(ctx, g, fd_camera(), tile_size=64, min_alpha=0.0,
+    _, r, img, ro, ref = _forward_both(ctx, g, fd_camera(), tile_size=tile, min_alpha=0.0,
                                        min_transmittance=0.0, background=(0.1, 0.2, 0.3))
     _assert_forward_equal(r, img, ro, ref)
 
@@ -343,8 +345,9 @@ def test_trainer_runs_and_recovers(ctx):
 @pytest.mark.parametrize("kw", [dict(), dict(min_alpha=0.0, min_transmittance=0.0),
                                 dict(background=(0.3, 0.6, 0.9))])
 def test_segmented_backward_matches_whole_list_walk(ctx, kw):
-    """The segmented backward (checkpoints every 256 list positions) against the whole-list
-    walk and the oracle: same gradients to the 1e-4 bar, on tiles with several segments."""
+    """The segmented backward (checkpoints every 64 list positions, the forward's per-quadrant
+    cull reused) against the whole-list walk and the oracle: same gradients to the 1e-4 bar, on
+    tiles with several segments."""
     g, cams = config0_scene(20000, 3)
     su, so = _settings(**kw)
     dg = U.DeviceGaussians.from_host(ctx, g)
