@@ -593,6 +593,7 @@ __device__ __forceinline__ void forwardQuadrant(
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f; // colour of the current segment (checkpoints only)
     int last = 0, batches = 0;             // batches: checkpoint segments started
+    int lastSlot = 0;                      // the pixel's last checkpoint written
     // +0 while the pixel takes contributions, NaN once it is done (saturated or outside the
     // image): alpha + doneNaN is NaN exactly when the entry is no contribution.
     float doneNaN = inside ? 0.0f : __builtin_nanf("");
@@ -657,10 +658,18 @@ __device__ __forceinline__ void forwardQuadrant(
                 L.stop[st & 1] = allDone ? 1 : 0;
             if (!allDone) {
                 if ((st * kQStep) % kBatch == 0) {
-                    if (kCkpt && st > 0) { // T before this segment and the previous one's colour
-                        if (inside)
+                    if (kCkpt && st > 0) {
+                        // T before this segment and the previous one's colour, for a pixel
+                        // still walking: the backward reads a boundary's checkpoint only for
+                        // pixels with more contributors than the boundary. A done pixel keeps
+                        // its last segment's colour (later segments add nothing to it).
+                        const bool walking = !__builtin_isnan(doneNaN);
+                        if (inside && walking)
                             seg.ckpt[(slot0 + size_t(batches)) * 256 + ckTid] = make_float4(T, sr, sg, sb);
-                        sr = sg = sb = 0.0f;
+                        sr = walking ? 0.0f : sr;
+                        sg = walking ? 0.0f : sg;
+                        sb = walking ? 0.0f : sb;
+                        lastSlot = walking ? batches : lastSlot;
                     }
                     ++batches;
                 }
@@ -702,11 +711,12 @@ __device__ __forceinline__ void forwardQuadrant(
         nContrib[pix] = last;
         if (kCkpt) {
             // Reverse pass over this pixel's checkpoints: each segment colour becomes the colour
-            // of everything from that boundary on (as forwardBlock). Eight checkpoints are
-            // loaded before any is rewritten, so their round trips overlap.
+            // of everything from that boundary on (as forwardBlock; the same values in every
+            // checkpoint written). Eight checkpoints are loaded before any is rewritten, so
+            // their round trips overlap.
             float ar = sr, ag = sg, ab = sb;
             constexpr int kAhead = 8;
-            for (int k = batches - 1; k >= 1; k -= kAhead) {
+            for (int k = lastSlot; k >= 1; k -= kAhead) {
                 float4 v[kAhead];
 #pragma unroll
                 for (int u = 0; u < kAhead; ++u)
