@@ -144,7 +144,7 @@ int u3d_memcpy_d2h(u3d_ctx *ctx, void *dst_host, const void *src, size_t bytes);
 int u3d_memset(u3d_ctx *ctx, void *dst, int value, size_t bytes);
 /* Context options (no reference counterpart). U3D_OPT_SEGMENTED_BACKWARD (default 1): with
  * 16-pixel tiles the forward keeps a per-pixel checkpoint (transmittance, front colour) at
- * every 256-record boundary of each tile list, and the backward walks those segments as
+ * every 64-record boundary of each tile list, and the backward walks those segments as
  * independent work items instead of each tile's whole list in one workgroup. Images are
  * unchanged; gradients differ from the unsegmented walk by float rounding only. Applies from
  * the next forward. */
