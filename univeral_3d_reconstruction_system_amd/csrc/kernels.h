@@ -324,9 +324,12 @@ struct RasterGeometry {
 };
 // Forward state the segmented backward starts from (16x16 tiles; ckpt == nullptr: not kept).
 // For a tile t whose list starts at pair `begin`, checkpoint s >= 1 of its pixel p,
-// ckpt[((begin / B) + t + s) * 256 + p] (B = LDS batch), holds the transmittance before list
-// position s * B and the colour composited from that position on; segCount[t] = ceil(max
-// contributor count of the tile / B). Capacity: (K / B + tiles + 1) * 256 entries.
+// ckpt[((begin / B) + t + s) * 256 + p] (B = segment length: kSegEntries for the single-image
+// forward, the LDS batch in the batched engine), holds the transmittance before list position
+// s * B and the colour composited from that position on (the single-image forward writes it
+// only for pixels still walking there, the only ones whose backward reads it); the segment
+// count is ceil(max contributor count / B), per tile quadrant in the single-image forward
+// (segCount[4 t + q]) and per tile in the batched engine. Capacity: (K / B + tiles + 1) * 256.
 struct SegmentState {
     float4 *ckpt;
     uint32_t *segCount;

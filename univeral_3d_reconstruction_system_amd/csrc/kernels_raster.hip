@@ -1,10 +1,12 @@
 // Tile compositing, forward (Rasterizer.cpp:263-330) and backward stage 1 (Rasterizer.cpp:
 // 334-427), for gfx950.
 //
-// One 256-thread workgroup per 16x16 pixel block; each wave64 owns one 8x8 quadrant, one pixel
-// per lane. A logical tile of tile_size (a multiple of 16) is covered by (tile_size/16)^2 blocks
-// that share its list. The tile's Gaussians are staged through LDS in batches of 256 records
-// (36 B each, gathered once per batch, one per thread).
+// The single-image forward with 16-pixel tiles is rasterForwardQuadKernel (a workgroup per 8x8
+// quadrant whose wavefronts split the walk, forwardQuadrant below). Elsewhere (larger tiles, the
+// batched engine, the backward): one 256-thread workgroup per 16x16 pixel block; each wave64
+// owns one 8x8 quadrant, one pixel per lane. A logical tile of tile_size (a multiple of 16) is
+// covered by (tile_size/16)^2 blocks that share its list. The tile's Gaussians are staged
+// through LDS in batches of records (36 B each, gathered once per batch, one per thread).
 //
 // Exact culling. While staging, every Gaussian gets a 4-bit mask of the quadrants in which it
 // can contribute: the minimum of its quadratic form over the quadrant's pixel rectangle
