@@ -446,14 +446,6 @@ __device__ __forceinline__ void forwardBlock(
     }
 }
 
-// A uniform 64-bit value read from LDS, moved to scalar registers.
-__device__ __forceinline__ uint64_t uniform64(uint64_t v)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
-    return (uint64_t(hi) << 32) | lo;
-}
-
 // Forward of one 8x8 quadrant of a 16x16 tile by a whole 256-thread workgroup. A wavefront
 // walking a long list alone issues ~90 instructions per Gaussian, most of them the pixel-local
 // part (power, exact exp, alpha, the skip tests), which does not depend on the walk's state;
