@@ -857,8 +857,8 @@ __device__ __forceinline__ void backwardSegment(
         // Reverse walk over the set bits, two register sets in turn: the next Gaussian's LDS
         // records are loaded while the current one is processed (same Gaussians, same order).
         auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
-            backStep<kSegFewLanes, kFastExp>(bs, lo + jj < jEnd, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1, dp2,
-                                  fT, bgDot, minAlpha, stab, lane, acc[jj]);
+            backStep<kSegFewLanes, kFastExp>(bs, lo + jj < jEnd, ra, rb, rthr, rc2, fpx, fpy, dp0,
+                                             dp1, dp2, fT, bgDot, minAlpha, stab, lane, acc[jj]);
         };
         int jA = c * 64 + 63 - __builtin_clzll(bits);
         bits &= ~(uint64_t(1) << (jA - c * 64));
