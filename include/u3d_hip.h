@@ -149,6 +149,12 @@ int u3d_memset(u3d_ctx *ctx, void *dst, int value, size_t bytes);
  * unchanged; gradients differ from the unsegmented walk by float rounding only. Applies from
  * the next forward. */
 #define U3D_OPT_SEGMENTED_BACKWARD 1
+/* U3D_OPT_SORT_SPIN_LIMIT (default 0 = a bound no wait that ends comes near): how many lookback
+ * re-reads a depth-sort pass makes before it gives up a wait and flags the sort as failed; a
+ * negative value gives every wait up at once. The kernels stay in bounds either way (that
+ * forward's tile lists come out empty), and the next forward or trainer step on that rasterizer
+ * returns U3D_ERR_HIP. A test hook for that error path. */
+#define U3D_OPT_SORT_SPIN_LIMIT 2
 int u3d_ctx_set_option(u3d_ctx *ctx, int option, int value);
 
 /* ------------------------------------------------------------------- rasterizer (L1 core) */
@@ -239,8 +245,15 @@ int u3d_trainer_get_model(u3d_trainer *t, float *means_host, float *scales_host,
 /* Adam moments (Adam.h:30-31) as 14 * size floats each: means 3n, scales 3n, quats 4n,
  * opacities n, colours 3n (checkpointing; tests). */
 int u3d_trainer_get_moments(u3d_trainer *t, float *m_host, float *v_host);
-/* SH coefficients [45 * size] of a trainer created with sh_degree > 0. */
+/* SH coefficients [45 * size] of a trainer created with sh_degree > 0 ([n][15][3], as
+ * u3d_raster_forward_sh). set_sh_rest replaces them (the trainer starts them at zero). */
 int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host);
+int u3d_trainer_set_sh_rest(u3d_trainer *t, const float *sh_rest_host);
+/* Adam moments of the SH coefficients, [45 * size] each, same layout (checkpointing; tests). */
+int u3d_trainer_get_sh_moments(u3d_trainer *t, float *m_host, float *v_host);
+/* The densification statistics since the last densification (Trainer.cpp:76-79, :442-445):
+ * accumulated |dL/d mean2d| in NDC [size] and the count of steps each Gaussian was visible. */
+int u3d_trainer_get_densify_stats(u3d_trainer *t, float *accum_host, int *count_host);
 int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out);      /* Trainer::stats */
 int u3d_trainer_evaluate_psnr(u3d_trainer *t, float *out_host);   /* Trainer.cpp:180-192 */
 /* Mean absolute error over all training views (time-to-target metric). */

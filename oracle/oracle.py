@@ -183,6 +183,12 @@ def lib():
             getattr(L, f"orc_trainer_{name}").argtypes = [C.c_void_p]
             getattr(L, f"orc_trainer_{name}").restype = C.c_float
         L.orc_trainer_render.argtypes = [C.c_void_p, C.POINTER(Camera), F32P]
+        L.orc_project_f64.argtypes = [C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.POINTER(Camera),
+                                                                   C.POINTER(Settings)] + [C.c_void_p] * 5
+        L.orc_chain_rule_f64.argtypes = [C.c_void_p, C.POINTER(Model), C.c_void_p, C.c_int, F32P] + \
+            [C.c_void_p] * 6
+        L.orc_trainer_set_sh.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_void_p]
+        L.orc_trainer_get_sh.argtypes = [C.c_void_p, F32P]
     return _orc
 
 
@@ -331,6 +337,20 @@ class Raster:
         self.L.orc_raster_get_grad2d(self.h, out)
         return out
 
+    def chain_rule_f64(self, g: "Gaussians", grad2d, sh_rest=None, sh_degree=0) -> dict:
+        """The chain-rule stage of the backward in FP64 from 2D gradients [9n] (e.g. grad2d()
+        after a backward) and this rasterizer's last forward; fields of the model plus 'sh'."""
+        assert not self.use_ref
+        n = g.n
+        out = {f: np.zeros(n * d, np.float64) for f, d in FIELDS}
+        out["sh"] = np.zeros(n * 45, np.float64)
+        rest = None if sh_rest is None else np.ascontiguousarray(sh_rest, np.float32)
+        m = g.struct()
+        self.L.orc_chain_rule_f64(self.h, C.byref(m), _ptr(rest), sh_degree,
+                                  np.ascontiguousarray(grad2d, np.float32),
+                                  *(_ptr(out[f]) for f, _ in FIELDS), _ptr(out["sh"]))
+        return out
+
 
 def render_f64(g: dict, cam: dict, st: Settings) -> np.ndarray:
     c = make_camera(cam)
@@ -347,6 +367,19 @@ def render_f64_sh(g: dict, rest, degree, cam: dict, st: Settings) -> np.ndarray:
     lib().orc_render_f64_sh(len(arr[3]), *arr, np.ascontiguousarray(rest, np.float64), degree,
                             C.byref(c), C.byref(st), rgb)
     return rgb
+
+
+def project_f64(g: dict, cam: dict, st: Settings, sh_rest=None, sh_degree=0) -> dict:
+    """The projection in FP64 (double parameters): mean2d, conic, rgb, alpha, radius."""
+    n = len(g["opacities"])
+    arr = [np.ascontiguousarray(g[f], np.float64) for f, _ in FIELDS]
+    rest = None if sh_rest is None else np.ascontiguousarray(sh_rest, np.float64)
+    out = dict(mean2d=np.zeros(2 * n), conic=np.zeros(3 * n), rgb=np.zeros(3 * n), alpha=np.zeros(n),
+               radius=np.zeros(n, np.int32))
+    c = make_camera(cam)
+    lib().orc_project_f64(n, *(_ptr(a) for a in arr), _ptr(rest), sh_degree, C.byref(c), C.byref(st),
+                          *(_ptr(out[k]) for k in ("mean2d", "conic", "rgb", "alpha", "radius")))
+    return out
 
 
 def sh_basis(x, y, z, degree=3) -> np.ndarray:
@@ -464,7 +497,8 @@ class Trainer:
     """Oracle Trainer (Trainer.h:249-292); ``use_ref`` drives the compiled reference instead."""
 
     def __init__(self, init: Gaussians, cams: list, images_u8: list, extent: float,
-                 opt: TrainOptions, use_ref=False, threads=1):
+                 opt: TrainOptions, use_ref=False, threads=1, sh_degree=0, sh_lr_scale=0.0,
+                 sh_rest=None):
         self.use_ref = use_ref
         self.L = ref_lib() if use_ref else lib()
         arr = (Camera * len(cams))(*[make_camera(c) for c in cams])
@@ -475,6 +509,16 @@ class Trainer:
         else:
             self.h = self.L.orc_trainer_new(C.byref(m), arr, _ptr(self._rgb), len(cams), extent, C.byref(opt))
         self.p = "ref_trainer_" if use_ref else "orc_trainer_"
+        self.sh_degree = sh_degree
+        if sh_degree:
+            assert not use_ref, "the reference has no SH beyond degree 0"
+            self._rest0 = None if sh_rest is None else np.ascontiguousarray(sh_rest, np.float32)
+            self.L.orc_trainer_set_sh(self.h, sh_degree, sh_lr_scale, _ptr(self._rest0))
+
+    def sh_rest(self) -> np.ndarray:
+        out = np.zeros(self.size() * 45, np.float32)
+        self.L.orc_trainer_get_sh(self.h, out)
+        return out
 
     def _f(self, name):
         return getattr(self.L, self.p + name)

@@ -544,6 +544,152 @@ void orc_raster_forward(orc_raster *r, const orc_model *m, const orc_camera *cam
 
 // Rasterizer::backward, Rasterizer.cpp:334-545, with the per-tile merge done serially in tile
 // order (the reference's order when it runs on one thread).
+} // extern "C"
+
+namespace {
+// Stage 2 of the backward (Rasterizer.cpp:432-544): per visible Gaussian of the last forward,
+// the chain rule from its 2D gradient (grad2d [9n]) to the parameters, ADDED into the outputs,
+// and the NDC statistic into ndc (nullable). T = float is the reference's arithmetic (same
+// operation order); T = double evaluates the same expressions in FP64 from the same inputs.
+template <typename T>
+void chainRule(const orc_raster *r, const orc_model *m, const float *rest, int degree,
+               const float *grad2d, T *gMeans, T *gScales, T *gQuats, T *gOpac, T *gColors,
+               T *gRest, float *ndc)
+{
+    const Screen<float> &s = r->s;
+    const Cam<T> cam = camOf<T>(r->cam);
+    const size_t n = r->n;
+    for (size_t i = 0; i < n; ++i) {
+        if (s.radius[i] == 0)
+            continue;
+        const float *g = &grad2d[i * kG2];
+        if (ndc) {
+            const float nx = g[0] * 0.5f * float(cam.width), ny = g[1] * 0.5f * float(cam.height);
+            ndc[i] += std::sqrt(nx * nx + ny * ny);
+        }
+        const T dmx = T(g[0]), dmy = T(g[1]), dA = T(g[2]), dB = T(g[3]), dC = T(g[4]);
+        for (int ch = 0; ch < 3; ++ch)
+            if (!(s.flags[i] & (4 << ch)))
+                gColors[i * 3 + ch] += T(kShC0) * T(g[5 + ch]);
+        T mu[3], qr[4], ls[3];
+        for (int k = 0; k < 3; ++k) {
+            mu[k] = T(m->means[i * 3 + k]);
+            ls[k] = T(m->scales[i * 3 + k]);
+        }
+        for (int k = 0; k < 4; ++k)
+            qr[k] = T(m->quats[i * 4 + k]);
+        T dDir[3] = {0, 0, 0}, viewD[3] = {0, 0, 0}, viewLen = 1;
+        const int nb = rest ? shCount(degree) : 0;
+        if (nb) {
+            // View-dependent colour: coefficient gradients and the direction -> mean term.
+            T campos[3], b[15], bx[15], by[15], bz[15];
+            cameraCentre(cam, campos);
+            viewDir(mu, campos, viewD, &viewLen);
+            shBasis(viewD[0], viewD[1], viewD[2], degree, b, bx, by, bz);
+            for (int ch = 0; ch < 3; ++ch) {
+                if (s.flags[i] & (4 << ch))
+                    continue;
+                const T gc = T(g[5 + ch]);
+                for (int k = 0; k < nb; ++k) {
+                    const T coef = T(rest[i * 45 + k * 3 + ch]);
+                    if (gRest)
+                        gRest[i * 45 + k * 3 + ch] += b[k] * gc;
+                    dDir[0] += coef * bx[k] * gc;
+                    dDir[1] += coef * by[k] * gc;
+                    dDir[2] += coef * bz[k] * gc;
+                }
+            }
+        }
+        const T op = T(s.alpha[i]);
+        gOpac[i] += T(g[8]) * op * (T(1) - op);
+
+        Proj<T> p;
+        projectOne<T>(mu, qr, ls, cam, T(r->st.near_plane), T(r->st.eps2d), p);
+        const T z = p.pc[2];
+        const T a = p.a, b = p.b, c = p.c;
+        const T det = a * c - b * b;
+        const T det2 = det * det;
+        const T da = (dA * -c * c + dB * b * c + dC * -b * b) / det2;
+        const T db = (dA * T(2) * b * c + dC * T(2) * a * b) / det2 + dB * (T(-1) / det - T(2) * b * b / det2);
+        const T dc = (dA * -b * b + dB * a * b + dC * -a * a) / det2;
+        const T G00 = da, G01 = T(0.5) * db, G11 = dc;
+        T dSigma[9];
+        for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l)
+                dSigma[k * 3 + l] = p.T0[k] * (G00 * p.T0[l] + G01 * p.T1[l])
+                                    + p.T1[k] * (G01 * p.T0[l] + G11 * p.T1[l]);
+        T TS0[3], TS1[3];
+        for (int col = 0; col < 3; ++col) {
+            TS0[col] = p.T0[0] * p.Sigma[col] + p.T0[1] * p.Sigma[3 + col] + p.T0[2] * p.Sigma[6 + col];
+            TS1[col] = p.T1[0] * p.Sigma[col] + p.T1[1] * p.Sigma[3 + col] + p.T1[2] * p.Sigma[6 + col];
+        }
+        T dT0[3], dT1[3];
+        for (int col = 0; col < 3; ++col) {
+            dT0[col] = T(2) * (G00 * TS0[col] + G01 * TS1[col]);
+            dT1[col] = T(2) * (G01 * TS0[col] + G11 * TS1[col]);
+        }
+        T dM[9];
+        for (int rr = 0; rr < 3; ++rr)
+            for (int col = 0; col < 3; ++col)
+                dM[rr * 3 + col] = T(2) * (dSigma[rr * 3] * p.M[col] + dSigma[rr * 3 + 1] * p.M[3 + col]
+                                           + dSigma[rr * 3 + 2] * p.M[6 + col]);
+        T dRq[9];
+        for (int col = 0; col < 3; ++col) {
+            T ds = 0;
+            for (int rr = 0; rr < 3; ++rr) {
+                ds += dM[rr * 3 + col] * p.Rq[rr * 3 + col];
+                dRq[rr * 3 + col] = dM[rr * 3 + col] * p.s[col];
+            }
+            gScales[i * 3 + col] += ds * p.s[col];
+        }
+        const T w = p.q[0], x = p.q[1], y = p.q[2], zq = p.q[3];
+        const T two = T(2);
+        const T dw = two * (-zq * dRq[1] + y * dRq[2] + zq * dRq[3] - x * dRq[5] - y * dRq[6] + x * dRq[7]);
+        const T dx = two * (y * dRq[1] + zq * dRq[2] + y * dRq[3] - two * x * dRq[4] - w * dRq[5]
+                            + zq * dRq[6] + w * dRq[7] - two * x * dRq[8]);
+        const T dy = two * (-two * y * dRq[0] + x * dRq[1] + w * dRq[2] + x * dRq[3] + zq * dRq[5]
+                            - w * dRq[6] + zq * dRq[7] - two * y * dRq[8]);
+        const T dz = two * (-two * zq * dRq[0] - w * dRq[1] + x * dRq[2] + w * dRq[3] - two * zq * dRq[4]
+                            + y * dRq[5] + x * dRq[6] + y * dRq[7]);
+        const T dq[4] = {dw, dx, dy, dz};
+        const T qdot = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
+        for (int k = 0; k < 4; ++k)
+            gQuats[i * 4 + k] += (dq[k] - qdot * p.q[k]) / p.qNorm;
+        const T *R = cam.R;
+        const T dJ00 = dT0[0] * R[0] + dT0[1] * R[1] + dT0[2] * R[2];
+        const T dJ02 = dT0[0] * R[6] + dT0[1] * R[7] + dT0[2] * R[8];
+        const T dJ11 = dT1[0] * R[3] + dT1[1] * R[4] + dT1[2] * R[5];
+        const T dJ12 = dT1[0] * R[6] + dT1[1] * R[7] + dT1[2] * R[8];
+        const T z2 = z * z, z3 = z2 * z;
+        T dpc[3] = {0, 0, 0};
+        dpc[2] += dJ00 * -cam.fx / z2 + dJ02 * two * cam.fx * p.tx / z3 + dJ11 * -cam.fy / z2
+                  + dJ12 * two * cam.fy * p.ty / z3;
+        const T dtx = dJ02 * -cam.fx / z2;
+        const T dty = dJ12 * -cam.fy / z2;
+        if (p.clampX)
+            dpc[2] += dtx * p.limTx;
+        else
+            dpc[0] += dtx;
+        if (p.clampY)
+            dpc[2] += dty * p.limTy;
+        else
+            dpc[1] += dty;
+        dpc[0] += dmx * cam.fx / z;
+        dpc[1] += dmy * cam.fy / z;
+        dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
+        for (int k = 0; k < 3; ++k)
+            gMeans[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
+        if (nb) {
+            const T dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
+            for (int k = 0; k < 3; ++k)
+                gMeans[i * 3 + k] += (dDir[k] - dot * viewD[k]) / viewLen;
+        }
+    }
+}
+} // namespace
+
+extern "C" {
+
 void orc_raster_backward_sh(orc_raster *r, const orc_model *m, const float *rest, int degree,
                             const float *dl, orc_model *grads, float *grad_rest, float *ndc)
 {
@@ -618,126 +764,45 @@ void orc_raster_backward_sh(orc_raster *r, const orc_model *m, const float *rest
         }
     }
 
-    const Cam<float> cam = camOf<float>(r->cam);
-    for (size_t i = 0; i < n; ++i) {
-        if (s.radius[i] == 0)
-            continue;
-        const float *g = &r->grad2d[i * kG2];
-        const float dmx = g[0], dmy = g[1], dA = g[2], dB = g[3], dC = g[4];
-        if (ndc) {
-            const float nx = dmx * 0.5f * float(cam.width), ny = dmy * 0.5f * float(cam.height);
-            ndc[i] += std::sqrt(nx * nx + ny * ny);
-        }
-        for (int ch = 0; ch < 3; ++ch)
-            if (!(s.flags[i] & (4 << ch)))
-                grads->colors[i * 3 + ch] += kShC0 * g[5 + ch];
-        float dDir[3] = {0, 0, 0}, viewD[3] = {0, 0, 0}, viewLen = 1;
-        const int nb = rest ? shCount(degree) : 0;
-        if (nb) {
-            // View-dependent colour: coefficient gradients and the direction -> mean term.
-            float campos[3], b[15], bx[15], by[15], bz[15];
-            cameraCentre(cam, campos);
-            viewDir(&m->means[i * 3], campos, viewD, &viewLen);
-            shBasis(viewD[0], viewD[1], viewD[2], degree, b, bx, by, bz);
-            for (int ch = 0; ch < 3; ++ch) {
-                if (s.flags[i] & (4 << ch))
-                    continue;
-                const float gc = g[5 + ch];
-                for (int k = 0; k < nb; ++k) {
-                    const float coef = rest[i * 45 + k * 3 + ch];
-                    if (grad_rest)
-                        grad_rest[i * 45 + k * 3 + ch] += b[k] * gc;
-                    dDir[0] += coef * bx[k] * gc;
-                    dDir[1] += coef * by[k] * gc;
-                    dDir[2] += coef * bz[k] * gc;
-                }
-            }
-        }
-        const float op = s.alpha[i];
-        grads->opacities[i] += g[8] * op * (1.0f - op);
+    chainRule<float>(r, m, rest, degree, r->grad2d.data(), grads->means, grads->scales,
+                     grads->quats, grads->opacities, grads->colors, grad_rest, ndc);
+}
 
-        Proj<float> p;
-        projectOne<float>(&m->means[i * 3], &m->quats[i * 4], &m->scales[i * 3], cam,
-                          r->st.near_plane, r->st.eps2d, p);
-        const float z = p.pc[2];
-        const float a = p.a, b = p.b, c = p.c;
-        const float det = a * c - b * b;
-        const float det2 = det * det;
-        const float da = (dA * -c * c + dB * b * c + dC * -b * b) / det2;
-        const float db = (dA * 2 * b * c + dC * 2 * a * b) / det2 + dB * (-1.0f / det - 2 * b * b / det2);
-        const float dc = (dA * -b * b + dB * a * b + dC * -a * a) / det2;
-        const float G00 = da, G01 = 0.5f * db, G11 = dc;
-        float dSigma[9];
-        for (int k = 0; k < 3; ++k)
-            for (int l = 0; l < 3; ++l)
-                dSigma[k * 3 + l] = p.T0[k] * (G00 * p.T0[l] + G01 * p.T1[l])
-                                    + p.T1[k] * (G01 * p.T0[l] + G11 * p.T1[l]);
-        float TS0[3], TS1[3];
-        for (int col = 0; col < 3; ++col) {
-            TS0[col] = p.T0[0] * p.Sigma[col] + p.T0[1] * p.Sigma[3 + col] + p.T0[2] * p.Sigma[6 + col];
-            TS1[col] = p.T1[0] * p.Sigma[col] + p.T1[1] * p.Sigma[3 + col] + p.T1[2] * p.Sigma[6 + col];
+// Rasterizer.cpp:137-204 in FP64 for n Gaussians (double arrays); outputs only where
+// radius[i] > 0 (radius 0 elsewhere). rest may be NULL (degree 0).
+void orc_project_f64(int n, const double *means, const double *scales, const double *quats,
+                     const double *opacities, const double *colors, const double *rest, int degree,
+                     const orc_camera *camera, const orc_settings *st, double *mean2d,
+                     double *conic, double *rgb, double *alpha, int *radius)
+{
+    const Cam<double> cam = camOf<double>(*camera);
+    const int ts = st->tile_size;
+    const int tilesX = (camera->width + ts - 1) / ts, tilesY = (camera->height + ts - 1) / ts;
+    Screen<double> s;
+    s.resize(size_t(n));
+    for (size_t i = 0; i < size_t(n); ++i)
+        projectScreen<double>(i, means, scales, quats, opacities, colors, cam, *st, tilesX, tilesY, s,
+                              rest, degree);
+    for (size_t i = 0; i < size_t(n); ++i) {
+        radius[i] = s.radius[i];
+        for (int k = 0; k < 2; ++k)
+            mean2d[i * 2 + k] = s.mean2d[i * 2 + k];
+        for (int k = 0; k < 3; ++k) {
+            conic[i * 3 + k] = s.conic[i * 3 + k];
+            rgb[i * 3 + k] = s.rgb[i * 3 + k];
         }
-        float dT0[3], dT1[3];
-        for (int col = 0; col < 3; ++col) {
-            dT0[col] = 2.0f * (G00 * TS0[col] + G01 * TS1[col]);
-            dT1[col] = 2.0f * (G01 * TS0[col] + G11 * TS1[col]);
-        }
-        float dM[9];
-        for (int rr = 0; rr < 3; ++rr)
-            for (int col = 0; col < 3; ++col)
-                dM[rr * 3 + col] = 2.0f * (dSigma[rr * 3] * p.M[col] + dSigma[rr * 3 + 1] * p.M[3 + col]
-                                           + dSigma[rr * 3 + 2] * p.M[6 + col]);
-        float dRq[9];
-        for (int col = 0; col < 3; ++col) {
-            float ds = 0;
-            for (int rr = 0; rr < 3; ++rr) {
-                ds += dM[rr * 3 + col] * p.Rq[rr * 3 + col];
-                dRq[rr * 3 + col] = dM[rr * 3 + col] * p.s[col];
-            }
-            grads->scales[i * 3 + col] += ds * p.s[col];
-        }
-        const float w = p.q[0], x = p.q[1], y = p.q[2], zq = p.q[3];
-        const float dw = 2 * (-zq * dRq[1] + y * dRq[2] + zq * dRq[3] - x * dRq[5] - y * dRq[6] + x * dRq[7]);
-        const float dx = 2 * (y * dRq[1] + zq * dRq[2] + y * dRq[3] - 2 * x * dRq[4] - w * dRq[5]
-                              + zq * dRq[6] + w * dRq[7] - 2 * x * dRq[8]);
-        const float dy = 2 * (-2 * y * dRq[0] + x * dRq[1] + w * dRq[2] + x * dRq[3] + zq * dRq[5]
-                              - w * dRq[6] + zq * dRq[7] - 2 * y * dRq[8]);
-        const float dz = 2 * (-2 * zq * dRq[0] - w * dRq[1] + x * dRq[2] + w * dRq[3] - 2 * zq * dRq[4]
-                              + y * dRq[5] + x * dRq[6] + y * dRq[7]);
-        const float dq[4] = {dw, dx, dy, dz};
-        const float qdot = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
-        for (int k = 0; k < 4; ++k)
-            grads->quats[i * 4 + k] += (dq[k] - qdot * p.q[k]) / p.qNorm;
-        const float *R = cam.R;
-        const float dJ00 = dT0[0] * R[0] + dT0[1] * R[1] + dT0[2] * R[2];
-        const float dJ02 = dT0[0] * R[6] + dT0[1] * R[7] + dT0[2] * R[8];
-        const float dJ11 = dT1[0] * R[3] + dT1[1] * R[4] + dT1[2] * R[5];
-        const float dJ12 = dT1[0] * R[6] + dT1[1] * R[7] + dT1[2] * R[8];
-        const float z2 = z * z, z3 = z2 * z;
-        float dpc[3] = {0, 0, 0};
-        dpc[2] += dJ00 * -cam.fx / z2 + dJ02 * 2 * cam.fx * p.tx / z3 + dJ11 * -cam.fy / z2
-                  + dJ12 * 2 * cam.fy * p.ty / z3;
-        const float dtx = dJ02 * -cam.fx / z2;
-        const float dty = dJ12 * -cam.fy / z2;
-        if (p.clampX)
-            dpc[2] += dtx * p.limTx;
-        else
-            dpc[0] += dtx;
-        if (p.clampY)
-            dpc[2] += dty * p.limTy;
-        else
-            dpc[1] += dty;
-        dpc[0] += dmx * cam.fx / z;
-        dpc[1] += dmy * cam.fy / z;
-        dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
-        for (int k = 0; k < 3; ++k)
-            grads->means[i * 3 + k] += R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2];
-        if (nb) {
-            const float dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
-            for (int k = 0; k < 3; ++k)
-                grads->means[i * 3 + k] += (dDir[k] - dot * viewD[k]) / viewLen;
-        }
+        alpha[i] = s.alpha[i];
     }
+}
+
+// The chain-rule stage in FP64 from given 2D gradients (e.g. orc_raster_get_grad2d after a
+// backward) and the last forward's state; outputs (zeroed by the caller) are added to.
+void orc_chain_rule_f64(const orc_raster *r, const orc_model *m, const float *rest, int degree,
+                        const float *grad2d, double *g_means, double *g_scales, double *g_quats,
+                        double *g_opacities, double *g_colors, double *g_rest)
+{
+    chainRule<double>(r, m, rest, degree, grad2d, g_means, g_scales, g_quats, g_opacities, g_colors,
+                      g_rest, nullptr);
 }
 
 void orc_raster_backward(orc_raster *r, const orc_model *m, const float *dl, orc_model *grads,
@@ -1223,6 +1288,11 @@ struct orc_trainer {
     int statIteration = 0;
     float statLoss = 0, statPsnr = 0;
     long statGaussians = 0;
+    // SH extension (no reference counterpart): degree, rate scale, coefficients [n][15][3] and
+    // their gradient and Adam moments.
+    int shDegree = 0;
+    float shLrScale = 0.05f;
+    std::vector<float> shRest, shGrad, shM, shV;
 
     void loadTarget(size_t k)
     {
@@ -1264,10 +1334,48 @@ orc_trainer *orc_trainer_new(const orc_model *initial, const orc_camera *cams, c
     return t;
 }
 
+void orc_trainer_set_sh(orc_trainer *t, int degree, float lr_scale, const float *rest)
+{
+    t->shDegree = degree;
+    t->shLrScale = lr_scale > 0.0f ? lr_scale : 1.0f / 20.0f;
+    const size_t len = t->model.size() * 45;
+    t->shRest.assign(len, 0.0f);
+    if (rest)
+        t->shRest.assign(rest, rest + len);
+    t->shM.assign(len, 0.0f);
+    t->shV.assign(len, 0.0f);
+}
+
+void orc_trainer_get_sh(const orc_trainer *t, float *rest)
+{
+    std::copy(t->shRest.begin(), t->shRest.end(), rest);
+}
+
 void orc_trainer_free(orc_trainer *t) { delete t; }
 int orc_trainer_done(const orc_trainer *t) { return t->iteration >= t->opt.iterations; }
 int orc_trainer_iteration(const orc_trainer *t) { return t->iteration; }
 int orc_trainer_size(const orc_trainer *t) { return int(t->model.size()); }
+
+// Rows of a [n][width] array after densification: kept rows in order, then one row per clone
+// and two per split in index order, copied from the parent (or zero when zeroExtras).
+static void densifyRows(std::vector<float> &a, size_t width, const std::vector<uint8_t> &keep,
+                        const std::vector<uint8_t> &kind, bool zeroExtras)
+{
+    std::vector<float> out, extras;
+    for (size_t i = 0; i < keep.size(); ++i) {
+        const float *row = &a[i * width];
+        if (keep[i])
+            out.insert(out.end(), row, row + width);
+        for (int c = 0; c < (kind[i] == 1 ? 1 : kind[i] == 2 ? 2 : 0); ++c) {
+            if (zeroExtras)
+                extras.insert(extras.end(), width, 0.0f);
+            else
+                extras.insert(extras.end(), row, row + width);
+        }
+    }
+    out.insert(out.end(), extras.begin(), extras.end());
+    a.swap(out);
+}
 
 static void trainerDensify(orc_trainer *t)
 {
@@ -1278,6 +1386,11 @@ static void trainerDensify(orc_trainer *t)
     densifyLoop(mv, t->gradAccum.data(), t->gradCount.data(), t->extent, t->opt.max_gaussians,
                 t->opt.densify_grad_threshold, t->opt.percent_dense, t->opt.prune_opacity,
                 t->opacityWasReset, [&]() { return normal(t->rng); }, keep, kind, extra);
+    if (t->shDegree > 0) {
+        densifyRows(t->shRest, 45, keep, kind, false);
+        densifyRows(t->shM, 45, keep, kind, true);
+        densifyRows(t->shV, 45, keep, kind, true);
+    }
     t->model.compact(keep);
     t->adamM.compact(keep);
     t->adamV.compact(keep);
@@ -1315,14 +1428,24 @@ void orc_trainer_step(orc_trainer *t)
     t->loadTarget(view);
     t->render.resize(size_t(W) * size_t(H) * 3);
     orc_model mv = t->model.view();
-    orc_raster_forward(&t->raster, &mv, &cam, &t->opt.raster, t->render.data());
+    if (t->shDegree > 0)
+        orc_raster_forward_sh(&t->raster, &mv, t->shRest.data(), t->shDegree, &cam, &t->opt.raster,
+                              t->render.data());
+    else
+        orc_raster_forward(&t->raster, &mv, &cam, &t->opt.raster, t->render.data());
     t->dLoss.resize(t->render.size());
     const orc_loss_value loss = orc_photometric_loss(t->render.data(), t->target.data(), W, H,
                                                      t->opt.ssim_lambda, t->dLoss.data());
     t->grads.resize(t->model.size());
     t->grads.zero();
     orc_model gv = t->grads.view();
-    orc_raster_backward(&t->raster, &mv, t->dLoss.data(), &gv, t->gradAccum.data());
+    if (t->shDegree > 0) {
+        t->shGrad.assign(t->shRest.size(), 0.0f);
+        orc_raster_backward_sh(&t->raster, &mv, t->shRest.data(), t->shDegree, t->dLoss.data(), &gv,
+                               t->shGrad.data(), t->gradAccum.data());
+    } else {
+        orc_raster_backward(&t->raster, &mv, t->dLoss.data(), &gv, t->gradAccum.data());
+    }
     for (size_t i = 0; i < t->raster.s.radius.size(); ++i)
         if (t->raster.s.radius[i] > 0)
             ++t->gradCount[i];
@@ -1331,6 +1454,9 @@ void orc_trainer_step(orc_trainer *t)
     lr.means = std::exp(std::log(t->opt.lr.means) * (1 - tt) + std::log(t->opt.means_lr_final) * tt) * t->extent;
     orc_model am = t->adamM.view(), av = t->adamV.view();
     orc_adam_step(&mv, &gv, &am, &av, &lr, ++t->adamStep);
+    if (t->shDegree > 0)
+        orc_adam_array(t->shRest.data(), t->shGrad.data(), t->shM.data(), t->shV.data(),
+                       long(t->shRest.size()), t->opt.lr.colors * t->shLrScale, t->adamStep);
     ++t->iteration;
     const float p = orc_psnr(t->render.data(), t->target.data(), long(t->render.size()));
     const float k = t->statIteration == 0 ? 1.0f : 0.05f;
@@ -1379,7 +1505,11 @@ float orc_trainer_evaluate_psnr(orc_trainer *t)
     for (size_t k = 0; k < t->images.size(); ++k) {
         t->loadTarget(k);
         rgb.resize(t->target.size());
-        orc_raster_forward(&t->raster, &mv, &t->cams[k], &t->opt.raster, rgb.data());
+        if (t->shDegree > 0)
+            orc_raster_forward_sh(&t->raster, &mv, t->shRest.data(), t->shDegree, &t->cams[k],
+                                  &t->opt.raster, rgb.data());
+        else
+            orc_raster_forward(&t->raster, &mv, &t->cams[k], &t->opt.raster, rgb.data());
         sum += orc_psnr(rgb.data(), t->target.data(), long(rgb.size()));
     }
     return float(sum / double(t->images.size()));
@@ -1396,7 +1526,11 @@ float orc_trainer_evaluate_l1(orc_trainer *t)
     for (size_t k = 0; k < t->images.size(); ++k) {
         t->loadTarget(k);
         rgb.resize(t->target.size());
-        orc_raster_forward(&t->raster, &mv, &t->cams[k], &t->opt.raster, rgb.data());
+        if (t->shDegree > 0)
+            orc_raster_forward_sh(&t->raster, &mv, t->shRest.data(), t->shDegree, &t->cams[k],
+                                  &t->opt.raster, rgb.data());
+        else
+            orc_raster_forward(&t->raster, &mv, &t->cams[k], &t->opt.raster, rgb.data());
         double s = 0;
         for (size_t i = 0; i < rgb.size(); ++i)
             s += std::fabs(double(rgb[i]) - double(t->target[i]));

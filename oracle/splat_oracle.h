@@ -99,6 +99,18 @@ void orc_raster_get_state(const orc_raster *r, int *radius, int *tile_offsets, i
 void orc_raster_get_projection(const orc_raster *r, float *mean2d, float *conic, float *depth,
                                float *rgb, float *alpha, int *rect, uint8_t *flags);
 void orc_raster_get_grad2d(const orc_raster *r, float *grad2d); // [9n] after backward
+// Projection (Rasterizer.cpp:137-204) in FP64 of double-precision parameters: mean2d [2n],
+// conic [3n], clamped rgb [3n], alpha [n] where radius [n] > 0.
+void orc_project_f64(int n, const double *means, const double *scales, const double *quats,
+                     const double *opacities, const double *colors, const double *rest, int degree,
+                     const orc_camera *camera, const orc_settings *st, double *mean2d,
+                     double *conic, double *rgb, double *alpha, int *radius);
+// The backward's chain-rule stage (Rasterizer.cpp:432-544) in FP64, from the given 2D gradients
+// [9n] and the last forward's state; the outputs (zeroed by the caller) are added to. g_rest may
+// be NULL; rest is NULL for degree 0.
+void orc_chain_rule_f64(const orc_raster *r, const orc_model *m, const float *rest, int degree,
+                        const float *grad2d, double *g_means, double *g_scales, double *g_quats,
+                        double *g_opacities, double *g_colors, double *g_rest);
 
 // ---- FP64 forward with the same algorithm (finite-difference oracle). Model as doubles.
 void orc_render_f64(int n, const double *means, const double *scales, const double *quats,
@@ -136,11 +148,16 @@ void orc_init_from_points(int n, const float *xyz, const float *rgb, int rgb_cou
                           orc_model *out);
 
 // ---- trainer (Trainer.cpp:37-197). Images: cams[n_images], rgb8 concatenated HWC uint8.
+// orc_trainer_set_sh (extension, call before the first step): SH degree 1..3 coefficients
+// starting at rest (NULL: zero), trained with rate lr.colors * lr_scale (0 -> 1/20); orc_trainer_get_sh
+// copies them out ([size][15][3]).
 typedef struct orc_trainer orc_trainer;
 orc_trainer *orc_trainer_new(const orc_model *initial, const orc_camera *cams,
                              const uint8_t *rgb8, int n_images, float extent,
                              const orc_train_options *opt);
 void orc_trainer_free(orc_trainer *t);
+void orc_trainer_set_sh(orc_trainer *t, int degree, float lr_scale, const float *rest);
+void orc_trainer_get_sh(const orc_trainer *t, float *rest);
 void orc_trainer_step(orc_trainer *t);
 int orc_trainer_done(const orc_trainer *t);
 int orc_trainer_iteration(const orc_trainer *t);

@@ -375,3 +375,127 @@ def test_init_from_points_brute_force_3nn():
     np.testing.assert_allclose(out.scales.reshape(-1, 3)[:, 0], 0.5 * np.log(best), rtol=1e-4, atol=1e-4)
     assert np.all(out.quats.reshape(-1, 4) == [1, 0, 0, 0])
     assert np.allclose(out.opacities, math.log(0.1 / 0.9))
+
+
+# ------------------------------------------------------------------ SH extension of the trainer
+def _sh_trainer_scene():
+    from scenes import config0_scene
+    truth, cams = config0_scene(1500, 31, 96, 72, 6)
+    st = O.make_settings()
+    rest_t = S.rng(32).normal(0, 0.05, 1500 * 45).astype(np.float32)
+    ro = O.Raster()
+    imgs = [S.quantize(ro.forward(O.Gaussians(**truth), c, st, sh_rest=rest_t, sh_degree=3)) for c in cams]
+    init, _ = config0_scene(1500, 33, 96, 72, 6)
+    return init, cams, imgs
+
+
+def test_sh_trainer_first_step_is_adams_closed_form():
+    """One step of the SH-3 oracle trainer from zero coefficients: the geometry and DC colour
+    take exactly the plain trainer's step (zero coefficients add nothing to the colour or to the
+    mean gradient), and every coefficient moves by Adam's first step -lr * sign(g), where g is
+    the oracle rasterizer's SH gradient of the loss of that view."""
+    init, cams, imgs = _sh_trainer_scene()
+    cams, imgs = cams[2:3], imgs[2:3]  # one view: the step's view is fixed
+    opt = O.options_for_iterations(100)
+    opt.refine_start = 10 ** 6
+    t0 = O.Trainer(O.Gaussians(**init), cams, imgs, 3.0, opt)
+    t3 = O.Trainer(O.Gaussians(**init), cams, imgs, 3.0, opt, sh_degree=3, sh_lr_scale=0.05)
+    t0.step()
+    t3.step()
+    m0, m3 = t0.model(), t3.model()
+    for f, _ in O.FIELDS:
+        assert np.array_equal(getattr(m0, f), getattr(m3, f)), f
+    # the view's SH gradient, from the rasterizer and loss directly
+    g = O.Gaussians(**init)
+    ro = O.Raster()
+    zero = np.zeros(1500 * 45, np.float32)
+    img = ro.forward(g, cams[0], O.make_settings(), sh_rest=zero, sh_degree=3)
+    target = imgs[0].reshape(-1).astype(np.float32) / np.float32(255)
+    _, dl = O.photometric_loss(img, target, cams[0]["width"], cams[0]["height"], 0.2)
+    _, grest = ro.backward_sh(g, dl, zero, 3)
+    rest = t3.sh_rest()
+    lr = np.float32(opt.lr.colors) * np.float32(0.05)
+    moved = grest != 0
+    assert moved.sum() > 1000
+    assert np.all(rest[~moved] == 0)
+    # first Adam step: m = 0.1 g, v = 0.001 g^2, bias corrections 0.1 and 0.001, so the update is
+    # lr * g / (|g| + 1e-15) (Adam.cpp:23-25)
+    g64 = grest[moved].astype(np.float64)
+    np.testing.assert_allclose(rest[moved], -float(lr) * g64 / (np.abs(g64) + 1e-15), rtol=1e-5)
+
+
+def test_sh_trainer_with_negligible_rate_is_the_plain_trainer():
+    """At an SH rate of 1e-30 the coefficients stay too small to change any float sum, so the
+    SH-3 oracle trainer must reproduce the plain trainer bit for bit over a schedule with
+    densification and an opacity reset (the coefficient rows follow the compaction and the
+    appended clones and split children)."""
+    init, cams, imgs = _sh_trainer_scene()
+    opt = O.options_for_iterations(60)
+    opt.refine_start, opt.refine_stop, opt.refine_interval = 10, 50, 10
+    opt.opacity_reset_interval = 25
+    opt.densify_grad_threshold = 5e-5
+    opt.max_gaussians = 2200
+    t0 = O.Trainer(O.Gaussians(**init), cams, imgs, 3.0, opt)
+    t3 = O.Trainer(O.Gaussians(**init), cams, imgs, 3.0, opt, sh_degree=3, sh_lr_scale=1e-30)
+    sizes = set()
+    while not t0.done():
+        t0.step()
+        t3.step()
+        assert t0.size() == t3.size()
+        sizes.add(t0.size())
+    assert len(sizes) > 2  # densification changed the size
+    m0, m3 = t0.model(), t3.model()
+    for f, _ in O.FIELDS:
+        assert np.array_equal(getattr(m0, f), getattr(m3, f)), f
+    assert t3.sh_rest().size == t3.size() * 45
+
+
+def test_fp64_chain_rule_matches_finite_differences_of_the_fp64_projection():
+    """The FP64 chain rule of the oracle (the backward's second stage, Rasterizer.cpp:432-544,
+    evaluated in double) is the gradient of F = sum_i g_i . proj_i, the 2D quantities of the
+    projection (mean2d, conic, rgb, alpha) weighted by given 2D gradients g_i. Central
+    differences of the FP64 projection in every parameter (including the SH coefficients, which
+    also move the colour's view direction terms through the mean) must reproduce it."""
+    from scenes import config0_scene
+    n = 24
+    g, cams = config0_scene(n, 51, 160, 120, 4)
+    g["scales"] = (g["scales"] + np.float32(2.0)).astype(np.float32)  # visible footprints
+    rest = S.rng(52).normal(0, 0.05, n * 45).astype(np.float32)
+    cam, st = cams[1], O.make_settings()
+    ro = O.Raster()
+    ro.forward(O.Gaussians(**g), cam, st, sh_rest=rest, sh_degree=3)
+    vis = ro.state()["radius"] > 0
+    assert vis.sum() >= 12
+    w = S.rng(53).normal(size=n * 9).astype(np.float32)
+    got = ro.chain_rule_f64(O.Gaussians(**g), w, rest, 3)
+    wv = w.reshape(n, 9).astype(np.float64)
+
+    def F(params, rst):
+        p = O.project_f64(params, cam, st, rst, 3)
+        mask = vis & (p["radius"] > 0)
+        assert np.array_equal(mask, vis)
+        terms = (wv[:, 0:2] * p["mean2d"].reshape(n, 2)).sum(1) + (wv[:, 2:5] * p["conic"].reshape(n, 3)).sum(1)
+        terms += (wv[:, 5:8] * p["rgb"].reshape(n, 3)).sum(1) + wv[:, 8] * p["alpha"]
+        return terms[vis].sum()
+
+    base = {k: v.astype(np.float64) for k, v in g.items()}
+    rst = rest.astype(np.float64)
+    h = 1e-6
+    for f, d in O.FIELDS + (("sh", 45),):
+        fd = np.zeros(n * d)
+        for i in np.nonzero(vis)[0]:
+            for k in range(d):
+                j = i * d + k
+                plus = {kk: vv.copy() for kk, vv in base.items()}
+                minus = {kk: vv.copy() for kk, vv in base.items()}
+                rp, rm = rst.copy(), rst.copy()
+                if f == "sh":
+                    rp[j] += h
+                    rm[j] -= h
+                else:
+                    plus[f][j] += h
+                    minus[f][j] -= h
+                fd[j] = (F(plus, rp) - F(minus, rm)) / (2 * h)
+        mask = np.repeat(vis, d)
+        err = np.linalg.norm(got[f][mask] - fd[mask]) / np.linalg.norm(fd[mask])
+        assert err < 1e-6, (f, err)

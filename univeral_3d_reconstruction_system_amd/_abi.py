@@ -100,6 +100,9 @@ SIGNATURES = {
                                   C.POINTER(RasterSettings), P]),
     "u3d_raster_backward_sh": (I, [P, C.POINTER(Gaussians), P, I, P, C.POINTER(Gaussians), P, P]),
     "u3d_trainer_get_sh_rest": (I, [P, P]),
+    "u3d_trainer_set_sh_rest": (I, [P, P]),
+    "u3d_trainer_get_sh_moments": (I, [P, P, P]),
+    "u3d_trainer_get_densify_stats": (I, [P, P, P]),
     "u3d_raster_radii": (I, [P, C.POINTER(P), C.POINTER(I64)]),
     "u3d_raster_get_state": (I, [P, C.POINTER(RasterState)]),
     "u3d_raster_get_grad2d": (I, [P, C.POINTER(P)]),
@@ -166,6 +169,7 @@ STAGES = ("project", "sort", "raster_fwd", "loss", "raster_bwd", "chain_rule", "
           "graph_step")
 U3D_NUM_STAGES = len(STAGES)
 OPT_SEGMENTED_BACKWARD = 1
+OPT_SORT_SPIN_LIMIT = 2
 FEATURE_DIM = 80
 NET_OUT = 16
 NET_LAYERS = ((80, 64), (64, 256), (256, 256), (256, 256), (256, 16))  # (in, out)

@@ -25,8 +25,12 @@ int failHip(hipError_t e, const char *what, const char *file, int line)
     return U3D_ERR_HIP;
 }
 
-int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value)
+int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value, bool *flag)
 {
+    // the word is (flag << 63 | seq << 32 | value) with a 31-bit seq
+    auto seqOf = [](uint64_t w) { return uint32_t(w >> 32) & 0x7fffffffu; };
+    if (flag)
+        *flag = false;
     // Tight polling for the first millisecond (the usual wait is tens of microseconds); after
     // that the stream is also asked for an error about every 100 microseconds.
     using Clock = std::chrono::steady_clock;
@@ -34,8 +38,10 @@ int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *val
     Clock::time_point nextQuery = start + std::chrono::milliseconds(1);
     for (uint32_t polls = 1;; ++polls) {
         const uint64_t w = __atomic_load_n(p, __ATOMIC_ACQUIRE);
-        if (uint32_t(w >> 32) == seq) {
+        if (seqOf(w) == seq) {
             *value = uint32_t(w);
+            if (flag)
+                *flag = (w >> 63) != 0;
             return U3D_OK;
         }
         __builtin_ia32_pause();
@@ -46,9 +52,11 @@ int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *val
         if (e == hipSuccess) {
             // the stream has drained, so the word was stored unless something went wrong
             const uint64_t w2 = __atomic_load_n(p, __ATOMIC_ACQUIRE);
-            U3D_CHECK(uint32_t(w2 >> 32) == seq, U3D_ERR_HIP,
+            U3D_CHECK(seqOf(w2) == seq, U3D_ERR_HIP,
                       "the device-side pair count never arrived");
             *value = uint32_t(w2);
+            if (flag)
+                *flag = (w2 >> 63) != 0;
             return U3D_OK;
         }
         if (e != hipErrorNotReady)
@@ -113,7 +121,9 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     // the device stores (seq << 32 | K) there; seq is never 0, the word's value until then
     uint64_t *pinnedWord = static_cast<uint64_t *>(r->pinnedTotal.p);
     __atomic_store_n(pinnedWord, uint64_t(0), __ATOMIC_RELEASE);
-    r->totalSeq = r->totalSeq + 1u == 0u ? 1u : r->totalSeq + 1u;
+    r->totalSeq = (r->totalSeq + 1u) & 0x7fffffffu; // 31 bits, never 0
+    if (r->totalSeq == 0u)
+        r->totalSeq = 1u;
     const uint32_t seq = r->totalSeq;
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
@@ -134,6 +144,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         rs.onesweep = r->sortScratch.p;
         rs.onesweepCap = r->sortScratch.cap;
         rs.onesweepEpoch = &r->sortEpoch;
+        rs.spinLimit = r->ctx->sortSpinLimit;
     }
     // The pair count K is known after the projection; it is summed with the sort's first
     // launch and the host reads it (pinned memory) while the depth sort runs, so the device
@@ -149,8 +160,15 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
     const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
     uint32_t K = 0;
+    bool sortFailed = false;
     if (n > 0)
-        U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K));
+        U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K, &sortFailed));
+    // An earlier depth sort of this rasterizer gave up a lookback wait (its kernels stayed in
+    // bounds, but that forward's order, image and gradients were wrong): report it here, the
+    // first point where the host reads the device without an extra wait.
+    U3D_CHECK(!sortFailed, U3D_ERR_HIP,
+              "depth sort: a lookback wait of an earlier forward on this rasterizer was abandoned; "
+              "its image and gradients are invalid");
     r->numPairs = int64_t(K);
     const size_t kk = size_t(K > 0 ? K : 1);
     const int *order = nullptr; // heavy-first tile order for the forward (binning path)
@@ -163,12 +181,21 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, r->ntSorted.p,
                                   r->rectSorted.p, tilesX, tilesY, r->binCounts.p,
                                   r->binCounts.cap, r->tileTotal.p, r->tileOffsets.p,
-                                  r->tileListBuf.p, r->heavyFirst.p));
+                                  r->tileListBuf.p, kk, r->heavyFirst.p,
+                                  onesweepErrorPtr(r->sortScratch.p)));
         order = n > 0 ? r->heavyFirst.p : nullptr;
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {
         // Many tiles: pairs emitted in depth order, then a stable radix sort on the tile id.
-        U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n));
+        // This path has no in-kernel check of the depth sort, so its error word is read here
+        // (a synchronisation, on this rare path only).
+        if (n > 0) {
+            uint32_t sortError = 0;
+            U3D_TRY(onesweepErrorWord(r->sortScratch.p, s, &sortError));
+            U3D_CHECK(sortError == 0, U3D_ERR_HIP,
+                      "depth sort: a lookback wait was abandoned (the order is not valid)");
+        }
+        U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n, n));
         U3D_TRY(exclusiveScanU32(s, r->ntSorted.p, r->pairOffsets.p, n, r->total.p,
                                  r->scanScratch.p, r->scanScratch.cap));
         U3D_TRY(r->pairTile.ensure(kk));
@@ -179,7 +206,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(kk))));
         RadixScratch rs2{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
         U3D_TRY(launchEmitPairs(s, n, sorted, r->pairOffsets.p, r->rect.p, tilesX, r->pairTile.p,
-                                r->pairGauss.p, r->ntSorted.p));
+                                r->pairGauss.p, r->ntSorted.p, int64_t(K)));
         bool alt2 = false;
         U3D_TRY(radixSortPairs(s, r->pairTile.p, r->pairGauss.p, r->pairTileAlt.p,
                                r->pairGaussAlt.p, int64_t(K), bitsFor(tiles), rs2, &alt2));
@@ -1149,6 +1176,51 @@ int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host)
     return U3D_OK;
 }
 
+int u3d_trainer_set_sh_rest(u3d_trainer *t, const float *sh_rest_host)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
+    U3D_TRY(useDevice(t->ctx));
+    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
+    if (bytes) {
+        U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null input");
+        U3D_HIP(hipMemcpyAsync(t->shP.p, sh_rest_host, bytes, hipMemcpyHostToDevice, t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
+    return U3D_OK;
+}
+
+int u3d_trainer_get_sh_moments(u3d_trainer *t, float *m_host, float *v_host)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
+    U3D_TRY(useDevice(t->ctx));
+    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
+    if (bytes) {
+        U3D_CHECK(m_host && v_host, U3D_ERR_INVALID, "null output");
+        U3D_HIP(hipMemcpyAsync(m_host, t->shM.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+        U3D_HIP(hipMemcpyAsync(v_host, t->shV.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
+    return U3D_OK;
+}
+
+int u3d_trainer_get_densify_stats(u3d_trainer *t, float *accum_host, int *count_host)
+{
+    U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
+    U3D_TRY(useDevice(t->ctx));
+    const size_t n = size_t(t->params.n);
+    if (n) {
+        U3D_CHECK(accum_host && count_host, U3D_ERR_INVALID, "null output");
+        U3D_HIP(hipMemcpyAsync(accum_host, t->gradAccum.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                               t->ctx->stream));
+        U3D_HIP(hipMemcpyAsync(count_host, t->gradCount.p, n * sizeof(int), hipMemcpyDeviceToHost,
+                               t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
+    return U3D_OK;
+}
+
 int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out)
 {
     // TrainStats EMA (Trainer.cpp:88-92), folded lazily from the per-step device values.
@@ -1250,6 +1322,9 @@ int u3d_ctx_set_option(u3d_ctx *ctx, int option, int value)
     switch (option) {
     case U3D_OPT_SEGMENTED_BACKWARD:
         ctx->segmentedBackward = value != 0;
+        return U3D_OK;
+    case U3D_OPT_SORT_SPIN_LIMIT:
+        ctx->sortSpinLimit = value;
         return U3D_OK;
     default:
         U3D_CHECK(false, U3D_ERR_INVALID, "unknown option");

@@ -500,7 +500,7 @@ int BatchEngine::enqueue()
         bool alt = false;
         U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs, &alt));
         const uint32_t *sorted = alt ? valAlt.p : valA.p;
-        U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries));
+        U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries, totalEntries));
         U3D_TRY(exclusiveScanU32(s, ntSorted.p, pairOffsets.p, totalEntries, total.p,
                                  scanScratch.p, scanScratch.cap));
         U3D_TRY(launchEmitPairsBatched(s, totalEntries, sorted, pairOffsets.p, rect.p, entryItem.p,

@@ -122,11 +122,12 @@ struct PinnedScratch {
     }
 };
 
-// Waits until the pinned word *p carries seq in its high half (stored by a device-side
-// system-scope release) and returns its low half. Polls without any event on the stream; every
-// few thousand polls it asks the stream for an error, and fails if the stream has drained or
-// faulted without the word arriving.
-int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value);
+// Waits until the pinned word *p carries the 31-bit seq in bits 32-62 (stored by a device-side
+// system-scope release) and returns its low half, and bit 63 in *flag (nullable). Polls without
+// any event on the stream; every few thousand polls it asks the stream for an error, and fails
+// if the stream has drained or faulted without the word arriving.
+int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value,
+                   bool *flag = nullptr);
 
 // Per-stage GPU time accumulated from HIP events recorded on the context's stream.
 struct StageTimer {
@@ -185,6 +186,7 @@ struct u3d_ctx {
     u3d::DevBuf<unsigned char> scratch; // shared short-lived device scratch (loss partials, ...)
     u3d::StageTimer timer;
     bool segmentedBackward = true; // U3D_OPT_SEGMENTED_BACKWARD
+    int sortSpinLimit = 0;         // U3D_OPT_SORT_SPIN_LIMIT (0: the default bound)
 };
 
 namespace u3d {

@@ -70,7 +70,7 @@ int buildGrid(u3d_ctx *ctx, PointScratch &ps, const float *xyz, int64_t n)
     uint32_t *order = alt ? ps.idxAlt.p : ps.idx.p;
     uint32_t *loSorted = alt ? ps.keyLoAlt.p : ps.keyLo.p;
     uint32_t *hiPerm = alt ? ps.keyLo.p : ps.keyLoAlt.p; // free buffer
-    U3D_TRY(launchGatherU32(s, ps.keyHi.p, order, hiPerm, n));
+    U3D_TRY(launchGatherU32(s, ps.keyHi.p, order, hiPerm, n, n));
     // (hi, position in the low-sorted order) -> stable sort by hi
     uint32_t *pos = alt ? ps.idx.p : ps.idxAlt.p; // free buffer
     U3D_TRY(launchIota(s, pos, n));
@@ -82,8 +82,8 @@ int buildGrid(u3d_ctx *ctx, PointScratch &ps, const float *xyz, int64_t n)
     // final arrays: lo[posSorted], order[posSorted]
     uint32_t *loFinal = alt2 ? pos : posOut;
     uint32_t *idxFinal = alt2 ? hiPerm : hiOut;
-    U3D_TRY(launchGatherU32(s, loSorted, posSorted, loFinal, n));
-    U3D_TRY(launchGatherU32(s, order, posSorted, idxFinal, n));
+    U3D_TRY(launchGatherU32(s, loSorted, posSorted, loFinal, n, n));
+    U3D_TRY(launchGatherU32(s, order, posSorted, idxFinal, n, n));
     ps.grid = PointGrid{hiSorted, loFinal, idxFinal};
     return U3D_OK;
 }

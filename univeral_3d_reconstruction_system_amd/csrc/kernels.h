@@ -258,10 +258,17 @@ struct RadixScratch {
     uint32_t *sideDev = nullptr;
     uint64_t *sideHost = nullptr;
     uint32_t sideSeq = 0;
+    // Bound on a small-sort pass's lookback re-reads before it gives up and sets the error word
+    // (0: the default, far beyond any wait that ends; negative: every wait is given up at once,
+    // a test hook for the error path).
+    int spinLimit = 0;
 };
 size_t onesweepScratchWords();
-// The small sort's error word (non-zero once a lookback wait was abandoned; synchronous read).
-int onesweepErrorWord(const uint64_t *scratch, hipStream_t s, uint32_t *value);
+// The small sort's error word (non-zero once a lookback wait was abandoned): synchronous read,
+// then cleared. A histogram launch with a side sum to a pinned word also reports and clears it
+// (bit 63 of that word).
+int onesweepErrorWord(uint64_t *scratch, hipStream_t s, uint32_t *value);
+const uint32_t *onesweepErrorPtr(const uint64_t *scratch);
 int radixSortPairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt,
                    uint32_t *valsAlt, int64_t count, int bits, RadixScratch &scratch,
                    bool *resultInAlt, const uint32_t *countDev = nullptr);
@@ -269,9 +276,9 @@ size_t radixHistSize(int64_t count);
 size_t radixScanScratchSize(int64_t count);
 // values[i] = i
 int launchIota(hipStream_t s, uint32_t *vals, int64_t count);
-// out[i] = in[idx[i]] for i < count
+// out[i] = in[idx[i]] for i < count (0 where idx[i] >= inCount)
 int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
-                    int64_t count);
+                    int64_t count, int64_t inCount);
 // Sum of count u32 values (one workgroup) to *devOut and, if given, to *hostOut (pinned host
 // memory mapped for the device) as the word (seq << 32 | sum), published by one system-scope
 // release store; waitPinnedWord() reads it without an event on the stream.
@@ -290,16 +297,19 @@ int tileBinningGroup(int64_t n, int tilesX, int tilesY); // Gaussians per group,
 size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
 // ntS, rectS (n entries each): scratch for the tile counts and rectangles in depth order.
 // heavyFirst (optional, tiles entries): the tiles by decreasing list length, written unless
-// n <= 0.
+// n <= 0. Ids sorted[i] >= n count as invisible; list slots and offsets stay below listCapacity.
+// sortError (nullable): the small sort's error word (onesweepErrorPtr); when set, every
+// Gaussian counts as invisible.
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
-                      uint32_t *tileList, int *heavyFirst = nullptr);
+                      uint32_t *tileList, size_t listCapacity, int *heavyFirst = nullptr,
+                      const uint32_t *sortError = nullptr);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
-// offsets[i] its first pair slot.
+// offsets[i] its first pair slot; slots at or past capacity are not written.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
                     const int4 *rect, int tilesX, uint32_t *pairTile, uint32_t *pairGauss,
-                    const uint32_t *nTilesSorted);
+                    const uint32_t *nTilesSorted, int64_t capacity);
 // Batched emission: tile ids are global (item tileBase + local tile), values are screen entries.
 // Pairs at or past `capacity` are dropped (the farthest in depth order) and *overflow is set.
 int launchEmitPairsBatched(hipStream_t s, int64_t n, const uint32_t *sorted,

@@ -167,6 +167,31 @@ struct GradRegs {
     __device__ void color(int k, float v) { colors[k] += v; }
 };
 
+// Backward of the projection for one Gaussian (the quantities of Rasterizer.cpp:432-544, derived
+// here in matrix form). With U = T M (the Gaussian's axes after the camera and the local affine
+// map T = J W; M = R(q) diag(exp s)), the screen covariance is S = U U^T + eps2d I and the conic
+// is Q = S^-1. Backward:
+//   dS = -Q dQ Q                 (dQ symmetric; the off-diagonal conic entry B counts twice)
+//   dU = 2 dS U,  dT = dU M^T,  dM = T^T dU
+//   d log s_k = s_k sum_r dM[r][k] R[r][k],  dR = dM diag(s)
+//   unit quaternion (w, v): R = (1 - 2|v|^2) I + 2 v v^T + 2 w [v]x, so with
+//   a = (dR21 - dR12, dR02 - dR20, dR10 - dR01):
+//     dw = 2 v.a,  dv = 2 (dR + dR^T) v - 4 tr(dR) v + 2 w a
+//   raw quaternion: dq = (dq_unit - (dq_unit . q_unit) q_unit) / |q|
+//   T = J W: dJ = dT W^T; J's entries fx/z, -fx tx/z^2 (and y) give the camera-space mean
+//   (tx = x, or z * bound when clamped), plus the projected mean u = fx x/z + cx;
+//   world mean: d mu = W^T d pc.
+
+// Small row-major 2x3 helpers of the backward.
+struct Rows2x3 {
+    float r[2][3];
+};
+
+__device__ __forceinline__ float dot3(const float *a, const float *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
 // Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o.
 template <bool kSH, class Sink>
 __device__ __forceinline__ void chainBody(
@@ -185,21 +210,24 @@ __device__ __forceinline__ void chainBody(
 #pragma unroll
     for (int k = 0; k < kGrad2dStride; ++k)
         g[k] = grad2d[o * kGrad2dStride + k];
-    const float dmx = g[0], dmy = g[1], dA = g[2], dB = g[3], dC = g[4];
+    const float du = g[0], dv = g[1];
     if (ndc) {
-        const float nx = dmx * 0.5f * float(cam.width), ny = dmy * 0.5f * float(cam.height);
-        ndc[i] += sqrtf(nx * nx + ny * ny) * ndcScale;
+        // densification statistic: |dL/d mean2d| in NDC units
+        const float hx = du * 0.5f * float(cam.width), hy = dv * 0.5f * float(cam.height);
+        ndc[i] += sqrtf(hx * hx + hy * hy) * ndcScale;
     }
     const uint8_t flags = flagsArr[o];
+    // colour: rgb = max(0.5 + C0 c, 0); the clamp stops the gradient
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
         if (!(flags & (4 << ch)))
             out.color(ch, kShC0 * g[5 + ch]);
-    const float op = rec1[o].y;
-    out.opacity(g[8] * op * (1.0f - op));
+    // opacity: alpha = sigmoid(o), d alpha / d o = alpha (1 - alpha)
+    const float alpha = rec1[o].y;
+    out.opacity(g[8] * alpha * (1.0f - alpha));
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
-    // View-dependent colour: coefficient gradients and the direction -> mean term.
+    // View-dependent colour: coefficient gradients and the gradient w.r.t. the unit direction.
     const int nb = (kSH && shRest) ? shCount(shDegree) : 0;
     float dDir[3] = {0.0f, 0.0f, 0.0f}, viewD[3] = {0.0f, 0.0f, 0.0f}, viewLen = 1.0f;
     if (nb) {
@@ -215,9 +243,10 @@ __device__ __forceinline__ void chainBody(
                 const float coef = shRest[i * 45 + k * 3 + ch];
                 if (gShRest)
                     gShRest[i * 45 + k * 3 + ch] += b[k] * gc;
-                dDir[0] += coef * gb[k].x * gc;
-                dDir[1] += coef * gb[k].y * gc;
-                dDir[2] += coef * gb[k].z * gc;
+                const float w = coef * gc;
+                dDir[0] += w * gb[k].x;
+                dDir[1] += w * gb[k].y;
+                dDir[2] += w * gb[k].z;
             }
         }
     }
@@ -225,95 +254,105 @@ __device__ __forceinline__ void chainBody(
     const float ls[3] = {scales[i * 3], scales[i * 3 + 1], scales[i * 3 + 2]};
     Projection p;
     projectGaussian(mu, qr, ls, cam, nearPlane, eps2d, p);
-    const float z = p.pc[2];
-    const float a = p.a, b = p.b, c = p.c;
-    const float det = a * c - b * b;
-    const float det2 = det * det;
-    // Inverse-covariance gradient -> 2D covariance gradient.
-    const float da = (dA * -c * c + dB * b * c + dC * -b * b) / det2;
-    const float db = (dA * 2 * b * c + dC * 2 * a * b) / det2 + dB * (-1.0f / det - 2 * b * b / det2);
-    const float dc = (dA * -b * b + dB * a * b + dC * -a * a) / det2;
-    const float G00 = da, G01 = 0.5f * db, G11 = dc;
-    float dSigma[9];
+
+    // dS = -Q dQ Q with Q = S^-1 = [[c, -b], [-b, a]] / det
+    const float det = p.a * p.c - p.b * p.b;
+    const float q00 = p.c / det, q01 = -p.b / det, q11 = p.a / det;
+    const float h00 = g[2], h01 = 0.5f * g[3], h11 = g[4]; // dQ as a symmetric matrix
+    // P = dQ Q, then dS = -Q P
+    const float p00 = h00 * q00 + h01 * q01, p01 = h00 * q01 + h01 * q11;
+    const float p10 = h01 * q00 + h11 * q01, p11 = h01 * q01 + h11 * q11;
+    const float s00 = -(q00 * p00 + q01 * p10);
+    const float s01 = -(q00 * p01 + q01 * p11);
+    const float s11 = -(q01 * p01 + q11 * p11);
+
+    // U = T M (rows: the two screen axes of the Gaussian's three principal directions)
+    Rows2x3 T, U, dU, dT;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-            dSigma[k * 3 + l] = p.T0[k] * (G00 * p.T0[l] + G01 * p.T1[l])
-                                + p.T1[k] * (G01 * p.T0[l] + G11 * p.T1[l]);
-    float TS0[3], TS1[3];
-#pragma unroll
-    for (int col = 0; col < 3; ++col) {
-        TS0[col] = p.T0[0] * p.Sigma[col] + p.T0[1] * p.Sigma[3 + col] + p.T0[2] * p.Sigma[6 + col];
-        TS1[col] = p.T1[0] * p.Sigma[col] + p.T1[1] * p.Sigma[3 + col] + p.T1[2] * p.Sigma[6 + col];
+    for (int k = 0; k < 3; ++k) {
+        T.r[0][k] = p.T0[k];
+        T.r[1][k] = p.T1[k];
     }
-    float dT0[3], dT1[3];
 #pragma unroll
-    for (int col = 0; col < 3; ++col) {
-        dT0[col] = 2.0f * (G00 * TS0[col] + G01 * TS1[col]);
-        dT1[col] = 2.0f * (G01 * TS0[col] + G11 * TS1[col]);
-    }
-    float dM[9];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int col = 0; col < 3; ++col)
-            dM[r * 3 + col] = 2.0f * (dSigma[r * 3] * p.M[col] + dSigma[r * 3 + 1] * p.M[3 + col]
-                                      + dSigma[r * 3 + 2] * p.M[6 + col]);
-    float dRq[9];
-#pragma unroll
-    for (int col = 0; col < 3; ++col) {
-        float ds = 0.0f;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            ds += dM[r * 3 + col] * p.Rq[r * 3 + col];
-            dRq[r * 3 + col] = dM[r * 3 + col] * p.s[col];
-        }
-        out.scale(col, ds * p.s[col]);
-    }
-    const float w = p.q[0], x = p.q[1], y = p.q[2], zq = p.q[3];
-    const float dw = 2 * (-zq * dRq[1] + y * dRq[2] + zq * dRq[3] - x * dRq[5] - y * dRq[6] + x * dRq[7]);
-    const float dx = 2 * (y * dRq[1] + zq * dRq[2] + y * dRq[3] - 2 * x * dRq[4] - w * dRq[5]
-                          + zq * dRq[6] + w * dRq[7] - 2 * x * dRq[8]);
-    const float dy = 2 * (-2 * y * dRq[0] + x * dRq[1] + w * dRq[2] + x * dRq[3] + zq * dRq[5]
-                          - w * dRq[6] + zq * dRq[7] - 2 * y * dRq[8]);
-    const float dz = 2 * (-2 * zq * dRq[0] - w * dRq[1] + x * dRq[2] + w * dRq[3] - 2 * zq * dRq[4]
-                          + y * dRq[5] + x * dRq[6] + y * dRq[7]);
-    const float dq[4] = {dw, dx, dy, dz};
-    const float qdot = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        out.quat(k, (dq[k] - qdot * p.q[k]) / p.qNorm);
-    const float *R = cam.R;
-    const float dJ00 = dT0[0] * R[0] + dT0[1] * R[1] + dT0[2] * R[2];
-    const float dJ02 = dT0[0] * R[6] + dT0[1] * R[7] + dT0[2] * R[8];
-    const float dJ11 = dT1[0] * R[3] + dT1[1] * R[4] + dT1[2] * R[5];
-    const float dJ12 = dT1[0] * R[6] + dT1[1] * R[7] + dT1[2] * R[8];
-    const float z2 = z * z, z3 = z2 * z;
-    float dpc[3] = {0.0f, 0.0f, 0.0f};
-    dpc[2] += dJ00 * -cam.fx / z2 + dJ02 * 2 * cam.fx * p.tx / z3 + dJ11 * -cam.fy / z2
-              + dJ12 * 2 * cam.fy * p.ty / z3;
-    const float dtx = dJ02 * -cam.fx / z2;
-    const float dty = dJ12 * -cam.fy / z2;
-    if (p.clampX)
-        dpc[2] += dtx * p.limTx;
-    else
-        dpc[0] += dtx;
-    if (p.clampY)
-        dpc[2] += dty * p.limTy;
-    else
-        dpc[1] += dty;
-    dpc[0] += dmx * cam.fx / z;
-    dpc[1] += dmy * cam.fy / z;
-    dpc[2] += -dmx * cam.fx * p.pc[0] / z2 - dmy * cam.fy * p.pc[1] / z2;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        out.mean(k, R[k] * dpc[0] + R[3 + k] * dpc[1] + R[6 + k] * dpc[2]);
-    if (nb) {
-        const float dot = dDir[0] * viewD[0] + dDir[1] * viewD[1] + dDir[2] * viewD[2];
+    for (int row = 0; row < 2; ++row)
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            out.mean(k, (dDir[k] - dot * viewD[k]) / viewLen);
+            U.r[row][k] = T.r[row][0] * p.M[k] + T.r[row][1] * p.M[3 + k] + T.r[row][2] * p.M[6 + k];
+    // dU = 2 dS U
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dU.r[0][k] = 2.0f * (s00 * U.r[0][k] + s01 * U.r[1][k]);
+        dU.r[1][k] = 2.0f * (s01 * U.r[0][k] + s11 * U.r[1][k]);
+    }
+    // dT = dU M^T
+#pragma unroll
+    for (int row = 0; row < 2; ++row)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            dT.r[row][k] = dot3(dU.r[row], &p.M[k * 3]);
+    // dM = T^T dU, folded straight into the scale and rotation gradients
+    float dR[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float along = 0.0f; // sum_r dM[r][k] R[r][k]
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float dm = T.r[0][r] * dU.r[0][k] + T.r[1][r] * dU.r[1][k];
+            along += dm * p.Rq[r * 3 + k];
+            dR[r * 3 + k] = dm * p.s[k];
+        }
+        out.scale(k, along * p.s[k]);
+    }
+    // rotation -> unit quaternion (vector form), then the normalisation's projection
+    const float qw = p.q[0];
+    const float v[3] = {p.q[1], p.q[2], p.q[3]};
+    const float ax[3] = {dR[7] - dR[5], dR[2] - dR[6], dR[3] - dR[1]};
+    const float tr = dR[0] + dR[4] + dR[8];
+    float dq[4];
+    dq[0] = 2.0f * dot3(v, ax);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // ((dR + dR^T) v)_k
+        const float sym = (dR[k * 3] + dR[k]) * v[0] + (dR[k * 3 + 1] + dR[3 + k]) * v[1]
+                          + (dR[k * 3 + 2] + dR[6 + k]) * v[2];
+        dq[1 + k] = 2.0f * sym - 4.0f * tr * v[k] + 2.0f * qw * ax[k];
+    }
+    const float radial = dq[0] * p.q[0] + dq[1] * p.q[1] + dq[2] * p.q[2] + dq[3] * p.q[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out.quat(k, (dq[k] - radial * p.q[k]) / p.qNorm);
+
+    // camera-space mean through J (dJ = dT W^T) and the projected centre
+    const float *W = cam.R;
+    const float z = p.pc[2];
+    const float iz = 1.0f / z, iz2 = iz * iz;
+    const float gJ00 = dot3(dT.r[0], &W[0]), gJ02 = dot3(dT.r[0], &W[6]);
+    const float gJ11 = dot3(dT.r[1], &W[3]), gJ12 = dot3(dT.r[1], &W[6]);
+    const float gtx = -cam.fx * iz2 * gJ02; // d/d tx of J02 = -fx tx / z^2
+    const float gty = -cam.fy * iz2 * gJ12;
+    float dpc[3];
+    dpc[0] = cam.fx * iz * du;
+    dpc[1] = cam.fy * iz * dv;
+    dpc[2] = -(cam.fx * p.pc[0] * du + cam.fy * p.pc[1] * dv) * iz2
+             - cam.fx * iz2 * gJ00 - cam.fy * iz2 * gJ11
+             + 2.0f * iz2 * iz * (cam.fx * p.tx * gJ02 + cam.fy * p.ty * gJ12);
+    if (p.clampX)
+        dpc[2] += gtx * p.limTx;
+    else
+        dpc[0] += gtx;
+    if (p.clampY)
+        dpc[2] += gty * p.limTy;
+    else
+        dpc[1] += gty;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        out.mean(k, W[k] * dpc[0] + W[3 + k] * dpc[1] + W[6 + k] * dpc[2]);
+    if (nb) {
+        // d = (mu - c) / |mu - c|: project out the radial part
+        const float along = dot3(dDir, viewD);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            out.mean(k, (dDir[k] - along * viewD[k]) / viewLen);
     }
 }
 

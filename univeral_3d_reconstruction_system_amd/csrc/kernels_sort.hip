@@ -286,9 +286,12 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
 }
 
 // Sum on a workgroup of 1024 threads. hostOut (optional): one 64-bit word (seq << 32 | sum)
-// stored with a system-scope release.
+// stored with a system-scope release; seq has 31 bits, and bit 63 of the word is set when
+// errWord (optional: the small sort's error word, set by an earlier call whose lookback wait
+// gave up) was non-zero. The word is cleared here, so each abandoned wait is reported once.
 __device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_t count,
-                                         uint32_t *devOut, uint64_t *hostOut, uint32_t seq)
+                                         uint32_t *devOut, uint64_t *hostOut, uint32_t seq,
+                                         uint32_t *errWord = nullptr)
 {
     __shared__ uint32_t ws[16];
     const int tid = threadIdx.x;
@@ -307,9 +310,14 @@ __device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_
         for (int w = 0; w < 16; ++w)
             t += ws[w];
         *devOut = t;
+        uint64_t err = 0;
+        if (errWord)
+            err = __hip_atomic_exchange(errWord, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u
+                      ? uint64_t(1) << 63
+                      : 0u;
         if (hostOut)
-            __hip_atomic_store(hostOut, (uint64_t(seq) << 32) | uint64_t(t), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(hostOut, err | (uint64_t(seq & 0x7fffffffu) << 32) | uint64_t(t),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -320,8 +328,9 @@ __device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_
 // the current order), ranks its keys per wavefront, publishes its per-digit counts as 64-bit
 // words tagged with the call's epoch and the pass, and adds up the counts of every block with a
 // smaller ticket. Those blocks started earlier and wait for nobody before publishing, so every
-// wait ends; a thread that has re-read kOsSpinLimit words in all sets the error word and stops
-// waiting, so a fault elsewhere cannot hang the stream. The tags make resetting the lookback
+// wait ends; a thread that has re-read spinLimit words in all (kOsSpinLimit unless the host
+// lowers it) sets the error word and stops waiting, so a fault elsewhere cannot hang the
+// stream. The tags make resetting the lookback
 // words between calls unnecessary; the epoch comes from the host, one per call.
 constexpr int kOsThreads = 1024;
 constexpr int kOsWaves = kOsThreads / 64;
@@ -368,7 +377,8 @@ __global__ __launch_bounds__(kOsThreads) void onesweepHistKernel(
     int64_t sideCount, uint32_t *sideDev, uint64_t *sideHost, uint32_t sideSeq)
 {
     if (blockIdx.x == kOsHistBlocks) {
-        sumBlock(sideIn, sideCount, sideDev, sideHost, sideSeq);
+        // the passes of earlier calls have finished (stream order): report their error word
+        sumBlock(sideIn, sideCount, sideDev, sideHost, sideSeq, state + kOsError);
         return;
     }
     __shared__ uint32_t h[4][256];
@@ -410,7 +420,7 @@ __global__ __launch_bounds__(kOsThreads) void onesweepPassKernel(
     const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
     uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut, int64_t count, int shift,
     int bits, int pass, uint32_t epoch, const uint32_t *__restrict__ partials,
-    uint64_t *lookback, uint32_t *state)
+    uint64_t *lookback, uint32_t *state, int spinLimit)
 {
     __shared__ uint32_t waveCount[kOsWaves][256];
     __shared__ uint32_t sKey[kOsBlockKeys], sVal[kOsBlockKeys];
@@ -535,7 +545,12 @@ __global__ __launch_bounds__(kOsThreads) void onesweepPassKernel(
     };
     uint32_t pre = 0;
     int spins = 0; // rounds of re-reads, for the whole wait
-    for (int b0 = quarter; b0 < blk; b0 += 4 * kOsBatch) {
+    // spinLimit < 0 (set by the host to exercise the error path): every block with a wait gives
+    // it up at once, whether or not its predecessors have published
+    const bool forced = spinLimit < 0 && blk > 0;
+    if (forced && tid == 0)
+        __hip_atomic_store(&state[kOsError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b0 = quarter; b0 < (forced ? 0 : blk); b0 += 4 * kOsBatch) {
         uint64_t w[kOsBatch];
         uint32_t pending = 0;
 #pragma unroll
@@ -547,7 +562,7 @@ __global__ __launch_bounds__(kOsThreads) void onesweepPassKernel(
         for (int j = 0; j < kOsBatch; ++j)
             pending |= uint32_t(w[j] >> 32) != tag ? 1u << j : 0u;
         while (pending) {
-            if (++spins > kOsSpinLimit) {
+            if (++spins > spinLimit) {
                 __hip_atomic_store(&state[kOsError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -594,12 +609,16 @@ __global__ void iotaKernel(uint32_t *vals, int64_t count)
         vals[i] = uint32_t(i);
 }
 
+// out[i] = in[idx[i]]; an index at or beyond inCount (only after a sort whose lookback wait
+// gave up) reads as 0.
 __global__ void gatherKernel(const uint32_t *__restrict__ in, const uint32_t *__restrict__ idx,
-                             uint32_t *out, int64_t count)
+                             uint32_t *out, int64_t count, int64_t inCount)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < count)
-        out[i] = in[idx[i]];
+    if (i < count) {
+        const uint32_t j = idx[i];
+        out[i] = int64_t(j) < inCount ? in[j] : 0u;
+    }
 }
 
 // Block-level partial sums for the multi-block scan.
@@ -730,21 +749,26 @@ __global__ __launch_bounds__(256) void scanDownsweepFused(const uint32_t *__rest
     }
 }
 
+// Pairs past `capacity` (only when the depth order is not a permutation, after a sort whose
+// lookback wait gave up) are not written.
 __global__ void emitPairsKernel(int64_t n, const uint32_t *__restrict__ sorted,
                                 const uint32_t *__restrict__ offsets,
                                 const int4 *__restrict__ rect, int tilesX, uint32_t *pairTile,
-                                uint32_t *pairGauss, const uint32_t *__restrict__ nTilesSorted)
+                                uint32_t *pairGauss, const uint32_t *__restrict__ nTilesSorted,
+                                int64_t capacity)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n || nTilesSorted[i] == 0)
         return;
     const uint32_t g = sorted[i];
     const int4 r = rect[g];
-    uint32_t o = offsets[i];
+    int64_t o = offsets[i];
     for (int ty = r.y; ty < r.w; ++ty)
         for (int tx = r.x; tx < r.z; ++tx) {
-            pairTile[o] = uint32_t(ty * tilesX + tx);
-            pairGauss[o] = g;
+            if (o < capacity) {
+                pairTile[o] = uint32_t(ty * tilesX + tx);
+                pairGauss[o] = g;
+            }
             ++o;
         }
 }
@@ -851,12 +875,14 @@ struct BinLds {
 // and rectangles are gathered through sorted (nTiles, rect by Gaussian) and written in depth
 // order to ntS / rectS for every position (rectS only where the count is non-zero); otherwise
 // they are read from ntS / rectS.
+// failed (kGather only): the depth sort gave up a lookback wait, so its order may not be a
+// permutation; every position then counts as invisible (empty tile lists, nothing out of bounds).
 template <bool kGather>
 __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
                                          const uint32_t *__restrict__ nTiles,
                                          const int4 *__restrict__ rect, uint32_t *ntS,
                                          int4 *rectS, int64_t first, int count, uint32_t *D,
-                                         int stride)
+                                         int stride, bool failed = false)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -866,7 +892,7 @@ __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__
         if (i < n) {
             if (kGather) {
                 const uint32_t g = sorted[i];
-                nt = nTiles[g];
+                nt = (!failed && int64_t(g) < n) ? nTiles[g] : 0u;
                 if (nt)
                     r = rect[g];
                 ntS[i] = nt;
@@ -945,7 +971,7 @@ constexpr int kCountChunk = 32, kEmitChunk = 8;
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
     const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
-    int group, int numGroups, uint32_t *counts)
+    int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError)
 {
     extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -957,8 +983,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     const int grp = xcdGroup<kCountChunk>(int(blockIdx.x), numGroups);
     if (grp < 0)
         return;
+    // the sort's passes have finished (stream order), so every workgroup reads the same word
+    const bool failed = sortError && *sortError != 0u;
     addRects<true>(n, sorted, nTiles, rect, ntS, rectS, int64_t(grp) * group + int64_t(wave) * slice,
-                   slice, D, stride);
+                   slice, D, stride, failed);
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
     for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
@@ -1073,7 +1101,7 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
                                           const uint32_t *__restrict__ ntS,
                                           const int4 *__restrict__ rectS, int64_t first,
                                           int count, int stride, BinLds &L, uint32_t *mine,
-                                          uint32_t *tileList)
+                                          uint32_t *tileList, uint32_t capacity)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -1130,7 +1158,7 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
                     const uint32_t v = atomicAdd(dst, (act && q == sl) ? 1u : 0u);
                     slot = q == sl ? v : slot;
                 }
-                if (act)
+                if (act && slot < capacity) // see tileEmitKernel
                     tileList[slot] = gg;
                 if (shift < 6 || c0 + 64u >= n0)
                     break;
@@ -1145,12 +1173,14 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
 // per-wavefront cursors (tile offset + earlier groups + earlier slices) in place, then each
 // emits its slice in order (emitSlice).
 // Every workgroup scans the tile totals for the tile offsets itself; group 0 also writes them
-// (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order.
+// (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order. The totals equal the pair
+// count the list was sized for whenever the depth order is a permutation; after a sort whose
+// lookback wait gave up they may not, and then slots and offsets past `capacity` are clamped.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
     const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
-    int *tileOffsets, int *heavyFirst, uint32_t *tileList)
+    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1199,7 +1229,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         if (t < tiles) {
             const uint32_t off = before + x - tot;
             if (grp == 0)
-                tileOffsets[t] = int(off);
+                tileOffsets[t] = int(off < capacity ? off : capacity);
             const int ty = t / tilesX, tx = t - ty * tilesX;
             const int cell = ty * stride + tx;
             uint32_t base = off + counts[size_t(t) * size_t(numGroups) + grp];
@@ -1214,11 +1244,11 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     }
     if (grp == 0) {
         if (tid == 0)
-            tileOffsets[tiles] = int(carry);
+            tileOffsets[tiles] = int(carry < capacity ? carry : capacity);
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls);
     }
-    emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList);
+    emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList, capacity);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1272,11 +1302,18 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
 
 size_t onesweepScratchWords() { return kOsStateWords + kOsLookbackWords + kOsPartialWords; }
 
-int onesweepErrorWord(const uint64_t *scratch, hipStream_t s, uint32_t *value)
+const uint32_t *onesweepErrorPtr(const uint64_t *scratch)
 {
-    const uint32_t *state = reinterpret_cast<const uint32_t *>(scratch);
+    return reinterpret_cast<const uint32_t *>(scratch) + kOsError;
+}
+
+int onesweepErrorWord(uint64_t *scratch, hipStream_t s, uint32_t *value)
+{
+    uint32_t *state = reinterpret_cast<uint32_t *>(scratch);
     U3D_HIP(hipMemcpyAsync(value, state + kOsError, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     U3D_HIP(hipStreamSynchronize(s));
+    if (*value)
+        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(state + kOsError), 0, 1));
     return U3D_OK;
 }
 
@@ -1295,6 +1332,7 @@ int onesweepSort(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAl
     if (epoch == 0)
         epoch = 1;
     const int64_t per = (count + kOsHistBlocks - 1) / kOsHistBlocks;
+    const int spinLimit = scratch.spinLimit != 0 ? scratch.spinLimit : kOsSpinLimit;
     hipLaunchKernelGGL(onesweepHistKernel, dim3(kOsHistBlocks + (scratch.sideIn ? 1 : 0)),
                        dim3(kOsThreads), 0, s, keys, count, per, bits, partials, state,
                        scratch.sideIn, scratch.sideCount, scratch.sideDev, scratch.sideHost,
@@ -1307,11 +1345,11 @@ int onesweepSort(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAl
         if (pass == 0 && scratch.valsAreIndex)
             hipLaunchKernelGGL(onesweepPassKernel<true>, dim3(numBlocks), dim3(kOsThreads), 0, s,
                                kin, vin, kout, vout, count, shift, b, pass, epoch, partials,
-                               lookback, state);
+                               lookback, state, spinLimit);
         else
             hipLaunchKernelGGL(onesweepPassKernel<false>, dim3(numBlocks), dim3(kOsThreads), 0, s,
                                kin, vin, kout, vout, count, shift, b, pass, epoch, partials,
-                               lookback, state);
+                               lookback, state, spinLimit);
         U3D_LAUNCH_CHECK();
         std::swap(kin, kout);
         std::swap(vin, vout);
@@ -1356,12 +1394,12 @@ int launchIota(hipStream_t s, uint32_t *vals, int64_t count)
 }
 
 int launchGatherU32(hipStream_t s, const uint32_t *in, const uint32_t *idx, uint32_t *out,
-                    int64_t count)
+                    int64_t count, int64_t inCount)
 {
     if (count <= 0)
         return U3D_OK;
     hipLaunchKernelGGL(gatherKernel, dim3(blocksFor(count, 256)), dim3(256), 0, s, in, idx, out,
-                       count);
+                       count, inCount);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
@@ -1424,7 +1462,8 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
-                      uint32_t *tileList, int *heavyFirst)
+                      uint32_t *tileList, size_t listCapacity, int *heavyFirst,
+                      const uint32_t *sortError)
 {
     const int group = tileBinningGroup(n, tilesX, tilesY);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
@@ -1436,7 +1475,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     const int cells = (tilesX + 1) * (tilesY + 1);
     hipLaunchKernelGGL(tileCountKernel, dim3(xcdGridSize(numGroups, kCountChunk)), dim3(64 * kBinWaves),
                        size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
-                       tilesX, tilesY, group, numGroups, counts);
+                       tilesX, tilesY, group, numGroups, counts, sortError);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
                        tileTotal);
@@ -1448,19 +1487,20 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     hipLaunchKernelGGL(tileEmitKernel, dim3(xcdGridSize(numGroups, kEmitChunk)), dim3(64 * waves),
                        size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
-                       heavyFirst, tileList);
+                       heavyFirst, tileList,
+                       uint32_t(listCapacity < size_t(INT32_MAX) ? listCapacity : size_t(INT32_MAX)));
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
 
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
                     const int4 *rect, int tilesX, uint32_t *pairTile, uint32_t *pairGauss,
-                    const uint32_t *nTilesSorted)
+                    const uint32_t *nTilesSorted, int64_t capacity)
 {
     if (n <= 0)
         return U3D_OK;
     hipLaunchKernelGGL(emitPairsKernel, dim3(blocksFor(n, 256)), dim3(256), 0, s, n, sorted,
-                       offsets, rect, tilesX, pairTile, pairGauss, nTilesSorted);
+                       offsets, rect, tilesX, pairTile, pairGauss, nTilesSorted, capacity);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

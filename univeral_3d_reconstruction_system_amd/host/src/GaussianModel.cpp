@@ -1,10 +1,10 @@
-// Parameter container and kNN-distance initialisation of the drop-in API
-// (engine/src/splat/GaussianModel.cpp semantics: GaussianModel.cpp:9-66, 68-144).
+// Parameter container of the drop-in API (GaussianModel.cpp:9-66) and its kNN-distance
+// initialisation (GaussianModel.cpp:68-144), which runs on the device through the C ABI.
 #include "u3d/splat/GaussianModel.h"
 
+#include "device.h"
+
 #include <algorithm>
-#include <limits>
-#include <unordered_map>
 
 namespace u3d {
 
@@ -63,77 +63,27 @@ void GaussianModel::appendFrom(const GaussianModel &other, std::size_t index)
 
 GaussianModel initGaussiansFromPoints(const std::vector<float> &xyz, const std::vector<float> &rgb)
 {
+    // The kNN-distance initialisation runs on the device (u3d_points_knn_init): the points go up
+    // once, the hash-grid 3-NN search and the parameter fill run there, the model comes back.
     const std::size_t n = xyz.size() / 3;
     GaussianModel m;
     m.resize(n);
     if (n == 0)
         return m;
-    const float big = std::numeric_limits<float>::max();
-    float lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
-    for (std::size_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(lo[k], xyz[i * 3 + k]);
-            hi[k] = std::max(hi[k], xyz[i * 3 + k]);
-        }
-    // Grid cell of twice the mean spacing; cells keyed by an XOR of prime multiples, so
-    // colliding cells share a bucket exactly as in the reference.
-    const float span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-6f});
-    const float cell = std::max(span / std::cbrt(float(n)) * 2.0f, 1e-6f);
-    auto cellKey = [](int x, int y, int z) {
-        return (std::int64_t(x) * 73856093) ^ (std::int64_t(y) * 19349663) ^ (std::int64_t(z) * 83492791);
-    };
-    auto cellOf = [&](std::size_t i, int c[3]) {
-        for (int k = 0; k < 3; ++k)
-            c[k] = int(std::floor((xyz[i * 3 + k] - lo[k]) / cell));
-    };
-    std::unordered_map<std::int64_t, std::vector<std::uint32_t>> buckets;
-    buckets.reserve(n);
-    for (std::size_t i = 0; i < n; ++i) {
-        int c[3];
-        cellOf(i, c);
-        buckets[cellKey(c[0], c[1], c[2])].push_back(std::uint32_t(i));
-    }
-    const float logitTenth = logit(0.1f);
-    for (std::size_t i = 0; i < n; ++i) {
-        float best[3] = {big, big, big}; // three smallest squared distances, ascending
-        int c[3];
-        cellOf(i, c);
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    auto it = buckets.find(cellKey(c[0] + dx, c[1] + dy, c[2] + dz));
-                    if (it == buckets.end())
-                        continue;
-                    for (std::uint32_t j : it->second) {
-                        if (j == i)
-                            continue;
-                        const float ex = xyz[j * 3] - xyz[i * 3], ey = xyz[j * 3 + 1] - xyz[i * 3 + 1],
-                                    ez = xyz[j * 3 + 2] - xyz[i * 3 + 2];
-                        float d2 = ex * ex + ey * ey + ez * ez;
-                        for (int k = 0; k < 3; ++k)
-                            if (d2 < best[k])
-                                std::swap(d2, best[k]);
-                    }
-                }
-        float sum = 0;
-        int found = 0;
-        for (float b : best)
-            if (b < big) {
-                sum += b;
-                ++found;
-            }
-        const float meanD2 = found ? sum / float(found) : cell * cell;
-        const float logScale = std::log(std::sqrt(std::max(meanD2, 1e-14f)));
-        for (int k = 0; k < 3; ++k) {
-            m.means[i * 3 + k] = xyz[i * 3 + k];
-            m.scales[i * 3 + k] = logScale;
-            const float col = rgb.size() >= (i + 1) * 3 ? rgb[i * 3 + k] : 0.5f;
-            m.colors[i * 3 + k] = (col - 0.5f) / kShC0;
-        }
-        m.quats[i * 4] = 1.0f;
-        m.quats[i * 4 + 1] = m.quats[i * 4 + 2] = m.quats[i * 4 + 3] = 0.0f;
-        m.opacities[i] = logitTenth;
-    }
+    u3d_ctx *ctx = dev::context();
+    // rgb entries beyond 3n are never read (the reference reads rgb[i*3+k] for i < n only)
+    const std::size_t rgbCount = std::min(rgb.size(), n * 3);
+    dev::Buffer points, colours;
+    float *dXyz = static_cast<float *>(points.ensure(n * 3 * sizeof(float)));
+    float *dRgb = rgbCount ? static_cast<float *>(colours.ensure(rgbCount * sizeof(float))) : nullptr;
+    dev::check(u3d_memcpy_h2d(ctx, dXyz, xyz.data(), n * 3 * sizeof(float)), "upload points");
+    if (rgbCount)
+        dev::check(u3d_memcpy_h2d(ctx, dRgb, rgb.data(), rgbCount * sizeof(float)), "upload colours");
+    dev::Model out;
+    out.zero(n);
+    u3d_gaussians ov = out.view();
+    dev::check(u3d_points_knn_init(ctx, dXyz, dRgb, int(rgbCount), int64_t(n), &ov), "u3d_points_knn_init");
+    out.download(m);
     return m;
 }
 

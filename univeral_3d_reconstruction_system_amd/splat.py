@@ -435,6 +435,26 @@ class Trainer:
         check(self.ctx.lib.u3d_trainer_get_sh_rest(self.h, out.ctypes.data))
         return out
 
+    def set_sh_rest(self, rest: np.ndarray):
+        a = np.ascontiguousarray(rest, np.float32).reshape(-1)
+        assert a.size == self.size() * 45
+        check(self.ctx.lib.u3d_trainer_set_sh_rest(self.h, a.ctypes.data))
+
+    def sh_moments(self) -> tuple:
+        """Adam moments (m, v) of the SH coefficients, [size x 15 x 3] each."""
+        m = np.zeros(self.size() * 45, np.float32)
+        v = np.zeros(self.size() * 45, np.float32)
+        check(self.ctx.lib.u3d_trainer_get_sh_moments(self.h, m.ctypes.data, v.ctypes.data))
+        return m, v
+
+    def densify_stats(self) -> tuple:
+        """(accumulated NDC mean-gradient norms, visible-step counts) since the last densify."""
+        n = self.size()
+        accum = np.zeros(n, np.float32)
+        count = np.zeros(n, np.int32)
+        check(self.ctx.lib.u3d_trainer_get_densify_stats(self.h, accum.ctypes.data, count.ctypes.data))
+        return accum, count
+
     def stats(self) -> dict:
         s = _abi.TrainStats()
         check(self.ctx.lib.u3d_trainer_stats(self.h, C.byref(s)))
