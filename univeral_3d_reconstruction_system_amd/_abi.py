@@ -107,6 +107,7 @@ SIGNATURES = {
     "u3d_raster_get_state": (I, [P, C.POINTER(RasterState)]),
     "u3d_raster_get_grad2d": (I, [P, C.POINTER(P)]),
     "u3d_photometric_loss": (I, [P, P, P, I, I, F, P, C.POINTER(LossValue)]),
+    "u3d_photometric_loss_noalloc": (I, [P, P, P, I, I, F, P, P]),
     "u3d_psnr": (I, [P, P, P, I64, C.POINTER(F)]),
     "u3d_adam_create": (I, [P, C.POINTER(P)]),
     "u3d_adam_destroy": (None, [P]),

@@ -6,10 +6,18 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
 #include "trainer.h"
+
+// Device scratch of u3d_photometric_loss_noalloc, for callers that cannot allocate: the loss
+// kernels' per-block partial sums and the reduced value. Each GPU holds its own copy for the
+// life of the process; host code reaches it only through the address hipGetSymbolAddress
+// returns for that device (the array's name is a host-side stand-in).
+constexpr size_t kStaticScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
 
 namespace u3d {
 
@@ -62,6 +70,24 @@ int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *val
         if (e != hipErrorNotReady)
             return failHip(e, "hipStreamQuery", __FILE__, __LINE__);
     }
+}
+
+// The device address of `scratch` on `device` (current), resolved once per device.
+int staticScratchAddress(int device, unsigned char **out)
+{
+    static std::mutex mu;
+    static std::vector<unsigned char *> perDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (size_t(device) >= perDevice.size())
+        perDevice.resize(size_t(device) + 1, nullptr);
+    if (!perDevice[size_t(device)]) {
+        void *p = nullptr;
+        U3D_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        U3D_CHECK(p != nullptr, U3D_ERR_HIP, "static scratch has no device address");
+        perDevice[size_t(device)] = static_cast<unsigned char *>(p);
+    }
+    *out = perDevice[size_t(device)];
+    return U3D_OK;
 }
 
 int useDevice(u3d_ctx *ctx)
@@ -338,6 +364,7 @@ int u3d_ctx_create(int device, u3d_ctx **out)
     hipDeviceProp_t prop;
     U3D_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
+    U3D_TRY(staticScratchAddress(device, &c->staticScratch));
     *out = c.release();
     return U3D_OK;
 }
@@ -542,6 +569,27 @@ int u3d_photometric_loss(u3d_ctx *ctx, const float *render, const float *target,
     if (out_host)
         *out_host = {host[0], host[1], host[2]};
     return U3D_OK;
+}
+
+int u3d_photometric_loss_noalloc(u3d_ctx *ctx, const float *render, const float *target, int width,
+                                 int height, float ssim_lambda, float *grad, float *value_device)
+{
+    U3D_TRY(useDevice(ctx));
+    U3D_CHECK(width >= 0 && height >= 0, U3D_ERR_INVALID, "negative image size");
+    U3D_CHECK((render && target && grad) || size_t(width) * size_t(height) == 0, U3D_ERR_INVALID, "null image");
+    U3D_CHECK(ctx->staticScratch != nullptr, U3D_ERR_HIP, "static scratch unavailable");
+    // [partials: doubles][result: 4 floats, 256-byte aligned]
+    const size_t np = lossPartialsSize(width, height) + 1;
+    const size_t resultOffset = (np * sizeof(double) + 255) / 256 * 256;
+    U3D_CHECK(resultOffset + 4 * sizeof(float) <= kStaticScratchBytes, U3D_ERR_CAPACITY,
+              "image too large for the static loss scratch (" + std::to_string(kStaticScratchBytes) +
+                  " bytes: at most " + std::to_string((kStaticScratchBytes - 256) / sizeof(double) / 9 - 1) +
+                  " blocks of 32x32 pixels)");
+    double *partials = reinterpret_cast<double *>(ctx->staticScratch);
+    float *result = value_device ? value_device
+                                 : reinterpret_cast<float *>(ctx->staticScratch + resultOffset);
+    return launchPhotometricLoss(ctx->stream, render, target, nullptr, width, height, ssim_lambda,
+                                 grad, partials, np, result);
 }
 
 int u3d_psnr(u3d_ctx *ctx, const float *a, const float *b, int64_t count, float *out_host)

@@ -187,6 +187,8 @@ struct u3d_ctx {
     u3d::StageTimer timer;
     bool segmentedBackward = true; // U3D_OPT_SEGMENTED_BACKWARD
     int sortSpinLimit = 0;         // U3D_OPT_SORT_SPIN_LIMIT (0: the default bound)
+    // this device's copy of the library's static device scratch (u3d_photometric_loss_noalloc)
+    unsigned char *staticScratch = nullptr;
 };
 
 namespace u3d {
