@@ -9,8 +9,11 @@ Adam, and the TrainOptions::forIterations(7000) densify / reset schedule).
 
 Prints one JSON line: iters/s (value, timed with no instrumentation on the stream), per-stage
 GPU times from HIP events recorded over a second timed region of the same length, the dominant
-kernel's roofline, and the reference CPU trainer timed on this host's cores on a bounded sample
-of the same workload.
+kernel's roofline (evaluated pairs counted on the device over every timed step), and the
+reference CPU trainer timed on this host's cores on a bounded sample of the same workload.
+Beside it: the reference's own loop shape (step() then stats() every iteration), the SH-3
+step, the batched engine (B views per step in a graph), a 16-scene set, and the seconds to the
+plain trainer's final L1 for the kNN and learned initialisations.
 """
 from __future__ import annotations
 
@@ -47,23 +50,48 @@ BYTES_RASTER_FWD_PER_PAIR, BYTES_RASTER_FWD_PER_PIXEL = 40, 20
 BYTES_RASTER_BWD_PER_PAIR, BYTES_RASTER_BWD_PER_PIXEL = 40 + 36, 20
 FLOPS_FWD_PER_EVAL, FLOPS_BWD_PER_EVAL = 20, 70  # per evaluated (pixel, Gaussian) pair
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3            # MI355X FP32 vector peak (published; counts packed FP32)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6    # the same without packed FP32 (the build disables it)
+REF_FLAGS = "g++ -std=c++17 -O3 -DNDEBUG -fPIC -pthread (oracle/Makefile)"
 
 
-def host_cores() -> int:
+def host_cpu() -> dict:
+    """This process's CPU share: logical CPUs of the host, those it may run on, the cgroup
+    quota, and the CPU model."""
+    nproc = os.cpu_count() or 1
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        affinity = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return dict(nproc=nproc, affinity=affinity, cgroup_quota_cpus=quota, usable=usable, model=model)
 
 
 def cpu_baseline(init, cams, images, extent, opt_gpu, budget_s=15.0):
-    """The compiled reference trainer (oracle/_ref) on this host's cores, same workload."""
+    """The compiled reference trainer (oracle/_ref) on every CPU this process may use, same
+    workload."""
     import oracle as O
     if O.ref_lib() is None:
         return None
-    threads = host_cores()
+    cpu = host_cpu()
+    threads = cpu["usable"]
     opt = O.options_for_iterations(ITERATIONS)
     opt.max_gaussians = opt_gpu.max_gaussians
     t = O.Trainer(O.Gaussians(**{k: np.ascontiguousarray(v) for k, v in init.items()}), cams, images,
@@ -77,7 +105,55 @@ def cpu_baseline(init, cams, images, extent, opt_gpu, budget_s=15.0):
     dt = time.perf_counter() - t0
     return dict(value=steps / dt, unit="iters/s", cores=threads, kind="reference",
                 sample=f"{steps} reference Trainer::step calls (after 2 warm-up) at 300k Gaussians, "
-                       f"640x480, 200 views, {threads} threads, g++ -O3 build of engine/src/splat")
+                       f"640x480, 200 views, {threads} threads (every CPU of this process's share), "
+                       f"engine/src/splat built with {REF_FLAGS}",
+                host=cpu, flags=REF_FLAGS)
+
+
+def loop_shape_rate(trainer, steps):
+    """The reference pipeline's loop (Reconstructor.cpp:334-348): step() then stats() every
+    iteration, through the C ABI calls the drop-in's Trainer::step / Trainer::stats make."""
+    trainer.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        trainer.step()
+        trainer.stats()
+    trainer.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(steps=steps, iters_per_s=round(steps / dt, 3), ms_per_step=round(1e3 * dt / steps, 4))
+
+
+def sh3_rate(ctx, cams, images, extent, steps, warmup):
+    """The plain single-view step with SH degree 3 (45 more coefficients per Gaussian, N(0, 0.05)
+    start), 300k Gaussians, 640x480."""
+    init = S.random_gaussians(N_GAUSSIANS, 2)
+    opt = U.options_for_iterations(ITERATIONS)
+    opt.max_gaussians = N_GAUSSIANS
+    opt.sh_degree = 3
+    t = U.Trainer(ctx, init, cams, images, extent, opt)
+    t.set_sh_rest(S.rng(3).normal(0.0, 0.05, N_GAUSSIANS * 45).astype(np.float32))
+    for _ in range(warmup):
+        t.step()
+    t.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t.step()
+    t.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(sh_degree=3, steps=steps, iters_per_s=round(steps / dt, 3),
+                ms_per_step=round(1e3 * dt / steps, 4))
+
+
+def time_to_target(ctx, plain_iterations, seed=100_000):
+    """Seconds for the kNN + plain trainer to run plain_iterations (its final mean L1 over all
+    views is the target), and for learned + batched, kNN + batched and learned + plain to reach
+    that L1, on a held-out synthetic scene at the headline shape (300k ground-truth Gaussians,
+    50k seed points, 200 views, 640x480); initialisation included, evaluation passes excluded."""
+    from univeral_3d_reconstruction_system_amd import learned_init as L
+    net = L.load_weights(ctx)
+    r = L.evaluate_scene(ctx, net, seed, N_GAUSSIANS, 50_000, N_VIEWS, WIDTH, HEIGHT, plain_iterations)
+    r["weights"] = "univeral_3d_reconstruction_system_amd/weights/init_net.npz" if net else None
+    return r
 
 
 def batched_rate(ctx, lib, init, cams, images, extent, views, steps, warmup):
@@ -174,6 +250,11 @@ def main(argv=None) -> int:
     ap.add_argument("--scene-set", type=int, default=16,
                     help="also time this many independent uneven scenes trained as one scene set "
                          "(0: skip)")
+    ap.add_argument("--no-loop-shape", dest="loop_shape", action="store_false",
+                    help="skip the step()+stats() loop timing")
+    ap.add_argument("--no-sh3", dest="sh3", action="store_false", help="skip the SH-3 step timing")
+    ap.add_argument("--no-time-to-l1", dest="time_to_l1", action="store_false",
+                    help="skip the time-to-target-L1 runs (~1 minute)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters FETCH_SIZE / WRITE_SIZE against a known traffic) and exit")
@@ -224,6 +305,7 @@ def main(argv=None) -> int:
     mean_n = (c1[1] - c0[1]) / steps
     pixels = (c1[2] - c0[2]) / steps
     pairs = (c1[3] - c0[3]) / steps
+    evals = (c1[6] - c0[6]) / steps  # evaluated (pixel, Gaussian) pairs, counted on the device
     stats = trainer.stats()
 
     # Per-stage kernel times: the same training continued for the same number of steps with a
@@ -240,12 +322,9 @@ def main(argv=None) -> int:
     _abi.check(lib.u3d_ctx_stage_times(ctx.h, ms, cnt))
     _abi.check(lib.u3d_ctx_set_timing(ctx.h, 0))
 
-    # Evaluated pairs (sum of contributor counts) from one forward of the trained model.
-    r = U.Rasterizer(ctx)
-    m = trainer.model()
-    dm = U.DeviceGaussians.from_host(ctx, m)
-    r.forward(dm, cams[0], out=out)
-    evals = float(r.state()["n_contrib"].astype(np.int64).sum())
+    c2 = (C.c_int64 * _abi.U3D_NUM_COUNTERS)()
+    _abi.check(lib.u3d_trainer_counters(trainer.h, c2))
+    evals_instr = (c2[6] - c1[6]) / max(c2[0] - c1[0], 1)
 
     stage_ms = {name: ms[k] / max(cnt[k], 1) for k, name in enumerate(_abi.STAGES)}
     per_step_ms = {name: ms[k] / a.steps for k, name in enumerate(_abi.STAGES)}
@@ -258,8 +337,8 @@ def main(argv=None) -> int:
         loss=("hbm", BYTES_LOSS_PER_PIXEL * pixels),
         sort=("hbm", 4 * 16 * mean_n + 2 * 24 * mean_n + 4 * pairs),
         chain_rule=("hbm", BYTES_CHAIN_ADAM_PER_GAUSSIAN * mean_n),
-        raster_fwd=("mfma", FLOPS_FWD_PER_EVAL * evals),
-        raster_bwd=("mfma", FLOPS_BWD_PER_EVAL * evals),
+        raster_fwd=("valu", FLOPS_FWD_PER_EVAL * evals_instr),
+        raster_bwd=("valu", FLOPS_BWD_PER_EVAL * evals_instr),
     )
 
     def stage_roof(name):
@@ -288,12 +367,19 @@ def main(argv=None) -> int:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = dict(bound=bound, kernel=dominant, achieved=round(achieved, 4), peak=peak, unit=unit,
                     frac=round(achieved / peak, 6), traffic=traffic,
-                    note=("compute roof = MI355X FP32 peak (the compositing kernels run on the "
-                          "FP32 vector ALU); work = algorithmic flops per evaluated pair x pairs")
-                    if bound == "mfma" else "work = algorithmic HBM bytes per launch",
+                    note=(f"compute roof = MI355X FP32 vector peak {PEAK_FP32_TFLOPS} TFLOP/s (the "
+                          "compositing kernels run on the FP32 vector ALU; packed FP32, which the "
+                          f"published peak counts, is disabled in this build: frac vs the "
+                          f"{PEAK_FP32_UNPACKED_TFLOPS} TFLOP/s unpacked peak = "
+                          f"{round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4)}); work = "
+                          f"{FLOPS_BWD_PER_EVAL if dominant == 'raster_bwd' else FLOPS_FWD_PER_EVAL} "
+                          "flops per evaluated pair x the pairs of the timed steps")
+                    if bound == "valu" else "work = algorithmic HBM bytes per launch",
                     stages=stages)
 
     value = steps / elapsed
+    loop_shape = loop_shape_rate(trainer, a.steps) if a.loop_shape else None
+    sh3 = sh3_rate(ctx, cams, images, extent, max(1, a.steps // 3), max(1, a.warmup // 3)) if a.sh3 else None
     batched = None
     if a.batched_views > 0:
         batched = batched_rate(ctx, lib, init, cams, images, extent, a.batched_views,
@@ -301,6 +387,7 @@ def main(argv=None) -> int:
     multi = None
     if a.scene_set > 0:
         multi = scene_set_rate(ctx, a.scene_set, max(1, a.steps // 4), max(1, a.warmup // 4))
+    ttl1 = time_to_target(ctx, ITERATIONS) if a.time_to_l1 else None
     base = None if a.no_cpu_baseline else cpu_baseline(init, cams, images, extent, opt, a.cpu_seconds)
     line = dict(
         metric="train iters/s @300k Gaussians 640px", value=round(value, 3), unit="iters/s",
@@ -311,14 +398,18 @@ def main(argv=None) -> int:
                     gaussians=N_GAUSSIANS, width=WIDTH, height=HEIGHT, views=N_VIEWS,
                     global_batch=1, parallelism="none"),
         gaussian_pixels_per_s=round(mean_n * pixels * value, 1),
-        mean_pairs_per_step=round(pairs, 1), evaluated_pairs_last_view=evals,
+        mean_pairs_per_step=round(pairs, 1), evaluated_pairs_per_step=round(evals, 1),
+        evaluated_pairs_per_s=round(evals * value, 1),
         stage_ms_per_step={k: round(v, 4) for k, v in per_step_ms.items()},
         instrumented_ms_per_step=round(1e3 * instrumented_elapsed / a.steps, 4),
         roofline=roofline, cpu_baseline=base,
         gpu_vs_cpu=round(value / base["value"], 2) if base else None,
         train_psnr_ema=round(stats["psnr"], 3),
+        reference_loop_shape=loop_shape,
+        sh3_step=sh3,
         batched=batched,
         scene_set=multi,
+        time_to_target_l1=ttl1,
     )
     print(json.dumps(line))
     return 0

@@ -398,9 +398,11 @@ int u3d_ctx_set_timing(u3d_ctx *ctx, int enabled);
 int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES],
                         int64_t counts[U3D_NUM_STAGES]);
 /* Work counters summed over the trainer's steps since creation: steps, Gaussians (per view),
- * pixels, tile intersections K, views, and steps whose tile-pair list overflowed its capacity
- * (batched engine; the farthest pairs of that step were dropped). */
-#define U3D_NUM_COUNTERS 6
+ * pixels, tile intersections K, views, steps whose tile-pair list overflowed its capacity
+ * (batched engine; the farthest pairs of that step were dropped), and evaluated (pixel,
+ * Gaussian) pairs -- the sum of the contributor counts of every training forward, counted on
+ * the device (single-view steps with 16-pixel tiles; 0 otherwise). Synchronises the stream. */
+#define U3D_NUM_COUNTERS 7
 int u3d_trainer_counters(u3d_trainer *t, int64_t values[U3D_NUM_COUNTERS]);
 
 /* ------------------------------------------------------------------------------ helpers */

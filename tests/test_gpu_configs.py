@@ -195,6 +195,11 @@ def test_trainer_fused_gradient_matches_oracle_backward(ctx, degree):
     accum, count = t.densify_stats()
     assert np.array_equal(count, (radius > 0).astype(np.int32))
     assert _rel(accum, ndc) < GRAD_RTOL
+    # the device's count of evaluated (pixel, Gaussian) pairs: the oracle's contributor counts
+    ro = O.Raster()
+    ro.forward(gaussians(g), cams[0], O.make_settings(), sh_rest=rest if degree else None,
+               sh_degree=degree)
+    assert t.counters()["evaluated_pairs"] == int(ro.state()["n_contrib"].astype(np.int64).sum())
 
 
 def test_config0_sh3_trainer_follows_oracle(ctx):

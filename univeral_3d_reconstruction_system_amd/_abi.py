@@ -175,7 +175,7 @@ FEATURE_DIM = 80
 NET_OUT = 16
 NET_LAYERS = ((80, 64), (64, 256), (256, 256), (256, 256), (256, 16))  # (in, out)
 NET_GELU = (False, True, True, True, False)
-COUNTERS = ("steps", "gaussians", "pixels", "pairs", "views", "overflow_steps")
+COUNTERS = ("steps", "gaussians", "pixels", "pairs", "views", "overflow_steps", "evaluated_pairs")
 U3D_NUM_COUNTERS = len(COUNTERS)
 
 _lib = None

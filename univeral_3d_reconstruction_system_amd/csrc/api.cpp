@@ -90,6 +90,29 @@ int staticScratchAddress(int device, unsigned char **out)
     return U3D_OK;
 }
 
+int waitPinnedU32(const uint32_t *p, uint32_t pending, hipStream_t s)
+{
+    // As waitPinnedWord: tight polling first, then the stream is asked for an error.
+    using Clock = std::chrono::steady_clock;
+    Clock::time_point nextQuery = Clock::now() + std::chrono::milliseconds(1);
+    for (uint32_t polls = 1;; ++polls) {
+        if (__atomic_load_n(p, __ATOMIC_ACQUIRE) != pending)
+            return U3D_OK;
+        __builtin_ia32_pause();
+        if ((polls & 255u) != 0 || Clock::now() < nextQuery)
+            continue;
+        nextQuery = Clock::now() + std::chrono::microseconds(100);
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) {
+            U3D_CHECK(__atomic_load_n(p, __ATOMIC_ACQUIRE) != pending, U3D_ERR_HIP,
+                      "a step's loss statistics never arrived");
+            return U3D_OK;
+        }
+        if (e != hipErrorNotReady)
+            return failHip(e, "hipStreamQuery", __FILE__, __LINE__);
+    }
+}
+
 int useDevice(u3d_ctx *ctx)
 {
     U3D_CHECK(ctx != nullptr, U3D_ERR_INVALID, "null context");
@@ -104,7 +127,8 @@ using namespace u3d;
 namespace u3d {
 
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
-                      const u3d_raster_settings &st, float *rgb, const float *shRest, int shDegree)
+                      const u3d_raster_settings &st, float *rgb, const float *shRest, int shDegree,
+                      unsigned long long *evalCount)
 {
     U3D_CHECK(shDegree >= 0 && shDegree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
     U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
@@ -254,7 +278,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
     U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
-                                r->finalT.p, r->nContrib.p, seg, order));
+                                r->finalT.p, r->nContrib.p, seg, order, evalCount));
     r->haveForward = true;
     return U3D_OK;
 }
@@ -941,6 +965,8 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_TRY(t->dloss.ensure(maxPix * 3));
     U3D_TRY(t->gradAccum.ensure(size_t(cap)));
     U3D_TRY(t->gradCount.ensure(size_t(cap)));
+    U3D_TRY(t->evalPairs.ensure(64));
+    U3D_HIP(hipMemsetAsync(t->evalPairs.p, 0, 64 * sizeof(unsigned long long), s));
     U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(cap) * sizeof(float), s));
     U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(cap) * sizeof(int), s));
     U3D_CHECK(options->sh_degree >= 0 && options->sh_degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
@@ -955,7 +981,13 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
             U3D_HIP(hipMemsetAsync(b->p, 0, b->cap * sizeof(float), s));
         }
     }
-    U3D_TRY(t->stepStats.ensure(size_t(std::max(1, options->iterations)) * 4 * size_t(t->viewsPerStep)));
+    {
+        const size_t slots = size_t(std::max(1, options->iterations)) * size_t(t->viewsPerStep);
+        U3D_TRY(t->stepStats.ensure(slots * 4 * sizeof(float)));
+        uint32_t *w = static_cast<uint32_t *>(t->stepStats.p);
+        for (size_t k = 0; k < slots * 4; ++k)
+            w[k] = kStatsPending;
+    }
     U3D_TRY(t->evalOut.ensure(size_t(std::max(1, n_images)) + 4));
     size_t parts = 1024;
     for (const u3d_camera &c : t->cams)
@@ -1130,13 +1162,14 @@ int u3d_trainer_step(u3d_trainer *t)
     const u3d_camera &cam = t->cams[view];
     const hipStream_t s = t->ctx->stream;
     const u3d_gaussians pv = t->params.view();
-    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree));
+    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree,
+                              t->evalPairs.p));
     t->counters[0] += 1;
     t->counters[1] += t->params.n;
     t->counters[2] += int64_t(cam.width) * cam.height;
     t->counters[3] += t->raster.numPairs;
     t->counters[4] += 1;
-    float *stats = t->stepStats.p + size_t(t->iteration) * 4;
+    float *stats = t->statsSlot(size_t(t->iteration));
     // The loss value's final reduction rides on the backward's first launch, and the
     // segmented backward's work-item scan on the loss launch.
     LossFinalize lossFinalize;
@@ -1274,18 +1307,20 @@ int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out)
     // TrainStats EMA (Trainer.cpp:88-92), folded lazily from the per-step device values.
     U3D_CHECK(t && out, U3D_ERR_INVALID, "null argument");
     U3D_TRY(useDevice(t->ctx));
-    // With several views per step, a step's value is the mean over its views.
+    // With several views per step, a step's value is the mean over its views. The values are in
+    // pinned host memory; waiting for the last step's psnr word (stored by the backward's first
+    // launch) lets the device go on with the rest of the step: no stream synchronisation.
     const int from = t->statsFolded, to = t->iteration;
     const size_t B = size_t(t->viewsPerStep);
     if (to > from) {
-        std::vector<float> v(size_t(to - from) * 4 * B);
-        U3D_HIP(hipMemcpyAsync(v.data(), t->stepStats.p + size_t(from) * 4 * B, v.size() * sizeof(float),
-                               hipMemcpyDeviceToHost, t->ctx->stream));
-        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+        for (int it = from; it < to; ++it)
+            for (size_t k = 0; k < B; ++k)
+                U3D_TRY(waitPinnedU32(reinterpret_cast<const uint32_t *>(t->statsSlot(size_t(it) * B + k) + 3),
+                                      kStatsPending, t->ctx->stream));
         for (int it = from; it < to; ++it) {
             float loss = 0.0f, psnr = 0.0f;
             for (size_t k = 0; k < B; ++k) {
-                const float *x = &v[(size_t(it - from) * B + k) * 4];
+                const float *x = t->statsSlot(size_t(it) * B + k);
                 loss += x[0];
                 psnr += x[3];
             }
@@ -1409,6 +1444,16 @@ int u3d_ctx_stage_times(u3d_ctx *ctx, double total_ms[U3D_NUM_STAGES], int64_t c
 int u3d_trainer_counters(u3d_trainer *t, int64_t values[U3D_NUM_COUNTERS])
 {
     U3D_CHECK(t && values, U3D_ERR_INVALID, "null argument");
+    U3D_TRY(useDevice(t->ctx));
+    unsigned long long part[64] = {};
+    if (t->evalPairs.p) {
+        U3D_HIP(hipMemcpyAsync(part, t->evalPairs.p, sizeof part, hipMemcpyDeviceToHost, t->ctx->stream));
+        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
+    unsigned long long evals = 0;
+    for (unsigned long long p : part)
+        evals += p;
+    t->counters[U3D_NUM_COUNTERS - 1] = int64_t(evals);
     for (int k = 0; k < U3D_NUM_COUNTERS; ++k)
         values[k] = t->counters[k];
     return U3D_OK;

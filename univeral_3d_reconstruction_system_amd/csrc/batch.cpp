@@ -350,7 +350,7 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             li.render = it.rgb;
             li.target = t->targets.p + t->imageOffset[view];
             li.grad = dl.p + pix * 3;
-            li.result = t->stepStats.p + (size_t(t->iteration) * size_t(B) + size_t(k)) * 4;
+            li.result = t->statsSlot(size_t(t->iteration) * size_t(B) + size_t(k));
             li.partials = partials.p + part;
             li.width = W;
             li.height = H;

@@ -128,6 +128,11 @@ struct PinnedScratch {
 // if the stream has drained or faulted without the word arriving.
 int waitPinnedWord(const uint64_t *p, uint32_t seq, hipStream_t s, uint32_t *value,
                    bool *flag = nullptr);
+// Waits (the same way) until the pinned word *p no longer holds `pending`.
+int waitPinnedU32(const uint32_t *p, uint32_t pending, hipStream_t s);
+// The psnr word of a step's statistics before the device has stored them (a NaN payload the
+// loss reduction never produces).
+constexpr uint32_t kStatsPending = 0x7fc0dead;
 
 // Per-stage GPU time accumulated from HIP events recorded on the context's stream.
 struct StageTimer {

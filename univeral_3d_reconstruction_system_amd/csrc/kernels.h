@@ -107,7 +107,11 @@ __device__ __forceinline__ void lossFinalizeBlock(const LossFinalize &f)
     f.result[0] = total;
     f.result[1] = l1;
     f.result[2] = ssim;
-    f.result[3] = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
+    // psnr last, released at system scope: a host that reads it from pinned memory (the
+    // trainer's per-step statistics) also sees the three values before it
+    const float psnr = mse <= 1e-12 ? 99.0f : float(-10.0 * log10(mse));
+    __hip_atomic_store(reinterpret_cast<unsigned *>(&f.result[3]), __float_as_uint(psnr),
+                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Work-item scan of the segmented backward: segStart = exclusive scan of the per-tile segment
@@ -357,7 +361,7 @@ constexpr int kSegEntries = 64;
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
-                        const int *blockOrder = nullptr);
+                        const int *blockOrder = nullptr, unsigned long long *evalCount = nullptr);
 // Segmented backward: one work item per (tile, segment of kSegEntries list positions) from the
 // forward's per-tile segment counts. Scratch written here: segStart (tiles + 1 entries, the
 // exclusive scan of segCount), *totalItems, counter, itemTile (at least Σ segCount entries:

@@ -489,7 +489,7 @@ __device__ __forceinline__ void forwardQuadrant(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, const SegmentState &seg)
+    int *__restrict__ nContrib, const SegmentState &seg, unsigned long long *evalCount)
 {
     constexpr int kBatch = kSegEntries; // checkpoint segment, list entries
     static_assert(kBatch % kQStep == 0, "segments are whole steps");
@@ -741,6 +741,16 @@ __device__ __forceinline__ void forwardQuadrant(
             m = max(m, __shfl_xor(m, off, 64));
         if (lane == 0)
             seg.segCount[size_t(tile) * kSegCountsPerTile + quad] = uint32_t((m + kBatch - 1) / kBatch);
+    }
+    if (evalCount) {
+        // evaluated (pixel, Gaussian) pairs of this quadrant: the sum of its contributor counts,
+        // added to one of 64 counters so that the quadrants do not all hit one address
+        unsigned sum = inside ? unsigned(last) : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            sum += __shfl_xor(sum, off, 64);
+        if (lane == 0)
+            atomicAdd(&evalCount[blockIdx.x & 63], (unsigned long long)sum);
     }
 }
 
@@ -1047,7 +1057,8 @@ __global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder)
+    int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder,
+    unsigned long long *evalCount)
 {
     const int x = int(blockIdx.x), k = x >> 3;
     const int b = (k >> 2) * 8 + (x & 7);
@@ -1056,7 +1067,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
     const int blk = blockOrder ? blockOrder[b] : b;
     forwardQuadrant<kCkpt, kFastExp>(blk % tilesX, blk / tilesX, k & 3, W, H, tilesX,
                                      tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2,
-                                     minAlpha, minT, rgb, finalT, nContrib, seg);
+                                     minAlpha, minT, rgb, finalT, nContrib, seg, evalCount);
 }
 
 template <bool kCkpt, bool kFastExp>
@@ -1317,7 +1328,7 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
-                        const int *blockOrder)
+                        const int *blockOrder, unsigned long long *evalCount)
 {
     U3D_CHECK(!seg.ckpt || g.tileSize == 16, U3D_ERR_INVALID, "segment checkpoints need 16-pixel tiles");
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
@@ -1331,7 +1342,7 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
         hipLaunchKernelGGL(quad, dim3(32 * ((numTiles + 7) / 8)), dim3(kThreads), 0, s, g.width,
                            g.height, g.tilesX, numTiles, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2,
                            st.background[0], st.background[1], st.background[2], st.min_alpha,
-                           st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder);
+                           st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder, evalCount);
         U3D_LAUNCH_CHECK();
         return U3D_OK;
     }

@@ -146,13 +146,20 @@ struct u3d_trainer {
     size_t orderPos = 0;
     int iteration = 0;
     bool opacityWasReset = false;
-    u3d::DevBuf<float> render, dloss, gradAccum, stepStats, evalOut;
+    u3d::DevBuf<float> render, dloss, gradAccum, evalOut;
+    // Per-step loss statistics {total, l1, ssim, psnr} per view, [iterations][B][4], in pinned
+    // host memory: the loss's final reduction stores them there (psnr last, with a release),
+    // over a sentinel in the psnr slot, so the host reads a step's values without a stream sync.
+    u3d::PinnedScratch stepStats;
+    float *statsSlot(size_t k) { return static_cast<float *>(stepStats.p) + k * 4; }
     u3d::DevBuf<int> gradCount;
     u3d::DevBuf<double> partials;
     int statsFolded = 0;
     u3d_train_stats stats{};
-    // steps, Gaussians, pixels, pairs, views, steps whose pair list overflowed its capacity
+    // steps, Gaussians, pixels, pairs, views, steps whose pair list overflowed its capacity;
+    // the evaluated pairs (last counter) accumulate on the device, in evalPairs
     int64_t counters[U3D_NUM_COUNTERS] = {};
+    u3d::DevBuf<unsigned long long> evalPairs; // 64 partial counts
     // Spherical-harmonic coefficients beyond DC (extension), [cap][15][3].
     int shDegree = 0;
     u3d::DevBuf<float> shP, shG, shM, shV, shSpare;
@@ -165,9 +172,11 @@ struct u3d_trainer {
 
 namespace u3d {
 
+// evalCount (nullable, 64 device counters): the forward adds its evaluated (pixel, Gaussian)
+// pairs, the sum of the contributor counts, when it composites 16-pixel tiles.
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
                       const u3d_raster_settings &st, float *rgb, const float *shRest = nullptr,
-                      int shDegree = 0);
+                      int shDegree = 0, unsigned long long *evalCount = nullptr);
 void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2);
 // Next view of the trainer's shuffled epoch order (Trainer.cpp:57-64).
 int trainerNextView(u3d_trainer *t);
