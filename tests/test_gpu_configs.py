@@ -234,16 +234,21 @@ def test_config0_sh3_trainer_follows_oracle(ctx):
     assert abs(su["loss"] - so["loss"]) <= 1e-4 * abs(so["loss"])
 
 
-def test_trainer_densify_and_reset_follow_oracle(ctx):
+@pytest.mark.parametrize("degree", [0, 3])
+def test_trainer_densify_and_reset_follow_oracle(ctx, degree):
     """Densification every 10 steps from step 10 and an opacity reset at step 20, against the
     oracle trainer (Trainer.cpp:94-178): the same model size after every step (same decisions,
     the same split draws from the shared generator), opacities capped at logit(0.01) with zeroed
     opacity moments right after the reset, the large-scale prune active afterwards, and the
-    parameters close at the end."""
+    parameters close at the end. With SH degree 3 the coefficient planes (and their moments)
+    follow the compaction and the appended clones and split children."""
     truth, cams = config0_scene(3000, 4, 160, 120, 8)
-    images = [S.quantize(O.Raster().forward(gaussians(truth), c, O.make_settings())) for c in cams]
+    rest_t = S.rng(6).normal(0, 0.1, 3000 * 45).astype(np.float32)
+    images = [S.quantize(O.Raster().forward(gaussians(truth), c, O.make_settings(), sh_rest=rest_t,
+                                            sh_degree=3)) for c in cams]
     init, _ = config0_scene(3000, 5, 160, 120, 8)
     init["scales"] = init["scales"] + np.float32(1.0)  # some Gaussians above the split size
+    rest0 = S.rng(7).normal(0, 0.05, 3000 * 45).astype(np.float32) if degree else None
 
     def opts(mod):
         o = mod.options_for_iterations(70)
@@ -251,10 +256,14 @@ def test_trainer_densify_and_reset_follow_oracle(ctx):
         o.opacity_reset_interval = 20
         o.densify_grad_threshold = 5e-5
         o.max_gaussians = 4000
+        if mod is U:
+            o.sh_degree = degree
         return o
 
     tu = U.Trainer(ctx, init, cams, images, 3.0, opts(U))
-    to = O.Trainer(gaussians(init), cams, images, 3.0, opts(O))
+    if degree:
+        tu.set_sh_rest(rest0)
+    to = O.Trainer(gaussians(init), cams, images, 3.0, opts(O), sh_degree=degree, sh_rest=rest0)
     sizes = []
     cap = np.float32(np.log(np.float32(0.01) / (np.float32(1) - np.float32(0.01))))
     for it in range(1, 71):
@@ -272,6 +281,8 @@ def test_trainer_densify_and_reset_follow_oracle(ctx):
     mu, mo = tu.model(), to.model()
     for f, _ in O.FIELDS:
         assert _rel(mu[f], getattr(mo, f)) < 2e-3, (f, _rel(mu[f], getattr(mo, f)))
+    if degree:
+        assert _rel(tu.sh_rest(), to.sh_rest()) < 2e-3, _rel(tu.sh_rest(), to.sh_rest())
 
 
 def test_evaluate_psnr_and_l1_match_oracle(ctx):

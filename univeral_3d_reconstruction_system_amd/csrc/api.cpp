@@ -127,10 +127,11 @@ using namespace u3d;
 namespace u3d {
 
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
-                      const u3d_raster_settings &st, float *rgb, const float *shRest, int shDegree,
+                      const u3d_raster_settings &st, float *rgb, const ShCoefs *shp,
                       unsigned long long *evalCount)
 {
-    U3D_CHECK(shDegree >= 0 && shDegree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    const ShCoefs sh = shp ? *shp : ShCoefs{};
+    U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
     U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
               "tile_size must be a positive multiple of 16");
     U3D_CHECK(cam.width >= 0 && cam.height >= 0, U3D_ERR_INVALID, "negative image size");
@@ -177,7 +178,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     const uint32_t seq = r->totalSeq;
     {
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
-        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, shRest, shDegree, r->blockTiles.p));
+        U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, sh, r->blockTiles.p));
     }
     std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
     // Depth order of the visible Gaussians (invisible keys sort last; ties keep index order).
@@ -309,10 +310,11 @@ int segmentPrepArgs(u3d_raster *r, SegmentPrep *p)
 // prepDone: the work-item scan already ran (with the loss, segmentPrepArgs' buffers).
 int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
                        u3d_gaussians &grads, float *ndc, int *gradCount,
-                       const float *shRest = nullptr, int shDegree = 0, float *gShRest = nullptr,
-                       bool consumeGrad2d = false, const LossFinalize *lossFinalize = nullptr,
+                       const ShCoefs *shp = nullptr, bool consumeGrad2d = false,
+                       const LossFinalize *lossFinalize = nullptr,
                        const AdamStepArgs *adam = nullptr, bool prepDone = false)
 {
+    const ShCoefs sh = shp ? *shp : ShCoefs{};
     U3D_CHECK(r->haveForward, U3D_ERR_INVALID, "backward without a forward");
     U3D_CHECK(m.n == r->n && grads.n == r->n, U3D_ERR_INVALID, "model size differs from the last forward");
     const hipStream_t s = r->ctx->stream;
@@ -345,13 +347,12 @@ int rasterBackwardImpl(u3d_raster *r, const u3d_gaussians &m, const float *dl,
     StageScope scope(r->ctx, U3D_STAGE_CHAIN);
     if (adam) {
         // the trainer's step: chain rule and Adam in one pass, grads untouched
-        U3D_TRY(launchChainAdam(s, m, r->cam, r->st, sb, r->grad2d.p, ndc, gradCount, shRest,
-                                shDegree, gShRest, *adam));
+        U3D_TRY(launchChainAdam(s, m, r->cam, r->st, sb, r->grad2d.p, ndc, gradCount, sh, *adam));
         r->grad2dZero = true;
         return U3D_OK;
     }
-    U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount, shRest,
-                            shDegree, gShRest, consumeGrad2d));
+    U3D_TRY(launchChainRule(s, m, r->cam, r->st, sb, r->grad2d.p, grads, ndc, gradCount, sh,
+                            consumeGrad2d));
     r->grad2dZero = consumeGrad2d;
     return U3D_OK;
 }
@@ -504,7 +505,11 @@ int u3d_raster_forward_sh(u3d_raster *r, const u3d_gaussians *model, const float
     U3D_TRY(useDevice(r->ctx));
     U3D_TRY(checkModel(model, "model"));
     U3D_CHECK(rgb != nullptr || size_t(camera->width) * size_t(camera->height) == 0, U3D_ERR_INVALID, "null rgb");
-    return rasterForwardImpl(r, *model, *camera, *settings, rgb, sh_rest, sh_rest ? sh_degree : 0);
+    // the caller's coefficients are [n][15][3]
+    ShCoefs sh;
+    sh.coef = const_cast<float *>(sh_rest);
+    sh.degree = sh_rest ? sh_degree : 0;
+    return rasterForwardImpl(r, *model, *camera, *settings, rgb, &sh);
 }
 
 int u3d_raster_backward_sh(u3d_raster *r, const u3d_gaussians *model, const float *sh_rest,
@@ -516,8 +521,11 @@ int u3d_raster_backward_sh(u3d_raster *r, const u3d_gaussians *model, const floa
     U3D_TRY(useDevice(r->ctx));
     U3D_TRY(checkModel(model, "model"));
     U3D_TRY(checkModel(grads, "grads"));
-    return rasterBackwardImpl(r, *model, dl_drgb, *grads, mean2d_grad_ndc, nullptr, sh_rest,
-                              sh_rest ? sh_degree : 0, grad_sh_rest);
+    ShCoefs sh;
+    sh.coef = const_cast<float *>(sh_rest);
+    sh.grad = grad_sh_rest;
+    sh.degree = sh_rest ? sh_degree : 0;
+    return rasterBackwardImpl(r, *model, dl_drgb, *grads, mean2d_grad_ndc, nullptr, &sh);
 }
 
 int u3d_raster_radii(u3d_raster *r, const int **radii, int64_t *n)
@@ -1053,13 +1061,14 @@ int trainerDensify(u3d_trainer *t)
     t->params.swap(t->spare);
     t->params.n = newN;
     if (t->shDegree > 0) {
-        U3D_TRY(launchScatterRows(s, t->shP.p, t->shSpare.p, 45, n, keep.p, kind.p, ds.keepPos.p,
-                                  ds.extraPos.p, ds.counts.p, false));
+        // coefficient planes of stride n -> planes of stride newN
+        U3D_TRY(launchScatterPlanes(s, t->shP.p, t->shSpare.p, 45, n, newN, keep.p, kind.p,
+                                    ds.keepPos.p, ds.extraPos.p, ds.counts.p, false));
         std::swap(t->shP.p, t->shSpare.p);
         std::swap(t->shP.cap, t->shSpare.cap);
         for (DevBuf<float> *mom : {&t->shM, &t->shV}) {
-            U3D_TRY(launchScatterRows(s, mom->p, t->shSpare.p, 45, n, keep.p, kind.p, ds.keepPos.p,
-                                      ds.extraPos.p, ds.counts.p, true));
+            U3D_TRY(launchScatterPlanes(s, mom->p, t->shSpare.p, 45, n, newN, keep.p, kind.p,
+                                        ds.keepPos.p, ds.extraPos.p, ds.counts.p, true));
             std::swap(mom->p, t->shSpare.p);
             std::swap(mom->cap, t->shSpare.cap);
         }
@@ -1162,8 +1171,8 @@ int u3d_trainer_step(u3d_trainer *t)
     const u3d_camera &cam = t->cams[view];
     const hipStream_t s = t->ctx->stream;
     const u3d_gaussians pv = t->params.view();
-    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree,
-                              t->evalPairs.p));
+    const ShCoefs shv = trainerSh(t);
+    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, &shv, t->evalPairs.p));
     t->counters[0] += 1;
     t->counters[1] += t->params.n;
     t->counters[2] += int64_t(cam.width) * cam.height;
@@ -1186,8 +1195,6 @@ int u3d_trainer_step(u3d_trainer *t)
     }
     t->grads.n = t->params.n;
     u3d_gaussians gv = t->grads.view();
-    if (t->shDegree > 0 && t->params.n > 0)
-        U3D_HIP(hipMemsetAsync(t->shG.p, 0, size_t(t->params.n) * 45 * sizeof(float), s));
     // Exponentially decaying position rate scaled by the scene size (Trainer.cpp:82-84).
     u3d_learning_rates lr = t->opt.lr;
     lr.means = trainerMeansLr(t);
@@ -1196,18 +1203,19 @@ int u3d_trainer_step(u3d_trainer *t)
     adamCoefficients(t->adamStep, lr, adam.stepSize, &adam.invSqrtBc2);
     adam.m = t->adamM.view();
     adam.v = t->adamV.view();
-    const float inv = adam.invSqrtBc2;
     // Backward, then the chain rule and Adam in one pass: the 14 parameter gradients of a
-    // Gaussian stay in registers (the gradient buffer is not used by this step).
-    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
-                               t->shP.p, t->shDegree, t->shG.p, true, &lossFinalize, &adam,
-                               prepWithLoss));
+    // Gaussian stay in registers (the gradient buffer is not used by this step), and its SH
+    // coefficients take their update in the same pass (rate lr.colors * scale, Adam.cpp:70-72).
+    ShCoefs shAdam = shv;
     if (t->shDegree > 0) {
-        StageScope scope(t->ctx, U3D_STAGE_ADAM);
         const float bc1 = 1.0f - std::pow(0.9f, float(t->adamStep));
-        U3D_TRY(launchAdamArray(s, t->shP.p, t->shG.p, t->shM.p, t->shV.p, t->params.n * 45,
-                                (lr.colors * shRateScale(t)) / bc1, inv));
+        shAdam.m = t->shM.p;
+        shAdam.v = t->shV.p;
+        shAdam.stepSize = (lr.colors * shRateScale(t)) / bc1;
+        shAdam.invSqrtBc2 = adam.invSqrtBc2;
     }
+    U3D_TRY(rasterBackwardImpl(&t->raster, pv, t->dloss.p, gv, t->gradAccum.p, t->gradCount.p,
+                               &shAdam, true, &lossFinalize, &adam, prepWithLoss));
     return trainerFinishStep(t);
 }
 
@@ -1243,18 +1251,36 @@ int u3d_trainer_get_moments(u3d_trainer *t, float *m_host, float *v_host)
     return U3D_OK;
 }
 
+} // extern "C"
+
+namespace {
+// The trainer keeps coefficient c of Gaussian i at plane[c * n + i]; the API's layout is
+// [n][15][3] (row i, column c).
+int shPlanesToRows(u3d_trainer *t, const float *planes, float *rows)
+{
+    const size_t n = size_t(t->params.n);
+    std::vector<float> tmp(n * 45);
+    U3D_HIP(hipMemcpyAsync(tmp.data(), planes, tmp.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           t->ctx->stream));
+    U3D_HIP(hipStreamSynchronize(t->ctx->stream));
+    for (size_t c = 0; c < 45; ++c)
+        for (size_t i = 0; i < n; ++i)
+            rows[i * 45 + c] = tmp[c * n + i];
+    return U3D_OK;
+}
+} // namespace
+
+extern "C" {
+
 int u3d_trainer_get_sh_rest(u3d_trainer *t, float *sh_rest_host)
 {
     U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
     U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
     U3D_TRY(useDevice(t->ctx));
-    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
-    if (bytes) {
-        U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null output");
-        U3D_HIP(hipMemcpyAsync(sh_rest_host, t->shP.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
-        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
-    }
-    return U3D_OK;
+    if (t->params.n == 0)
+        return U3D_OK;
+    U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null output");
+    return shPlanesToRows(t, t->shP.p, sh_rest_host);
 }
 
 int u3d_trainer_set_sh_rest(u3d_trainer *t, const float *sh_rest_host)
@@ -1262,12 +1288,17 @@ int u3d_trainer_set_sh_rest(u3d_trainer *t, const float *sh_rest_host)
     U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
     U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
     U3D_TRY(useDevice(t->ctx));
-    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
-    if (bytes) {
-        U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null input");
-        U3D_HIP(hipMemcpyAsync(t->shP.p, sh_rest_host, bytes, hipMemcpyHostToDevice, t->ctx->stream));
-        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
-    }
+    const size_t n = size_t(t->params.n);
+    if (n == 0)
+        return U3D_OK;
+    U3D_CHECK(sh_rest_host != nullptr, U3D_ERR_INVALID, "null input");
+    std::vector<float> planes(n * 45);
+    for (size_t c = 0; c < 45; ++c)
+        for (size_t i = 0; i < n; ++i)
+            planes[c * n + i] = sh_rest_host[i * 45 + c];
+    U3D_HIP(hipMemcpyAsync(t->shP.p, planes.data(), planes.size() * sizeof(float),
+                           hipMemcpyHostToDevice, t->ctx->stream));
+    U3D_HIP(hipStreamSynchronize(t->ctx->stream));
     return U3D_OK;
 }
 
@@ -1276,14 +1307,11 @@ int u3d_trainer_get_sh_moments(u3d_trainer *t, float *m_host, float *v_host)
     U3D_CHECK(t != nullptr, U3D_ERR_INVALID, "null trainer");
     U3D_CHECK(t->shDegree > 0, U3D_ERR_INVALID, "trainer has no SH coefficients (sh_degree 0)");
     U3D_TRY(useDevice(t->ctx));
-    const size_t bytes = size_t(t->params.n) * 45 * sizeof(float);
-    if (bytes) {
-        U3D_CHECK(m_host && v_host, U3D_ERR_INVALID, "null output");
-        U3D_HIP(hipMemcpyAsync(m_host, t->shM.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
-        U3D_HIP(hipMemcpyAsync(v_host, t->shV.p, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
-        U3D_HIP(hipStreamSynchronize(t->ctx->stream));
-    }
-    return U3D_OK;
+    if (t->params.n == 0)
+        return U3D_OK;
+    U3D_CHECK(m_host && v_host, U3D_ERR_INVALID, "null output");
+    U3D_TRY(shPlanesToRows(t, t->shM.p, m_host));
+    return shPlanesToRows(t, t->shV.p, v_host);
 }
 
 int u3d_trainer_get_densify_stats(u3d_trainer *t, float *accum_host, int *count_host)
@@ -1352,7 +1380,8 @@ int evaluateAll(u3d_trainer *t, int mode, float *out)
     const u3d_gaussians pv = t->params.view();
     for (size_t k = 0; k < t->cams.size(); ++k) {
         const u3d_camera &cam = t->cams[k];
-        U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, t->shP.p, t->shDegree));
+        const ShCoefs shv = trainerSh(t);
+        U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, &shv));
         const int64_t count = int64_t(cam.width) * cam.height * 3;
         if (mode == 0)
             U3D_TRY(launchPsnr(s, t->render.p, nullptr, t->targets.p + t->imageOffset[k], count,
@@ -1391,8 +1420,8 @@ int u3d_trainer_render(u3d_trainer *t, const u3d_camera *camera, float *rgb_host
     const size_t count = size_t(camera->width) * size_t(camera->height) * 3;
     DevBuf<float> out;
     U3D_TRY(out.ensure(count > 0 ? count : 1));
-    U3D_TRY(rasterForwardImpl(&t->raster, t->params.view(), *camera, t->opt.raster, out.p, t->shP.p,
-                              t->shDegree));
+    const ShCoefs shv = trainerSh(t);
+    U3D_TRY(rasterForwardImpl(&t->raster, t->params.view(), *camera, t->opt.raster, out.p, &shv));
     U3D_HIP(hipMemcpyAsync(rgb_host, out.p, count * sizeof(float), hipMemcpyDeviceToHost, t->ctx->stream));
     U3D_HIP(hipStreamSynchronize(t->ctx->stream));
     return U3D_OK;

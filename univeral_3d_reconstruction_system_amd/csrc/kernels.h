@@ -32,7 +32,8 @@ struct RenderItem {
     int tileBase, tilesX, tilesY, blocksX;
     int shDegree;
     float ndcScale;     // densification statistic per view: undoes the 1/B of the averaged loss
-    const float *means, *scales, *quats, *opacities, *colors, *shRest;
+    const float *means, *scales, *quats, *opacities, *colors;
+    float *shRest; // coefficient planes of the scene (stride n), or null
     float *rgb, *finalT;
     int *nContrib;
     const float *dl;
@@ -188,18 +189,31 @@ struct SceneStep {
     int64_t n, cap;
     float stepSize[5];
     float invSqrtBc2;
-    float *shP, *shG, *shM, *shV; // 45 floats per Gaussian when shDegree > 0
+    float *shP, *shG, *shM, *shV; // 45 coefficient planes of n floats when shDegree > 0
     float shStepSize;
     int shDegree;
     float beta1, beta2;           // Adam decay rates (0.9, 0.999 unless batch-scaled)
 };
 
 // ---- kernels_project.hip
-// shRest (nullable): 15 x 3 SH coefficients per Gaussian used up to shDegree (1..3).
+// Spherical-harmonic colour beyond DC, degree 1..3 (an extension). Coefficient c (0..44,
+// c = 3 k + colour channel, basis function k) of Gaussian i is coef[i * gs + c * cs]: the API
+// layout [n][15][3] is gs = 45, cs = 1; the trainer keeps one plane per coefficient, gs = 1 and
+// cs = its current size n, so the lanes of a wavefront read consecutive addresses.
+struct ShCoefs {
+    float *coef = nullptr;   // read (written only by the fused Adam)
+    float *grad = nullptr;   // gradient in the same layout, added to (nullable)
+    int64_t gs = 45, cs = 1;
+    int degree = 0;          // 0: no view-dependent colour
+    // The trainer's fused update: Adam moments in the same layout (nullable: none). Every
+    // coefficient of every Gaussian is updated, with a zero gradient where it has none.
+    float *m = nullptr, *v = nullptr;
+    float stepSize = 0.0f, invSqrtBc2 = 0.0f;
+};
 // blockTiles (optional, projectBlocks(n) entries): per projection block, the tiles covered.
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                   const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
-                  const float *shRest, int shDegree, uint32_t *blockTiles = nullptr);
+                  const ShCoefs &sh, uint32_t *blockTiles = nullptr);
 int projectBlocks(int64_t n);
 // Chain rule of the backward pass (Rasterizer.cpp:432-544): per visible Gaussian, ADDS into
 // grads / ndc. gradCount (nullable) is incremented where radius > 0 (Trainer.cpp:76-79).
@@ -215,14 +229,14 @@ struct AdamStepArgs {
 // its Adam update from them (Adam runs for every Gaussian, visible or not), so the parameter
 // gradients never reach memory; the 2D gradients are cleared after use. Same arithmetic as
 // launchChainRule into zeroed gradients followed by launchAdamStep.
+// sh: with m / v set, the SH coefficients take their Adam update in the same pass.
 int launchChainAdam(hipStream_t s, const u3d_gaussians &params, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
-                    float *ndc, int *gradCount, const float *shRest, int shDegree, float *gShRest,
-                    const AdamStepArgs &adam);
+                    float *ndc, int *gradCount, const ShCoefs &sh, const AdamStepArgs &adam);
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
-                    u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
-                    int shDegree, float *gShRest, bool clearGrad2d = false);
+                    u3d_gaussians &grads, float *ndc, int *gradCount, const ShCoefs &sh,
+                    bool clearGrad2d = false);
 // Batched projection over items (grid y = item); entryItem[e] = owning item of screen entry e.
 int launchProjectBatched(hipStream_t s, const RenderItem *items, int numItems, int64_t maxN,
                          const u3d_raster_settings &st, const ScreenBuffers &out, int *entryItem);
@@ -500,6 +514,11 @@ int launchAdamArray(hipStream_t s, float *p, const float *g, float *m, float *v,
 int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, int64_t n,
                       const uint8_t *keep, const uint8_t *kind, const uint32_t *keepPos,
                       const uint32_t *extraPos, const uint32_t *counts, bool zeroRows);
+// The same for `planes` planes of n floats (stride n) into planes of stride newN.
+int launchScatterPlanes(hipStream_t s, const float *src, float *dst, int planes, int64_t n,
+                        int64_t newN, const uint8_t *keep, const uint8_t *kind,
+                        const uint32_t *keepPos, const uint32_t *extraPos,
+                        const uint32_t *counts, bool zeroRows);
 // Adam over every scene of a batched step (blockIdx.y = scene); clears the gradients it read.
 int launchAdamBatched(hipStream_t s, const SceneStep *scenes, int numScenes, int64_t maxN);
 int launchFill(hipStream_t s, float *p, float value, int64_t count);

@@ -325,6 +325,31 @@ __global__ __launch_bounds__(kThreads) void scatterRowsKernel(
     }
 }
 
+// Densification of planes of n floats (plane stride n) into planes of stride newN: the kept
+// entry i goes to keepPos[i], its extras (1 clone, 2 split children) after the kept ones.
+__global__ __launch_bounds__(kThreads) void scatterPlanesKernel(
+    const float *__restrict__ src, float *__restrict__ dst, int planes, int64_t n, int64_t newN,
+    const uint8_t *__restrict__ keep, const uint8_t *__restrict__ kind,
+    const uint32_t *__restrict__ keepPos, const uint32_t *__restrict__ extraPos,
+    const uint32_t *__restrict__ counts, int zeroRows)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n)
+        return;
+    const int64_t nKeep = counts[2];
+    const bool kept = keep[i] != 0;
+    const int copies = kind[i] == 1 ? 1 : (kind[i] == 2 ? 2 : 0);
+    const int64_t kp = kept ? int64_t(keepPos[i]) : 0, ep = nKeep + int64_t(extraPos[i]);
+    for (int c = 0; c < planes; ++c) {
+        const float v = src[int64_t(c) * n + i];
+        float *plane = dst + int64_t(c) * newN;
+        if (kept)
+            plane[kp] = v;
+        for (int k = 0; k < copies; ++k)
+            plane[ep + k] = zeroRows ? 0.0f : v;
+    }
+}
+
 inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
 } // namespace
@@ -351,6 +376,19 @@ int launchScatterRows(hipStream_t s, const float *src, float *dst, int width, in
         return U3D_OK;
     hipLaunchKernelGGL(scatterRowsKernel, dim3(blocksFor(n)), dim3(kThreads), 0, s, src, dst, width,
                        n, keep, kind, keepPos, extraPos, counts, zeroRows ? 1 : 0);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchScatterPlanes(hipStream_t s, const float *src, float *dst, int planes, int64_t n,
+                        int64_t newN, const uint8_t *keep, const uint8_t *kind,
+                        const uint32_t *keepPos, const uint32_t *extraPos,
+                        const uint32_t *counts, bool zeroRows)
+{
+    if (n <= 0)
+        return U3D_OK;
+    hipLaunchKernelGGL(scatterPlanesKernel, dim3(blocksFor(n)), dim3(kThreads), 0, s, src, dst, planes,
+                       n, newN, keep, kind, keepPos, extraPos, counts, zeroRows ? 1 : 0);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

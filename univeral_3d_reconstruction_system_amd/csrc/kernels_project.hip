@@ -10,17 +10,23 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Basis functions of SH degree kDeg (1..3), or of the runtime degree (kDeg < 0; sh.coef
+// non-null); 0: none. A compile-time degree keeps the basis arrays in registers.
+template <int kDeg>
+__device__ __forceinline__ int shTerms(const ShCoefs &sh)
+{
+    return kDeg > 0 ? shCount(kDeg) : (kDeg < 0 && sh.coef ? shCount(sh.degree) : 0);
+}
+
 // Projects Gaussian i of a model and writes its screen state at entry o; returns the number of
-// tiles it covers.
-// kSH: the spherical-harmonics colour terms are compiled in (degree > 0 callers); without them
-// the kernel needs no private memory for the basis arrays.
-template <bool kSH>
+// tiles it covers. kDeg: the SH degree of the colour (0: the reference's DC colour; < 0: the
+// runtime degree of sh).
+template <int kDeg>
 __device__ __forceinline__ uint32_t projectBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
-    int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
-    float3 campos)
+    int tilesX, int tilesY, ScreenBuffers out, const ShCoefs &sh, float3 campos)
 {
     int radius = 0;
     uint8_t flags = 0;
@@ -59,12 +65,12 @@ __device__ __forceinline__ uint32_t projectBody(
             break;
         flags = uint8_t((p.clampX ? 1 : 0) | (p.clampY ? 2 : 0));
         float rgb[3];
-        const int nb = (kSH && shRest) ? shCount(shDegree) : 0;
+        const int nb = shTerms<kDeg>(sh);
         float basis[15];
         if (nb) {
             float d[3], len;
             viewDir(mu, campos, d, &len);
-            shBasis(d[0], d[1], d[2], shDegree, basis, nullptr);
+            shBasis(d[0], d[1], d[2], kDeg > 0 ? kDeg : sh.degree, basis, nullptr);
         }
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
@@ -74,7 +80,7 @@ __device__ __forceinline__ uint32_t projectBody(
             } else {
                 float sum = kShC0 * colors[i * 3 + ch];
                 for (int k = 0; k < nb; ++k)
-                    sum += basis[k] * shRest[i * 45 + k * 3 + ch];
+                    sum += basis[k] * sh.coef[i * sh.gs + int64_t(k * 3 + ch) * sh.cs];
                 col = 0.5f + sum;
             }
             if (col < 0.0f)
@@ -98,19 +104,18 @@ __device__ __forceinline__ uint32_t projectBody(
     return nt;
 }
 
-template <bool kSH>
+template <int kDeg>
 __global__ __launch_bounds__(kThreads) void projectKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, const float *__restrict__ opac,
     const float *__restrict__ colors, u3d_camera cam, float nearPlane, float eps2d, int ts,
-    int tilesX, int tilesY, ScreenBuffers out, const float *__restrict__ shRest, int shDegree,
-    float3 campos, uint32_t *blockTiles)
+    int tilesX, int tilesY, ScreenBuffers out, ShCoefs sh, float3 campos, uint32_t *blockTiles)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     uint32_t nt = 0;
     if (i < n)
-        nt = projectBody<kSH>(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts,
-                         tilesX, tilesY, out, shRest, shDegree, campos);
+        nt = projectBody<kDeg>(i, i, means, scales, quats, opac, colors, cam, nearPlane, eps2d, ts,
+                               tilesX, tilesY, out, sh, campos);
     if (!blockTiles)
         return;
     // this block's sum of the covered tiles
@@ -142,8 +147,13 @@ __global__ __launch_bounds__(kThreads) void projectBatchedKernel(const RenderIte
         return;
     const int64_t o = it.screenBase + i;
     entryItem[o] = int(blockIdx.y);
-    projectBody<true>(i, o, it.means, it.scales, it.quats, it.opacities, it.colors, it.cam, nearPlane,
-                eps2d, ts, it.tilesX, it.tilesY, out, it.shRest, it.shDegree, cameraCentre(it.cam));
+    ShCoefs sh;
+    sh.coef = it.shRest;
+    sh.gs = 1;
+    sh.cs = it.n;
+    sh.degree = it.shRest ? it.shDegree : 0;
+    projectBody<-1>(i, o, it.means, it.scales, it.quats, it.opacities, it.colors, it.cam, nearPlane,
+                    eps2d, ts, it.tilesX, it.tilesY, out, sh, cameraCentre(it.cam));
 }
 
 // Where chainBody adds the parameter gradients of Gaussian i: the model-shaped gradient arrays
@@ -192,15 +202,15 @@ __device__ __forceinline__ float dot3(const float *a, const float *b)
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o.
-template <bool kSH, class Sink>
+// Chain rule for Gaussian i of a model whose screen state / 2D gradients sit at entry o. The
+// SH coefficients' gradients go to sh.grad, or (sh.m set) their Adam update is applied in place.
+template <int kDeg, class Sink>
 __device__ __forceinline__ void chainBody(
     int64_t i, int64_t o, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
     const float4 *__restrict__ rec1, const float *__restrict__ grad2d, Sink &out, float *ndc,
-    int *gradCount, const float *__restrict__ shRest, int shDegree, float3 campos,
-    float *gShRest, float ndcScale)
+    int *gradCount, const ShCoefs &sh, float3 campos, float ndcScale)
 {
     if (radius[o] == 0)
         return;
@@ -227,26 +237,38 @@ __device__ __forceinline__ void chainBody(
     out.opacity(g[8] * alpha * (1.0f - alpha));
 
     const float mu[3] = {means[i * 3], means[i * 3 + 1], means[i * 3 + 2]};
-    // View-dependent colour: coefficient gradients and the gradient w.r.t. the unit direction.
-    const int nb = (kSH && shRest) ? shCount(shDegree) : 0;
+    // View-dependent colour: coefficient gradients (or their fused Adam update) and the
+    // gradient w.r.t. the unit view direction, from the coefficients before the update.
+    const int nb = shTerms<kDeg>(sh);
     float dDir[3] = {0.0f, 0.0f, 0.0f}, viewD[3] = {0.0f, 0.0f, 0.0f}, viewLen = 1.0f;
     if (nb) {
         float b[15];
         float3 gb[15];
         viewDir(mu, campos, viewD, &viewLen);
-        shBasis(viewD[0], viewD[1], viewD[2], shDegree, b, gb);
+        shBasis(viewD[0], viewD[1], viewD[2], kDeg > 0 ? kDeg : sh.degree, b, gb);
+#pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            if (flags & (4 << ch))
-                continue;
+            const bool on = !(flags & (4 << ch)); // a clamped channel passes no gradient
             const float gc = g[5 + ch];
             for (int k = 0; k < nb; ++k) {
-                const float coef = shRest[i * 45 + k * 3 + ch];
-                if (gShRest)
-                    gShRest[i * 45 + k * 3 + ch] += b[k] * gc;
-                const float w = coef * gc;
-                dDir[0] += w * gb[k].x;
-                dDir[1] += w * gb[k].y;
-                dDir[2] += w * gb[k].z;
+                const int64_t idx = i * sh.gs + int64_t(k * 3 + ch) * sh.cs;
+                const float coef = sh.coef[idx];
+                const float gk = on ? b[k] * gc : 0.0f;
+                if (on) {
+                    const float w = coef * gc;
+                    dDir[0] += w * gb[k].x;
+                    dDir[1] += w * gb[k].y;
+                    dDir[2] += w * gb[k].z;
+                }
+                if (sh.m) {
+                    float pp = coef, mm = sh.m[idx], vv = sh.v[idx];
+                    adamScalar(pp, mm, vv, gk, sh.stepSize, sh.invSqrtBc2);
+                    sh.coef[idx] = pp;
+                    sh.m[idx] = mm;
+                    sh.v[idx] = vv;
+                } else if (sh.grad && on) {
+                    sh.grad[idx] += gk;
+                }
             }
         }
     }
@@ -358,21 +380,21 @@ __device__ __forceinline__ void chainBody(
 
 // clear: the 2D gradients are zeroed after use (only visible Gaussians' entries are ever
 // written, so the buffer is all zero again for the next backward).
-template <bool kSH>
+template <int kDeg>
 __global__ __launch_bounds__(kThreads) void chainRuleKernel(
     int64_t n, const float *__restrict__ means, const float *__restrict__ scales,
     const float *__restrict__ quats, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
     const float4 *__restrict__ rec1, float *grad2d, float *gMeans,
     float *gScales, float *gQuats, float *gOpac, float *gColors, float *ndc, int *gradCount,
-    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest, bool clear)
+    ShCoefs sh, float3 campos, bool clear)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n)
         return;
     GradArrays out{gMeans, gScales, gQuats, gOpac, gColors, i};
-    chainBody<kSH>(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1, grad2d,
-              out, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
+    chainBody<kDeg>(i, i, means, scales, quats, cam, nearPlane, eps2d, radius, flagsArr, rec1,
+                    grad2d, out, ndc, gradCount, sh, campos, 1.0f);
     if (clear && radius[i] != 0) {
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
@@ -383,24 +405,34 @@ __global__ __launch_bounds__(kThreads) void chainRuleKernel(
 // Chain rule of Gaussian i into registers, then the Adam update of its 14 parameters from them.
 // The registers start at zero and take the same terms in the same order as the zeroed gradient
 // arrays would, so the update equals chainRuleKernel + adamKernel bit for bit.
-template <bool kSH>
+template <int kDeg>
 __global__ __launch_bounds__(kThreads) void chainAdamKernel(
     int64_t n, u3d_gaussians prm, u3d_camera cam, float nearPlane, float eps2d,
     const int *__restrict__ radius, const uint8_t *__restrict__ flagsArr,
-    const float4 *__restrict__ rec1, float *grad2d, float *ndc, int *gradCount,
-    const float *__restrict__ shRest, int shDegree, float3 campos, float *gShRest,
-    AdamStepArgs adam)
+    const float4 *__restrict__ rec1, float *grad2d, float *ndc, int *gradCount, ShCoefs sh,
+    float3 campos, AdamStepArgs adam)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n)
         return;
     GradRegs gr;
-    chainBody<kSH>(i, i, prm.means, prm.scales, prm.quats, cam, nearPlane, eps2d, radius, flagsArr,
-              rec1, grad2d, gr, ndc, gradCount, shRest, shDegree, campos, gShRest, 1.0f);
+    chainBody<kDeg>(i, i, prm.means, prm.scales, prm.quats, cam, nearPlane, eps2d, radius, flagsArr,
+                    rec1, grad2d, gr, ndc, gradCount, sh, campos, 1.0f);
     if (radius[i] != 0) {
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
             grad2d[i * kGrad2dStride + k] = 0.0f;
+    } else if (kDeg != 0 && sh.m) {
+        // not visible: its coefficients still take the step with a zero gradient
+        const int nb = shTerms<kDeg>(sh);
+        for (int c = 0; c < 3 * nb; ++c) {
+            const int64_t idx = i * sh.gs + int64_t(c) * sh.cs;
+            float pp = sh.coef[idx], mm = sh.m[idx], vv = sh.v[idx];
+            adamScalar(pp, mm, vv, 0.0f, sh.stepSize, sh.invSqrtBc2);
+            sh.coef[idx] = pp;
+            sh.m[idx] = mm;
+            sh.v[idx] = vv;
+        }
     }
     const float inv = adam.invSqrtBc2;
     // each group's parameter, first and second moment as one 12- or 16-byte access
@@ -455,9 +487,15 @@ __global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem 
         if (sb.radius[o] == 0)
             continue;
         GradArrays out{it.gMeans, it.gScales, it.gQuats, it.gOpac, it.gColors, i};
-        chainBody<true>(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
-                  sb.flags, sb.rec1, grad2d, out, it.ndc, it.gradCount, it.shRest, it.shDegree,
-                  cameraCentre(it.cam), it.gShRest, it.ndcScale);
+        ShCoefs sh;
+        sh.coef = it.shRest;
+        sh.grad = it.gShRest;
+        sh.gs = 1;
+        sh.cs = it.n;
+        sh.degree = it.shRest ? it.shDegree : 0;
+        chainBody<-1>(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
+                      sb.flags, sb.rec1, grad2d, out, it.ndc, it.gradCount, sh,
+                      cameraCentre(it.cam), it.ndcScale);
 #pragma unroll
         for (int k = 0; k < kGrad2dStride; ++k)
             grad2d[o * kGrad2dStride + k] = 0.0f;
@@ -489,51 +527,64 @@ inline unsigned blocksFor(int64_t n) { return unsigned((n + kThreads - 1) / kThr
 
 int projectBlocks(int64_t n) { return n > 0 ? int(blocksFor(n)) : 0; }
 
+// The kernel instance for the SH degree of sh (0 without coefficients).
+template <template <int> class K>
+inline auto byDegree(const ShCoefs &sh) -> decltype(K<0>::fn)
+{
+    switch (sh.coef ? sh.degree : 0) {
+    case 1: return K<1>::fn;
+    case 2: return K<2>::fn;
+    case 3: return K<3>::fn;
+    default: return K<0>::fn;
+    }
+}
+template <int D> struct ProjectK { static constexpr auto fn = projectKernel<D>; };
+template <int D> struct ChainRuleK { static constexpr auto fn = chainRuleKernel<D>; };
+template <int D> struct ChainAdamK { static constexpr auto fn = chainAdamKernel<D>; };
+
 int launchProject(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                   const u3d_raster_settings &st, int tilesX, int tilesY, const ScreenBuffers &out,
-                  const float *shRest, int shDegree, uint32_t *blockTiles)
+                  const ShCoefs &sh, uint32_t *blockTiles)
 {
     if (m.n <= 0)
         return U3D_OK;
-    const bool sh = shRest && shDegree > 0;
-    hipLaunchKernelGGL(sh ? projectKernel<true> : projectKernel<false>, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
+    U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    hipLaunchKernelGGL(byDegree<ProjectK>(sh), dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n, m.means,
                        m.scales, m.quats, m.opacities, m.colors, cam, st.near_plane, st.eps2d,
-                       st.tile_size, tilesX, tilesY, out, shRest, shDegree, cameraCentre(cam),
-                       blockTiles);
+                       st.tile_size, tilesX, tilesY, out, sh, cameraCentre(cam), blockTiles);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
 
 int launchChainRule(hipStream_t s, const u3d_gaussians &m, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
-                    u3d_gaussians &grads, float *ndc, int *gradCount, const float *shRest,
-                    int shDegree, float *gShRest, bool clearGrad2d)
+                    u3d_gaussians &grads, float *ndc, int *gradCount, const ShCoefs &sh,
+                    bool clearGrad2d)
 {
     if (m.n <= 0)
         return U3D_OK;
-    const bool sh = shRest && shDegree > 0;
-    hipLaunchKernelGGL(sh ? chainRuleKernel<true> : chainRuleKernel<false>, dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
+    U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    hipLaunchKernelGGL(byDegree<ChainRuleK>(sh), dim3(blocksFor(m.n)), dim3(kThreads), 0, s, m.n,
                        m.means, m.scales, m.quats, cam, st.near_plane, st.eps2d, sb.radius,
                        sb.flags, sb.rec1, grad2d, grads.means, grads.scales, grads.quats,
-                       grads.opacities, grads.colors, ndc, gradCount, shRest, shDegree,
-                       cameraCentre(cam), gShRest, clearGrad2d);
+                       grads.opacities, grads.colors, ndc, gradCount, sh, cameraCentre(cam),
+                       clearGrad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
 
 int launchChainAdam(hipStream_t s, const u3d_gaussians &params, const u3d_camera &cam,
                     const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d,
-                    float *ndc, int *gradCount, const float *shRest, int shDegree, float *gShRest,
-                    const AdamStepArgs &adam)
+                    float *ndc, int *gradCount, const ShCoefs &sh, const AdamStepArgs &adam)
 {
     if (params.n <= 0)
         return U3D_OK;
     U3D_CHECK(adam.m.n == params.n && adam.v.n == params.n, U3D_ERR_INVALID,
               "Adam moments differ in size from the parameters");
-    const bool sh = shRest && shDegree > 0;
-    hipLaunchKernelGGL(sh ? chainAdamKernel<true> : chainAdamKernel<false>, dim3(blocksFor(params.n)), dim3(kThreads), 0, s, params.n,
+    U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
+    hipLaunchKernelGGL(byDegree<ChainAdamK>(sh), dim3(blocksFor(params.n)), dim3(kThreads), 0, s, params.n,
                        params, cam, st.near_plane, st.eps2d, sb.radius, sb.flags, sb.rec1, grad2d,
-                       ndc, gradCount, shRest, shDegree, cameraCentre(cam), gShRest, adam);
+                       ndc, gradCount, sh, cameraCentre(cam), adam);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

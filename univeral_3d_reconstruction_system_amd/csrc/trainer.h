@@ -160,7 +160,9 @@ struct u3d_trainer {
     // the evaluated pairs (last counter) accumulate on the device, in evalPairs
     int64_t counters[U3D_NUM_COUNTERS] = {};
     u3d::DevBuf<unsigned long long> evalPairs; // 64 partial counts
-    // Spherical-harmonic coefficients beyond DC (extension), [cap][15][3].
+    // Spherical-harmonic coefficients beyond DC (extension): 45 planes of n floats (coefficient
+    // c of Gaussian i at c * n + i), their Adam moments, the batched engine's gradient and a
+    // spare for densification; capacity 45 * cap each.
     int shDegree = 0;
     u3d::DevBuf<float> shP, shG, shM, shV, shSpare;
     // Batched training (extension): views per step, whole-step graph, the engine that runs it
@@ -174,9 +176,10 @@ namespace u3d {
 
 // evalCount (nullable, 64 device counters): the forward adds its evaluated (pixel, Gaussian)
 // pairs, the sum of the contributor counts, when it composites 16-pixel tiles.
+// sh (nullable): view-dependent colour coefficients.
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
-                      const u3d_raster_settings &st, float *rgb, const float *shRest = nullptr,
-                      int shDegree = 0, unsigned long long *evalCount = nullptr);
+                      const u3d_raster_settings &st, float *rgb, const ShCoefs *sh = nullptr,
+                      unsigned long long *evalCount = nullptr);
 void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2);
 // Next view of the trainer's shuffled epoch order (Trainer.cpp:57-64).
 int trainerNextView(u3d_trainer *t);
@@ -184,6 +187,18 @@ int trainerNextView(u3d_trainer *t);
 float trainerMeansLr(const u3d_trainer *t);
 // ++iteration, then the densification / opacity-reset schedule (Trainer.cpp:94-101).
 int trainerFinishStep(u3d_trainer *t);
+// The trainer's SH coefficients as kernels read them: one plane of n floats per coefficient.
+inline ShCoefs trainerSh(const u3d_trainer *t)
+{
+    ShCoefs sh;
+    if (t->shDegree > 0) {
+        sh.coef = t->shP.p;
+        sh.gs = 1;
+        sh.cs = t->params.n;
+        sh.degree = t->shDegree;
+    }
+    return sh;
+}
 // SH coefficients train at the colour rate times this scale (3DGS uses 1/20).
 inline float shRateScale(const u3d_trainer *t)
 {
