@@ -7,6 +7,9 @@ set starts at the 300k cap, so every timed step works on 300k Gaussians. One ste
 Trainer::step of the reference API (project, sort/bin, forward, L1 + 0.2 D-SSIM, backward,
 Adam, and the TrainOptions::forIterations(7000) densify / reset schedule).
 
+The default warm-up is one epoch over the 200 views plus 50 steps, so the timed steps are
+steady-state training steps (a 7000-iteration run spends 97% of its steps after its first epoch).
+
 Prints one JSON line: iters/s (value, timed with no instrumentation on the stream), per-stage
 GPU times from HIP events recorded over a second timed region of the same length, the dominant
 kernel's roofline (evaluated pairs counted on the device over every timed step), and the
@@ -241,7 +244,9 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=50)
+    # one epoch over the 200 views plus 50 steps: the timed steps are steady-state training steps
+    # (each view composited before, so its tiles are ordered by their last walk lengths)
+    ap.add_argument("--warmup", type=int, default=250)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batched-views", type=int, default=8,

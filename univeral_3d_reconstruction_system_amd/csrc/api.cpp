@@ -128,7 +128,7 @@ namespace u3d {
 
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
                       const u3d_raster_settings &st, float *rgb, const ShCoefs *shp,
-                      unsigned long long *evalCount)
+                      unsigned long long *evalCount, uint32_t *walkCost)
 {
     const ShCoefs sh = shp ? *shp : ShCoefs{};
     U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
@@ -233,7 +233,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
                                   r->rectSorted.p, tilesX, tilesY, r->binCounts.p,
                                   r->binCounts.cap, r->tileTotal.p, r->tileOffsets.p,
                                   r->tileListBuf.p, kk, r->heavyFirst.p,
-                                  onesweepErrorPtr(r->sortScratch.p)));
+                                  onesweepErrorPtr(r->sortScratch.p), ts == 16 ? walkCost : nullptr));
         order = n > 0 ? r->heavyFirst.p : nullptr;
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {
@@ -279,7 +279,8 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     }
     StageScope scope(r->ctx, U3D_STAGE_RASTER_FWD);
     U3D_TRY(launchRasterForward(s, r->geo, st, r->tileOffsets.p, r->tileList, sb, rgb,
-                                r->finalT.p, r->nContrib.p, seg, order, evalCount));
+                                r->finalT.p, r->nContrib.p, seg, order, evalCount,
+                                order && ts == 16 ? walkCost : nullptr));
     r->haveForward = true;
     return U3D_OK;
 }
@@ -974,6 +975,16 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_TRY(t->gradAccum.ensure(size_t(cap)));
     U3D_TRY(t->gradCount.ensure(size_t(cap)));
     U3D_TRY(t->evalPairs.ensure(64));
+    {
+        const int ts = options->raster.tile_size;
+        size_t costs = 0;
+        for (const u3d_camera &c : t->cams) {
+            t->walkCostOffset.push_back(costs);
+            costs += size_t((c.width + ts - 1) / ts) * size_t((c.height + ts - 1) / ts);
+        }
+        U3D_TRY(t->walkCost.ensure(costs > 0 ? costs : 1));
+        U3D_HIP(hipMemsetAsync(t->walkCost.p, 0, t->walkCost.cap * sizeof(uint32_t), s));
+    }
     U3D_HIP(hipMemsetAsync(t->evalPairs.p, 0, 64 * sizeof(unsigned long long), s));
     U3D_HIP(hipMemsetAsync(t->gradAccum.p, 0, size_t(cap) * sizeof(float), s));
     U3D_HIP(hipMemsetAsync(t->gradCount.p, 0, size_t(cap) * sizeof(int), s));
@@ -1172,7 +1183,8 @@ int u3d_trainer_step(u3d_trainer *t)
     const hipStream_t s = t->ctx->stream;
     const u3d_gaussians pv = t->params.view();
     const ShCoefs shv = trainerSh(t);
-    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, &shv, t->evalPairs.p));
+    U3D_TRY(rasterForwardImpl(&t->raster, pv, cam, t->opt.raster, t->render.p, &shv, t->evalPairs.p,
+                              t->walkCost.p + t->walkCostOffset[view]));
     t->counters[0] += 1;
     t->counters[1] += t->params.n;
     t->counters[2] += int64_t(cam.width) * cam.height;

@@ -317,12 +317,13 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
 // heavyFirst (optional, tiles entries): the tiles by decreasing list length, written unless
 // n <= 0. Ids sorted[i] >= n count as invisible; list slots and offsets stay below listCapacity.
 // sortError (nullable): the small sort's error word (onesweepErrorPtr); when set, every
-// Gaussian counts as invisible.
+// Gaussian counts as invisible. walkCost (nullable, tiles entries): per-tile walk costs of the
+// last forward of this view (0: unknown), which order heavyFirst; cleared for the next forward.
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
                       uint32_t *tileList, size_t listCapacity, int *heavyFirst = nullptr,
-                      const uint32_t *sortError = nullptr);
+                      const uint32_t *sortError = nullptr, uint32_t *walkCost = nullptr);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot; slots at or past capacity are not written.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
@@ -375,7 +376,8 @@ constexpr int kSegEntries = 64;
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
-                        const int *blockOrder = nullptr, unsigned long long *evalCount = nullptr);
+                        const int *blockOrder = nullptr, unsigned long long *evalCount = nullptr,
+                        uint32_t *walkCost = nullptr);
 // Segmented backward: one work item per (tile, segment of kSegEntries list positions) from the
 // forward's per-tile segment counts. Scratch written here: segStart (tiles + 1 entries, the
 // exclusive scan of segCount), *totalItems, counter, itemTile (at least Σ segCount entries:

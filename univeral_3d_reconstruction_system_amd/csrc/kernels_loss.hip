@@ -298,30 +298,40 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
     }
 }
 
-// With prep.segCount set, an extra first column of workgroups (blockIdx.x == 0) runs the
-// segmented backward's work-item scan instead (channel 0's; the other two exit), dispatched
-// ahead of the loss blocks so that it never extends the launch.
+// Workgroup -> (32x32 block, channel). Workgroups are dealt to the 8 XCDs in turn, so the
+// three channels of block b get workgroups 24 (b / 8) + 8 c + b % 8: one XCD, dispatched
+// together, so their reads of the interleaved images and their partial-line writes of the
+// gradient meet in one L2 (HBM traffic of one pass instead of three).
+constexpr int kLossXcds = 8;
+inline int lossWorkgroups(int blocks)
+{
+    return 3 * ((blocks + kLossXcds - 1) / kLossXcds) * kLossXcds;
+}
+
+// With prep.segCount set, an extra first workgroup runs the segmented backward's work-item scan
+// instead, dispatched ahead of the loss blocks so that it never extends the launch.
 template <bool kU8>
 __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__ render,
                                                        const float *__restrict__ targetF,
                                                        const uint8_t *__restrict__ targetU8,
                                                        int W, int H, float lambda, float invCount,
-                                                       Window win, int blocksX,
+                                                       Window win, int blocksX, int blocks,
                                                        float *__restrict__ grad,
                                                        double *__restrict__ partials,
                                                        SegmentPrep prep)
 {
     const int skip = prep.segCount ? 1 : 0;
     if (int(blockIdx.x) < skip) {
-        if (blockIdx.y == 0)
-            segmentPrepBlock(prep);
+        segmentPrepBlock(prep);
         return;
     }
-    const int block = int(blockIdx.x) - skip;
-    const size_t perChannel = gridDim.x - unsigned(skip); // blocks of one channel
+    const int x = int(blockIdx.x) - skip;
+    const int grp = x / (3 * kLossXcds), r = x % (3 * kLossXcds);
+    const int c = r / kLossXcds, block = grp * kLossXcds + r % kLossXcds;
+    if (block >= blocks)
+        return;
     lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win, blocksX,
-                          block, int(blockIdx.y), grad,
-                          partials + (size_t(blockIdx.y) * perChannel + size_t(block)) * 3);
+                          block, c, grad, partials + (size_t(c) * size_t(blocks) + size_t(block)) * 3);
 }
 
 // Batched: item blockIdx.z, channel blockIdx.y; blocks past the item's count exit.
@@ -429,16 +439,16 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
     const float invCount = 1.0f / float(P * 3);
     static const Window win = makeWindow();
     const SegmentPrep side = prep ? *prep : SegmentPrep{};
-    const int cols = nb + (prep ? 1 : 0);
-    if (cols > 0) {
+    const int wgs = (nb > 0 ? lossWorkgroups(nb) : 0) + (prep ? 1 : 0);
+    if (wgs > 0) {
         if (targetU8)
-            hipLaunchKernelGGL(lossKernel<true>, dim3(cols, 3), dim3(kThreads), 0, s, render,
-                               target, targetU8, width, height, lambda, invCount, win, bx, grad,
-                               partials, side);
+            hipLaunchKernelGGL(lossKernel<true>, dim3(wgs), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               grad, partials, side);
         else
-            hipLaunchKernelGGL(lossKernel<false>, dim3(cols, 3), dim3(kThreads), 0, s, render,
-                               target, targetU8, width, height, lambda, invCount, win, bx, grad,
-                               partials, side);
+            hipLaunchKernelGGL(lossKernel<false>, dim3(wgs), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               grad, partials, side);
         U3D_LAUNCH_CHECK();
     }
     LossFinalize f;

@@ -489,7 +489,8 @@ __device__ __forceinline__ void forwardQuadrant(
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
-    int *__restrict__ nContrib, const SegmentState &seg, unsigned long long *evalCount)
+    int *__restrict__ nContrib, const SegmentState &seg, unsigned long long *evalCount,
+    uint32_t *walkCost)
 {
     constexpr int kBatch = kSegEntries; // checkpoint segment, list entries
     static_assert(kBatch % kQStep == 0, "segments are whole steps");
@@ -741,6 +742,16 @@ __device__ __forceinline__ void forwardQuadrant(
             m = max(m, __shfl_xor(m, off, 64));
         if (lane == 0)
             seg.segCount[size_t(tile) * kSegCountsPerTile + quad] = uint32_t((m + kBatch - 1) / kBatch);
+    }
+    if (walkCost) {
+        // how far this quadrant walked (its largest contributor count), for the ordering of
+        // the next forward of the same view
+        int m = inside ? last : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 0 && m > 0)
+            atomicMax(&walkCost[tile], unsigned(m));
     }
     if (evalCount) {
         // evaluated (pixel, Gaussian) pairs of this quadrant: the sum of its contributor counts,
@@ -1058,7 +1069,7 @@ __global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
     const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
     float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
     int *__restrict__ nContrib, SegmentState seg, const int *__restrict__ blockOrder,
-    unsigned long long *evalCount)
+    unsigned long long *evalCount, uint32_t *walkCost)
 {
     const int x = int(blockIdx.x), k = x >> 3;
     const int b = (k >> 2) * 8 + (x & 7);
@@ -1067,7 +1078,8 @@ __global__ __launch_bounds__(kThreads) void rasterForwardQuadKernel(
     const int blk = blockOrder ? blockOrder[b] : b;
     forwardQuadrant<kCkpt, kFastExp>(blk % tilesX, blk / tilesX, k & 3, W, H, tilesX,
                                      tileOffsets, tileList, rec0, rec1, rec2, bg0, bg1, bg2,
-                                     minAlpha, minT, rgb, finalT, nContrib, seg, evalCount);
+                                     minAlpha, minT, rgb, finalT, nContrib, seg, evalCount,
+                                     walkCost);
 }
 
 template <bool kCkpt, bool kFastExp>
@@ -1328,7 +1340,7 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
 int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                         const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,
                         float *rgb, float *finalT, int *nContrib, const SegmentState &seg,
-                        const int *blockOrder, unsigned long long *evalCount)
+                        const int *blockOrder, unsigned long long *evalCount, uint32_t *walkCost)
 {
     U3D_CHECK(!seg.ckpt || g.tileSize == 16, U3D_ERR_INVALID, "segment checkpoints need 16-pixel tiles");
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
@@ -1342,7 +1354,8 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
         hipLaunchKernelGGL(quad, dim3(32 * ((numTiles + 7) / 8)), dim3(kThreads), 0, s, g.width,
                            g.height, g.tilesX, numTiles, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2,
                            st.background[0], st.background[1], st.background[2], st.min_alpha,
-                           st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder, evalCount);
+                           st.min_transmittance, rgb, finalT, nContrib, seg, blockOrder, evalCount,
+                           walkCost);
         U3D_LAUNCH_CHECK();
         return U3D_OK;
     }

@@ -1057,11 +1057,15 @@ __global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *coun
         tileTotal[blockIdx.x] = total;
 }
 
-// The tiles ordered by decreasing list length (64 log-scale classes, any order inside a class),
-// so that a launch that maps workgroup i to tile heavyFirst[i] spreads the longest lists over
-// different compute units (the whole workgroup; LDS counters cls[64]).
+// The tiles ordered by decreasing cost (64 log-scale classes, any order inside a class), so
+// that a launch that maps workgroup i to tile heavyFirst[i] starts the longest walks first
+// (the whole workgroup; LDS counters cls[64]). The cost of a tile is walkCost[t] when given and
+// non-zero -- the largest contributor count of its pixels the last time this view was
+// composited, which bounds how far the walk goes -- else its list length. walkCost is then
+// cleared for the forward that follows to record the new maxima.
 __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ tileTotal, int tiles,
-                                                int *heavyFirst, uint32_t *cls)
+                                                int *heavyFirst, uint32_t *cls,
+                                                uint32_t *walkCost)
 {
     const int tid = threadIdx.x, nt = int(blockDim.x);
     auto classOf = [](uint32_t len) {
@@ -1074,8 +1078,12 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
     for (int k = tid; k < 64; k += nt)
         cls[k] = 0;
     __syncthreads();
+    auto costOf = [&](int t) {
+        const uint32_t w = walkCost ? walkCost[t] : 0u;
+        return w ? w : tileTotal[t];
+    };
     for (int t = tid; t < tiles; t += nt)
-        atomicAdd(&cls[classOf(tileTotal[t])], 1u);
+        atomicAdd(&cls[classOf(costOf(t))], 1u);
     __syncthreads();
     if (tid == 0) {
         uint32_t run = 0;
@@ -1087,7 +1095,12 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
     }
     __syncthreads();
     for (int t = tid; t < tiles; t += nt)
-        heavyFirst[atomicAdd(&cls[classOf(tileTotal[t])], 1u)] = t;
+        heavyFirst[atomicAdd(&cls[classOf(costOf(t))], 1u)] = t;
+    if (walkCost) {
+        __syncthreads();
+        for (int t = tid; t < tiles; t += nt)
+            walkCost[t] = 0u;
+    }
 }
 
 // Writes the ids of the Gaussians [first, first + count) (depth order; one wavefront) at their
@@ -1180,7 +1193,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
     const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
-    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity)
+    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1246,7 +1259,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         if (tid == 0)
             tileOffsets[tiles] = int(carry < capacity ? carry : capacity);
         if (heavyFirst)
-            writeHeavyFirst(tileTotal, tiles, heavyFirst, cls);
+            writeHeavyFirst(tileTotal, tiles, heavyFirst, cls, walkCost);
     }
     emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList, capacity);
 }
@@ -1463,7 +1476,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
                       uint32_t *tileList, size_t listCapacity, int *heavyFirst,
-                      const uint32_t *sortError)
+                      const uint32_t *sortError, uint32_t *walkCost)
 {
     const int group = tileBinningGroup(n, tilesX, tilesY);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
@@ -1488,7 +1501,8 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                        size_t(waves) * size_t(cells) * sizeof(uint32_t), s, n, sorted, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList,
-                       uint32_t(listCapacity < size_t(INT32_MAX) ? listCapacity : size_t(INT32_MAX)));
+                       uint32_t(listCapacity < size_t(INT32_MAX) ? listCapacity : size_t(INT32_MAX)),
+                       walkCost);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

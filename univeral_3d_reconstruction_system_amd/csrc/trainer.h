@@ -160,6 +160,10 @@ struct u3d_trainer {
     // the evaluated pairs (last counter) accumulate on the device, in evalPairs
     int64_t counters[U3D_NUM_COUNTERS] = {};
     u3d::DevBuf<unsigned long long> evalPairs; // 64 partial counts
+    // Per-view tile walk costs (largest contributor count per tile at the view's last forward)
+    // that order the next forward of the same view; view k at walkCostOffset[k].
+    u3d::DevBuf<uint32_t> walkCost;
+    std::vector<size_t> walkCostOffset;
     // Spherical-harmonic coefficients beyond DC (extension): 45 planes of n floats (coefficient
     // c of Gaussian i at c * n + i), their Adam moments, the batched engine's gradient and a
     // spare for densification; capacity 45 * cap each.
@@ -177,9 +181,11 @@ namespace u3d {
 // evalCount (nullable, 64 device counters): the forward adds its evaluated (pixel, Gaussian)
 // pairs, the sum of the contributor counts, when it composites 16-pixel tiles.
 // sh (nullable): view-dependent colour coefficients.
+// walkCost (nullable, tiles entries): this view's per-tile walk costs from its last forward,
+// which order the tiles heavy-first, replaced by this forward's (16-pixel tiles).
 int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &cam,
                       const u3d_raster_settings &st, float *rgb, const ShCoefs *sh = nullptr,
-                      unsigned long long *evalCount = nullptr);
+                      unsigned long long *evalCount = nullptr, uint32_t *walkCost = nullptr);
 void adamCoefficients(int step, const u3d_learning_rates &lr, float stepSize[5], float *invSqrtBc2);
 // Next view of the trainer's shuffled epoch order (Trainer.cpp:57-64).
 int trainerNextView(u3d_trainer *t);
