@@ -318,14 +318,44 @@ int u3d_write_splat_ply(const char *path, const float *means_host, const float *
                         const float *quats_host, const float *opacities_host,
                         const float *colors_host, int64_t n, const float *sh_rest_host,
                         int sh_degree);
-/* readSplatPly for 3DGS files (binary little endian or ASCII; SplatPly.cpp:284-342): with
- * capacity 0 only *n_out and *sh_degree_out are set; otherwise fills host arrays of at least
- * *n_out entries (sh_rest [n][15][3] when the file has f_rest_*, missing ones zero). */
+/* Vertex count of a PLY file, whether it is a Gaussian splat file (it has f_dc_0, opacity,
+ * scale_0 and rot_0, as readSplatPly decides, SplatPly.cpp:295) and the SH degree of its
+ * f_rest_* properties (0 without). */
+int u3d_ply_info(const char *path, int64_t *vertices, int *is_splat, int *sh_degree);
+/* readSplatPly for splat files (ASCII or binary of either byte order, other elements skipped,
+ * SplatPly.cpp:133-235, 284-342): vertices whose x, y or z is not finite are dropped and
+ * *n_out is the count kept; with capacity 0 only *n_out and *sh_degree_out are set, otherwise
+ * host arrays of at least *n_out entries are filled (sh_rest [n][15][3] when the file has
+ * f_rest_*, missing ones zero). A point-cloud file returns U3D_ERR_INVALID: the reference turns
+ * one into Gaussians with initGaussiansFromPoints (u3d_read_points_ply, u3d_points_knn_init). */
 int u3d_read_splat_ply(const char *path, int64_t capacity, float *means_host, float *scales_host,
                        float *quats_host, float *opacities_host, float *colors_host,
                        float *sh_rest_host, int64_t *n_out, int *sh_degree_out);
+/* writePointsPly (SplatPly.cpp:344-362): x y z float, red green blue uchar (rgb in 0..1,
+ * clamped and rounded; 0.8 for points beyond rgb_count / 3). */
+int u3d_write_points_ply(const char *path, const float *xyz_host, const float *rgb_host,
+                         int64_t rgb_count, int64_t n);
+/* readPointsPly (SplatPly.cpp:364-386): rgb in 0..1 (uchar / 255, ushort / 65535; 0.8 when the
+ * file has no colours); points with a non-finite coordinate dropped. capacity 0: only *n_out. */
+int u3d_read_points_ply(const char *path, int64_t capacity, float *xyz_host, float *rgb_host,
+                        int64_t *n_out);
 /* writeSplatPly(trainer.model()) (Reconstructor.cpp:361): one copy from the device. */
 int u3d_trainer_export_ply(u3d_trainer *t, const char *path);
+
+/* ------------------------------------------------------------------------------- viewer */
+/* Display attributes of a loaded model (app/src/SplatView.cpp:117-180), device arrays: rgba
+ * [4n] uint8 = round(clamp(0.5 + C0 * colour, 0, 1) * 255) and round(sigmoid(opacity) * 255),
+ * cov6 [6n] = the upper triangle xx xy xz yy yz zz of M M^T, M = R(q / (|q| + 1e-12))
+ * diag(exp(scale)). Synchronous. */
+int u3d_view_splat_attributes(u3d_ctx *ctx, const u3d_gaussians *model, uint8_t *rgba, float *cov6);
+/* Front-to-back order for a view (SplatView.cpp:349-397): depth z_i = -(v . (c_i, 1)) with v
+ * the third row of the row-major 4x4 view matrix (view_row2 host [4]), 16-bit keys
+ * (z_i - min z) * 65535 / (max z - min z) truncated, and order [n] (device) the stable sort of
+ * the keys (the reference's counting sort). packed (device, nullable) receives the 40-byte
+ * records {float center[3]; uint8 rgba[4]; float cov[6]} in that order from centers [3n],
+ * rgba [4n] and cov6 [6n]. Synchronous. */
+int u3d_view_depth_order(u3d_ctx *ctx, const float *centers, int64_t n, const float *view_row2,
+                         uint32_t *order, const uint8_t *rgba, const float *cov6, void *packed);
 
 /* --------------------------------------------------------------------------- scene sets */
 /* Independent scenes trained together (no reference counterpart: the reference trains one

@@ -187,6 +187,8 @@ def lib():
                                                                    C.POINTER(Settings)] + [C.c_void_p] * 5
         L.orc_chain_rule_f64.argtypes = [C.c_void_p, C.POINTER(Model), C.c_void_p, C.c_int, F32P] + \
             [C.c_void_p] * 6
+        L.orc_view_attributes.argtypes = [C.POINTER(Model), C.c_void_p, C.c_void_p]
+        L.orc_view_depth_order.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
         L.orc_trainer_set_sh.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_void_p]
         L.orc_trainer_get_sh.argtypes = [C.c_void_p, F32P]
     return _orc
@@ -222,6 +224,12 @@ def ref_lib():
         L.ref_write_splat_ply.argtypes = [C.c_char_p, C.POINTER(Model)]
         L.ref_read_splat_ply.restype = C.c_long
         L.ref_read_splat_ply.argtypes = [C.c_char_p, C.c_void_p]
+        L.ref_write_points_ply.restype = C.c_int
+        L.ref_write_points_ply.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long]
+        L.ref_read_points_ply.restype = C.c_long
+        L.ref_read_points_ply.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p]
+        L.ref_read_splat_ply_any.restype = C.c_long
+        L.ref_read_splat_ply_any.argtypes = [C.c_char_p, C.c_void_p, C.POINTER(C.c_int)]
         L.ref_trainer_new.restype = C.c_void_p
         L.ref_trainer_new.argtypes = [C.POINTER(Model), C.POINTER(Camera), C.c_void_p, C.c_int,
                                       C.c_float, C.POINTER(TrainOptions), C.c_int]
@@ -380,6 +388,23 @@ def project_f64(g: dict, cam: dict, st: Settings, sh_rest=None, sh_degree=0) -> 
     lib().orc_project_f64(n, *(_ptr(a) for a in arr), _ptr(rest), sh_degree, C.byref(c), C.byref(st),
                           *(_ptr(out[k]) for k in ("mean2d", "conic", "rgb", "alpha", "radius")))
     return out
+
+
+def view_attributes(g: Gaussians):
+    """SplatView.cpp:117-180: (rgba [4n] uint8, cov6 [6n] float32)."""
+    rgba = np.zeros(4 * g.n, np.uint8)
+    cov = np.zeros(6 * g.n, np.float32)
+    lib().orc_view_attributes(C.byref(g.struct()), _ptr(rgba), _ptr(cov))
+    return rgba, cov
+
+
+def view_depth_order(centers, row2) -> np.ndarray:
+    """SplatView.cpp:349-397: front-to-back order of the splats for a view."""
+    c = np.ascontiguousarray(centers, np.float32)
+    r = np.ascontiguousarray(row2, np.float32)
+    order = np.zeros(len(c) // 3, np.uint32)
+    lib().orc_view_depth_order(_ptr(c), len(c) // 3, _ptr(r), _ptr(order))
+    return order
 
 
 def sh_basis(x, y, z, degree=3) -> np.ndarray:

@@ -214,6 +214,43 @@ long ref_read_splat_ply(const char *path, orc_model *out)
     return long(g.size());
 }
 
+// SplatPly.h: the reference's point-cloud writer and reader. ref_read_points_ply returns the
+// point count (-1 on error) and fills xyz / rgb (3 per point) when they are non-null.
+int ref_write_points_ply(const char *path, const float *xyz, const float *rgb, long rgb_count, long n)
+{
+    std::vector<float> x(xyz, xyz + size_t(n) * 3), c;
+    if (rgb)
+        c.assign(rgb, rgb + rgb_count);
+    return writePointsPly(path, x, c) ? 1 : 0;
+}
+
+long ref_read_points_ply(const char *path, float *xyz, float *rgb)
+{
+    std::vector<float> x, c;
+    if (!readPointsPly(path, x, c))
+        return -1;
+    if (xyz)
+        std::copy(x.begin(), x.end(), xyz);
+    if (rgb)
+        std::copy(c.begin(), c.end(), rgb);
+    return long(x.size() / 3);
+}
+
+// readSplatPly of any file: the model (splat or point-cloud fallback); returns the size or -1,
+// *is_splat the kind.
+long ref_read_splat_ply_any(const char *path, orc_model *out, int *is_splat)
+{
+    GaussianModel g;
+    bool splat = false;
+    if (!readSplatPly(path, g, nullptr, &splat))
+        return -1;
+    if (is_splat)
+        *is_splat = splat ? 1 : 0;
+    if (out)
+        fromModel(g, out);
+    return long(g.size());
+}
+
 void ref_init_from_points(int n, const float *xyz, const float *rgb, int rgb_count, orc_model *out)
 {
     std::vector<float> x(xyz, xyz + size_t(n) * 3), c;

@@ -1545,4 +1545,58 @@ void orc_trainer_render(orc_trainer *t, const orc_camera *cam, float *rgb)
     orc_raster_forward(&t->raster, &mv, cam, &t->opt.raster, rgb);
 }
 
+// ---- viewer (app/src/SplatView.cpp): display attributes at load (:117-180) and the
+// front-to-back order of a view (:349-397), with the reference's float expressions.
+void orc_view_attributes(const orc_model *m, uint8_t *rgba, float *cov6)
+{
+    for (size_t i = 0; i < size_t(m->n); ++i) {
+        for (int k = 0; k < 3; ++k) {
+            const float c = std::clamp(0.5f + kShC0 * m->colors[i * 3 + k], 0.0f, 1.0f);
+            rgba[i * 4 + k] = static_cast<uint8_t>(std::lround(c * 255.0f));
+        }
+        rgba[i * 4 + 3] = static_cast<uint8_t>(std::lround(sigmoidf_(m->opacities[i]) * 255.0f));
+        const float *q = &m->quats[i * 4];
+        const float qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + 1e-12f;
+        float R[9];
+        rotOfQuat(q[0] / qn, q[1] / qn, q[2] / qn, q[3] / qn, R);
+        float M[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                M[r * 3 + c] = R[r * 3 + c] * std::exp(m->scales[i * 3 + c]);
+        auto sigma = [&](int r, int c) {
+            return M[r * 3] * M[c * 3] + M[r * 3 + 1] * M[c * 3 + 1] + M[r * 3 + 2] * M[c * 3 + 2];
+        };
+        float *o = &cov6[i * 6];
+        o[0] = sigma(0, 0);
+        o[1] = sigma(0, 1);
+        o[2] = sigma(0, 2);
+        o[3] = sigma(1, 1);
+        o[4] = sigma(1, 2);
+        o[5] = sigma(2, 2);
+    }
+}
+
+void orc_view_depth_order(const float *c, long n, const float *row2, uint32_t *order)
+{
+    std::vector<float> depth(static_cast<size_t>(n));
+    float lo = std::numeric_limits<float>::max(), hi = -lo;
+    for (size_t i = 0; i < size_t(n); ++i) {
+        const float z = -(row2[0] * c[i * 3] + row2[1] * c[i * 3 + 1] + row2[2] * c[i * 3 + 2] + row2[3]);
+        depth[i] = z;
+        lo = std::min(lo, z);
+        hi = std::max(hi, z);
+    }
+    const float scale = hi > lo ? 65535.0f / (hi - lo) : 0.0f;
+    std::vector<uint32_t> counts(65537, 0);
+    std::vector<uint16_t> keys(static_cast<size_t>(n));
+    for (size_t i = 0; i < size_t(n); ++i) {
+        keys[i] = uint16_t((depth[i] - lo) * scale);
+        ++counts[size_t(keys[i]) + 1];
+    }
+    for (size_t k = 1; k < counts.size(); ++k)
+        counts[k] += counts[k - 1];
+    for (size_t i = 0; i < size_t(n); ++i)
+        order[counts[keys[i]]++] = uint32_t(i);
+}
+
 } // extern "C"

@@ -147,6 +147,10 @@ long orc_densify(const orc_model *m, const float *grad_accum, const int *grad_co
 void orc_init_from_points(int n, const float *xyz, const float *rgb, int rgb_count,
                           orc_model *out);
 
+// ---- viewer (app/src/SplatView.cpp:117-180, 349-397): rgba [4n] and cov6 [6n] of a model; the
+// front-to-back order [n] of centers [3n] for the view matrix's third row row2 [4].
+void orc_view_attributes(const orc_model *m, uint8_t *rgba, float *cov6);
+void orc_view_depth_order(const float *centers, long n, const float *row2, uint32_t *order);
 // ---- trainer (Trainer.cpp:37-197). Images: cams[n_images], rgb8 concatenated HWC uint8.
 // orc_trainer_set_sh (extension, call before the first step): SH degree 1..3 coefficients
 // starting at rest (NULL: zero), trained with rate lr.colors * lr_scale (0 -> 1/20); orc_trainer_get_sh

@@ -499,3 +499,64 @@ def test_fp64_chain_rule_matches_finite_differences_of_the_fp64_projection():
         mask = np.repeat(vis, d)
         err = np.linalg.norm(got[f][mask] - fd[mask]) / np.linalg.norm(fd[mask])
         assert err < 1e-6, (f, err)
+
+
+def _random_quats(rng, n):
+    q = rng.normal(size=(n, 4)).astype(np.float32)
+    q[: n // 8] *= 1e-3  # far from unit length
+    return q
+
+
+def test_view_attributes_closed_forms():
+    """loadSplats (SplatView.cpp:117-180): colour bytes from float32 arithmetic rounded half away
+    from zero, opacity bytes from the sigmoid, and cov6 = the upper triangle of
+    R diag(exp(2 s)) R^T in FP64 (a transposed M, M^T M, fails it)."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    g = O.Gaussians(rng.normal(size=3 * n).astype(np.float32),
+                    rng.uniform(-4, 0.5, size=3 * n).astype(np.float32),
+                    _random_quats(rng, n).reshape(-1),
+                    rng.normal(0, 4, size=n).astype(np.float32),
+                    rng.normal(0, 2.5, size=3 * n).astype(np.float32))
+    rgba, cov = O.view_attributes(g)
+    rgba = rgba.reshape(n, 4)
+    c = np.clip(np.float32(0.5) + np.float32(0.28209479177387814) * g.colors.reshape(n, 3),
+                np.float32(0), np.float32(1)).astype(np.float32) * np.float32(255)
+    np.testing.assert_array_equal(rgba[:, :3], np.floor(c.astype(np.float64) + 0.5).astype(np.uint8))
+    a = 255.0 / (1.0 + np.exp(-g.opacities.astype(np.float64)))
+    assert np.abs(rgba[:, 3].astype(np.float64) - np.floor(a + 0.5)).max() <= 1
+    clear = np.abs(a - np.floor(a) - 0.5) > 1e-3
+    np.testing.assert_array_equal(rgba[clear, 3], np.floor(a[clear] + 0.5).astype(np.uint8))
+    q = g.quats.reshape(n, 4).astype(np.float64)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(n, 3, 3)
+    s2 = np.exp(2 * g.scales.reshape(n, 3).astype(np.float64))
+    sig = np.einsum("nij,nj,nkj->nik", R, s2, R)
+    want = sig[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]
+    scale = s2.max(1, keepdims=True)
+    assert (np.abs(cov.reshape(n, 6) - want) <= 1e-5 * scale).all()
+
+
+@pytest.mark.parametrize("case", ["random", "ties", "flat", "one"])
+def test_view_depth_order_is_the_stable_sort_of_the_16_bit_keys(case):
+    """sortAndUpload (SplatView.cpp:349-397): float32 depths z = -(v . (c, 1)), keys
+    uint16((z - lo) * (65535 / (hi - lo))), order = the stable argsort of the keys."""
+    rng = np.random.default_rng(12)
+    n = {"random": 50000, "ties": 20000, "flat": 1000, "one": 1}[case]
+    c = rng.normal(0, 3, size=(n, 3)).astype(np.float32)
+    if case == "ties":
+        c = np.round(c * 2) / 2  # many equal depths: stability decides their order
+    row = np.array([0.3, -0.5, 0.81, 2.0], np.float32)
+    if case == "flat":
+        c[:, :] = 1.0
+    order = O.view_depth_order(c.reshape(-1), row)
+    f = np.float32
+    z = -(((row[0] * c[:, 0] + row[1] * c[:, 1]) + row[2] * c[:, 2]) + row[3])
+    lo, hi = z.min(), z.max()
+    scale = f(65535) / (hi - lo) if hi > lo else f(0)
+    keys = ((z - lo) * scale).astype(np.uint16)
+    np.testing.assert_array_equal(order, np.argsort(keys, kind="stable").astype(np.uint32))
+    assert (np.diff(keys[order].astype(np.int64)) >= 0).all()

@@ -151,7 +151,12 @@ SIGNATURES = {
     "u3d_points_learned_init": (I, [P, P, P, P, I, I64, P, I, F, C.POINTER(Gaussians)]),
     "u3d_write_splat_ply": (I, [C.c_char_p, P, P, P, P, P, I64, P, I]),
     "u3d_read_splat_ply": (I, [C.c_char_p, I64, P, P, P, P, P, P, C.POINTER(I64), C.POINTER(I)]),
+    "u3d_ply_info": (I, [C.c_char_p, C.POINTER(I64), C.POINTER(I), C.POINTER(I)]),
+    "u3d_write_points_ply": (I, [C.c_char_p, P, P, I64, I64]),
+    "u3d_read_points_ply": (I, [C.c_char_p, I64, P, P, C.POINTER(I64)]),
     "u3d_trainer_export_ply": (I, [P, C.c_char_p]),
+    "u3d_view_splat_attributes": (I, [P, C.POINTER(Gaussians), P, P]),
+    "u3d_view_depth_order": (I, [P, P, I64, P, P, P, P, P]),
     "u3d_scene_set_create": (I, [P, I, C.c_size_t, C.POINTER(P)]),
     "u3d_scene_set_destroy": (None, [P]),
     "u3d_scene_set_add": (I, [P, P]),

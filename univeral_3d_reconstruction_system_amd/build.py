@@ -23,8 +23,8 @@ LIB = os.path.join(OUT_DIR, "libu3d_hip.so")
 ARCH = os.environ.get("U3D_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels_project.hip", "kernels_sort.hip", "kernels_raster.hip", "kernels_loss.hip",
-           "kernels_adam.hip", "kernels_points.hip", "kernels_mlp.hip", "api.cpp", "batch.cpp",
-           "initnet.cpp", "ply.cpp"]
+           "kernels_adam.hip", "kernels_points.hip", "kernels_mlp.hip", "kernels_view.hip", "api.cpp",
+           "batch.cpp", "initnet.cpp", "ply.cpp", "view.cpp"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
@@ -72,7 +72,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
 HOST_DIR = os.path.join(HERE, "host")
 SPLAT_LIB = os.path.join(OUT_DIR, "libu3d_splat.so")
 HOST_SOURCES = ["Camera.cpp", "ThreadPool.cpp", "GaussianModel.cpp", "device.cpp", "Rasterizer.cpp",
-                "Loss.cpp", "Adam.cpp", "Trainer.cpp"]
+                "Loss.cpp", "Adam.cpp", "Trainer.cpp", "SplatPly.cpp"]
 
 
 def _compile_host(src: str) -> str:

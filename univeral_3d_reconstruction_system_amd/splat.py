@@ -551,6 +551,72 @@ def read_splat_ply(path: str):
     return model, (sh[:n.value * 45] if deg.value > 0 else None), deg.value
 
 
+def ply_info(path: str) -> dict:
+    """Vertex count, whether the file is a Gaussian splat file, and its SH degree."""
+    lib = _abi.load()
+    n, splat, deg = C.c_int64(), C.c_int(), C.c_int()
+    check(lib.u3d_ply_info(path.encode(), C.byref(n), C.byref(splat), C.byref(deg)))
+    return dict(vertices=n.value, is_splat=bool(splat.value), sh_degree=deg.value)
+
+
+def write_points_ply(path: str, xyz, rgb=None):
+    """writePointsPly (SplatPly.cpp:344-362): rgb in 0..1 stored as uchar."""
+    lib = _abi.load()
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    rgb = np.zeros(0, np.float32) if rgb is None else np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    check(lib.u3d_write_points_ply(path.encode(), xyz.ctypes.data, rgb.ctypes.data if len(rgb) else None,
+                                   len(rgb), len(xyz) // 3))
+
+
+def read_points_ply(path: str):
+    """readPointsPly (SplatPly.cpp:364-386): (xyz [3n], rgb [3n] in 0..1)."""
+    lib = _abi.load()
+    n = C.c_int64()
+    check(lib.u3d_read_points_ply(path.encode(), 0, None, None, C.byref(n)))
+    xyz = np.zeros(max(3 * n.value, 1), np.float32)
+    rgb = np.zeros(max(3 * n.value, 1), np.float32)
+    check(lib.u3d_read_points_ply(path.encode(), max(n.value, 1), xyz.ctypes.data, rgb.ctypes.data,
+                                  C.byref(n)))
+    return xyz[:3 * n.value], rgb[:3 * n.value]
+
+
+def view_attributes(ctx: Context, model) -> tuple:
+    """loadSplats' display attributes (SplatView.cpp:117-180) on the device: (rgba [4n] uint8,
+    cov6 [6n] float32) as host copies."""
+    d = model if isinstance(model, DeviceGaussians) else DeviceGaussians.from_host(ctx, model)
+    rgba = DeviceArray(ctx, (max(d.n, 1),), np.uint32)
+    cov = DeviceArray(ctx, (max(6 * d.n, 1),))
+    m = d.struct()
+    check(ctx.lib.u3d_view_splat_attributes(ctx.h, C.byref(m), rgba.ptr, cov.ptr))
+    return (_view_numpy(ctx, rgba.ptr, 4 * d.n, np.uint8),
+            _view_numpy(ctx, cov.ptr, 6 * d.n, np.float32))
+
+
+def view_depth_order(ctx: Context, centers, view_row2, rgba=None, cov6=None):
+    """sortAndUpload (SplatView.cpp:349-397) on the device: the front-to-back order [n] and,
+    given rgba and cov6, the packed 40-byte records as a structured array."""
+    c = np.ascontiguousarray(centers, np.float32).reshape(-1)
+    n = len(c) // 3
+    row = np.ascontiguousarray(view_row2, np.float32).reshape(4)
+    dc = DeviceArray.from_numpy(ctx, c if n else np.zeros(3, np.float32))
+    order = DeviceArray(ctx, (max(n, 1),), np.uint32)
+    pack = rgba is not None
+    if pack:
+        drgba = DeviceArray.from_numpy(ctx, np.ascontiguousarray(rgba, np.uint8).reshape(-1).view(np.uint32)
+                                       if n else np.zeros(1, np.uint32))
+        dcov = DeviceArray.from_numpy(ctx, np.ascontiguousarray(cov6, np.float32).reshape(-1)
+                                      if n else np.zeros(1, np.float32))
+        packed = DeviceArray(ctx, (max(10 * n, 1),), np.float32)
+    check(ctx.lib.u3d_view_depth_order(ctx.h, dc.ptr, n, row.ctypes.data, order.ptr,
+                                       drgba.ptr if pack else None, dcov.ptr if pack else None,
+                                       packed.ptr if pack else None))
+    o = _view_numpy(ctx, order.ptr, n, np.uint32)
+    if not pack:
+        return o
+    rec = np.dtype([("center", np.float32, 3), ("color", np.uint8, 4), ("cov", np.float32, 6)])
+    return o, _view_numpy(ctx, packed.ptr, 40 * n, np.uint8).view(rec)
+
+
 class InitNet:
     """The learned initialiser's network on the device (u3d_init_net)."""
 
