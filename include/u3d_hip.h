@@ -46,7 +46,10 @@ typedef struct {
     float t[3];
 } u3d_camera;
 
-/* RasterSettings (Rasterizer.h:23-32). tile_size must be a multiple of 16 here. */
+/* RasterSettings (Rasterizer.h:23-32). Any tile_size > 0; multiples of 16 take the LDS-staged
+ * compositing kernels (16: the quadrant forward and the segmented backward), other sizes a
+ * per-pixel walk. The batched engine (views_per_step > 1, use_graph, scene sets) needs a
+ * multiple of 16 and returns U3D_ERR_UNSUPPORTED otherwise. */
 typedef struct {
     int tile_size;
     float near_plane;

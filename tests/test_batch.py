@@ -167,6 +167,24 @@ def test_scene_set_members_train_as_alone(ctx):
         assert s.member(k).iteration() == 5
 
 
+def test_batched_engine_refuses_tile_sizes_off_multiples_of_16(ctx):
+    """The plain trainer takes any tile size; the batched engine's kernels stage 16x16 blocks of
+    one tile, so it refuses other sizes with U3D_ERR_UNSUPPORTED instead of failing later."""
+    init, cams, images = _scene(500, 2, 64, 48)
+    opt = _frozen_options(2, False)
+    opt.raster.tile_size = 20
+    with pytest.raises(U.U3DError) as e:
+        U.Trainer(ctx, init, cams, images, 3.0, opt)
+    assert e.value.code == 4  # U3D_ERR_UNSUPPORTED
+    plain = _frozen_options(1, False)
+    plain.raster.tile_size = 20
+    t = U.Trainer(ctx, init, cams, images, 3.0, plain)
+    s = U.SceneSet(ctx, use_graph=False)
+    with pytest.raises(U.U3DError):
+        s.add(t)
+    t.step()
+
+
 def test_scene_set_admission_by_budget(ctx):
     init, cams, images = _scene(2000, 2, 96, 72)
     opt = U.options_for_iterations(10)

@@ -203,10 +203,14 @@ def load() -> C.CDLL:
 
 
 class U3DError(RuntimeError):
-    pass
+    """A non-zero status of the C ABI; ``code`` is the U3D_ERR_* value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"u3d error {code}: {message}")
+        self.code = code
 
 
 def check(rc: int) -> None:
     if rc != 0:
         msg = load().u3d_last_error().decode(errors="replace")
-        raise U3DError(f"u3d error {rc}: {msg}")
+        raise U3DError(rc, msg)

@@ -132,8 +132,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
 {
     const ShCoefs sh = shp ? *shp : ShCoefs{};
     U3D_CHECK(sh.degree >= 0 && sh.degree <= 3, U3D_ERR_INVALID, "sh_degree must be 0..3");
-    U3D_CHECK(st.tile_size > 0 && st.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
-              "tile_size must be a positive multiple of 16");
+    U3D_CHECK(st.tile_size > 0, U3D_ERR_INVALID, "tile_size must be positive");
     U3D_CHECK(cam.width >= 0 && cam.height >= 0, U3D_ERR_INVALID, "negative image size");
     const hipStream_t s = r->ctx->stream;
     const int64_t n = m.n;
@@ -938,8 +937,10 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
     U3D_CHECK(out && options && n >= 0 && n_images >= 0, U3D_ERR_INVALID, "bad argument");
     U3D_CHECK(n == 0 || (means && scales && quats && opacities && colors), U3D_ERR_INVALID, "null model");
     U3D_CHECK(n_images == 0 || (cameras && rgb8), U3D_ERR_INVALID, "null images");
-    U3D_CHECK(options->raster.tile_size > 0 && options->raster.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
-              "tile_size must be a positive multiple of 16");
+    U3D_CHECK(options->raster.tile_size > 0, U3D_ERR_INVALID, "tile_size must be positive");
+    U3D_CHECK(options->raster.tile_size % 16 == 0 || (options->views_per_step <= 1 && !options->use_graph),
+              U3D_ERR_UNSUPPORTED, "the batched engine (views_per_step > 1 or use_graph) needs a tile_size "
+              "that is a multiple of 16");
     std::unique_ptr<u3d_trainer> t(new u3d_trainer());
     t->ctx = ctx;
     t->raster.ctx = ctx;

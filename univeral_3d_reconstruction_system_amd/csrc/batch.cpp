@@ -824,6 +824,8 @@ int u3d_scene_set_add(u3d_scene_set *set, u3d_trainer *scene)
     U3D_CHECK(scene->ctx == set->ctx, U3D_ERR_INVALID, "scene belongs to another context");
     for (u3d_trainer *t : set->members)
         U3D_CHECK(t != scene, U3D_ERR_INVALID, "scene already in the set");
+    U3D_CHECK(scene->opt.raster.tile_size % 16 == 0, U3D_ERR_UNSUPPORTED,
+              "scene sets need a tile_size that is a multiple of 16");
     const size_t own = trainerBytes(scene);
     size_t pix = 0;
     for (const u3d_camera &c : scene->cams)

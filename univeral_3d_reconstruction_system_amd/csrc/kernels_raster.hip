@@ -1096,6 +1096,142 @@ __global__ __launch_bounds__(kThreads) void rasterForwardKernel(
                                        bg2, minAlpha, minT, rgb, finalT, nContrib, seg);
 }
 
+// Tiles of any size (RasterSettings::tileSize is free in the reference, Rasterizer.cpp:297-300):
+// one thread per pixel walks its own tile's list with the records read from memory -- the
+// lanes of a wavefront may lie in different tiles, so nothing is staged in LDS. The forward
+// takes forwardBlock's expressions in the same order (same image, final transmittance and
+// contributor counts); the backward adds each contribution with float atomics.
+__device__ __forceinline__ bool anyTilePixel(int W, int H, int blocksX, int &px, int &py)
+{
+    const int bx = int(blockIdx.x) % blocksX, by = int(blockIdx.x) / blocksX;
+    px = bx * kEdge + int(threadIdx.x & 15);
+    py = by * kEdge + int(threadIdx.x >> 4);
+    return px < W && py < H;
+}
+
+__global__ __launch_bounds__(kThreads) void rasterForwardAnyKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, float minT, float *__restrict__ rgb, float *__restrict__ finalT,
+    int *__restrict__ nContrib)
+{
+    __shared__ uint64_t stab[32];
+    if (threadIdx.x < 32)
+        stab[threadIdx.x] = kExp2Table[threadIdx.x];
+    __syncthreads();
+    int px, py;
+    if (!anyTilePixel(W, H, blocksX, px, py))
+        return;
+    const int tile = (py / ts) * tilesX + px / ts;
+    const int begin = tileOffsets[tile], end = tileOffsets[tile + 1];
+    const float fpx = float(px), fpy = float(py);
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int last = 0;
+    for (int j = begin; j < end; ++j) {
+        const int g = tileList[j];
+        const float4 a = rec0[g], b = rec1[g];
+        const float dx = a.x - fpx, dy = a.y - fpy;
+        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+        if (power > 0.0f)
+            continue;
+        const float raw = b.y * expfExactLdsIf(power, stab, false);
+        const float alpha = (raw < kMaxAlpha) ? raw : kMaxAlpha;
+        if (alpha < minAlpha)
+            continue;
+        const float nextT = T * (1.0f - alpha);
+        if (nextT < minT)
+            break;
+        const float w = alpha * T;
+        cr = cr + b.z * w;
+        cg = cg + b.w * w;
+        cb = cb + rec2[g] * w;
+        T = nextT;
+        last = j - begin + 1;
+    }
+    const size_t pix = size_t(py) * size_t(W) + size_t(px);
+    rgb[pix * 3] = cr + T * bg0;
+    rgb[pix * 3 + 1] = cg + T * bg1;
+    rgb[pix * 3 + 2] = cb + T * bg2;
+    finalT[pix] = T;
+    nContrib[pix] = last;
+}
+
+__global__ __launch_bounds__(kThreads) void rasterBackwardAnyKernel(
+    int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
+    const int *__restrict__ tileList, const float4 *__restrict__ rec0,
+    const float4 *__restrict__ rec1, const float *__restrict__ rec2, float bg0, float bg1,
+    float bg2, float minAlpha, const float *__restrict__ dl, const float *__restrict__ finalTArr,
+    const int *__restrict__ nContribArr, float *__restrict__ grad2d)
+{
+    __shared__ uint64_t stab[32];
+    if (threadIdx.x < 32)
+        stab[threadIdx.x] = kExp2Table[threadIdx.x];
+    __syncthreads();
+    int px, py;
+    if (!anyTilePixel(W, H, blocksX, px, py))
+        return;
+    const int tile = (py / ts) * tilesX + px / ts;
+    const int begin = tileOffsets[tile];
+    const size_t pix = size_t(py) * size_t(W) + size_t(px);
+    const float fT = finalTArr[pix];
+    const float dp0 = dl[pix * 3], dp1 = dl[pix * 3 + 1], dp2 = dl[pix * 3 + 2];
+    const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
+    const float fpx = float(px), fpy = float(py);
+    // the walk state of the reference's reverse pass (Rasterizer.cpp:379-417)
+    float T = fT, acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+    float last0 = 0.0f, last1 = 0.0f, last2 = 0.0f, lastAlpha = 0.0f;
+    for (int j = nContribArr[pix] - 1; j >= 0; --j) {
+        const int g = tileList[begin + j];
+        const float4 a = rec0[g], b = rec1[g];
+        const float c2 = rec2[g];
+        const float dx = a.x - fpx, dy = a.y - fpy;
+        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+        if (power > 0.0f)
+            continue;
+        const float G = expfExactLdsIf(power, stab, false);
+        const float rawAlpha = b.y * G;
+        const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
+        if (alpha < minAlpha)
+            continue;
+        T = T / (1.0f - alpha);
+        const float w = alpha * T;
+        acc0 = lastAlpha * last0 + (1.0f - lastAlpha) * acc0;
+        acc1 = lastAlpha * last1 + (1.0f - lastAlpha) * acc1;
+        acc2 = lastAlpha * last2 + (1.0f - lastAlpha) * acc2;
+        last0 = b.z;
+        last1 = b.w;
+        last2 = c2;
+        float dAlpha = 0.0f;
+        dAlpha += (b.z - acc0) * dp0;
+        dAlpha += (b.w - acc1) * dp1;
+        dAlpha += (c2 - acc2) * dp2;
+        dAlpha *= T;
+        lastAlpha = alpha;
+        dAlpha += -fT / (1.0f - alpha) * bgDot;
+        float *dst = grad2d + size_t(g) * kGrad2dStride;
+        const float v5 = w * dp0, v6 = w * dp1, v7 = w * dp2;
+        if (v5 != 0.0f)
+            atomicAdd(dst + 5, v5);
+        if (v6 != 0.0f)
+            atomicAdd(dst + 6, v6);
+        if (v7 != 0.0f)
+            atomicAdd(dst + 7, v7);
+        if (rawAlpha > kMaxAlpha)
+            continue; // clamped: no gradient through alpha
+        const float dPower = b.y * dAlpha * G;
+        const float v[6] = {dPower * -(a.z * dx + a.w * dy), dPower * -(b.x * dy + a.w * dx),
+                            dPower * -0.5f * dx * dx, dPower * -dx * dy, dPower * -0.5f * dy * dy,
+                            G * dAlpha};
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (v[k] != 0.0f)
+                atomicAdd(dst + k, v[k]);
+        if (v[5] != 0.0f)
+            atomicAdd(dst + 8, v[5]);
+    }
+}
+
 // First index in sorted[0..n) greater than w, minus one: the tile owning work item w.
 __device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart, int tiles, uint32_t w)
 {
@@ -1346,6 +1482,15 @@ int launchRasterForward(hipStream_t s, const RasterGeometry &g, const u3d_raster
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
     if (blocksX * blocksY == 0)
         return U3D_OK;
+    if (g.tileSize % kEdge != 0) {
+        hipLaunchKernelGGL(rasterForwardAnyKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
+                           g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
+                           sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                           st.background[2], st.min_alpha, st.min_transmittance, rgb, finalT,
+                           nContrib);
+        U3D_LAUNCH_CHECK();
+        return U3D_OK;
+    }
     const bool fast = st.min_alpha >= 1e-37f; // as noSpecialExp
     if (g.tileSize == kEdge) {
         auto quad = seg.ckpt ? (fast ? rasterForwardQuadKernel<true, true> : rasterForwardQuadKernel<true, false>)
@@ -1378,6 +1523,14 @@ int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raste
     const int blocksX = (g.width + kEdge - 1) / kEdge, blocksY = (g.height + kEdge - 1) / kEdge;
     if (blocksX * blocksY == 0)
         return U3D_OK;
+    if (g.tileSize % kEdge != 0) {
+        hipLaunchKernelGGL(rasterBackwardAnyKernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
+                           g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,
+                           sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
+                           st.background[2], st.min_alpha, dlDrgb, finalT, nContrib, grad2d);
+        U3D_LAUNCH_CHECK();
+        return U3D_OK;
+    }
     auto kernel = st.min_alpha >= 1e-37f ? rasterBackwardKernel<true> : rasterBackwardKernel<false>;
     hipLaunchKernelGGL(kernel, dim3(blocksX * blocksY), dim3(kThreads), 0, s,
                        g.width, g.height, g.tileSize, g.tilesX, blocksX, tileOffsets, tileList,

@@ -23,7 +23,7 @@ struct RenderCamera {
 };
 
 struct RasterSettings {
-    int tileSize = 16;             // multiple of 16 on this implementation
+    int tileSize = 16;             // any size; multiples of 16 take the fastest kernels
     float nearPlane = 0.01f;
     float eps2d = 0.3f;
     float background[3] = {0.0f, 0.0f, 0.0f};

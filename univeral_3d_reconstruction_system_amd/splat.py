@@ -14,7 +14,7 @@ import weakref
 import numpy as np
 
 from . import _abi
-from ._abi import check
+from ._abi import U3DError, check
 
 FIELDS = (("means", 3), ("scales", 3), ("quats", 4), ("opacities", 1), ("colors", 3))
 
