@@ -14,7 +14,9 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle_splat.so")
+# U3D_ORACLE_SO names another build of the same source (the sanitizer build of
+# tests/sanitize/Makefile); the default is the plain build.
+ORACLE_SO = os.environ.get("U3D_ORACLE_SO") or os.path.join(HERE, "liboracle_splat.so")
 REF_SO = os.path.join(HERE, "_ref", "libu3d_ref.so")
 
 F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -219,3 +219,45 @@ def test_bad_files_fail_cleanly(tmp_path):
     p.write_bytes(b"ply\nformat binary_middle_endian 1.0\nelement vertex 0\nend_header\n")
     with pytest.raises(Exception, match="unknown PLY format"):
         U.ply_info(str(p))
+
+
+def test_counts_beyond_the_data_are_refused_before_allocating(tmp_path):
+    """A vertex or element count the file cannot hold is refused at once (a reader that sized
+    its buffers by it would ask for terabytes); an element of no properties takes no bytes."""
+    p = tmp_path / "huge.ply"
+    head = (b"ply\nformat binary_little_endian 1.0\nelement vertex 99999999999\nproperty float x\n"
+            b"property float y\nproperty float z\nend_header\n")
+    p.write_bytes(head + b"\0" * 24)
+    with pytest.raises(U.U3DError, match="more rows than the file holds"):
+        U.read_points_ply(str(p))
+    p.write_bytes(b"ply\nformat ascii 1.0\nelement empty 99999999999999\nelement vertex 1\n"
+                  b"property float x\nproperty float y\nproperty float z\nend_header\n1 2 3\n")
+    xyz, _ = U.read_points_ply(str(p))
+    np.testing.assert_array_equal(xyz, np.array([1, 2, 3], np.float32))
+    p.write_bytes(b"ply\nformat ascii 1.0\nelement vertex 1\nproperty float x\nproperty float y\n"
+                  b"property float z\nproperty list uint float l\nend_header\n1 2 3 1e300 0\n")
+    with pytest.raises(U.U3DError, match="truncated"):
+        U.read_points_ply(str(p))
+
+
+def test_doubles_beyond_the_float_range_read_as_infinity(tmp_path):
+    """A double property outside the float range becomes +-inf (IEEE rounding; the reference's
+    float(v) does the same on this host) -- here in a colour, so the point is kept."""
+    p = tmp_path / "wide.ply"
+    head = (b"ply\nformat binary_little_endian 1.0\nelement vertex 2\nproperty float x\n"
+            b"property float y\nproperty float z\nproperty double red\nproperty double green\n"
+            b"property double blue\nend_header\n")
+    rows = b""
+    fmax = 3.4028234663852886e38  # the largest float; its ulp is 2^104
+    for c in ((1e300, -1e300, fmax + 2.0 ** 102), (fmax + 2.0 ** 103, 0.5, -1e39)):
+        rows += np.array([0, 1, 2], np.float32).tobytes() + np.array(c, np.float64).tobytes()
+    p.write_bytes(head + rows)
+    xyz, rgb = U.read_points_ply(str(p))
+    assert len(xyz) == 6
+    f = np.float32
+    np.testing.assert_array_equal(rgb, np.array([np.inf, -np.inf, np.finfo(f).max, np.inf, 0.5, -np.inf], f))
+    L = O.ref_lib()
+    if L is not None:
+        x, c = np.zeros(6, np.float32), np.zeros(6, np.float32)
+        assert L.ref_read_points_ply(str(p).encode(), x.ctypes.data, c.ctypes.data) == 2
+        np.testing.assert_array_equal(c, rgb)

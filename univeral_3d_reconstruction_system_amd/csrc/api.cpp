@@ -1343,6 +1343,21 @@ int u3d_trainer_get_densify_stats(u3d_trainer *t, float *accum_host, int *count_
     return U3D_OK;
 }
 
+/* writeSplatPly(trainer.model()) (Reconstructor.cpp:361): one copy from the device, then the
+ * host writer of ply.cpp. */
+int u3d_trainer_export_ply(u3d_trainer *t, const char *path)
+{
+    U3D_CHECK(t != nullptr && path != nullptr, U3D_ERR_INVALID, "null argument");
+    const int64_t n = t->params.n;
+    std::vector<float> h(size_t(n) * 14), sh(t->shDegree > 0 ? size_t(n) * 45 : 0);
+    float *b = h.data();
+    U3D_TRY(u3d_trainer_get_model(t, b, b + 3 * n, b + 6 * n, b + 10 * n, b + 11 * n));
+    if (t->shDegree > 0)
+        U3D_TRY(u3d_trainer_get_sh_rest(t, sh.data()));
+    return u3d_write_splat_ply(path, b, b + 3 * n, b + 6 * n, b + 10 * n, b + 11 * n, n,
+                               t->shDegree > 0 ? sh.data() : nullptr, t->shDegree);
+}
+
 int u3d_trainer_stats(u3d_trainer *t, u3d_train_stats *out)
 {
     // TrainStats EMA (Trainer.cpp:88-92), folded lazily from the per-step device values.

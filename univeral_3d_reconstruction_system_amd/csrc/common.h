@@ -10,12 +10,12 @@
 #include <vector>
 
 #include "../../include/u3d_hip.h"
+#include "status.h"
 #include "u3d_expf.h"
 
 namespace u3d {
 
 // ------------------------------------------------------------------------------ errors
-void setError(const std::string &msg);
 int failHip(hipError_t e, const char *what, const char *file, int line);
 
 #define U3D_HIP(call)                                                       \
@@ -23,21 +23,6 @@ int failHip(hipError_t e, const char *what, const char *file, int line);
         hipError_t e_ = (call);                                             \
         if (e_ != hipSuccess)                                               \
             return ::u3d::failHip(e_, #call, __FILE__, __LINE__);           \
-    } while (0)
-
-#define U3D_CHECK(cond, code, msg)                                          \
-    do {                                                                    \
-        if (!(cond)) {                                                      \
-            ::u3d::setError(msg);                                           \
-            return code;                                                    \
-        }                                                                   \
-    } while (0)
-
-#define U3D_TRY(call)                                                       \
-    do {                                                                    \
-        int rc_ = (call);                                                   \
-        if (rc_ != U3D_OK)                                                  \
-            return rc_;                                                     \
     } while (0)
 
 #define U3D_LAUNCH_CHECK() U3D_HIP(hipGetLastError())

@@ -21,7 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "trainer.h"
+#include "status.h"
 
 using namespace u3d;
 
@@ -186,12 +186,26 @@ public:
         if (!f.list)
             return m_enc == Encoding::Ascii ? token(v) : advance(f.value.bytes);
         double n;
-        if (!next(f.count, n) || n < 0)
+        if (!next(f.count, n) || !(n >= 0.0 && n <= 4294967295.0)) // a 32-bit count at most
             return false;
         for (int64_t k = 0; k < int64_t(n); ++k)
             if (!(m_enc == Encoding::Ascii ? token(v) : advance(f.value.bytes)))
                 return false;
         return true;
+    }
+
+private:
+public:
+    size_t remaining() const { return size_t(m_end - m_p); }
+
+    // Fewest bytes one row of e can take: every binary value its size, every ASCII value one
+    // character (0 when e has no properties).
+    size_t minRowBytes(const Element &e) const
+    {
+        size_t b = 0;
+        for (const Field &f : e.fields)
+            b += m_enc == Encoding::Ascii ? 1 : size_t(f.list ? f.count.bytes : f.value.bytes);
+        return b;
     }
 
 private:
@@ -229,6 +243,20 @@ private:
     Encoding m_enc;
 };
 
+// double -> float as IEEE round-to-nearest gives it, also beyond the float range (where a C++
+// conversion is undefined): up to half an ulp (2^103) above the largest float rounds to it, from
+// there on (the tie rounds to even) to infinity.
+float toFloat(double v)
+{
+    constexpr float kMaxF = std::numeric_limits<float>::max();
+    constexpr double kMax = double(kMaxF), kInf = kMax + 0x1p103;
+    if (v >= kInf || v <= -kInf)
+        return v > 0 ? std::numeric_limits<float>::infinity() : -std::numeric_limits<float>::infinity();
+    if (v > kMax || v < -kMax)
+        return v > 0 ? kMaxF : -kMaxF;
+    return float(v);
+}
+
 // The vertex element's columns `names`: values [rows x names] (float, NaN where the file has no
 // such property), present[k] whether the file has names[k]. Rows with a non-finite x, y or z
 // (names 0..2 must be x y z) are dropped.
@@ -264,7 +292,13 @@ int readVertexColumns(const char *path, const std::vector<std::string> &names, V
     Body body(file, h.body, h.enc);
     const size_t W = names.size();
     for (const Element &e : h.elements) {
+        // a count the rest of the file cannot hold is refused before anything is sized by it
+        const size_t minRow = body.minRowBytes(e);
+        U3D_CHECK(minRow == 0 || uint64_t(e.count) <= body.remaining() / minRow, U3D_ERR_INVALID,
+                  "PLY data truncated: element " + e.name + " claims more rows than the file holds");
         if (e.name != "vertex") {
+            if (minRow == 0) // rows of no properties take no bytes
+                continue;
             for (int64_t i = 0; i < e.count; ++i)
                 for (const Field &f : e.fields)
                     U3D_CHECK(body.skip(f), U3D_ERR_INVALID, "PLY data truncated in element " + e.name);
@@ -288,6 +322,7 @@ int readVertexColumns(const char *path, const std::vector<std::string> &names, V
                         scale[f] = 65535.0;
                 }
         }
+        U3D_CHECK(minRow > 0, U3D_ERR_INVALID, "PLY vertices have no properties");
         const float nan = std::numeric_limits<float>::quiet_NaN();
         out.values.assign(size_t(e.count) * W, nan);
         std::vector<float> row(W);
@@ -305,7 +340,7 @@ int readVertexColumns(const char *path, const std::vector<std::string> &names, V
                           "PLY data truncated at vertex " + std::to_string(i));
                 if (scale[f] != 1.0)
                     v /= scale[f];
-                row[size_t(target[f])] = float(v);
+                row[size_t(target[f])] = toFloat(v);
             }
             if (!std::isfinite(row[0]) || !std::isfinite(row[1]) || !std::isfinite(row[2]))
                 continue;
@@ -516,19 +551,6 @@ int u3d_read_points_ply(const char *path, int64_t capacity, float *xyz_host, flo
         }
     }
     return U3D_OK;
-}
-
-int u3d_trainer_export_ply(u3d_trainer *t, const char *path)
-{
-    U3D_CHECK(t != nullptr && path != nullptr, U3D_ERR_INVALID, "null argument");
-    const int64_t n = t->params.n;
-    std::vector<float> h(size_t(n) * 14), sh(t->shDegree > 0 ? size_t(n) * 45 : 0);
-    float *b = h.data();
-    U3D_TRY(u3d_trainer_get_model(t, b, b + 3 * n, b + 6 * n, b + 10 * n, b + 11 * n));
-    if (t->shDegree > 0)
-        U3D_TRY(u3d_trainer_get_sh_rest(t, sh.data()));
-    return u3d_write_splat_ply(path, b, b + 3 * n, b + 6 * n, b + 10 * n, b + 11 * n, n,
-                               t->shDegree > 0 ? sh.data() : nullptr, t->shDegree);
 }
 
 } // extern "C"
