@@ -161,12 +161,15 @@ def time_to_target(ctx, plain_iterations, seed=100_000):
 
 def batched_rate(ctx, lib, init, cams, images, extent, views, steps, warmup):
     """The batched engine on the same workload: `views` views per optimiser step, one persistent
-    compositing launch per direction, the whole step replayed as a hipGraph."""
+    compositing launch per direction, the whole step replayed as a hipGraph, launch choices
+    picked by the engine's autotuner (its candidate steps are training steps, before the
+    warm-up)."""
     opt = U.options_for_iterations(ITERATIONS)
     opt.max_gaussians = N_GAUSSIANS
     opt.views_per_step = views
     opt.use_graph = 1
     t = U.Trainer(ctx, init, cams, images, extent, opt)
+    t.autotune(steps_per_candidate=2)
     for _ in range(warmup):
         t.step()
     t.synchronize()
@@ -220,6 +223,7 @@ def scene_set_rate(ctx, scenes: int, steps: int, warmup: int, seed: int = 3):
         if not sset.add(t):
             break
         info.append(dict(gaussians=n, width=w, height=h, views=views))
+    tuning = sset.autotune(steps_per_candidate=1)
     for _ in range(warmup):
         sset.step()
     ctx.synchronize()
@@ -235,7 +239,7 @@ def scene_set_rate(ctx, scenes: int, steps: int, warmup: int, seed: int = 3):
         per.append(dict(d, iters_per_s=round(rate, 2),
                         gaussian_pixels_per_s=round(m.size() * d["width"] * d["height"] * rate, 1),
                         overflow_steps=m.counters()["overflow_steps"]))
-    return dict(scenes=len(info), steps=steps, ms_per_step=round(1e3 * dt / steps, 4),
+    return dict(scenes=len(info), steps=steps, ms_per_step=round(1e3 * dt / steps, 4), tuning=tuning,
                 aggregate_gaussian_pixels_per_s=round(sum(p["gaussian_pixels_per_s"] for p in per), 1),
                 resident_bytes=sset.bytes(), per_scene=per)
 

@@ -218,16 +218,17 @@ def test_batched_training_with_densification_converges(ctx):
     assert c["overflow_steps"] == 0 and c["views"] == 4 * 600
 
 
-@pytest.mark.parametrize("lds,heavy,per_cu,items,few", [(64, 1, 0, 0, 4), (128, 0, 2, 4, 16),
-                                                       (256, 1, 1, 8, 0)])
-def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items, few):
-    """LDS batch, block order, workgroups per CU and radix items per lane: same forward (first
-    loss bit-identical), same gradient (1e-4)."""
+@pytest.mark.parametrize("lds,heavy,per_cu,items,few,radix,pairs", [
+    (64, 1, 0, 0, 4, 4, 1), (128, 0, 2, 4, 16, 6, 0), (256, 1, 1, 8, 0, 8, 1), (256, 1, 0, 0, 0, 5, 0)])
+def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items, few, radix, pairs):
+    """LDS batch, block order, workgroups per CU, radix items per lane, radix digit width and
+    how the tile lists are built (difference grids per image or a sort of emitted pairs): same
+    forward (first loss bit-identical), same gradient (1e-4)."""
     init, cams, images = _scene(3000, 4, 160, 120)
     t = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
     t.set_tuning(heavy_first=heavy, workgroups_per_cu=per_cu, lds_batch=lds, sort_items=items,
-                 atomic_lanes=few)
-    assert t.tuning()["lds_batch"] == lds
+                 atomic_lanes=few, sort_radix=radix, pair_sort=pairs)
+    assert t.tuning()["lds_batch"] == lds and t.tuning()["pair_sort"] == pairs
     t.step()
     m, _ = t.moments()
     got = {f: m[f].astype(np.float64) / np.float64(ONE_MINUS_B1) for f in m}
@@ -248,7 +249,7 @@ def test_autotuner_picks_a_configuration_and_training_continues(ctx):
     assert chosen["lds_batch"] in (64, 128, 256) and chosen["tile_size"] == 16
     assert t.tuning() == chosen
     it = t.iteration()
-    assert it == 24 * 3  # 24 candidates x (1 warm-up + 2 timed) steps
+    assert it == (24 + 6) * 3  # 24 + 6 candidates x (1 warm-up + 2 timed) steps
     for _ in range(5):
         t.step()
     assert t.iteration() == it + 5 and t.counters()["overflow_steps"] == 0

@@ -57,7 +57,8 @@ class TrainOptions(C.Structure):
 class BatchTuning(C.Structure):
     """u3d_batch_tuning: launch choices of the batched engine."""
     _fields_ = [("heavy_first", C.c_int), ("workgroups_per_cu", C.c_int), ("lds_batch", C.c_int),
-                ("sort_items", C.c_int), ("tile_size", C.c_int), ("atomic_lanes", C.c_int)]
+                ("sort_items", C.c_int), ("tile_size", C.c_int), ("atomic_lanes", C.c_int),
+                ("sort_radix", C.c_int), ("pair_sort", C.c_int)]
 
 
 class TrainStats(C.Structure):

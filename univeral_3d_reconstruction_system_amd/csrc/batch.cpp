@@ -2,19 +2,22 @@
 // per step, Trainer.cpp:53-103).
 //
 // Every (scene, view) render of a step is an item. All items share one projection launch, one
-// depth sort over their concatenated screen entries, one tile sort over their concatenated
-// tile ids, one persistent compositing launch per direction (workgroups pull 16x16 blocks of
-// any item from an atomic counter, heaviest tile lists first), one loss launch, one chain-rule
-// launch and one Adam launch. What changes from step to step (cameras, target and output
+// depth sort over their concatenated screen entries (then a stable pass on the item, so each
+// item's entries are contiguous and in depth order), the tile lists of each item built from
+// difference grids straight into its own slice of one list (no (tile, Gaussian) pairs are
+// materialised or sorted; images with more tiles than the binning's LDS grid holds fall back to
+// emitting pairs and sorting them by global tile id), one persistent compositing launch per
+// direction (workgroups pull 16x16 blocks of any item from an atomic counter, heaviest tile
+// lists first), one loss launch, one chain-rule launch and one Adam launch. What changes from step to step (cameras, target and output
 // pointers, optimiser coefficients, the stats slot) travels in one small table uploaded before
 // the step, so the launches themselves only depend on the layout (Gaussian counts, image
 // sizes, pair capacity): they are captured once as a hipGraph and replayed until the layout
 // changes (densification, a grown pair capacity).
 //
-// The pair count K is never read back inside a step: sorts and tile offsets take it from
-// device memory, with grids sized for a capacity. Pairs past the capacity are dropped (the
-// farthest in depth order) and flagged; the host reads each step's K and flag a few steps later
-// and grows the capacity before it is reached again.
+// The pair count K is never read back inside a step: the lists have a fixed capacity per item
+// (or in all, on the fallback path). Pairs past the capacity are dropped (the farthest in depth
+// order) and flagged; the host reads each step's counts and flag a few steps later and grows
+// the capacity before it is reached again.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,7 +29,8 @@
 namespace u3d {
 
 namespace {
-constexpr int kRing = 4; // table uploads / status read-backs in flight
+constexpr int kRing = 4;     // table uploads / status read-backs in flight
+constexpr int kBranches = 4; // items binned side by side (their launches are small)
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 int ceilDiv(int a, int b) { return (a + b - 1) / b; }
 } // namespace
@@ -48,6 +52,14 @@ public:
             (void)hipHostFree(pinned);
         if (statusHost)
             (void)hipHostFree(statusHost);
+        for (int b = 0; b < kBranches; ++b) {
+            if (branchDone[b])
+                (void)hipEventDestroy(branchDone[b]);
+            if (branch[b])
+                (void)hipStreamDestroy(branch[b]);
+        }
+        if (forkEvent)
+            (void)hipEventDestroy(forkEvent);
     }
 
     int init()
@@ -58,12 +70,18 @@ public:
             slotUsed[k] = false;
             statusStep[k] = -1;
         }
-        U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 2 * kRing));
+        U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 4 * kRing));
+        // default-priority streams for the per-item binning branches (forked from and joined to
+        // the context's stream by events, so a captured step records them as graph branches)
+        for (int b = 0; b < kBranches; ++b) {
+            U3D_HIP(hipStreamCreateWithFlags(&branch[b], hipStreamNonBlocking));
+            U3D_HIP(hipEventCreateWithFlags(&branchDone[b], hipEventDisableTiming));
+        }
+        U3D_HIP(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming));
         for (int k = 0; k < 3; ++k) {
             occFwd[k] = rasterBatchedOccupancy(false, 64 << k);
             occBwd[k] = rasterBatchedOccupancy(true, 64 << k);
         }
-        occSeg = rasterSegmentedOccupancy();
         return U3D_OK;
     }
 
@@ -78,6 +96,9 @@ public:
                   "sort_items must be 0, 4 or 8");
         U3D_CHECK(t.atomic_lanes == 0 || t.atomic_lanes == 4 || t.atomic_lanes == 16,
                   U3D_ERR_INVALID, "atomic_lanes must be 0, 4 or 16");
+        U3D_CHECK(t.sort_radix == 0 || (t.sort_radix >= 4 && t.sort_radix <= 8), U3D_ERR_INVALID,
+                  "sort_radix must be 0 or 4..8");
+        U3D_CHECK(t.pair_sort == 0 || t.pair_sort == 1, U3D_ERR_INVALID, "pair_sort must be 0 or 1");
         U3D_HIP(hipStreamSynchronize(ctx->stream));
         tuning.heavy_first = t.heavy_first != 0;
         tuning.workgroups_per_cu = t.workgroups_per_cu;
@@ -85,10 +106,14 @@ public:
         tuning.sort_items = t.sort_items;
         tuning.tile_size = t.tile_size;
         tuning.atomic_lanes = t.atomic_lanes;
+        tuning.sort_radix = t.sort_radix;
+        if ((t.pair_sort != 0) != (tuning.pair_sort != 0))
+            needRelayout = true; // other buffers and list layout
+        tuning.pair_sort = t.pair_sort;
         dropGraph();
         return U3D_OK;
     }
-    u3d_batch_tuning tuning{1, 0, 256, 0, 0, 0};
+    u3d_batch_tuning tuning{1, 0, 256, 0, 0, 0, 0, 0};
 
     int step(u3d_trainer *const *scenes, int numScenes);
     int autotune(u3d_trainer *const *scenes, int numScenes, int stepsPerCandidate, bool tuneTile,
@@ -141,6 +166,14 @@ private:
 
     // layout
     std::vector<Member> key;
+    struct ItemGeometry {
+        int64_t n, screenBase;
+        int tileBase, tilesX, tilesY;
+    };
+    std::vector<ItemGeometry> geo; // per item, fixed for the layout
+    bool binned = false;           // per-item difference-grid binning (else pairs + tile sort)
+    int64_t itemCap = 0;           // list slots per item (binned)
+    int itemBits = 1;
     u3d_raster_settings st{};
     int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
     int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
@@ -155,7 +188,7 @@ private:
     size_t pinnedSlot = 0;
     hipEvent_t slotDone[kRing] = {};
     bool slotUsed[kRing] = {};
-    uint32_t *statusHost = nullptr; // pinned: {K, overflow} per slot
+    uint32_t *statusHost = nullptr; // pinned: {K, overflow, longest item K, -} per slot
     hipEvent_t statusDone[kRing] = {};
     int64_t statusStep[kRing] = {};
     std::vector<u3d_trainer *> statusScenes[kRing];
@@ -172,15 +205,13 @@ private:
     DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets, total;
     DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt, hist, histScan, scanScratch;
     DevBuf<uint32_t> blockKey, blockOrder, blockKeyAlt, blockOrderAlt;
+    // binned path: group counts and tile totals of one item per branch
+    DevBuf<uint32_t> binCounts[kBranches], tileTotal[kBranches];
+    hipStream_t branch[kBranches] = {};
+    hipEvent_t branchDone[kBranches] = {}, forkEvent = nullptr;
+    DevBuf<int4> rectSorted;
     DevBuf<unsigned> counters; // persistent work counters (forward, backward)
     DevBuf<double> partials;
-    // segmented backward (16-pixel tiles): forward checkpoints, per-tile segment counts
-    bool segmented = false;
-    int occSeg = 1;
-    DevBuf<float4> ckpt;
-    DevBuf<uint32_t> segCount, segStart, segTotal;
-    DevBuf<unsigned> segCounter;
-    int ensureSegments();
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -202,6 +233,8 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     totalBlocks = 0;
     totalPix = 0;
     partialsLen = 0;
+    geo.clear();
+    binned = true;
     const int ts = st.tile_size;
     for (u3d_trainer *t : act) {
         U3D_CHECK(std::memcmp(&t->opt.raster, &st, sizeof st) == 0, U3D_ERR_UNSUPPORTED,
@@ -218,6 +251,10 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         key.push_back({t, n, W, H, B, t->shDegree});
         int lbx = 0, lnb = 0;
         lossItemGeometry(W, H, &lbx, &lnb);
+        for (int k = 0; k < B; ++k)
+            geo.push_back({n, totalEntries + n * k, totalTiles + ceilDiv(W, ts) * ceilDiv(H, ts) * k,
+                           ceilDiv(W, ts), ceilDiv(H, ts)});
+        binned = binned && tuning.pair_sort == 0 && tileBinningGroup(n, ceilDiv(W, ts), ceilDiv(H, ts)) > 0;
         numItems += B;
         totalEntries += n * B;
         maxN = std::max(maxN, n);
@@ -230,6 +267,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     U3D_CHECK(totalEntries < (int64_t(1) << 32) && totalTiles < (1 << 30), U3D_ERR_CAPACITY,
               "batch too large for 32-bit pair values");
     tileBits = bitsFor(totalTiles);
+    itemBits = bitsFor(numItems);
     offItems = 0;
     offLoss = align256(size_t(numItems) * sizeof(RenderItem));
     offSteps = offLoss + align256(size_t(numItems) * sizeof(LossItem));
@@ -266,7 +304,20 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     U3D_TRY(grad2d.ensure(ne * kGrad2dStride));
     U3D_TRY(total.ensure(4));
     U3D_TRY(counters.ensure(4));
-    U3D_TRY(tileOffsets.ensure(size_t(totalTiles) + 1));
+    // binned: every item's offsets with their own end (totalTiles + numItems entries)
+    U3D_TRY(tileOffsets.ensure(size_t(totalTiles) + size_t(numItems)));
+    if (binned) {
+        size_t counts = 1, tiles = 1;
+        for (const ItemGeometry &g : geo) {
+            counts = std::max(counts, tileBinningCountsSize(g.n, g.tilesX, g.tilesY));
+            tiles = std::max(tiles, size_t(g.tilesX) * size_t(g.tilesY));
+        }
+        for (int b = 0; b < kBranches; ++b) {
+            U3D_TRY(binCounts[b].ensure(counts));
+            U3D_TRY(tileTotal[b].ensure(tiles));
+        }
+        U3D_TRY(rectSorted.ensure(ne));
+    }
     const size_t nb = size_t(std::max<int64_t>(totalBlocks, 1));
     U3D_TRY(blockKey.ensure(nb));
     U3D_TRY(blockOrder.ensure(nb));
@@ -289,6 +340,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     }
     needRelayout = false;
     pairCap = 0; // sized by the probe of the first step of this layout (or grownCap)
+    itemCap = 0;
     return U3D_OK;
 }
 
@@ -325,6 +377,7 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             it.tilesX = tilesX;
             it.tilesY = tilesY;
             it.blocksX = blocksX;
+            it.offsetShift = binned ? v : 0;
             it.shDegree = t->shDegree;
             it.ndcScale = float(B);
             it.means = pv.means;
@@ -411,33 +464,6 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
 
 // Sizes the pair capacity from this step's real pair count (projection, depth sort and scan
 // run once more inside the step itself).
-// Checkpoint storage of the segmented backward for the current pair capacity and LDS batch;
-// a reallocation or a switch of mode invalidates the captured graph. The segmented walk is
-// switched off in the batched engine (kSegmentedBatched): its first GPU run did not finish and
-// the cause is not established; the batched engine keeps the whole-list walk, whose many
-// concurrent views already keep the GPU busy.
-constexpr bool kSegmentedBatched = false;
-
-int BatchEngine::ensureSegments()
-{
-    const bool want = kSegmentedBatched && ctx->segmentedBackward && st.tile_size == 16;
-    if (want != segmented)
-        dropGraph();
-    segmented = want;
-    if (!segmented)
-        return U3D_OK;
-    const void *before[3] = {ckpt.p, segCount.p, segStart.p};
-    const size_t slots = size_t(pairCap) / size_t(tuning.lds_batch) + size_t(totalTiles) + 1;
-    U3D_TRY(ckpt.ensure(slots * 256));
-    U3D_TRY(segCount.ensure(size_t(totalTiles) + 1));
-    U3D_TRY(segStart.ensure(size_t(totalTiles) + 1));
-    U3D_TRY(segTotal.ensure(4));
-    U3D_TRY(segCounter.ensure(4));
-    if (before[0] != ckpt.p || before[1] != segCount.p || before[2] != segStart.p)
-        dropGraph();
-    return U3D_OK;
-}
-
 int BatchEngine::probePairs()
 {
     const hipStream_t s = ctx->stream;
@@ -449,19 +475,42 @@ int BatchEngine::probePairs()
                              scanScratch.cap));
     uint32_t K = 0;
     U3D_HIP(hipMemcpyAsync(&K, total.p, sizeof K, hipMemcpyDeviceToHost, s));
+    // the pairs of each item: the scan at the item's first entry and at the next item's
+    std::vector<uint32_t> starts(size_t(numItems), 0u);
+    if (binned)
+        for (int v = 0; v < numItems; ++v)
+            if (geo[size_t(v)].screenBase < totalEntries)
+                U3D_HIP(hipMemcpyAsync(&starts[size_t(v)], pairOffsets.p + geo[size_t(v)].screenBase,
+                                       sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     U3D_HIP(hipStreamSynchronize(s));
     // Views differ in coverage: leave room for the spread between steps.
     const double margin = numItems >= 4 ? 1.3 : 1.6;
-    int64_t cap = int64_t(double(K) * margin) + 65536;
-    cap = std::max(cap, grownCap);
-    cap = std::min<int64_t>(cap, (int64_t(1) << 32) - 1);
-    pairCap = cap;
-    const size_t pc = size_t(pairCap);
-    U3D_TRY(pairTile.ensure(pc));
-    U3D_TRY(pairGauss.ensure(pc));
-    U3D_TRY(pairTileAlt.ensure(pc));
-    U3D_TRY(pairGaussAlt.ensure(pc));
-    const int64_t sortMax = std::max(std::max(totalEntries, pairCap), totalBlocks);
+    if (binned) {
+        uint32_t most = 0;
+        for (int v = 0; v < numItems; ++v) {
+            const uint32_t end = v + 1 < numItems ? starts[size_t(v) + 1] : K;
+            most = std::max(most, end - starts[size_t(v)]);
+        }
+        // one capacity for every item: the views of a scene change from step to step
+        int64_t cap = int64_t(double(most) * margin) + 65536;
+        cap = std::max(cap, grownCap);
+        const int64_t limit = (int64_t(1) << 31) / std::max(numItems, 1) - 1;
+        U3D_CHECK(cap <= limit, U3D_ERR_CAPACITY, "tile lists of the batch exceed 2^31 entries");
+        itemCap = cap;
+        pairCap = cap * numItems;
+        U3D_TRY(pairGauss.ensure(size_t(pairCap)));
+    } else {
+        int64_t cap = int64_t(double(K) * margin) + 65536;
+        cap = std::max(cap, grownCap);
+        cap = std::min<int64_t>(cap, (int64_t(1) << 32) - 1);
+        pairCap = cap;
+        const size_t pc = size_t(pairCap);
+        U3D_TRY(pairTile.ensure(pc));
+        U3D_TRY(pairGauss.ensure(pc));
+        U3D_TRY(pairTileAlt.ensure(pc));
+        U3D_TRY(pairGaussAlt.ensure(pc));
+    }
+    const int64_t sortMax = std::max(std::max(totalEntries, binned ? 0 : pairCap), totalBlocks);
     U3D_TRY(hist.ensure(radixHistSize(sortMax)));
     U3D_TRY(histScan.ensure(radixScanScratchSize(sortMax)));
     U3D_TRY(scanScratch.ensure(scanScratchSize(sortMax)));
@@ -469,7 +518,9 @@ int BatchEngine::probePairs()
     for (size_t b : {rec0.cap * 16, rec1.cap * 16, rec2.cap * 4, depth.cap * 4, grad2d.cap * 4,
                      radius.cap * 4, rect.cap * 16, flags.cap, depthKey.cap * 4, nTiles.cap * 4,
                      keyAlt.cap * 4, valA.cap * 4, valAlt.cap * 4, ntSorted.cap * 4,
-                     pairOffsets.cap * 4, entryItem.cap * 4, pairTile.cap * 16, hist.cap * 4,
+                     pairOffsets.cap * 4, entryItem.cap * 4, pairTile.cap * 12, pairGauss.cap * 4,
+                     binCounts[0].cap * 4 * kBranches, tileTotal[0].cap * 4 * kBranches,
+                     rectSorted.cap * 16, hist.cap * 4,
                      histScan.cap * 4, scanScratch.cap * 4, rgb.cap * 4, dl.cap * 4, finalT.cap * 4,
                      nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4})
         allocated += b;
@@ -494,12 +545,48 @@ int BatchEngine::enqueue()
     const int *tileList = nullptr;
     {
         StageScope scope(ctx, U3D_STAGE_SORT);
-        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4)); // K, overflow flag
+        // K, overflow flag, longest item list set
+        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4));
         U3D_TRY(launchIota(s, valA.p, totalEntries));
         RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap, tuning.sort_items};
+        rs.radixBits = tuning.sort_radix ? tuning.sort_radix : 8;
         bool alt = false;
         U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs, &alt));
         const uint32_t *sorted = alt ? valAlt.p : valA.p;
+        if (binned) {
+            // stable pass on the item: each item's entries contiguous (at its screenBase, every
+            // entry of the item is in the sort), in depth order, invisible ones last
+            uint32_t *k0 = alt ? depthKey.p : keyAlt.p, *k1 = alt ? keyAlt.p : depthKey.p;
+            uint32_t *v1 = alt ? valAlt.p : valA.p, *v0 = alt ? valA.p : valAlt.p;
+            U3D_TRY(launchGatherU32(s, reinterpret_cast<const uint32_t *>(entryItem.p), v1, k0,
+                                    totalEntries, totalEntries));
+            bool alt4 = false;
+            U3D_TRY(radixSortPairs(s, k0, v1, k1, v0, totalEntries, itemBits, rs, &alt4));
+            const uint32_t *byItem = alt4 ? v0 : v1;
+            // items in kBranches branches: each item's three binning launches are small, and
+            // the items write disjoint parts of every buffer but the branch's own scratch
+            const int nb = std::min(kBranches, numItems);
+            U3D_HIP(hipEventRecord(forkEvent, s));
+            for (int b = 0; b < nb; ++b)
+                U3D_HIP(hipStreamWaitEvent(branch[b], forkEvent, 0));
+            for (int v = 0; v < numItems; ++v) {
+                const ItemGeometry &g = geo[size_t(v)];
+                const int b = v % nb;
+                const BinningList bl{totalEntries, uint32_t(int64_t(v) * itemCap), total.p, total.p + 2,
+                                     total.p + 1};
+                U3D_TRY(launchTileBinning(branch[b], g.n, byItem + g.screenBase, nTiles.p, rect.p,
+                                          ntSorted.p + g.screenBase, rectSorted.p + g.screenBase,
+                                          g.tilesX, g.tilesY, binCounts[b].p, binCounts[b].cap,
+                                          tileTotal[b].p, tileOffsets.p + g.tileBase + v, pairGauss.p,
+                                          size_t(itemCap), nullptr, nullptr, nullptr, &bl));
+            }
+            for (int b = 0; b < nb; ++b) {
+                U3D_HIP(hipEventRecord(branchDone[b], branch[b]));
+                U3D_HIP(hipStreamWaitEvent(s, branchDone[b], 0));
+            }
+            tileList = reinterpret_cast<const int *>(pairGauss.p);
+        }
+        if (!binned) {
         U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries, totalEntries));
         U3D_TRY(exclusiveScanU32(s, ntSorted.p, pairOffsets.p, totalEntries, total.p,
                                  scanScratch.p, scanScratch.cap));
@@ -512,6 +599,7 @@ int BatchEngine::enqueue()
         const uint32_t *tileSorted = alt2 ? pairTileAlt.p : pairTile.p;
         tileList = reinterpret_cast<const int *>(alt2 ? pairGaussAlt.p : pairGauss.p);
         U3D_TRY(launchTileOffsets(s, tileSorted, pairCap, totalTiles, tileOffsets.p, total.p));
+        }
         if (tuning.heavy_first) {
             U3D_TRY(launchBlockCosts(s, dItems, numItems, totalBlocks, st.tile_size, tileOffsets.p,
                                      blockKey.p, blockOrder.p));
@@ -528,11 +616,9 @@ int BatchEngine::enqueue()
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(counters.p), 0, 2));
         const int occ = perCu > 0 ? perCu : occFwd[li];
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
-        const SegmentState seg = segmented ? SegmentState{ckpt.p, segCount.p}
-                                           : SegmentState{nullptr, nullptr};
         U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
-                                    st, nullptr, lds, 0, seg));
+                                    st, nullptr, lds, 0, SegmentState{nullptr, nullptr}));
     }
     {
         StageScope scope(ctx, U3D_STAGE_LOSS);
@@ -541,23 +627,12 @@ int BatchEngine::enqueue()
     }
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
-        if (segmented) {
-            U3D_TRY(exclusiveScanU32(s, segCount.p, segStart.p, totalTiles, segTotal.p,
-                                     scanScratch.p, scanScratch.cap));
-            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(segCounter.p), 0, 1));
-            const int occ = perCu > 0 ? perCu : occSeg;
-            U3D_TRY(launchRasterSegmentedBatched(s, dItems, numItems, totalTiles, segStart.p,
-                                                 segTotal.p, segCounter.p, ctx->num_cus * occ,
-                                                 tileOffsets.p, tileList, sb, st, ckpt.p,
-                                                 grad2d.p, lds));
-        } else {
-            const int occ = perCu > 0 ? perCu : occBwd[li];
-            const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
-            U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks,
-                                        counters.p + 1, int(std::max<int64_t>(wgs, 1)),
-                                        tileOffsets.p, tileList, sb, st, grad2d.p, lds,
-                                        tuning.atomic_lanes, SegmentState{nullptr, nullptr}));
-        }
+        const int occ = perCu > 0 ? perCu : occBwd[li];
+        const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
+        U3D_TRY(launchRasterBatched(s, true, dItems, numItems, order, totalBlocks, counters.p + 1,
+                                    int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
+                                    st, grad2d.p, lds, tuning.atomic_lanes,
+                                    SegmentState{nullptr, nullptr}));
     }
     {
         StageScope scope(ctx, U3D_STAGE_CHAIN);
@@ -603,8 +678,9 @@ int BatchEngine::poll(bool wait)
             U3D_HIP(hipEventSynchronize(statusDone[k]));
         else if (hipEventQuery(statusDone[k]) != hipSuccess)
             continue;
-        const int64_t K = int64_t(statusHost[2 * k]);
-        const bool overflow = statusHost[2 * k + 1] != 0;
+        const int64_t K = int64_t(statusHost[4 * k]);
+        const bool overflow = statusHost[4 * k + 1] != 0;
+        const int64_t most = int64_t(statusHost[4 * k + 2]); // binned: the longest item list set
         for (u3d_trainer *t : statusScenes[k]) {
             if (statusScenes[k].size() == 1)
                 t->counters[3] += K;
@@ -612,8 +688,10 @@ int BatchEngine::poll(bool wait)
                 t->counters[5] += 1;
         }
         statusStep[k] = -1;
-        if (double(K) > 0.9 * double(pairCap)) {
-            grownCap = std::max(grownCap, int64_t(double(K) * 1.3) + 65536);
+        // binned: capacities are per item; otherwise for the whole batch
+        const int64_t used = binned ? most : K, cap = binned ? itemCap : pairCap;
+        if (double(used) > 0.9 * double(cap)) {
+            grownCap = std::max(grownCap, int64_t(double(used) * 1.3) + 65536);
             needRelayout = true;
         }
     }
@@ -652,7 +730,6 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
     if (totalEntries > 0) {
         if (pairCap == 0)
             U3D_TRY(probePairs());
-        U3D_TRY(ensureSegments());
         if (useGraph) {
             if (!exec)
                 U3D_TRY(capture());
@@ -663,7 +740,7 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
         }
         if (statusStep[slot] >= 0)
             U3D_TRY(poll(true));
-        U3D_HIP(hipMemcpyAsync(statusHost + 2 * slot, total.p, 2 * sizeof(uint32_t),
+        U3D_HIP(hipMemcpyAsync(statusHost + 4 * slot, total.p, 4 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, s));
         U3D_HIP(hipEventRecord(statusDone[slot], s));
         statusStep[slot] = stepNo;
@@ -693,12 +770,14 @@ int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPe
     const hipStream_t s = ctx->stream;
     std::vector<u3d_batch_tuning> cands;
     const int baseTile = numScenes > 0 ? scenes[0]->opt.raster.tile_size : 16;
+    // stage 1: the compositing launches and the tile size (sorts at their defaults)
     for (int tile : tuneTile ? std::vector<int>{16, 32} : std::vector<int>{baseTile})
         for (int lds : {64, 128, 256})
             for (int heavy : {1, 0})
                 for (int perCu : {0, 2})
                     for (int items : {0, 4})
-                        cands.push_back({heavy, perCu, lds, items, tile, 0});
+                        cands.push_back({heavy, perCu, lds, items, tile, 0, 0, 0});
+    const size_t stage1 = cands.size();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     U3D_HIP(hipEventCreate(&e0));
     U3D_HIP(hipEventCreate(&e1));
@@ -706,7 +785,8 @@ int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPe
     u3d_batch_tuning best = tuning;
     best.tile_size = baseTile;
     int rc = U3D_OK;
-    for (const u3d_batch_tuning &c : cands) {
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
+        const u3d_batch_tuning c = cands[ci];
         bool live = false;
         for (int k = 0; k < numScenes; ++k)
             live |= scenes[k]->iteration < scenes[k]->opt.iterations;
@@ -732,6 +812,16 @@ int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPe
         if (done > 0 && ms / float(done) < bestMs) {
             bestMs = ms / float(done);
             best = c;
+        }
+        if (ci + 1 == stage1) {
+            // stage 2, from the best of stage 1: the sort radix and how the tile lists are built
+            for (int radix : {8, 6, 4})
+                for (int pairs : {0, 1}) {
+                    u3d_batch_tuning next = best;
+                    next.sort_radix = radix;
+                    next.pair_sort = pairs;
+                    cands.push_back(next);
+                }
         }
     }
     (void)hipEventDestroy(e0);

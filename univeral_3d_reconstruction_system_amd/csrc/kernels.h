@@ -30,6 +30,9 @@ struct RenderItem {
     int64_t blockBase;  // first global 16x16 raster block
     int64_t pixBase;    // first pixel of the item in the concatenated per-pixel buffers
     int tileBase, tilesX, tilesY, blocksX;
+    // the item's tile offsets are tileOffsets[tileBase + offsetShift + t], t in [0, tiles]
+    // (offsetShift = item index when every item's offsets are stored with their own end)
+    int offsetShift;
     int shDegree;
     float ndcScale;     // densification statistic per view: undoes the 1/B of the averaged loss
     const float *means, *scales, *quats, *opacities, *colors;
@@ -260,6 +263,7 @@ struct RadixScratch {
     uint32_t *scanTmp; // radixScanScratchSize(count)
     size_t scanTmpCap;
     int items = 0;     // keys per lane per block: 4 or 8 (0: by size)
+    int radixBits = 8; // digit bits per pass of the multi-launch sort (1..8)
     bool valsAreIndex = false; // the input values are 0..count-1 (vals is then not read)
     // Optional small-sort scratch (onesweepScratchWords() words, zero-filled once when
     // allocated and then left to the sort) and its host-side call counter: counts up to 2^20
@@ -319,11 +323,21 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
 // sortError (nullable): the small sort's error word (onesweepErrorPtr); when set, every
 // Gaussian counts as invisible. walkCost (nullable, tiles entries): per-tile walk costs of the
 // last forward of this view (0: unknown), which order heavyFirst; cleared for the next forward.
+// Placement of one image's lists in a list shared by several images (the batched engine):
+// ids in sorted are below idLimit; the lists start at listBase of tileList (the offsets written
+// include it) and take at most listCapacity slots; listSum / listMax (atomically) receive the
+// image's pair count and overflow is set when it exceeds listCapacity.
+struct BinningList {
+    int64_t idLimit;
+    uint32_t listBase;
+    uint32_t *listSum, *listMax, *overflow;
+};
 int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
                       uint32_t *tileList, size_t listCapacity, int *heavyFirst = nullptr,
-                      const uint32_t *sortError = nullptr, uint32_t *walkCost = nullptr);
+                      const uint32_t *sortError = nullptr, uint32_t *walkCost = nullptr,
+                      const BinningList *list = nullptr);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot; slots at or past capacity are not written.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,
@@ -389,12 +403,6 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                           const float *dl, const float *finalT, const int *nContrib,
                           const float4 *ckpt, float *grad2d, const LossFinalize *fin = nullptr,
                           bool prepDone = false, const uint64_t *hitBits = nullptr);
-int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
-                                 const uint32_t *segStart, const uint32_t *totalItems,
-                                 unsigned *counter, int numWorkgroups, const int *tileOffsets,
-                                 const int *tileList, const ScreenBuffers &sb,
-                                 const u3d_raster_settings &st, const float4 *ckpt,
-                                 float *grad2d, int ldsBatch);
 int rasterSegmentedOccupancy();
 int launchRasterBackward(hipStream_t s, const RasterGeometry &g, const u3d_raster_settings &st,
                          const int *tileOffsets, const int *tileList, const ScreenBuffers &sb,

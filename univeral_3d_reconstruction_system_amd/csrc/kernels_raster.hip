@@ -1232,20 +1232,6 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardAnyKernel(
     }
 }
 
-// First index in sorted[0..n) greater than w, minus one: the tile owning work item w.
-__device__ __forceinline__ int tileOfItem(const uint32_t *__restrict__ segStart, int tiles, uint32_t w)
-{
-    int lo = 0, hi = tiles - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (segStart[mid] <= w)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 // One workgroup: segStart = exclusive scan of segCount over the tiles (tiles + 1 entries),
 // *totalItems, the owning tile of every work item (itemTile[segStart[t] + k] = t for
 // k < segCount[t]) and the work counter cleared.
@@ -1296,45 +1282,6 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
     }
 }
 
-// Batched form: tiles are global (16x16, so global tile = global raster block).
-template <int kBatch>
-__global__ __launch_bounds__(kThreads) void rasterSegmentedBatchedKernel(
-    const RenderItem *__restrict__ items, int numItems, int tiles,
-    const uint32_t *__restrict__ segStart, const uint32_t *__restrict__ totalItems,
-    unsigned *counter, const int *__restrict__ tileOffsets, const int *__restrict__ tileList,
-    const float4 *__restrict__ rec0, const float4 *__restrict__ rec1,
-    const float *__restrict__ rec2, const u3d_raster_settings st, const float4 *__restrict__ ckpt,
-    float *__restrict__ grad2d)
-{
-    __shared__ uint32_t sWork;
-    const uint32_t total = *totalItems;
-    for (;;) {
-        if (threadIdx.x == 0)
-            sWork = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t w = sWork;
-        __syncthreads();
-        if (w >= total)
-            break;
-        const int tile = tileOfItem(segStart, tiles, w);
-        const int v = itemOfBlock(items, numItems, tile);
-        const RenderItem &it = items[v];
-        const int local = int(tile - it.blockBase);
-        if (noSpecialExp(st.min_alpha))
-            backwardSegment<kBatch, true>(local % it.blocksX, local / it.blocksX, it.tileBase,
-                                          int(w - segStart[tile]), it.cam.width, it.cam.height,
-                                          it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,
-                                          st.background[0], st.background[1], st.background[2],
-                                          st.min_alpha, it.dl, it.finalT, it.nContrib, ckpt, grad2d);
-        else
-            backwardSegment<kBatch, false>(local % it.blocksX, local / it.blocksX, it.tileBase,
-                                           int(w - segStart[tile]), it.cam.width, it.cam.height,
-                                           it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,
-                                           st.background[0], st.background[1], st.background[2],
-                                           st.min_alpha, it.dl, it.finalT, it.nContrib, ckpt, grad2d);
-    }
-}
-
 template <bool kFastExp>
 __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
     int W, int H, int ts, int tilesX, int blocksX, const int *__restrict__ tileOffsets,
@@ -1372,15 +1319,16 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
         const RenderItem &it = items[v];
         const int local = int(b - it.blockBase);
         const int bx = local % it.blocksX, by = local / it.blocksX;
+        const int *offs = tileOffsets + it.offsetShift;
 #define U3D_FWD(CK, FE)                                                                          \
     forwardBlock<kBatch, CK, FE>(bx, by, it.tileBase, it.cam.width, it.cam.height, st.tile_size,   \
-                                 it.tilesX, tileOffsets, tileList, rec0, rec1, rec2,               \
+                                 it.tilesX, offs, tileList, rec0, rec1, rec2,                      \
                                  st.background[0], st.background[1], st.background[2],             \
                                  st.min_alpha, st.min_transmittance, it.rgb, it.finalT,            \
                                  it.nContrib, seg)
 #define U3D_BWD(FE)                                                                              \
     backwardBlock<kBatch, kFewLanes, FE>(bx, by, it.tileBase, it.cam.width, it.cam.height,         \
-                                         st.tile_size, it.tilesX, tileOffsets, tileList, rec0,     \
+                                         st.tile_size, it.tilesX, offs, tileList, rec0,            \
                                          rec1, rec2, st.background[0], st.background[1],           \
                                          st.background[2], st.min_alpha, it.dl, it.finalT,         \
                                          it.nContrib, grad2d)
@@ -1574,32 +1522,6 @@ int launchRasterSegmented(hipStream_t s, const RasterGeometry &g, const u3d_rast
                        tileList, sb.rec0, sb.rec1, sb.rec2, st.background[0], st.background[1],
                        st.background[2], st.min_alpha, dl, finalT, nContrib, ckpt, grad2d, inMain,
                        hitBits);
-    U3D_LAUNCH_CHECK();
-    return U3D_OK;
-}
-
-int launchRasterSegmentedBatched(hipStream_t s, const RenderItem *items, int numItems, int tiles,
-                                 const uint32_t *segStart, const uint32_t *totalItems,
-                                 unsigned *counter, int numWorkgroups, const int *tileOffsets,
-                                 const int *tileList, const ScreenBuffers &sb,
-                                 const u3d_raster_settings &st, const float4 *ckpt,
-                                 float *grad2d, int ldsBatch)
-{
-    if (tiles <= 0)
-        return U3D_OK;
-    U3D_CHECK(st.tile_size == 16, U3D_ERR_INVALID, "segmented backward needs 16-pixel tiles");
-    if (ldsBatch == 64)
-        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<64>, dim3(numWorkgroups), dim3(kThreads), 0, s,
-                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
-                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
-    else if (ldsBatch == 128)
-        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<128>, dim3(numWorkgroups), dim3(kThreads), 0, s,
-                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
-                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
-    else
-        hipLaunchKernelGGL(rasterSegmentedBatchedKernel<256>, dim3(numWorkgroups), dim3(kThreads), 0, s,
-                           items, numItems, tiles, segStart, totalItems, counter, tileOffsets,
-                           tileList, sb.rec0, sb.rec1, sb.rec2, st, ckpt, grad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

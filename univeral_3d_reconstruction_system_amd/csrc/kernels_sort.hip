@@ -821,7 +821,7 @@ __global__ void blockCostKernel(const RenderItem *__restrict__ items, int numIte
     const RenderItem &it = items[lo];
     const int local = int(b - it.blockBase);
     const int bx = local % it.blocksX, by = local / it.blocksX;
-    const int tile = it.tileBase + (by * 16 / tileSize) * it.tilesX + bx * 16 / tileSize;
+    const int tile = it.tileBase + it.offsetShift + (by * 16 / tileSize) * it.tilesX + bx * 16 / tileSize;
     const uint32_t len = uint32_t(tileOffsets[tile + 1] - tileOffsets[tile]);
     // 8-bit log-scale cost (exponent, 3 mantissa bits), inverted so that an ascending one-pass
     // radix sort lists the heaviest blocks first.
@@ -877,12 +877,13 @@ struct BinLds {
 // they are read from ntS / rectS.
 // failed (kGather only): the depth sort gave up a lookback wait, so its order may not be a
 // permutation; every position then counts as invisible (empty tile lists, nothing out of bounds).
+// Ids at or above idLimit (the length of nTiles / rect) count as invisible too.
 template <bool kGather>
 __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
                                          const uint32_t *__restrict__ nTiles,
                                          const int4 *__restrict__ rect, uint32_t *ntS,
                                          int4 *rectS, int64_t first, int count, uint32_t *D,
-                                         int stride, bool failed = false)
+                                         int stride, bool failed = false, int64_t idLimit = 0)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -892,7 +893,7 @@ __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__
         if (i < n) {
             if (kGather) {
                 const uint32_t g = sorted[i];
-                nt = (!failed && int64_t(g) < n) ? nTiles[g] : 0u;
+                nt = (!failed && int64_t(g) < idLimit) ? nTiles[g] : 0u;
                 if (nt)
                     r = rect[g];
                 ntS[i] = nt;
@@ -971,7 +972,8 @@ constexpr int kCountChunk = 32, kEmitChunk = 8;
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
     const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
-    int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError)
+    int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError,
+    int64_t idLimit)
 {
     extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -986,7 +988,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     // the sort's passes have finished (stream order), so every workgroup reads the same word
     const bool failed = sortError && *sortError != 0u;
     addRects<true>(n, sorted, nTiles, rect, ntS, rectS, int64_t(grp) * group + int64_t(wave) * slice,
-                   slice, D, stride, failed);
+                   slice, D, stride, failed, idLimit);
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
     for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
@@ -1193,7 +1195,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
     const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
-    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost)
+    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost,
+    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1242,7 +1245,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         if (t < tiles) {
             const uint32_t off = before + x - tot;
             if (grp == 0)
-                tileOffsets[t] = int(off < capacity ? off : capacity);
+                tileOffsets[t] = int(listBase + (off < capacity ? off : capacity));
             const int ty = t / tilesX, tx = t - ty * tilesX;
             const int cell = ty * stride + tx;
             uint32_t base = off + counts[size_t(t) * size_t(numGroups) + grp];
@@ -1256,12 +1259,20 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         __syncthreads();
     }
     if (grp == 0) {
-        if (tid == 0)
-            tileOffsets[tiles] = int(carry < capacity ? carry : capacity);
+        if (tid == 0) {
+            tileOffsets[tiles] = int(listBase + (carry < capacity ? carry : capacity));
+            if (listSum) { // the batched engine's step status: pairs, longest list set, overflow
+                atomicAdd(listSum, carry);
+                atomicMax(listMax, carry);
+                if (carry > capacity)
+                    atomicOr(overflow, 1u);
+            }
+        }
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls, walkCost);
     }
-    emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList, capacity);
+    emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList + listBase,
+              capacity);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1285,8 +1296,9 @@ int radixPasses(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keysAlt
     U3D_CHECK(size_t(numBlocks) * 256 <= scratch.histCap, U3D_ERR_CAPACITY, "radix histogram scratch too small");
     U3D_CHECK(scratch.scanTmpCap >= 256, U3D_ERR_CAPACITY, "radix digit-total scratch too small");
     uint32_t *kin = keys, *vin = vals, *kout = keysAlt, *vout = valsAlt;
-    for (int shift = 0; shift < bits; shift += 8) {
-        const int b = bits - shift < 8 ? bits - shift : 8;
+    const int digit = scratch.radixBits >= 1 && scratch.radixBits <= 8 ? scratch.radixBits : 8;
+    for (int shift = 0; shift < bits; shift += digit) {
+        const int b = bits - shift < digit ? bits - shift : digit;
         hipLaunchKernelGGL(radixUpsweep<kItems>, dim3(numBlocks), dim3(kRadixThreads), 0, s, kin,
                            count, shift, b, numBlocks, scratch.hist, countDev);
         U3D_LAUNCH_CHECK();
@@ -1476,19 +1488,21 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                       const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
                       uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
                       uint32_t *tileList, size_t listCapacity, int *heavyFirst,
-                      const uint32_t *sortError, uint32_t *walkCost)
+                      const uint32_t *sortError, uint32_t *walkCost, const BinningList *list)
 {
+    const BinningList plain{n, 0, nullptr, nullptr, nullptr};
+    const BinningList &bl = list ? *list : plain;
     const int group = tileBinningGroup(n, tilesX, tilesY);
     U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
     const int tiles = tilesX * tilesY;
     if (n <= 0)
-        return launchFillI32(s, tileOffsets, 0, int64_t(tiles) + 1);
+        return launchFillI32(s, tileOffsets, int(bl.listBase), int64_t(tiles) + 1);
     const int numGroups = blocksFor(n, group);
     U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
     const int cells = (tilesX + 1) * (tilesY + 1);
     hipLaunchKernelGGL(tileCountKernel, dim3(xcdGridSize(numGroups, kCountChunk)), dim3(64 * kBinWaves),
                        size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
-                       tilesX, tilesY, group, numGroups, counts, sortError);
+                       tilesX, tilesY, group, numGroups, counts, sortError, bl.idLimit);
     U3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
                        tileTotal);
@@ -1502,7 +1516,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList,
                        uint32_t(listCapacity < size_t(INT32_MAX) ? listCapacity : size_t(INT32_MAX)),
-                       walkCost);
+                       walkCost, bl.listBase, bl.listSum, bl.listMax, bl.overflow);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -415,9 +415,9 @@ class Trainer:
         return _tuning_dict(t)
 
     def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0,
-                   atomic_lanes=0):
+                   atomic_lanes=0, sort_radix=0, pair_sort=0):
         t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
-                             int(tile_size), int(atomic_lanes))
+                             int(tile_size), int(atomic_lanes), int(sort_radix), int(pair_sort))
         check(self.ctx.lib.u3d_trainer_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
@@ -483,7 +483,7 @@ class Trainer:
 def _tuning_dict(t) -> dict:
     return dict(heavy_first=t.heavy_first, workgroups_per_cu=t.workgroups_per_cu,
                 lds_batch=t.lds_batch, sort_items=t.sort_items, tile_size=t.tile_size,
-                atomic_lanes=t.atomic_lanes)
+                atomic_lanes=t.atomic_lanes, sort_radix=t.sort_radix, pair_sort=t.pair_sort)
 
 
 def _split_fields(flat: np.ndarray, n: int) -> dict:
@@ -756,9 +756,9 @@ class SceneSet:
         return int(self.ctx.lib.u3d_scene_set_bytes(self.h))
 
     def set_tuning(self, heavy_first=1, workgroups_per_cu=0, lds_batch=256, sort_items=0, tile_size=0,
-                   atomic_lanes=0):
+                   atomic_lanes=0, sort_radix=0, pair_sort=0):
         t = _abi.BatchTuning(int(heavy_first), int(workgroups_per_cu), int(lds_batch), int(sort_items),
-                             int(tile_size), int(atomic_lanes))
+                             int(tile_size), int(atomic_lanes), int(sort_radix), int(pair_sort))
         check(self.ctx.lib.u3d_scene_set_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, steps_per_candidate: int = 3, tune_tile_size: bool = False) -> dict:
