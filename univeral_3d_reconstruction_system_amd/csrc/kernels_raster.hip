@@ -115,6 +115,12 @@ struct BackState {
 // (kept by selects) and the Gaussian's nine screen-space gradient terms are summed over the
 // wavefront into accRow (LDS). kFewLanes > 0: when at most that many lanes contribute they add
 // their terms to accRow directly instead.
+// kFastExp (min_alpha >= 1e-37, so every power that passes the threshold test lies in
+// (-86, 0]): the gradient is held to 1e-4, not to bit-exactness, so G comes from the hardware
+// exp2 (relative error below 6e-6 for these powers) and 1 / (1 - alpha) from the hardware
+// reciprocal (1 ulp). Which Gaussians contribute and which are clamped must still be the
+// forward's decisions: a lane whose alpha lies within 2e-5 (relative) of min_alpha or of the
+// 0.99 clamp takes the exact exp instead.
 template <int kFewLanes, bool kFastExp>
 __device__ __forceinline__ void backStep(BackState &st, bool active, const float4 &a, const float4 &b,
                                          float thr, float c2, float fpx, float fpy, float dp0,
@@ -127,9 +133,21 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     const bool inRange = active & !(power > 0.0f) & !(power < thr);
     if (__ballot(inRange) == 0)
         return;
-    const float G = expfExactLdsIf(power, stab, kFastExp);
     const float op = b.y;
-    const float rawAlpha = op * G;
+    float G, rawAlpha;
+    if (kFastExp) {
+        G = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
+        rawAlpha = op * G;
+        const bool nearCut = inRange & ((fabsf(rawAlpha - minAlpha) <= 2e-5f * minAlpha) |
+                                        (fabsf(rawAlpha - kMaxAlpha) <= 2e-5f * kMaxAlpha));
+        if (__ballot(nearCut) != 0 && nearCut) {
+            G = expfExactLdsIf(power, stab, true);
+            rawAlpha = op * G;
+        }
+    } else {
+        G = expfExactLdsIf(power, stab, false);
+        rawAlpha = op * G;
+    }
     const float alpha = (rawAlpha < kMaxAlpha) ? rawAlpha : kMaxAlpha;
     const bool contrib = inRange & !(alpha < minAlpha);
     // A lane in range almost always contributes (the threshold is within 1e-3 of min_alpha),
@@ -142,7 +160,8 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
             return;
     }
     const float c0 = b.z, c1 = b.w;
-    const float Tn = st.T / (1.0f - alpha);
+    const float inv = kFastExp ? __builtin_amdgcn_rcpf(1.0f - alpha) : 0.0f;
+    const float Tn = kFastExp ? st.T * inv : st.T / (1.0f - alpha);
     const float w = alpha * Tn;
     const float acc0 = st.lastAlpha * st.last0 + (1.0f - st.lastAlpha) * st.acc0;
     const float acc1 = st.lastAlpha * st.last1 + (1.0f - st.lastAlpha) * st.acc1;
@@ -153,7 +172,7 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     dAlpha += (c2 - acc2) * dp2;
     dAlpha *= Tn;
     if (bgDot != 0.0f || bgDot != bgDot) // the background term; adding +-0 changes nothing
-        dAlpha += -fT / (1.0f - alpha) * bgDot;
+        dAlpha += kFastExp ? -fT * inv * bgDot : -fT / (1.0f - alpha) * bgDot;
     const bool grads = contrib & !(rawAlpha > kMaxAlpha);
     const float dPower = op * dAlpha * G;
     const float A = a.z, B = a.w, C = b.x;
