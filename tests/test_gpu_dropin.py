@@ -1,7 +1,14 @@
-"""The reference's own splat test suite (tests/test_splat.cpp of the reference), compiled
-unchanged against the drop-in C++ API (host/include, libu3d_splat.so over libu3d_hip.so) by
-`make -C oracle dropin`, run on the GPU: gradient check against finite differences,
-closed-form projection, D-SSIM gradient, training recovery, PLY round trip, crop."""
+"""The drop-in C++ API (host/include, libu3d_splat.so over libu3d_hip.so) on the GPU.
+
+* The reference's own splat test suite (tests/test_splat.cpp of the reference), compiled
+  unchanged against it by `make -C oracle dropin`: gradient check against finite differences,
+  closed-form projection, D-SSIM gradient, training recovery, PLY round trip and the
+  point-cloud fallback of readSplatPly, crop.
+* The pipeline's training loop (Reconstructor.cpp:325-361: stats() after every step, progress
+  every 10 iterations, cancellation before a step, export with writeSplatPly and read back),
+  written against the drop-in headers (tests/dropin/pipeline_loop.cpp, `make -C tests/dropin`).
+"""
+import json
 import os
 import subprocess
 
@@ -9,11 +16,13 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "oracle", "_ref", "u3d_tests_dropin")
+LOOP = os.path.join(ROOT, "tests", "dropin", "_build", "pipeline_loop")
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.path.exists(BIN), reason="drop-in suite not built (needs the reference sources)")]
+pytestmark = pytest.mark.gpu
+need_suite = pytest.mark.skipif(not os.path.exists(BIN), reason="drop-in suite not built (needs the reference sources)")
 
 
+@need_suite
 @pytest.mark.parametrize("name", ["splat_gradients", "splat_projection", "ssim_loss",
                                   "training_recovers", "ply_round_trip", "crop"])
 def test_reference_suite_on_gpu(name, tmp_path):
@@ -21,3 +30,28 @@ def test_reference_suite_on_gpu(name, tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failed" in r.stdout
+
+
+def _loop(tmp_path, iterations, cancel_at):
+    assert os.path.exists(LOOP), "tests/dropin not built (make -C tests/dropin)"
+    r = subprocess.run([LOOP, str(iterations), str(cancel_at), str(tmp_path / "model.ply")],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_pipeline_loop_trains_reports_and_exports(tmp_path):
+    d = _loop(tmp_path, 500, 0)
+    assert d["iterations"] == 500 and not d["cancelled"]
+    assert d["consistent"]  # stats().iteration follows every step, loss / PSNR finite
+    assert d["reports"] == 50 and d["last_report"] == 500 and d["logs"] == 2
+    assert d["psnr_after"] > d["psnr_before"] + 3.0, d
+    assert d["gaussians"] != d["initial"]  # densification ran (forIterations(500): 50-300)
+    assert d["round_trip"] and d["is_splat"] and d["file_rows"] == d["gaussians"], d
+
+
+def test_pipeline_loop_cancellation_stops_before_the_next_step(tmp_path):
+    d = _loop(tmp_path, 500, 120)
+    assert d["cancelled"] and d["iterations"] == 120 and d["last_report"] == 120
+    assert d["consistent"] and d["round_trip"] and d["file_rows"] == d["gaussians"]
