@@ -246,29 +246,51 @@ __device__ __forceinline__ void chainBody(
         float3 gb[15];
         viewDir(mu, campos, viewD, &viewLen);
         shBasis(viewD[0], viewD[1], viewD[2], kDeg > 0 ? kDeg : sh.degree, b, gb);
+        // One channel at a time: its coefficients (and moments) are loaded together before
+        // any of them is stored, so the loads of a channel overlap instead of each waiting for
+        // the previous coefficient's stores (the arrays could alias as far as the compiler
+        // knows).
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             const bool on = !(flags & (4 << ch)); // a clamped channel passes no gradient
             const float gc = g[5 + ch];
-            for (int k = 0; k < nb; ++k) {
-                const int64_t idx = i * sh.gs + int64_t(k * 3 + ch) * sh.cs;
-                const float coef = sh.coef[idx];
-                const float gk = on ? b[k] * gc : 0.0f;
-                if (on) {
-                    const float w = coef * gc;
+            float cf[15];
+#pragma unroll
+            for (int k = 0; k < 15; ++k)
+                if (k < nb)
+                    cf[k] = sh.coef[i * sh.gs + int64_t(k * 3 + ch) * sh.cs];
+#pragma unroll
+            for (int k = 0; k < 15; ++k)
+                if (k < nb && on) {
+                    const float w = cf[k] * gc;
                     dDir[0] += w * gb[k].x;
                     dDir[1] += w * gb[k].y;
                     dDir[2] += w * gb[k].z;
                 }
-                if (sh.m) {
-                    float pp = coef, mm = sh.m[idx], vv = sh.v[idx];
-                    adamScalar(pp, mm, vv, gk, sh.stepSize, sh.invSqrtBc2);
-                    sh.coef[idx] = pp;
-                    sh.m[idx] = mm;
-                    sh.v[idx] = vv;
-                } else if (sh.grad && on) {
-                    sh.grad[idx] += gk;
-                }
+            if (sh.m) {
+                float mm[15], vv[15];
+#pragma unroll
+                for (int k = 0; k < 15; ++k)
+                    if (k < nb) {
+                        const int64_t idx = i * sh.gs + int64_t(k * 3 + ch) * sh.cs;
+                        mm[k] = sh.m[idx];
+                        vv[k] = sh.v[idx];
+                    }
+#pragma unroll
+                for (int k = 0; k < 15; ++k)
+                    if (k < nb) {
+                        const int64_t idx = i * sh.gs + int64_t(k * 3 + ch) * sh.cs;
+                        float pp = cf[k];
+                        adamScalar(pp, mm[k], vv[k], on ? b[k] * gc : 0.0f, sh.stepSize, sh.invSqrtBc2);
+                        sh.coef[idx] = pp;
+                        sh.m[idx] = mm[k];
+                        sh.v[idx] = vv[k];
+                    }
+            } else if (sh.grad && on) {
+#pragma unroll
+                for (int k = 0; k < 15; ++k)
+                    if (k < nb)
+                        sh.grad[i * sh.gs + int64_t(k * 3 + ch) * sh.cs] += b[k] * gc;
             }
         }
     }
@@ -425,13 +447,25 @@ __global__ __launch_bounds__(kThreads) void chainAdamKernel(
     } else if (kDeg != 0 && sh.m) {
         // not visible: its coefficients still take the step with a zero gradient
         const int nb = shTerms<kDeg>(sh);
-        for (int c = 0; c < 3 * nb; ++c) {
-            const int64_t idx = i * sh.gs + int64_t(c) * sh.cs;
-            float pp = sh.coef[idx], mm = sh.m[idx], vv = sh.v[idx];
-            adamScalar(pp, mm, vv, 0.0f, sh.stepSize, sh.invSqrtBc2);
-            sh.coef[idx] = pp;
-            sh.m[idx] = mm;
-            sh.v[idx] = vv;
+        for (int c0 = 0; c0 < 3 * nb; c0 += 15) { // 15 coefficients' loads before their stores
+            float pp[15], mm[15], vv[15];
+#pragma unroll
+            for (int k = 0; k < 15; ++k)
+                if (c0 + k < 3 * nb) {
+                    const int64_t idx = i * sh.gs + int64_t(c0 + k) * sh.cs;
+                    pp[k] = sh.coef[idx];
+                    mm[k] = sh.m[idx];
+                    vv[k] = sh.v[idx];
+                }
+#pragma unroll
+            for (int k = 0; k < 15; ++k)
+                if (c0 + k < 3 * nb) {
+                    const int64_t idx = i * sh.gs + int64_t(c0 + k) * sh.cs;
+                    adamScalar(pp[k], mm[k], vv[k], 0.0f, sh.stepSize, sh.invSqrtBc2);
+                    sh.coef[idx] = pp[k];
+                    sh.m[idx] = mm[k];
+                    sh.v[idx] = vv[k];
+                }
         }
     }
     const float inv = adam.invSqrtBc2;
