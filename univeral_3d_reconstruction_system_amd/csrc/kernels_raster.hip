@@ -121,7 +121,10 @@ struct BackState {
 // reciprocal (1 ulp). Which Gaussians contribute and which are clamped must still be the
 // forward's decisions: a lane whose alpha lies within 2e-5 (relative) of min_alpha or of the
 // 0.99 clamp takes the exact exp instead.
-template <int kFewLanes, bool kFastExp>
+// kSkipIdle: return at once when no lane is in range. The segmented walk, whose entries come
+// from the forward's cull, rarely meets such a Gaussian and runs faster without the branch
+// (the compiler copies the walk state around it); the whole-list walk culls less and keeps it.
+template <int kFewLanes, bool kFastExp, bool kSkipIdle = false>
 __device__ __forceinline__ void backStep(BackState &st, bool active, const float4 &a, const float4 &b,
                                          float thr, float c2, float fpx, float fpy, float dp0,
                                          float dp1, float dp2, float fT, float bgDot, float minAlpha,
@@ -131,7 +134,7 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
     // bitwise tests: short-circuit ones become branches whose merges copy the walk state
     const bool inRange = active & !(power > 0.0f) & !(power < thr);
-    if (__ballot(inRange) == 0)
+    if (kSkipIdle && __ballot(inRange) == 0)
         return;
     const float op = b.y;
     float G, rawAlpha;
@@ -153,12 +156,9 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     // A lane in range almost always contributes (the threshold is within 1e-3 of min_alpha),
     // so only the few-lane path tests for no contribution; otherwise the selects below leave
     // the state unchanged.
-    uint64_t who = 0;
-    if (kFewLanes > 0) {
-        who = __ballot(contrib);
-        if (who == 0)
-            return;
-    }
+    // (no early exit when no lane contributes: the selects below keep the state and the
+    // few-lane path then adds nothing; a branch here made the compiler copy the walk state)
+    const uint64_t who = kFewLanes > 0 ? __ballot(contrib) : 0;
     const float c0 = b.z, c1 = b.w;
     const float inv = kFastExp ? __builtin_amdgcn_rcpf(1.0f - alpha) : 0.0f;
     const float Tn = kFastExp ? st.T * inv : st.T / (1.0f - alpha);
@@ -1014,36 +1014,16 @@ __device__ __forceinline__ void backwardBlock(
             uint64_t bits = __ballot((smask[c * 64 + lane] >> wave) & 1u);
             if (!bits)
                 continue;
-            // Reverse walk over the set bits, two register sets in turn (as backwardSegment).
-            auto step = [&](const float4 &ra, const float4 &rb, float rthr, float rc2, int jj) {
-                backStep<kFewLanes, kFastExp>(bs, lo + jj < nc, ra, rb, rthr, rc2, fpx, fpy, dp0, dp1,
-                                              dp2, fT, bgDot, minAlpha, stab, lane, acc[jj]);
-            };
-            int jA = c * 64 + 63 - __builtin_clzll(bits);
-            bits &= ~(uint64_t(1) << (jA - c * 64));
-            float4 aA = s0[jA], bA = s1[jA];
-            float thrA = sthr[jA], cA = s2[jA];
-            for (;;) {
-                if (!bits) {
-                    step(aA, bA, thrA, cA, jA);
-                    break;
-                }
-                const int jB = c * 64 + 63 - __builtin_clzll(bits);
-                bits &= ~(uint64_t(1) << (jB - c * 64));
-                const float4 aB = s0[jB], bB = s1[jB];
-                const float thrB = sthr[jB], cB = s2[jB];
-                step(aA, bA, thrA, cA, jA);
-                if (!bits) {
-                    step(aB, bB, thrB, cB, jB);
-                    break;
-                }
-                jA = c * 64 + 63 - __builtin_clzll(bits);
-                bits &= ~(uint64_t(1) << (jA - c * 64));
-                aA = s0[jA];
-                bA = s1[jA];
-                thrA = sthr[jA];
-                cA = s2[jA];
-                step(aB, bB, thrB, cB, jB);
+            // Reverse walk over the set bits. Measured against backwardSegment's double-buffered
+            // walk: this plain loop needs 30 fewer register copies per Gaussian and is faster
+            // here (the batched engine keeps more wavefronts per CU to hide the LDS latency),
+            // slower there.
+            while (bits) {
+                const int jj = c * 64 + 63 - __builtin_clzll(bits);
+                bits &= ~(uint64_t(1) << (jj - c * 64));
+                backStep<kFewLanes, kFastExp, true>(bs, lo + jj < nc, s0[jj], s1[jj], sthr[jj], s2[jj], fpx,
+                                              fpy, dp0, dp1, dp2, fT, bgDot, minAlpha, stab, lane,
+                                              acc[jj]);
             }
         }
         __syncthreads();
