@@ -1265,11 +1265,15 @@ __global__ __launch_bounds__(kThreads) void rasterSegmentedKernel(
     if (fin.partials && blockIdx.x == 0) // the step's loss value, before taking work items
         lossFinalizeBlock(fin);
     const uint32_t total = *totalItems;
-    for (;;) {
-        if (threadIdx.x == 0)
-            sWork = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t w = sWork;
+    // The first item of each workgroup is its own index; later ones come from the counter (so
+    // the launch does not start with every workgroup queueing on one atomic).
+    for (uint32_t round = 0;; ++round) {
+        if (round > 0) {
+            if (threadIdx.x == 0)
+                sWork = atomicAdd(counter, 1u) + gridDim.x;
+            __syncthreads();
+        }
+        const uint32_t w = round > 0 ? sWork : blockIdx.x;
         __syncthreads();
         if (w >= total)
             break;
