@@ -225,7 +225,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
     if (tileBinningGroup(n, tilesX, tilesY) > 0) {
         U3D_TRY(r->tileListBuf.ensure(kk));
         U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tilesX, tilesY))));
-        U3D_TRY(r->tileTotal.ensure(size_t(tiles)));
+        U3D_TRY(r->tileTotal.ensure(tileBinningTotalsSize(tilesX, tilesY)));
         U3D_TRY(r->heavyFirst.ensure(size_t(tiles)));
         U3D_TRY(r->rectSorted.ensure(nn));
         U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, r->ntSorted.p,

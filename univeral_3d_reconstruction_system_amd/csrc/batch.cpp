@@ -310,7 +310,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         size_t counts = 1, tiles = 1;
         for (const ItemGeometry &g : geo) {
             counts = std::max(counts, tileBinningCountsSize(g.n, g.tilesX, g.tilesY));
-            tiles = std::max(tiles, size_t(g.tilesX) * size_t(g.tilesY));
+            tiles = std::max(tiles, tileBinningTotalsSize(g.tilesX, g.tilesY));
         }
         for (int b = 0; b < kBranches; ++b) {
             U3D_TRY(binCounts[b].ensure(counts));

@@ -314,9 +314,11 @@ size_t scanScratchSize(int64_t count);
 // each tile's list of Gaussian ids in depth order to tileList (tileOffsets[tiles] entries,
 // tileOffsets[0, tiles] exclusive) without materialising or sorting (tile, Gaussian) pairs.
 // Supports tile grids with (tilesX + 1) (tilesY + 1) <= 12288 (one LDS grid cell per tile).
-// counts: tileBinningCountsSize entries; tileTotal: tilesX * tilesY entries.
+// counts: tileBinningCountsSize entries; tileTotal: tileBinningTotalsSize entries (the tile
+// totals, then the totals of blocks of 32 tiles).
 int tileBinningGroup(int64_t n, int tilesX, int tilesY); // Gaussians per group, 0: unsupported
 size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY);
+size_t tileBinningTotalsSize(int tilesX, int tilesY);
 // ntS, rectS (n entries each): scratch for the tile counts and rectangles in depth order.
 // heavyFirst (optional, tiles entries): the tiles by decreasing list length, written unless
 // n <= 0. Ids sorted[i] >= n count as invisible; list slots and offsets stay below listCapacity.

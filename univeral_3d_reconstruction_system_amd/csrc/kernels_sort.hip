@@ -858,8 +858,11 @@ __global__ void tileOffsetsKernel(const uint32_t *__restrict__ pairTile, int64_t
 // Gaussian by a binary search over the 64 exclusive pair prefixes of the current 64 Gaussians.
 
 constexpr int kBinWaves = 8;           // wavefronts per binning workgroup
-constexpr int kBinScanThreads = 256;
 constexpr int kBinLdsCells = 12288;    // u32 grid cells in LDS (W wavefronts x cells each)
+constexpr int kScanTiles = 32;         // tiles per scan workgroup (a lane each in both wave halves)
+constexpr int kScanWaves = 16;         // wavefronts per scan workgroup
+constexpr int kScanSubs = kScanWaves * (64 / kScanTiles); // ranges of groups per scan workgroup
+constexpr int kMaxScanBlocks = (kBinLdsCells + kScanTiles - 1) / kScanTiles;
 
 // Per-wavefront record of 64 Gaussians of the emission.
 struct BinLds {
@@ -966,8 +969,8 @@ template <int kChunk> __device__ __forceinline__ int xcdGroup(int b, int numGrou
 }
 constexpr int kCountChunk = 32, kEmitChunk = 8;
 
-// counts[t * numGroups + grp] = pairs of group grp in tile t (tile-major, so the scan over the
-// groups reads contiguous rows).
+// counts[grp * tiles + t] = pairs of group grp in tile t (group-major: each workgroup writes one
+// contiguous row, and the emission reads it back as one).
 // Also writes the tile counts and rectangles in depth order (ntS, rectS) for the emission.
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
@@ -991,72 +994,76 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
                    slice, D, stride, failed, idLimit);
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
-    for (int t = tid; t < tilesX * tilesY; t += 64 * kBinWaves) {
+    const int tiles = tilesX * tilesY;
+    uint32_t *row = counts + size_t(grp) * size_t(tiles);
+    for (int t = tid; t < tiles; t += 64 * kBinWaves) {
         const int ty = t / tilesX, tx = t - ty * tilesX;
-        counts[size_t(t) * size_t(numGroups) + grp] = D[ty * stride + tx];
+        row[t] = D[ty * stride + tx];
     }
 }
 
-// Exclusive scan of `len` values at data (kBinScanThreads x 4 per round, whole workgroup), written
-// back in place or, when asInt is given, to asInt; returns the total.
-__device__ __forceinline__ uint32_t workgroupScan(uint32_t *data, int len, int *asInt,
-                                                  uint32_t *carry)
+// Scan of the group counts, one workgroup per block of kScanTiles consecutive tiles. Lane l of
+// wavefront w takes tile l % kScanTiles and the (2w + l / kScanTiles)-th of kScanSubs consecutive
+// ranges of groups, so each load and store of a wavefront is two contiguous runs of counts rows.
+// Each count becomes, in place, the pairs of the earlier groups in its tile plus the pairs of
+// the earlier tiles of its block (the offset of the tile's list inside the block's lists); adding
+// the block's own offset (the exclusive scan of the block totals, which the emission does) gives
+// the group's first slot in the tile's list. tileTotal[t] receives tile t's pair count,
+// tileTotal[tiles + b] block b's.
+__global__ __launch_bounds__(64 * kScanWaves) void tileScanKernel(uint32_t *counts, int numGroups,
+                                                                 int tiles, uint32_t *tileTotal)
 {
-    constexpr int kW = kBinScanThreads / 64;
-    constexpr int kPer = 4;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t base = 0;
-    for (int start = 0; start < len; start += kBinScanThreads * kPer) {
-        const int i0 = start + tid * kPer;
-        uint32_t v[kPer], local = 0;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            v[k] = i0 + k < len ? data[i0 + k] : 0u;
-            local += v[k];
-        }
-        uint32_t x = local;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off)
-                x += y;
-        }
-        if (lane == 63)
-            carry[wave] = x;
-        __syncthreads();
-        uint32_t run = base, chunk = 0;
-#pragma unroll
-        for (int w = 0; w < kW; ++w) {
-            run += w < wave ? carry[w] : 0u;
-            chunk += carry[w];
-        }
-        run += x - local;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            if (i0 + k < len) {
-                if (asInt)
-                    asInt[i0 + k] = int(run);
-                else
-                    data[i0 + k] = run;
-            }
-            run += v[k];
-        }
-        base += chunk;
-        __syncthreads();
+    static_assert(kScanTiles == 32, "two ranges of groups per wavefront");
+    __shared__ uint32_t waveSum[kScanWaves][kScanTiles];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & (kScanTiles - 1), half = lane >> 5;
+    const int t = int(blockIdx.x) * kScanTiles + col;
+    const bool valid = t < tiles;
+    const int per = (numGroups + kScanSubs - 1) / kScanSubs;
+    const int sub = wave * 2 + half;
+    const int g0 = min(sub * per, numGroups), g1 = min(g0 + per, numGroups);
+    uint32_t *cnt = counts + (valid ? t : 0);
+    const size_t step = size_t(tiles);
+    uint32_t sum = 0;
+    if (valid) {
+#pragma unroll 16
+        for (int g = g0; g < g1; ++g)
+            sum += cnt[size_t(g) * step];
     }
-    return base;
-}
-
-// One workgroup per tile: exclusive scan of its row of group counts (in place), total to
-// tileTotal[t].
-__global__ __launch_bounds__(kBinScanThreads) void tileScanKernel(uint32_t *counts, int numGroups,
-                                                                  uint32_t *tileTotal)
-{
-    __shared__ uint32_t carry[kBinScanThreads / 64];
-    const uint32_t total = workgroupScan(counts + size_t(blockIdx.x) * size_t(numGroups),
-                                         numGroups, nullptr, carry);
-    if (threadIdx.x == 0)
-        tileTotal[blockIdx.x] = total;
+    // the range of the lower half comes first within the wavefront
+    const uint32_t low = uint32_t(__shfl(int(sum), col, 64));
+    if (half == 1)
+        waveSum[wave][col] = low + sum;
+    __syncthreads();
+    uint32_t before = half ? low : 0u, total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanWaves; ++w) {
+        const uint32_t c = waveSum[w][col];
+        before += w < wave ? c : 0u;
+        total += c;
+    }
+    uint32_t x = total; // inclusive scan of the tile totals over the block
+#pragma unroll
+    for (int off = 1; off < kScanTiles; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, kScanTiles);
+        if (col >= off)
+            x += y;
+    }
+    if (valid) {
+        uint32_t run = before + (x - total);
+#pragma unroll 16
+        for (int g = g0; g < g1; ++g) {
+            const uint32_t c = cnt[size_t(g) * step];
+            cnt[size_t(g) * step] = run;
+            run += c;
+        }
+    }
+    if (wave == 0 && half == 0) {
+        if (valid)
+            tileTotal[t] = total;
+        if (col == kScanTiles - 1)
+            tileTotal[size_t(tiles) + blockIdx.x] = x;
+    }
 }
 
 // The tiles ordered by decreasing cost (64 log-scale classes, any order inside a class), so
@@ -1187,7 +1194,10 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
 // split the group; each counts its slice per tile in its own difference grid, the counts become
 // per-wavefront cursors (tile offset + earlier groups + earlier slices) in place, then each
 // emits its slice in order (emitSlice).
-// Every workgroup scans the tile totals for the tile offsets itself; group 0 also writes them
+// The offset of a block of kScanTiles tiles is the exclusive scan of the block totals, which
+// every workgroup computes (at most kMaxScanBlocks values, in one wavefront); with the scanned counts (earlier
+// groups and earlier tiles of the block) it gives the group's first slot in each tile's list.
+// Group 0's row holds the in-block tile offsets alone, so group 0 writes the tile offsets
 // (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order. The totals equal the pair
 // count the list was sized for whenever the depth order is a permutation; after a sort whose
 // lookback wait gave up they may not, and then slots and offsets past `capacity` are clamped.
@@ -1200,15 +1210,56 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
-    const int tid = threadIdx.x, wave = tid >> 6;
+    __shared__ uint32_t blockBase[kMaxScanBlocks];
+    __shared__ uint32_t listTotal;
+    __shared__ uint32_t cls[64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int W = int(blockDim.x) >> 6;
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
+    const int tiles = tilesX * tilesY;
     const int slice = group / W;
     const int grp = xcdGroup<kEmitChunk>(int(blockIdx.x), numGroups);
     if (grp < 0)
         return;
     const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
     uint32_t *mine = cur + size_t(wave) * size_t(cells);
+    // the group's row of scanned counts; each thread's first kPre tiles are loaded here, so the
+    // loads overlap the difference grids
+    const uint32_t *row = counts + size_t(grp) * size_t(tiles);
+    constexpr int kPre = 4;
+    uint32_t pre[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int t = tid + k * int(blockDim.x);
+        pre[k] = t < tiles ? row[t] : 0u;
+    }
+    if (wave == 0) { // exclusive scan of the block totals, kPerLane per lane
+        constexpr int kPerLane = (kMaxScanBlocks + 63) / 64;
+        const int nb = (tiles + kScanTiles - 1) / kScanTiles;
+        uint32_t v[kPerLane], local = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            const int b = lane * kPerLane + k;
+            v[k] = b < nb ? tileTotal[size_t(tiles) + size_t(b)] : 0u;
+            local += v[k];
+        }
+        uint32_t x = local;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off)
+                x += y;
+        }
+        uint32_t run = x - local;
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            if (lane * kPerLane + k < nb)
+                blockBase[lane * kPerLane + k] = run;
+            run += v[k];
+        }
+        if (lane == 63)
+            listTotal = x;
+    }
     for (int t = tid; t < W * cells; t += int(blockDim.x))
         cur[t] = 0;
     __syncthreads();
@@ -1218,59 +1269,43 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
         __syncthreads();
         prefix2d(cur, W, tilesY + 1, stride);
     }
-    // tile offsets chunk by chunk (exclusive scan of the totals), then the cursors of the tiles
-    __shared__ uint32_t waveTot[kBinWaves];
-    __shared__ uint32_t cls[64];
-    const int lane = tid & 63, tiles = tilesX * tilesY;
-    uint32_t carry = 0;
-    for (int t0 = 0; t0 < tiles; t0 += int(blockDim.x)) {
-        const int t = t0 + tid;
-        const uint32_t tot = t < tiles ? tileTotal[t] : 0u;
-        uint32_t x = tot;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off)
-                x += y;
-        }
-        if (lane == 63)
-            waveTot[wave] = x;
-        __syncthreads();
-        uint32_t before = carry, chunk = 0;
+    // the cursors of tile t: its slots for this group, wavefront by wavefront
+    auto setCursors = [&](int t, uint32_t scanned) {
+        const uint32_t off = blockBase[t / kScanTiles] + scanned;
+        if (grp == 0)
+            tileOffsets[t] = int(listBase + (off < capacity ? off : capacity));
+        const int ty = t / tilesX, tx = t - ty * tilesX;
+        const int cell = ty * stride + tx;
+        uint32_t base = off;
         for (int w = 0; w < W; ++w) {
-            const uint32_t sw = waveTot[w];
-            before += w < wave ? sw : 0u;
-            chunk += sw;
+            const uint32_t c = cur[w * cells + cell];
+            cur[w * cells + cell] = base;
+            base += c;
         }
-        if (t < tiles) {
-            const uint32_t off = before + x - tot;
-            if (grp == 0)
-                tileOffsets[t] = int(listBase + (off < capacity ? off : capacity));
-            const int ty = t / tilesX, tx = t - ty * tilesX;
-            const int cell = ty * stride + tx;
-            uint32_t base = off + counts[size_t(t) * size_t(numGroups) + grp];
-            for (int w = 0; w < W; ++w) {
-                const uint32_t c = cur[w * cells + cell];
-                cur[w * cells + cell] = base;
-                base += c;
-            }
-        }
-        carry += chunk;
-        __syncthreads();
+    };
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int t = tid + k * int(blockDim.x);
+        if (t < tiles)
+            setCursors(t, pre[k]);
     }
+    for (int t = tid + kPre * int(blockDim.x); t < tiles; t += int(blockDim.x))
+        setCursors(t, row[t]);
     if (grp == 0) {
         if (tid == 0) {
-            tileOffsets[tiles] = int(listBase + (carry < capacity ? carry : capacity));
+            const uint32_t total = listTotal;
+            tileOffsets[tiles] = int(listBase + (total < capacity ? total : capacity));
             if (listSum) { // the batched engine's step status: pairs, longest list set, overflow
-                atomicAdd(listSum, carry);
-                atomicMax(listMax, carry);
-                if (carry > capacity)
+                atomicAdd(listSum, total);
+                atomicMax(listMax, total);
+                if (total > capacity)
                     atomicOr(overflow, 1u);
             }
         }
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls, walkCost);
     }
+    __syncthreads(); // every cursor is set before the wavefronts emit
     emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList + listBase,
               capacity);
 }
@@ -1478,6 +1513,12 @@ int tileBinningGroup(int64_t n, int tilesX, int tilesY)
     return int(g < (int64_t(1) << 22) ? g : (int64_t(1) << 22));
 }
 
+size_t tileBinningTotalsSize(int tilesX, int tilesY)
+{
+    const size_t tiles = size_t(tilesX) * size_t(tilesY);
+    return tiles + (tiles + kScanTiles - 1) / kScanTiles;
+}
+
 size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
 {
     const int g = tileBinningGroup(n, tilesX, tilesY);
@@ -1504,8 +1545,8 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                        size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
                        tilesX, tilesY, group, numGroups, counts, sortError, bl.idLimit);
     U3D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tileScanKernel, dim3(tiles), dim3(kBinScanThreads), 0, s, counts, numGroups,
-                       tileTotal);
+    hipLaunchKernelGGL(tileScanKernel, dim3((tiles + kScanTiles - 1) / kScanTiles),
+                       dim3(64 * kScanWaves), 0, s, counts, numGroups, tiles, tileTotal);
     U3D_LAUNCH_CHECK();
     // wavefronts per emission workgroup: as many slices as their grids fit in LDS
     int waves = kBinWaves;
