@@ -86,22 +86,11 @@ def test_forward_many_tiles(ctx, w, h):
     assert r.state()["num_pairs"] == ro.num_pairs() > 0
 
 
-@pytest.mark.parametrize("n_base", [200_000, 400_000])
-def test_tile_lists_at_scale(ctx, n_base):
-    """600k and 1.2M Gaussians (every one repeated three times, so equal depths occur and must
-    keep index order): the tile lists equal a numpy restatement of the reference's order --
-    a stable sort of the depth bits with the invisible last, then a stable scatter by tile
-    (Rasterizer.cpp:206-261) -- built from the device's own rectangles and depths. 600k keys
-    take 147 blocks of the one-launch-per-pass depth sort; 1.2M keys exceed its 2^20 limit and
-    take the three-launch passes."""
-    base = S.random_gaussians(n_base, 5)
-    g = {k: np.tile(v.reshape(n_base, -1), (3, 1)).reshape(-1) for k, v in base.items()}
-    cam = S.ring_cameras(8, 640, 480)[2]
-    r = U.Rasterizer(ctx)
-    r.forward(U.DeviceGaussians.from_host(ctx, g), cam, U.make_settings())
-    s = r.state()
+def _assert_reference_tile_lists(s):
+    """The tile lists and offsets of a raster state equal a numpy restatement of the reference's
+    order -- a stable sort of the depth bits with the invisible last, then a stable scatter by
+    tile (Rasterizer.cpp:206-261) -- built from the device's own rectangles and depths."""
     n, tiles_x, tiles = s["n"], s["tiles_x"], s["tiles_x"] * s["tiles_y"]
-    assert n == 3 * n_base
     vis = s["radius"] > 0
     key = np.where(vis, s["depth"].view(np.uint32), np.uint32(0xFFFFFFFF))
     order = np.argsort(key, kind="stable")
@@ -116,6 +105,37 @@ def test_tile_lists_at_scale(ctx, n_base):
     assert np.array_equal(s["tile_list"], order[rep][by].astype(np.int32))
     offsets = np.concatenate([[0], np.cumsum(np.bincount(tile, minlength=tiles))])
     assert np.array_equal(s["tile_offsets"], offsets.astype(np.int32))
+    return np.diff(offsets)
+
+
+@pytest.mark.parametrize("n_base", [200_000, 400_000])
+def test_tile_lists_at_scale(ctx, n_base):
+    """600k and 1.2M Gaussians (every one repeated three times, so equal depths occur and must
+    keep index order): the tile lists, binned in index order and sorted per tile on the depth
+    keys, equal the reference's order."""
+    base = S.random_gaussians(n_base, 5)
+    g = {k: np.tile(v.reshape(n_base, -1), (3, 1)).reshape(-1) for k, v in base.items()}
+    cam = S.ring_cameras(8, 640, 480)[2]
+    r = U.Rasterizer(ctx)
+    r.forward(U.DeviceGaussians.from_host(ctx, g), cam, U.make_settings())
+    s = r.state()
+    assert s["n"] == 3 * n_base
+    _assert_reference_tile_lists(s)
+
+
+def test_tile_lists_longer_than_one_workgroup_sorts(ctx):
+    """Lists of tens of thousands of entries (large Gaussians over a 160x120 image, every one
+    twice): the per-tile sort runs its chunked passes through global scratch for every list
+    beyond 11264 entries; lists, image, final transmittance and contributor counts equal the
+    reference's."""
+    base = S.random_gaussians(30_000, 11)
+    base["scales"] = (base["scales"] + np.float32(3.0)).astype(np.float32)
+    g = {k: np.tile(v.reshape(30_000, -1), (2, 1)).reshape(-1) for k, v in base.items()}
+    cam = S.ring_cameras(4, 160, 120)[1]
+    _, r, img, ro, ref = _forward_both(ctx, g, cam)
+    lengths = _assert_reference_tile_lists(r.state())
+    assert (lengths > 11264).sum() >= 10 and lengths.max() > 3 * 8192, lengths.max()
+    _assert_forward_equal(r, img, ro, ref)
 
 
 def test_forward_edge_cases(ctx):

@@ -179,72 +179,84 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         StageScope scope(r->ctx, U3D_STAGE_PROJECT);
         U3D_TRY(launchProject(s, m, cam, st, tilesX, tilesY, sb, sh, r->blockTiles.p));
     }
-    std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
-    // Depth order of the visible Gaussians (invisible keys sort last; ties keep index order).
-    U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
-    U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(nn))));
-    RadixScratch rs{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
-    rs.valsAreIndex = true;
-    {
-        const uint64_t *before = r->sortScratch.p;
-        U3D_TRY(r->sortScratch.ensure(onesweepScratchWords()));
-        if (r->sortScratch.p != before) // the small sort expects zeros in a new scratch
-            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(r->sortScratch.p), 0,
-                                  int64_t(r->sortScratch.cap) * 2));
-        rs.onesweep = r->sortScratch.p;
-        rs.onesweepCap = r->sortScratch.cap;
-        rs.onesweepEpoch = &r->sortEpoch;
-        rs.spinLimit = r->ctx->sortSpinLimit;
-    }
-    // The pair count K is known after the projection; it is summed with the sort's first
-    // launch and the host reads it (pinned memory) while the depth sort runs, so the device
-    // never waits for the host.
+    // The pair count K is the sum of the projection's block sums; it runs beside the next
+    // launch and reaches the host through pinned memory while the device works on.
+    SideSum pairSum;
     if (n > 0) {
-        rs.sideIn = r->blockTiles.p;
-        rs.sideCount = projectBlocks(n);
-        rs.sideDev = r->total.p;
-        rs.sideHost = pinnedWord;
-        rs.sideSeq = seq;
+        pairSum.in = r->blockTiles.p;
+        pairSum.count = projectBlocks(n);
+        pairSum.dev = r->total.p;
+        pairSum.host = pinnedWord;
+        pairSum.seq = seq;
     }
-    bool alt = false;
-    U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs, &alt));
-    const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
+    std::unique_ptr<StageScope> sortScope(new StageScope(r->ctx, U3D_STAGE_SORT));
     uint32_t K = 0;
-    bool sortFailed = false;
-    if (n > 0)
-        U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K, &sortFailed));
-    // An earlier depth sort of this rasterizer gave up a lookback wait (its kernels stayed in
-    // bounds, but that forward's order, image and gradients were wrong): report it here, the
-    // first point where the host reads the device without an extra wait.
-    U3D_CHECK(!sortFailed, U3D_ERR_HIP,
-              "depth sort: a lookback wait of an earlier forward on this rasterizer was abandoned; "
-              "its image and gradients are invalid");
-    r->numPairs = int64_t(K);
-    const size_t kk = size_t(K > 0 ? K : 1);
     const int *order = nullptr; // heavy-first tile order for the forward (binning path)
     if (tileBinningGroup(n, tilesX, tilesY) > 0) {
-        U3D_TRY(r->tileListBuf.ensure(kk));
+        // Tile lists in Gaussian index order (no pair count needed for the counts), then each
+        // list sorted on the depth keys. The host reads the pair count K, summed by the first
+        // workgroup of the count launch, while the counts run, and sizes the lists by it.
         U3D_TRY(r->binCounts.ensure(std::max<size_t>(1, tileBinningCountsSize(n, tilesX, tilesY))));
         U3D_TRY(r->tileTotal.ensure(tileBinningTotalsSize(tilesX, tilesY)));
         U3D_TRY(r->heavyFirst.ensure(size_t(tiles)));
-        U3D_TRY(r->rectSorted.ensure(nn));
-        U3D_TRY(launchTileBinning(s, n, sorted, r->nTiles.p, r->rect.p, r->ntSorted.p,
-                                  r->rectSorted.p, tilesX, tilesY, r->binCounts.p,
-                                  r->binCounts.cap, r->tileTotal.p, r->tileOffsets.p,
-                                  r->tileListBuf.p, kk, r->heavyFirst.p,
-                                  onesweepErrorPtr(r->sortScratch.p), ts == 16 ? walkCost : nullptr));
+        U3D_TRY(r->longFirst.ensure(size_t(tiles)));
+        U3D_TRY(launchTileCounts(s, n, nullptr, r->nTiles.p, r->rect.p, nullptr, nullptr, tilesX,
+                                 tilesY, r->binCounts.p, r->binCounts.cap, r->tileTotal.p,
+                                 nullptr, n, &pairSum));
+        if (n > 0)
+            U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K, nullptr));
+        const size_t kk = size_t(K > 0 ? K : 1);
+        U3D_TRY(r->tileListBuf.ensure(kk));
+        U3D_TRY(r->indexList.ensure(kk));
+        U3D_TRY(r->tileSortScratch.ensure(tileSortScratchSize(r->indexList.cap)));
+        U3D_TRY(launchTileEmit(s, n, nullptr, r->nTiles.p, r->rect.p, tilesX, tilesY,
+                               r->binCounts.p, r->tileTotal.p, r->tileOffsets.p, r->indexList.p,
+                               kk, r->heavyFirst.p, ts == 16 ? walkCost : nullptr, nullptr,
+                               r->longFirst.p));
+        if (n > 0)
+            U3D_TRY(launchTileSort(s, tiles, r->tileOffsets.p, r->longFirst.p, r->indexList.p,
+                                   r->tileListBuf.p, r->depthKey.p, r->tileSortScratch.p,
+                                   r->tileSortScratch.cap, r->indexList.cap));
         order = n > 0 ? r->heavyFirst.p : nullptr;
         r->tileList = reinterpret_cast<const int *>(r->tileListBuf.p);
     } else {
-        // Many tiles: pairs emitted in depth order, then a stable radix sort on the tile id.
-        // This path has no in-kernel check of the depth sort, so its error word is read here
-        // (a synchronisation, on this rare path only).
+        // Many tiles: depth order of the visible Gaussians (invisible keys sort last; ties keep
+        // index order), pairs emitted in depth order, then a stable radix sort on the tile id.
+        U3D_TRY(r->hist.ensure(radixHistSize(int64_t(nn))));
+        U3D_TRY(r->histScan.ensure(radixScanScratchSize(int64_t(nn))));
+        RadixScratch rs{r->hist.p, r->hist.cap, r->histScan.p, r->histScan.cap};
+        rs.valsAreIndex = true;
+        {
+            const uint64_t *before = r->sortScratch.p;
+            U3D_TRY(r->sortScratch.ensure(onesweepScratchWords()));
+            if (r->sortScratch.p != before) // the small sort expects zeros in a new scratch
+                U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(r->sortScratch.p), 0,
+                                      int64_t(r->sortScratch.cap) * 2));
+            rs.onesweep = r->sortScratch.p;
+            rs.onesweepCap = r->sortScratch.cap;
+            rs.onesweepEpoch = &r->sortEpoch;
+            rs.spinLimit = r->ctx->sortSpinLimit;
+        }
+        rs.sideIn = pairSum.in;
+        rs.sideCount = pairSum.count;
+        rs.sideDev = pairSum.dev;
+        rs.sideHost = pairSum.host;
+        rs.sideSeq = pairSum.seq;
+        bool alt = false;
+        U3D_TRY(radixSortPairs(s, r->depthKey.p, r->valA.p, r->keyAlt.p, r->valAlt.p, n, 32, rs,
+                               &alt));
+        const uint32_t *sorted = alt ? r->valAlt.p : r->valA.p;
+        if (n > 0)
+            U3D_TRY(waitPinnedWord(pinnedWord, seq, s, &K, nullptr));
+        // The pair emission has no check of the depth sort, so its error word is read here (a
+        // synchronisation, on this rare path only).
         if (n > 0) {
             uint32_t sortError = 0;
             U3D_TRY(onesweepErrorWord(r->sortScratch.p, s, &sortError));
             U3D_CHECK(sortError == 0, U3D_ERR_HIP,
                       "depth sort: a lookback wait was abandoned (the order is not valid)");
         }
+        const size_t kk = size_t(K > 0 ? K : 1);
         U3D_TRY(launchGatherU32(s, r->nTiles.p, sorted, r->ntSorted.p, n, n));
         U3D_TRY(exclusiveScanU32(s, r->ntSorted.p, r->pairOffsets.p, n, r->total.p,
                                  r->scanScratch.p, r->scanScratch.cap));
@@ -264,6 +276,7 @@ int rasterForwardImpl(u3d_raster *r, const u3d_gaussians &m, const u3d_camera &c
         r->tileList = reinterpret_cast<const int *>(alt2 ? r->pairGaussAlt.p : r->pairGauss.p);
         U3D_TRY(launchTileOffsets(s, tileSorted, int64_t(K), tiles, r->tileOffsets.p));
     }
+    r->numPairs = int64_t(K);
     sortScope.reset();
     // Checkpoints for the segmented backward (16-pixel tiles): state at every boundary of
     // kSegEntries records of each tile list.

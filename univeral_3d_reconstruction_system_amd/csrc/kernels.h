@@ -340,6 +340,38 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                       uint32_t *tileList, size_t listCapacity, int *heavyFirst = nullptr,
                       const uint32_t *sortError = nullptr, uint32_t *walkCost = nullptr,
                       const BinningList *list = nullptr);
+// The two halves of launchTileBinning: the per-(group, tile) counts and their scan, and the
+// emission (ntS / rectS are the arrays the counts were taken from: the depth-ordered copies, or
+// nTiles / rect themselves when sorted is null). sorted == nullptr bins the Gaussians in index
+// order: each tile's list comes out in index order (for launchTileSort). longFirst (nullable,
+// tiles entries): the tiles by decreasing list length.
+// side (nullable): a sum of side->in[0, side->count) as launchSumU32 does, run by an extra first
+// workgroup of the count launch (the pair count K from the projection's block sums, published
+// to the host while the counts run).
+struct SideSum {
+    const uint32_t *in = nullptr;
+    int64_t count = 0;
+    uint32_t *dev = nullptr;
+    uint64_t *host = nullptr;
+    uint32_t seq = 0;
+};
+int launchTileCounts(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
+                     const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+                     uint32_t *counts, size_t countsCap, uint32_t *tileTotal,
+                     const uint32_t *sortError, int64_t idLimit, const SideSum *side = nullptr);
+int launchTileEmit(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *ntS,
+                   const int4 *rectS, int tilesX, int tilesY, const uint32_t *counts,
+                   const uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList,
+                   size_t listCapacity, int *heavyFirst, uint32_t *walkCost,
+                   const BinningList *list, int *longFirst);
+// Sorts each tile's list (listIn at the tile's offsets, Gaussian ids in index order) stably on
+// depthKey[id] into listOut: the (depth bits, index) order. order (nullable): a permutation of
+// the tiles, sorted in that order (the longest first). scratch: tileSortScratchSize(capacity)
+// words for lists longer than one workgroup holds.
+size_t tileSortScratchSize(size_t listCapacity);
+int launchTileSort(hipStream_t s, int tiles, const int *tileOffsets, const int *order,
+                   const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,
+                   uint32_t *scratch, size_t scratchCap, size_t listCapacity);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot; slots at or past capacity are not written.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,

@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kRadixThreads) void radixDownsweep(
     }
 }
 
-// Sum on a workgroup of 1024 threads. hostOut (optional): one 64-bit word (seq << 32 | sum)
+// Sum on one workgroup (up to 1024 threads). hostOut (optional): one 64-bit word (seq << 32 | sum)
 // stored with a system-scope release; seq has 31 bits, and bit 63 of the word is set when
 // errWord (optional: the small sort's error word, set by an earlier call whose lookback wait
 // gave up) was non-zero. The word is cleared here, so each abandoned wait is reported once.
@@ -294,9 +294,9 @@ __device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_
                                          uint32_t *errWord = nullptr)
 {
     __shared__ uint32_t ws[16];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = int(blockDim.x);
     uint32_t v = 0;
-    for (int64_t i = tid; i < count; i += 1024)
+    for (int64_t i = tid; i < count; i += nt)
         v += in[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -306,8 +306,7 @@ __device__ __forceinline__ void sumBlock(const uint32_t *__restrict__ in, int64_
     __syncthreads();
     if (tid == 0) {
         uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w)
+        for (int w = 0; w < nt / 64; ++w)
             t += ws[w];
         *devOut = t;
         uint64_t err = 0;
@@ -881,12 +880,15 @@ struct BinLds {
 // failed (kGather only): the depth sort gave up a lookback wait, so its order may not be a
 // permutation; every position then counts as invisible (empty tile lists, nothing out of bounds).
 // Ids at or above idLimit (the length of nTiles / rect) count as invisible too.
+// indexOrder (kGather false): ntS / rectS are the tile counts and rectangles by Gaussian index
+// (positions are Gaussians, invisible ones anywhere) instead of a depth order.
 template <bool kGather>
 __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__ sorted,
                                          const uint32_t *__restrict__ nTiles,
                                          const int4 *__restrict__ rect, uint32_t *ntS,
                                          int4 *rectS, int64_t first, int count, uint32_t *D,
-                                         int stride, bool failed = false, int64_t idLimit = 0)
+                                         int stride, bool failed = false, int64_t idLimit = 0,
+                                         bool indexOrder = false)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -908,9 +910,9 @@ __device__ __forceinline__ void addRects(int64_t n, const uint32_t *__restrict__
                     r = rectS[i];
             }
         }
-        if (__ballot(nt != 0) == 0) { // invisible Gaussians sort last
-            if (kGather)
-                continue; // every position is still written
+        if (__ballot(nt != 0) == 0) { // invisible Gaussians sort last in a depth order
+            if (kGather || indexOrder)
+                continue; // every position is still written / more may follow
             break;
         }
         if (nt) {
@@ -972,12 +974,18 @@ constexpr int kCountChunk = 32, kEmitChunk = 8;
 // counts[grp * tiles + t] = pairs of group grp in tile t (group-major: each workgroup writes one
 // contiguous row, and the emission reads it back as one).
 // Also writes the tile counts and rectangles in depth order (ntS, rectS) for the emission.
+// sorted == nullptr: the groups are Gaussians in index order, read from nTiles / rect directly.
+// side.in: workgroup 0 runs that sum instead (the groups start at workgroup 1).
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
     const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
     int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError,
-    int64_t idLimit)
+    int64_t idLimit, SideSum side)
 {
+    if (side.in && blockIdx.x == 0) {
+        sumBlock(side.in, side.count, side.dev, side.host, side.seq);
+        return;
+    }
     extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
@@ -985,13 +993,20 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
         D[t] = 0;
     __syncthreads();
     const int slice = group / kBinWaves;
-    const int grp = xcdGroup<kCountChunk>(int(blockIdx.x), numGroups);
+    // the shift keeps each chunk of groups on one XCD (a rotation of the XCDs)
+    const int grp = xcdGroup<kCountChunk>(int(blockIdx.x) - (side.in ? 1 : 0), numGroups);
     if (grp < 0)
         return;
-    // the sort's passes have finished (stream order), so every workgroup reads the same word
-    const bool failed = sortError && *sortError != 0u;
-    addRects<true>(n, sorted, nTiles, rect, ntS, rectS, int64_t(grp) * group + int64_t(wave) * slice,
-                   slice, D, stride, failed, idLimit);
+    const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
+    if (sorted) {
+        // the sort's passes have finished (stream order), so every workgroup reads the same word
+        const bool failed = sortError && *sortError != 0u;
+        addRects<true>(n, sorted, nTiles, rect, ntS, rectS, first, slice, D, stride, failed,
+                       idLimit);
+    } else {
+        addRects<false>(n, nullptr, nullptr, nullptr, const_cast<uint32_t *>(nTiles),
+                        const_cast<int4 *>(rect), first, slice, D, stride, false, 0, true);
+    }
     __syncthreads();
     prefix2d(D, 1, tilesY + 1, stride);
     const int tiles = tilesX * tilesY;
@@ -1112,13 +1127,14 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
     }
 }
 
-// Writes the ids of the Gaussians [first, first + count) (depth order; one wavefront) at their
-// tiles' cursors in mine, advancing them. A step gives each of up to four Gaussians a run of
+// Writes the ids of the Gaussians at positions [first, first + count) of the depth order sorted
+// (or of the index order, sorted == nullptr; one wavefront) at their tiles' cursors in mine,
+// advancing them. A step gives each of up to four Gaussians a run of
 // lanes, one lane per tile of its rectangle (16 lanes each when all four have at most 16 tiles,
 // else 32 lanes for two with at most 32, else 64 lanes for one, in as many rounds as it needs).
 // Inside one Gaussian the tiles are distinct, so its lanes take their slots with one LDS atomic
 // add; the Gaussians of a step take theirs in separate rounds in list order, so equal tiles keep
-// the depth order.
+// the order of the positions.
 __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict__ sorted,
                                           const uint32_t *__restrict__ ntS,
                                           const int4 *__restrict__ rectS, int64_t first,
@@ -1131,30 +1147,38 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
         uint32_t g = 0, nt = 0;
         int4 r = make_int4(0, 0, 1, 1);
         if (i < n) {
-            g = sorted[i];
+            g = sorted ? sorted[i] : uint32_t(i);
             nt = ntS[i];
             if (nt)
                 r = rectS[i];
         }
         const uint64_t vis = __ballot(nt != 0);
-        if (vis == 0) // invisible Gaussians sort last
-            break;
-        // the visible Gaussians are a prefix of the lanes (depth order puts invisible ones last)
+        if (vis == 0) { // invisible Gaussians sort last in a depth order
+            if (sorted)
+                break;
+            continue;
+        }
+        // the visible Gaussians in lane order, packed to the front of the records (in a depth
+        // order they already are a prefix of the lanes)
         const int cnt = __popcll(vis);
+        const int rk = __popcll(vis & laneMaskLt());
         const uint32_t w = uint32_t(r.z - r.x), h = uint32_t(r.w - r.y);
         waveSync();
-        L.nt[lane] = nt;
-        L.origin[lane] = uint32_t(r.x) | (uint32_t(r.y) << 16);
-        L.size[lane] = w | (h << 16);
-        // q = mulhi(k, magic) = k / w exactly for k * w < 2^32 (w >= 2); w == 1 is q = k
-        L.magic[lane] = w > 1u ? 0xffffffffu / w + 1u : 0u;
-        L.gauss[lane] = g;
+        if (nt) {
+            L.nt[rk] = nt;
+            L.origin[rk] = uint32_t(r.x) | (uint32_t(r.y) << 16);
+            L.size[rk] = w | (h << 16);
+            // q = mulhi(k, magic) = k / w exactly for k * w < 2^32 (w >= 2); w == 1 is q = k
+            L.magic[rk] = w > 1u ? 0xffffffffu / w + 1u : 0u;
+            L.gauss[rk] = g;
+        }
         waveSync();
+        const uint32_t ntc = L.nt[lane]; // lanes >= cnt: not read
         for (int j = 0; j < cnt;) {
-            const uint32_t n0 = uint32_t(__builtin_amdgcn_readlane(int(nt), j));
-            const uint32_t n1 = j + 1 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 1)) : 0u;
-            const uint32_t n2 = j + 2 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 2)) : 0u;
-            const uint32_t n3 = j + 3 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(nt), j + 3)) : 0u;
+            const uint32_t n0 = uint32_t(__builtin_amdgcn_readlane(int(ntc), j));
+            const uint32_t n1 = j + 1 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(ntc), j + 1)) : 0u;
+            const uint32_t n2 = j + 2 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(ntc), j + 2)) : 0u;
+            const uint32_t n3 = j + 3 < cnt ? uint32_t(__builtin_amdgcn_readlane(int(ntc), j + 3)) : 0u;
             const uint32_t m4 = max(max(n0, n1), max(n2, n3));
             const int shift = m4 <= 16u ? 4 : (max(n0, n1) <= 32u ? 5 : 6); // lanes per Gaussian
             const int slots = 64 >> shift;
@@ -1198,15 +1222,18 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
 // every workgroup computes (at most kMaxScanBlocks values, in one wavefront); with the scanned counts (earlier
 // groups and earlier tiles of the block) it gives the group's first slot in each tile's list.
 // Group 0's row holds the in-block tile offsets alone, so group 0 writes the tile offsets
-// (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order. The totals equal the pair
-// count the list was sized for whenever the depth order is a permutation; after a sort whose
-// lookback wait gave up they may not, and then slots and offsets past `capacity` are clamped.
+// (tileOffsets[0, tiles]) and, if asked, the heavy-first tile order and the tiles by decreasing
+// list length (longFirst). The totals equal the pair count the list was sized for whenever the
+// depth order is a permutation; after a sort whose lookback wait gave up they may not, and then
+// slots and offsets past `capacity` are clamped.
+// sorted == nullptr: positions are Gaussians in index order (ntS / rectS by Gaussian), and each
+// tile's list comes out in index order.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
     const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
     int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost,
-    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow)
+    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1265,7 +1292,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     __syncthreads();
     if (W > 1) {
         addRects<false>(n, sorted, nullptr, nullptr, const_cast<uint32_t *>(ntS),
-                        const_cast<int4 *>(rectS), first, slice, mine, stride);
+                        const_cast<int4 *>(rectS), first, slice, mine, stride, false, 0,
+                        sorted == nullptr);
         __syncthreads();
         prefix2d(cur, W, tilesY + 1, stride);
     }
@@ -1302,12 +1330,323 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
                     atomicOr(overflow, 1u);
             }
         }
+        if (longFirst) {
+            writeHeavyFirst(tileTotal, tiles, longFirst, cls, nullptr);
+            __syncthreads(); // cls is reused
+        }
         if (heavyFirst)
             writeHeavyFirst(tileTotal, tiles, heavyFirst, cls, walkCost);
     }
     __syncthreads(); // every cursor is set before the wavefronts emit
     emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList + listBase,
               capacity);
+}
+
+// ------------------------------------------------------------------ per-tile depth order
+// The plain path emits each tile's list in Gaussian index order and then sorts every list
+// stably on the Gaussians' depth keys (the float bits of the camera-space depth: positive, so
+// their unsigned order is the numeric one). The result is the reference's (depth bits, index)
+// order of each tile (Rasterizer.cpp:215-260) with no sort over all Gaussians, whose passes
+// are global round trips. One workgroup per tile, the longest lists first. Keys are taken
+// relative to the smallest key of the list, and only the 8-bit digits below the list's key
+// range are sorted (three passes when its depths lie within a factor of two).
+// A list of at most kTsCap entries stays in registers: each pass ranks the keys per wavefront
+// (ballot matching, stable in position order), scatters keys and positions to LDS and reads
+// them back in the new order. A longer list runs the same passes in chunks of kTsLongChunk through
+// global scratch (its histogram first, then the chunks in order, each advancing the digits'
+// output cursors).
+constexpr int kTsThreads = 512;
+constexpr int kTsWaves = kTsThreads / 64;
+constexpr int kTsItems = 22;
+constexpr int kTsCap = kTsThreads * kTsItems; // 11264: two workgroups' LDS fit in one CU
+constexpr int kTsLongItems = 8;                // keys per lane of a chunk of a longer list
+constexpr int kTsLongChunk = kTsThreads * kTsLongItems;
+
+struct TileSortLds {
+    uint32_t key[kTsCap];
+    uint16_t pos[kTsCap];
+    uint32_t waveCount[kTsWaves][256];
+    uint32_t digitRun[256];
+    uint32_t part[kTsWaves];
+    uint32_t lo[kTsWaves], hi[kTsWaves];
+};
+
+__device__ __forceinline__ int tsPosition(int wave, int ipw, int it, int lane)
+{
+    return (wave * ipw + it) * 64 + lane;
+}
+
+// Per-wavefront stable ranks of one chunk, whose position p = tsPosition(wave, ipw, it, lane)
+// (p < count) holds k[it]: L.waveCount[w][d] = keys of digit d in wavefront w, rank[it] = the
+// key's place among the keys of its digit in its wavefront. The lanes of an item that share a
+// digit find each other through a 64-bit mask per (wavefront, digit) in LDS: each lane ORs in its
+// bit, then every lane reads the mask back (a wavefront's LDS instructions complete in order),
+// and the lowest lane of the digit advances the count and clears the mask. The masks live in
+// L.key, which holds nothing while keys are ranked.
+template <int kItems>
+__device__ __forceinline__ void tsCount(const uint32_t (&k)[kItems], uint32_t (&rank)[kItems],
+                                        int ipw, int count, uint32_t base, int shift,
+                                        TileSortLds &L)
+{
+    static_assert(sizeof(TileSortLds::key) >= kTsWaves * 256 * sizeof(uint64_t), "digit masks");
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint64_t *masks = reinterpret_cast<uint64_t *>(L.key);
+    uint64_t *mask = masks + wave * 256;
+    __syncthreads(); // earlier readers of waveCount and of key are done
+    for (int j = tid; j < kTsWaves * 256; j += kTsThreads) {
+        (&L.waveCount[0][0])[j] = 0;
+        masks[j] = 0;
+    }
+    __syncthreads();
+    const uint64_t bit = uint64_t(1) << lane, below = bit - 1;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        rank[it] = 0;
+        const int p = tsPosition(wave, ipw, it, lane);
+        if (it < ipw && p < count) {
+            const uint32_t d = ((k[it] - base) >> shift) & 255u;
+            atomicOr(reinterpret_cast<unsigned long long *>(&mask[d]), (unsigned long long)bit);
+            const uint64_t m = mask[d];
+            const uint32_t prev = L.waveCount[wave][d];
+            rank[it] = prev + uint32_t(__popcll(m & below));
+            if ((m & below) == 0) {
+                L.waveCount[wave][d] = prev + uint32_t(__popcll(m));
+                mask[d] = 0;
+            }
+        }
+        // one item at a time (each item's LDS instructions after the previous item's)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+}
+
+// L.digitRun[d] = sum of v over the digits below d, where thread d < 256 holds v (all threads).
+__device__ __forceinline__ void tsScan(uint32_t v, TileSortLds &L)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (tid < 256 && lane == 63)
+        L.part[wave] = x;
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t before = x - v;
+        for (int w = 0; w < wave; ++w)
+            before += L.part[w];
+        L.digitRun[tid] = before;
+    }
+    __syncthreads();
+}
+
+// Turns the chunk's counts into output slots: L.waveCount[w][d] = the slot of wavefront w's
+// first key of digit d, from the digit's cursor L.digitRun[d], which moves past the chunk.
+__device__ __forceinline__ void tsBases(TileSortLds &L)
+{
+    const int tid = threadIdx.x;
+    if (tid < 256) {
+        uint32_t run = L.digitRun[tid];
+#pragma unroll
+        for (int w = 0; w < kTsWaves; ++w) {
+            const uint32_t c = L.waveCount[w][tid];
+            L.waveCount[w][tid] = run;
+            run += c;
+        }
+        L.digitRun[tid] = run;
+    }
+    __syncthreads();
+}
+
+// The smallest and largest of the workgroup's values, and the 8-bit passes that sort keys
+// relative to the smallest.
+__device__ __forceinline__ int tsRange(uint32_t lo, uint32_t hi, TileSortLds &L, uint32_t *base)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, uint32_t(__shfl_xor(int(lo), off, 64)));
+        hi = max(hi, uint32_t(__shfl_xor(int(hi), off, 64)));
+    }
+    if (lane == 0) {
+        L.lo[wave] = lo;
+        L.hi[wave] = hi;
+    }
+    __syncthreads();
+    lo = L.lo[0];
+    hi = L.hi[0];
+#pragma unroll
+    for (int w = 1; w < kTsWaves; ++w) {
+        lo = min(lo, L.lo[w]);
+        hi = max(hi, L.hi[w]);
+    }
+    *base = lo;
+    const uint32_t range = hi > lo ? hi - lo : 0u;
+    const int bits = range ? 32 - __clz(int(range)) : 0;
+    return (bits + 7) >> 3;
+}
+
+// Sorts a list of count <= kTsCap entries at listIn + off into listOut + off.
+__device__ __forceinline__ void tsSortShort(int off, int count, const uint32_t *__restrict__ listIn,
+                                            uint32_t *__restrict__ listOut,
+                                            const uint32_t *__restrict__ depthKey, TileSortLds &L)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ipw = (count + kTsThreads - 1) / kTsThreads;
+    uint32_t k[kTsItems], v[kTsItems];
+    uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+    for (int it = 0; it < kTsItems; ++it) {
+        k[it] = 0u;
+        v[it] = 0u;
+        const int p = tsPosition(wave, ipw, it, lane);
+        if (it < ipw && p < count) {
+            k[it] = depthKey[listIn[off + p]];
+            v[it] = uint32_t(p);
+            lo = min(lo, k[it]);
+            hi = max(hi, k[it]);
+        }
+    }
+    uint32_t base;
+    const int passes = tsRange(lo, hi, L, &base);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        uint32_t rank[kTsItems];
+        tsCount<kTsItems>(k, rank, ipw, count, base, shift, L);
+        uint32_t tot = 0;
+        if (tid < 256) {
+#pragma unroll
+            for (int w = 0; w < kTsWaves; ++w)
+                tot += L.waveCount[w][tid];
+        }
+        tsScan(tot, L);
+        tsBases(L);
+#pragma unroll
+        for (int it = 0; it < kTsItems; ++it) {
+            const int p = tsPosition(wave, ipw, it, lane);
+            if (it < ipw && p < count) {
+                const uint32_t d = ((k[it] - base) >> shift) & 255u;
+                const uint32_t slot = L.waveCount[wave][d] + rank[it];
+                L.key[slot] = k[it];
+                L.pos[slot] = uint16_t(v[it]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < kTsItems; ++it) {
+            const int p = tsPosition(wave, ipw, it, lane);
+            if (it < ipw && p < count) {
+                k[it] = L.key[p];
+                v[it] = L.pos[p];
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kTsItems; ++it) {
+        const int p = tsPosition(wave, ipw, it, lane);
+        if (it < ipw && p < count)
+            listOut[off + p] = listIn[off + int(v[it])];
+    }
+}
+
+// Sorts a list of any length at listIn + off into listOut + off, passing through (gk0, gv0)
+// and (gk1, gv1) at the same offsets.
+__device__ __forceinline__ void tsSortLong(int off, int count, const uint32_t *__restrict__ listIn,
+                                           uint32_t *__restrict__ listOut,
+                                           const uint32_t *__restrict__ depthKey, uint32_t *gk0,
+                                           uint32_t *gv0, uint32_t *gk1, uint32_t *gv1,
+                                           TileSortLds &L)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (int q = tid; q < count; q += kTsThreads) {
+        const uint32_t key = depthKey[listIn[off + q]];
+        lo = min(lo, key);
+        hi = max(hi, key);
+    }
+    uint32_t base;
+    const int passes = tsRange(lo, hi, L, &base);
+    if (passes == 0) {
+        for (int q = tid; q < count; q += kTsThreads)
+            listOut[off + q] = listIn[off + q];
+        return;
+    }
+    const uint32_t *kin = nullptr, *vin = nullptr;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        const bool last = pass == passes - 1;
+        uint32_t *kout = (pass & 1) ? gk1 : gk0, *vout = (pass & 1) ? gv1 : gv0;
+        // the digit histogram of the whole list (waveCount[0] as counters), then its scan
+        __syncthreads();
+        if (tid < 256)
+            L.waveCount[0][tid] = 0u;
+        __syncthreads();
+        for (int q = tid; q < count; q += kTsThreads) {
+            const uint32_t key = pass == 0 ? depthKey[listIn[off + q]] : kin[off + q];
+            atomicAdd(&L.waveCount[0][((key - base) >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        tsScan(tid < 256 ? L.waveCount[0][tid] : 0u, L);
+        for (int c0 = 0; c0 < count; c0 += kTsLongChunk) {
+            const int cnt = count - c0 < kTsLongChunk ? count - c0 : kTsLongChunk;
+            uint32_t k[kTsLongItems], v[kTsLongItems], rank[kTsLongItems];
+#pragma unroll
+            for (int it = 0; it < kTsLongItems; ++it) {
+                k[it] = 0u;
+                v[it] = 0u;
+                const int p = tsPosition(wave, kTsLongItems, it, lane);
+                if (p < cnt) {
+                    const int q = off + c0 + p;
+                    if (pass == 0) {
+                        v[it] = listIn[q];
+                        k[it] = depthKey[v[it]];
+                    } else {
+                        k[it] = kin[q];
+                        v[it] = vin[q];
+                    }
+                }
+            }
+            tsCount<kTsLongItems>(k, rank, kTsLongItems, cnt, base, shift, L);
+            tsBases(L);
+#pragma unroll
+            for (int it = 0; it < kTsLongItems; ++it) {
+                const int p = tsPosition(wave, kTsLongItems, it, lane);
+                if (p < cnt) {
+                    const uint32_t d = ((k[it] - base) >> shift) & 255u;
+                    const int slot = off + int(L.waveCount[wave][d] + rank[it]);
+                    if (last) {
+                        listOut[slot] = v[it];
+                    } else {
+                        kout[slot] = k[it];
+                        vout[slot] = v[it];
+                    }
+                }
+            }
+        }
+        kin = kout;
+        vin = vout;
+    }
+}
+
+// Workgroup b sorts the list of tile order[b] (every tile when order lists them all).
+__global__ __launch_bounds__(kTsThreads, 4) void tileSortKernel(
+    const int *__restrict__ tileOffsets, const int *__restrict__ order,
+    const uint32_t *__restrict__ listIn, uint32_t *__restrict__ listOut,
+    const uint32_t *__restrict__ depthKey, uint32_t *gk0, uint32_t *gv0, uint32_t *gk1,
+    uint32_t *gv1)
+{
+    __shared__ TileSortLds L;
+    const int t = order ? order[blockIdx.x] : int(blockIdx.x);
+    const int off = tileOffsets[t], count = tileOffsets[t + 1] - off;
+    if (count <= 0)
+        return;
+    if (count <= kTsCap)
+        tsSortShort(off, count, listIn, listOut, depthKey, L);
+    else
+        tsSortLong(off, count, listIn, listOut, depthKey, gk0, gv0, gk1, gv1, L);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1525,11 +1864,37 @@ size_t tileBinningCountsSize(int64_t n, int tilesX, int tilesY)
     return g > 0 ? size_t(blocksFor(n > 0 ? n : 1, g)) * size_t(tilesX) * size_t(tilesY) : 0;
 }
 
-int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
-                      const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
-                      uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
-                      uint32_t *tileList, size_t listCapacity, int *heavyFirst,
-                      const uint32_t *sortError, uint32_t *walkCost, const BinningList *list)
+int launchTileCounts(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
+                     const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+                     uint32_t *counts, size_t countsCap, uint32_t *tileTotal,
+                     const uint32_t *sortError, int64_t idLimit, const SideSum *side)
+{
+    const int group = tileBinningGroup(n, tilesX, tilesY);
+    U3D_CHECK(group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
+    if (n <= 0)
+        return U3D_OK;
+    const int tiles = tilesX * tilesY;
+    const int numGroups = blocksFor(n, group);
+    U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
+    const int cells = (tilesX + 1) * (tilesY + 1);
+    const SideSum sd = side ? *side : SideSum{};
+    hipLaunchKernelGGL(tileCountKernel,
+                       dim3(xcdGridSize(numGroups, kCountChunk) + (sd.in ? 1 : 0)),
+                       dim3(64 * kBinWaves), size_t(cells) * sizeof(uint32_t), s, n, sorted,
+                       nTiles, rect, ntS, rectS, tilesX, tilesY, group, numGroups, counts,
+                       sortError, sorted ? idLimit : n, sd);
+    U3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tileScanKernel, dim3((tiles + kScanTiles - 1) / kScanTiles),
+                       dim3(64 * kScanWaves), 0, s, counts, numGroups, tiles, tileTotal);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchTileEmit(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *ntS,
+                   const int4 *rectS, int tilesX, int tilesY, const uint32_t *counts,
+                   const uint32_t *tileTotal, int *tileOffsets, uint32_t *tileList,
+                   size_t listCapacity, int *heavyFirst, uint32_t *walkCost,
+                   const BinningList *list, int *longFirst)
 {
     const BinningList plain{n, 0, nullptr, nullptr, nullptr};
     const BinningList &bl = list ? *list : plain;
@@ -1539,15 +1904,7 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
     if (n <= 0)
         return launchFillI32(s, tileOffsets, int(bl.listBase), int64_t(tiles) + 1);
     const int numGroups = blocksFor(n, group);
-    U3D_CHECK(size_t(numGroups) * size_t(tiles) <= countsCap, U3D_ERR_CAPACITY, "tile binning counters too small");
     const int cells = (tilesX + 1) * (tilesY + 1);
-    hipLaunchKernelGGL(tileCountKernel, dim3(xcdGridSize(numGroups, kCountChunk)), dim3(64 * kBinWaves),
-                       size_t(cells) * sizeof(uint32_t), s, n, sorted, nTiles, rect, ntS, rectS,
-                       tilesX, tilesY, group, numGroups, counts, sortError, bl.idLimit);
-    U3D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tileScanKernel, dim3((tiles + kScanTiles - 1) / kScanTiles),
-                       dim3(64 * kScanWaves), 0, s, counts, numGroups, tiles, tileTotal);
-    U3D_LAUNCH_CHECK();
     // wavefronts per emission workgroup: as many slices as their grids fit in LDS
     int waves = kBinWaves;
     while (waves > 1 && waves * cells > kBinLdsCells)
@@ -1557,7 +1914,39 @@ int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const ui
                        tilesX, tilesY, group, numGroups, counts, tileTotal, tileOffsets,
                        heavyFirst, tileList,
                        uint32_t(listCapacity < size_t(INT32_MAX) ? listCapacity : size_t(INT32_MAX)),
-                       walkCost, bl.listBase, bl.listSum, bl.listMax, bl.overflow);
+                       walkCost, bl.listBase, bl.listSum, bl.listMax, bl.overflow, longFirst);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchTileBinning(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *nTiles,
+                      const int4 *rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+                      uint32_t *counts, size_t countsCap, uint32_t *tileTotal, int *tileOffsets,
+                      uint32_t *tileList, size_t listCapacity, int *heavyFirst,
+                      const uint32_t *sortError, uint32_t *walkCost, const BinningList *list)
+{
+    const int64_t idLimit = list ? list->idLimit : n;
+    U3D_TRY(launchTileCounts(s, n, sorted, nTiles, rect, ntS, rectS, tilesX, tilesY, counts,
+                             countsCap, tileTotal, sortError, idLimit));
+    return launchTileEmit(s, n, sorted, sorted ? ntS : nTiles, sorted ? rectS : rect, tilesX,
+                          tilesY, counts, tileTotal, tileOffsets, tileList, listCapacity,
+                          heavyFirst, walkCost, list, nullptr);
+}
+
+size_t tileSortScratchSize(size_t listCapacity) { return 4 * listCapacity; }
+
+int launchTileSort(hipStream_t s, int tiles, const int *tileOffsets, const int *order,
+                   const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,
+                   uint32_t *scratch, size_t scratchCap, size_t listCapacity)
+{
+    if (tiles <= 0)
+        return U3D_OK;
+    U3D_CHECK(scratchCap >= tileSortScratchSize(listCapacity), U3D_ERR_CAPACITY,
+              "tile sort scratch too small");
+    uint32_t *gk0 = scratch, *gv0 = scratch + listCapacity;
+    uint32_t *gk1 = scratch + 2 * listCapacity, *gv1 = scratch + 3 * listCapacity;
+    hipLaunchKernelGGL(tileSortKernel, dim3(tiles), dim3(kTsThreads), 0, s, tileOffsets, order,
+                       listIn, listOut, depthKey, gk0, gv0, gk1, gv1);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

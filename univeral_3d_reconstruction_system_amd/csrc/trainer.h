@@ -92,7 +92,7 @@ struct u3d_raster {
     u3d::DevBuf<float4> rec0, rec1;
     u3d::DevBuf<float> rec2, depth;
     u3d::DevBuf<int> radius;
-    u3d::DevBuf<int4> rect, rectSorted; // rectSorted: in depth order (tile binning)
+    u3d::DevBuf<int4> rect;
     u3d::DevBuf<uint8_t> flags;
     u3d::DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets;
     u3d::DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt;
@@ -101,11 +101,16 @@ struct u3d_raster {
     uint32_t sortEpoch = 0;            // small-sort calls so far (tags its lookback words)
     u3d::DevBuf<int> tileOffsets;
     const int *tileList = nullptr;
-    // tile binning (tiles <= kMaxBinTiles): final lists, per-(group, tile) counters, the pair
-    // total read back early through pinned memory
-    u3d::DevBuf<uint32_t> tileListBuf, binCounts, tileTotal, blockTiles;
-    u3d::DevBuf<int> heavyFirst;
-    u3d::PinnedScratch pinnedTotal; // (call number << 32 | pair count), written by the device
+    // tile binning (tile grids the binning supports): the final lists (sorted on depth),
+    // per-(group, tile) counters, the per-tile sort's scratch, the lists in index order, the tile
+    // orders
+    // (heavy-first walks, longest lists first)
+    u3d::DevBuf<uint32_t> tileListBuf, indexList, binCounts, tileTotal, tileSortScratch;
+    u3d::DevBuf<int> heavyFirst, longFirst;
+    // the pair count: the projection's per-block sums and the pinned host word
+    // (call number << 32 | pair count) their sum is published to
+    u3d::DevBuf<uint32_t> blockTiles;
+    u3d::PinnedScratch pinnedTotal;
     uint32_t totalSeq = 0;
     u3d::DevBuf<float> finalT, grad2d;
     bool grad2dZero = false; // grad2d holds zeros in all cap entries
