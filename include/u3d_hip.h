@@ -153,10 +153,11 @@ int u3d_memset(u3d_ctx *ctx, void *dst, int value, size_t bytes);
  * the next forward. */
 #define U3D_OPT_SEGMENTED_BACKWARD 1
 /* U3D_OPT_SORT_SPIN_LIMIT (default 0 = a bound no wait that ends comes near): how many lookback
- * re-reads a depth-sort pass makes before it gives up a wait and flags the sort as failed; a
- * negative value gives every wait up at once. The kernels stay in bounds either way (that
- * forward's tile lists come out empty), and the next forward or trainer step on that rasterizer
- * returns U3D_ERR_HIP. A test hook for that error path. */
+ * re-reads a pass of the depth sort over all Gaussians (used for tile grids beyond the direct
+ * binning's, (tiles_x + 1)(tiles_y + 1) > 12288) makes before it gives up a wait and flags the
+ * sort as failed; a negative value gives every wait up at once. The kernels stay in bounds
+ * either way, and that forward (or the trainer step that ran it) returns U3D_ERR_HIP. A test
+ * hook for that error path. */
 #define U3D_OPT_SORT_SPIN_LIMIT 2
 int u3d_ctx_set_option(u3d_ctx *ctx, int option, int value);
 
@@ -420,7 +421,8 @@ int u3d_scene_set_autotune(u3d_scene_set *set, int steps_per_candidate, int tune
 /* Pipeline stages timed with HIP events on the context's stream (no reference counterpart:
  * the reference has no profiling; these feed the roofline report of the benchmark). */
 #define U3D_STAGE_PROJECT 0    /* EWA projection + SH-DC colour */
-#define U3D_STAGE_SORT 1       /* depth sort, pair emission, tile sort, tile offsets */
+#define U3D_STAGE_SORT 1       /* tile binning and per-tile depth order (or depth sort, pair
+                                  emission, tile sort, tile offsets) */
 #define U3D_STAGE_RASTER_FWD 2 /* front-to-back compositing */
 #define U3D_STAGE_LOSS 3       /* fused L1 + D-SSIM value and gradient */
 #define U3D_STAGE_RASTER_BWD 4 /* reverse compositing -> per-Gaussian 2D gradients */
