@@ -277,10 +277,13 @@ def time_to_l1(ctx, init: dict, cams, images, extent, opt, target_l1: float, max
 
 
 def batched_options(plain_iterations: int, views_per_step: int, max_gaussians: int,
-                    views_factor: float = 2.0):
+                    views_factor: float = 1.5):
     """The plain schedule (TrainOptions::forIterations) counted in views: B views per step,
-    densify / reset steps divided by B, sqrt(B) rates and beta^B decay (batch_scaling), room
-    for up to views_factor x the plain trainer's views."""
+    densify / reset steps divided by B, sqrt(B) rates and beta^B decay (batch_scaling), and the
+    means rate decayed over views_factor x the plain trainer's views (also the most it runs).
+    Swept on two held-out high-preset scenes at B = 8: rates x B^0.75 or x B never reached the
+    plain trainer's final L1, x B^0.4 or x B^0.3 took 4-20% longer than x sqrt(B); a decay over
+    1.25 or 1.5 x the plain views reached it 5-8% sooner than over 2 x, over 1 x not at all."""
     B = views_per_step
     base = U.options_for_iterations(plain_iterations)
     o = U.options_for_iterations(max(1, int(plain_iterations * views_factor) // B))
@@ -343,6 +346,49 @@ def evaluate_scene(ctx, net, seed: int, n_gt: int, n_seed: int, views: int, widt
     return out
 
 
+def sweep_batched(ctx, net, seed: int, specs, plain_iterations: int = 7000, n_gt: int = 300_000,
+                  n_seed: int = 50_000, views: int = 200, width: int = 640, height: int = 480,
+                  log=print) -> dict:
+    """Wall time to the plain trainer's final L1 on one held-out scene for batched variants
+    spec = (init "knn" | "learned", views per step B, decay horizon views_factor, rate
+    exponent e): batched_options(plain_iterations, B, n_gt, views_factor) with the rates
+    multiplied by B^e (B^0.5 is the engine's own scaling)."""
+    sc = mesh_scene(seed, n_gt, n_seed, views, width, height)
+    images = render_targets(ctx, sc["gt"], sc["cams"])
+    cams, extent = sc["cams"], sc["extent"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    knn = U.learned_init(ctx, None, sc["xyz"], sc["rgb"], cams)
+    t_knn = time.perf_counter() - t0
+    learned = U.learned_init(ctx, net, sc["xyz"], sc["rgb"], cams) if net else None
+    opt = U.options_for_iterations(plain_iterations)
+    opt.max_gaussians = n_gt
+    t = U.Trainer(ctx, knn, cams, images, extent, opt)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    while not t.done():
+        t.step()
+    t.synchronize()
+    plain_s = time.perf_counter() - t0 + t_knn
+    target = t.evaluate_l1()
+    del t
+    out = dict(seed=seed, plain_seconds=round(plain_s, 3), target_l1=target, runs=[])
+    for init, B, vf, ex in specs:
+        o = batched_options(plain_iterations, B, n_gt, views_factor=vf)
+        m = float(B) ** ex / float(B) ** 0.5
+        for f in ("means", "scales", "quats", "opacities", "colors"):
+            setattr(o.lr, f, getattr(o.lr, f) * m)
+        model = learned if init == "learned" and learned is not None else knn
+        sec, steps, l1 = time_to_l1(ctx, model, cams, images, extent, o, target, o.iterations,
+                                    check_every=max(1, 100 // B), init_seconds=t_knn)
+        rec = dict(init=init, views_per_step=B, views_factor=vf, rate_exponent=ex,
+                   seconds=None if sec is None else round(sec, 3), steps=steps, views=steps * B,
+                   fraction_of_plain=None if sec is None else round(sec / plain_s, 4), final_l1=l1)
+        out["runs"].append(rec)
+        log(json.dumps(rec))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--train-scenes", type=int, default=48)
@@ -358,9 +404,26 @@ def main(argv=None):
     ap.add_argument("--skip-train", action="store_true", help="evaluate the saved weights")
     ap.add_argument("--out", default=None, help="write the evaluation JSON here")
     ap.add_argument("--weights", default=WEIGHTS, help="network weights file (.npz)")
+    ap.add_argument("--sweep", default=None, metavar="SPECS",
+                    help="only sweep batched options on held-out scenes: 'init,B,views_factor,"
+                         "rate_exponent;...' (init knn or learned)")
+    ap.add_argument("--sweep-seeds", type=int, nargs="+", default=[100_000])
     a = ap.parse_args(argv)
     ctx = U.Context(0)
     log = lambda m: print(m, flush=True)
+    if a.sweep:
+        specs = []
+        for item in a.sweep.split(";"):
+            init, B, vf, ex = item.split(",")
+            specs.append((init, int(B), float(vf), float(ex)))
+        net = load_weights(ctx, a.weights)
+        res = [sweep_batched(ctx, net, sd, specs, a.plain_iterations, a.eval_gt, a.eval_seeds,
+                             a.eval_views, a.eval_size[0], a.eval_size[1], log=log)
+               for sd in a.sweep_seeds]
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return 0
     if not a.skip_train:
         g = np.random.default_rng(7)
         feats, targs = [], []
