@@ -29,8 +29,7 @@
 namespace u3d {
 
 namespace {
-constexpr int kRing = 4;     // table uploads / status read-backs in flight
-constexpr int kBranches = 4; // items binned side by side (their launches are small)
+constexpr int kRing = 4; // table uploads / status read-backs in flight
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 int ceilDiv(int a, int b) { return (a + b - 1) / b; }
 } // namespace
@@ -52,14 +51,6 @@ public:
             (void)hipHostFree(pinned);
         if (statusHost)
             (void)hipHostFree(statusHost);
-        for (int b = 0; b < kBranches; ++b) {
-            if (branchDone[b])
-                (void)hipEventDestroy(branchDone[b]);
-            if (branch[b])
-                (void)hipStreamDestroy(branch[b]);
-        }
-        if (forkEvent)
-            (void)hipEventDestroy(forkEvent);
     }
 
     int init()
@@ -71,13 +62,6 @@ public:
             statusStep[k] = -1;
         }
         U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 4 * kRing));
-        // default-priority streams for the per-item binning branches (forked from and joined to
-        // the context's stream by events, so a captured step records them as graph branches)
-        for (int b = 0; b < kBranches; ++b) {
-            U3D_HIP(hipStreamCreateWithFlags(&branch[b], hipStreamNonBlocking));
-            U3D_HIP(hipEventCreateWithFlags(&branchDone[b], hipEventDisableTiming));
-        }
-        U3D_HIP(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming));
         for (int k = 0; k < 3; ++k) {
             occFwd[k] = rasterBatchedOccupancy(false, 64 << k);
             occBwd[k] = rasterBatchedOccupancy(true, 64 << k);
@@ -171,9 +155,10 @@ private:
         int tileBase, tilesX, tilesY;
     };
     std::vector<ItemGeometry> geo; // per item, fixed for the layout
-    bool binned = false;           // per-item difference-grid binning (else pairs + tile sort)
-    int64_t itemCap = 0;           // list slots per item (binned)
-    int itemBits = 1;
+    // binned: per-image difference-grid binning in index order and per-tile depth sorts (else
+    // a depth sort of all entries, pairs and their sort on the tile id)
+    bool binned = false;
+    int64_t itemCap = 0; // list slots per item (binned)
     u3d_raster_settings st{};
     int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
     int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
@@ -205,11 +190,15 @@ private:
     DevBuf<uint32_t> depthKey, nTiles, keyAlt, valA, valAlt, ntSorted, pairOffsets, total;
     DevBuf<uint32_t> pairTile, pairGauss, pairTileAlt, pairGaussAlt, hist, histScan, scanScratch;
     DevBuf<uint32_t> blockKey, blockOrder, blockKeyAlt, blockOrderAlt;
-    // binned path: group counts and tile totals of one item per branch
-    DevBuf<uint32_t> binCounts[kBranches], tileTotal[kBranches];
-    hipStream_t branch[kBranches] = {};
-    hipEvent_t branchDone[kBranches] = {}, forkEvent = nullptr;
-    DevBuf<int4> rectSorted;
+    // binned path: the images' binning table (host copy and device copy, whose list slices
+    // follow the per-item capacity), their (group, tile) counters, tile totals and longest-first
+    // orders, the lists in index order and the per-tile sort's scratch
+    std::vector<BinItem> binItems;
+    BinItemsGrid binGrid;
+    DevBuf<BinItem> binItemsDev;
+    DevBuf<uint32_t> binCounts, binTotals, indexList, sortScratch, sortMap;
+    PinnedScratch binStage; // the tables' upload staging (the stream is synchronised after)
+    DevBuf<int> longFirst;
     DevBuf<unsigned> counters; // persistent work counters (forward, backward)
     DevBuf<double> partials;
 
@@ -267,7 +256,6 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     U3D_CHECK(totalEntries < (int64_t(1) << 32) && totalTiles < (1 << 30), U3D_ERR_CAPACITY,
               "batch too large for 32-bit pair values");
     tileBits = bitsFor(totalTiles);
-    itemBits = bitsFor(numItems);
     offItems = 0;
     offLoss = align256(size_t(numItems) * sizeof(RenderItem));
     offSteps = offLoss + align256(size_t(numItems) * sizeof(LossItem));
@@ -307,16 +295,30 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     // binned: every item's offsets with their own end (totalTiles + numItems entries)
     U3D_TRY(tileOffsets.ensure(size_t(totalTiles) + size_t(numItems)));
     if (binned) {
-        size_t counts = 1, tiles = 1;
-        for (const ItemGeometry &g : geo) {
-            counts = std::max(counts, tileBinningCountsSize(g.n, g.tilesX, g.tilesY));
-            tiles = std::max(tiles, tileBinningTotalsSize(g.tilesX, g.tilesY));
+        binItems.assign(size_t(numItems), BinItem{});
+        for (int v = 0; v < numItems; ++v) {
+            const ItemGeometry &g = geo[size_t(v)];
+            BinItem &it = binItems[size_t(v)];
+            it.n = g.n;
+            it.idBase = uint32_t(g.screenBase);
+            it.tilesX = g.tilesX;
+            it.tilesY = g.tilesY;
+            it.offsetsBase = uint32_t(g.tileBase + v); // each image's offsets with their own end
         }
-        for (int b = 0; b < kBranches; ++b) {
-            U3D_TRY(binCounts[b].ensure(counts));
-            U3D_TRY(tileTotal[b].ensure(tiles));
+        std::vector<uint32_t> map;
+        U3D_TRY(binItemsLayout(binItems, &binGrid, &map));
+        U3D_TRY(binItemsDev.ensure(size_t(numItems)));
+        U3D_TRY(sortMap.ensure(std::max<size_t>(map.size(), 1)));
+        if (!map.empty()) {
+            const size_t bytes = map.size() * sizeof(uint32_t);
+            U3D_TRY(binStage.ensure(bytes));
+            std::memcpy(binStage.p, map.data(), bytes);
+            U3D_HIP(hipMemcpyAsync(sortMap.p, binStage.p, bytes, hipMemcpyHostToDevice, s));
+            U3D_HIP(hipStreamSynchronize(s));
         }
-        U3D_TRY(rectSorted.ensure(ne));
+        U3D_TRY(binCounts.ensure(std::max<size_t>(binGrid.counts, 1)));
+        U3D_TRY(binTotals.ensure(std::max<size_t>(binGrid.totals, 1)));
+        U3D_TRY(longFirst.ensure(std::max<size_t>(binGrid.orders, 1)));
     }
     const size_t nb = size_t(std::max<int64_t>(totalBlocks, 1));
     U3D_TRY(blockKey.ensure(nb));
@@ -499,6 +501,16 @@ int BatchEngine::probePairs()
         itemCap = cap;
         pairCap = cap * numItems;
         U3D_TRY(pairGauss.ensure(size_t(pairCap)));
+        U3D_TRY(indexList.ensure(size_t(pairCap)));
+        U3D_TRY(sortScratch.ensure(tileSortScratchSize(indexList.cap)));
+        // the images' list slices follow the capacity
+        for (int v = 0; v < numItems; ++v)
+            binItems[size_t(v)].listBase = uint32_t(int64_t(v) * itemCap);
+        const size_t bytes = binItems.size() * sizeof(BinItem);
+        U3D_TRY(binStage.ensure(bytes));
+        std::memcpy(binStage.p, binItems.data(), bytes);
+        U3D_HIP(hipMemcpyAsync(binItemsDev.p, binStage.p, bytes, hipMemcpyHostToDevice, s));
+        U3D_HIP(hipStreamSynchronize(s));
     } else {
         int64_t cap = int64_t(double(K) * margin) + 65536;
         cap = std::max(cap, grownCap);
@@ -519,8 +531,8 @@ int BatchEngine::probePairs()
                      radius.cap * 4, rect.cap * 16, flags.cap, depthKey.cap * 4, nTiles.cap * 4,
                      keyAlt.cap * 4, valA.cap * 4, valAlt.cap * 4, ntSorted.cap * 4,
                      pairOffsets.cap * 4, entryItem.cap * 4, pairTile.cap * 12, pairGauss.cap * 4,
-                     binCounts[0].cap * 4 * kBranches, tileTotal[0].cap * 4 * kBranches,
-                     rectSorted.cap * 16, hist.cap * 4,
+                     binCounts.cap * 4, binTotals.cap * 4, indexList.cap * 4,
+                     sortScratch.cap * 4, longFirst.cap * 4, sortMap.cap * 4, hist.cap * 4,
                      histScan.cap * 4, scanScratch.cap * 4, rgb.cap * 4, dl.cap * 4, finalT.cap * 4,
                      nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4})
         allocated += b;
@@ -547,58 +559,39 @@ int BatchEngine::enqueue()
         StageScope scope(ctx, U3D_STAGE_SORT);
         // K, overflow flag, longest item list set
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4));
-        U3D_TRY(launchIota(s, valA.p, totalEntries));
         RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap, tuning.sort_items};
         rs.radixBits = tuning.sort_radix ? tuning.sort_radix : 8;
-        bool alt = false;
-        U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs, &alt));
-        const uint32_t *sorted = alt ? valAlt.p : valA.p;
         if (binned) {
-            // stable pass on the item: each item's entries contiguous (at its screenBase, every
-            // entry of the item is in the sort), in depth order, invisible ones last
-            uint32_t *k0 = alt ? depthKey.p : keyAlt.p, *k1 = alt ? keyAlt.p : depthKey.p;
-            uint32_t *v1 = alt ? valAlt.p : valA.p, *v0 = alt ? valA.p : valAlt.p;
-            U3D_TRY(launchGatherU32(s, reinterpret_cast<const uint32_t *>(entryItem.p), v1, k0,
-                                    totalEntries, totalEntries));
-            bool alt4 = false;
-            U3D_TRY(radixSortPairs(s, k0, v1, k1, v0, totalEntries, itemBits, rs, &alt4));
-            const uint32_t *byItem = alt4 ? v0 : v1;
-            // items in kBranches branches: each item's three binning launches are small, and
-            // the items write disjoint parts of every buffer but the branch's own scratch
-            const int nb = std::min(kBranches, numItems);
-            U3D_HIP(hipEventRecord(forkEvent, s));
-            for (int b = 0; b < nb; ++b)
-                U3D_HIP(hipStreamWaitEvent(branch[b], forkEvent, 0));
-            for (int v = 0; v < numItems; ++v) {
-                const ItemGeometry &g = geo[size_t(v)];
-                const int b = v % nb;
-                const BinningList bl{totalEntries, uint32_t(int64_t(v) * itemCap), total.p, total.p + 2,
-                                     total.p + 1};
-                U3D_TRY(launchTileBinning(branch[b], g.n, byItem + g.screenBase, nTiles.p, rect.p,
-                                          ntSorted.p + g.screenBase, rectSorted.p + g.screenBase,
-                                          g.tilesX, g.tilesY, binCounts[b].p, binCounts[b].cap,
-                                          tileTotal[b].p, tileOffsets.p + g.tileBase + v, pairGauss.p,
-                                          size_t(itemCap), nullptr, nullptr, nullptr, &bl));
-            }
-            for (int b = 0; b < nb; ++b) {
-                U3D_HIP(hipEventRecord(branchDone[b], branch[b]));
-                U3D_HIP(hipStreamWaitEvent(s, branchDone[b], 0));
-            }
+            // every image's tiles binned in index order into its slice of the lists, then each
+            // list sorted on the depth keys: one launch per phase for all images
+            U3D_TRY(launchTileBinningItems(s, binItemsDev.p, numItems, binGrid, nTiles.p, rect.p,
+                                           binCounts.p, binTotals.p, tileOffsets.p, indexList.p,
+                                           size_t(itemCap), total.p, total.p + 2, total.p + 1,
+                                           longFirst.p));
+            U3D_TRY(launchTileSortItems(s, binItemsDev.p, sortMap.p, binGrid, tileOffsets.p,
+                                        longFirst.p, indexList.p, pairGauss.p, depthKey.p,
+                                        sortScratch.p, sortScratch.cap, indexList.cap));
             tileList = reinterpret_cast<const int *>(pairGauss.p);
-        }
-        if (!binned) {
-        U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries, totalEntries));
-        U3D_TRY(exclusiveScanU32(s, ntSorted.p, pairOffsets.p, totalEntries, total.p,
-                                 scanScratch.p, scanScratch.cap));
-        U3D_TRY(launchEmitPairsBatched(s, totalEntries, sorted, pairOffsets.p, rect.p, entryItem.p,
-                                       dItems, pairTile.p, pairGauss.p, ntSorted.p, pairCap,
-                                       total.p + 1));
-        bool alt2 = false;
-        U3D_TRY(radixSortPairs(s, pairTile.p, pairGauss.p, pairTileAlt.p, pairGaussAlt.p, pairCap,
-                               tileBits, rs, &alt2, total.p));
-        const uint32_t *tileSorted = alt2 ? pairTileAlt.p : pairTile.p;
-        tileList = reinterpret_cast<const int *>(alt2 ? pairGaussAlt.p : pairGauss.p);
-        U3D_TRY(launchTileOffsets(s, tileSorted, pairCap, totalTiles, tileOffsets.p, total.p));
+        } else {
+            // all entries in depth order (a radix sort of the depth keys), pairs emitted in that
+            // order and sorted on the global tile id
+            U3D_TRY(launchIota(s, valA.p, totalEntries));
+            bool alt = false;
+            U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs,
+                                   &alt));
+            const uint32_t *sorted = alt ? valAlt.p : valA.p;
+            U3D_TRY(launchGatherU32(s, nTiles.p, sorted, ntSorted.p, totalEntries, totalEntries));
+            U3D_TRY(exclusiveScanU32(s, ntSorted.p, pairOffsets.p, totalEntries, total.p,
+                                     scanScratch.p, scanScratch.cap));
+            U3D_TRY(launchEmitPairsBatched(s, totalEntries, sorted, pairOffsets.p, rect.p,
+                                           entryItem.p, dItems, pairTile.p, pairGauss.p,
+                                           ntSorted.p, pairCap, total.p + 1));
+            bool alt2 = false;
+            U3D_TRY(radixSortPairs(s, pairTile.p, pairGauss.p, pairTileAlt.p, pairGaussAlt.p,
+                                   pairCap, tileBits, rs, &alt2, total.p));
+            const uint32_t *tileSorted = alt2 ? pairTileAlt.p : pairTile.p;
+            tileList = reinterpret_cast<const int *>(alt2 ? pairGaussAlt.p : pairGauss.p);
+            U3D_TRY(launchTileOffsets(s, tileSorted, pairCap, totalTiles, tileOffsets.p, total.p));
         }
         if (tuning.heavy_first) {
             U3D_TRY(launchBlockCosts(s, dItems, numItems, totalBlocks, st.tile_size, tileOffsets.p,

@@ -372,6 +372,37 @@ size_t tileSortScratchSize(size_t listCapacity);
 int launchTileSort(hipStream_t s, int tiles, const int *tileOffsets, const int *order,
                    const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,
                    uint32_t *scratch, size_t scratchCap, size_t listCapacity);
+// Index-order binning and per-tile sort of several images in one launch per phase (the batched
+// engine). Image v bins entries [idBase, idBase + n) (its Gaussians; ids written are global);
+// its (group, tile) counters, tile and block totals, offsets (tiles + 1 entries, including
+// listBase), list slice and longest-first order start at the given offsets of shared buffers.
+// binItemsLayout fills the derived fields and returns the buffer sizes and grids.
+struct BinItem {
+    int64_t n;
+    uint32_t idBase;
+    int tilesX, tilesY, group, numGroups;
+    uint32_t countsBase, totalsBase, offsetsBase, listBase, orderBase;
+    int countBlock0, scanBlock0, emitBlock0, sortBlock0; // first workgroup in each launch
+};
+struct BinItemsGrid {
+    size_t counts = 0, totals = 0, orders = 0; // buffer entries
+    int countBlocks = 0, scanBlocks = 0, emitBlocks = 0, sortBlocks = 0;
+    int maxCells = 0, emitWaves = 0;
+};
+// items[v].n, idBase, tilesX, tilesY, offsetsBase and listBase are set by the caller. sortMap
+// (sortBlocks entries, item << 24 | rank) deals the images' tiles to the sort's workgroups in
+// rounds: round k takes the k-th longest list of every image that has one, so the longest lists
+// of all images start first.
+int binItemsLayout(std::vector<BinItem> &items, BinItemsGrid *grid, std::vector<uint32_t> *sortMap);
+int launchTileBinningItems(hipStream_t s, const BinItem *items, int numItems,
+                           const BinItemsGrid &grid, const uint32_t *nTiles, const int4 *rect,
+                           uint32_t *counts, uint32_t *tileTotal, int *tileOffsets,
+                           uint32_t *tileList, size_t capacity, uint32_t *listSum,
+                           uint32_t *listMax, uint32_t *overflow, int *longFirst);
+int launchTileSortItems(hipStream_t s, const BinItem *items, const uint32_t *sortMap,
+                        const BinItemsGrid &grid, const int *tileOffsets, const int *longFirst,
+                        const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,
+                        uint32_t *scratch, size_t scratchCap, size_t listCapacity);
 // Writes (tile id, Gaussian) pairs for Gaussians in depth order: sorted[i] is the Gaussian,
 // offsets[i] its first pair slot; slots at or past capacity are not written.
 int launchEmitPairs(hipStream_t s, int64_t n, const uint32_t *sorted, const uint32_t *offsets,

@@ -975,17 +975,14 @@ constexpr int kCountChunk = 32, kEmitChunk = 8;
 // contiguous row, and the emission reads it back as one).
 // Also writes the tile counts and rectangles in depth order (ntS, rectS) for the emission.
 // sorted == nullptr: the groups are Gaussians in index order, read from nTiles / rect directly.
-// side.in: workgroup 0 runs that sum instead (the groups start at workgroup 1).
-__global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
-    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
+// The counts of the groups that workgroup blk takes (its chunk of groups; workgroups without a
+// group return).
+__device__ __forceinline__ void tileCountBody(
+    int blk, int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
     const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
     int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError,
-    int64_t idLimit, SideSum side)
+    int64_t idLimit)
 {
-    if (side.in && blockIdx.x == 0) {
-        sumBlock(side.in, side.count, side.dev, side.host, side.seq);
-        return;
-    }
     extern __shared__ uint32_t D[];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
@@ -993,8 +990,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
         D[t] = 0;
     __syncthreads();
     const int slice = group / kBinWaves;
-    // the shift keeps each chunk of groups on one XCD (a rotation of the XCDs)
-    const int grp = xcdGroup<kCountChunk>(int(blockIdx.x) - (side.in ? 1 : 0), numGroups);
+    const int grp = xcdGroup<kCountChunk>(blk, numGroups);
     if (grp < 0)
         return;
     const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
@@ -1017,6 +1013,45 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
     }
 }
 
+// side.in: workgroup 0 runs that sum instead (the groups start at workgroup 1; the shift keeps
+// each chunk of groups on one XCD, a rotation of the XCDs).
+__global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
+    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ nTiles,
+    const int4 *__restrict__ rect, uint32_t *ntS, int4 *rectS, int tilesX, int tilesY,
+    int group, int numGroups, uint32_t *counts, const uint32_t *__restrict__ sortError,
+    int64_t idLimit, SideSum side)
+{
+    if (side.in && blockIdx.x == 0) {
+        sumBlock(side.in, side.count, side.dev, side.host, side.seq);
+        return;
+    }
+    tileCountBody(int(blockIdx.x) - (side.in ? 1 : 0), n, sorted, nTiles, rect, ntS, rectS,
+                  tilesX, tilesY, group, numGroups, counts, sortError, idLimit);
+}
+
+// The item of workgroup blk of a launch over several images: the last one whose first
+// workgroup (field) is at or before blk.
+template <int BinItem::*kFirst>
+__device__ __forceinline__ int binItemOf(const BinItem *__restrict__ items, int numItems, int blk)
+{
+    int v = 0;
+    while (v + 1 < numItems && items[v + 1].*kFirst <= blk)
+        ++v;
+    return v;
+}
+
+// Counts of every image of a batch in index order, one launch (items[v].countBlock0 first).
+__global__ __launch_bounds__(64 * kBinWaves) void tileCountItemsKernel(
+    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ nTiles,
+    const int4 *__restrict__ rect, uint32_t *counts)
+{
+    const int blk = int(blockIdx.x);
+    const BinItem it = items[binItemOf<&BinItem::countBlock0>(items, numItems, blk)];
+    tileCountBody(blk - it.countBlock0, it.n, nullptr, nTiles + it.idBase, rect + it.idBase,
+                  nullptr, nullptr, it.tilesX, it.tilesY, it.group, it.numGroups,
+                  counts + it.countsBase, nullptr, it.n);
+}
+
 // Scan of the group counts, one workgroup per block of kScanTiles consecutive tiles. Lane l of
 // wavefront w takes tile l % kScanTiles and the (2w + l / kScanTiles)-th of kScanSubs consecutive
 // ranges of groups, so each load and store of a wavefront is two contiguous runs of counts rows.
@@ -1025,14 +1060,14 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileCountKernel(
 // the block's own offset (the exclusive scan of the block totals, which the emission does) gives
 // the group's first slot in the tile's list. tileTotal[t] receives tile t's pair count,
 // tileTotal[tiles + b] block b's.
-__global__ __launch_bounds__(64 * kScanWaves) void tileScanKernel(uint32_t *counts, int numGroups,
-                                                                 int tiles, uint32_t *tileTotal)
+__device__ __forceinline__ void tileScanBody(int blk, uint32_t *counts, int numGroups, int tiles,
+                                             uint32_t *tileTotal)
 {
     static_assert(kScanTiles == 32, "two ranges of groups per wavefront");
     __shared__ uint32_t waveSum[kScanWaves][kScanTiles];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & (kScanTiles - 1), half = lane >> 5;
-    const int t = int(blockIdx.x) * kScanTiles + col;
+    const int t = int(blk) * kScanTiles + col;
     const bool valid = t < tiles;
     const int per = (numGroups + kScanSubs - 1) / kScanSubs;
     const int sub = wave * 2 + half;
@@ -1077,8 +1112,23 @@ __global__ __launch_bounds__(64 * kScanWaves) void tileScanKernel(uint32_t *coun
         if (valid)
             tileTotal[t] = total;
         if (col == kScanTiles - 1)
-            tileTotal[size_t(tiles) + blockIdx.x] = x;
+            tileTotal[size_t(tiles) + blk] = x;
     }
+}
+
+__global__ __launch_bounds__(64 * kScanWaves) void tileScanKernel(uint32_t *counts, int numGroups,
+                                                                 int tiles, uint32_t *tileTotal)
+{
+    tileScanBody(int(blockIdx.x), counts, numGroups, tiles, tileTotal);
+}
+
+__global__ __launch_bounds__(64 * kScanWaves) void tileScanItemsKernel(
+    const BinItem *__restrict__ items, int numItems, uint32_t *counts, uint32_t *tileTotal)
+{
+    const int blk = int(blockIdx.x);
+    const BinItem it = items[binItemOf<&BinItem::scanBlock0>(items, numItems, blk)];
+    tileScanBody(blk - it.scanBlock0, counts + it.countsBase, it.numGroups,
+                 it.tilesX * it.tilesY, tileTotal + it.totalsBase);
 }
 
 // The tiles ordered by decreasing cost (64 log-scale classes, any order inside a class), so
@@ -1135,11 +1185,12 @@ __device__ __forceinline__ void writeHeavyFirst(const uint32_t *__restrict__ til
 // Inside one Gaussian the tiles are distinct, so its lanes take their slots with one LDS atomic
 // add; the Gaussians of a step take theirs in separate rounds in list order, so equal tiles keep
 // the order of the positions.
+// (idBase: added to the index-order positions, the ids of a batch's image.)
 __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict__ sorted,
                                           const uint32_t *__restrict__ ntS,
                                           const int4 *__restrict__ rectS, int64_t first,
                                           int count, int stride, BinLds &L, uint32_t *mine,
-                                          uint32_t *tileList, uint32_t capacity)
+                                          uint32_t *tileList, uint32_t capacity, uint32_t idBase)
 {
     const int lane = threadIdx.x & 63;
     for (int sub = 0; sub < count; sub += 64) {
@@ -1147,7 +1198,7 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
         uint32_t g = 0, nt = 0;
         int4 r = make_int4(0, 0, 1, 1);
         if (i < n) {
-            g = sorted ? sorted[i] : uint32_t(i);
+            g = sorted ? sorted[i] : idBase + uint32_t(i);
             nt = ntS[i];
             if (nt)
                 r = rectS[i];
@@ -1228,12 +1279,13 @@ __device__ __forceinline__ void emitSlice(int64_t n, const uint32_t *__restrict_
 // slots and offsets past `capacity` are clamped.
 // sorted == nullptr: positions are Gaussians in index order (ntS / rectS by Gaussian), and each
 // tile's list comes out in index order.
-__global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
-    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
+__device__ __forceinline__ void tileEmitBody(
+    int blk, int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
     const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
     int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
     int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost,
-    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst)
+    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst,
+    uint32_t idBase)
 {
     extern __shared__ uint32_t cur[]; // [W][cells]
     __shared__ BinLds L[kBinWaves];
@@ -1245,7 +1297,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     const int stride = tilesX + 1, cells = stride * (tilesY + 1);
     const int tiles = tilesX * tilesY;
     const int slice = group / W;
-    const int grp = xcdGroup<kEmitChunk>(int(blockIdx.x), numGroups);
+    const int grp = xcdGroup<kEmitChunk>(blk, numGroups);
     if (grp < 0)
         return;
     const int64_t first = int64_t(grp) * group + int64_t(wave) * slice;
@@ -1339,7 +1391,37 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
     }
     __syncthreads(); // every cursor is set before the wavefronts emit
     emitSlice(n, sorted, ntS, rectS, first, slice, stride, L[wave], mine, tileList + listBase,
-              capacity);
+              capacity, idBase);
+}
+
+__global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
+    int64_t n, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ ntS,
+    const int4 *__restrict__ rectS, int tilesX, int tilesY, int group,
+    int numGroups, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal,
+    int *tileOffsets, int *heavyFirst, uint32_t *tileList, uint32_t capacity, uint32_t *walkCost,
+    uint32_t listBase, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst)
+{
+    tileEmitBody(int(blockIdx.x), n, sorted, ntS, rectS, tilesX, tilesY, group, numGroups, counts,
+                 tileTotal, tileOffsets, heavyFirst, tileList, capacity, walkCost, listBase,
+                 listSum, listMax, overflow, longFirst, 0u);
+}
+
+// Emission of every image of a batch in index order, one launch: image v's ids (its entries
+// plus items[v].idBase) go to its slice of the lists (items[v].listBase, capacity slots), its
+// offsets and longest-first order to its places; listSum / listMax / overflow as for one image.
+__global__ __launch_bounds__(64 * kBinWaves) void tileEmitItemsKernel(
+    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ nTiles,
+    const int4 *__restrict__ rect, const uint32_t *__restrict__ counts,
+    const uint32_t *__restrict__ tileTotal, int *tileOffsets, uint32_t *tileList,
+    uint32_t capacity, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst)
+{
+    const int blk = int(blockIdx.x);
+    const BinItem it = items[binItemOf<&BinItem::emitBlock0>(items, numItems, blk)];
+    tileEmitBody(blk - it.emitBlock0, it.n, nullptr, nTiles + it.idBase, rect + it.idBase,
+                 it.tilesX, it.tilesY, it.group, it.numGroups, counts + it.countsBase,
+                 tileTotal + it.totalsBase, tileOffsets + it.offsetsBase, nullptr, tileList,
+                 capacity, nullptr, it.listBase, listSum, listMax, overflow,
+                 longFirst + it.orderBase, it.idBase);
 }
 
 // ------------------------------------------------------------------ per-tile depth order
@@ -1631,6 +1713,43 @@ __device__ __forceinline__ void tsSortLong(int off, int count, const uint32_t *_
     }
 }
 
+// The workgroup sorts the list of tile order[blk] (tile blk without an order).
+__device__ __forceinline__ void tileSortBody(int blk, const int *__restrict__ tileOffsets,
+                                             const int *__restrict__ order,
+                                             const uint32_t *__restrict__ listIn,
+                                             uint32_t *__restrict__ listOut,
+                                             const uint32_t *__restrict__ depthKey, uint32_t *gk0,
+                                             uint32_t *gv0, uint32_t *gk1, uint32_t *gv1)
+{
+    __shared__ TileSortLds L;
+    const int t = order ? order[blk] : blk;
+    const int off = tileOffsets[t], count = tileOffsets[t + 1] - off;
+    if (count <= 0)
+        return;
+    if (count <= 64) {
+        // one wavefront, no barrier: lane i's key is ranked by counting the smaller keys (and
+        // the equal keys of lower lanes) among the list's
+        if (threadIdx.x >= 64)
+            return;
+        const int lane = int(threadIdx.x);
+        const bool valid = lane < count;
+        const uint32_t id = listIn[off + (valid ? lane : 0)];
+        const uint32_t key = depthKey[id];
+        uint32_t rank = 0;
+        for (int j = 0; j < count; ++j) {
+            const uint32_t kj = uint32_t(__builtin_amdgcn_readlane(int(key), j));
+            rank += (kj < key) | ((kj == key) & (j < lane)) ? 1u : 0u;
+        }
+        if (valid)
+            listOut[off + int(rank)] = id;
+        return;
+    }
+    if (count <= kTsCap)
+        tsSortShort(off, count, listIn, listOut, depthKey, L);
+    else
+        tsSortLong(off, count, listIn, listOut, depthKey, gk0, gv0, gk1, gv1, L);
+}
+
 // Workgroup b sorts the list of tile order[b] (every tile when order lists them all).
 __global__ __launch_bounds__(kTsThreads, 4) void tileSortKernel(
     const int *__restrict__ tileOffsets, const int *__restrict__ order,
@@ -1638,15 +1757,23 @@ __global__ __launch_bounds__(kTsThreads, 4) void tileSortKernel(
     const uint32_t *__restrict__ depthKey, uint32_t *gk0, uint32_t *gv0, uint32_t *gk1,
     uint32_t *gv1)
 {
-    __shared__ TileSortLds L;
-    const int t = order ? order[blockIdx.x] : int(blockIdx.x);
-    const int off = tileOffsets[t], count = tileOffsets[t + 1] - off;
-    if (count <= 0)
-        return;
-    if (count <= kTsCap)
-        tsSortShort(off, count, listIn, listOut, depthKey, L);
-    else
-        tsSortLong(off, count, listIn, listOut, depthKey, gk0, gv0, gk1, gv1, L);
+    tileSortBody(int(blockIdx.x), tileOffsets, order, listIn, listOut, depthKey, gk0, gv0, gk1,
+                 gv1);
+}
+
+// The tiles of every image of a batch: workgroup b sorts rank sortMap[b] & 0xffffff of image
+// sortMap[b] >> 24 (its longest-first order).
+__global__ __launch_bounds__(kTsThreads, 4) void tileSortItemsKernel(
+    const BinItem *__restrict__ items, const uint32_t *__restrict__ sortMap,
+    const int *__restrict__ tileOffsets, const int *__restrict__ longFirst,
+    const uint32_t *__restrict__ listIn, uint32_t *__restrict__ listOut,
+    const uint32_t *__restrict__ depthKey, uint32_t *gk0, uint32_t *gv0, uint32_t *gk1,
+    uint32_t *gv1)
+{
+    const uint32_t m = sortMap[blockIdx.x];
+    const BinItem &it = items[m >> 24];
+    tileSortBody(int(m & 0xffffffu), tileOffsets + it.offsetsBase, longFirst + it.orderBase,
+                 listIn, listOut, depthKey, gk0, gv0, gk1, gv1);
 }
 
 inline int blocksFor(int64_t n, int per) { return int((n + per - 1) / per); }
@@ -1947,6 +2074,100 @@ int launchTileSort(hipStream_t s, int tiles, const int *tileOffsets, const int *
     uint32_t *gk1 = scratch + 2 * listCapacity, *gv1 = scratch + 3 * listCapacity;
     hipLaunchKernelGGL(tileSortKernel, dim3(tiles), dim3(kTsThreads), 0, s, tileOffsets, order,
                        listIn, listOut, depthKey, gk0, gv0, gk1, gv1);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int binItemsLayout(std::vector<BinItem> &items, BinItemsGrid *grid, std::vector<uint32_t> *sortMap)
+{
+    U3D_CHECK(items.size() <= 256, U3D_ERR_CAPACITY, "at most 256 images per binning launch");
+    BinItemsGrid g;
+    g.emitWaves = kBinWaves;
+    int mostTiles = 0;
+    for (BinItem &it : items) {
+        const int tiles = it.tilesX * it.tilesY;
+        it.group = tileBinningGroup(it.n > 0 ? it.n : 1, it.tilesX, it.tilesY);
+        U3D_CHECK(it.group > 0, U3D_ERR_UNSUPPORTED, "tile binning needs (tilesX + 1) (tilesY + 1) <= " + std::to_string(kBinLdsCells));
+        it.numGroups = blocksFor(it.n > 0 ? it.n : 1, it.group); // group 0 writes the offsets
+        it.countsBase = uint32_t(g.counts);
+        it.totalsBase = uint32_t(g.totals);
+        it.orderBase = uint32_t(g.orders);
+        it.countBlock0 = g.countBlocks;
+        it.scanBlock0 = g.scanBlocks;
+        it.emitBlock0 = g.emitBlocks;
+        it.sortBlock0 = g.sortBlocks;
+        g.counts += size_t(it.numGroups) * size_t(tiles);
+        g.totals += tileBinningTotalsSize(it.tilesX, it.tilesY);
+        g.orders += size_t(tiles);
+        g.countBlocks += xcdGridSize(it.numGroups, kCountChunk);
+        g.scanBlocks += (tiles + kScanTiles - 1) / kScanTiles;
+        g.emitBlocks += xcdGridSize(it.numGroups, kEmitChunk);
+        g.sortBlocks += tiles;
+        const int cells = (it.tilesX + 1) * (it.tilesY + 1);
+        g.maxCells = std::max(g.maxCells, cells);
+        int waves = kBinWaves;
+        while (waves > 1 && waves * cells > kBinLdsCells)
+            waves >>= 1;
+        g.emitWaves = std::min(g.emitWaves, waves);
+        mostTiles = std::max(mostTiles, tiles);
+    }
+    U3D_CHECK(mostTiles < (1 << 24), U3D_ERR_CAPACITY, "too many tiles in one image");
+    sortMap->clear();
+    sortMap->reserve(size_t(g.sortBlocks));
+    for (int k = 0; k < mostTiles; ++k)
+        for (size_t v = 0; v < items.size(); ++v)
+            if (k < items[v].tilesX * items[v].tilesY)
+                sortMap->push_back(uint32_t(v << 24) | uint32_t(k));
+    U3D_CHECK(g.counts < (size_t(1) << 32) && g.totals < (size_t(1) << 32), U3D_ERR_CAPACITY,
+              "binning counters of the batch exceed 2^32 entries");
+    *grid = g;
+    return U3D_OK;
+}
+
+int launchTileBinningItems(hipStream_t s, const BinItem *items, int numItems,
+                           const BinItemsGrid &grid, const uint32_t *nTiles, const int4 *rect,
+                           uint32_t *counts, uint32_t *tileTotal, int *tileOffsets,
+                           uint32_t *tileList, size_t capacity, uint32_t *listSum,
+                           uint32_t *listMax, uint32_t *overflow, int *longFirst)
+{
+    if (numItems <= 0)
+        return U3D_OK;
+    if (grid.countBlocks > 0) {
+        hipLaunchKernelGGL(tileCountItemsKernel, dim3(grid.countBlocks), dim3(64 * kBinWaves),
+                           size_t(grid.maxCells) * sizeof(uint32_t), s, items, numItems, nTiles,
+                           rect, counts);
+        U3D_LAUNCH_CHECK();
+    }
+    if (grid.scanBlocks > 0) {
+        hipLaunchKernelGGL(tileScanItemsKernel, dim3(grid.scanBlocks), dim3(64 * kScanWaves), 0, s,
+                           items, numItems, counts, tileTotal);
+        U3D_LAUNCH_CHECK();
+    }
+    if (grid.emitBlocks > 0) {
+        hipLaunchKernelGGL(tileEmitItemsKernel, dim3(grid.emitBlocks), dim3(64 * grid.emitWaves),
+                           size_t(grid.emitWaves) * size_t(grid.maxCells) * sizeof(uint32_t), s,
+                           items, numItems, nTiles, rect, counts, tileTotal, tileOffsets, tileList,
+                           uint32_t(capacity < size_t(INT32_MAX) ? capacity : size_t(INT32_MAX)),
+                           listSum, listMax, overflow, longFirst);
+        U3D_LAUNCH_CHECK();
+    }
+    return U3D_OK;
+}
+
+int launchTileSortItems(hipStream_t s, const BinItem *items, const uint32_t *sortMap,
+                        const BinItemsGrid &grid, const int *tileOffsets, const int *longFirst,
+                        const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,
+                        uint32_t *scratch, size_t scratchCap, size_t listCapacity)
+{
+    if (grid.sortBlocks <= 0)
+        return U3D_OK;
+    U3D_CHECK(scratchCap >= tileSortScratchSize(listCapacity), U3D_ERR_CAPACITY,
+              "tile sort scratch too small");
+    uint32_t *gk0 = scratch, *gv0 = scratch + listCapacity;
+    uint32_t *gk1 = scratch + 2 * listCapacity, *gv1 = scratch + 3 * listCapacity;
+    hipLaunchKernelGGL(tileSortItemsKernel, dim3(grid.sortBlocks), dim3(kTsThreads), 0, s, items,
+                       sortMap, tileOffsets, longFirst, listIn, listOut, depthKey, gk0, gv0, gk1,
+                       gv1);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
