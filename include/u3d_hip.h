@@ -400,9 +400,12 @@ typedef struct {
     int atomic_lanes;       /* backward: Gaussians reaching at most this many lanes of a wave
                              * (0, 4, 16) add per lane to LDS instead of a wave reduction */
     int sort_radix;         /* digit bits per radix pass of the step's sorts: 4..8 (0: 8) */
-    int pair_sort;          /* 1: tile lists from emitted (tile, Gaussian) pairs radix-sorted on
-                             * the tile id; 0: per-image difference grids where every image's
-                             * tile grid fits the binning's LDS grid */
+    int pair_sort;          /* how tile lists are built: 0: per-image difference grids in
+                             * Gaussian index order, then every list sorted on the depth keys;
+                             * 2: a radix sort of every entry's depth key, then per-image
+                             * difference grids in that order; (0 and 2 where every image's tile
+                             * grid fits the binning's LDS grid) 1: emitted (tile, Gaussian) pairs
+                             * radix-sorted on the tile id */
 } u3d_batch_tuning;
 int u3d_trainer_get_tuning(u3d_trainer *t, u3d_batch_tuning *out);
 int u3d_trainer_set_tuning(u3d_trainer *t, const u3d_batch_tuning *tuning);

@@ -219,10 +219,12 @@ def test_batched_training_with_densification_converges(ctx):
 
 
 @pytest.mark.parametrize("lds,heavy,per_cu,items,few,radix,pairs", [
-    (64, 1, 0, 0, 4, 4, 1), (128, 0, 2, 4, 16, 6, 0), (256, 1, 1, 8, 0, 8, 1), (256, 1, 0, 0, 0, 5, 0)])
+    (64, 1, 0, 0, 4, 4, 1), (128, 0, 2, 4, 16, 6, 0), (256, 1, 1, 8, 0, 8, 1), (256, 1, 0, 0, 0, 5, 0),
+    (128, 1, 0, 0, 0, 8, 2), (64, 0, 1, 4, 4, 6, 2)])
 def test_launch_choices_do_not_change_results(ctx, lds, heavy, per_cu, items, few, radix, pairs):
     """LDS batch, block order, workgroups per CU, radix items per lane, radix digit width and
-    how the tile lists are built (difference grids per image or a sort of emitted pairs): same
+    how the tile lists are built (difference grids per image in index order with per-tile depth
+    sorts, a sort of emitted pairs, or difference grids in the order of a depth sort): same
     forward (first loss bit-identical), same gradient (1e-4)."""
     init, cams, images = _scene(3000, 4, 160, 120)
     t = U.Trainer(ctx, init, cams, images, S.scene_extent(cams), _frozen_options(4, True))
@@ -249,7 +251,7 @@ def test_autotuner_picks_a_configuration_and_training_continues(ctx):
     assert chosen["lds_batch"] in (64, 128, 256) and chosen["tile_size"] == 16
     assert t.tuning() == chosen
     it = t.iteration()
-    assert it == (24 + 6) * 3  # 24 + 6 candidates x (1 warm-up + 2 timed) steps
+    assert it == (24 + 9) * 3  # 24 + 9 candidates x (1 warm-up + 2 timed) steps
     for _ in range(5):
         t.step()
     assert t.iteration() == it + 5 and t.counters()["overflow_steps"] == 0

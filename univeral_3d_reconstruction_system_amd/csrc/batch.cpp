@@ -82,7 +82,7 @@ public:
                   U3D_ERR_INVALID, "atomic_lanes must be 0, 4 or 16");
         U3D_CHECK(t.sort_radix == 0 || (t.sort_radix >= 4 && t.sort_radix <= 8), U3D_ERR_INVALID,
                   "sort_radix must be 0 or 4..8");
-        U3D_CHECK(t.pair_sort == 0 || t.pair_sort == 1, U3D_ERR_INVALID, "pair_sort must be 0 or 1");
+        U3D_CHECK(t.pair_sort >= 0 && t.pair_sort <= 2, U3D_ERR_INVALID, "pair_sort must be 0, 1 or 2");
         U3D_HIP(hipStreamSynchronize(ctx->stream));
         tuning.heavy_first = t.heavy_first != 0;
         tuning.workgroups_per_cu = t.workgroups_per_cu;
@@ -91,7 +91,7 @@ public:
         tuning.tile_size = t.tile_size;
         tuning.atomic_lanes = t.atomic_lanes;
         tuning.sort_radix = t.sort_radix;
-        if ((t.pair_sort != 0) != (tuning.pair_sort != 0))
+        if (t.pair_sort != tuning.pair_sort)
             needRelayout = true; // other buffers and list layout
         tuning.pair_sort = t.pair_sort;
         dropGraph();
@@ -155,10 +155,13 @@ private:
         int tileBase, tilesX, tilesY;
     };
     std::vector<ItemGeometry> geo; // per item, fixed for the layout
-    // binned: per-image difference-grid binning in index order and per-tile depth sorts (else
-    // a depth sort of all entries, pairs and their sort on the tile id)
+    // binned: per-image difference-grid binning, in index order followed by per-tile depth sorts
+    // (pair_sort 0) or in the depth order of a sort of all entries (pair_sort 2); else a depth
+    // sort of all entries, pairs and their sort on the tile id
     bool binned = false;
+    bool depthBinned = false;
     int64_t itemCap = 0; // list slots per item (binned)
+    int itemBits = 1;
     u3d_raster_settings st{};
     int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
     int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
@@ -198,6 +201,7 @@ private:
     DevBuf<BinItem> binItemsDev;
     DevBuf<uint32_t> binCounts, binTotals, indexList, sortScratch, sortMap;
     PinnedScratch binStage; // the tables' upload staging (the stream is synchronised after)
+    DevBuf<int4> rectSorted; // depth-order binning: the rectangles in depth order
     DevBuf<int> longFirst;
     DevBuf<unsigned> counters; // persistent work counters (forward, backward)
     DevBuf<double> partials;
@@ -224,6 +228,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     partialsLen = 0;
     geo.clear();
     binned = true;
+    depthBinned = false;
     const int ts = st.tile_size;
     for (u3d_trainer *t : act) {
         U3D_CHECK(std::memcmp(&t->opt.raster, &st, sizeof st) == 0, U3D_ERR_UNSUPPORTED,
@@ -243,7 +248,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         for (int k = 0; k < B; ++k)
             geo.push_back({n, totalEntries + n * k, totalTiles + ceilDiv(W, ts) * ceilDiv(H, ts) * k,
                            ceilDiv(W, ts), ceilDiv(H, ts)});
-        binned = binned && tuning.pair_sort == 0 && tileBinningGroup(n, ceilDiv(W, ts), ceilDiv(H, ts)) > 0;
+        binned = binned && tuning.pair_sort != 1 && tileBinningGroup(n, ceilDiv(W, ts), ceilDiv(H, ts)) > 0;
         numItems += B;
         totalEntries += n * B;
         maxN = std::max(maxN, n);
@@ -256,6 +261,8 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     U3D_CHECK(totalEntries < (int64_t(1) << 32) && totalTiles < (1 << 30), U3D_ERR_CAPACITY,
               "batch too large for 32-bit pair values");
     tileBits = bitsFor(totalTiles);
+    itemBits = bitsFor(numItems);
+    depthBinned = binned && tuning.pair_sort == 2;
     offItems = 0;
     offLoss = align256(size_t(numItems) * sizeof(RenderItem));
     offSteps = offLoss + align256(size_t(numItems) * sizeof(LossItem));
@@ -319,6 +326,8 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         U3D_TRY(binCounts.ensure(std::max<size_t>(binGrid.counts, 1)));
         U3D_TRY(binTotals.ensure(std::max<size_t>(binGrid.totals, 1)));
         U3D_TRY(longFirst.ensure(std::max<size_t>(binGrid.orders, 1)));
+        if (depthBinned)
+            U3D_TRY(rectSorted.ensure(ne));
     }
     const size_t nb = size_t(std::max<int64_t>(totalBlocks, 1));
     U3D_TRY(blockKey.ensure(nb));
@@ -532,7 +541,8 @@ int BatchEngine::probePairs()
                      keyAlt.cap * 4, valA.cap * 4, valAlt.cap * 4, ntSorted.cap * 4,
                      pairOffsets.cap * 4, entryItem.cap * 4, pairTile.cap * 12, pairGauss.cap * 4,
                      binCounts.cap * 4, binTotals.cap * 4, indexList.cap * 4,
-                     sortScratch.cap * 4, longFirst.cap * 4, sortMap.cap * 4, hist.cap * 4,
+                     sortScratch.cap * 4, longFirst.cap * 4, sortMap.cap * 4, rectSorted.cap * 16,
+                     hist.cap * 4,
                      histScan.cap * 4, scanScratch.cap * 4, rgb.cap * 4, dl.cap * 4, finalT.cap * 4,
                      nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4})
         allocated += b;
@@ -561,7 +571,28 @@ int BatchEngine::enqueue()
         U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4));
         RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap, tuning.sort_items};
         rs.radixBits = tuning.sort_radix ? tuning.sort_radix : 8;
-        if (binned) {
+        if (depthBinned) {
+            // all entries in depth order, then a stable pass on the item (each item's entries
+            // contiguous at its screen base, in depth order, invisible ones last); every image's
+            // tiles binned in that order straight into its slice of the lists
+            U3D_TRY(launchIota(s, valA.p, totalEntries));
+            bool alt = false;
+            U3D_TRY(radixSortPairs(s, depthKey.p, valA.p, keyAlt.p, valAlt.p, totalEntries, 32, rs,
+                                   &alt));
+            uint32_t *k0 = alt ? depthKey.p : keyAlt.p, *k1 = alt ? keyAlt.p : depthKey.p;
+            uint32_t *v1 = alt ? valAlt.p : valA.p, *v0 = alt ? valA.p : valAlt.p;
+            U3D_TRY(launchGatherU32(s, reinterpret_cast<const uint32_t *>(entryItem.p), v1, k0,
+                                    totalEntries, totalEntries));
+            bool alt4 = false;
+            U3D_TRY(radixSortPairs(s, k0, v1, k1, v0, totalEntries, itemBits, rs, &alt4));
+            const uint32_t *byItem = alt4 ? v0 : v1;
+            U3D_TRY(launchTileBinningItems(s, binItemsDev.p, numItems, binGrid, nTiles.p, rect.p,
+                                           binCounts.p, binTotals.p, tileOffsets.p, pairGauss.p,
+                                           size_t(itemCap), total.p, total.p + 2, total.p + 1,
+                                           nullptr, byItem, ntSorted.p, rectSorted.p,
+                                           totalEntries));
+            tileList = reinterpret_cast<const int *>(pairGauss.p);
+        } else if (binned) {
             // every image's tiles binned in index order into its slice of the lists, then each
             // list sorted on the depth keys: one launch per phase for all images
             U3D_TRY(launchTileBinningItems(s, binItemsDev.p, numItems, binGrid, nTiles.p, rect.p,
@@ -809,7 +840,7 @@ int BatchEngine::autotune(u3d_trainer *const *scenes, int numScenes, int stepsPe
         if (ci + 1 == stage1) {
             // stage 2, from the best of stage 1: the sort radix and how the tile lists are built
             for (int radix : {8, 6, 4})
-                for (int pairs : {0, 1}) {
+                for (int pairs : {0, 1, 2}) {
                     u3d_batch_tuning next = best;
                     next.sort_radix = radix;
                     next.pair_sort = pairs;

@@ -394,11 +394,16 @@ struct BinItemsGrid {
 // rounds: round k takes the k-th longest list of every image that has one, so the longest lists
 // of all images start first.
 int binItemsLayout(std::vector<BinItem> &items, BinItemsGrid *grid, std::vector<uint32_t> *sortMap);
+// sorted (nullable): every image's entries in depth order (global ids below idLimit, image v's
+// at sorted + idBase); the lists then come out in depth order (no per-tile sort needed) and
+// ntS / rectS receive the gathered tile counts and rectangles. longFirst is nullable.
 int launchTileBinningItems(hipStream_t s, const BinItem *items, int numItems,
                            const BinItemsGrid &grid, const uint32_t *nTiles, const int4 *rect,
                            uint32_t *counts, uint32_t *tileTotal, int *tileOffsets,
                            uint32_t *tileList, size_t capacity, uint32_t *listSum,
-                           uint32_t *listMax, uint32_t *overflow, int *longFirst);
+                           uint32_t *listMax, uint32_t *overflow, int *longFirst,
+                           const uint32_t *sorted = nullptr, uint32_t *ntS = nullptr,
+                           int4 *rectS = nullptr, int64_t idLimit = 0);
 int launchTileSortItems(hipStream_t s, const BinItem *items, const uint32_t *sortMap,
                         const BinItemsGrid &grid, const int *tileOffsets, const int *longFirst,
                         const uint32_t *listIn, uint32_t *listOut, const uint32_t *depthKey,

@@ -1040,16 +1040,24 @@ __device__ __forceinline__ int binItemOf(const BinItem *__restrict__ items, int 
     return v;
 }
 
-// Counts of every image of a batch in index order, one launch (items[v].countBlock0 first).
+// Counts of every image of a batch, one launch (items[v].countBlock0 first): in index order,
+// or (sorted) in the depth order sorted + idBase of the image's entries (global ids below
+// idLimit), gathering their tile counts and rectangles to ntS / rectS + idBase.
 __global__ __launch_bounds__(64 * kBinWaves) void tileCountItemsKernel(
-    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, uint32_t *counts)
+    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ sorted,
+    const uint32_t *__restrict__ nTiles, const int4 *__restrict__ rect, uint32_t *ntS,
+    int4 *rectS, int64_t idLimit, uint32_t *counts)
 {
     const int blk = int(blockIdx.x);
     const BinItem it = items[binItemOf<&BinItem::countBlock0>(items, numItems, blk)];
-    tileCountBody(blk - it.countBlock0, it.n, nullptr, nTiles + it.idBase, rect + it.idBase,
-                  nullptr, nullptr, it.tilesX, it.tilesY, it.group, it.numGroups,
-                  counts + it.countsBase, nullptr, it.n);
+    if (sorted)
+        tileCountBody(blk - it.countBlock0, it.n, sorted + it.idBase, nTiles, rect,
+                      ntS + it.idBase, rectS + it.idBase, it.tilesX, it.tilesY, it.group,
+                      it.numGroups, counts + it.countsBase, nullptr, idLimit);
+    else
+        tileCountBody(blk - it.countBlock0, it.n, nullptr, nTiles + it.idBase, rect + it.idBase,
+                      nullptr, nullptr, it.tilesX, it.tilesY, it.group, it.numGroups,
+                      counts + it.countsBase, nullptr, it.n);
 }
 
 // Scan of the group counts, one workgroup per block of kScanTiles consecutive tiles. Lane l of
@@ -1406,22 +1414,26 @@ __global__ __launch_bounds__(64 * kBinWaves) void tileEmitKernel(
                  listSum, listMax, overflow, longFirst, 0u);
 }
 
-// Emission of every image of a batch in index order, one launch: image v's ids (its entries
-// plus items[v].idBase) go to its slice of the lists (items[v].listBase, capacity slots), its
-// offsets and longest-first order to its places; listSum / listMax / overflow as for one image.
+// Emission of every image of a batch, one launch: image v's ids (its entries plus
+// items[v].idBase in index order, or sorted[idBase + i] in the depth order with ntS / rectS as
+// gathered by the count launch) go to its slice of the lists (items[v].listBase, capacity
+// slots), its offsets and longest-first order (nullable) to their places; listSum / listMax /
+// overflow as for one image.
 __global__ __launch_bounds__(64 * kBinWaves) void tileEmitItemsKernel(
-    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ nTiles,
-    const int4 *__restrict__ rect, const uint32_t *__restrict__ counts,
-    const uint32_t *__restrict__ tileTotal, int *tileOffsets, uint32_t *tileList,
-    uint32_t capacity, uint32_t *listSum, uint32_t *listMax, uint32_t *overflow, int *longFirst)
+    const BinItem *__restrict__ items, int numItems, const uint32_t *__restrict__ sorted,
+    const uint32_t *__restrict__ ntS, const int4 *__restrict__ rectS,
+    const uint32_t *__restrict__ counts, const uint32_t *__restrict__ tileTotal, int *tileOffsets,
+    uint32_t *tileList, uint32_t capacity, uint32_t *listSum, uint32_t *listMax,
+    uint32_t *overflow, int *longFirst)
 {
     const int blk = int(blockIdx.x);
     const BinItem it = items[binItemOf<&BinItem::emitBlock0>(items, numItems, blk)];
-    tileEmitBody(blk - it.emitBlock0, it.n, nullptr, nTiles + it.idBase, rect + it.idBase,
-                 it.tilesX, it.tilesY, it.group, it.numGroups, counts + it.countsBase,
-                 tileTotal + it.totalsBase, tileOffsets + it.offsetsBase, nullptr, tileList,
-                 capacity, nullptr, it.listBase, listSum, listMax, overflow,
-                 longFirst + it.orderBase, it.idBase);
+    tileEmitBody(blk - it.emitBlock0, it.n, sorted ? sorted + it.idBase : nullptr,
+                 ntS + it.idBase, rectS + it.idBase, it.tilesX, it.tilesY, it.group,
+                 it.numGroups, counts + it.countsBase, tileTotal + it.totalsBase,
+                 tileOffsets + it.offsetsBase, nullptr, tileList, capacity, nullptr, it.listBase,
+                 listSum, listMax, overflow, longFirst ? longFirst + it.orderBase : nullptr,
+                 it.idBase);
 }
 
 // ------------------------------------------------------------------ per-tile depth order
@@ -1742,6 +1754,26 @@ __device__ __forceinline__ void tileSortBody(int blk, const int *__restrict__ ti
         }
         if (valid)
             listOut[off + int(rank)] = id;
+        return;
+    }
+    if (count <= kTsThreads) {
+        // one key per thread, the keys in LDS (one barrier), each ranked by counting as above
+        const int tid = int(threadIdx.x);
+        const bool valid = tid < count;
+        const uint32_t id = listIn[off + (valid ? tid : 0)];
+        const uint32_t key = depthKey[id];
+        if (valid)
+            L.key[tid] = key;
+        __syncthreads();
+        if (!valid)
+            return;
+        uint32_t rank = 0;
+#pragma unroll 8
+        for (int j = 0; j < count; ++j) {
+            const uint32_t kj = L.key[j];
+            rank += (kj < key) | ((kj == key) & (j < tid)) ? 1u : 0u;
+        }
+        listOut[off + int(rank)] = id;
         return;
     }
     if (count <= kTsCap)
@@ -2128,14 +2160,16 @@ int launchTileBinningItems(hipStream_t s, const BinItem *items, int numItems,
                            const BinItemsGrid &grid, const uint32_t *nTiles, const int4 *rect,
                            uint32_t *counts, uint32_t *tileTotal, int *tileOffsets,
                            uint32_t *tileList, size_t capacity, uint32_t *listSum,
-                           uint32_t *listMax, uint32_t *overflow, int *longFirst)
+                           uint32_t *listMax, uint32_t *overflow, int *longFirst,
+                           const uint32_t *sorted, uint32_t *ntS, int4 *rectS, int64_t idLimit)
 {
     if (numItems <= 0)
         return U3D_OK;
+    U3D_CHECK(!sorted || (ntS && rectS), U3D_ERR_INVALID, "depth-order binning needs ntS / rectS");
     if (grid.countBlocks > 0) {
         hipLaunchKernelGGL(tileCountItemsKernel, dim3(grid.countBlocks), dim3(64 * kBinWaves),
-                           size_t(grid.maxCells) * sizeof(uint32_t), s, items, numItems, nTiles,
-                           rect, counts);
+                           size_t(grid.maxCells) * sizeof(uint32_t), s, items, numItems, sorted,
+                           nTiles, rect, ntS, rectS, idLimit, counts);
         U3D_LAUNCH_CHECK();
     }
     if (grid.scanBlocks > 0) {
@@ -2146,7 +2180,8 @@ int launchTileBinningItems(hipStream_t s, const BinItem *items, int numItems,
     if (grid.emitBlocks > 0) {
         hipLaunchKernelGGL(tileEmitItemsKernel, dim3(grid.emitBlocks), dim3(64 * grid.emitWaves),
                            size_t(grid.emitWaves) * size_t(grid.maxCells) * sizeof(uint32_t), s,
-                           items, numItems, nTiles, rect, counts, tileTotal, tileOffsets, tileList,
+                           items, numItems, sorted, sorted ? ntS : nTiles,
+                           sorted ? rectS : rect, counts, tileTotal, tileOffsets, tileList,
                            uint32_t(capacity < size_t(INT32_MAX) ? capacity : size_t(INT32_MAX)),
                            listSum, listMax, overflow, longFirst);
         U3D_LAUNCH_CHECK();
