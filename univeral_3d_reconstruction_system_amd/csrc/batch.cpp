@@ -165,6 +165,8 @@ private:
     u3d_raster_settings st{};
     int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
     int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
+    int maxViews = 1;    // largest group
+    bool anySh = false;  // a group with view-dependent colour (chain and Adam run apart)
     size_t totalPix = 0, partialsLen = 0;
     size_t offItems = 0, offLoss = 0, offSteps = 0, offGroups = 0, tableBytes = 0;
     bool needRelayout = true;
@@ -223,6 +225,8 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     maxLossBlocks = 0;
     totalEntries = 0;
     maxN = 0;
+    maxViews = 1;
+    anySh = false;
     totalBlocks = 0;
     totalPix = 0;
     partialsLen = 0;
@@ -252,6 +256,8 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         numItems += B;
         totalEntries += n * B;
         maxN = std::max(maxN, n);
+        maxViews = std::max(maxViews, B);
+        anySh = anySh || t->shDegree > 0;
         totalTiles += ceilDiv(W, ts) * ceilDiv(H, ts) * B;
         totalBlocks += int64_t(ceilDiv(W, 16)) * ceilDiv(H, 16) * B;
         totalPix += size_t(W) * size_t(H) * size_t(B);
@@ -657,6 +663,12 @@ int BatchEngine::enqueue()
                                     int(std::max<int64_t>(wgs, 1)), tileOffsets.p, tileList, sb,
                                     st, grad2d.p, lds, tuning.atomic_lanes,
                                     SegmentState{nullptr, nullptr}));
+    }
+    if (!anySh) {
+        StageScope scope(ctx, U3D_STAGE_CHAIN);
+        U3D_TRY(launchChainAdamBatched(s, dItems, dGroups, dSteps, numGroups, maxN, maxViews, st,
+                                       sb, grad2d.p));
+        return U3D_OK;
     }
     {
         StageScope scope(ctx, U3D_STAGE_CHAIN);

@@ -249,6 +249,13 @@ int launchProjectBatched(hipStream_t s, const RenderItem *items, int numItems, i
 int launchChainBatched(hipStream_t s, const RenderItem *items, const int2 *groups, int numGroups,
                        int64_t maxN, const u3d_raster_settings &st, const ScreenBuffers &sb,
                        float *grad2d);
+// Batched chain rule and Adam in one pass, for groups with no view-dependent colour: the views of
+// a scene run on separate wavefronts (up to four per block of 64 Gaussians, maxViews = the
+// largest group), their gradient sums are added in wavefront order and the update is
+// adamBatchedKernel's; the gradient arrays are not written.
+int launchChainAdamBatched(hipStream_t s, const RenderItem *items, const int2 *groups,
+                           const SceneStep *scenes, int numGroups, int64_t maxN, int maxViews,
+                           const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d);
 // Unpacks the records into the separate views of u3d_raster_state.
 int launchUnpackScreen(hipStream_t s, int64_t n, const ScreenBuffers &sb, float *mean2d,
                        float *conic, float *color, float *opacity);

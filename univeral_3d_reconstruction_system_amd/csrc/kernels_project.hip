@@ -536,6 +536,118 @@ __global__ __launch_bounds__(kThreads) void chainBatchedKernel(const RenderItem 
     }
 }
 
+// Batched chain rule fused with Adam, for scene groups without view-dependent colour. A block
+// holds 64 Gaussians of scene group blockIdx.y; wavefront w takes the group's views w,
+// w + kWaves, ... (a lane runs one (Gaussian, view) pair at a time, so a group of B views keeps
+// up to B times the wavefronts of one view busy), adds the parameter gradients in registers and
+// clears the 2D gradients it consumed. The wavefronts' sums meet in LDS and are added in
+// wavefront order (a fixed order: the result does not depend on timing); each of the block's
+// 64 x 14 parameters then takes adamBatchedKernel's update from its summed gradient, so the
+// gradients never pass through memory and the gradient arrays stay zero. The densification
+// statistics (|d mean2d| in NDC units, views that saw the Gaussian) are summed the same way.
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void chainAdamViewsKernel(
+    const RenderItem *__restrict__ items, const int2 *__restrict__ groups,
+    const SceneStep *__restrict__ scenes, float nearPlane, float eps2d, ScreenBuffers sb,
+    float *grad2d)
+{
+    constexpr int kVals = 16; // 14 parameter gradients, the NDC statistic, the views seen
+    __shared__ float part[kWaves][kVals][64];
+    const int2 grp = groups[blockIdx.y];
+    const SceneStep &sc = scenes[blockIdx.y];
+    const int64_t n = sc.n, base = int64_t(blockIdx.x) * 64;
+    if (grp.y <= 0 || base >= n)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int64_t i = base + lane;
+    const RenderItem &first = items[grp.x];
+    float *const ndc = first.ndc;
+    int *const gradCount = first.gradCount;
+    GradRegs gr;
+    float ndcSum = 0.0f, seen = 0.0f;
+    if (i < n) {
+        for (int v = grp.x + wave; v < grp.x + grp.y; v += kWaves) {
+            const RenderItem &it = items[v];
+            const int64_t o = it.screenBase + i;
+            const int r = sb.radius[o];
+            if (r == 0)
+                continue;
+            if (r > 0)
+                seen += 1.0f;
+            if (ndc) {
+                const float hx = grad2d[o * kGrad2dStride] * 0.5f * float(it.cam.width);
+                const float hy = grad2d[o * kGrad2dStride + 1] * 0.5f * float(it.cam.height);
+                ndcSum += sqrtf(hx * hx + hy * hy) * it.ndcScale;
+            }
+            ShCoefs sh;
+            chainBody<0>(i, o, it.means, it.scales, it.quats, it.cam, nearPlane, eps2d, sb.radius,
+                         sb.flags, sb.rec1, grad2d, gr, nullptr, nullptr, sh,
+                         cameraCentre(it.cam), it.ndcScale);
+#pragma unroll
+            for (int k = 0; k < kGrad2dStride; ++k)
+                grad2d[o * kGrad2dStride + k] = 0.0f;
+        }
+    }
+    // slots: means 0-2, scales 3-5, quats 6-9, opacity 10, colours 11-13, NDC 14, seen 15
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        part[wave][k][lane] = gr.means[k];
+        part[wave][3 + k][lane] = gr.scales[k];
+        part[wave][11 + k][lane] = gr.colors[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        part[wave][6 + k][lane] = gr.quats[k];
+    part[wave][10][lane] = gr.opac;
+    part[wave][14][lane] = ndcSum;
+    part[wave][15][lane] = seen;
+    __syncthreads();
+
+    // The block's parameters in the scene's layout [means 3n | scales 3n | quats 4n | opacity n |
+    // colours 3n] (each group at its capacity offset): value f of the block is group g's entry
+    // q = f - start[g], Gaussian q / comps, component q % comps.
+    const int64_t c = sc.cap;
+    const int start[6] = {0, 192, 384, 640, 704, 896};
+    const int comps[5] = {3, 3, 4, 1, 3};
+    const int slot0[5] = {0, 3, 6, 10, 11};
+    const int64_t off[5] = {0, 3 * c, 6 * c, 10 * c, 11 * c};
+    for (int f = threadIdx.x; f < 896; f += 64 * kWaves) {
+        int g = 0;
+#pragma unroll
+        for (int k = 1; k < 5; ++k)
+            g += f >= start[k] ? 1 : 0;
+        const int q = f - start[g], gi = q / comps[g], comp = q - gi * comps[g];
+        if (base + gi >= n)
+            continue;
+        float grad = part[0][slot0[g] + comp][gi];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w)
+            grad += part[w][slot0[g] + comp][gi];
+        const int64_t idx = off[g] + (base + gi) * comps[g] + comp;
+        float *__restrict__ p = sc.params;
+        float *__restrict__ m = sc.adamM;
+        float *__restrict__ v = sc.adamV;
+        const float mk = sc.beta1 * m[idx] + (1.0f - sc.beta1) * grad;
+        const float vk = sc.beta2 * v[idx] + (1.0f - sc.beta2) * grad * grad;
+        m[idx] = mk;
+        v[idx] = vk;
+        p[idx] = p[idx] - sc.stepSize[g] * mk / (sqrtf(vk) * sc.invSqrtBc2 + 1e-15f);
+    }
+    if (threadIdx.x < 64 && i < n) {
+        float stat = part[0][14][lane], views = part[0][15][lane];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            stat += part[w][14][lane];
+            views += part[w][15][lane];
+        }
+        if (ndc)
+            ndc[i] += stat;
+        if (gradCount)
+            gradCount[i] += int(views);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void unpackKernel(int64_t n, ScreenBuffers sb,
                                                          float *mean2d, float *conic,
                                                          float *color, float *opacity)
@@ -642,6 +754,26 @@ int launchChainBatched(hipStream_t s, const RenderItem *items, const int2 *group
         return U3D_OK;
     hipLaunchKernelGGL(chainBatchedKernel, dim3(blocksFor(maxN), numGroups), dim3(kThreads), 0, s,
                        items, groups, st.near_plane, st.eps2d, sb, grad2d);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchChainAdamBatched(hipStream_t s, const RenderItem *items, const int2 *groups,
+                           const SceneStep *scenes, int numGroups, int64_t maxN, int maxViews,
+                           const u3d_raster_settings &st, const ScreenBuffers &sb, float *grad2d)
+{
+    if (numGroups <= 0 || maxN <= 0)
+        return U3D_OK;
+    const dim3 grid(unsigned((maxN + 63) / 64), unsigned(numGroups));
+    if (maxViews >= 4)
+        hipLaunchKernelGGL(chainAdamViewsKernel<4>, grid, dim3(256), 0, s, items, groups, scenes,
+                           st.near_plane, st.eps2d, sb, grad2d);
+    else if (maxViews >= 2)
+        hipLaunchKernelGGL(chainAdamViewsKernel<2>, grid, dim3(128), 0, s, items, groups, scenes,
+                           st.near_plane, st.eps2d, sb, grad2d);
+    else
+        hipLaunchKernelGGL(chainAdamViewsKernel<1>, grid, dim3(64), 0, s, items, groups, scenes,
+                           st.near_plane, st.eps2d, sb, grad2d);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
