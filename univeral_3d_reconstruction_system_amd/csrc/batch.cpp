@@ -62,6 +62,7 @@ public:
             statusStep[k] = -1;
         }
         U3D_HIP(hipHostMalloc(reinterpret_cast<void **>(&statusHost), sizeof(uint32_t) * 4 * kRing));
+        U3D_TRY(ringPos.ensure(1));
         for (int k = 0; k < 3; ++k) {
             occFwd[k] = rasterBatchedOccupancy(false, 64 << k);
             occBwd[k] = rasterBatchedOccupancy(true, 64 << k);
@@ -183,6 +184,11 @@ private:
     int64_t statusStep[kRing] = {};
     std::vector<u3d_trainer *> statusScenes[kRing];
     int ring = 0;
+    // The device's copy of ring: the step's first launch reads its table from pinned slot
+    // *ringPos, the loss finalisation writes the status to that slot and advances it (both
+    // inside the captured graph). ringSynced: *ringPos == ring.
+    DevBuf<uint32_t> ringPos;
+    bool ringSynced = false;
     int64_t stepNo = 0;
 
     // device buffers
@@ -217,6 +223,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     const hipStream_t s = ctx->stream;
     U3D_HIP(hipStreamSynchronize(s));
     dropGraph();
+    ringSynced = false;
     st = act[0]->opt.raster;
     key.clear();
     numItems = 0;
@@ -486,6 +493,8 @@ int BatchEngine::probePairs()
     const hipStream_t s = ctx->stream;
     const RenderItem *dItems = reinterpret_cast<const RenderItem *>(table.p + offItems);
     const ScreenBuffers sb = screen();
+    U3D_TRY(launchStepPrologue(s, pinned, pinnedSlot, tableBytes, table.p, ringPos.p, nullptr, 0,
+                               nullptr, 0));
     U3D_TRY(launchProjectBatched(s, dItems, numItems, maxN, st, sb, entryItem.p));
     U3D_TRY(scanScratch.ensure(scanScratchSize(totalEntries)));
     U3D_TRY(exclusiveScanU32(s, nTiles.p, pairOffsets.p, totalEntries, total.p, scanScratch.p,
@@ -567,14 +576,16 @@ int BatchEngine::enqueue()
     const ScreenBuffers sb = screen();
     {
         StageScope scope(ctx, U3D_STAGE_PROJECT);
+        // the step's table from its pinned slot; K, overflow flag, longest item list set and
+        // the compositing work counters start at zero
+        U3D_TRY(launchStepPrologue(s, pinned, pinnedSlot, tableBytes, table.p, ringPos.p, total.p,
+                                   4, counters.p, 2));
         U3D_TRY(launchProjectBatched(s, dItems, numItems, maxN, st, sb, entryItem.p));
     }
     const uint32_t *order = nullptr;
     const int *tileList = nullptr;
     {
         StageScope scope(ctx, U3D_STAGE_SORT);
-        // K, overflow flag, longest item list set
-        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(total.p), 0, 4));
         RadixScratch rs{hist.p, hist.cap, histScan.p, histScan.cap, tuning.sort_items};
         rs.radixBits = tuning.sort_radix ? tuning.sort_radix : 8;
         if (depthBinned) {
@@ -643,7 +654,6 @@ int BatchEngine::enqueue()
     const int lds = tuning.lds_batch, li = lds == 64 ? 0 : (lds == 128 ? 1 : 2);
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_FWD);
-        U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(counters.p), 0, 2));
         const int occ = perCu > 0 ? perCu : occFwd[li];
         const int64_t wgs = std::min<int64_t>(int64_t(ctx->num_cus) * occ, totalBlocks);
         U3D_TRY(launchRasterBatched(s, false, dItems, numItems, order, totalBlocks, counters.p,
@@ -652,8 +662,14 @@ int BatchEngine::enqueue()
     }
     {
         StageScope scope(ctx, U3D_STAGE_LOSS);
+        StepStatus status;
+        status.total = total.p;
+        status.host = statusHost;
+        status.ring = ringPos.p;
+        status.ringSize = kRing;
         U3D_TRY(launchPhotometricLossBatched(s, dLoss, numItems, maxLossBlocks,
-                                             key.empty() ? 0.2f : key[0].t->opt.ssim_lambda));
+                                             key.empty() ? 0.2f : key[0].t->opt.ssim_lambda,
+                                             status));
     }
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
@@ -754,16 +770,22 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
         U3D_TRY(poll(true));
         U3D_TRY(relayout(act));
     }
-    // This step's table: wait until the slot's previous upload has been consumed.
+    // This step's table: wait until the step that last used the slot has finished (its first
+    // launch copies the table to the device, its loss finalisation writes the slot's status).
     const int slot = ring;
     if (slotUsed[slot])
         U3D_HIP(hipEventSynchronize(slotDone[slot]));
     unsigned char *host = pinned + size_t(slot) * pinnedSlot;
     U3D_TRY(fillTable(act, host));
-    U3D_HIP(hipMemcpyAsync(table.p, host, tableBytes, hipMemcpyHostToDevice, s));
-    U3D_HIP(hipEventRecord(slotDone[slot], s));
-    slotUsed[slot] = true;
     if (totalEntries > 0) {
+        if (statusStep[slot] >= 0)
+            U3D_TRY(poll(true));
+        if (!ringSynced) {
+            U3D_TRY(launchFillI32(s, reinterpret_cast<int *>(ringPos.p), slot, 1));
+            ringSynced = true;
+        }
+        // a failed launch below leaves the device ring unknown
+        ringSynced = false;
         if (pairCap == 0)
             U3D_TRY(probePairs());
         if (useGraph) {
@@ -774,15 +796,14 @@ int BatchEngine::step(u3d_trainer *const *scenes, int numScenes)
         } else {
             U3D_TRY(enqueue());
         }
-        if (statusStep[slot] >= 0)
-            U3D_TRY(poll(true));
-        U3D_HIP(hipMemcpyAsync(statusHost + 4 * slot, total.p, 4 * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, s));
+        ringSynced = true;
+        U3D_HIP(hipEventRecord(slotDone[slot], s));
+        slotUsed[slot] = true;
         U3D_HIP(hipEventRecord(statusDone[slot], s));
         statusStep[slot] = stepNo;
         statusScenes[slot] = act;
+        ring = (ring + 1) % kRing;
     }
-    ring = (ring + 1) % kRing;
     ++stepNo;
     for (u3d_trainer *t : act) {
         const int64_t B = t->viewsPerStep;

@@ -577,8 +577,17 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
 int launchLossFinalize(hipStream_t s, const LossFinalize &f);
 size_t lossPartialsSize(int width, int height);
 void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks);
+// Where the batched step leaves its pair-count status: the loss finalisation's first workgroup
+// copies total[0..3] to host[4 * slot .. 4 * slot + 3] (pinned host memory, system-scope stores)
+// with slot = *ring, then advances *ring to (slot + 1) % ringSize (ring null: no status).
+struct StepStatus {
+    const uint32_t *total = nullptr;
+    uint32_t *host = nullptr;
+    uint32_t *ring = nullptr;
+    int ringSize = 1;
+};
 int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
-                                 int maxBlocks, float lambda);
+                                 int maxBlocks, float lambda, const StepStatus &status = {});
 // Squared-error sum of two float arrays -> result[0] = psnr (double partials scratch).
 int launchPsnr(hipStream_t s, const float *a, const float *b, const uint8_t *bU8, int64_t count,
                double *partials, size_t partialsCap, float *result);
@@ -610,6 +619,12 @@ int launchScatterPlanes(hipStream_t s, const float *src, float *dst, int planes,
 int launchAdamBatched(hipStream_t s, const SceneStep *scenes, int numScenes, int64_t maxN);
 int launchFill(hipStream_t s, float *p, float value, int64_t count);
 int launchFillI32(hipStream_t s, int *p, int value, int64_t count);
+// First launch of a batched step: copies the step's table (tableBytes, a multiple of 16) from
+// pinned ring slot *ring (slot k at pinned + k * slotBytes) to the device table, and zeroes
+// zeroA[0..countA) and zeroB[0..countB) (counters of the step). One workgroup.
+int launchStepPrologue(hipStream_t s, const unsigned char *pinned, size_t slotBytes,
+                       size_t tableBytes, unsigned char *table, const uint32_t *ring,
+                       uint32_t *zeroA, int countA, uint32_t *zeroB, int countB);
 // o = min(o, cap) (Trainer.cpp:171-178)
 int launchOpacityCap(hipStream_t s, float *opacities, int64_t n, float cap);
 // Densification: per-Gaussian demand (Trainer.cpp:116-135 before the budget test).

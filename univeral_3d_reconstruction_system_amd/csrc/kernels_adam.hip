@@ -95,6 +95,19 @@ __global__ void fillI32Kernel(int *p, int value, int64_t count)
         p[i] = value;
 }
 
+__global__ __launch_bounds__(kThreads) void stepPrologueKernel(
+    const uint4 *__restrict__ pinned, size_t slotWords, size_t tableWords, uint4 *__restrict__ table,
+    const uint32_t *__restrict__ ring, uint32_t *zeroA, int countA, uint32_t *zeroB, int countB)
+{
+    const uint4 *src = pinned + size_t(*ring) * slotWords;
+    for (size_t k = threadIdx.x; k < tableWords; k += kThreads)
+        table[k] = src[k];
+    if (int(threadIdx.x) < countA)
+        zeroA[threadIdx.x] = 0u;
+    if (int(threadIdx.x) < countB)
+        zeroB[threadIdx.x] = 0u;
+}
+
 __global__ void opacityCapKernel(float *o, int64_t n, float cap)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -432,6 +445,20 @@ int launchAdamBatched(hipStream_t s, const SceneStep *scenes, int numScenes, int
     blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
     hipLaunchKernelGGL(adamBatchedKernel, dim3(unsigned(blocks), numScenes), dim3(kThreads), 0, s,
                        scenes);
+    U3D_LAUNCH_CHECK();
+    return U3D_OK;
+}
+
+int launchStepPrologue(hipStream_t s, const unsigned char *pinned, size_t slotBytes,
+                       size_t tableBytes, unsigned char *table, const uint32_t *ring,
+                       uint32_t *zeroA, int countA, uint32_t *zeroB, int countB)
+{
+    U3D_CHECK(slotBytes % 16 == 0 && tableBytes % 16 == 0 && tableBytes <= slotBytes &&
+                  countA <= kThreads && countB <= kThreads,
+              U3D_ERR_INVALID, "step table layout");
+    hipLaunchKernelGGL(stepPrologueKernel, dim3(1), dim3(kThreads), 0, s,
+                       reinterpret_cast<const uint4 *>(pinned), slotBytes / 16, tableBytes / 16,
+                       reinterpret_cast<uint4 *>(table), ring, zeroA, countA, zeroB, countB);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }

@@ -349,8 +349,17 @@ __global__ __launch_bounds__(kThreads) void lossBatchedKernel(const LossItem *__
 __global__ __launch_bounds__(kThreads) void lossFinalize(LossFinalize f) { lossFinalizeBlock(f); }
 
 __global__ __launch_bounds__(kThreads) void lossFinalizeBatched(const LossItem *__restrict__ items,
-                                                                float lambda)
+                                                                float lambda, StepStatus status)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && status.ring) {
+        // the step's pair counts are final (the binning ran before the compositing)
+        const uint32_t slot = *status.ring;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            __hip_atomic_store(&status.host[4 * slot + k], status.total[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        *status.ring = (slot + 1) % uint32_t(status.ringSize);
+    }
     const LossItem &it = items[blockIdx.x];
     LossFinalize f;
     f.partials = it.partials;
@@ -480,7 +489,7 @@ void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks)
 }
 
 int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
-                                 int maxBlocks, float lambda)
+                                 int maxBlocks, float lambda, const StepStatus &status)
 {
     if (numItems <= 0)
         return U3D_OK;
@@ -490,7 +499,8 @@ int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numIt
                            items, lambda, win);
         U3D_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(lossFinalizeBatched, dim3(numItems), dim3(kThreads), 0, s, items, lambda);
+    hipLaunchKernelGGL(lossFinalizeBatched, dim3(numItems), dim3(kThreads), 0, s, items, lambda,
+                       status);
     U3D_LAUNCH_CHECK();
     return U3D_OK;
 }
