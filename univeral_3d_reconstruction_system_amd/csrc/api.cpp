@@ -601,13 +601,18 @@ int u3d_photometric_loss(u3d_ctx *ctx, const float *render, const float *target,
     U3D_TRY(useDevice(ctx));
     U3D_CHECK(width >= 0 && height >= 0, U3D_ERR_INVALID, "negative image size");
     U3D_CHECK((render && target && grad) || size_t(width) * size_t(height) == 0, U3D_ERR_INVALID, "null image");
+    // [partials: doubles][result: 4 floats][SSIM partial maps: 9 floats per pixel], 256-byte
+    // aligned parts
     const size_t np = lossPartialsSize(width, height) + 1;
+    const size_t resultOffset = (np * sizeof(double) + 255) / 256 * 256;
+    const size_t mapsOffset = resultOffset + 256;
     unsigned char *sc = nullptr;
-    U3D_TRY(ctxScratch(ctx, np * sizeof(double) + 64, &sc));
+    U3D_TRY(ctxScratch(ctx, mapsOffset + lossMapsSize(width, height) * sizeof(float), &sc));
     double *partials = reinterpret_cast<double *>(sc);
-    float *result = reinterpret_cast<float *>(sc + np * sizeof(double));
+    float *result = reinterpret_cast<float *>(sc + resultOffset);
+    float *maps = reinterpret_cast<float *>(sc + mapsOffset);
     U3D_TRY(launchPhotometricLoss(ctx->stream, render, target, nullptr, width, height, ssim_lambda,
-                                  grad, partials, np, result));
+                                  grad, partials, np, result, nullptr, nullptr, maps));
     float host[4];
     U3D_HIP(hipMemcpyAsync(host, result, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
     U3D_HIP(hipStreamSynchronize(ctx->stream));
@@ -986,6 +991,7 @@ int u3d_trainer_create(u3d_ctx *ctx, const float *means, const float *scales, co
         U3D_HIP(hipMemcpyAsync(t->targets.p, rgb8, total, hipMemcpyHostToDevice, s));
     U3D_TRY(t->render.ensure(maxPix * 3));
     U3D_TRY(t->dloss.ensure(maxPix * 3));
+    U3D_TRY(t->ssimMaps.ensure(maxPix * 9));
     U3D_TRY(t->gradAccum.ensure(size_t(cap)));
     U3D_TRY(t->gradCount.ensure(size_t(cap)));
     U3D_TRY(t->evalPairs.ensure(64));
@@ -1217,7 +1223,7 @@ int u3d_trainer_step(u3d_trainer *t)
         U3D_TRY(launchPhotometricLoss(s, t->render.p, nullptr, t->targets.p + t->imageOffset[view],
                                       cam.width, cam.height, t->opt.ssim_lambda, t->dloss.p,
                                       t->partials.p, t->partials.cap, stats, &lossFinalize,
-                                      prepWithLoss ? &prep : nullptr));
+                                      prepWithLoss ? &prep : nullptr, t->ssimMaps.p));
     }
     t->grads.n = t->params.n;
     u3d_gaussians gv = t->grads.view();

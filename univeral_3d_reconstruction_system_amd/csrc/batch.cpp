@@ -110,7 +110,8 @@ public:
         const size_t entries = size_t(n) * size_t(views);
         const size_t perEntry = 2 * 16 + 4 * 4 + 16 + 1 + 8 * 4 + 9 * 4;
         const size_t pairs = entries * 24; // capacity for ~12 tiles per entry, 2 copies
-        return entries * perEntry + pairs * 8 + size_t(views) * pixels * 40;
+        // per pixel: render, loss gradient, transmittance, contributor count, loss SSIM maps
+        return entries * perEntry + pairs * 8 + size_t(views) * pixels * (40 + 36);
     }
 
     u3d_ctx *ctx;
@@ -167,6 +168,7 @@ private:
     int numItems = 0, numGroups = 0, totalTiles = 0, maxLossBlocks = 0, tileBits = 1;
     int64_t totalEntries = 0, maxN = 0, totalBlocks = 0, pairCap = 0;
     int maxViews = 1;    // largest group
+    bool splitLoss = true; // every image small enough for the split loss's 32-bit offsets
     bool anySh = false;  // a group with view-dependent colour (chain and Adam run apart)
     size_t totalPix = 0, partialsLen = 0;
     size_t offItems = 0, offLoss = 0, offSteps = 0, offGroups = 0, tableBytes = 0;
@@ -213,6 +215,7 @@ private:
     DevBuf<int> longFirst;
     DevBuf<unsigned> counters; // persistent work counters (forward, backward)
     DevBuf<double> partials;
+    DevBuf<float> ssimMaps; // the loss's SSIM partials, 9 floats per pixel of every item
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -234,6 +237,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     maxN = 0;
     maxViews = 1;
     anySh = false;
+    splitLoss = true;
     totalBlocks = 0;
     totalPix = 0;
     partialsLen = 0;
@@ -264,6 +268,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
         totalEntries += n * B;
         maxN = std::max(maxN, n);
         maxViews = std::max(maxViews, B);
+        splitLoss = splitLoss && size_t(W) * size_t(H) * 3 < (size_t(1) << 32);
         anySh = anySh || t->shDegree > 0;
         totalTiles += ceilDiv(W, ts) * ceilDiv(H, ts) * B;
         totalBlocks += int64_t(ceilDiv(W, 16)) * ceilDiv(H, 16) * B;
@@ -353,6 +358,7 @@ int BatchEngine::relayout(const std::vector<u3d_trainer *> &act)
     U3D_TRY(finalT.ensure(np));
     U3D_TRY(nContrib.ensure(np));
     U3D_TRY(partials.ensure(std::max<size_t>(partialsLen, 1)));
+    U3D_TRY(ssimMaps.ensure(np * 9));
     // Gradients start at zero; afterwards the chain rule clears the 2D gradients it consumed and
     // Adam the parameter gradients it applied.
     U3D_HIP(hipMemsetAsync(grad2d.p, 0, grad2d.cap * sizeof(float), s));
@@ -429,6 +435,7 @@ int BatchEngine::fillTable(const std::vector<u3d_trainer *> &act, unsigned char 
             li.grad = dl.p + pix * 3;
             li.result = t->statsSlot(size_t(t->iteration) * size_t(B) + size_t(k));
             li.partials = partials.p + part;
+            li.maps = ssimMaps.p + pix * 9;
             li.width = W;
             li.height = H;
             li.blocksX = lbx;
@@ -559,7 +566,8 @@ int BatchEngine::probePairs()
                      sortScratch.cap * 4, longFirst.cap * 4, sortMap.cap * 4, rectSorted.cap * 16,
                      hist.cap * 4,
                      histScan.cap * 4, scanScratch.cap * 4, rgb.cap * 4, dl.cap * 4, finalT.cap * 4,
-                     nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4})
+                     nContrib.cap * 4, partials.cap * 8, blockKey.cap * 16, tileOffsets.cap * 4,
+                     ssimMaps.cap * 4})
         allocated += b;
     return U3D_OK;
 }
@@ -669,7 +677,7 @@ int BatchEngine::enqueue()
         status.ringSize = kRing;
         U3D_TRY(launchPhotometricLossBatched(s, dLoss, numItems, maxLossBlocks,
                                              key.empty() ? 0.2f : key[0].t->opt.ssim_lambda,
-                                             status));
+                                             status, splitLoss));
     }
     {
         StageScope scope(ctx, U3D_STAGE_RASTER_BWD);
@@ -913,7 +921,7 @@ size_t trainerBytes(const u3d_trainer *t)
 {
     size_t b = (t->params.buf.cap + t->spare.buf.cap + t->grads.buf.cap + t->adamM.buf.cap
                 + t->adamV.buf.cap) * sizeof(float);
-    b += t->targets.cap + (t->render.cap + t->dloss.cap + t->gradAccum.cap) * sizeof(float);
+    b += t->targets.cap + (t->render.cap + t->dloss.cap + t->ssimMaps.cap + t->gradAccum.cap) * sizeof(float);
     b += t->gradCount.cap * sizeof(int);
     b += (t->shP.cap + t->shG.cap + t->shM.cap + t->shV.cap + t->shSpare.cap) * sizeof(float);
     return b;

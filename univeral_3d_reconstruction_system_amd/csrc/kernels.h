@@ -181,6 +181,7 @@ struct LossItem {
     float *grad;       // dL/drgb of the item, scaled by gradScale
     float *result;     // {total, l1, ssim, psnr}
     double *partials;  // 9 doubles per 32x32 block
+    float *maps;       // SSIM partials, 9 floats per pixel (the split loss; null: fused)
     int width, height, blocksX, numBlocks;
     float invCount, gradScale;
 };
@@ -570,10 +571,15 @@ int launchAdamBf16(hipStream_t s, float *p, const float *g, float *m, float *v, 
 // Writes per-block double partials (l1, ssim, squared error) into partials and the final
 // {total, l1, ssim, psnr} floats into result (device) via a one-block finalize kernel; with
 // `deferred`, the finalize is not launched but described there, for a later launch to run it.
+// With ssimMaps (9 * width * height floats of scratch) the loss runs as two launches that pass
+// the SSIM partials through it (the same gradient and values, bit for bit, with a third less
+// arithmetic); without, as one fused launch.
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
                           float *grad, double *partials, size_t partialsCap, float *result,
-                          LossFinalize *deferred = nullptr, const SegmentPrep *prep = nullptr);
+                          LossFinalize *deferred = nullptr, const SegmentPrep *prep = nullptr,
+                          float *ssimMaps = nullptr);
+inline size_t lossMapsSize(int width, int height) { return size_t(9) * size_t(width) * size_t(height); }
 int launchLossFinalize(hipStream_t s, const LossFinalize &f);
 size_t lossPartialsSize(int width, int height);
 void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks);
@@ -586,8 +592,10 @@ struct StepStatus {
     uint32_t *ring = nullptr;
     int ringSize = 1;
 };
+// splitMaps: every item has maps (the two-launch loss).
 int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
-                                 int maxBlocks, float lambda, const StepStatus &status = {});
+                                 int maxBlocks, float lambda, const StepStatus &status = {},
+                                 bool splitMaps = false);
 // Squared-error sum of two float arrays -> result[0] = psnr (double partials scratch).
 int launchPsnr(hipStream_t s, const float *a, const float *b, const uint8_t *bU8, int64_t count,
                double *partials, size_t partialsCap, float *result);

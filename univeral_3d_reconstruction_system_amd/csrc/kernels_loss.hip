@@ -33,6 +33,8 @@ static_assert(kIH * ((kHW + kH1Seg - 1) / kH1Seg) <= kThreads, "one H1 task per 
 static_assert(kAW * ((kAH + kV1Seg - 1) / kV1Seg) <= kThreads, "one V1 task per thread");
 static_assert(kGH * ((kGW + kH2Seg - 1) / kH2Seg) <= kThreads, "one H2 task per thread");
 static_assert(kTW * ((kTH + kV2Seg - 1) / kV2Seg) <= kThreads, "one output task per thread");
+static_assert(kTW * (kTH / kV2Seg) == kThreads && kTH % kV2Seg == 0,
+              "every thread owns kV2Seg pixels of one column");
 
 struct Window {
     float w[kTaps];
@@ -86,6 +88,7 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
     __shared__ __align__(16) unsigned char ldsA[kInBytes > kABytes ? kInBytes : kABytes];
     __shared__ __align__(16) unsigned char ldsB[kHBytes > kGBytes ? kHBytes : kGBytes];
     __shared__ double red[3][kThreads / 64];
+    __shared__ float sV[kTH][kTW]; // SSIM of the block's pixels (summed in the split order)
     auto sIn = reinterpret_cast<float2(*)[kInStride]>(ldsA);
     auto sA = reinterpret_cast<float(*)[kAH][kAStride]>(ldsA);
     auto sH = reinterpret_cast<float(*)[kHH][kHStride]>(ldsB);
@@ -204,10 +207,8 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
                     const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
                     const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
                     const float vv = (a1 * a2) / (b1 * b2);
-                    // interior pixels once each (the overlapping last segment repeats rows)
-                    const bool counted = (tid / kAW) * kV1Seg <= ly;
-                    if (counted && lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
-                        ssim += double(vv);
+                    if (lx >= kR && lx < kR + kTW && ly >= kR && ly < kR + kTH)
+                        sV[ly - kR][lx - kR] = vv;
                     const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - vv * (2 * mx / b1 - 2 * mx / b2);
                     p0 = dLds * dmu;
                     p1 = dLds * (-vv / b2);
@@ -219,6 +220,14 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
             }
         }
         __syncthreads();
+        {
+            // the block's SSIM sum, in the order of ssimBlock
+            const int lx = tid % kTW, y0 = (tid / kTW) * kV2Seg;
+#pragma unroll
+            for (int o = 0; o < kV2Seg; ++o)
+                if (ox + lx < W && oy + y0 + o < H)
+                    ssim += double(sV[y0 + o][lx]);
+        }
         // 4. horizontal blur of the partials (Loss.cpp:142-157): row ly, kH2Seg outputs
         if (tid < kGH * ((kGW + kH2Seg - 1) / kH2Seg)) {
             const int ly = tid % kGH, x0 = segStart(tid / kGH, kH2Seg, kGW);
@@ -268,8 +277,10 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
             }
         }
     } else {
-        for (int k = tid; k < kTH * kTW; k += kThreads) {
-            const int gx = ox + k % kTW, gy = oy + k / kTW;
+        const int lx = tid % kTW, y0 = (tid / kTW) * kV2Seg;
+#pragma unroll
+        for (int o = 0; o < kV2Seg; ++o) {
+            const int gx = ox + lx, gy = oy + y0 + o;
             const float gp[3] = {0.0f, 0.0f, 0.0f};
             if (gx < W && gy < H)
                 finish(gx, gy, gp);
@@ -295,6 +306,297 @@ __device__ __forceinline__ void lossBlock(const float *__restrict__ render,
         for (int wv = 0; wv < kThreads / 64; ++wv)
             sum += red[tid][wv];
         partials[tid] = sum;
+    }
+}
+
+// ---- The loss as two launches: SSIM partials through memory.
+//
+// The fused block recomputes the moment blurs and the SSIM partials over a 10-pixel halo (the
+// partials' own 5-pixel halo, blurred again): 2.1x the horizontal moment blurs and 1.7x the
+// SSIM points of its 32x32 pixels. With a scratch of nine floats per pixel the work splits at
+// the partials: ssimBlock computes them once per pixel (moment blurs over a 5-pixel halo) and
+// stores them, planar per channel; gradBlock blurs them (again a 5-pixel halo) into the
+// gradient. Same expressions in the same order, so the gradient is bit-identical to the fused
+// block's; the block sums are taken in the same thread order as there (the fused block routes
+// its SSIM values through LDS for that), so the loss values are identical too.
+constexpr int kSH = 3, kSV = 2;                         // outputs per thread: horizontal, vertical
+constexpr int kMW = kTW + 2 * kR, kMH = kTH + 2 * kR;   // 42 x 42: a block with a 5-pixel halo
+constexpr int kMStride = kMW + 1, kBStride = kTW + 1;   // padded LDS rows
+constexpr int kHSegs = (kTW + kSH - 1) / kSH;           // horizontal runs per row
+static_assert(kMH * kHSegs <= kThreads, "one horizontal run per thread");
+static_assert(kTW * (kTH / kSV) == kThreads && kSV == kV2Seg, "vertical runs as the fused V2");
+// Window loads: thread t < kLoadRows * kMW reads window column t % kMW of rows
+// t / kMW + kLoadRows * q (one division per thread, a row step per pass).
+constexpr int kLoadRows = kThreads / kMW;                        // 12
+constexpr int kLoadPasses = (kMH + kLoadRows - 1) / kLoadRows;   // 4
+
+// Map j (0..2) of channel c: maps[(3 c + j) * P + pixel].
+__device__ __forceinline__ size_t mapIndex(int c, int j, size_t P, size_t pix)
+{
+    return size_t(3 * c + j) * P + pix;
+}
+
+// The same memory as a global-segment pointer: accesses through it use the global
+// instructions with a uniform base and a 32-bit offset (a pointer read from an item table is
+// otherwise a generic one, accessed with flat instructions that also count as LDS traffic in
+// the waits). Offsets: images of 3 * W * H < 2^32 values (checked by the launchers).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *glob(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+template <bool kU8>
+__device__ __forceinline__ void ssimBlock(const float *__restrict__ render,
+                                          const float *__restrict__ targetF,
+                                          const uint8_t *__restrict__ targetU8, int W, int H,
+                                          float lambda, float invCount, const Window &win,
+                                          int blocksX, int blockId, int c,
+                                          float *__restrict__ maps, double *__restrict__ partials)
+{
+    __shared__ float2 sIn[kMH][kMStride];
+    __shared__ float sH[5][kMH][kBStride];
+    __shared__ double red[kThreads / 64];
+    const int tid = threadIdx.x;
+    float w[kTaps];
+#pragma unroll
+    for (int t = 0; t < kTaps; ++t)
+        w[t] = win.w[t];
+    const int ox = (blockId % blocksX) * kTW, oy = (blockId / blocksX) * kTH;
+    const size_t P = size_t(W) * size_t(H);
+    const float dLds = -lambda * invCount;
+    // 1. inputs with a 5-pixel halo, zero outside the image (every load is issued: an
+    //    outside position reads the image's first value and is then replaced by zero)
+    {
+        const auto gR = glob(render);
+        const auto gT = glob(targetF);
+        const auto gU = glob(targetU8);
+        const int wc = tid % kMW, wr = tid / kMW;
+        const int gx = ox - kR + wc;
+        const bool colIn = wr < kLoadRows && gx >= 0 && gx < W;
+        float2 v[kLoadPasses];
+#pragma unroll
+        for (int q = 0; q < kLoadPasses; ++q) {
+            const int gy = oy - kR + wr + kLoadRows * q;
+            const bool in = colIn && wr + kLoadRows * q < kMH && gy >= 0 && gy < H;
+            const uint32_t idx = in ? (uint32_t(gy) * uint32_t(W) + uint32_t(gx)) * 3u + uint32_t(c) : 0u;
+            const float r = gR[idx];
+            const float t = kU8 ? float(gU[idx]) * (1.0f / 255.0f) : gT[idx];
+            v[q] = in ? make_float2(r, t) : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < kLoadPasses; ++q)
+            if (wr < kLoadRows && wr + kLoadRows * q < kMH)
+                sIn[wr + kLoadRows * q][wc] = v[q];
+    }
+    __syncthreads();
+    // 2. horizontal blur of x, y, x^2, y^2, xy (Loss.cpp:77-94): window row ly, block columns
+    //    x0 .. x0 + kSH - 1
+    if (tid < kMH * kHSegs) {
+        const int ly = tid % kMH, x0 = segStart(tid / kMH, kSH, kTW);
+        float2 in[kSH + 2 * kR];
+#pragma unroll
+        for (int i = 0; i < kSH + 2 * kR; ++i)
+            in[i] = sIn[ly][x0 + i];
+#pragma unroll
+        for (int o = 0; o < kSH; ++o) {
+            float sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+#pragma unroll
+            for (int t = 0; t < kTaps; ++t) {
+                const float a = in[o + t].x, b = in[o + t].y;
+                sx += w[t] * a;
+                sy += w[t] * b;
+                sxx += w[t] * a * a;
+                syy += w[t] * b * b;
+                sxy += w[t] * a * b;
+            }
+            sH[0][ly][x0 + o] = sx;
+            sH[1][ly][x0 + o] = sy;
+            sH[2][ly][x0 + o] = sxx;
+            sH[3][ly][x0 + o] = syy;
+            sH[4][ly][x0 + o] = sxy;
+        }
+    }
+    __syncthreads();
+    // 3. vertical blur, SSIM and its partial derivatives (Loss.cpp:110-139) at block column lx,
+    //    rows y0, y0 + 1
+    const int lx = tid % kTW, y0 = (tid / kTW) * kSV;
+    float m[5][kSV];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        float col[kSV + 2 * kR];
+#pragma unroll
+        for (int i = 0; i < kSV + 2 * kR; ++i)
+            col[i] = sH[j][y0 + i][lx];
+#pragma unroll
+        for (int o = 0; o < kSV; ++o) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < kTaps; ++t)
+                acc += w[t] * col[o + t];
+            m[j][o] = acc;
+        }
+    }
+    double ssim = 0.0;
+#pragma unroll
+    for (int o = 0; o < kSV; ++o) {
+        const int gx = ox + lx, gy = oy + y0 + o;
+        if (gx < W && gy < H) {
+            const float mx = m[0][o], my = m[1][o];
+            const float sx = m[2][o] - mx * mx, sy = m[3][o] - my * my;
+            const float sxy = m[4][o] - mx * my;
+            const float a1 = 2 * mx * my + kC1, a2 = 2 * sxy + kC2;
+            const float b1 = mx * mx + my * my + kC1, b2 = sx + sy + kC2;
+            const float vv = (a1 * a2) / (b1 * b2);
+            ssim += double(vv);
+            const float dmu = (2 * my * a2 - 2 * my * a1) / (b1 * b2) - vv * (2 * mx / b1 - 2 * mx / b2);
+            const uint32_t pix = uint32_t(gy) * uint32_t(W) + uint32_t(gx);
+            const auto m0 = glob(maps + mapIndex(c, 0, P, 0));
+            m0[pix] = dLds * dmu;
+            m0[uint32_t(P) + pix] = dLds * (-vv / b2);
+            m0[2u * uint32_t(P) + pix] = dLds * (2 * a1 / (b1 * b2));
+        }
+    }
+    ssim = blockSumD(ssim, red);
+    if (tid == 0)
+        partials[1] = ssim;
+}
+
+template <bool kU8, bool kScaled>
+__device__ __forceinline__ void gradBlock(const float *__restrict__ render,
+                                          const float *__restrict__ targetF,
+                                          const uint8_t *__restrict__ targetU8, int W, int H,
+                                          float lambda, float invCount, float gradScale,
+                                          const Window &win, int blocksX, int blockId, int c,
+                                          const float *__restrict__ maps, float *__restrict__ grad,
+                                          double *__restrict__ partials)
+{
+    __shared__ float sA[3][kMH][kMStride];
+    __shared__ float sG[3][kMH][kBStride];
+    __shared__ double red[2][kThreads / 64];
+    const int tid = threadIdx.x;
+    float w[kTaps];
+#pragma unroll
+    for (int t = 0; t < kTaps; ++t)
+        w[t] = win.w[t];
+    const int ox = (blockId % blocksX) * kTW, oy = (blockId / blocksX) * kTH;
+    const size_t P = size_t(W) * size_t(H);
+    const bool ssimOn = lambda > 0.0f;
+    const float l1Scale = (1.0f - lambda) * invCount;
+    const int lx = tid % kTW, y0 = (tid / kTW) * kSV;
+    // the thread's pixels (read first: their loads overlap the partials' below)
+    float rv[kSV], tv[kSV];
+    uint32_t pidx[kSV];
+#pragma unroll
+    for (int o = 0; o < kSV; ++o) {
+        const int gx = ox + lx, gy = oy + y0 + o;
+        pidx[o] = gx < W && gy < H ? (uint32_t(gy) * uint32_t(W) + uint32_t(gx)) * 3u + uint32_t(c) : 0u;
+        rv[o] = glob(render)[pidx[o]];
+        tv[o] = kU8 ? float(glob(targetU8)[pidx[o]]) * (1.0f / 255.0f) : glob(targetF)[pidx[o]];
+    }
+    float g[3][kSV] = {};
+    if (ssimOn) {
+        // 1. the partials with a 5-pixel halo, zero outside the image (as the fused block's);
+        //    all loads are issued before the first LDS store
+        const int wc = tid % kMW, wr = tid / kMW;
+        const int gx = ox - kR + wc;
+        const bool colIn = wr < kLoadRows && gx >= 0 && gx < W;
+        const auto m0 = glob(maps + mapIndex(c, 0, P, 0));
+        float v[3][kLoadPasses];
+#pragma unroll
+        for (int q = 0; q < kLoadPasses; ++q) {
+            const int gy = oy - kR + wr + kLoadRows * q;
+            const bool in = colIn && wr + kLoadRows * q < kMH && gy >= 0 && gy < H;
+            const uint32_t pix = in ? uint32_t(gy) * uint32_t(W) + uint32_t(gx) : 0u;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float x = m0[uint32_t(j) * uint32_t(P) + pix];
+                v[j][q] = in ? x : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kLoadPasses; ++q)
+            if (wr < kLoadRows && wr + kLoadRows * q < kMH) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    sA[j][wr + kLoadRows * q][wc] = v[j][q];
+            }
+        __syncthreads();
+        // 2. horizontal blur of the partials (Loss.cpp:142-157)
+        if (tid < kMH * kHSegs) {
+            const int ly = tid % kMH, x0 = segStart(tid / kMH, kSH, kTW);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float row[kSH + 2 * kR];
+#pragma unroll
+                for (int i = 0; i < kSH + 2 * kR; ++i)
+                    row[i] = sA[j][ly][x0 + i];
+#pragma unroll
+                for (int o = 0; o < kSH; ++o) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < kTaps; ++t)
+                        acc += w[t] * row[o + t];
+                    sG[j][ly][x0 + o] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // 3. vertical blur of the partials
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float col[kSV + 2 * kR];
+#pragma unroll
+            for (int i = 0; i < kSV + 2 * kR; ++i)
+                col[i] = sG[j][y0 + i][lx];
+#pragma unroll
+            for (int o = 0; o < kSV; ++o) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < kTaps; ++t)
+                    acc += w[t] * col[o + t];
+                g[j][o] = acc;
+            }
+        }
+    }
+    // 4. L1 term and chain rule (Loss.cpp:64-74, 160-177)
+    double l1 = 0.0, sq = 0.0;
+#pragma unroll
+    for (int o = 0; o < kSV; ++o) {
+        const int gx = ox + lx, gy = oy + y0 + o;
+        if (gx < W && gy < H) {
+            const float r = rv[o], t = tv[o];
+            const float d = r - t;
+            l1 += double(fabsf(d));
+            const double dd = double(r) - double(t);
+            sq += dd * dd;
+            float gval = l1Scale * (d > 0 ? 1.0f : (d < 0 ? -1.0f : 0.0f));
+            if (ssimOn)
+                gval += g[0][o] + 2.0f * r * g[1][o] + t * g[2][o];
+            if (kScaled)
+                gval *= gradScale;
+            glob(grad)[pidx[o]] = gval;
+        }
+    }
+    // block sums (the fused block's reduction, component by component)
+    double v2[2] = {l1, sq};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v2[j] += __shfl_xor(v2[j], off, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = v2[0];
+        red[1][tid >> 6] = v2[1];
+    }
+    __syncthreads();
+    if (tid < 2) {
+        double sum = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kThreads / 64; ++wv)
+            sum += red[tid][wv];
+        partials[tid == 0 ? 0 : 2] = sum;
     }
 }
 
@@ -332,6 +634,82 @@ __global__ __launch_bounds__(kThreads) void lossKernel(const float *__restrict__
         return;
     lossBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win, blocksX,
                           block, c, grad, partials + (size_t(c) * size_t(blocks) + size_t(block)) * 3);
+}
+
+// The two launches of the split loss, with lossKernel's workgroup order (and its optional
+// first workgroup running the segmented backward's work-item scan, on the first launch).
+template <bool kU8>
+__global__ __launch_bounds__(kThreads) void lossSsimKernel(const float *__restrict__ render,
+                                                           const float *__restrict__ targetF,
+                                                           const uint8_t *__restrict__ targetU8,
+                                                           int W, int H, float lambda,
+                                                           float invCount, Window win, int blocksX,
+                                                           int blocks, float *__restrict__ maps,
+                                                           double *__restrict__ partials,
+                                                           SegmentPrep prep)
+{
+    const int skip = prep.segCount ? 1 : 0;
+    if (int(blockIdx.x) < skip) {
+        segmentPrepBlock(prep);
+        return;
+    }
+    const int x = int(blockIdx.x) - skip;
+    const int grp = x / (3 * kLossXcds), r = x % (3 * kLossXcds);
+    const int c = r / kLossXcds, block = grp * kLossXcds + r % kLossXcds;
+    if (block >= blocks)
+        return;
+    ssimBlock<kU8>(render, targetF, targetU8, W, H, lambda, invCount, win, blocksX, block, c, maps,
+                   partials + (size_t(c) * size_t(blocks) + size_t(block)) * 3);
+}
+
+template <bool kU8>
+__global__ __launch_bounds__(kThreads) void lossGradKernel(const float *__restrict__ render,
+                                                           const float *__restrict__ targetF,
+                                                           const uint8_t *__restrict__ targetU8,
+                                                           int W, int H, float lambda,
+                                                           float invCount, Window win, int blocksX,
+                                                           int blocks, const float *__restrict__ maps,
+                                                           float *__restrict__ grad,
+                                                           double *__restrict__ partials)
+{
+    const int x = int(blockIdx.x);
+    const int grp = x / (3 * kLossXcds), r = x % (3 * kLossXcds);
+    const int c = r / kLossXcds, block = grp * kLossXcds + r % kLossXcds;
+    if (block >= blocks)
+        return;
+    gradBlock<kU8, false>(render, targetF, targetU8, W, H, lambda, invCount, 1.0f, win, blocksX,
+                          block, c, maps, grad,
+                          partials + (size_t(c) * size_t(blocks) + size_t(block)) * 3);
+}
+
+// Batched split loss: item blockIdx.z; blockIdx.x -> (32x32 block, channel) in lossKernel's
+// order, so the three channels of a block share an XCD's L2 (interleaved images).
+__global__ __launch_bounds__(kThreads) void lossSsimBatchedKernel(const LossItem *__restrict__ items,
+                                                                  float lambda, Window win)
+{
+    const LossItem &it = items[blockIdx.z];
+    const int x = int(blockIdx.x);
+    const int grp = x / (3 * kLossXcds), r = x % (3 * kLossXcds);
+    const int c = r / kLossXcds, block = grp * kLossXcds + r % kLossXcds;
+    if (block >= it.numBlocks)
+        return;
+    ssimBlock<true>(it.render, nullptr, it.target, it.width, it.height, lambda, it.invCount, win,
+                    it.blocksX, block, c, it.maps,
+                    it.partials + (size_t(c) * it.numBlocks + block) * 3);
+}
+
+__global__ __launch_bounds__(kThreads) void lossGradBatchedKernel(const LossItem *__restrict__ items,
+                                                                  float lambda, Window win)
+{
+    const LossItem &it = items[blockIdx.z];
+    const int x = int(blockIdx.x);
+    const int grp = x / (3 * kLossXcds), r = x % (3 * kLossXcds);
+    const int c = r / kLossXcds, block = grp * kLossXcds + r % kLossXcds;
+    if (block >= it.numBlocks)
+        return;
+    gradBlock<true, true>(it.render, nullptr, it.target, it.width, it.height, lambda, it.invCount,
+                          it.gradScale, win, it.blocksX, block, c, it.maps, it.grad,
+                          it.partials + (size_t(c) * it.numBlocks + block) * 3);
 }
 
 // Batched: item blockIdx.z, channel blockIdx.y; blocks past the item's count exit.
@@ -439,7 +817,7 @@ size_t lossPartialsSize(int width, int height)
 int launchPhotometricLoss(hipStream_t s, const float *render, const float *target,
                           const uint8_t *targetU8, int width, int height, float lambda,
                           float *grad, double *partials, size_t partialsCap, float *result,
-                          LossFinalize *deferred, const SegmentPrep *prep)
+                          LossFinalize *deferred, const SegmentPrep *prep, float *ssimMaps)
 {
     const int bx = (width + kTW - 1) / kTW, by = (height + kTH - 1) / kTH;
     const int nb = bx * by; // per channel
@@ -449,7 +827,28 @@ int launchPhotometricLoss(hipStream_t s, const float *render, const float *targe
     static const Window win = makeWindow();
     const SegmentPrep side = prep ? *prep : SegmentPrep{};
     const int wgs = (nb > 0 ? lossWorkgroups(nb) : 0) + (prep ? 1 : 0);
-    if (wgs > 0) {
+    // the split kernels index an image with 32-bit offsets
+    if (ssimMaps && lambda > 0.0f && nb > 0 && P * 3 < (size_t(1) << 32)) {
+        if (targetU8)
+            hipLaunchKernelGGL(lossSsimKernel<true>, dim3(wgs), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               ssimMaps, partials, side);
+        else
+            hipLaunchKernelGGL(lossSsimKernel<false>, dim3(wgs), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               ssimMaps, partials, side);
+        U3D_LAUNCH_CHECK();
+        const int wg2 = lossWorkgroups(nb);
+        if (targetU8)
+            hipLaunchKernelGGL(lossGradKernel<true>, dim3(wg2), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               ssimMaps, grad, partials);
+        else
+            hipLaunchKernelGGL(lossGradKernel<false>, dim3(wg2), dim3(kThreads), 0, s, render,
+                               target, targetU8, width, height, lambda, invCount, win, bx, nb,
+                               ssimMaps, grad, partials);
+        U3D_LAUNCH_CHECK();
+    } else if (wgs > 0) {
         if (targetU8)
             hipLaunchKernelGGL(lossKernel<true>, dim3(wgs), dim3(kThreads), 0, s, render,
                                target, targetU8, width, height, lambda, invCount, win, bx, nb,
@@ -489,12 +888,19 @@ void lossItemGeometry(int width, int height, int *blocksX, int *numBlocks)
 }
 
 int launchPhotometricLossBatched(hipStream_t s, const LossItem *items, int numItems,
-                                 int maxBlocks, float lambda, const StepStatus &status)
+                                 int maxBlocks, float lambda, const StepStatus &status,
+                                 bool splitMaps)
 {
     if (numItems <= 0)
         return U3D_OK;
     static const Window win = makeWindow();
-    if (maxBlocks > 0) {
+    if (maxBlocks > 0 && splitMaps && lambda > 0.0f) {
+        const dim3 grid(unsigned(lossWorkgroups(maxBlocks)), 1, unsigned(numItems));
+        hipLaunchKernelGGL(lossSsimBatchedKernel, grid, dim3(kThreads), 0, s, items, lambda, win);
+        U3D_LAUNCH_CHECK();
+        hipLaunchKernelGGL(lossGradBatchedKernel, grid, dim3(kThreads), 0, s, items, lambda, win);
+        U3D_LAUNCH_CHECK();
+    } else if (maxBlocks > 0) {
         hipLaunchKernelGGL(lossBatchedKernel, dim3(maxBlocks, 3, numItems), dim3(kThreads), 0, s,
                            items, lambda, win);
         U3D_LAUNCH_CHECK();

@@ -152,6 +152,7 @@ struct u3d_trainer {
     int iteration = 0;
     bool opacityWasReset = false;
     u3d::DevBuf<float> render, dloss, gradAccum, evalOut;
+    u3d::DevBuf<float> ssimMaps; // the loss's SSIM partials, 9 floats per pixel
     // Per-step loss statistics {total, l1, ssim, psnr} per view, [iterations][B][4], in pinned
     // host memory: the loss's final reduction stores them there (psnr last, with a release),
     // over a sentinel in the psnr slot, so the host reads a step's values without a stream sync.
