@@ -197,7 +197,9 @@ int u3d_raster_get_grad2d(u3d_raster *r, const float **grad2d);
 int u3d_photometric_loss(u3d_ctx *ctx, const float *render, const float *target, int width,
                          int height, float ssim_lambda, float *grad, u3d_loss_value *out_host);
 /* No counterpart in the reference: u3d_photometric_loss for callers that cannot allocate device
- * memory (code under HIP graph capture, tight serving loops). Same kernels and results; its
+ * memory (code under HIP graph capture, tight serving loops). Same results bit for bit (the
+ * loss as one fused launch, where u3d_photometric_loss passes the SSIM partials through its
+ * scratch between two launches); its
  * scratch is a static 1 MiB array of the library on each GPU (images up to 14,559 blocks of
  * 32x32 pixels, e.g. 3840x3840; larger ones return U3D_ERR_CAPACITY), so it calls no hipMalloc
  * and never synchronises. value_device (nullable) receives {total, l1, ssim, psnr} as 4 device
