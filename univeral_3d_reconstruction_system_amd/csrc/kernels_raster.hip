@@ -104,10 +104,13 @@ __device__ __forceinline__ float waveReduceScatter9(const float v[9], int lane, 
     return t;
 }
 
-// State of one pixel's reverse walk (Rasterizer.cpp:370-430): transmittance, the running
-// colour behind, the previous contributor's colour and alpha.
+// State of one pixel's reverse walk (Rasterizer.cpp:370-430): transmittance and the colour
+// behind the next contributor. The reference blends the previous contributor's colour into the
+// colour behind when it reaches the next one; here the blend (the same expression, so the same
+// value) is taken right after the contributor is walked, which keeps four state registers
+// instead of eight.
 struct BackState {
-    float T, acc0, acc1, acc2, last0, last1, last2, lastAlpha;
+    float T, acc0, acc1, acc2;
 };
 
 // One Gaussian of the reverse walk for every lane of the wavefront (uniform control flow):
@@ -160,16 +163,15 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     // few-lane path then adds nothing; a branch here made the compiler copy the walk state)
     const uint64_t who = kFewLanes > 0 ? __ballot(contrib) : 0;
     const float c0 = b.z, c1 = b.w;
-    const float inv = kFastExp ? __builtin_amdgcn_rcpf(1.0f - alpha) : 0.0f;
-    const float Tn = kFastExp ? st.T * inv : st.T / (1.0f - alpha);
+    const float oneMinus = 1.0f - alpha;
+    const float inv = kFastExp ? __builtin_amdgcn_rcpf(oneMinus) : 0.0f;
+    const float Tn = kFastExp ? st.T * inv : st.T / oneMinus;
     const float w = alpha * Tn;
-    const float acc0 = st.lastAlpha * st.last0 + (1.0f - st.lastAlpha) * st.acc0;
-    const float acc1 = st.lastAlpha * st.last1 + (1.0f - st.lastAlpha) * st.acc1;
-    const float acc2 = st.lastAlpha * st.last2 + (1.0f - st.lastAlpha) * st.acc2;
-    float dAlpha = 0.0f;
-    dAlpha += (c0 - acc0) * dp0;
-    dAlpha += (c1 - acc1) * dp1;
-    dAlpha += (c2 - acc2) * dp2;
+    // (the reference's sum starts from +0; starting from the first term differs only in the
+    // sign of a zero sum, which no later sum keeps)
+    float dAlpha = (c0 - st.acc0) * dp0;
+    dAlpha += (c1 - st.acc1) * dp1;
+    dAlpha += (c2 - st.acc2) * dp2;
     dAlpha *= Tn;
     if (bgDot != 0.0f || bgDot != bgDot) // the background term; adding +-0 changes nothing
         dAlpha += kFastExp ? -fT * inv * bgDot : -fT / (1.0f - alpha) * bgDot;
@@ -187,13 +189,9 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
     v[7] = contrib ? w * dp2 : 0.0f;
     v[8] = grads ? G * dAlpha : 0.0f;
     st.T = contrib ? Tn : st.T;
-    st.acc0 = contrib ? acc0 : st.acc0;
-    st.acc1 = contrib ? acc1 : st.acc1;
-    st.acc2 = contrib ? acc2 : st.acc2;
-    st.last0 = contrib ? c0 : st.last0;
-    st.last1 = contrib ? c1 : st.last1;
-    st.last2 = contrib ? c2 : st.last2;
-    st.lastAlpha = contrib ? alpha : st.lastAlpha;
+    st.acc0 = contrib ? alpha * c0 + oneMinus * st.acc0 : st.acc0;
+    st.acc1 = contrib ? alpha * c1 + oneMinus * st.acc1 : st.acc1;
+    st.acc2 = contrib ? alpha * c2 + oneMinus * st.acc2 : st.acc2;
     if (kFewLanes > 0 && __popcll(who) <= kFewLanes) {
         if (contrib) {
 #pragma unroll
@@ -841,7 +839,7 @@ __device__ __forceinline__ void backwardSegment(
     const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
     const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
     const float fpx = float(px), fpy = float(py);
-    BackState bs{fT, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    BackState bs{fT, 0.0f, 0.0f, 0.0f};
     if (nc > hi) {
         const float4 ck = ckpt[(size_t(begin / kBatch + tile) + size_t(segIdx + 1)) * kThreads + tid];
         bs.T = ck.x;
@@ -977,7 +975,7 @@ __device__ __forceinline__ void backwardBlock(
     const float dp2 = inside ? dl[pix * 3 + 2] : 0.0f;
     const float bgDot = bg0 * dp0 + bg1 * dp1 + bg2 * dp2;
     const float fpx = float(px), fpy = float(py);
-    BackState bs{fT, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    BackState bs{fT, 0.0f, 0.0f, 0.0f};
     {
         int m = nc;
 #pragma unroll
@@ -1300,8 +1298,16 @@ __global__ __launch_bounds__(kThreads) void rasterBackwardKernel(
 
 // Persistent batched compositing: every workgroup takes the next block of the work order from
 // an atomic counter until the list is exhausted. The order lists heavy tiles first.
-template <bool kBackward, int kBatch, int kFewLanes>
-__global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
+// kCkpt (forward: checkpoints for a segmented backward) and kFastExp are template parameters
+// chosen on the host: each instantiation holds one block function, so its LDS arrays are those
+// of one variant (runtime branches between variants had added up the variants' static LDS,
+// 42 KB for the forward). At most three wavefronts per SIMD: with the register budget that
+// leaves (up to 168), the compiler keeps the walks' independent chains apart; aiming at the
+// five wavefronts the smaller LDS would allow (88-94 registers) measured 5% (backward) and 17%
+// (forward) slower, and the whole step 2-3% slower with four wavefronts, 24% with two (8 views
+// of 640x480, 50k Gaussians).
+template <bool kBackward, int kBatch, int kFewLanes, bool kCkpt, bool kFastExp>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 3))) void rasterBatchedKernel(
     const RenderItem *__restrict__ items, int numItems, const uint32_t *__restrict__ order,
     int64_t totalBlocks, unsigned *counter, const int *__restrict__ tileOffsets,
     const int *__restrict__ tileList, const float4 *__restrict__ rec0,
@@ -1335,23 +1341,10 @@ __global__ __launch_bounds__(kThreads) void rasterBatchedKernel(
                                          rec1, rec2, st.background[0], st.background[1],           \
                                          st.background[2], st.min_alpha, it.dl, it.finalT,         \
                                          it.nContrib, grad2d)
-        const bool fast = noSpecialExp(st.min_alpha);
-        if (kBackward) {
-            if (fast)
-                U3D_BWD(true);
-            else
-                U3D_BWD(false);
-        } else if (seg.ckpt) {
-            if (fast)
-                U3D_FWD(true, true);
-            else
-                U3D_FWD(true, false);
-        } else {
-            if (fast)
-                U3D_FWD(false, true);
-            else
-                U3D_FWD(false, false);
-        }
+        if constexpr (kBackward)
+            U3D_BWD(kFastExp);
+        else
+            U3D_FWD(kCkpt, kFastExp);
 #undef U3D_BWD
 #undef U3D_FWD
     }
@@ -1366,9 +1359,29 @@ static void launchBatchedVariant(hipStream_t s, int numWorkgroups, const RenderI
                                  const ScreenBuffers &sb, const u3d_raster_settings &st,
                                  float *grad2d, const SegmentState &seg)
 {
-    hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch, kFewLanes>), dim3(numWorkgroups), dim3(kThreads),
-                       0, s, items, numItems, order, totalBlocks, counter, tileOffsets, tileList,
-                       sb.rec0, sb.rec1, sb.rec2, st, grad2d, seg);
+#define U3D_VARIANT(CK, FE)                                                                      \
+    hipLaunchKernelGGL((rasterBatchedKernel<kBackward, kBatch, kFewLanes, CK, FE>),               \
+                       dim3(numWorkgroups), dim3(kThreads), 0, s, items, numItems, order,         \
+                       totalBlocks, counter, tileOffsets, tileList, sb.rec0, sb.rec1, sb.rec2, st, \
+                       grad2d, seg)
+    const bool fast = st.min_alpha >= 1e-37f; // noSpecialExp, on the host
+    if constexpr (kBackward) {
+        if (fast)
+            U3D_VARIANT(false, true);
+        else
+            U3D_VARIANT(false, false);
+    } else if (!seg.ckpt) {
+        if (fast)
+            U3D_VARIANT(false, true);
+        else
+            U3D_VARIANT(false, false);
+    } else {
+        if (fast)
+            U3D_VARIANT(true, true);
+        else
+            U3D_VARIANT(true, false);
+    }
+#undef U3D_VARIANT
 }
 
 int launchRasterBatched(hipStream_t s, bool backward, const RenderItem *items, int numItems,
@@ -1414,13 +1427,13 @@ int rasterBatchedOccupancy(bool backward, int ldsBatch)
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
     if (backward)
-        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 64, 0>, kThreads, 0)
-            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 128, 0>, kThreads, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 256, 0>, kThreads, 0);
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 64, 0, false, true>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 128, 0, false, true>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<true, 256, 0, false, true>, kThreads, 0);
     else
-        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 64, 0>, kThreads, 0)
-            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 128, 0>, kThreads, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 256, 0>, kThreads, 0);
+        e = ldsBatch == 64    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 64, 0, false, true>, kThreads, 0)
+            : ldsBatch == 128 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 128, 0, false, true>, kThreads, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rasterBatchedKernel<false, 256, 0, false, true>, kThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
