@@ -279,18 +279,25 @@ def time_to_l1(ctx, init: dict, cams, images, extent, opt, target_l1: float, max
 def batched_options(plain_iterations: int, views_per_step: int, max_gaussians: int,
                     views_factor: float = 1.5):
     """The plain schedule (TrainOptions::forIterations) counted in views: B views per step,
-    densify / reset steps divided by B, sqrt(B) rates and beta^B decay (batch_scaling), and the
-    means rate decayed over views_factor x the plain trainer's views (also the most it runs).
+    densify steps divided by B, sqrt(B) rates and beta^B decay (batch_scaling), the means rate
+    decayed over views_factor x the plain trainer's views (also the most it runs), and the
+    opacity resets at the plain schedule's fractions of that horizon (every views_factor x the
+    plain interval, in views).
     Swept on two held-out high-preset scenes at B = 8: rates x B^0.75 or x B never reached the
     plain trainer's final L1, x B^0.4 or x B^0.3 took 4-20% longer than x sqrt(B); a decay over
-    1.25 or 1.5 x the plain views reached it 5-8% sooner than over 2 x, over 1 x not at all."""
+    1.25 or 1.5 x the plain views reached it 5-8% sooner than over 2 x, over 1 x not at all.
+    On four held-out scenes (kNN and learned starts): resets every 1.5 x the plain interval
+    reached it 6-18% sooner than at the plain interval (whose second reset falls shortly before
+    the batched run reaches the target), and as soon as with no reset at all; stretching the
+    densify schedule by 1.5 x as well was up to 12% slower than even the plain reset interval."""
     B = views_per_step
     base = U.options_for_iterations(plain_iterations)
     o = U.options_for_iterations(max(1, int(plain_iterations * views_factor) // B))
     o.refine_start = max(1, base.refine_start // B)
     o.refine_stop = max(1, base.refine_stop // B)
     o.refine_interval = max(1, base.refine_interval // B)
-    o.opacity_reset_interval = base.opacity_reset_interval // B if base.opacity_reset_interval else 0
+    o.opacity_reset_interval = (max(1, int(base.opacity_reset_interval * views_factor) // B)
+                                if base.opacity_reset_interval else 0)
     o.max_gaussians = max_gaussians
     o.views_per_step = B
     o.use_graph = int(B > 1)
@@ -351,8 +358,10 @@ def sweep_batched(ctx, net, seed: int, specs, plain_iterations: int = 7000, n_gt
                   log=print) -> dict:
     """Wall time to the plain trainer's final L1 on one held-out scene for batched variants
     spec = (init "knn" | "learned", views per step B, decay horizon views_factor, rate
-    exponent e): batched_options(plain_iterations, B, n_gt, views_factor) with the rates
-    multiplied by B^e (B^0.5 is the engine's own scaling)."""
+    exponent e[, reset factor r[, densify factor d]]): batched_options(plain_iterations, B, n_gt,
+    views_factor) with the rates multiplied by B^e (B^0.5 is the engine's own scaling), the
+    opacity resets every r x the plain interval in views (default views_factor; 0: none) and
+    the densify schedule stretched by d (default 1)."""
     sc = mesh_scene(seed, n_gt, n_seed, views, width, height)
     images = render_targets(ctx, sc["gt"], sc["cams"])
     cams, extent = sc["cams"], sc["extent"]
@@ -373,15 +382,25 @@ def sweep_batched(ctx, net, seed: int, specs, plain_iterations: int = 7000, n_gt
     target = t.evaluate_l1()
     del t
     out = dict(seed=seed, plain_seconds=round(plain_s, 3), target_l1=target, runs=[])
-    for init, B, vf, ex in specs:
+    base = U.options_for_iterations(plain_iterations)
+    for spec in specs:
+        init, B, vf, ex = spec[:4]
+        rf = spec[4] if len(spec) > 4 else vf
+        df = spec[5] if len(spec) > 5 else 1.0
         o = batched_options(plain_iterations, B, n_gt, views_factor=vf)
         m = float(B) ** ex / float(B) ** 0.5
         for f in ("means", "scales", "quats", "opacities", "colors"):
             setattr(o.lr, f, getattr(o.lr, f) * m)
+        o.opacity_reset_interval = (max(1, int(base.opacity_reset_interval * rf) // B)
+                                    if base.opacity_reset_interval and rf > 0 else 0)
+        o.refine_start = max(1, int(base.refine_start * df) // B)
+        o.refine_stop = max(1, int(base.refine_stop * df) // B)
+        o.refine_interval = max(1, int(base.refine_interval * df) // B)
         model = learned if init == "learned" and learned is not None else knn
         sec, steps, l1 = time_to_l1(ctx, model, cams, images, extent, o, target, o.iterations,
                                     check_every=max(1, 100 // B), init_seconds=t_knn)
-        rec = dict(init=init, views_per_step=B, views_factor=vf, rate_exponent=ex,
+        rec = dict(init=init, views_per_step=B, views_factor=vf, rate_exponent=ex, reset_factor=rf,
+                   densify_factor=df,
                    seconds=None if sec is None else round(sec, 3), steps=steps, views=steps * B,
                    fraction_of_plain=None if sec is None else round(sec / plain_s, 4), final_l1=l1)
         out["runs"].append(rec)
@@ -406,7 +425,7 @@ def main(argv=None):
     ap.add_argument("--weights", default=WEIGHTS, help="network weights file (.npz)")
     ap.add_argument("--sweep", default=None, metavar="SPECS",
                     help="only sweep batched options on held-out scenes: 'init,B,views_factor,"
-                         "rate_exponent;...' (init knn or learned)")
+                         "rate_exponent[,reset_factor[,densify_factor]];...' (init knn or learned)")
     ap.add_argument("--sweep-seeds", type=int, nargs="+", default=[100_000])
     a = ap.parse_args(argv)
     ctx = U.Context(0)
@@ -414,8 +433,8 @@ def main(argv=None):
     if a.sweep:
         specs = []
         for item in a.sweep.split(";"):
-            init, B, vf, ex = item.split(",")
-            specs.append((init, int(B), float(vf), float(ex)))
+            f = item.split(",")
+            specs.append((f[0], int(f[1]), *[float(x) for x in f[2:]]))
         net = load_weights(ctx, a.weights)
         res = [sweep_batched(ctx, net, sd, specs, a.plain_iterations, a.eval_gt, a.eval_seeds,
                              a.eval_views, a.eval_size[0], a.eval_size[1], log=log)
