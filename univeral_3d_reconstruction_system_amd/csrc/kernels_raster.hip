@@ -208,8 +208,15 @@ __device__ __forceinline__ void backStep(BackState &st, bool active, const float
 
 // log(min_alpha / alpha) - 1e-3: below it, min(0.99, alpha * exp(power)) < min_alpha holds for
 // the rounded float result as well (logf error and the two roundings are < 1e-5).
+// kFast (min_alpha >= 1e-37): from the hardware log2 (within 1 ulp), ln 2 (log2 min_alpha -
+// log2 alpha) - 1e-3; its error, below 3e-5 for any float alpha, stays far inside the 1e-3
+// margin, so the threshold still lies below the exact one. A denormal or zero alpha gives +inf
+// (no contribution: alpha below min_alpha).
+template <bool kFast>
 __device__ __forceinline__ float skipThreshold(float minAlpha, float alpha)
 {
+    if (kFast)
+        return (__builtin_amdgcn_logf(minAlpha) - __builtin_amdgcn_logf(alpha)) * 0.693147180559945309f - 1e-3f;
     return minAlpha > 0.0f ? logf(minAlpha / alpha) - 1e-3f : -INFINITY;
 }
 
@@ -353,7 +360,7 @@ __device__ __forceinline__ void forwardBlock(
         }
         bool hit = false;
         if (gCur >= 0) {
-            const float thr = skipThreshold(minAlpha, rb.y);
+            const float thr = skipThreshold<kFastExp>(minAlpha, rb.y);
             ls0[lane] = ra;
             ls1[lane] = rb;
             ls2[lane] = rc;
@@ -548,7 +555,7 @@ __device__ __forceinline__ void forwardQuadrant(
         bool hit = false;
         float thr = 0.0f;
         if (gCur >= 0) {
-            thr = skipThreshold(minAlpha, rb.y);
+            thr = skipThreshold<kFastExp>(minAlpha, rb.y);
             hit = boxHit(ra, rb.x, thr, qx0 + (box & 15), qx0 + ((box >> 4) & 15),
                          qy0 + ((box >> 8) & 15), qy0 + ((box >> 12) & 15));
         }
@@ -866,7 +873,7 @@ __device__ __forceinline__ void backwardSegment(
     uint32_t m = 0;
     if (tid < cnt) {
         const int j = lo + tid;
-        const float thr = skipThreshold(minAlpha, rb.y);
+        const float thr = skipThreshold<kFastExp>(minAlpha, rb.y);
         sid[tid] = g;
         s0[tid] = ra;
         s1[tid] = rb;
@@ -996,7 +1003,7 @@ __device__ __forceinline__ void backwardBlock(
             const int g = tileList[begin + j];
             const float4 a = rec0[g];
             const float4 b = rec1[g];
-            const float thr = skipThreshold(minAlpha, b.y);
+            const float thr = skipThreshold<kFastExp>(minAlpha, b.y);
             sid[tid] = g;
             s0[tid] = a;
             s1[tid] = b;
