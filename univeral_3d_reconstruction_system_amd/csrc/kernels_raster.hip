@@ -236,9 +236,10 @@ __device__ __forceinline__ float boxMinQuad(float A, float B, float C, float x0,
     auto q = [&](float dx, float dy) { return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy; };
     auto clampf = [](float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); };
     // edges dx = x0, x1 (minimise over dy) and dy = y0, y1 (minimise over dx). Any point of an
-    // edge gives an upper bound of the minimum; a minimiser off by an ulp raises it by
-    // ~1e-14 * scale, far inside the caller's 1e-6 * scale margin.
-    const float bc = B / C, ba = B / A;
+    // edge gives an upper bound of the minimum; a minimiser off by a few ulps raises it by
+    // ~1e-13 * scale, far inside the caller's 1e-6 * scale margin, so the hardware reciprocal
+    // (1 ulp) serves for the two ratios (A, C > 0 here).
+    const float bc = B * __builtin_amdgcn_rcpf(C), ba = B * __builtin_amdgcn_rcpf(A);
     const float ya = clampf(-bc * x0, y0, y1), yb = clampf(-bc * x1, y0, y1);
     const float xa = clampf(-ba * y0, x0, x1), xb = clampf(-ba * y1, x0, x1);
     return fminf(fminf(q(x0, ya), q(x1, yb)), fminf(q(xa, y0), q(xb, y1)));
