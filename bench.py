@@ -269,8 +269,9 @@ def main(argv=None) -> int:
                          "counters FETCH_SIZE / WRITE_SIZE against a known traffic) and exit")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the trainer runs "
+              "on one GPU; several GPUs would each train their own views or scenes)",
+              file=sys.stderr)
         return 2
     if a.copy_probe:
         import torch
